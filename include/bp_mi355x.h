@@ -1,0 +1,162 @@
+/*
+ * bp_mi355x.h -- C ABI of the MI355X belief-propagation engine (libbpmi355x.so).
+ *
+ * Plain C types only (pointers, sizes, status codes); no HIP or torch types cross the
+ * boundary.  Every call returns an int status (BP_OK = 0); bp_last_error() gives the
+ * message of the calling thread's last failure.  No C++ exception crosses the ABI.
+ *
+ * Reference interfaces these entry points replace (ITensorNetworksNext.jl, paths relative
+ * to its source root):
+ *   bp_sweep / bp_run        AI.solve over BeliefPropagationSweepAlgorithm with
+ *                            message_update!(::SimpleMessageUpdate, ...)
+ *                            src/beliefpropagation/beliefpropagation.jl:69-92, 200-210, 242-257
+ *   bp_update_slot           one message_update!(alg, cache, factors, edge) call
+ *                            src/beliefpropagation/beliefpropagation.jl:242-257
+ *   bp_iterate_diff          AIE.iterate_diff(::MessageCache, ::MessageCache)
+ *                            src/beliefpropagation/beliefpropagation.jl:261-267
+ *   bp_vertex_scalars        vertex_scalars src/beliefpropagation/messagecache.jl:139-151
+ *   bp_edge_scalars          edge_scalars   src/beliefpropagation/messagecache.jl:153-178
+ *   bp_lognorm               bethe_free_energy src/beliefpropagation/messagecache.jl:185-201
+ *   bp_init_messages         message_environment(f, nn) src/beliefpropagation/messagecache.jl:205-229
+ *   bp_set_topology          MessageCache / NormNetwork graph structure
+ *                            src/beliefpropagation/messagecache.jl:15-49, src/normnetwork.jl:13-28
+ *   bp_upload_sites          NormNetwork vertex data lazy(A)*lazy(conj(A')) src/normnetwork.jl:49-54
+ * The adjoint entry points (bp_grad_*) and the energy estimator have no reference
+ * counterpart: the reference has no AD and no expectation-value API.
+ *
+ * Layout (row-major):
+ *   - vertex u owns ports port_off[u] .. port_off[u+1]-1; port q points to port_nbr[q]
+ *     over undirected edge port_edge[q]; port_rev[q] is the neighbour's port back to u;
+ *   - message slot q holds the directed message u -> port_nbr[q] (q a port of u);
+ *   - site tensor of u: [phys_dim[u]][D_port0]...[D_port(k-1)], concatenated over u;
+ *   - norm-network message: D x D, [ket][bra]; plain-network message: length D.
+ *   - a batch is a disjoint union of networks: vertex_net[u] gives u's network; the
+ *     vertices (hence ports) of one network must be contiguous.
+ * Host buffers are FP64 and copied; the caller keeps ownership.
+ *
+ * Threading: a context is used by one host thread at a time.  The library keeps no
+ * global mutable state beyond the HIP runtime's.
+ */
+#ifndef BP_MI355X_H
+#define BP_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    BP_OK = 0,
+    BP_ERR_ARG = 1,         /* invalid argument or shape */
+    BP_ERR_HIP = 2,         /* HIP runtime error */
+    BP_ERR_STATE = 3,       /* call out of order (e.g. no topology yet) */
+    BP_ERR_UNSUPPORTED = 4, /* configuration outside what the kernels cover */
+    BP_ERR_NOMEM = 5
+};
+
+/* contraction precision of a context */
+enum {
+    BP_F64 = 0,  /* FP64 storage and arithmetic */
+    BP_F32 = 1,  /* FP32 storage and arithmetic (MFMA FP32 on the vertex path) */
+    BP_BF16 = 2  /* BF16 MFMA operands, FP32 accumulation and FP32 messages */
+};
+
+/* message initialisations (message_environment(f, nn)) */
+enum { BP_INIT_IDENTITY = 0, BP_INIT_ONES = 1, BP_INIT_UPLOAD = 2 };
+
+/* convergence criterion of bp_run */
+enum { BP_CRIT_FIDELITY = 0, /* reference iterate_diff: 1 - |<m1^, m2^>|^2 */
+       BP_CRIT_MAXABS = 1 };  /* max |m_new - m_old| */
+
+/* kernel path selection */
+enum { BP_PATH_AUTO = 0, BP_PATH_GENERIC = 1, BP_PATH_VERTEX = 2 };
+
+typedef struct bp_context bp_context;
+
+const char *bp_last_error(void);
+int bp_version(void);
+
+/* Lifecycle.  device: HIP device ordinal.  dtype: BP_F64 / BP_F32 / BP_BF16. */
+int bp_create(bp_context **out, int device, int dtype);
+int bp_destroy(bp_context *ctx);
+/* Run on a caller-provided hipStream_t (passed as void*); NULL restores the
+ * context's own stream.  Streams are used at default priority. */
+int bp_set_stream(bp_context *ctx, void *hip_stream);
+
+/* Topology.  plain != 0: single-layer network with vector messages (phys_dim ignored).
+ * vertex_net may be NULL (one network).  nnet = number of networks. */
+int bp_set_topology(bp_context *ctx, int nv, int ne, const int *port_off, const int *port_nbr,
+                    const int *port_edge, const int *port_rev, const int *edge_dim,
+                    const int *phys_dim, const int *vertex_net, int nnet, int plain);
+
+/* Site tensors (FP64 host buffer in the site layout); converted to the context dtype. */
+int bp_upload_sites(bp_context *ctx, const double *sites);
+
+/* Messages: kind = BP_INIT_*; `host` (FP64, message layout) only for BP_INIT_UPLOAD. */
+int bp_init_messages(bp_context *ctx, int kind, const double *host);
+int bp_download_messages(bp_context *ctx, double *out);
+
+/* Choose the kernel path (default BP_PATH_AUTO). */
+int bp_set_path(bp_context *ctx, int path);
+/* The path the next sweep will use (after AUTO resolution). */
+int bp_get_path(bp_context *ctx, int *path);
+
+/* One Jacobi sweep over every message of every not-yet-converged network:
+ * new = normalise(update(old)) with per-slot residuals. */
+int bp_sweep(bp_context *ctx);
+
+/* Full solve, reference stopping semantics per network:
+ *   StopAfterIteration(maxiter) | StopWhenConverged(tol), delta skipped at iteration 0.
+ * maxiter <= 0: not given; tol < 0: not given.  Per-network outputs (length nnet, any may
+ * be NULL): sweeps done, last delta, converged-at iteration (-1 if never).
+ * use_graph != 0 captures the sweep sequence in a hipGraph and replays it. */
+int bp_run(bp_context *ctx, int maxiter, double tol, int criterion, int use_graph,
+           int *out_sweeps, double *out_delta, int *out_at_iteration);
+
+/* Sequential (Gauss-Seidel) update of one message slot in place, normalised. */
+int bp_update_slot(bp_context *ctx, int slot);
+
+/* Per-slot residuals of the last sweep (FP64, length nports), either may be NULL. */
+int bp_residuals(bp_context *ctx, double *fid, double *maxabs);
+
+/* iterate_diff between the current messages and an FP64 host message buffer. */
+int bp_iterate_diff(bp_context *ctx, const double *other, double *out);
+
+/* Vertex scalars Z_v (length nv) and edge scalars Z_e (length ne, edge order of the
+ * topology), FP64 accumulation. */
+int bp_vertex_scalars(bp_context *ctx, double *zv);
+int bp_edge_scalars(bp_context *ctx, double *ze);
+
+/* BP log-norm per network (length nnet): real and imaginary parts (imaginary part is
+ * pi * (#negative vertex terms - #negative edge terms), as the reference's complex
+ * promotion gives); -Inf when an edge term is zero.  Compensated FP64 sums. */
+int bp_lognorm(bp_context *ctx, double *out_re, double *out_im);
+
+/* FP64 polishing: run `nsweeps` FP64 Jacobi sweeps from the current messages on the FP64
+ * copy of the sites, and make the FP64 result the context's messages for scalars and
+ * download.  Per-network last delta (max-abs) in out_delta (may be NULL). */
+int bp_polish_fp64(bp_context *ctx, int nsweeps, double *out_delta);
+
+/* Gradient of the BP log-norm with respect to every site tensor at the current fixed
+ * point (FP64 host output in the site layout). */
+int bp_grad_lognorm(bp_context *ctx, double *grad);
+
+/* Transverse-field Ising energy E = sum_<uv> -J <Z_u Z_v>_BP + sum_v -h <X_v>_BP at the
+ * current messages (phys dim 2), and optionally its gradient with respect to every site
+ * tensor by fixed-point adjoint sweeps (grad may be NULL).  adj_tol / adj_maxiter bound
+ * the adjoint solve; out_adj_iters / out_adj_res report it. */
+int bp_tfim_energy(bp_context *ctx, double J, double h, double *energy, double *grad,
+                   double adj_tol, int adj_maxiter, int *out_adj_iters, double *out_adj_res);
+
+/* Benchmark helper: launch `n` Jacobi sweeps back to back (no convergence bookkeeping,
+ * messages ping-pong) and report the average device duration of one sweep's kernel
+ * launches, timed with HIP events on the context's stream around each sweep. */
+int bp_time_sweeps(bp_context *ctx, int n, double *avg_ms);
+
+/* Elapsed milliseconds of the last bp_run, measured with HIP events on the context's
+ * stream around the sweep launches. */
+int bp_last_run_ms(bp_context *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

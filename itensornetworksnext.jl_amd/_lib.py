@@ -1,0 +1,84 @@
+"""ctypes binding of the C ABI in include/bp_mi355x.h (libbpmi355x.so).
+
+Loading fails loudly: there is no CPU fallback for any compute entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "csrc", "libbpmi355x.so")
+
+BP_OK, BP_ERR_ARG, BP_ERR_HIP, BP_ERR_STATE, BP_ERR_UNSUPPORTED, BP_ERR_NOMEM = range(6)
+BP_F64, BP_F32, BP_BF16 = 0, 1, 2
+BP_INIT_IDENTITY, BP_INIT_ONES, BP_INIT_UPLOAD = 0, 1, 2
+BP_CRIT_FIDELITY, BP_CRIT_MAXABS = 0, 1
+BP_PATH_AUTO, BP_PATH_GENERIC, BP_PATH_VERTEX = 0, 1, 2
+
+DTYPES = {"f64": BP_F64, "float64": BP_F64, "f32": BP_F32, "float32": BP_F32,
+          "bf16": BP_BF16, "bfloat16": BP_BF16}
+PATHS = {"auto": BP_PATH_AUTO, "generic": BP_PATH_GENERIC, "vertex": BP_PATH_VERTEX}
+
+# Every exported symbol and its signature: (restype, argtypes).
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+SIGNATURES = {
+    "bp_last_error": (C.c_char_p, []),
+    "bp_version": (C.c_int, []),
+    "bp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int]),
+    "bp_destroy": (C.c_int, [_vp]),
+    "bp_set_stream": (C.c_int, [_vp, _vp]),
+    "bp_set_topology": (C.c_int, [_vp, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _ip,
+                                  C.c_int, C.c_int]),
+    "bp_upload_sites": (C.c_int, [_vp, _dp]),
+    "bp_init_messages": (C.c_int, [_vp, C.c_int, _dp]),
+    "bp_download_messages": (C.c_int, [_vp, _dp]),
+    "bp_set_path": (C.c_int, [_vp, C.c_int]),
+    "bp_get_path": (C.c_int, [_vp, _ip]),
+    "bp_sweep": (C.c_int, [_vp]),
+    "bp_run": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int, _ip, _dp, _ip]),
+    "bp_update_slot": (C.c_int, [_vp, C.c_int]),
+    "bp_residuals": (C.c_int, [_vp, _dp, _dp]),
+    "bp_iterate_diff": (C.c_int, [_vp, _dp, _dp]),
+    "bp_vertex_scalars": (C.c_int, [_vp, _dp]),
+    "bp_edge_scalars": (C.c_int, [_vp, _dp]),
+    "bp_lognorm": (C.c_int, [_vp, _dp, _dp]),
+    "bp_polish_fp64": (C.c_int, [_vp, C.c_int, _dp]),
+    "bp_grad_lognorm": (C.c_int, [_vp, _dp]),
+    "bp_tfim_energy": (C.c_int, [_vp, C.c_double, C.c_double, _dp, _dp, C.c_double, C.c_int,
+                                 _ip, _dp]),
+    "bp_last_run_ms": (C.c_int, [_vp, _dp]),
+    "bp_time_sweeps": (C.c_int, [_vp, C.c_int, _dp]),
+}
+
+_lib = None
+
+
+class BPError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where}: status {code}: {msg}")
+        self.code = code
+
+
+def load(path: str = LIB_PATH):
+    """Load the engine library; raises if it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP engine library not built: {path} (run __graft_entry__.build())")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, where: str) -> None:
+    if rc != BP_OK:
+        msg = load().bp_last_error().decode(errors="replace")
+        raise BPError(rc, where, msg)

@@ -1,0 +1,968 @@
+// bp_engine.hip -- C-ABI implementation of the MI355X belief-propagation engine.
+//
+// Host side of libbpmi355x.so: context, topology/tensor upload, sweep scheduling with
+// per-network convergence on the device, hipGraph replay, scalars and log-norm, FP64
+// polish, adjoint.  Kernels live in generic.h (runtime shapes), vertex.h (the
+// vertex-centric MFMA path) and adjoint.h.  Entry points are declared in
+// include/bp_mi355x.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bp_mi355x.h"
+#include "common.h"
+#include "generic.h"
+#include "vertex.h"
+#include "adjoint.h"
+
+// ------------------------------------------------------------------ error state
+
+static thread_local std::string g_err;
+
+static int set_err(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return set_err(BP_ERR_HIP, "%s failed: %s (%s:%d)", #expr,                \
+                           hipGetErrorString(e_), __FILE__, __LINE__);                \
+    } while (0)
+
+#define RETCHK(expr)            \
+    do {                        \
+        int rc_ = (expr);       \
+        if (rc_ != BP_OK) return rc_; \
+    } while (0)
+
+// ------------------------------------------------------------------ small kernels
+
+template <typename T>
+__global__ void k_init_messages(DevTopo tp, T *m, int kind) {
+    const int q = blockIdx.x;
+    if (q >= tp.nports) return;
+    const long long off = tp.msg_off[q];
+    const int len = (int)(tp.msg_off[q + 1] - off);
+    const int D = tp.edge_dim[tp.port_edge[q]];
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+        T v = T(1);
+        if (kind == BP_INIT_IDENTITY && !tp.plain) v = (i / D == i % D) ? T(1) : T(0);
+        m[off + i] = v;
+    }
+}
+
+template <typename Tin, typename Tout>
+__global__ void k_convert(const Tin *in, Tout *out, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        out[i] = (Tout)in[i];
+}
+
+__global__ void k_reset_nets(int nnet, int *done, int *iter, double *delta, int *at) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nnet) return;
+    done[n] = 0;
+    iter[n] = 0;
+    delta[n] = INFINITY;
+    at[n] = -1;
+}
+
+// Per-network convergence bookkeeping after one Jacobi sweep (one workgroup per
+// network): StopWhenConverged on `res` (fidelity or max-abs per slot) and
+// StopAfterIteration(maxiter) (AlgorithmsInterfaceExtensions.jl:84-109,
+// beliefpropagation.jl:27-55).
+__global__ __launch_bounds__(BP_BLOCK) void k_reduce_nets(const int *pbeg, const double *res,
+                                                           int maxiter, double tol, int *done,
+                                                           int *iter, double *delta, int *at) {
+    __shared__ double red[BP_BLOCK / 64];
+    const int n = blockIdx.x;
+    if (done[n]) return;  // uniform over the workgroup
+    double m = 0.0;
+    for (int q = pbeg[n] + threadIdx.x; q < pbeg[n + 1]; q += blockDim.x) {
+        const double r = res[q];
+        m = (r > m || r != r) ? r : m;
+    }
+    m = block_max(m, red);
+    if (threadIdx.x == 0) {
+        const int it = iter[n] + 1;
+        iter[n] = it;
+        delta[n] = m;
+        if (tol >= 0.0 && m < tol) {
+            done[n] = 1;
+            at[n] = it;
+        } else if (maxiter > 0 && it >= maxiter) {
+            done[n] = 1;
+        }
+    }
+}
+
+// BP log-norm per network (one workgroup per network): Neumaier-compensated
+// sum log|Z_v| - sum log|Z_e| with the reference's complex promotion and -Inf rule
+// (messagecache.jl:185-201).
+struct NeuSum {
+    double s = 0.0, c = 0.0;
+    __device__ void add(double x) {
+        const double t = s + x;
+        if (fabs(s) >= fabs(x)) c += (s - t) + x; else c += (x - t) + s;
+        s = t;
+    }
+};
+
+__global__ __launch_bounds__(BP_BLOCK) void k_lognorm_nets(DevTopo tp, const int *vbeg,
+                                                            const int *pbeg, const double *zv,
+                                                            const double *ze, double *out_re,
+                                                            double *out_im) {
+    __shared__ double ss[BP_BLOCK], cc[BP_BLOCK];
+    __shared__ int cnt[4];
+    const int n = blockIdx.x;
+    if (threadIdx.x < 4) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    NeuSum acc;
+    int negv = 0, nege = 0, zv0 = 0, ze0 = 0;
+    for (int u = vbeg[n] + threadIdx.x; u < vbeg[n + 1]; u += blockDim.x) {
+        const double x = zv[u];
+        if (x == 0.0) { zv0 = 1; continue; }
+        if (x < 0.0) ++negv;
+        acc.add(log(fabs(x)));
+    }
+    for (int q = pbeg[n] + threadIdx.x; q < pbeg[n + 1]; q += blockDim.x) {
+        if (q > tp.port_rev[q]) continue;
+        const double x = ze[tp.port_edge[q]];
+        if (x == 0.0) { ze0 = 1; continue; }
+        if (x < 0.0) ++nege;
+        acc.add(-log(fabs(x)));
+    }
+    ss[threadIdx.x] = acc.s;
+    cc[threadIdx.x] = acc.c;
+    atomicAdd(&cnt[0], negv);
+    atomicAdd(&cnt[1], nege);
+    atomicOr(&cnt[2], zv0);
+    atomicOr(&cnt[3], ze0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        NeuSum t;
+        for (int i = 0; i < (int)blockDim.x; ++i) t.add(ss[i]);
+        double c = 0.0;
+        for (int i = 0; i < (int)blockDim.x; ++i) c += cc[i];
+        double re = t.s + (t.c + c);
+        double im = M_PI * (double)(cnt[0] - cnt[1]);
+        if (cnt[3]) { re = -INFINITY; im = 0.0; }
+        else if (cnt[2]) re = -INFINITY;
+        out_re[n] = re;
+        out_im[n] = im;
+    }
+}
+
+// ------------------------------------------------------------------ context
+
+struct bp_context {
+    int device = 0;
+    int dtype = BP_F64;
+    int path = BP_PATH_AUTO;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+
+    // topology (host)
+    int nv = 0, ne = 0, nports = 0, nnet = 0, plain = 0;
+    std::vector<int> port_off, port_nbr, port_edge, port_rev, edge_dim, phys_dim, owner, vnet;
+    std::vector<int> vbeg, pbeg;
+    std::vector<long long> site_off, msg_off;
+    long long nsite = 0, nmsg = 0;
+    int maxD = 0;
+    long long max_site = 0;
+    int max_mats = 0;
+    bool have_topo = false, have_sites = false, have_msgs = false;
+
+    // topology (device)
+    int *d_port_off = nullptr, *d_port_nbr = nullptr, *d_port_edge = nullptr,
+        *d_port_rev = nullptr, *d_edge_dim = nullptr, *d_phys_dim = nullptr, *d_owner = nullptr,
+        *d_vnet = nullptr, *d_vbeg = nullptr, *d_pbeg = nullptr;
+    long long *d_site_off = nullptr, *d_msg_off = nullptr;
+
+    // data
+    double *d_sites64 = nullptr;  // FP64 master copy
+    void *d_sites = nullptr;      // working dtype copy (FP32 for F32/BF16)
+    void *d_msg[2] = {nullptr, nullptr};
+    int cur = 0;
+    double *d_msg64[2] = {nullptr, nullptr};  // FP64 polish buffers
+    int cur64 = 0;
+    bool fp64_active = false;  // messages currently live in d_msg64[cur64]
+    double *d_res_fid = nullptr, *d_res_abs = nullptr;
+    double *d_zv = nullptr, *d_ze = nullptr, *d_lre = nullptr, *d_lim = nullptr;
+    int *d_done = nullptr, *d_iter = nullptr, *d_at = nullptr;
+    double *d_delta = nullptr;
+    void *d_ws = nullptr;
+    size_t ws_bytes = 0;
+
+    // vertex path plan
+    VertexPlan vplan;
+    bool vplan_ok = false;
+    void *d_vsites = nullptr;  // site tensors re-laid out for the vertex kernel
+
+    // adjoint scratch
+    AdjointScratch adj;
+
+    size_t elt() const { return dtype == BP_F64 ? sizeof(double) : sizeof(float); }
+
+    DevTopo topo() const {
+        DevTopo t;
+        t.port_off = d_port_off;
+        t.port_nbr = d_port_nbr;
+        t.port_edge = d_port_edge;
+        t.port_rev = d_port_rev;
+        t.edge_dim = d_edge_dim;
+        t.phys_dim = d_phys_dim;
+        t.owner = d_owner;
+        t.vnet = d_vnet;
+        t.site_off = d_site_off;
+        t.msg_off = d_msg_off;
+        t.nv = nv;
+        t.nports = nports;
+        t.plain = plain;
+        return t;
+    }
+};
+
+template <typename T>
+static int upload(T **dst, const std::vector<T> &v) {
+    if (*dst) HIPCHK(hipFree(*dst));
+    *dst = nullptr;
+    if (v.empty()) return BP_OK;
+    HIPCHK(hipMalloc((void **)dst, sizeof(T) * v.size()));
+    HIPCHK(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return BP_OK;
+}
+
+static int dev_alloc(void **p, size_t bytes) {
+    if (*p) HIPCHK(hipFree(*p));
+    *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    HIPCHK(hipMalloc(p, bytes));
+    return BP_OK;
+}
+
+static void free_all(bp_context *c) {
+    void *ptrs[] = {c->d_port_off, c->d_port_nbr, c->d_port_edge, c->d_port_rev, c->d_edge_dim,
+                    c->d_phys_dim, c->d_owner, c->d_vnet, c->d_vbeg, c->d_pbeg, c->d_site_off,
+                    c->d_msg_off, c->d_sites64, c->d_sites, c->d_msg[0], c->d_msg[1],
+                    c->d_msg64[0], c->d_msg64[1], c->d_res_fid, c->d_res_abs, c->d_zv, c->d_ze,
+                    c->d_lre, c->d_lim, c->d_done, c->d_iter, c->d_at, c->d_delta, c->d_ws,
+                    c->d_vsites};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    c->adj.release();
+    vertex_plan_release(&c->vplan);
+}
+
+// ------------------------------------------------------------------ generic launches
+
+struct GenLaunch {
+    int mats_elems, out_elems, red_elems;
+    long long buf_elems;
+    int use_lds;
+    size_t lds_bytes;
+    int grid;
+    long long ws_stride;
+};
+
+static GenLaunch gen_launch(bp_context *c, int nitems, size_t esz) {
+    GenLaunch g;
+    g.mats_elems = std::max(c->max_mats, 1);
+    g.out_elems = std::max(c->maxD * c->maxD, 1);
+    g.red_elems = std::max(c->maxD * c->maxD, BP_BLOCK);
+    g.buf_elems = std::max(c->max_site, 1LL);
+    size_t base = sizeof(double) * g.red_elems + esz * (g.mats_elems + g.out_elems) + 16;
+    size_t with = base + 2 * esz * g.buf_elems;
+    g.use_lds = with <= 64 * 1024 ? 1 : 0;
+    g.lds_bytes = g.use_lds ? with : base;
+    g.grid = std::max(1, std::min(nitems, g.use_lds ? 65535 : 2048));
+    g.ws_stride = 2 * g.buf_elems;
+    return g;
+}
+
+static int ensure_ws(bp_context *c, const GenLaunch &g, size_t esz) {
+    if (g.use_lds) return BP_OK;
+    size_t need = esz * (size_t)g.ws_stride * (size_t)g.grid;
+    if (need > c->ws_bytes) {
+        RETCHK(dev_alloc(&c->d_ws, need));
+        c->ws_bytes = need;
+    }
+    return BP_OK;
+}
+
+template <typename T>
+static int launch_update_generic(bp_context *c, const T *sites, const T *mold, T *mnew,
+                                 const int *done, int q0, int q1) {
+    GenLaunch g = gen_launch(c, q1 - q0, sizeof(T));
+    RETCHK(ensure_ws(c, g, sizeof(T)));
+    hipLaunchKernelGGL(k_update_generic<T>, dim3(g.grid), dim3(BP_BLOCK), g.lds_bytes, c->stream,
+                       c->topo(), sites, mold, mnew, done, c->d_res_fid, c->d_res_abs, q0, q1,
+                       g.mats_elems, g.out_elems, g.red_elems, g.buf_elems, g.use_lds,
+                       (T *)c->d_ws, g.ws_stride);
+    HIPCHK(hipGetLastError());
+    return BP_OK;
+}
+
+template <typename T>
+static int launch_vertex_scalars(bp_context *c, const T *sites, const T *msgs) {
+    GenLaunch g = gen_launch(c, c->nv, sizeof(T));
+    RETCHK(ensure_ws(c, g, sizeof(T)));
+    hipLaunchKernelGGL(k_vertex_scalar_generic<T>, dim3(g.grid), dim3(BP_BLOCK), g.lds_bytes,
+                       c->stream, c->topo(), sites, msgs, c->d_zv, g.mats_elems, g.out_elems,
+                       g.red_elems, g.buf_elems, g.use_lds, (T *)c->d_ws, g.ws_stride);
+    HIPCHK(hipGetLastError());
+    return BP_OK;
+}
+
+template <typename T>
+static int launch_edge_scalars(bp_context *c, const T *msgs) {
+    const int blocks = (c->nports + 255) / 256;
+    if (blocks > 0)
+        hipLaunchKernelGGL(k_edge_scalars<T>, dim3(blocks), dim3(256), 0, c->stream, c->topo(),
+                           msgs, c->d_ze);
+    HIPCHK(hipGetLastError());
+    return BP_OK;
+}
+
+// ------------------------------------------------------------------ path selection
+
+static int resolve_path(bp_context *c) {
+    if (c->path == BP_PATH_GENERIC) return BP_PATH_GENERIC;
+    if (c->path == BP_PATH_VERTEX) return c->vplan_ok ? BP_PATH_VERTEX : -1;
+    // AUTO: the vertex-centric MFMA path whenever the plan supports this network
+    return c->vplan_ok ? BP_PATH_VERTEX : BP_PATH_GENERIC;
+}
+
+// Allocate whatever workspace the next sweeps need, before any graph capture (no
+// allocation may happen inside a captured region).
+static int prepare_sweep(bp_context *c) {
+    const int path = resolve_path(c);
+    if (path == BP_PATH_GENERIC || path < 0) {
+        const size_t es = c->dtype == BP_F64 ? sizeof(double) : sizeof(float);
+        GenLaunch g = gen_launch(c, c->nports, es);
+        RETCHK(ensure_ws(c, g, es));
+    }
+    return BP_OK;
+}
+
+// One Jacobi sweep on the working dtype: reads d_msg[cur], writes d_msg[cur^1].
+static int sweep_once(bp_context *c, const int *done) {
+    const int path = resolve_path(c);
+    if (path < 0) return set_err(BP_ERR_UNSUPPORTED, "vertex path not available: %s",
+                                 c->vplan.why.c_str());
+    void *src = c->d_msg[c->cur], *dst = c->d_msg[c->cur ^ 1];
+    if (path == BP_PATH_VERTEX) {
+        RETCHK(vertex_sweep(&c->vplan, c->stream, c->topo(), c->d_vsites, src, dst, done,
+                            c->d_res_fid, c->d_res_abs));
+        HIPCHK(hipGetLastError());
+    } else if (c->dtype == BP_F64) {
+        RETCHK(launch_update_generic<double>(c, (const double *)c->d_sites, (const double *)src,
+                                             (double *)dst, done, 0, c->nports));
+    } else {
+        RETCHK(launch_update_generic<float>(c, (const float *)c->d_sites, (const float *)src,
+                                            (float *)dst, done, 0, c->nports));
+    }
+    c->cur ^= 1;
+    return BP_OK;
+}
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+const char *bp_last_error(void) { return g_err.c_str(); }
+
+int bp_version(void) { return 10000; }
+
+int bp_create(bp_context **out, int device, int dtype) {
+    if (!out) return set_err(BP_ERR_ARG, "out is NULL");
+    if (dtype != BP_F64 && dtype != BP_F32 && dtype != BP_BF16)
+        return set_err(BP_ERR_ARG, "unknown dtype %d", dtype);
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return set_err(BP_ERR_ARG, "device %d out of range (%d devices)", device, ndev);
+    HIPCHK(hipSetDevice(device));
+    bp_context *c = new bp_context();
+    c->device = device;
+    c->dtype = dtype;
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return set_err(BP_ERR_HIP, "stream/event creation failed");
+    }
+    c->stream = c->own_stream;
+    *out = c;
+    return BP_OK;
+}
+
+int bp_destroy(bp_context *c) {
+    if (!c) return BP_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return BP_OK;
+}
+
+int bp_set_stream(bp_context *c, void *s) {
+    if (!c) return set_err(BP_ERR_ARG, "ctx is NULL");
+    c->stream = s ? (hipStream_t)s : c->own_stream;
+    return BP_OK;
+}
+
+int bp_set_topology(bp_context *c, int nv, int ne, const int *port_off, const int *port_nbr,
+                    const int *port_edge, const int *port_rev, const int *edge_dim,
+                    const int *phys_dim, const int *vertex_net, int nnet, int plain) {
+    if (!c) return set_err(BP_ERR_ARG, "ctx is NULL");
+    if (nv <= 0 || ne < 0 || !port_off) return set_err(BP_ERR_ARG, "empty topology");
+    if (nnet <= 0) nnet = 1;
+    HIPCHK(hipSetDevice(c->device));
+    c->nv = nv;
+    c->ne = ne;
+    c->plain = plain ? 1 : 0;
+    c->nnet = nnet;
+    c->port_off.assign(port_off, port_off + nv + 1);
+    if (c->port_off[0] != 0) return set_err(BP_ERR_ARG, "port_off[0] must be 0");
+    for (int u = 0; u < nv; ++u) {
+        const int k = c->port_off[u + 1] - c->port_off[u];
+        if (k < 0) return set_err(BP_ERR_ARG, "port_off not monotone at %d", u);
+        if (k + 1 > BP_MAXR)
+            return set_err(BP_ERR_UNSUPPORTED, "vertex %d has degree %d > %d", u, k,
+                           BP_MAXR - 1);
+    }
+    c->nports = c->port_off[nv];
+    const int np = c->nports;
+    c->port_nbr.assign(port_nbr, port_nbr + np);
+    c->port_edge.assign(port_edge, port_edge + np);
+    c->port_rev.assign(port_rev, port_rev + np);
+    c->edge_dim.assign(edge_dim, edge_dim + ne);
+    c->phys_dim.assign(nv, 1);
+    if (!plain) {
+        if (!phys_dim) return set_err(BP_ERR_ARG, "phys_dim is NULL for a norm network");
+        c->phys_dim.assign(phys_dim, phys_dim + nv);
+    }
+    c->vnet.assign(nv, 0);
+    if (vertex_net) c->vnet.assign(vertex_net, vertex_net + nv);
+    // validation
+    c->owner.assign(np, 0);
+    for (int u = 0; u < nv; ++u)
+        for (int q = c->port_off[u]; q < c->port_off[u + 1]; ++q) c->owner[q] = u;
+    for (int q = 0; q < np; ++q) {
+        const int w = c->port_nbr[q], r = c->port_rev[q], e = c->port_edge[q];
+        if (w < 0 || w >= nv || r < 0 || r >= np || e < 0 || e >= ne)
+            return set_err(BP_ERR_ARG, "port %d out of range", q);
+        if (c->owner[r] != w || c->port_rev[r] != q || c->port_nbr[r] != c->owner[q] ||
+            c->port_edge[r] != e)
+            return set_err(BP_ERR_ARG, "port %d: inconsistent reverse port", q);
+    }
+    c->maxD = 0;
+    for (int e = 0; e < ne; ++e) {
+        if (c->edge_dim[e] <= 0) return set_err(BP_ERR_ARG, "edge %d has dim %d", e, c->edge_dim[e]);
+        c->maxD = std::max(c->maxD, c->edge_dim[e]);
+    }
+    if (c->maxD > 64) return set_err(BP_ERR_UNSUPPORTED, "bond dimension %d > 64", c->maxD);
+    // offsets
+    c->site_off.assign(nv + 1, 0);
+    c->max_site = 0;
+    c->max_mats = 0;
+    for (int u = 0; u < nv; ++u) {
+        long long n = c->phys_dim[u];
+        int mats = 0;
+        for (int q = c->port_off[u]; q < c->port_off[u + 1]; ++q) {
+            const int D = c->edge_dim[c->port_edge[q]];
+            n *= D;
+            mats += plain ? D : D * D;
+        }
+        c->site_off[u + 1] = c->site_off[u] + n;
+        c->max_site = std::max(c->max_site, n);
+        c->max_mats = std::max(c->max_mats, mats);
+    }
+    c->msg_off.assign(np + 1, 0);
+    for (int q = 0; q < np; ++q) {
+        const int D = c->edge_dim[c->port_edge[q]];
+        c->msg_off[q + 1] = c->msg_off[q] + (plain ? D : (long long)D * D);
+    }
+    c->nsite = c->site_off[nv];
+    c->nmsg = c->msg_off[np];
+    // networks must be contiguous in vertex order
+    c->vbeg.assign(nnet + 1, 0);
+    c->pbeg.assign(nnet + 1, 0);
+    for (int u = 0; u < nv; ++u) {
+        const int n = c->vnet[u];
+        if (n < 0 || n >= nnet) return set_err(BP_ERR_ARG, "vertex_net[%d]=%d out of range", u, n);
+        if (u > 0 && n < c->vnet[u - 1])
+            return set_err(BP_ERR_ARG, "networks must be contiguous and ordered in vertex order");
+    }
+    {
+        int u = 0;
+        for (int n = 0; n < nnet; ++n) {
+            c->vbeg[n] = u;
+            while (u < nv && c->vnet[u] == n) ++u;
+        }
+        c->vbeg[nnet] = nv;
+        for (int n = 0; n <= nnet; ++n) c->pbeg[n] = c->port_off[c->vbeg[n]];
+        for (int q = 0; q < np; ++q)
+            if (c->vnet[c->port_nbr[q]] != c->vnet[c->owner[q]])
+                return set_err(BP_ERR_ARG, "edge crosses networks at port %d", q);
+    }
+    RETCHK(upload(&c->d_port_off, c->port_off));
+    RETCHK(upload(&c->d_port_nbr, c->port_nbr));
+    RETCHK(upload(&c->d_port_edge, c->port_edge));
+    RETCHK(upload(&c->d_port_rev, c->port_rev));
+    RETCHK(upload(&c->d_edge_dim, c->edge_dim));
+    RETCHK(upload(&c->d_phys_dim, c->phys_dim));
+    RETCHK(upload(&c->d_owner, c->owner));
+    RETCHK(upload(&c->d_vnet, c->vnet));
+    RETCHK(upload(&c->d_vbeg, c->vbeg));
+    RETCHK(upload(&c->d_pbeg, c->pbeg));
+    RETCHK(upload(&c->d_site_off, c->site_off));
+    RETCHK(upload(&c->d_msg_off, c->msg_off));
+    const size_t es = c->elt();
+    RETCHK(dev_alloc(&c->d_msg[0], es * c->nmsg));
+    RETCHK(dev_alloc(&c->d_msg[1], es * c->nmsg));
+    RETCHK(dev_alloc((void **)&c->d_res_fid, sizeof(double) * np));
+    RETCHK(dev_alloc((void **)&c->d_res_abs, sizeof(double) * np));
+    RETCHK(dev_alloc((void **)&c->d_zv, sizeof(double) * nv));
+    RETCHK(dev_alloc((void **)&c->d_ze, sizeof(double) * std::max(ne, 1)));
+    RETCHK(dev_alloc((void **)&c->d_lre, sizeof(double) * nnet));
+    RETCHK(dev_alloc((void **)&c->d_lim, sizeof(double) * nnet));
+    RETCHK(dev_alloc((void **)&c->d_done, sizeof(int) * nnet));
+    RETCHK(dev_alloc((void **)&c->d_iter, sizeof(int) * nnet));
+    RETCHK(dev_alloc((void **)&c->d_at, sizeof(int) * nnet));
+    RETCHK(dev_alloc((void **)&c->d_delta, sizeof(double) * nnet));
+    RETCHK(dev_alloc((void **)&c->d_sites64, sizeof(double) * c->nsite));
+    RETCHK(dev_alloc(&c->d_sites, es * c->nsite));
+    if (c->d_msg64[0]) { (void)hipFree(c->d_msg64[0]); c->d_msg64[0] = nullptr; }
+    if (c->d_msg64[1]) { (void)hipFree(c->d_msg64[1]); c->d_msg64[1] = nullptr; }
+    c->fp64_active = false;
+    c->cur = 0;
+    // vertex-path plan (may decline; the reason is kept for bp_set_path errors)
+    vertex_plan_release(&c->vplan);
+    c->vplan_ok = vertex_plan_build(&c->vplan, c->dtype, c->plain, nv, c->port_off.data(),
+                                    c->port_edge.data(), c->edge_dim.data(), c->phys_dim.data(),
+                                    c->site_off.data(), c->msg_off.data(), c->port_rev.data(),
+                                    c->vnet.data()) == BP_OK;
+    c->have_topo = true;
+    c->have_sites = false;
+    c->have_msgs = false;
+    return BP_OK;
+}
+
+int bp_upload_sites(bp_context *c, const double *sites) {
+    if (!c || !sites) return set_err(BP_ERR_ARG, "NULL argument");
+    if (!c->have_topo) return set_err(BP_ERR_STATE, "bp_set_topology first");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(c->d_sites64, sites, sizeof(double) * c->nsite,
+                          hipMemcpyHostToDevice, c->stream));
+    const int blocks = (int)std::min<long long>((c->nsite + 255) / 256, 65535);
+    if (c->dtype == BP_F64) {
+        HIPCHK(hipMemcpyAsync(c->d_sites, c->d_sites64, sizeof(double) * c->nsite,
+                              hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(std::max(blocks, 1)), dim3(256), 0,
+                           c->stream, c->d_sites64, (float *)c->d_sites, c->nsite);
+        HIPCHK(hipGetLastError());
+    }
+    if (c->vplan_ok) RETCHK(vertex_upload_sites(&c->vplan, c->stream, c->d_sites64, &c->d_vsites));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_sites = true;
+    return BP_OK;
+}
+
+int bp_init_messages(bp_context *c, int kind, const double *host) {
+    if (!c) return set_err(BP_ERR_ARG, "ctx is NULL");
+    if (!c->have_topo) return set_err(BP_ERR_STATE, "bp_set_topology first");
+    HIPCHK(hipSetDevice(c->device));
+    c->cur = 0;
+    c->fp64_active = false;
+    if (kind == BP_INIT_UPLOAD) {
+        if (!host) return set_err(BP_ERR_ARG, "host buffer required for BP_INIT_UPLOAD");
+        if (c->dtype == BP_F64) {
+            HIPCHK(hipMemcpyAsync(c->d_msg[0], host, sizeof(double) * c->nmsg,
+                                  hipMemcpyHostToDevice, c->stream));
+        } else {
+            double *tmp = nullptr;
+            HIPCHK(hipMalloc((void **)&tmp, sizeof(double) * c->nmsg));
+            HIPCHK(hipMemcpyAsync(tmp, host, sizeof(double) * c->nmsg, hipMemcpyHostToDevice,
+                                  c->stream));
+            const int blocks = (int)std::min<long long>((c->nmsg + 255) / 256, 65535);
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(std::max(blocks, 1)), dim3(256),
+                               0, c->stream, tmp, (float *)c->d_msg[0], c->nmsg);
+            HIPCHK(hipStreamSynchronize(c->stream));
+            HIPCHK(hipFree(tmp));
+        }
+    } else if (kind == BP_INIT_IDENTITY || kind == BP_INIT_ONES) {
+        if (c->nports == 0) {
+            // no edges: nothing to initialise
+        } else if (c->dtype == BP_F64)
+            hipLaunchKernelGGL(k_init_messages<double>, dim3(c->nports), dim3(64), 0, c->stream,
+                               c->topo(), (double *)c->d_msg[0], kind);
+        else
+            hipLaunchKernelGGL(k_init_messages<float>, dim3(c->nports), dim3(64), 0, c->stream,
+                               c->topo(), (float *)c->d_msg[0], kind);
+        HIPCHK(hipGetLastError());
+    } else {
+        return set_err(BP_ERR_ARG, "unknown init kind %d", kind);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_msgs = true;
+    return BP_OK;
+}
+
+int bp_download_messages(bp_context *c, double *out) {
+    if (!c || !out) return set_err(BP_ERR_ARG, "NULL argument");
+    if (!c->have_msgs) return set_err(BP_ERR_STATE, "no messages");
+    HIPCHK(hipSetDevice(c->device));
+    if (c->fp64_active) {
+        HIPCHK(hipMemcpyAsync(out, c->d_msg64[c->cur64], sizeof(double) * c->nmsg,
+                              hipMemcpyDeviceToHost, c->stream));
+    } else if (c->dtype == BP_F64) {
+        HIPCHK(hipMemcpyAsync(out, c->d_msg[c->cur], sizeof(double) * c->nmsg,
+                              hipMemcpyDeviceToHost, c->stream));
+    } else {
+        std::vector<float> tmp(c->nmsg);
+        HIPCHK(hipMemcpyAsync(tmp.data(), c->d_msg[c->cur], sizeof(float) * c->nmsg,
+                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (long long i = 0; i < c->nmsg; ++i) out[i] = tmp[i];
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BP_OK;
+}
+
+int bp_set_path(bp_context *c, int path) {
+    if (!c) return set_err(BP_ERR_ARG, "ctx is NULL");
+    if (path != BP_PATH_AUTO && path != BP_PATH_GENERIC && path != BP_PATH_VERTEX)
+        return set_err(BP_ERR_ARG, "unknown path %d", path);
+    if (path == BP_PATH_VERTEX && !c->vplan_ok)
+        return set_err(BP_ERR_UNSUPPORTED, "vertex path not available: %s",
+                       c->vplan.why.c_str());
+    c->path = path;
+    return BP_OK;
+}
+
+int bp_get_path(bp_context *c, int *path) {
+    if (!c || !path) return set_err(BP_ERR_ARG, "NULL argument");
+    *path = resolve_path(c);
+    return BP_OK;
+}
+
+static int ready(bp_context *c) {
+    if (!c) return set_err(BP_ERR_ARG, "ctx is NULL");
+    if (!c->have_topo || !c->have_sites || !c->have_msgs)
+        return set_err(BP_ERR_STATE, "topology, sites and messages must be set first");
+    return BP_OK;
+}
+
+int bp_sweep(bp_context *c) {
+    RETCHK(ready(c));
+    HIPCHK(hipSetDevice(c->device));
+    c->fp64_active = false;
+    RETCHK(sweep_once(c, nullptr));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BP_OK;
+}
+
+int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph, int *out_sweeps,
+           double *out_delta, int *out_at) {
+    RETCHK(ready(c));
+    if (maxiter <= 0 && tol < 0.0)
+        return set_err(BP_ERR_ARG, "at least one of maxiter or tol must be given");
+    if (criterion != BP_CRIT_FIDELITY && criterion != BP_CRIT_MAXABS)
+        return set_err(BP_ERR_ARG, "unknown criterion %d", criterion);
+    HIPCHK(hipSetDevice(c->device));
+    c->fp64_active = false;
+    RETCHK(prepare_sweep(c));
+    const int cap = maxiter > 0 ? maxiter : 100000;  // tol-only runs are capped
+    const double *res = criterion == BP_CRIT_FIDELITY ? c->d_res_fid : c->d_res_abs;
+    hipLaunchKernelGGL(k_reset_nets, dim3((c->nnet + 255) / 256), dim3(256), 0, c->stream,
+                       c->nnet, c->d_done, c->d_iter, c->d_delta, c->d_at);
+    HIPCHK(hipGetLastError());
+    const int chunk = 16;  // sweeps between host checks of the done flags (even)
+    std::vector<int> done(c->nnet);
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    int launched = 0;
+    while (launched < cap) {
+        const int n = std::min(chunk, cap - launched);
+        if (use_graph && n == chunk) {
+            if (!gexec) {
+                const int cur0 = c->cur;
+                HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+                int rc = BP_OK;
+                for (int i = 0; i < n && rc == BP_OK; ++i) {
+                    rc = sweep_once(c, c->d_done);
+                    if (rc == BP_OK) {
+                        hipLaunchKernelGGL(k_reduce_nets, dim3(c->nnet), dim3(BP_BLOCK), 0,
+                                           c->stream, c->d_pbeg, res, cap, tol, c->d_done,
+                                           c->d_iter, c->d_delta, c->d_at);
+                    }
+                }
+                hipError_t e = hipStreamEndCapture(c->stream, &graph);
+                if (rc != BP_OK) return rc;
+                if (e != hipSuccess) return set_err(BP_ERR_HIP, "graph capture failed: %s",
+                                                    hipGetErrorString(e));
+                HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+                c->cur = cur0;  // capture does not execute; replay flips parity n times
+            }
+            HIPCHK(hipGraphLaunch(gexec, c->stream));
+            if (n % 2) c->cur ^= 1;
+        } else {
+            for (int i = 0; i < n; ++i) {
+                RETCHK(sweep_once(c, c->d_done));
+                hipLaunchKernelGGL(k_reduce_nets, dim3(c->nnet), dim3(BP_BLOCK), 0, c->stream,
+                                   c->d_pbeg, res, cap, tol, c->d_done, c->d_iter, c->d_delta,
+                                   c->d_at);
+                HIPCHK(hipGetLastError());
+            }
+        }
+        launched += n;
+        HIPCHK(hipMemcpyAsync(done.data(), c->d_done, sizeof(int) * c->nnet,
+                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        bool all = true;
+        for (int x : done) all = all && x;
+        if (all) break;
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (out_sweeps)
+        HIPCHK(hipMemcpy(out_sweeps, c->d_iter, sizeof(int) * c->nnet, hipMemcpyDeviceToHost));
+    if (out_delta)
+        HIPCHK(hipMemcpy(out_delta, c->d_delta, sizeof(double) * c->nnet, hipMemcpyDeviceToHost));
+    if (out_at)
+        HIPCHK(hipMemcpy(out_at, c->d_at, sizeof(int) * c->nnet, hipMemcpyDeviceToHost));
+    return BP_OK;
+}
+
+int bp_time_sweeps(bp_context *c, int n, double *avg_ms) {
+    RETCHK(ready(c));
+    if (n <= 0 || !avg_ms) return set_err(BP_ERR_ARG, "n must be positive");
+    HIPCHK(hipSetDevice(c->device));
+    c->fp64_active = false;
+    RETCHK(prepare_sweep(c));
+    std::vector<hipEvent_t> ev(2 * n);
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    int rc = BP_OK;
+    for (int i = 0; i < n && rc == BP_OK; ++i) {
+        (void)hipEventRecord(ev[2 * i], c->stream);
+        rc = sweep_once(c, nullptr);
+        (void)hipEventRecord(ev[2 * i + 1], c->stream);
+    }
+    (void)hipStreamSynchronize(c->stream);
+    double tot = 0.0;
+    for (int i = 0; i < n && rc == BP_OK; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) != hipSuccess) rc = BP_ERR_HIP;
+        tot += ms;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    if (rc != BP_OK) return rc == BP_ERR_HIP ? set_err(rc, "event timing failed") : rc;
+    *avg_ms = tot / n;
+    return BP_OK;
+}
+
+int bp_last_run_ms(bp_context *c, double *ms) {
+    if (!c || !ms) return set_err(BP_ERR_ARG, "NULL argument");
+    *ms = c->last_ms;
+    return BP_OK;
+}
+
+int bp_update_slot(bp_context *c, int slot) {
+    RETCHK(ready(c));
+    if (slot < 0 || slot >= c->nports) return set_err(BP_ERR_ARG, "slot %d out of range", slot);
+    HIPCHK(hipSetDevice(c->device));
+    c->fp64_active = false;
+    void *m = c->d_msg[c->cur];
+    if (c->dtype == BP_F64)
+        RETCHK(launch_update_generic<double>(c, (const double *)c->d_sites, (const double *)m,
+                                             (double *)m, nullptr, slot, slot + 1));
+    else
+        RETCHK(launch_update_generic<float>(c, (const float *)c->d_sites, (const float *)m,
+                                            (float *)m, nullptr, slot, slot + 1));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BP_OK;
+}
+
+int bp_residuals(bp_context *c, double *fid, double *maxabs) {
+    if (!c) return set_err(BP_ERR_ARG, "ctx is NULL");
+    if (!c->have_topo) return set_err(BP_ERR_STATE, "no topology");
+    HIPCHK(hipSetDevice(c->device));
+    if (fid) HIPCHK(hipMemcpy(fid, c->d_res_fid, sizeof(double) * c->nports, hipMemcpyDeviceToHost));
+    if (maxabs)
+        HIPCHK(hipMemcpy(maxabs, c->d_res_abs, sizeof(double) * c->nports, hipMemcpyDeviceToHost));
+    return BP_OK;
+}
+
+// current messages as a device pointer + whether it is FP64
+static const void *current_msgs(bp_context *c, bool *is64) {
+    if (c->fp64_active) { *is64 = true; return c->d_msg64[c->cur64]; }
+    *is64 = c->dtype == BP_F64;
+    return c->d_msg[c->cur];
+}
+
+int bp_iterate_diff(bp_context *c, const double *other, double *out) {
+    RETCHK(ready(c));
+    if (!other || !out) return set_err(BP_ERR_ARG, "NULL argument");
+    std::vector<double> mine(c->nmsg);
+    RETCHK(bp_download_messages(c, mine.data()));
+    double mx = -INFINITY;
+    for (int q = 0; q < c->nports; ++q) {
+        double ab = 0, aa = 0, bb = 0;
+        for (long long i = c->msg_off[q]; i < c->msg_off[q + 1]; ++i) {
+            ab += mine[i] * other[i];
+            aa += mine[i] * mine[i];
+            bb += other[i] * other[i];
+        }
+        const double d = ab / std::sqrt(aa * bb);
+        const double v = 1.0 - d * d;
+        if (v > mx || v != v) mx = v;
+    }
+    *out = mx;
+    return BP_OK;
+}
+
+static int scalars(bp_context *c) {
+    bool is64 = false;
+    const void *m = current_msgs(c, &is64);
+    if (is64) {
+        RETCHK(launch_vertex_scalars<double>(c, c->d_sites64, (const double *)m));
+        RETCHK(launch_edge_scalars<double>(c, (const double *)m));
+    } else {
+        RETCHK(launch_vertex_scalars<float>(c, (const float *)c->d_sites, (const float *)m));
+        RETCHK(launch_edge_scalars<float>(c, (const float *)m));
+    }
+    return BP_OK;
+}
+
+int bp_vertex_scalars(bp_context *c, double *zv) {
+    RETCHK(ready(c));
+    if (!zv) return set_err(BP_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(c->device));
+    RETCHK(scalars(c));
+    HIPCHK(hipMemcpyAsync(zv, c->d_zv, sizeof(double) * c->nv, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BP_OK;
+}
+
+int bp_edge_scalars(bp_context *c, double *ze) {
+    RETCHK(ready(c));
+    if (!ze) return set_err(BP_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(c->device));
+    RETCHK(scalars(c));
+    if (c->ne > 0)
+        HIPCHK(hipMemcpyAsync(ze, c->d_ze, sizeof(double) * c->ne, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BP_OK;
+}
+
+int bp_lognorm(bp_context *c, double *out_re, double *out_im) {
+    RETCHK(ready(c));
+    if (!out_re) return set_err(BP_ERR_ARG, "NULL argument");
+    HIPCHK(hipSetDevice(c->device));
+    RETCHK(scalars(c));
+    hipLaunchKernelGGL(k_lognorm_nets, dim3(c->nnet), dim3(BP_BLOCK), 0, c->stream, c->topo(),
+                       c->d_vbeg, c->d_pbeg, c->d_zv, c->d_ze, c->d_lre, c->d_lim);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out_re, c->d_lre, sizeof(double) * c->nnet, hipMemcpyDeviceToHost,
+                          c->stream));
+    if (out_im)
+        HIPCHK(hipMemcpyAsync(out_im, c->d_lim, sizeof(double) * c->nnet, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BP_OK;
+}
+
+int bp_polish_fp64(bp_context *c, int nsweeps, double *out_delta) {
+    RETCHK(ready(c));
+    HIPCHK(hipSetDevice(c->device));
+    if (!c->d_msg64[0]) RETCHK(dev_alloc((void **)&c->d_msg64[0], sizeof(double) * c->nmsg));
+    if (!c->d_msg64[1]) RETCHK(dev_alloc((void **)&c->d_msg64[1], sizeof(double) * c->nmsg));
+    if (!c->fp64_active) {
+        const int blocks = (int)std::max<long long>(1, std::min<long long>((c->nmsg + 255) / 256, 65535));
+        if (c->dtype == BP_F64)
+            HIPCHK(hipMemcpyAsync(c->d_msg64[0], c->d_msg[c->cur], sizeof(double) * c->nmsg,
+                                  hipMemcpyDeviceToDevice, c->stream));
+        else
+            hipLaunchKernelGGL((k_convert<float, double>), dim3(blocks), dim3(256), 0, c->stream,
+                               (const float *)c->d_msg[c->cur], c->d_msg64[0], c->nmsg);
+        c->cur64 = 0;
+        c->fp64_active = true;
+    }
+    for (int i = 0; i < nsweeps; ++i) {
+        RETCHK(launch_update_generic<double>(c, c->d_sites64, c->d_msg64[c->cur64],
+                                             c->d_msg64[c->cur64 ^ 1], nullptr, 0, c->nports));
+        c->cur64 ^= 1;
+    }
+    if (out_delta) {
+        std::vector<double> r(c->nports);
+        HIPCHK(hipMemcpyAsync(r.data(), c->d_res_abs, sizeof(double) * c->nports,
+                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int n = 0; n < c->nnet; ++n) {
+            double m = 0;
+            for (int q = c->pbeg[n]; q < c->pbeg[n + 1]; ++q) m = std::max(m, r[q]);
+            out_delta[n] = nsweeps > 0 ? m : 0.0;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BP_OK;
+}
+
+int bp_grad_lognorm(bp_context *c, double *grad) {
+    RETCHK(ready(c));
+    if (!grad) return set_err(BP_ERR_ARG, "NULL argument");
+    if (c->plain) return set_err(BP_ERR_UNSUPPORTED, "gradient needs a norm network");
+    HIPCHK(hipSetDevice(c->device));
+    bool is64 = false;
+    const void *m = current_msgs(c, &is64);
+    return adjoint_grad_lognorm(&c->adj, c->stream, c->topo(), c->max_site, c->max_mats, c->maxD,
+                                is64 ? BP_F64 : c->dtype, is64 ? (const void *)c->d_sites64
+                                                            : (const void *)c->d_sites,
+                                m, c->nsite, grad);
+}
+
+int bp_tfim_energy(bp_context *c, double J, double h, double *energy, double *grad,
+                   double adj_tol, int adj_maxiter, int *out_iters, double *out_res) {
+    RETCHK(ready(c));
+    if (!energy) return set_err(BP_ERR_ARG, "NULL argument");
+    if (c->plain) return set_err(BP_ERR_UNSUPPORTED, "energy needs a norm network");
+    for (int u = 0; u < c->nv; ++u)
+        if (c->phys_dim[u] != 2) return set_err(BP_ERR_UNSUPPORTED, "TFIM needs phys dim 2");
+    HIPCHK(hipSetDevice(c->device));
+    bool is64 = false;
+    const void *m = current_msgs(c, &is64);
+    return adjoint_tfim(&c->adj, c->stream, c->topo(), c->ne, c->max_site, c->max_mats, c->maxD,
+                        is64 ? BP_F64 : c->dtype,
+                        is64 ? (const void *)c->d_sites64 : (const void *)c->d_sites, m, c->nsite,
+                        c->nmsg, J, h, energy, grad, adj_tol, adj_maxiter, out_iters, out_res);
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// common.h -- shared device/host definitions of the MI355X BP engine (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define BP_MAXR 12    // max site-tensor rank (physical index + ports)
+#define BP_BLOCK 256  // threads per workgroup of the generic kernels
+
+// Flat topology as the kernels read it (device pointers).
+struct DevTopo {
+    const int *port_off;        // nv + 1
+    const int *port_nbr;        // nports
+    const int *port_edge;       // nports
+    const int *port_rev;        // nports
+    const int *edge_dim;        // ne
+    const int *phys_dim;        // nv
+    const int *owner;           // nports: vertex owning each port
+    const int *vnet;            // nv: network of each vertex
+    const long long *site_off;  // nv + 1
+    const long long *msg_off;   // nports + 1
+    int nv;
+    int nports;
+    int plain;
+};
+
+// wave64 reductions (DPP/permute through the compiler's __shfl_xor lowering)
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, 64);
+        v = (w > v || w != w) ? w : v;  // NaN-propagating max
+    }
+    return v;
+}
+
+// block-wide sum of a double over BP_BLOCK threads; `red` is LDS of >= BP_BLOCK/64 doubles.
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) s += red[i];
+    __syncthreads();
+    return s;
+}
+
+__device__ __forceinline__ double block_max(double v, double *red) {
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = red[0];
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 1; i < nw; ++i) s = (red[i] > s || red[i] != red[i]) ? red[i] : s;
+    __syncthreads();
+    return s;
+}

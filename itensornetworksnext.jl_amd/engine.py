@@ -1,0 +1,207 @@
+"""Engine: one device context holding a network (or a batch), its site tensors and
+its messages, driven through the C ABI.
+
+This is the plumbing under the reference-shaped API in `beliefpropagation.py`; every
+call lands in HIP kernels of libbpmi355x.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+from .graphs import Topology
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int)) if a is not None else None
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+@dataclass
+class RunResult:
+    sweeps: np.ndarray       # per network
+    delta: np.ndarray        # last delta per network
+    at_iteration: np.ndarray  # converged-at iteration per network (-1: never)
+    ms: float                # device time of the run (HIP events)
+
+    @property
+    def converged(self) -> np.ndarray:
+        return self.at_iteration >= 0
+
+
+class Engine:
+    """Device-resident BP state for one topology.
+
+    dtype: "f64" | "f32" | "bf16" (contraction precision).  device: HIP ordinal.
+    """
+
+    def __init__(self, topo: Topology, sites=None, dtype: str = "f64", device: int = 0,
+                 path: str = "auto"):
+        self.lib = L.load()
+        self.topo = topo
+        self.dtype = dtype
+        h = C.c_void_p()
+        L.check(self.lib.bp_create(C.byref(h), int(device), L.DTYPES[dtype]), "bp_create")
+        self.h = h
+        self._keep = {k: _i32(getattr(topo, k)) for k in
+                      ("port_off", "port_nbr", "port_edge", "port_rev", "edge_dim", "phys_dim")}
+        self._keep["vnet"] = _i32(topo.vertex_net)
+        k = self._keep
+        L.check(self.lib.bp_set_topology(self.h, topo.nv, topo.ne, _ip(k["port_off"]),
+                                         _ip(k["port_nbr"]), _ip(k["port_edge"]),
+                                         _ip(k["port_rev"]), _ip(k["edge_dim"]),
+                                         _ip(k["phys_dim"]), _ip(k["vnet"]), int(topo.nnet),
+                                         int(topo.plain)), "bp_set_topology")
+        self.set_path(path)
+        if sites is not None:
+            self.set_sites(sites)
+
+    # -- lifecycle
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.bp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- data
+    def set_sites(self, sites) -> None:
+        s = _f64(sites)
+        if s.size != int(self.topo.site_off[-1]):
+            raise ValueError(f"sites has {s.size} entries, topology needs {self.topo.site_off[-1]}")
+        L.check(self.lib.bp_upload_sites(self.h, _dp(s)), "bp_upload_sites")
+
+    def set_messages(self, msgs) -> None:
+        m = _f64(msgs)
+        if m.size != int(self.topo.msg_off[-1]):
+            raise ValueError("message buffer size does not match the topology")
+        L.check(self.lib.bp_init_messages(self.h, L.BP_INIT_UPLOAD, _dp(m)), "bp_init_messages")
+
+    def init_messages(self, kind: str = "identity") -> None:
+        code = {"identity": L.BP_INIT_IDENTITY, "one": L.BP_INIT_IDENTITY,
+                "ones": L.BP_INIT_ONES}[kind]
+        L.check(self.lib.bp_init_messages(self.h, code, None), "bp_init_messages")
+
+    def messages(self) -> np.ndarray:
+        out = np.zeros(int(self.topo.msg_off[-1]))
+        L.check(self.lib.bp_download_messages(self.h, _dp(out)), "bp_download_messages")
+        return out
+
+    # -- path
+    def set_path(self, path: str) -> None:
+        L.check(self.lib.bp_set_path(self.h, L.PATHS[path]), "bp_set_path")
+
+    def path(self) -> str:
+        p = C.c_int(0)
+        L.check(self.lib.bp_get_path(self.h, C.byref(p)), "bp_get_path")
+        return {L.BP_PATH_GENERIC: "generic", L.BP_PATH_VERTEX: "vertex"}[p.value]
+
+    # -- BP
+    def sweep(self) -> None:
+        L.check(self.lib.bp_sweep(self.h), "bp_sweep")
+
+    def run(self, maxiter: Optional[int] = None, tol: Optional[float] = None,
+            criterion: str = "fidelity", use_graph: bool = True) -> RunResult:
+        nn = int(self.topo.nnet)
+        sw = np.zeros(nn, dtype=np.int32)
+        dl = np.zeros(nn)
+        at = np.zeros(nn, dtype=np.int32)
+        crit = L.BP_CRIT_FIDELITY if criterion == "fidelity" else L.BP_CRIT_MAXABS
+        L.check(self.lib.bp_run(self.h, int(maxiter) if maxiter is not None else 0,
+                                float(tol) if tol is not None else -1.0, crit,
+                                int(bool(use_graph)), _ip(sw), _dp(dl), _ip(at)), "bp_run")
+        ms = C.c_double(0)
+        L.check(self.lib.bp_last_run_ms(self.h, C.byref(ms)), "bp_last_run_ms")
+        return RunResult(sw, dl, at, ms.value)
+
+    def time_sweeps(self, n: int) -> float:
+        """Average device milliseconds of one sweep's kernels over n back-to-back sweeps."""
+        ms = C.c_double(0)
+        L.check(self.lib.bp_time_sweeps(self.h, int(n), C.byref(ms)), "bp_time_sweeps")
+        return ms.value
+
+    def update_slot(self, q: int) -> None:
+        L.check(self.lib.bp_update_slot(self.h, int(q)), "bp_update_slot")
+
+    def residuals(self):
+        fid = np.zeros(self.topo.nports)
+        mab = np.zeros(self.topo.nports)
+        L.check(self.lib.bp_residuals(self.h, _dp(fid), _dp(mab)), "bp_residuals")
+        return fid, mab
+
+    def iterate_diff(self, other) -> float:
+        out = C.c_double(0)
+        L.check(self.lib.bp_iterate_diff(self.h, _dp(_f64(other)), C.byref(out)),
+                "bp_iterate_diff")
+        return out.value
+
+    # -- scalars
+    def vertex_scalars(self) -> np.ndarray:
+        out = np.zeros(self.topo.nv)
+        L.check(self.lib.bp_vertex_scalars(self.h, _dp(out)), "bp_vertex_scalars")
+        return out
+
+    def edge_scalars(self) -> np.ndarray:
+        out = np.zeros(max(self.topo.ne, 1))
+        L.check(self.lib.bp_edge_scalars(self.h, _dp(out)), "bp_edge_scalars")
+        return out[: self.topo.ne]
+
+    def lognorms(self) -> np.ndarray:
+        nn = int(self.topo.nnet)
+        re = np.zeros(nn)
+        im = np.zeros(nn)
+        L.check(self.lib.bp_lognorm(self.h, _dp(re), _dp(im)), "bp_lognorm")
+        return np.where(im != 0.0, re + 1j * im, re) if np.any(im != 0.0) else re
+
+    def lognorm(self):
+        v = self.lognorms()[0]
+        return complex(v) if isinstance(v, complex) or np.iscomplexobj(v) else float(v)
+
+    def polish_fp64(self, nsweeps: int = 3) -> np.ndarray:
+        out = np.zeros(int(self.topo.nnet))
+        L.check(self.lib.bp_polish_fp64(self.h, int(nsweeps), _dp(out)), "bp_polish_fp64")
+        return out
+
+    # -- gradients
+    def grad_lognorm(self) -> np.ndarray:
+        out = np.zeros(int(self.topo.site_off[-1]))
+        L.check(self.lib.bp_grad_lognorm(self.h, _dp(out)), "bp_grad_lognorm")
+        return out
+
+    def tfim_energy(self, J: float = 1.0, h: float = 3.0, grad: bool = False,
+                    adj_tol: float = 1e-8, adj_maxiter: int = 1000):
+        E = C.c_double(0)
+        g = np.zeros(int(self.topo.site_off[-1])) if grad else None
+        it = C.c_int(0)
+        res = C.c_double(0)
+        L.check(self.lib.bp_tfim_energy(self.h, float(J), float(h), C.byref(E), _dp(g),
+                                        float(adj_tol), int(adj_maxiter), C.byref(it),
+                                        C.byref(res)), "bp_tfim_energy")
+        if grad:
+            return E.value, g, it.value, res.value
+        return E.value

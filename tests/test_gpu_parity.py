@@ -1,0 +1,181 @@
+"""GPU parity: the HIP engine against the FP64 CPU oracle, through the C ABI.
+
+Tolerances: FP64 runs agree with the oracle within 1e-10 (messages and BP log-norm);
+FP32 runs within the bounds stated per test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _grid(pkg, L, D, d=2, periodic=False):
+    return pkg.build_topology(pkg.named_grid((L, L), periodic=periodic), D, d)
+
+
+@pytest.mark.parametrize("path", ["generic", "auto"])
+def test_c1_fp64_converged_parity(pkg, orc, path):
+    """Config 1: 4x4, d=2, D=2, FP64, identity init, Jacobi, tol 1e-10 on max change."""
+    _say(f"case c1 fp64 path={path}")
+    t = _grid(pkg, 4, 2)
+    s = pkg.peps.random_sites(t, 1234)
+    m0 = pkg.peps.identity_messages(t)
+    want, sw_o, dl_o, at_o = orc.bp(t, s, m0, maxiter=100, tol=1e-10, criterion="maxabs")
+    with pkg.Engine(t, s, dtype="f64", path=path if path == "generic" else "auto") as e:
+        e.set_messages(m0)
+        r = e.run(maxiter=100, tol=1e-10, criterion="maxabs")
+        got = e.messages()
+        assert r.sweeps[0] == sw_o and r.at_iteration[0] == at_o
+        assert np.max(np.abs(got - want)) < 1e-10
+        ln = e.lognorm()
+        assert abs(ln - orc.bethe_free_energy(t, s, want)) < 1e-10
+        # BP vs exact <psi|psi>: the loopy-BP gap is reported, not asserted to vanish
+        exact = orc.norm_exact(t, s)
+        _say(f"c1 lognorm bp={ln:.12f} exact={math.log(exact):.12f}")
+
+
+def test_single_sweep_parity_fp64(pkg, orc):
+    """One Jacobi sweep from random non-symmetric messages on a ragged lattice
+    (degrees 2/3/4, mixed bond dims), before any convergence."""
+    _say("case single sweep fp64 ragged")
+    g = pkg.named_grid((5, 4))
+    dims = {e: 2 + (i % 3) for i, e in enumerate(g.edges())}
+    t = pkg.build_topology(g, dims, 2)
+    s = pkg.peps.random_sites(t, 7)
+    m = pkg.peps.random_messages(t, 7)
+    want, fid, mab = orc.sweep_jacobi(t, s, m)
+    with pkg.Engine(t, s, dtype="f64") as e:
+        e.set_messages(m)
+        e.sweep()
+        got = e.messages()
+        gfid, gmab = e.residuals()
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(gmab, mab, rtol=1e-9, atol=1e-14)
+    np.testing.assert_allclose(gfid, fid, rtol=1e-6, atol=1e-13)
+
+
+@pytest.mark.parametrize("dims,D", [((2, 1), 2), ((4, 3), 3)])
+def test_norm_tree_exact_on_gpu(pkg, orc, dims, D):
+    _say(f"case norm tree {dims} D={D}")
+    g = pkg.named_comb_tree(dims) if dims[1] > 1 else pkg.named_path_graph(dims[0])
+    t = pkg.build_topology(g, D, 2)
+    s = pkg.peps.random_sites(t, 11)
+    with pkg.Engine(t, s, dtype="f64") as e:
+        e.init_messages("ones")
+        r = e.run(maxiter=60, tol=1e-15, criterion="maxabs")
+        assert r.converged[0]
+        z = math.exp(e.lognorm())
+    assert z / orc.norm_exact(t, s) == pytest.approx(1.0, abs=1e-12)
+
+
+def test_plain_tree_and_spin_ice_sequential(pkg, orc):
+    """Plain networks: tree exactness (Jacobi) and spin ice via per-slot sequential
+    updates in the forest-cover order (the reference's default schedule)."""
+    _say("case plain tree + spin ice")
+    g = pkg.named_comb_tree((4, 3))
+    t = pkg.build_topology(g, 3, plain=True)
+    s = pkg.peps.random_sites(t, 3)
+    with pkg.Engine(t, s, dtype="f64") as e:
+        e.init_messages("ones")
+        e.run(maxiter=30, tol=1e-15, criterion="maxabs")
+        z = e.lognorm()
+    assert np.exp(complex(z)).real / orc.plain_exact(t, s) == pytest.approx(1.0, abs=1e-11)
+    for n in (3, 4, 5):
+        g = pkg.named_grid((n, n), periodic=True)
+        t = pkg.build_topology(g, 2, plain=True)
+        s = pkg.peps.spin_ice_sites(t)
+        order = [t.slot(a, b) for a, b in pkg.forest_cover_edge_sequence(g)]
+        with pkg.Engine(t, s, dtype="f64") as e:
+            e.set_messages(pkg.peps.random_messages(t, 123))
+            for _ in range(10):
+                for q in order:
+                    e.update_slot(q)
+            z = math.exp(e.lognorm())
+        assert z == pytest.approx(1.5 ** (n * n), rel=1e-8)
+
+
+def test_scalars_parity(pkg, orc):
+    _say("case scalars")
+    t = _grid(pkg, 5, 3)
+    s = pkg.peps.random_sites(t, 5)
+    m = pkg.peps.random_messages(t, 5)
+    with pkg.Engine(t, s, dtype="f64") as e:
+        e.set_messages(m)
+        np.testing.assert_allclose(e.vertex_scalars(), orc.vertex_scalars(t, s, m), rtol=1e-12)
+        np.testing.assert_allclose(e.edge_scalars(), orc.edge_scalars(t, m), rtol=1e-12)
+        assert e.lognorm() == pytest.approx(orc.bethe_free_energy(t, s, m), abs=1e-10)
+        assert e.iterate_diff(m) == pytest.approx(0.0, abs=1e-14)
+
+
+def test_graph_and_eager_agree(pkg, orc):
+    _say("case graph vs eager")
+    t = _grid(pkg, 6, 3)
+    s = pkg.peps.random_sites(t, 3, scale=pkg.peps.peps_scale(2, 3))
+    out = []
+    for use_graph in (False, True, True):
+        with pkg.Engine(t, s, dtype="f64") as e:
+            e.init_messages("identity")
+            r = e.run(maxiter=200, tol=1e-12, use_graph=use_graph)
+            out.append((e.messages(), r.sweeps[0], r.at_iteration[0]))
+    for m, sw, at in out[1:]:
+        assert sw == out[0][1] and at == out[0][2]
+        np.testing.assert_array_equal(m, out[0][0])
+
+
+def test_batch_matches_individual_runs(pkg, orc):
+    _say("case batch")
+    tops = [_grid(pkg, L, D) for L, D in ((3, 2), (5, 3), (4, 2))]
+    sites = [pkg.peps.random_sites(t, 40 + i, scale=pkg.peps.peps_scale(2, int(t.edge_dim[0])))
+             for i, t in enumerate(tops)]
+    b = pkg.concat_topologies(tops)
+    with pkg.Engine(b, np.concatenate(sites), dtype="f64") as e:
+        e.init_messages("identity")
+        r = e.run(maxiter=300, tol=1e-12, criterion="maxabs")
+        mb = e.messages()
+        ln = e.lognorms()
+    off = 0
+    for i, (t, s) in enumerate(zip(tops, sites)):
+        want, sw, dl, at = orc.bp(t, s, pkg.peps.identity_messages(t), maxiter=300, tol=1e-12,
+                                  criterion="maxabs")
+        n = int(t.msg_off[-1])
+        assert r.sweeps[i] == sw and r.at_iteration[i] == at
+        np.testing.assert_allclose(mb[off:off + n], want, atol=1e-10)
+        assert ln[i] == pytest.approx(orc.bethe_free_energy(t, s, want), abs=1e-10)
+        off += n
+
+
+def test_fp32_parity_16x16_d8(pkg, orc):
+    """FP32 contractions against the FP64 oracle on a 16x16 D=8 lattice."""
+    _say("case fp32 16x16 D=8")
+    t = _grid(pkg, 16, 8)
+    s = pkg.peps.random_sites(t, 1, scale=pkg.peps.peps_scale(2, 8))
+    m0 = pkg.peps.identity_messages(t)
+    want, sw, dl, at = orc.bp(t, s, m0, maxiter=200, tol=1e-7, nthreads=8)
+    with pkg.Engine(t, s, dtype="f32") as e:
+        e.set_messages(m0)
+        r = e.run(maxiter=200, tol=1e-7)
+        got = e.messages()
+        ln = e.lognorm()
+        _say(f"fp32 path={e.path()} sweeps={r.sweeps[0]} oracle sweeps={sw}")
+    rel = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    assert rel < 1e-4
+    lw = orc.bethe_free_energy(t, s, want)
+    assert abs(ln - lw) / abs(lw) < 1e-5
+
+
+def test_errors_are_reported(pkg):
+    _say("case errors")
+    t = _grid(pkg, 3, 2)
+    s = pkg.peps.random_sites(t, 0)
+    e = pkg.Engine(t, s, dtype="f64")
+    with pytest.raises(pkg._lib.BPError):
+        e.run()  # neither maxiter nor tol
+    with pytest.raises(pkg._lib.BPError):
+        e.update_slot(10 ** 6)
+    e.close()

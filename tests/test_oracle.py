@@ -1,0 +1,283 @@
+"""Pins of the FP64 CPU oracle (the checker) against the reference's own known answers
+and against what the mathematics fixes.  CPU only.
+
+Reference tests mirrored (ITensorNetworksNext.jl test/):
+  test_beliefpropagation.jl:153-202  BP on a chain / comb tree equals exact contraction
+  test_beliefpropagation.jl:204-225  spin ice: exp(bethe_free_energy) == 1.5^(n^2)
+  test_beliefpropagation.jl:134-150  iterate_diff(c, copy(c)) ~ 0
+  test_contract_network.jl:13-24     ABCD == 15.5 * [-2, 3, 4, 5, 1]
+  test_contract_network.jl:26-45     contraction order independence on a 4x4 grid
+  test_normnetwork.jl:137-166        <psi|psi> == ||psi||^2, single normalised tensor == 1
+  test_itensornetworkgenerators.jl:29-54 (with test/utils.jl:12-39) 1D Ising closed forms
+"""
+import math
+
+import numpy as np
+import pytest
+
+
+def _norm_tree(pkg, dims, D, seed):
+    g = pkg.named_comb_tree(dims) if dims[1] > 1 else pkg.named_path_graph(dims[0])
+    t = pkg.build_topology(g, D, 2)
+    s = pkg.peps.random_sites(t, seed)
+    return g, t, s
+
+
+@pytest.mark.parametrize("dims,D", [((2, 1), 2), ((4, 3), 3)])
+def test_norm_tree_bp_is_exact(pkg, orc, dims, D):
+    g, t, s = _norm_tree(pkg, dims, D, 11)
+    exact = orc.norm_exact(t, s)
+    # Jacobi to convergence
+    m, sw, dl, at = orc.bp(t, s, pkg.peps.ones_messages(t), maxiter=60, tol=1e-15,
+                           criterion="maxabs")
+    assert at > 0
+    z = math.exp(orc.bethe_free_energy(t, s, m))
+    assert abs(z / exact - 1) < 1e-12
+    # one sequential forest-cover sweep (the reference runs maxiter = 1)
+    order = [t.slot(a, b) for a, b in pkg.forest_cover_edge_sequence(g)]
+    ms = orc.sweep_sequential(t, s, pkg.peps.ones_messages(t), order)
+    assert abs(math.exp(orc.bethe_free_energy(t, s, ms)) / exact - 1) < 1e-12
+
+
+def test_plain_tree_bp_is_exact(pkg, orc):
+    g = pkg.named_comb_tree((4, 3))
+    t = pkg.build_topology(g, 3, plain=True)
+    s = pkg.peps.random_sites(t, 3)
+    order = [t.slot(a, b) for a, b in pkg.forest_cover_edge_sequence(g)]
+    ms = orc.sweep_sequential(t, s, pkg.peps.ones_messages(t), order)
+    z = orc.bethe_free_energy(t, s, ms)
+    exact = orc.plain_exact(t, s)
+    assert abs(np.exp(complex(z)) / exact - 1) < 1e-12
+
+
+@pytest.mark.parametrize("n", [3, 4, 5])
+def test_spin_ice_pauling(pkg, orc, n):
+    g = pkg.named_grid((n, n), periodic=True)
+    t = pkg.build_topology(g, 2, plain=True)
+    s = pkg.peps.spin_ice_sites(t)
+    order = [t.slot(a, b) for a, b in pkg.forest_cover_edge_sequence(g)]
+    m, sw, dl, at = orc.bp(t, s, pkg.peps.random_messages(t, 123), maxiter=10, tol=1e-10,
+                           schedule="sequential", order=order)
+    z = math.exp(orc.bethe_free_energy(t, s, m))
+    assert z == pytest.approx(1.5 ** (n * n), rel=1e-8)
+
+
+def test_product_state_exact(pkg, orc):
+    g = pkg.named_grid((3, 3))
+    t = pkg.build_topology(g, 1, 2)
+    s = pkg.peps.random_sites(t, 5)
+    m, _, _, _ = orc.bp(t, s, pkg.peps.identity_messages(t), maxiter=3)
+    z = math.exp(orc.bethe_free_energy(t, s, m))
+    prod = np.prod([np.sum(s[t.site_off[u]:t.site_off[u + 1]] ** 2) for u in range(t.nv)])
+    assert z == pytest.approx(prod, rel=1e-13)
+    assert orc.norm_exact(t, s) == pytest.approx(prod, rel=1e-13)
+
+
+def test_contract_abcd(orc):
+    A = np.array([[1.0, 1.0], [0.5, 1.0]])
+    B = np.array([2.0, 1.0])
+    Cv = np.array([5.0, 1.0])
+    Dv = np.array([-2.0, 3.0, 4.0, 5.0, 1.0])
+    val, labs = orc.contract_labeled([A, B, Cv, Dv], [("i", "j"), ("i",), ("j",), ("k",)])
+    assert labs == ["k"]
+    np.testing.assert_array_equal(val, 15.5 * Dv)
+    val2, _ = orc.contract_labeled([Dv, Cv, A, B], [("k",), ("j",), ("i", "j"), ("i",)])
+    np.testing.assert_array_equal(val2, 15.5 * Dv)
+
+
+def test_contract_order_independence(pkg, orc):
+    g = pkg.named_grid((4, 4))
+    t = pkg.build_topology(g, 2, plain=True)
+    s = pkg.peps.random_sites(t, 9)
+    tensors, labels = [], []
+    for u in range(t.nv):
+        shp = t.site_shape(u)[1:]
+        tensors.append(s[t.site_off[u]:t.site_off[u + 1]].reshape(shp))
+        labels.append(tuple(int(t.port_edge[q]) for q in range(t.port_off[u], t.port_off[u + 1])))
+    z1, _ = orc.contract_labeled(tensors, labels)
+    perm = np.random.default_rng(0).permutation(len(tensors))
+    z2, _ = orc.contract_labeled([tensors[i] for i in perm], [labels[i] for i in perm])
+    assert abs(z1 - z2) / abs(z1) <= 1e3 * np.finfo(float).eps
+
+
+def test_norm_network_physics(pkg, orc):
+    # single normalised tensor contracts to 1
+    g = pkg.NamedGraph(vertices=[1])
+    t = pkg.build_topology(g, 1, 3)
+    v = np.random.default_rng(1).standard_normal(3)
+    assert orc.norm_exact(t, v / np.linalg.norm(v)) == pytest.approx(1.0, rel=1e-14)
+    # 2x2 grid: <psi|psi> = ||psi||^2 with psi built by numpy einsum (a library routine)
+    g = pkg.named_grid((2, 2))
+    t = pkg.build_topology(g, 2, 2)
+    s = pkg.peps.random_sites(t, 2)
+    A = [s[t.site_off[u]:t.site_off[u + 1]].reshape(t.site_shape(u)) for u in range(t.nv)]
+    lab = "abcd"
+    subs = []
+    for u in range(t.nv):
+        subs.append("spqr"[u] + "".join(lab[t.port_edge[q]] for q in range(t.port_off[u], t.port_off[u + 1])))
+    psi = np.einsum(",".join(subs) + "->spqr", *A)
+    assert orc.norm_exact(t, s) == pytest.approx(float(np.sum(psi ** 2)), rel=1e-13)
+
+
+def test_iterate_diff_identical(pkg, orc):
+    t = pkg.build_topology(pkg.named_grid((2,)), 2, 2)
+    m = pkg.peps.ones_messages(t)
+    assert orc.iterate_diff(t, m, m.copy()) == pytest.approx(0.0, abs=10 * np.finfo(float).eps)
+
+
+def _ising_chain(pkg, n, beta, periodic):
+    """1D Ising plain network: each bond matrix split as sqrt(M) onto its two vertices
+    (the construction of src/ITensorNetworkGenerators/ising_network.jl:7-51 with h = 0)."""
+    g = pkg.named_path_graph(n)
+    if periodic:
+        g.add_edge(n, 1)
+    t = pkg.build_topology(g, 2, plain=True)
+    M = np.array([[math.exp(beta), math.exp(-beta)], [math.exp(-beta), math.exp(beta)]])
+    w, V = np.linalg.eigh(M)
+    S = V @ np.diag(np.sqrt(w)) @ V.T
+    s = np.zeros(int(t.site_off[-1]))
+    for u in range(t.nv):
+        k = t.degree(u)
+        T = np.zeros((2,) * k)
+        for spin in range(2):
+            T[(spin,) * k] = 1.0
+        for j in range(k):
+            T = np.moveaxis(np.tensordot(T, S, axes=([j], [0])), -1, j)
+        s[t.site_off[u]:t.site_off[u + 1]] = T.ravel()
+    return g, t, s
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_ising_1d_closed_form(pkg, orc, periodic):
+    n, beta = 4, 0.4
+    g, t, s = _ising_chain(pkg, n, beta, periodic)
+    Z = orc.plain_exact(t, s)
+    if periodic:
+        lp, lm = 2 * math.cosh(beta), 2 * math.sinh(beta)
+        want = lp ** n + lm ** n
+    else:
+        want = 2 * (2 * math.cosh(beta)) ** (n - 1)
+    assert Z == pytest.approx(want, rel=1e-13)
+    if not periodic:  # a path is a tree: BP exact
+        order = [t.slot(a, b) for a, b in pkg.forest_cover_edge_sequence(g)]
+        ms = orc.sweep_sequential(t, s, pkg.peps.ones_messages(t), order)
+        assert math.exp(orc.bethe_free_energy(t, s, ms)) == pytest.approx(want, rel=1e-12)
+
+
+def test_jacobi_and_sequential_share_fixed_point(pkg, orc):
+    g = pkg.named_grid((4, 4))
+    t = pkg.build_topology(g, 2, 2)
+    s = pkg.peps.random_sites(t, 1234)
+    mj, _, _, atj = orc.bp(t, s, pkg.peps.identity_messages(t), maxiter=400, tol=1e-14,
+                           criterion="maxabs")
+    order = [t.slot(a, b) for a, b in pkg.forest_cover_edge_sequence(g)]
+    ms, _, _, ats = orc.bp(t, s, pkg.peps.identity_messages(t), maxiter=400, tol=1e-14,
+                           criterion="maxabs", schedule="sequential", order=order)
+    assert atj > 0 and ats > 0
+    assert np.max(np.abs(mj - ms)) < 1e-11
+    assert orc.bethe_free_energy(t, s, mj) == pytest.approx(orc.bethe_free_energy(t, s, ms),
+                                                            abs=1e-11)
+
+
+def test_bethe_message_scale_invariance(pkg, orc):
+    t = pkg.build_topology(pkg.named_grid((3, 3)), 2, 2)
+    s = pkg.peps.random_sites(t, 4)
+    m, _, _, _ = orc.bp(t, s, pkg.peps.identity_messages(t), maxiter=50)
+    scaled = m.copy()
+    rng = np.random.default_rng(0)
+    for q in range(t.nports):
+        scaled[t.msg_off[q]:t.msg_off[q + 1]] *= rng.uniform(0.1, 10.0)
+    assert orc.bethe_free_energy(t, s, scaled) == pytest.approx(
+        orc.bethe_free_energy(t, s, m), abs=1e-12)
+
+
+def test_message_update_matches_einsum(pkg, orc):
+    """One interior message of a 3x3 lattice with random (non-symmetric) incoming
+    messages, against numpy.einsum of the defining contraction."""
+    t = pkg.build_topology(pkg.named_grid((3, 3)), 3, 2)
+    s = pkg.peps.random_sites(t, 8)
+    m = pkg.peps.random_messages(t, 8)
+    u = t.vertices.index((2, 2))
+    A = s[t.site_off[u]:t.site_off[u + 1]].reshape(t.site_shape(u))
+    ports = list(range(t.port_off[u], t.port_off[u + 1]))
+    M = [m[t.msg_off[t.port_rev[q]]:t.msg_off[t.port_rev[q] + 1]].reshape(3, 3) for q in ports]
+    for p, q in enumerate(ports):
+        got = orc.message(t, s, m, q).reshape(3, 3)
+        ket, bra = "abcd", "ABCD"
+        terms = ["s" + ket, "s" + bra]
+        for j in range(4):
+            if j != p:
+                terms.append(ket[j] + bra[j])
+        ops = [A, A] + [M[j] for j in range(4) if j != p]
+        want = np.einsum(",".join(terms) + "->" + ket[p] + bra[p], *ops)
+        np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-14)
+
+
+# ------------------------------------------------------------------ adjoint pins
+
+
+def _converged(pkg, orc, L, D, seed, scale=True):
+    t = pkg.build_topology(pkg.named_grid((L, L)), D, 2)
+    sc = pkg.peps.peps_scale(2, D) if scale else None
+    s = pkg.peps.random_sites(t, seed, scale=sc)
+    m, _, _, at = orc.bp(t, s, pkg.peps.identity_messages(t), maxiter=2000, tol=1e-15,
+                         criterion="maxabs")
+    assert at > 0
+    return t, s, m
+
+
+def _fd(pkg, orc, t, s, m, f, idx, h):
+    out = []
+    for i in idx:
+        vals = []
+        for sgn in (1, -1):
+            s2 = s.copy()
+            s2[i] += sgn * h
+            m2, _, _, _ = orc.bp(t, s2, m, maxiter=2000, tol=1e-15, criterion="maxabs")
+            vals.append(f(t, s2, m2))
+        out.append((vals[0] - vals[1]) / (2 * h))
+    return np.array(out)
+
+
+def test_grad_lognorm_closed_form_vs_fd(pkg, orc):
+    t, s, m = _converged(pkg, orc, 3, 2, 21, scale=False)
+    g = orc.grad_lognorm_explicit(t, s, m)
+    idx = np.random.default_rng(3).choice(s.size, 8, replace=False)
+    fd = _fd(pkg, orc, t, s, m, lambda t_, s_, m_: orc.bethe_free_energy(t_, s_, m_), idx, 1e-5)
+    np.testing.assert_allclose(g[idx], fd, rtol=1e-6, atol=1e-8)
+    # the full implicit-function gradient agrees: the message term vanishes at the fixed
+    # point (Bethe stationarity)
+    full, it, res = orc.fixed_point_gradient(t, s, m, None, g, tol=1e-13)
+    np.testing.assert_allclose(full, g, rtol=1e-9, atol=1e-11)
+
+
+def test_tfim_gradient_vs_fd(pkg, orc):
+    t, s, m = _converged(pkg, orc, 3, 2, 22, scale=False)
+    E, dm, da = orc.tfim_energy(t, s, m, J=1.0, h=3.0, partials=True)
+    grad, it, res = orc.fixed_point_gradient(t, s, m, dm, da, maxiter=5000, tol=1e-14)
+    assert res < 1e-12
+    idx = np.random.default_rng(4).choice(s.size, 8, replace=False)
+    fd = _fd(pkg, orc, t, s, m, lambda t_, s_, m_: orc.tfim_energy(t_, s_, m_, 1.0, 3.0), idx,
+             1e-5)
+    np.testing.assert_allclose(grad[idx], fd, rtol=1e-6, atol=1e-8)
+    # dropping the adjoint (message) term is visibly wrong: the adjoint matters here
+    assert np.max(np.abs(da[idx] - fd)) > 1e-4 * np.max(np.abs(fd))
+
+
+def test_tfim_product_state_closed_form(pkg, orc):
+    """D = 1: BP is exact and <O> factorises into single-site expectations."""
+    g = pkg.named_grid((3, 3))
+    t = pkg.build_topology(g, 1, 2)
+    s = pkg.peps.random_sites(t, 6)
+    m, _, _, _ = orc.bp(t, s, pkg.peps.identity_messages(t), maxiter=3)
+    J, h = 1.0, 3.0
+    ex, ez = [], []
+    for u in range(t.nv):
+        a = s[t.site_off[u]:t.site_off[u + 1]]
+        n2 = a @ a
+        ex.append(2 * a[0] * a[1] / n2)
+        ez.append((a[0] ** 2 - a[1] ** 2) / n2)
+    want = -h * sum(ex)
+    for (x, y) in t.edges:
+        want -= J * ez[t.vertices.index(x)] * ez[t.vertices.index(y)]
+    assert orc.tfim_energy(t, s, m, J, h) == pytest.approx(want, rel=1e-13)
