@@ -85,10 +85,10 @@ __global__ void k_reset_nets(int nnet, int *done, int *iter, double *delta, int 
 // network): StopWhenConverged on `res` (fidelity or max-abs per slot) and
 // StopAfterIteration(maxiter) (AlgorithmsInterfaceExtensions.jl:84-109,
 // beliefpropagation.jl:27-55).
-__global__ __launch_bounds__(BP_BLOCK) void k_reduce_nets(const int *pbeg, const double *res,
-                                                           int maxiter, double tol, int *done,
-                                                           int *iter, double *delta, int *at) {
-    __shared__ double red[BP_BLOCK / 64];
+__global__ __launch_bounds__(1024) void k_reduce_nets(const int *pbeg, const double *res,
+                                                      int maxiter, double tol, int *done,
+                                                      int *iter, double *delta, int *at) {
+    __shared__ double red[16];
     const int n = blockIdx.x;
     if (done[n]) return;  // uniform over the workgroup
     double m = 0.0;
@@ -308,11 +308,11 @@ static int ensure_ws(bp_context *c, const GenLaunch &g, size_t esz) {
 
 template <typename T>
 static int launch_update_generic(bp_context *c, const T *sites, const T *mold, T *mnew,
-                                 const int *done, int q0, int q1) {
+                                 const NetCtl &ctl, int q0, int q1) {
     GenLaunch g = gen_launch(c, q1 - q0, sizeof(T));
     RETCHK(ensure_ws(c, g, sizeof(T)));
     hipLaunchKernelGGL(k_update_generic<T>, dim3(g.grid), dim3(BP_BLOCK), g.lds_bytes, c->stream,
-                       c->topo(), sites, mold, mnew, done, c->d_res_fid, c->d_res_abs, q0, q1,
+                       c->topo(), sites, mold, mnew, ctl, c->d_res_fid, c->d_res_abs, q0, q1,
                        g.mats_elems, g.out_elems, g.red_elems, g.buf_elems, g.use_lds,
                        (T *)c->d_ws, g.ws_stride);
     HIPCHK(hipGetLastError());
@@ -361,8 +361,25 @@ static int prepare_sweep(bp_context *c) {
     return BP_OK;
 }
 
+// Convergence control for the sweep kernels (disabled: plain sweeps).
+static NetCtl make_ctl(bp_context *c, bool enabled) {
+    NetCtl k;
+    k.done = c->d_done;
+    k.enabled = enabled ? 1 : 0;
+    return k;
+}
+
+// Per-network bookkeeping after a sweep: one workgroup of 1024 threads per network.
+static int launch_reduce(bp_context *c, int crit, int maxiter, double tol) {
+    const double *res = crit == BP_CRIT_FIDELITY ? c->d_res_fid : c->d_res_abs;
+    hipLaunchKernelGGL(k_reduce_nets, dim3(c->nnet), dim3(1024), 0, c->stream, c->d_pbeg, res,
+                       maxiter, tol, c->d_done, c->d_iter, c->d_delta, c->d_at);
+    HIPCHK(hipGetLastError());
+    return BP_OK;
+}
+
 // One Jacobi sweep on the working dtype: reads d_msg[cur], writes d_msg[cur^1].
-static int sweep_once(bp_context *c, const int *done) {
+static int sweep_once(bp_context *c, const NetCtl &done) {
     const int path = resolve_path(c);
     if (path < 0) return set_err(BP_ERR_UNSUPPORTED, "vertex path not available: %s",
                                  c->vplan.why.c_str());
@@ -562,6 +579,10 @@ int bp_set_topology(bp_context *c, int nv, int ne, const int *port_off, const in
                                     c->port_edge.data(), c->edge_dim.data(), c->phys_dim.data(),
                                     c->site_off.data(), c->msg_off.data(), c->port_rev.data(),
                                     c->vnet.data()) == BP_OK;
+    if (c->vplan_ok && vertex_set_lds_limits(&c->vplan) != BP_OK) {
+        c->vplan_ok = false;
+        c->vplan.why = "could not raise the LDS limit of the vertex kernel";
+    }
     c->have_topo = true;
     c->have_sites = false;
     c->have_msgs = false;
@@ -583,7 +604,11 @@ int bp_upload_sites(bp_context *c, const double *sites) {
                            c->stream, c->d_sites64, (float *)c->d_sites, c->nsite);
         HIPCHK(hipGetLastError());
     }
-    if (c->vplan_ok) RETCHK(vertex_upload_sites(&c->vplan, c->stream, c->d_sites64, &c->d_vsites));
+    if (c->vplan_ok) {
+        if (vertex_upload_sites(&c->vplan, c->stream, c->d_sites64, &c->d_vsites, c->d_site_off,
+                                c->d_port_off) != BP_OK)
+            return set_err(BP_ERR_HIP, "vertex-path site layout failed");
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_sites = true;
     return BP_OK;
@@ -678,7 +703,7 @@ int bp_sweep(bp_context *c) {
     RETCHK(ready(c));
     HIPCHK(hipSetDevice(c->device));
     c->fp64_active = false;
-    RETCHK(sweep_once(c, nullptr));
+    RETCHK(sweep_once(c, make_ctl(c, false)));
     HIPCHK(hipStreamSynchronize(c->stream));
     return BP_OK;
 }
@@ -694,9 +719,9 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
     c->fp64_active = false;
     RETCHK(prepare_sweep(c));
     const int cap = maxiter > 0 ? maxiter : 100000;  // tol-only runs are capped
-    const double *res = criterion == BP_CRIT_FIDELITY ? c->d_res_fid : c->d_res_abs;
     hipLaunchKernelGGL(k_reset_nets, dim3((c->nnet + 255) / 256), dim3(256), 0, c->stream,
                        c->nnet, c->d_done, c->d_iter, c->d_delta, c->d_at);
+    const NetCtl ctl = make_ctl(c, true);
     HIPCHK(hipGetLastError());
     const int chunk = 16;  // sweeps between host checks of the done flags (even)
     std::vector<int> done(c->nnet);
@@ -712,12 +737,8 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
                 HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                 int rc = BP_OK;
                 for (int i = 0; i < n && rc == BP_OK; ++i) {
-                    rc = sweep_once(c, c->d_done);
-                    if (rc == BP_OK) {
-                        hipLaunchKernelGGL(k_reduce_nets, dim3(c->nnet), dim3(BP_BLOCK), 0,
-                                           c->stream, c->d_pbeg, res, cap, tol, c->d_done,
-                                           c->d_iter, c->d_delta, c->d_at);
-                    }
+                    rc = sweep_once(c, ctl);
+                    if (rc == BP_OK) rc = launch_reduce(c, criterion, cap, tol);
                 }
                 hipError_t e = hipStreamEndCapture(c->stream, &graph);
                 if (rc != BP_OK) return rc;
@@ -730,11 +751,8 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
             if (n % 2) c->cur ^= 1;
         } else {
             for (int i = 0; i < n; ++i) {
-                RETCHK(sweep_once(c, c->d_done));
-                hipLaunchKernelGGL(k_reduce_nets, dim3(c->nnet), dim3(BP_BLOCK), 0, c->stream,
-                                   c->d_pbeg, res, cap, tol, c->d_done, c->d_iter, c->d_delta,
-                                   c->d_at);
-                HIPCHK(hipGetLastError());
+                RETCHK(sweep_once(c, ctl));
+                RETCHK(launch_reduce(c, criterion, cap, tol));
             }
         }
         launched += n;
@@ -772,7 +790,7 @@ int bp_time_sweeps(bp_context *c, int n, double *avg_ms) {
     int rc = BP_OK;
     for (int i = 0; i < n && rc == BP_OK; ++i) {
         (void)hipEventRecord(ev[2 * i], c->stream);
-        rc = sweep_once(c, nullptr);
+        rc = sweep_once(c, make_ctl(c, false));
         (void)hipEventRecord(ev[2 * i + 1], c->stream);
     }
     (void)hipStreamSynchronize(c->stream);
@@ -802,10 +820,10 @@ int bp_update_slot(bp_context *c, int slot) {
     void *m = c->d_msg[c->cur];
     if (c->dtype == BP_F64)
         RETCHK(launch_update_generic<double>(c, (const double *)c->d_sites, (const double *)m,
-                                             (double *)m, nullptr, slot, slot + 1));
+                                             (double *)m, make_ctl(c, false), slot, slot + 1));
     else
         RETCHK(launch_update_generic<float>(c, (const float *)c->d_sites, (const float *)m,
-                                            (float *)m, nullptr, slot, slot + 1));
+                                            (float *)m, make_ctl(c, false), slot, slot + 1));
     HIPCHK(hipStreamSynchronize(c->stream));
     return BP_OK;
 }
@@ -918,7 +936,7 @@ int bp_polish_fp64(bp_context *c, int nsweeps, double *out_delta) {
     }
     for (int i = 0; i < nsweeps; ++i) {
         RETCHK(launch_update_generic<double>(c, c->d_sites64, c->d_msg64[c->cur64],
-                                             c->d_msg64[c->cur64 ^ 1], nullptr, 0, c->nports));
+                                             c->d_msg64[c->cur64 ^ 1], make_ctl(c, false), 0, c->nports));
         c->cur64 ^= 1;
     }
     if (out_delta) {
@@ -943,10 +961,12 @@ int bp_grad_lognorm(bp_context *c, double *grad) {
     HIPCHK(hipSetDevice(c->device));
     bool is64 = false;
     const void *m = current_msgs(c, &is64);
-    return adjoint_grad_lognorm(&c->adj, c->stream, c->topo(), c->max_site, c->max_mats, c->maxD,
-                                is64 ? BP_F64 : c->dtype, is64 ? (const void *)c->d_sites64
-                                                            : (const void *)c->d_sites,
-                                m, c->nsite, grad);
+    const int rc = adjoint_grad_lognorm(&c->adj, c->stream, c->topo(), c->max_site, c->max_mats,
+                                        c->maxD, is64 ? BP_F64 : c->dtype,
+                                        is64 ? (const void *)c->d_sites64
+                                             : (const void *)c->d_sites,
+                                        m, c->nsite, grad);
+    return rc == BP_OK ? BP_OK : set_err(rc, "log-norm gradient failed (status %d)", rc);
 }
 
 int bp_tfim_energy(bp_context *c, double J, double h, double *energy, double *grad,
@@ -959,10 +979,12 @@ int bp_tfim_energy(bp_context *c, double J, double h, double *energy, double *gr
     HIPCHK(hipSetDevice(c->device));
     bool is64 = false;
     const void *m = current_msgs(c, &is64);
-    return adjoint_tfim(&c->adj, c->stream, c->topo(), c->ne, c->max_site, c->max_mats, c->maxD,
-                        is64 ? BP_F64 : c->dtype,
-                        is64 ? (const void *)c->d_sites64 : (const void *)c->d_sites, m, c->nsite,
-                        c->nmsg, J, h, energy, grad, adj_tol, adj_maxiter, out_iters, out_res);
+    const int rc = adjoint_tfim(&c->adj, c->stream, c->topo(), c->ne, c->max_site, c->max_mats,
+                                c->maxD, is64 ? BP_F64 : c->dtype,
+                                is64 ? (const void *)c->d_sites64 : (const void *)c->d_sites, m,
+                                c->nsite, c->nmsg, J, h, energy, grad, adj_tol, adj_maxiter,
+                                out_iters, out_res);
+    return rc == BP_OK ? BP_OK : set_err(rc, "TFIM energy/gradient failed (status %d)", rc);
 }
 
 }  // extern "C"

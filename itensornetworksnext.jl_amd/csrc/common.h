@@ -24,6 +24,17 @@ struct DevTopo {
     int plain;
 };
 
+// Per-network convergence state read by the sweep kernels (a network whose `done` flag
+// is set is skipped: its messages are carried over unchanged) and updated once per sweep
+// by k_reduce_nets (StopWhenConverged / StopAfterIteration,
+// AlgorithmsInterfaceExtensions.jl:84-109, beliefpropagation.jl:27-55).  Folding the
+// reduction into the sweep kernels with device-scope atomics and fences was measured
+// slower on the 8-XCD part than this separate launch.
+struct NetCtl {
+    int *done;     // per network: 1 once stopped
+    int enabled;   // 0: plain sweep, every network updated
+};
+
 // wave64 reductions (DPP/permute through the compiler's __shfl_xor lowering)
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {

@@ -14,9 +14,10 @@
 #include "common.h"
 
 // out[pre, b, post] = sum_a in[pre, a, post] * m[a, b]; m is na x nb (row-major).
+// trans != 0 uses m[b, a] instead (m square).
 template <typename T>
 __device__ void block_mode_product(const T *in, T *out, const int *dims, int r, int ax,
-                                   const T *m, int nb) {
+                                   const T *m, int nb, int trans = 0) {
     long long pre = 1, post = 1;
     for (int i = 0; i < ax; ++i) pre *= dims[i];
     for (int i = ax + 1; i < r; ++i) post *= dims[i];
@@ -30,12 +31,21 @@ __device__ void block_mode_product(const T *in, T *out, const int *dims, int r, 
             T acc[16];
 #pragma unroll
             for (int b = 0; b < 16; ++b) acc[b] = T(0);
-            for (int a = 0; a < na; ++a) {
-                const T x = src[(long long)a * post];
-                const T *mr = m + (long long)a * nb + b0;
+            if (trans) {
+                for (int a = 0; a < na; ++a) {
+                    const T x = src[(long long)a * post];
 #pragma unroll
-                for (int b = 0; b < 16; ++b)
-                    if (b0 + b < nb) acc[b] += x * mr[b];
+                    for (int b = 0; b < 16; ++b)
+                        if (b0 + b < nb) acc[b] += x * m[(long long)(b0 + b) * na + a];
+                }
+            } else {
+                for (int a = 0; a < na; ++a) {
+                    const T x = src[(long long)a * post];
+                    const T *mr = m + (long long)a * nb + b0;
+#pragma unroll
+                    for (int b = 0; b < 16; ++b)
+                        if (b0 + b < nb) acc[b] += x * mr[b];
+                }
             }
 #pragma unroll
             for (int b = 0; b < 16; ++b)
@@ -149,7 +159,7 @@ __device__ void load_incoming(const DevTopo &tp, int u, const VertexShape &vs, c
 template <typename T>
 __global__ __launch_bounds__(BP_BLOCK) void k_update_generic(
     DevTopo tp, const T *__restrict__ sites, const T *mold, T *mnew,
-    const int *__restrict__ net_done, double *__restrict__ res_fid, double *__restrict__ res_abs,
+    NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs,
     int slot_begin, int slot_end, int mats_elems, int out_elems, int red_elems,
     long long buf_elems, int use_lds, T *ws, long long ws_stride) {
     extern __shared__ __align__(16) char smem[];
@@ -161,7 +171,7 @@ __global__ __launch_bounds__(BP_BLOCK) void k_update_generic(
         const int p = q - base;
         const long long mo = tp.msg_off[q];
         const int len = (int)(tp.msg_off[q + 1] - mo);
-        if (net_done != nullptr && net_done[tp.vnet[u]]) {
+        if (ctl.enabled && ctl.done[tp.vnet[u]]) {
             for (int i = threadIdx.x; i < len; i += blockDim.x) mnew[mo + i] = mold[mo + i];
             if (threadIdx.x == 0) {
                 res_fid[q] = 0.0;
@@ -215,7 +225,8 @@ __global__ __launch_bounds__(BP_BLOCK) void k_update_generic(
         dmx = block_max(dmx, sm.red);
         if (threadIdx.x == 0) {
             const double c = dno / sqrt(dnn * doo);
-            res_fid[q] = 1.0 - c * c;
+            const double fid = 1.0 - c * c;
+            res_fid[q] = fid;
             res_abs[q] = dmx;
         }
         __syncthreads();

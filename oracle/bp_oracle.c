@@ -850,7 +850,12 @@ int orc_fixed_point_gradient(const orc_graph *g, const double *sites, const doub
     while (it < maxiter) {
         if (dL_dm) memcpy(nxt, dL_dm, sizeof(double) * nm); else memset(nxt, 0, sizeof(double) * nm);
         orc_norm_vjp(g, sites, msgs, lam, nxt, NULL);
-        res = maxabs_diff(nxt, lam, nm);
+        {
+            double sc = 0.0;
+            for (long long i = 0; i < nm; ++i) sc = fmax(sc, fabs(nxt[i]));
+            res = maxabs_diff(nxt, lam, nm);
+            if (sc > 0.0) res /= sc; /* relative change of the adjoint iterate */
+        }
         double *t = lam; lam = nxt; nxt = t;
         ++it;
         if (res < tol) break;

@@ -140,7 +140,8 @@ int orc_tfim_energy(const orc_graph *g, const double *sites, const double *msgs,
 
 /* ---- full implicit-differentiation gradient of an objective L(m*, A) at the fixed
  * point m*: solves lambda = J_m^T lambda + dL/dm (projected onto the sum-normalised
- * tangent space) by adjoint sweeps to tolerance tol (max-abs), then
+ * tangent space) by adjoint sweeps to tolerance tol on the relative change
+ * max|lambda_new - lambda| / max|lambda_new|, then
  * grad = dL/dA + J_A^T lambda.  Writes the gradient (site layout), the sweep count and
  * the final adjoint residual. */
 int orc_fixed_point_gradient(const orc_graph *g, const double *sites, const double *msgs,

@@ -179,3 +179,84 @@ def test_errors_are_reported(pkg):
     with pytest.raises(pkg._lib.BPError):
         e.update_slot(10 ** 6)
     e.close()
+
+
+# ---------------------------------------------------------------- vertex (MFMA) path
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+from helpers import fp32_sweep_check as _fp32_sweep_check  # noqa: E402
+
+
+@pytest.mark.parametrize("shape,d", [((6, 5), 2), ((4, 4), 1), ((3, 5), 3), ((5, 4), 4)])
+def test_vertex_single_sweep_parity(pkg, orc, shape, d):
+    """One Jacobi sweep on the D=8 vertex path (FP32 MFMA) from random non-symmetric
+    messages against the FP64 oracle; corners, edges and interior vertices all present."""
+    _say(f"case vertex single sweep {shape} d={d}")
+    t = pkg.build_topology(pkg.named_grid(shape), 8, d)
+    s = pkg.peps.random_sites(t, 17, scale=pkg.peps.peps_scale(d, 8))
+    m = pkg.peps.random_messages(t, 17)
+    want, fid, mab = orc.sweep_jacobi(t, s, m)
+    with pkg.Engine(t, s, dtype="f32", path="vertex") as e:
+        assert e.path() == "vertex"
+        e.set_messages(m)
+        e.sweep()
+        got = e.messages()
+    # every entry within the FP32 error bound (a wrong index gives O(1) errors)
+    assert _fp32_sweep_check(pkg, orc, t, s, m, got) < 1.0
+    # and on well-conditioned (identity) incoming messages a plain relative check
+    mi = pkg.peps.identity_messages(t)
+    want_i, _, _ = orc.sweep_jacobi(t, s, mi)
+    with pkg.Engine(t, s, dtype="f32", path="vertex") as e:
+        e.set_messages(mi)
+        e.sweep()
+        assert _rel(e.messages(), want_i) < 1e-5
+
+
+def test_vertex_path_graph_degree2(pkg, orc):
+    _say("case vertex path graph")
+    t = pkg.build_topology(pkg.named_path_graph(5), 8, 2)
+    s = pkg.peps.random_sites(t, 3)
+    m = pkg.peps.random_messages(t, 3)
+    with pkg.Engine(t, s, dtype="f32", path="vertex") as e:
+        e.set_messages(m)
+        e.sweep()
+        assert _fp32_sweep_check(pkg, orc, t, s, m, e.messages()) < 1.0
+
+
+def test_vertex_matches_generic_fp32(pkg):
+    _say("case vertex vs generic fp32")
+    t = _grid(pkg, 8, 8)
+    s = pkg.peps.random_sites(t, 5, scale=pkg.peps.peps_scale(2, 8))
+    res = {}
+    for path in ("generic", "vertex"):
+        with pkg.Engine(t, s, dtype="f32", path=path) as e:
+            e.init_messages("identity")
+            r = e.run(maxiter=30)
+            res[path] = (e.messages(), e.lognorm())
+    assert _rel(res["vertex"][0], res["generic"][0]) < 1e-5
+    assert res["vertex"][1] == pytest.approx(res["generic"][1], rel=1e-6)
+
+
+def test_vertex_batch_with_masks(pkg, orc):
+    _say("case vertex batch")
+    tops = [_grid(pkg, L, 8) for L in (3, 6, 4)]
+    sites = [pkg.peps.random_sites(t, 60 + i, scale=pkg.peps.peps_scale(2, 8))
+             for i, t in enumerate(tops)]
+    b = pkg.concat_topologies(tops)
+    with pkg.Engine(b, np.concatenate(sites), dtype="f32") as e:
+        assert e.path() == "vertex"
+        e.init_messages("identity")
+        r = e.run(maxiter=300, tol=1e-6, criterion="maxabs")
+        mb = e.messages()
+    off = 0
+    for i, (t, s) in enumerate(zip(tops, sites)):
+        want, sw, dl, at = orc.bp(t, s, pkg.peps.identity_messages(t), maxiter=300, tol=1e-6,
+                                  criterion="maxabs")
+        n = int(t.msg_off[-1])
+        assert r.converged[i]
+        assert abs(int(r.sweeps[i]) - sw) <= 2
+        assert _rel(mb[off:off + n], want) < 1e-5
+        off += n
