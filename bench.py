@@ -53,6 +53,37 @@ def algorithmic_bytes_per_sweep(topo, elt: int) -> float:
     return (float(topo.site_off[-1]) + 2.0 * float(topo.msg_off[-1])) * elt
 
 
+def adjoint_timing(pkg, torch, L, D, dtype, seed):
+    """BP to convergence, then the BP-estimated TFIM energy (J = 1, h = 3) and its gradient
+    with respect to every site tensor by fixed-point adjoint sweeps; wall times of both."""
+    import time
+
+    g = pkg.named_grid((L, L))
+    topo = pkg.build_topology(g, D, 2)
+    sites = pkg.peps.random_sites(topo, seed, scale=pkg.peps.peps_scale(2, D))
+    e = pkg.Engine(topo, sites, dtype=dtype)
+    e.init_messages("identity")
+    e.run(maxiter=5)
+    e.tfim_energy(1.0, 3.0, grad=True, adj_tol=1e-6, adj_maxiter=20)  # warm the adjoint kernels
+    e.init_messages("identity")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = e.run(maxiter=500, tol=1e-8)
+    torch.cuda.synchronize()
+    t_bp = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    E, grad, it, res = e.tfim_energy(1.0, 3.0, grad=True, adj_tol=1e-6, adj_maxiter=2000)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t1
+    out = {"workload": f"{L}x{L} D={D} {dtype}: BP to fidelity 1e-8, then TFIM energy + "
+                       f"gradient (adjoint to relative 1e-6)",
+           "bp_sweeps": int(r.sweeps[0]), "bp_s": t_bp, "adjoint_sweeps": it,
+           "adjoint_residual": res, "adjoint_s": t_adj, "energy": E,
+           "bp_plus_adjoint_over_bp": (t_bp + t_adj) / t_bp, "path": e.path()}
+    e.close()
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +171,9 @@ def main(argv=None):
         extras["speedup_vs_fp64_plain_gpu"] = value / extras["fp64_plain_gpu_msg_per_s"]
         e64.close()
     eng.close()
+    if not args.no_extras:
+        extras["adjoint"] = [adjoint_timing(pkg, torch, L, D, args.dtype, args.seed),
+                             adjoint_timing(pkg, torch, 16, 6, args.dtype, args.seed)]
 
     cpu = None
     if not args.no_cpu:

@@ -115,6 +115,14 @@ int bp_run(bp_context *ctx, int maxiter, double tol, int criterion, int use_grap
 /* Sequential (Gauss-Seidel) update of one message slot in place, normalised. */
 int bp_update_slot(bp_context *ctx, int slot);
 
+/* Sequential solve with the reference's own schedule semantics: each sweep updates the
+ * slots of `order` one after another in place (beliefpropagation.jl:200-210, the
+ * default order being forest_cover_edge_sequence, beliefpropagation.jl:14), then the
+ * sweep's delta against the previous sweep decides StopWhenConverged(tol); stopping and
+ * outputs as bp_run (single network). */
+int bp_run_sequential(bp_context *ctx, const int *order, int norder, int maxiter, double tol,
+                      int criterion, int *out_sweeps, double *out_delta, int *out_at_iteration);
+
 /* Per-slot residuals of the last sweep (FP64, length nports), either may be NULL. */
 int bp_residuals(bp_context *ctx, double *fid, double *maxabs);
 

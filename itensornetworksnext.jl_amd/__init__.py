@@ -1,18 +1,42 @@
 """MI355X-native belief-propagation engine for ITensorNetworksNext-style networks.
 
 The package mirrors the reference's BP API (src/beliefpropagation/*.jl,
-src/select_algorithm.jl, src/AlgorithmsInterfaceExtensions) in Python and runs every
-message update, normalisation, residual and vertex contraction in hand-written HIP
-kernels for gfx950 behind the C-ABI library `csrc/libbpmi355x.so` (declared in
-include/bp_mi355x.h).  There is no CPU fallback: the compute entry points raise when the
-library or a GPU is missing.
+src/select_algorithm.jl, src/AlgorithmsInterfaceExtensions, src/normnetwork.jl,
+src/tensornetwork.jl) in Python and runs every message update, normalisation, residual,
+vertex contraction and adjoint in hand-written HIP kernels for gfx950 behind the C-ABI
+library `csrc/libbpmi355x.so` (declared in include/bp_mi355x.h).  There is no CPU
+fallback: the compute entry points raise when the library or a GPU is missing.
 """
-from . import _lib, build, engine, graphs, peps  # noqa: F401
+from . import _lib, algorithms, build, engine, graphs, itensors, peps  # noqa: F401
+from .algorithms import (AbstractAlgorithm, ArgumentError, MethodError,  # noqa: F401
+                         StopAfterIteration, StopWhenAny, StopWhenConverged, default_algorithm,
+                         select_algorithm, select_beliefpropagation_stopping_criterion)
+from .beliefpropagation import (JacobiGPUUpdate, MessageUpdateAlgorithm,  # noqa: F401
+                                SimpleMessageUpdate, beliefpropagation,
+                                default_beliefpropagation_edges, message_update_)
 from .engine import Engine, RunResult  # noqa: F401
 from .graphs import (NamedGraph, Topology, build_topology, concat_topologies,  # noqa: F401
                      forest_cover_edge_sequence, heavy_hex_lattice, named_comb_tree,
                      named_grid, named_path_graph)
+from .itensors import (Index, ITensor, ITensorNetwork, NormNetwork, linkinds,  # noqa: F401
+                       normnetwork, random_itensor, siteinds, tensornetwork, to_topology,
+                       uniquename)
+from .messagecache import (MessageCache, bethe_free_energy, edge_scalar,  # noqa: F401
+                           edge_scalars, incoming_edge_data, incoming_messages, iterate_diff,
+                           message_environment, messagecache, one, region_scalar,
+                           similar_message_environment, vertex_scalar, vertex_scalars)
 
-__all__ = ["Engine", "RunResult", "engine", "graphs", "peps", "NamedGraph", "Topology", "build_topology", "concat_topologies",
-           "forest_cover_edge_sequence", "heavy_hex_lattice", "named_comb_tree", "named_grid",
-           "named_path_graph"]
+__all__ = [
+    "AbstractAlgorithm", "ArgumentError", "Engine", "ITensor", "ITensorNetwork", "Index",
+    "JacobiGPUUpdate", "MessageCache", "MessageUpdateAlgorithm", "MethodError", "NamedGraph",
+    "NormNetwork", "RunResult", "SimpleMessageUpdate", "StopAfterIteration", "StopWhenAny",
+    "StopWhenConverged", "Topology", "beliefpropagation", "bethe_free_energy",
+    "build_topology", "concat_topologies", "default_algorithm",
+    "default_beliefpropagation_edges", "edge_scalar", "edge_scalars",
+    "forest_cover_edge_sequence", "heavy_hex_lattice", "incoming_edge_data",
+    "incoming_messages", "iterate_diff", "linkinds", "message_environment", "message_update_",
+    "messagecache", "named_comb_tree", "named_grid", "named_path_graph", "normnetwork", "one",
+    "random_itensor", "region_scalar", "select_algorithm",
+    "select_beliefpropagation_stopping_criterion", "similar_message_environment", "siteinds",
+    "tensornetwork", "to_topology", "uniquename", "vertex_scalar", "vertex_scalars",
+]

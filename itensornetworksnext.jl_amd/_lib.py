@@ -40,6 +40,8 @@ SIGNATURES = {
     "bp_sweep": (C.c_int, [_vp]),
     "bp_run": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int, _ip, _dp, _ip]),
     "bp_update_slot": (C.c_int, [_vp, C.c_int]),
+    "bp_run_sequential": (C.c_int, [_vp, _ip, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _dp,
+                                    _ip]),
     "bp_residuals": (C.c_int, [_vp, _dp, _dp]),
     "bp_iterate_diff": (C.c_int, [_vp, _dp, _dp]),
     "bp_vertex_scalars": (C.c_int, [_vp, _dp]),

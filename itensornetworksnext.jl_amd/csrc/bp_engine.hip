@@ -110,6 +110,35 @@ __global__ __launch_bounds__(1024) void k_reduce_nets(const int *pbeg, const dou
     }
 }
 
+// Per-slot distance between two message buffers (one wave per slot): the reference's
+// iterate_diff term 1 - |<a^, b^>|^2 (beliefpropagation.jl:261-267) and max |a - b|.
+template <typename Ta, typename Tb>
+__global__ void k_diff_slots(DevTopo tp, const Ta *a, const Tb *b, double *res_fid,
+                             double *res_abs) {
+    const int q = blockIdx.x;
+    if (q >= tp.nports) return;
+    const long long mo = tp.msg_off[q];
+    const int len = (int)(tp.msg_off[q + 1] - mo);
+    double ab = 0.0, aa = 0.0, bb = 0.0, mx = 0.0;
+    for (int i = threadIdx.x; i < len; i += blockDim.x) {
+        const double x = (double)a[mo + i], y = (double)b[mo + i];
+        ab += x * y;
+        aa += x * x;
+        bb += y * y;
+        const double d = fabs(x - y);
+        mx = (d > mx || d != d) ? d : mx;
+    }
+    ab = wave_sum(ab);
+    aa = wave_sum(aa);
+    bb = wave_sum(bb);
+    mx = wave_max(mx);
+    if (threadIdx.x == 0) {
+        const double c = ab / sqrt(aa * bb);
+        res_fid[q] = 1.0 - c * c;
+        res_abs[q] = mx;
+    }
+}
+
 // BP log-norm per network (one workgroup per network): Neumaier-compensated
 // sum log|Z_v| - sum log|Z_e| with the reference's complex promotion and -Inf rule
 // (messagecache.jl:185-201).
@@ -845,24 +874,121 @@ static const void *current_msgs(bp_context *c, bool *is64) {
     return c->d_msg[c->cur];
 }
 
+// per-slot distances of the current messages against `other` (device buffer)
+static int diff_launch(bp_context *c, const void *other, bool other64) {
+    bool is64 = false;
+    const void *m = current_msgs(c, &is64);
+    if (c->nports == 0) return BP_OK;
+    const dim3 g(c->nports), b(64);
+    if (is64 && other64)
+        hipLaunchKernelGGL((k_diff_slots<double, double>), g, b, 0, c->stream, c->topo(),
+                           (const double *)m, (const double *)other, c->d_res_fid, c->d_res_abs);
+    else if (is64)
+        hipLaunchKernelGGL((k_diff_slots<double, float>), g, b, 0, c->stream, c->topo(),
+                           (const double *)m, (const float *)other, c->d_res_fid, c->d_res_abs);
+    else if (other64)
+        hipLaunchKernelGGL((k_diff_slots<float, double>), g, b, 0, c->stream, c->topo(),
+                           (const float *)m, (const double *)other, c->d_res_fid, c->d_res_abs);
+    else
+        hipLaunchKernelGGL((k_diff_slots<float, float>), g, b, 0, c->stream, c->topo(),
+                           (const float *)m, (const float *)other, c->d_res_fid, c->d_res_abs);
+    HIPCHK(hipGetLastError());
+    return BP_OK;
+}
+
 int bp_iterate_diff(bp_context *c, const double *other, double *out) {
     RETCHK(ready(c));
     if (!other || !out) return set_err(BP_ERR_ARG, "NULL argument");
-    std::vector<double> mine(c->nmsg);
-    RETCHK(bp_download_messages(c, mine.data()));
-    double mx = -INFINITY;
-    for (int q = 0; q < c->nports; ++q) {
-        double ab = 0, aa = 0, bb = 0;
-        for (long long i = c->msg_off[q]; i < c->msg_off[q + 1]; ++i) {
-            ab += mine[i] * other[i];
-            aa += mine[i] * mine[i];
-            bb += other[i] * other[i];
-        }
-        const double d = ab / std::sqrt(aa * bb);
-        const double v = 1.0 - d * d;
-        if (v > mx || v != v) mx = v;
-    }
+    HIPCHK(hipSetDevice(c->device));
+    double *d_other = nullptr;
+    HIPCHK(hipMalloc((void **)&d_other, sizeof(double) * std::max<long long>(c->nmsg, 1)));
+    int rc = BP_OK;
+    if (hipMemcpyAsync(d_other, other, sizeof(double) * c->nmsg, hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess)
+        rc = set_err(BP_ERR_HIP, "copy failed");
+    if (rc == BP_OK) rc = diff_launch(c, d_other, true);
+    std::vector<double> fid(c->nports);
+    if (rc == BP_OK && c->nports > 0 &&
+        hipMemcpyAsync(fid.data(), c->d_res_fid, sizeof(double) * c->nports,
+                       hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = set_err(BP_ERR_HIP, "copy failed");
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(d_other);
+    if (rc != BP_OK) return rc;
+    double mx = -INFINITY;  // maximum over edges (an empty cache gives -Inf, as maximum() would fail)
+    for (double v : fid) mx = (v > mx || v != v) ? v : mx;
     *out = mx;
+    return BP_OK;
+}
+
+int bp_run_sequential(bp_context *c, const int *order, int norder, int maxiter, double tol,
+                      int criterion, int *out_sweeps, double *out_delta, int *out_at) {
+    RETCHK(ready(c));
+    if (!order || norder <= 0) return set_err(BP_ERR_ARG, "empty schedule");
+    if (maxiter <= 0 && tol < 0.0)
+        return set_err(BP_ERR_ARG, "at least one of maxiter or tol must be given");
+    if (c->nnet != 1) return set_err(BP_ERR_UNSUPPORTED, "sequential runs take one network");
+    for (int i = 0; i < norder; ++i)
+        if (order[i] < 0 || order[i] >= c->nports)
+            return set_err(BP_ERR_ARG, "schedule entry %d out of range", i);
+    HIPCHK(hipSetDevice(c->device));
+    c->fp64_active = false;
+    const int cap = maxiter > 0 ? maxiter : 100000;
+    const size_t es = c->elt();
+    void *prev = nullptr;
+    HIPCHK(hipMalloc(&prev, es * std::max<long long>(c->nmsg, 1)));
+    std::vector<double> res(c->nports);
+    const NetCtl off = make_ctl(c, false);
+    int it = 0, at = -1, rc = BP_OK;
+    double delta = INFINITY;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    while (rc == BP_OK && it < cap) {
+        if (hipMemcpyAsync(prev, c->d_msg[c->cur], es * c->nmsg, hipMemcpyDeviceToDevice,
+                           c->stream) != hipSuccess) {
+            rc = set_err(BP_ERR_HIP, "snapshot copy failed");
+            break;
+        }
+        void *m = c->d_msg[c->cur];
+        for (int i = 0; i < norder && rc == BP_OK; ++i) {
+            if (c->dtype == BP_F64)
+                rc = launch_update_generic<double>(c, (const double *)c->d_sites,
+                                                   (const double *)m, (double *)m, off, order[i],
+                                                   order[i] + 1);
+            else
+                rc = launch_update_generic<float>(c, (const float *)c->d_sites, (const float *)m,
+                                                  (float *)m, off, order[i], order[i] + 1);
+        }
+        if (rc != BP_OK) break;
+        ++it;
+        if (tol >= 0.0) {
+            rc = diff_launch(c, prev, c->dtype == BP_F64);
+            if (rc != BP_OK) break;
+            const double *src = criterion == BP_CRIT_FIDELITY ? c->d_res_fid : c->d_res_abs;
+            if (hipMemcpyAsync(res.data(), src, sizeof(double) * c->nports, hipMemcpyDeviceToHost,
+                               c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess) {
+                rc = set_err(BP_ERR_HIP, "residual copy failed");
+                break;
+            }
+            double mx = 0.0;
+            for (double v : res) mx = (v > mx || v != v) ? v : mx;
+            delta = mx;
+            if (delta < tol) {
+                at = it;
+                break;
+            }
+        }
+    }
+    (void)hipEventRecord(c->ev1, c->stream);
+    (void)hipEventSynchronize(c->ev1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->last_ms = ms;
+    (void)hipFree(prev);
+    if (rc != BP_OK) return rc;
+    if (out_sweeps) *out_sweeps = it;
+    if (out_delta) *out_delta = delta;
+    if (out_at) *out_at = at;
     return BP_OK;
 }
 

@@ -139,6 +139,22 @@ class Engine:
         L.check(self.lib.bp_last_run_ms(self.h, C.byref(ms)), "bp_last_run_ms")
         return RunResult(sw, dl, at, ms.value)
 
+    def run_sequential(self, order, maxiter: Optional[int] = None, tol: Optional[float] = None,
+                       criterion: str = "fidelity") -> RunResult:
+        """Gauss-Seidel sweeps over `order` (message slots), reference semantics."""
+        o = _i32(order)
+        sw = np.zeros(1, dtype=np.int32)
+        dl = np.zeros(1)
+        at = np.zeros(1, dtype=np.int32)
+        crit = L.BP_CRIT_FIDELITY if criterion == "fidelity" else L.BP_CRIT_MAXABS
+        L.check(self.lib.bp_run_sequential(self.h, _ip(o), len(o),
+                                           int(maxiter) if maxiter is not None else 0,
+                                           float(tol) if tol is not None else -1.0, crit,
+                                           _ip(sw), _dp(dl), _ip(at)), "bp_run_sequential")
+        ms = C.c_double(0)
+        L.check(self.lib.bp_last_run_ms(self.h, C.byref(ms)), "bp_last_run_ms")
+        return RunResult(sw, dl, at, ms.value)
+
     def time_sweeps(self, n: int) -> float:
         """Average device milliseconds of one sweep's kernels over n back-to-back sweeps."""
         ms = C.c_double(0)

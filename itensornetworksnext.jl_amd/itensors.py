@@ -1,0 +1,247 @@
+"""Named tensors and networks, the containers the reference's BP path consumes.
+
+Mirrors src/tensornetwork.jl (ITensorNetwork: vertex -> tensor, edges inferred from
+index names shared by exactly two tensors, tensornetwork.jl:138-173), src/normnetwork.jl
+(NormNetwork: lazy <psi|psi> whose bra layer is the conjugated ket with link names
+renamed, normnetwork.jl:13-93) and the link/site queries of src/abstracttensornetwork.jl
+(linkinds, siteinds: abstracttensornetwork.jl:52-95).  Flattening to the engine's port
+topology (`to_topology`) permutes each tensor to [site indices][port 0]...[port k-1].
+"""
+from __future__ import annotations
+
+import itertools
+from typing import Dict, Hashable, Iterable, List, Sequence, Tuple
+
+import numpy as np
+
+from .graphs import NamedGraph, Topology, build_topology
+
+_uid = itertools.count(1)
+
+
+class Index:
+    """A named dimension (ITensorBase `Index`): equal only to itself (or a copy sharing its
+    name)."""
+
+    __slots__ = ("dim", "name", "tags")
+
+    def __init__(self, dim: int, tags: str = "", name=None):
+        self.dim = int(dim)
+        self.name = name if name is not None else next(_uid)
+        self.tags = tags
+
+    def __eq__(self, other):
+        return isinstance(other, Index) and other.name == self.name
+
+    def __hash__(self):
+        return hash(("Index", self.name))
+
+    def __repr__(self):
+        return f"Index(dim={self.dim}, name={self.name}{', ' + self.tags if self.tags else ''})"
+
+
+def uniquename(ind: Index) -> Index:
+    """A fresh index of the same dimension (ITensorBase `uniquename`)."""
+    return Index(ind.dim, ind.tags)
+
+
+class ITensor:
+    """Dense array with one Index per axis."""
+
+    def __init__(self, data, inds: Sequence[Index]):
+        data = np.asarray(data)
+        inds = tuple(inds)
+        if data.ndim != len(inds) or any(d != i.dim for d, i in zip(data.shape, inds)):
+            raise ValueError("tensor shape does not match its indices")
+        self.data = data
+        self.inds = inds
+
+    def dimnames(self):
+        return list(self.inds)
+
+    def permute(self, inds: Sequence[Index]) -> np.ndarray:
+        perm = [self.inds.index(i) for i in inds]
+        return np.transpose(self.data, perm)
+
+    def __repr__(self):
+        return f"ITensor({self.data.shape}, {self.inds})"
+
+
+def random_itensor(inds: Sequence[Index], rng=None, dtype=np.float64) -> ITensor:
+    rng = rng if rng is not None else np.random.default_rng()
+    return ITensor(rng.standard_normal(tuple(i.dim for i in inds)).astype(dtype), inds)
+
+
+class ITensorNetwork:
+    """vertex -> ITensor with edges inferred from shared index names."""
+
+    def __init__(self, tensors: Dict[Hashable, ITensor] | None = None):
+        self.tensors: Dict[Hashable, ITensor] = {}
+        self.dimname_vertices: Dict[Index, set] = {}
+        self.graph = NamedGraph()
+        for v, t in (tensors or {}).items():
+            self[v] = t
+
+    # -- upsert with edge inference (tensornetwork.jl:138-173)
+    def __setitem__(self, v, t: ITensor):
+        if not isinstance(t, ITensor):
+            raise TypeError("ITensorNetwork holds ITensors")
+        old = set(self.tensors[v].inds) if v in self.tensors else set()
+        new = set(t.inds)
+        self.graph.add_vertex(v)
+        for ind in old - new:
+            vs = self.dimname_vertices.get(ind, set())
+            vs.discard(v)
+            if not vs:
+                self.dimname_vertices.pop(ind, None)
+            self._rebuild_edges()
+        for ind in new - old:
+            vs = self.dimname_vertices.setdefault(ind, set())
+            if len(vs) > 1:
+                raise ValueError(f"index {ind} can appear in at most one existing tensor, "
+                                 f"got {len(vs)}.")
+            vs.add(v)
+            if len(vs) == 2:
+                a, b = tuple(vs)
+                if not self.graph.has_edge(a, b):
+                    self.graph.add_edge(a, b)
+        self.tensors[v] = t
+
+    def _rebuild_edges(self):
+        g = NamedGraph(vertices=list(self.tensors.keys()) or self.graph.vertices())
+        for ind, vs in self.dimname_vertices.items():
+            if len(vs) == 2:
+                a, b = tuple(vs)
+                if not g.has_edge(a, b):
+                    g.add_edge(a, b)
+        self.graph = g
+
+    def __getitem__(self, v) -> ITensor:
+        return self.tensors[v]
+
+    def __contains__(self, v):
+        return v in self.tensors
+
+    def vertices(self) -> List[Hashable]:
+        return self.graph.vertices()
+
+    def edges(self):
+        return self.graph.edges()
+
+    def linkinds(self, edge) -> List[Index]:
+        u, v = edge
+        return [i for i in self.tensors[u].inds if i in set(self.tensors[v].inds)]
+
+    def siteinds(self, v) -> List[Index]:
+        return [i for i in self.tensors[v].inds if len(self.dimname_vertices.get(i, ())) == 1]
+
+    def copy(self) -> "ITensorNetwork":
+        return ITensorNetwork({v: ITensor(t.data.copy(), t.inds) for v, t in self.tensors.items()})
+
+
+def tensornetwork(f, vertices: Iterable[Hashable]) -> ITensorNetwork:
+    """tensornetwork(f, vs) (tensornetwork.jl:115): vertex v holds f(v)."""
+    return ITensorNetwork({v: f(v) for v in vertices})
+
+
+def linkinds(tn, edge):
+    return (tn.ket if isinstance(tn, NormNetwork) else tn).linkinds(edge)
+
+
+def siteinds(tn, v):
+    return (tn.ket if isinstance(tn, NormNetwork) else tn).siteinds(v)
+
+
+class NormNetwork:
+    """Lazy <tn|tn>: the ket network plus a ket -> bra link-name map; site names map to
+    themselves (normnetwork.jl:16-27, 66-73).  Only the ket tensors are stored."""
+
+    def __init__(self, tn: ITensorNetwork, braname: Dict[Index, Index] | None = None):
+        self.ket = tn
+        if braname is None:
+            braname = {i: uniquename(i) for i, vs in tn.dimname_vertices.items() if len(vs) == 2}
+        self.braname = {i: braname[i] for i, vs in tn.dimname_vertices.items() if len(vs) == 2}
+
+    def vertices(self):
+        return self.ket.vertices()
+
+    def edges(self):
+        return self.ket.edges()
+
+    @property
+    def graph(self):
+        return self.ket.graph
+
+    def bra_index(self, ind: Index) -> Index:
+        if ind not in self.ket.dimname_vertices:
+            raise KeyError(f"index name {ind} not found underlying tensor network.")
+        return self.braname.get(ind, ind)
+
+    def kettensor(self, v) -> ITensor:
+        return self.ket[v]
+
+    def conj_bratensor(self, v) -> ITensor:
+        t = self.ket[v]
+        return ITensor(t.data, [self.bra_index(i) for i in t.inds])
+
+    def bratensor(self, v) -> ITensor:
+        t = self.conj_bratensor(v)
+        return ITensor(np.conj(t.data), t.inds)
+
+
+def normnetwork(tn: ITensorNetwork, braname=None) -> NormNetwork:
+    return NormNetwork(tn, braname)
+
+
+# ------------------------------------------------------------------ flattening
+
+class Flattened:
+    """A network flattened to the engine's topology, with the maps back to names."""
+
+    def __init__(self, topo: Topology, sites: np.ndarray, port_inds: Dict[Tuple, List[Index]],
+                 plain: bool):
+        self.topo = topo
+        self.sites = sites
+        self.port_inds = port_inds  # (u, v) -> shared link indices, in the order they are fused
+        self.plain = plain
+
+    def slot(self, edge) -> int:
+        return self.topo.slot(edge[0], edge[1])
+
+
+def to_topology(factors) -> Flattened:
+    """Flatten an ITensorNetwork (plain BP: vector messages) or a NormNetwork (double layer:
+    matrix messages) to the engine's port topology.  Several indices shared by the same
+    pair of tensors are fused into one link; several site indices into one site index."""
+    plain = not isinstance(factors, NormNetwork)
+    tn = factors if plain else factors.ket
+    g = tn.graph
+    vs = g.vertices()
+    port_inds = {}
+    dims = {}
+    for (u, v) in g.edges():
+        li = tn.linkinds((u, v))
+        port_inds[(u, v)] = li
+        port_inds[(v, u)] = li
+        dims[(u, v)] = int(np.prod([i.dim for i in li]))
+    site_dims = {}
+    for v in vs:
+        si = tn.siteinds(v)
+        site_dims[v] = int(np.prod([i.dim for i in si])) if si else 1
+        if plain and si:
+            raise ValueError("plain BP needs every index shared by two tensors (vertex "
+                             f"{v} has unshared indices {si})")
+    topo = build_topology(g, dims, site_dims, plain=plain)
+    chunks = []
+    for ui, u in enumerate(topo.vertices):
+        t = tn[u]
+        order = list(tn.siteinds(u))
+        for q in range(topo.port_off[ui], topo.port_off[ui + 1]):
+            w = topo.vertices[topo.port_nbr[q]]
+            order += port_inds[(u, w)]
+        arr = t.permute(order)
+        chunks.append(np.ascontiguousarray(arr, dtype=np.float64).reshape(-1))
+    sites = np.concatenate(chunks) if chunks else np.zeros(0)
+    if np.iscomplexobj(sites):
+        raise TypeError("the engine computes with real tensors")
+    return Flattened(topo, sites, port_inds, plain)

@@ -1,0 +1,264 @@
+"""MessageCache and the BP scalars, evaluated on the GPU.
+
+Mirrors src/beliefpropagation/messagecache.jl: a directed-edge -> message store whose
+graph is built from its keys (:15-49), incoming_messages / incoming_edge_data (:124-137),
+vertex_scalar(s) (:139-151), edge_scalar(s) (:153-178), region_scalar (:180-182),
+bethe_free_energy (:185-201) and message_environment (:205-229).
+
+Message layout: a norm-network message on edge {u, v} is a (D, D) array indexed
+[ket link][bra link] (D the fused dimension of the indices u and v share); a plain-network
+message is a length-D vector.  Every contraction below runs in the HIP engine.
+"""
+from __future__ import annotations
+
+import math
+from typing import Dict, Hashable, Iterable, List, Tuple
+
+import numpy as np
+
+from .graphs import NamedGraph, build_topology
+from .itensors import NormNetwork, to_topology
+
+Edge = Tuple[Hashable, Hashable]
+
+
+def _edge(e) -> Edge:
+    if isinstance(e, tuple) and len(e) == 2:
+        return e
+    raise TypeError(f"expected a directed edge (src, dst), got {e!r}")
+
+
+class MessageCache:
+    """Directed edge -> message."""
+
+    def __init__(self, messages=None):
+        self.messages: Dict[Edge, np.ndarray] = {}
+        self.state = None  # stopping-criterion state of the run that produced this cache
+        if messages is not None:
+            items = messages.items() if hasattr(messages, "items") else messages
+            for e, m in items:
+                self[e] = m
+
+    # -- dictionary interface
+    def __getitem__(self, e) -> np.ndarray:
+        return self.messages[_edge(e)]
+
+    def __setitem__(self, e, m) -> None:
+        self.messages[_edge(e)] = np.asarray(m, dtype=np.float64)
+
+    def __contains__(self, e) -> bool:
+        return _edge(e) in self.messages
+
+    def __len__(self) -> int:
+        return len(self.messages)
+
+    def keys(self):
+        return self.messages.keys()
+
+    def values(self):
+        return self.messages.values()
+
+    def items(self):
+        return self.messages.items()
+
+    def edges(self) -> List[Edge]:
+        return list(self.messages.keys())
+
+    def vertices(self) -> List[Hashable]:
+        seen = {}
+        for (u, v) in self.messages:
+            seen.setdefault(u, None)
+            seen.setdefault(v, None)
+        return list(seen)
+
+    def has_edge(self, e) -> bool:
+        return _edge(e) in self.messages
+
+    def copy(self) -> "MessageCache":
+        c = MessageCache({e: m.copy() for e, m in self.messages.items()})
+        c.state = self.state
+        return c
+
+    def map(self, f) -> "MessageCache":
+        return MessageCache({e: f(m) for e, m in self.messages.items()})
+
+    def __repr__(self):
+        return f"MessageCache({len(self)} directed edges)"
+
+    # -- flat buffers in a topology's slot order
+    def to_flat(self, topo) -> np.ndarray:
+        out = np.zeros(int(topo.msg_off[-1]))
+        own = topo.port_owner()
+        for q in range(topo.nports):
+            e = (topo.vertices[own[q]], topo.vertices[topo.port_nbr[q]])
+            if e not in self.messages:
+                raise KeyError(f"no message for directed edge {e}")
+            m = self.messages[e]
+            if m.size != topo.msg_off[q + 1] - topo.msg_off[q]:
+                raise ValueError(f"message {e} has {m.size} entries, expected "
+                                 f"{topo.msg_off[q + 1] - topo.msg_off[q]}")
+            out[topo.msg_off[q]:topo.msg_off[q + 1]] = m.reshape(-1)
+        return out
+
+    @classmethod
+    def from_flat(cls, topo, flat) -> "MessageCache":
+        own = topo.port_owner()
+        c = cls()
+        for q in range(topo.nports):
+            e = (topo.vertices[own[q]], topo.vertices[topo.port_nbr[q]])
+            c.messages[e] = np.array(flat[topo.msg_off[q]:topo.msg_off[q + 1]]).reshape(
+                topo.msg_shape(q))
+        return c
+
+
+def messagecache(pairs) -> MessageCache:
+    """messagecache(pairs) / messagecache(f, edges) (messagecache.jl:51-52)."""
+    return MessageCache(dict(pairs))
+
+
+def incoming_messages(cache: MessageCache, edge) -> List[np.ndarray]:
+    """In-edges of src(edge) except reverse(edge) (messagecache.jl:124-131)."""
+    u, v = _edge(edge)
+    return [m for (a, b), m in cache.items() if b == u and a != v]
+
+
+def incoming_edge_data(cache: MessageCache, vertices: Iterable[Hashable]) -> List[np.ndarray]:
+    """Messages on the in-boundary of a vertex set (messagecache.jl:134-137)."""
+    vs = set(vertices)
+    return [m for (a, b), m in cache.items() if b in vs and a not in vs]
+
+
+# ------------------------------------------------------------------ device evaluation
+
+def _flat(factors):
+    cached = getattr(factors, "_engine_flat", None)
+    if cached is None:
+        cached = to_topology(factors)
+        try:
+            factors._engine_flat = cached
+        except AttributeError:
+            pass
+    return cached
+
+
+def _engine(factors, messages, dtype="f64"):
+    from .engine import Engine
+
+    fl = _flat(factors)
+    mc = messages if isinstance(messages, MessageCache) else MessageCache(messages)
+    e = Engine(fl.topo, fl.sites, dtype=dtype, path="generic" if dtype == "f64" else "auto")
+    e.set_messages(mc.to_flat(fl.topo))
+    return fl, e
+
+
+def _cache_engine(cache: MessageCache):
+    """An engine over the graph of a cache alone (no factors): zero site tensors, used for
+    the message-only quantities (iterate_diff, edge scalars)."""
+    from .engine import Engine
+
+    vs = cache.vertices()
+    g = NamedGraph(vertices=vs)
+    dims = {}
+    plain = None
+    for (u, v), m in cache.items():
+        if not g.has_edge(u, v):
+            g.add_edge(u, v)
+        is_plain = m.ndim == 1
+        plain = is_plain if plain is None else plain
+        if plain != is_plain:
+            raise ValueError("mixed vector and matrix messages")
+        dims[(u, v)] = m.shape[0]
+    topo = build_topology(g, dims, 1, plain=bool(plain))
+    e = Engine(topo, np.zeros(int(topo.site_off[-1])), dtype="f64", path="generic")
+    return topo, e
+
+
+def vertex_scalars(factors, messages, vertices=None) -> np.ndarray:
+    """Z_v for each vertex: the full contraction of factor v with its incoming messages."""
+    fl, e = _engine(factors, messages)
+    try:
+        z = e.vertex_scalars()
+    finally:
+        e.close()
+    if vertices is None:
+        return z
+    idx = {v: i for i, v in enumerate(fl.topo.vertices)}
+    return np.array([z[idx[v]] for v in vertices])
+
+
+def vertex_scalar(factors, messages, vertex) -> float:
+    return float(vertex_scalars(factors, messages, [vertex])[0])
+
+
+def edge_scalars(cache: MessageCache, edges=None) -> np.ndarray:
+    """One Z_e = <m_uv, m_vu> per undirected edge, skipping repeats and reverses
+    (messagecache.jl:159-178), in first-appearance order of `edges` (default: the keys)."""
+    topo, e = _cache_engine(cache)
+    try:
+        e.set_messages(cache.to_flat(topo))
+        ze = e.edge_scalars()
+    finally:
+        e.close()
+    eidx = {frozenset(ed): k for k, ed in enumerate(topo.edges)}
+    out, seen = [], set()
+    for ed in (edges if edges is not None else cache.edges()):
+        key = frozenset(ed)
+        if key in seen:
+            continue
+        seen.add(key)
+        out.append(ze[eidx[key]])
+    return np.array(out)
+
+
+def edge_scalar(cache: MessageCache, edge) -> float:
+    return float(edge_scalars(cache, [edge])[0])
+
+
+def region_scalar(factors, messages, region) -> float:
+    """The product of the vertex scalars of the region (messagecache.jl:180-182)."""
+    return float(np.prod(vertex_scalars(factors, messages, list(region))))
+
+
+def bethe_free_energy(factors, messages):
+    """sum log Z_v - sum log Z_e, complex when a term is negative, -Inf when an edge term
+    vanishes (messagecache.jl:185-201); compensated FP64 sums on the device."""
+    fl, e = _engine(factors, messages)
+    try:
+        v = e.lognorm()
+    finally:
+        e.close()
+    return v
+
+
+def iterate_diff(cache1: MessageCache, cache2: MessageCache) -> float:
+    """max over directed edges of 1 - |<m1^, m2^>|^2 (beliefpropagation.jl:261-267)."""
+    topo, e = _cache_engine(cache1)
+    try:
+        e.set_messages(cache1.to_flat(topo))
+        return e.iterate_diff(cache2.to_flat(topo))
+    finally:
+        e.close()
+
+
+def one(template: np.ndarray) -> np.ndarray:
+    """`f = one` of message_environment: the identity message."""
+    if template.ndim == 1:
+        return np.ones_like(template)
+    return np.eye(template.shape[0], dtype=template.dtype)
+
+
+def similar_message_environment(nn) -> MessageCache:
+    """A zero message for every in-edge of every vertex (messagecache.jl:205-225)."""
+    fl = _flat(nn)
+    topo = fl.topo
+    own = topo.port_owner()
+    c = MessageCache()
+    for q in range(topo.nports):
+        e = (topo.vertices[own[q]], topo.vertices[topo.port_nbr[q]])
+        c.messages[e] = np.zeros(topo.msg_shape(q))
+    return c
+
+
+def message_environment(f, nn) -> MessageCache:
+    """map(f, similar_message_environment(nn)) (messagecache.jl:227-229)."""
+    return similar_message_environment(nn).map(f)
