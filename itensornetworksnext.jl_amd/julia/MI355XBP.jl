@@ -1,0 +1,235 @@
+"""
+    MI355XBP
+
+Julia binding of libbpmi355x.so (include/bp_mi355x.h) for ITensorNetworksNext.jl.
+
+Adds two strategies to the package's BP plugin surface
+(src/beliefpropagation/beliefpropagation.jl:214-257, src/select_algorithm.jl:41-51):
+
+  * `DeviceJacobiUpdate(; dtype = :f32, path = :auto, polish = 0)` — passed as
+    `message_update_algorithm`, it makes `beliefpropagation` run whole Jacobi sweeps on the
+    GPU: the edge schedule collapses to one `JacobiSweep()` token per sweep, and
+    `message_update!(alg, cache, factors, ::JacobiSweep)` performs the sweep on the device
+    (bp_sweep), so the reference's outer loop, StopWhenConverged and iterate_diff keep
+    their meaning;
+  * `device_beliefpropagation(nn, messages; stopping_criterion, alg)` — the same solve in one
+    call with convergence tracked on the device (bp_run), returning a `MessageCache`.
+
+This file is not exercised in the build container (no Julia there); the same C ABI is
+exercised by the Python ctypes binding and its tests.
+"""
+module MI355XBP
+
+using ITensorNetworksNext: ITensorNetworksNext, MessageCache, MessageUpdateAlgorithm,
+    NormNetwork, beliefpropagation, kettensor, linkinds, siteinds
+using ITensorBase: dimnames, unnamed, name
+using Graphs: vertices, edges, src, dst, neighbors
+import ITensorNetworksNext: message_update!
+
+const libbp = Ref{String}(joinpath(@__DIR__, "..", "csrc", "libbpmi355x.so"))
+
+const BP_F64, BP_F32, BP_BF16 = Cint(0), Cint(1), Cint(2)
+const BP_INIT_UPLOAD = Cint(2)
+const BP_CRIT_FIDELITY = Cint(0)
+
+function check(rc::Cint, what)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:bp_last_error, libbp[]), Cstring, ()))
+    return error("$what failed (status $rc): $msg")
+end
+
+"""A device context holding one flattened NormNetwork."""
+mutable struct DeviceContext
+    handle::Ptr{Cvoid}
+    vertices::Vector{Any}
+    slots::Dict{Any, UnitRange{Int}}   # directed edge => range in the flat message buffer
+    dims::Dict{Any, Int}
+    nmsg::Int
+end
+
+function DeviceContext(nn::NormNetwork; dtype::Symbol = :f32, device::Integer = 0)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    dt = dtype === :f64 ? BP_F64 : dtype === :bf16 ? BP_BF16 : BP_F32
+    check(ccall((:bp_create, libbp[]), Cint, (Ref{Ptr{Cvoid}}, Cint, Cint), h, device, dt),
+        "bp_create")
+    vs = collect(vertices(nn))
+    vidx = Dict(v => i - 1 for (i, v) in enumerate(vs))
+    es = collect(edges(nn))
+    eidx = Dict{Any, Int}()
+    for (k, e) in enumerate(es)
+        eidx[(src(e), dst(e))] = k - 1
+        eidx[(dst(e), src(e))] = k - 1
+    end
+    port_off = Cint[0]
+    port_nbr = Cint[]
+    port_edge = Cint[]
+    nbrs = Dict(v => sort(collect(neighbors(nn, v)); by = w -> vidx[w]) for v in vs)
+    for v in vs
+        for w in nbrs[v]
+            push!(port_nbr, vidx[w])
+            push!(port_edge, eidx[(v, w)])
+        end
+        push!(port_off, length(port_nbr))
+    end
+    port_rev = similar(port_nbr)
+    for (i, v) in enumerate(vs), (j, w) in enumerate(nbrs[v])
+        q = port_off[i] + j - 1
+        k = findfirst(==(v), nbrs[w])
+        port_rev[q + 1] = port_off[vidx[w] + 1] + k - 1
+    end
+    edge_dim = Cint[prod(length ∘ unnamed, linkinds(nn.ket, (src(e), dst(e)))) for e in es]
+    phys_dim = Cint[prod(length ∘ unnamed, siteinds(nn.ket, v)) for v in vs]
+    check(ccall((:bp_set_topology, libbp[]), Cint,
+            (Ptr{Cvoid}, Cint, Cint, Ptr{Cint}, Ptr{Cint}, Ptr{Cint}, Ptr{Cint}, Ptr{Cint},
+                Ptr{Cint}, Ptr{Cint}, Cint, Cint),
+            h[], length(vs), length(es), port_off, port_nbr, port_edge, port_rev, edge_dim,
+            phys_dim, C_NULL, 1, 0), "bp_set_topology")
+    # site tensors permuted to [site][port 0]...[port k-1], row-major, Float64
+    sites = Float64[]
+    for v in vs
+        t = kettensor(nn, v)
+        order = vcat(collect(siteinds(nn.ket, v)),
+            reduce(vcat, [collect(linkinds(nn.ket, (v, w))) for w in nbrs[v]]; init = []))
+        a = permutedims(unnamed(t, order...), reverse(1:length(order)))  # row-major flatten
+        append!(sites, vec(Array{Float64}(a)))
+    end
+    GC.@preserve sites check(ccall((:bp_upload_sites, libbp[]), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}), h[], sites), "bp_upload_sites")
+    slots = Dict{Any, UnitRange{Int}}()
+    dims = Dict{Any, Int}()
+    off = 0
+    for (i, v) in enumerate(vs), w in nbrs[v]
+        D = edge_dim[eidx[(v, w)] + 1]
+        slots[(v, w)] = (off + 1):(off + D * D)
+        dims[(v, w)] = D
+        off += D * D
+    end
+    ctx = DeviceContext(h[], vs, slots, dims, off)
+    finalizer(c -> ccall((:bp_destroy, libbp[]), Cint, (Ptr{Cvoid},), c.handle), ctx)
+    return ctx
+end
+
+"""Messages [ket][bra] as row-major D x D blocks of the flat buffer (the message of
+edge v => w is produced at v and consumed at w)."""
+function upload!(ctx::DeviceContext, cache)
+    flat = zeros(Float64, ctx.nmsg)
+    for (e, r) in ctx.slots
+        m = Array{Float64}(unnamed(cache[e[1] => e[2]]))
+        flat[r] = vec(permutedims(m, (2, 1)))
+    end
+    GC.@preserve flat check(ccall((:bp_init_messages, libbp[]), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Float64}), ctx.handle, BP_INIT_UPLOAD, flat), "bp_init_messages")
+    return ctx
+end
+
+function download!(cache, ctx::DeviceContext)
+    flat = zeros(Float64, ctx.nmsg)
+    GC.@preserve flat check(ccall((:bp_download_messages, libbp[]), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}), ctx.handle, flat), "bp_download_messages")
+    for (e, r) in ctx.slots
+        D = ctx.dims[e]
+        msg = cache[e[1] => e[2]]
+        copyto!(unnamed(msg), permutedims(reshape(flat[r], D, D), (2, 1)))
+    end
+    return cache
+end
+
+struct JacobiSweep end
+
+Base.@kwdef struct DeviceJacobiUpdate <: MessageUpdateAlgorithm
+    dtype::Symbol = :f32
+    path::Symbol = :auto
+    polish::Int = 0
+end
+
+const CONTEXTS = IdDict{Any, DeviceContext}()
+
+context(alg::DeviceJacobiUpdate, nn) = get!(() -> DeviceContext(nn; alg.dtype), CONTEXTS, nn)
+
+# One device Jacobi sweep per outer iteration: `edges = [JacobiSweep()]`.
+function message_update!(alg::DeviceJacobiUpdate, cache, factors::NormNetwork, ::JacobiSweep)
+    ctx = context(alg, factors)
+    upload!(ctx, cache)
+    check(ccall((:bp_sweep, libbp[]), Cint, (Ptr{Cvoid},), ctx.handle), "bp_sweep")
+    download!(cache, ctx)
+    return cache
+end
+
+"""Whole solve on the device with the reference's stopping semantics."""
+function device_beliefpropagation(nn::NormNetwork, messages; stopping_criterion,
+        alg::DeviceJacobiUpdate = DeviceJacobiUpdate())
+    ctx = context(alg, nn)
+    cache = MessageCache(messages)
+    upload!(ctx, cache)
+    maxiter = get(stopping_criterion, :maxiter, 0)
+    tol = get(stopping_criterion, :tol, -1.0)
+    sweeps = Ref{Cint}(0)
+    delta = Ref{Float64}(0.0)
+    at = Ref{Cint}(-1)
+    check(ccall((:bp_run, libbp[]), Cint,
+            (Ptr{Cvoid}, Cint, Cdouble, Cint, Cint, Ref{Cint}, Ref{Float64}, Ref{Cint}),
+            ctx.handle, maxiter, tol, BP_CRIT_FIDELITY, 1, sweeps, delta, at), "bp_run")
+    if alg.polish > 0
+        check(ccall((:bp_polish_fp64, libbp[]), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}),
+            ctx.handle, alg.polish, C_NULL), "bp_polish_fp64")
+    end
+    download!(cache, ctx)
+    return cache, (; sweeps = sweeps[], delta = delta[], at_iteration = at[])
+end
+
+"""BP log-norm of the device's current messages (bethe_free_energy)."""
+function device_lognorm(ctx::DeviceContext)
+    re = Ref{Float64}(0.0)
+    im = Ref{Float64}(0.0)
+    check(ccall((:bp_lognorm, libbp[]), Cint, (Ptr{Cvoid}, Ref{Float64}, Ref{Float64}),
+        ctx.handle, re, im), "bp_lognorm")
+    return im[] == 0 ? re[] : complex(re[], im[])
+end
+
+"""Gradient of the BP log-norm with respect to every site tensor (flat, site layout)."""
+function device_grad_lognorm(ctx::DeviceContext, nsite::Integer)
+    g = zeros(Float64, nsite)
+    GC.@preserve g check(ccall((:bp_grad_lognorm, libbp[]), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}), ctx.handle, g), "bp_grad_lognorm")
+    return g
+end
+
+"""BP-estimated TFIM energy and its gradient by the device's fixed-point adjoint."""
+function device_tfim_energy_and_grad(ctx::DeviceContext, nsite::Integer; J = 1.0, h = 3.0,
+        adj_tol = 1.0e-8, adj_maxiter = 1000)
+    E = Ref{Float64}(0.0)
+    g = zeros(Float64, nsite)
+    it = Ref{Cint}(0)
+    res = Ref{Float64}(0.0)
+    GC.@preserve g check(ccall((:bp_tfim_energy, libbp[]), Cint,
+        (Ptr{Cvoid}, Cdouble, Cdouble, Ref{Float64}, Ptr{Float64}, Cdouble, Cint, Ref{Cint},
+            Ref{Float64}), ctx.handle, J, h, E, g, adj_tol, adj_maxiter, it, res),
+        "bp_tfim_energy")
+    return E[], g
+end
+
+# ChainRulesCore rrule: the pullback of the converged-BP energy with respect to the flat
+# site buffer is the adjoint gradient (implicit differentiation of the fixed point).
+function tfim_energy(ctx::DeviceContext, sites::AbstractVector{Float64}; J = 1.0, h = 3.0)
+    GC.@preserve sites check(ccall((:bp_upload_sites, libbp[]), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}), ctx.handle, sites), "bp_upload_sites")
+    check(ccall((:bp_init_messages, libbp[]), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}),
+        ctx.handle, Cint(0), C_NULL), "bp_init_messages")
+    check(ccall((:bp_run, libbp[]), Cint,
+            (Ptr{Cvoid}, Cint, Cdouble, Cint, Cint, Ptr{Cint}, Ptr{Float64}, Ptr{Cint}),
+            ctx.handle, 2000, 1.0e-12, BP_CRIT_FIDELITY, 1, C_NULL, C_NULL, C_NULL), "bp_run")
+    E, _ = device_tfim_energy_and_grad(ctx, length(sites); J, h)
+    return E
+end
+
+using ChainRulesCore: ChainRulesCore, NoTangent
+
+function ChainRulesCore.rrule(::typeof(tfim_energy), ctx::DeviceContext,
+        sites::AbstractVector{Float64}; J = 1.0, h = 3.0)
+    E = tfim_energy(ctx, sites; J, h)
+    _, g = device_tfim_energy_and_grad(ctx, length(sites); J, h)
+    pullback(ΔE) = (NoTangent(), NoTangent(), ΔE .* g)
+    return E, pullback
+end
+
+end # module

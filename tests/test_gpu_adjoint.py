@@ -102,3 +102,30 @@ def test_tfim_fp32_vs_fp64_oracle(pkg, orc):
     _say(f"fp32 E={E:.8f} oracle={Eo:.8f} adjoint sweeps={it} res={res:.2e}")
     assert E == pytest.approx(Eo, rel=1e-6)
     assert np.max(np.abs(g - want)) / np.max(np.abs(want)) < 1e-3
+
+
+def test_autograd_function(pkg):
+    """torch.autograd over the device adjoint: d/dA of the TFIM energy and of the log-norm
+    equal the engine's gradients, and a first-order Taylor check holds."""
+    import importlib
+
+    import torch
+
+    _say("case autograd")
+    ag = importlib.import_module(pkg.__name__ + ".autograd")
+    t, s = _setup(pkg, 4, 2, 35)
+    x = torch.tensor(s, dtype=torch.float64, requires_grad=True)
+    E = ag.bp_tfim_energy(x, t, 1.0, 3.0)
+    E.backward()
+    g = x.grad.numpy().copy()
+    e = _converged_engine(pkg, t, s, "f64", 1e-12)
+    E2, g2, _, _ = e.tfim_energy(1.0, 3.0, grad=True, adj_tol=1e-10)
+    e.close()
+    assert float(E) == pytest.approx(E2, rel=1e-10)
+    np.testing.assert_allclose(g, g2, rtol=1e-8, atol=1e-10 * np.max(np.abs(g2)))
+    d = np.random.default_rng(1).standard_normal(s.size) * 1e-6
+    E3 = ag.bp_tfim_energy(torch.tensor(s + d), t, 1.0, 3.0)
+    assert float(E3) - float(E) == pytest.approx(float(g @ d), rel=1e-3)
+    y = torch.tensor(s, dtype=torch.float64, requires_grad=True)
+    ag.bp_lognorm(y, t).backward()
+    assert np.all(np.isfinite(y.grad.numpy()))
