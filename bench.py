@@ -171,6 +171,20 @@ def main(argv=None):
         extras["speedup_vs_fp64_plain_gpu"] = value / extras["fp64_plain_gpu_msg_per_s"]
         e64.close()
     eng.close()
+    if not args.no_extras and args.dtype == "f32" and D == 8:
+        eb = pkg.Engine(topo, sites, dtype="bf16")
+        eb.init_messages("identity")
+        eb.run(maxiter=10)
+        eb.init_messages("identity")
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        eb.run(maxiter=args.steps, use_graph=True)
+        torch.cuda.synchronize()
+        wb = time.perf_counter() - t3
+        kb = eb.time_sweeps(max(20, min(args.steps, 200)))
+        extras["bf16"] = {"msg_per_s": nmsg * args.steps / wb, "kernel_ms": kb,
+                          "algorithmic_TFLOPs": flops / (kb * 1e-3) / 1e12}
+        eb.close()
     if not args.no_extras:
         extras["adjoint"] = [adjoint_timing(pkg, torch, L, D, args.dtype, args.seed),
                              adjoint_timing(pkg, torch, 16, 6, args.dtype, args.seed)]

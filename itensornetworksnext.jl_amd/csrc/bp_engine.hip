@@ -372,6 +372,8 @@ static int launch_edge_scalars(bp_context *c, const T *msgs) {
 // ------------------------------------------------------------------ path selection
 
 static int resolve_path(bp_context *c) {
+    // BF16 contractions exist only on the vertex path: no silent FP32 substitute
+    if (c->dtype == BP_BF16) return c->vplan_ok && c->path != BP_PATH_GENERIC ? BP_PATH_VERTEX : -1;
     if (c->path == BP_PATH_GENERIC) return BP_PATH_GENERIC;
     if (c->path == BP_PATH_VERTEX) return c->vplan_ok ? BP_PATH_VERTEX : -1;
     // AUTO: the vertex-centric MFMA path whenever the plan supports this network
@@ -711,6 +713,8 @@ int bp_set_path(bp_context *c, int path) {
     if (path == BP_PATH_VERTEX && !c->vplan_ok)
         return set_err(BP_ERR_UNSUPPORTED, "vertex path not available: %s",
                        c->vplan.why.c_str());
+    if (path == BP_PATH_GENERIC && c->dtype == BP_BF16)
+        return set_err(BP_ERR_UNSUPPORTED, "BF16 contractions run on the vertex path only");
     c->path = path;
     return BP_OK;
 }
@@ -718,6 +722,9 @@ int bp_set_path(bp_context *c, int path) {
 int bp_get_path(bp_context *c, int *path) {
     if (!c || !path) return set_err(BP_ERR_ARG, "NULL argument");
     *path = resolve_path(c);
+    if (*path < 0)
+        return set_err(BP_ERR_UNSUPPORTED, "no kernel path for this configuration: %s",
+                       c->vplan.why.c_str());
     return BP_OK;
 }
 

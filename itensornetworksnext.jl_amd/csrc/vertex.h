@@ -1,4 +1,4 @@
-// vertex.h -- vertex-centric BP sweep for bond dimension 8 on gfx950 (FP32, MFMA).
+// vertex.h -- vertex-centric BP sweep for bond dimension 8 on gfx950 (MFMA, FP32 or BF16).
 //
 // One workgroup per vertex u computes all of u's outgoing messages (the reference
 // computes each message separately, beliefpropagation.jl:242-257; the result is the
@@ -6,10 +6,11 @@
 // A_u[s] (4 ports of dimension 8 = 4096 values, 64 per lane) in registers in one of two
 // layouts
 //     X: register r = (p0, p1), lane l = (p2, p3)      Y: register r = (p2, p3), lane = (p0, p1)
-// A ket-side mode product on a register port ("T x_j m_j") runs on
-// v_mfma_f32_4x4x1_16b_f32: register i of lane 4k+j accumulates A(lane 4k+i) * B(lane 4k+j),
-// so with A = m_j[a][4h + lane%4] (the same for every block) and B = the lane's own
-// tensor column, each lane gets its output column without any data movement.
+// A ket-side mode product on a register port ("T x_j m_j") runs on the 16-block 4x4
+// MFMAs: register i of lane 4k+j accumulates A(lane 4k+i) . B(lane 4k+j) (one product for
+// v_mfma_f32_4x4x1_16b_f32, a 4-term dot of BF16 pairs for v_mfma_f32_4x4x4_16b_bf16), so
+// with A = m_j[a][4h + lane%4] (the same for every block) and B = the lane's own tensor
+// column, each lane gets its output column without any data movement.
 // Partial products are shared between the four outgoing messages:
 //     E01 = A x0 m0 x1 m1 (layout X)  -> transposed to Y -> out2, out3
 //     E23 = A x2 m2 x3 m3 (layout Y)  -> transposed to X -> out0, out1
@@ -20,6 +21,12 @@
 // beyond index 0 and whose message is the identity; the result is unchanged.
 // The two LDS buffers per wave hold A (transposable, row pitch 65 floats: conflict-free
 // row and column access) and the transposed partial product.
+//
+// BF16 mode: every MFMA operand (site entries, messages, partial products) is rounded to
+// BF16 (round to nearest even) and products accumulate in FP32; the open contraction
+// uses the same BF16-rounded operands with FP32 sums.  Incoming messages are scaled by
+// 1 / max|m| before rounding (the scale cancels in the sum normalisation).  Site slices
+// are stored as BF16 pairs (r, r+1) per lane so each load is a coalesced 256-byte row.
 #pragma once
 
 #include <string>
@@ -28,11 +35,13 @@
 #include "common.h"
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef short v4s __attribute__((ext_vector_type(4)));
 
 struct VertexPlan {
     std::string why;
     int d = 0;                  // physical dimension (waves per workgroup)
     int nv = 0;
+    int bf16 = 0;               // BF16 operands
     int4 *d_vin = nullptr;      // per vertex: incoming message slot per padded port (-1: dummy)
     int4 *d_vout = nullptr;     // per vertex: outgoing message slot per padded port (-1: dummy)
     int *d_vnet = nullptr;      // network of each vertex
@@ -59,7 +68,7 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     (void)msg_off;
     p->why.clear();
     if (plain) { p->why = "plain (vector-message) networks use the generic path"; return 4; }
-    if (dtype != 1) { p->why = "the vertex path computes in FP32"; return 4; }
+    if (dtype != 1 && dtype != 2) { p->why = "the vertex path computes in FP32 or BF16"; return 4; }
     const int d = phys_dim[0];
     for (int u = 0; u < nv; ++u) {
         const int k = port_off[u + 1] - port_off[u];
@@ -95,45 +104,88 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     }
     p->d = d;
     p->nv = nv;
+    p->bf16 = dtype == 2;
     // per wave: two transposable buffers; per workgroup: 4 messages + d x 4 x 64 partials
     p->lds_bytes = sizeof(float) * ((size_t)d * 2 * VX_BUF + 4 * 64 + (size_t)d * 4 * 64);
     return 0;
 }
 
-// vsites[u][s][4096]: the site tensor of u zero-padded to 4 ports of dimension 8, row-major
-// over (p0, p1, p2, p3) = layout X order (element r * 64 + l).
+__device__ __forceinline__ unsigned short bf16_rne(float x) {
+    unsigned u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float bf16_to_f(unsigned short h) {
+    return __uint_as_float(((unsigned)h) << 16);
+}
+
+// Hardware round-to-nearest-even of two floats into a BF16 pair (v_cvt_pk_bf16_f32).
+typedef float vf2 __attribute__((ext_vector_type(2)));
+typedef __bf16 vbf2 __attribute__((ext_vector_type(2)));
+typedef unsigned vu2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned bf16x2(float a, float b) {
+    const vf2 f = {a, b};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(f, vbf2));
+}
+
+__device__ __forceinline__ v4s bf16x4(float a, float b, float c, float d) {
+    const vu2 w = {bf16x2(a, b), bf16x2(c, d)};
+    return __builtin_bit_cast(v4s, w);
+}
+
+// x rounded to BF16 and back, two at a time
+__device__ __forceinline__ void bf16_round2(float &a, float &b) {
+    const unsigned p = bf16x2(a, b);
+    a = __uint_as_float(p << 16);
+    b = __uint_as_float(p & 0xffff0000u);
+}
+
+// Element e of the padded slice of u (layout X order, e = r * 64 + l).
+__device__ __forceinline__ float vx_pad_value(const double *sites64, const long long *site_off,
+                                              const int *port_off, int d, long long us, int e) {
+    const int s = (int)(us % d);
+    const int u = (int)(us / d);
+    const int k = port_off[u + 1] - port_off[u];
+    const int idx[4] = {(e >> 9) & 7, (e >> 6) & 7, (e >> 3) & 7, e & 7};
+    long long off = 0;
+    for (int j = 0; j < 4; ++j) {
+        if (j < k) off = off * 8 + idx[j];
+        else if (idx[j] != 0) return 0.f;
+    }
+    long long n = 1;
+    for (int j = 0; j < k; ++j) n *= 8;
+    return (float)sites64[site_off[u] + s * n + off];
+}
+
+// FP32: vsites[u][s][r * 64 + l].  BF16: vsites[u][s][(r / 2) * 128 + 2 l + (r % 2)].
 __global__ void k_vertex_pad_sites(const double *sites64, const long long *site_off,
-                                   const int *port_off, int d, int nv, float *vsites) {
+                                   const int *port_off, int d, int nv, int bf16, void *vsites) {
     const long long total = (long long)nv * d * 4096;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
          i += (long long)gridDim.x * blockDim.x) {
         const int e = (int)(i & 4095);
         const long long us = i >> 12;
-        const int s = (int)(us % d);
-        const int u = (int)(us / d);
-        const int k = port_off[u + 1] - port_off[u];
-        const int idx[4] = {(e >> 9) & 7, (e >> 6) & 7, (e >> 3) & 7, e & 7};
-        bool zero = false;
-        long long off = 0;
-        for (int j = 0; j < 4; ++j) {
-            if (j < k) off = off * 8 + idx[j];
-            else if (idx[j] != 0) zero = true;
+        const float v = vx_pad_value(sites64, site_off, port_off, d, us, e);
+        if (bf16) {
+            const int r = e >> 6, l = e & 63;
+            ((unsigned short *)vsites)[(us << 12) + (r >> 1) * 128 + 2 * l + (r & 1)] = bf16_rne(v);
+        } else {
+            ((float *)vsites)[i] = v;
         }
-        long long n = 1;
-        for (int j = 0; j < k; ++j) n *= 8;
-        vsites[i] = zero ? 0.f : (float)sites64[site_off[u] + s * n + off];
     }
 }
 
 static int vertex_upload_sites(VertexPlan *p, hipStream_t st, const double *d_sites64,
                                void **d_vsites, const long long *d_site_off,
                                const int *d_port_off) {
-    const size_t bytes = sizeof(float) * (size_t)p->nv * p->d * 4096;
+    const size_t bytes = (p->bf16 ? 2 : 4) * (size_t)p->nv * p->d * 4096;
     if (*d_vsites) (void)hipFree(*d_vsites);
     *d_vsites = nullptr;
     if (hipMalloc(d_vsites, bytes) != hipSuccess) return 5;
     hipLaunchKernelGGL(k_vertex_pad_sites, dim3(2048), dim3(256), 0, st, d_sites64, d_site_off,
-                       d_port_off, p->d, p->nv, (float *)*d_vsites);
+                       d_port_off, p->d, p->nv, p->bf16, *d_vsites);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
@@ -141,7 +193,7 @@ static int vertex_upload_sites(VertexPlan *p, hipStream_t st, const double *d_si
 
 // Mode product on a register port of an 8x8 register block T[hi*8 + lo]:
 //   HI: O[b*8 + lo] = sum_a T[a*8 + lo] m[a][b]     LO: O[hi*8 + b] = sum_a T[hi*8 + a] m[a][b]
-// M[a*2 + h] holds m[a][4h + lane%4] (the A operand of the 16 4x4 blocks).
+// FP32: M[a*2 + h] holds m[a][4h + lane%4].
 template <bool HI>
 __device__ __forceinline__ void vx_mp(const float (&T)[64], float (&O)[64], const float (&M)[16]) {
 #pragma unroll
@@ -163,12 +215,68 @@ __device__ __forceinline__ void vx_mp(const float (&T)[64], float (&O)[64], cons
     }
 }
 
+// BF16: Mb[g*2 + h] packs m[4g + t][4h + lane%4], t = 0..3 (g = 0, 1 halves of a).
+template <bool HI>
+__device__ __forceinline__ void vx_mp_bf16(const float (&T)[64], float (&O)[64], const v4s (&Mb)[4]) {
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        v4s bq[2];
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+            float v[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int a = 4 * g + t;
+                v[t] = HI ? T[a * 8 + x] : T[x * 8 + a];
+            }
+            bq[g] = bf16x4(v[0], v[1], v[2], v[3]);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int g = 0; g < 2; ++g)
+                acc = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(Mb[g * 2 + h], bq[g], acc, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int bb = 4 * h + i;
+                if (HI) O[bb * 8 + x] = acc[i]; else O[x * 8 + bb] = acc[i];
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ void vx_load_m(const float *msg, float (&M)[16]) {
     const int c = threadIdx.x & 3;
 #pragma unroll
     for (int a = 0; a < 8; ++a)
 #pragma unroll
         for (int h = 0; h < 2; ++h) M[a * 2 + h] = msg[a * 8 + 4 * h + c];
+}
+
+__device__ __forceinline__ void vx_load_m_bf16(const float *msg, v4s (&Mb)[4]) {
+    const int c = threadIdx.x & 3;
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const float *p = msg + 4 * g * 8 + 4 * h + c;
+            Mb[g * 2 + h] = bf16x4(p[0], p[8], p[16], p[24]);
+        }
+}
+
+// Mode product in the precision of the kernel.
+template <bool HI, bool BF>
+__device__ __forceinline__ void vx_mode(const float (&T)[64], float (&O)[64], const float *msg) {
+    if constexpr (BF) {
+        v4s Mb[4];
+        vx_load_m_bf16(msg, Mb);
+        vx_mp_bf16<HI>(T, O, Mb);
+    } else {
+        float M[16];
+        vx_load_m(msg, M);
+        vx_mp<HI>(T, O, M);
+    }
 }
 
 // Sum v[i] over the 64 lanes for every i; lane l returns entry l (halving exchange).
@@ -208,10 +316,18 @@ __device__ __forceinline__ float vx_reduce64(float (&v)[64]) {
 // Open contraction with the bra tensor read from the wave's A buffer.
 //   HI (open port = register hi index): out[a][a'] = sum_lo G[a*8+lo] A[a'*8+lo]
 //   LO (open port = register lo index): out[a][a'] = sum_hi G[hi*8+a] A[hi*8+a']
-// XL: the A buffer is read in layout X (else Y).  Returns entry (lane/8, lane%8).
-template <bool HI, bool XL>
-__device__ __forceinline__ float vx_final(const float (&G)[64], const float *bufA) {
+// XL: the A buffer is read in layout X (else Y).  Returns entry (lane/8, lane%8).  In BF16
+// mode the partial product G is rounded to BF16 first (the buffer already holds BF16 values).
+template <bool HI, bool XL, bool BF>
+__device__ __forceinline__ float vx_final(const float (&Gin)[64], const float *bufA) {
     const int lane = threadIdx.x & 63;
+    float G[64];
+#pragma unroll
+    for (int i = 0; i < 64; ++i) G[i] = Gin[i];
+    if constexpr (BF) {
+#pragma unroll
+        for (int i = 0; i < 64; i += 2) bf16_round2(G[i], G[i + 1]);
+    }
     float acc[64];
 #pragma unroll
     for (int i = 0; i < 64; ++i) acc[i] = 0.f;
@@ -230,9 +346,9 @@ __device__ __forceinline__ float vx_final(const float (&G)[64], const float *buf
 
 // ---------------------------------------------------------------- the sweep kernel
 
-template <int NS>
+template <int NS, bool BF>
 __global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(1, 1)))
-void k_vertex_d8(const float *__restrict__ vsites, const float *__restrict__ mold,
+void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mold,
                  float *__restrict__ mnew, const int4 *__restrict__ vin,
                  const int4 *__restrict__ vout, const int *__restrict__ vnet,
                  NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs) {
@@ -257,57 +373,67 @@ void k_vertex_d8(const float *__restrict__ vsites, const float *__restrict__ mol
         }
         return;
     }
-    // incoming messages (identity on padded ports)
+    // incoming messages (identity on padded ports); BF16 mode scales each by 1 / max|m|
     for (int i = threadIdx.x; i < 4 * 64; i += 64 * NS) {
         const int j = i >> 6, e = i & 63;
         msm[i] = ins[j] >= 0 ? mold[(long long)ins[j] * 64 + e] : ((e >> 3) == (e & 7) ? 1.f : 0.f);
     }
+    if constexpr (BF) {
+        __syncthreads();
+        for (int j = w; j < 4; j += NS) {
+            const float x = msm[j * 64 + lane];
+            const float mx = wave_max(fabsf(x));
+            msm[j * 64 + lane] = mx > 0.f ? x / mx : x;
+        }
+    }
     // slice A[s] in layout X
     float T[64], E[64], G[64];
-    const float *src = vsites + ((long long)u * NS + w) * 4096;
+    if constexpr (BF) {
+        const unsigned *src = (const unsigned *)vsites_v + ((long long)u * NS + w) * 2048;
 #pragma unroll
-    for (int r = 0; r < 64; ++r) T[r] = src[r * 64 + lane];
+        for (int r2 = 0; r2 < 32; ++r2) {
+            const unsigned pr = src[r2 * 64 + lane];
+            T[2 * r2] = bf16_to_f((unsigned short)(pr & 0xffffu));
+            T[2 * r2 + 1] = bf16_to_f((unsigned short)(pr >> 16));
+        }
+    } else {
+        const float *src = (const float *)vsites_v + ((long long)u * NS + w) * 4096;
+#pragma unroll
+        for (int r = 0; r < 64; ++r) T[r] = src[r * 64 + lane];
+    }
 #pragma unroll
     for (int r = 0; r < 64; ++r) bufA[r * VX_PITCH + lane] = T[r];
     __syncthreads();
-    float M[16], out[4];
+    float out[4];
 
     // ---- phase A: E01 = A x0 m0 x1 m1 (X) -> Y; out2 (open p2), out3 (open p3)
-    vx_load_m(msm + 0 * 64, M);
-    vx_mp<true>(T, E, M);
-    vx_load_m(msm + 1 * 64, M);
-    vx_mp<false>(E, G, M);
+    vx_mode<true, BF>(T, E, msm + 0 * 64);
+    vx_mode<false, BF>(E, G, msm + 1 * 64);
 #pragma unroll
     for (int r = 0; r < 64; ++r) bufB[r * VX_PITCH + lane] = G[r];
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 64; ++r) E[r] = bufB[lane * VX_PITCH + r];  // E01 in Y
-    vx_load_m(msm + 3 * 64, M);
-    vx_mp<false>(E, G, M);
-    out[2] = vx_final<true, false>(G, bufA);
-    vx_load_m(msm + 2 * 64, M);
-    vx_mp<true>(E, G, M);
-    out[3] = vx_final<false, false>(G, bufA);
+    vx_mode<false, BF>(E, G, msm + 3 * 64);
+    out[2] = vx_final<true, false, BF>(G, bufA);
+    vx_mode<true, BF>(E, G, msm + 2 * 64);
+    out[3] = vx_final<false, false, BF>(G, bufA);
 
     // ---- phase B: E23 = A x2 m2 x3 m3 (Y) -> X; out0 (open p0), out1 (open p1)
 #pragma unroll
     for (int r = 0; r < 64; ++r) T[r] = bufA[lane * VX_PITCH + r];  // A in Y
-    vx_load_m(msm + 2 * 64, M);
-    vx_mp<true>(T, E, M);
-    vx_load_m(msm + 3 * 64, M);
-    vx_mp<false>(E, G, M);
+    vx_mode<true, BF>(T, E, msm + 2 * 64);
+    vx_mode<false, BF>(E, G, msm + 3 * 64);
     __syncthreads();  // every lane has finished reading bufB (phase A)
 #pragma unroll
     for (int r = 0; r < 64; ++r) bufB[lane * VX_PITCH + r] = G[r];  // write from Y
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 64; ++r) E[r] = bufB[r * VX_PITCH + lane];  // E23 in X
-    vx_load_m(msm + 1 * 64, M);
-    vx_mp<false>(E, G, M);
-    out[0] = vx_final<true, true>(G, bufA);
-    vx_load_m(msm + 0 * 64, M);
-    vx_mp<true>(E, G, M);
-    out[1] = vx_final<false, true>(G, bufA);
+    vx_mode<false, BF>(E, G, msm + 1 * 64);
+    out[0] = vx_final<true, true, BF>(G, bufA);
+    vx_mode<true, BF>(E, G, msm + 0 * 64);
+    out[1] = vx_final<false, true, BF>(G, bufA);
 
     // ---- sum over s, normalise, residual, write
 #pragma unroll
@@ -329,8 +455,7 @@ void k_vertex_d8(const float *__restrict__ vsites, const float *__restrict__ mol
         const double mx = wave_max(fabs((double)xf - xo));
         if (lane == 0) {
             const double c = dn / sqrt(nn * oo);
-            const double fid = 1.0 - c * c;
-            res_fid[q] = fid;
+            res_fid[q] = 1.0 - c * c;
             res_abs[q] = mx;
         }
     }
@@ -340,17 +465,22 @@ static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *v
                         const void *src, void *dst, const NetCtl &ctl, double *res_fid,
                         double *res_abs) {
     (void)tp;
-    switch (p->d) {
-#define VX_CASE(NS)                                                                         \
-    case NS:                                                                                \
-        hipLaunchKernelGGL(k_vertex_d8<NS>, dim3(p->nv), dim3(64 * NS), p->lds_bytes, st,    \
-                           (const float *)vsites, (const float *)src, (float *)dst, p->d_vin, \
-                           p->d_vout, p->d_vnet, ctl, res_fid, res_abs);                     \
+    const int key = p->d * 2 + (p->bf16 ? 1 : 0);
+    switch (key) {
+#define VX_CASE(NS, BF)                                                                      \
+    case NS * 2 + BF:                                                                        \
+        hipLaunchKernelGGL((k_vertex_d8<NS, BF>), dim3(p->nv), dim3(64 * NS), p->lds_bytes,  \
+                           st, vsites, (const float *)src, (float *)dst, p->d_vin, p->d_vout, \
+                           p->d_vnet, ctl, res_fid, res_abs);                                \
         break;
-        VX_CASE(1)
-        VX_CASE(2)
-        VX_CASE(3)
-        VX_CASE(4)
+        VX_CASE(1, 0)
+        VX_CASE(2, 0)
+        VX_CASE(3, 0)
+        VX_CASE(4, 0)
+        VX_CASE(1, 1)
+        VX_CASE(2, 1)
+        VX_CASE(3, 1)
+        VX_CASE(4, 1)
 #undef VX_CASE
         default:
             return 4;
@@ -359,9 +489,12 @@ static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *v
 }
 
 static int vertex_set_lds_limits(const VertexPlan *p) {
-    const void *fns[4] = {(const void *)k_vertex_d8<1>, (const void *)k_vertex_d8<2>,
-                          (const void *)k_vertex_d8<3>, (const void *)k_vertex_d8<4>};
+    const void *fns[8] = {(const void *)k_vertex_d8<1, false>, (const void *)k_vertex_d8<2, false>,
+                          (const void *)k_vertex_d8<3, false>, (const void *)k_vertex_d8<4, false>,
+                          (const void *)k_vertex_d8<1, true>, (const void *)k_vertex_d8<2, true>,
+                          (const void *)k_vertex_d8<3, true>, (const void *)k_vertex_d8<4, true>};
     if (p->d < 1 || p->d > 4) return 4;
-    return hipFuncSetAttribute(fns[p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
+    return hipFuncSetAttribute(fns[(p->bf16 ? 4 : 0) + p->d - 1],
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)p->lds_bytes) == hipSuccess ? 0 : 2;
 }

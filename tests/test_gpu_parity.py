@@ -260,3 +260,62 @@ def test_vertex_batch_with_masks(pkg, orc):
         assert abs(int(r.sweeps[i]) - sw) <= 2
         assert _rel(mb[off:off + n], want) < 1e-5
         off += n
+
+
+# ---------------------------------------------------------------- BF16 + FP64 polish
+
+def test_bf16_single_sweep_within_bound(pkg, orc):
+    """BF16 operands / FP32 accumulation: one sweep within the error bound with the BF16
+    unit roundoff (gamma = 24 u, u = 2^-8, covering the ~10 roundings along a product
+    chain), from well-conditioned (identity) messages."""
+    _say("case bf16 single sweep")
+    t = pkg.build_topology(pkg.named_grid((6, 5)), 8, 2)
+    s = pkg.peps.random_sites(t, 19, scale=pkg.peps.peps_scale(2, 8))
+    m = pkg.peps.identity_messages(t)
+    with pkg.Engine(t, s, dtype="bf16") as e:
+        assert e.path() == "vertex"
+        e.set_messages(m)
+        e.sweep()
+        got = e.messages()
+    assert _fp32_sweep_check(pkg, orc, t, s, m, got, gamma=24 * 2.0 ** -8) < 1.0
+
+
+def test_bf16_converged_then_fp64_polish(pkg, orc):
+    """Mixed-precision workload at D = 8: converge in BF16, report the error against the
+    FP64 oracle, then 3 FP64 polishing sweeps shrink it (stated bounds: BF16 messages
+    within 5e-2 relative, log-norm within 1e-2 relative; after polishing, messages at
+    least 4x closer and log-norm within 1e-4 relative)."""
+    _say("case bf16 converged + polish")
+    t = _grid(pkg, 12, 8)
+    s = pkg.peps.random_sites(t, 23, scale=pkg.peps.peps_scale(2, 8))
+    m0 = pkg.peps.identity_messages(t)
+    want, _, _, at = orc.bp(t, s, m0, maxiter=500, tol=1e-14, criterion="maxabs", nthreads=8)
+    assert at > 0
+    lw = orc.bethe_free_energy(t, s, want)
+    with pkg.Engine(t, s, dtype="bf16") as e:
+        e.set_messages(m0)
+        r = e.run(maxiter=200, tol=1e-5)
+        m_bf = e.messages()
+        ln_bf = e.lognorm()
+        e.polish_fp64(3)
+        m_pol = e.messages()
+        ln_pol = e.lognorm()
+    err_bf = _rel(m_bf, want)
+    err_pol = _rel(m_pol, want)
+    _say(f"bf16 sweeps={r.sweeps[0]} msg rel err bf16={err_bf:.3e} polished={err_pol:.3e} "
+         f"lognorm rel err bf16={abs(ln_bf - lw) / abs(lw):.3e} "
+         f"polished={abs(ln_pol - lw) / abs(lw):.3e}")
+    assert err_bf < 5e-2
+    assert abs(ln_bf - lw) / abs(lw) < 1e-2
+    assert err_pol < err_bf / 4
+    assert abs(ln_pol - lw) / abs(lw) < 1e-4
+
+
+def test_bf16_refuses_unsupported_shapes(pkg):
+    _say("case bf16 refusal")
+    t = _grid(pkg, 4, 3)
+    s = pkg.peps.random_sites(t, 0)
+    with pytest.raises(pkg._lib.BPError):
+        with pkg.Engine(t, s, dtype="bf16") as e:
+            e.init_messages("identity")
+            e.sweep()
