@@ -68,7 +68,10 @@ enum { BP_CRIT_FIDELITY = 0, /* reference iterate_diff: 1 - |<m1^, m2^>|^2 */
        BP_CRIT_MAXABS = 1 };  /* max |m_new - m_old| */
 
 /* kernel path selection */
-enum { BP_PATH_AUTO = 0, BP_PATH_GENERIC = 1, BP_PATH_VERTEX = 2 };
+enum { BP_PATH_AUTO = 0, BP_PATH_GENERIC = 1, BP_PATH_VERTEX = 2,
+       BP_PATH_MIXED = 3 /* reported by bp_get_path only: the vertex kernel for the networks
+                            it covers (bond dimension 8, degree <= 4), the generic kernel for
+                            the rest, in the same sweep */ };
 
 typedef struct bp_context bp_context;
 

@@ -335,14 +335,16 @@ static int ensure_ws(bp_context *c, const GenLaunch &g, size_t esz) {
     return BP_OK;
 }
 
+// Work items [q0, q1): message slots, or entries of `list` (a bucket's slots) when given.
 template <typename T>
 static int launch_update_generic(bp_context *c, const T *sites, const T *mold, T *mnew,
-                                 const NetCtl &ctl, int q0, int q1) {
+                                 const NetCtl &ctl, int q0, int q1, const int *list = nullptr) {
+    if (q1 <= q0) return BP_OK;
     GenLaunch g = gen_launch(c, q1 - q0, sizeof(T));
     RETCHK(ensure_ws(c, g, sizeof(T)));
     hipLaunchKernelGGL(k_update_generic<T>, dim3(g.grid), dim3(BP_BLOCK), g.lds_bytes, c->stream,
                        c->topo(), sites, mold, mnew, ctl, c->d_res_fid, c->d_res_abs, q0, q1,
-                       g.mats_elems, g.out_elems, g.red_elems, g.buf_elems, g.use_lds,
+                       list, g.mats_elems, g.out_elems, g.red_elems, g.buf_elems, g.use_lds,
                        (T *)c->d_ws, g.ws_stride);
     HIPCHK(hipGetLastError());
     return BP_OK;
@@ -371,20 +373,25 @@ static int launch_edge_scalars(bp_context *c, const T *msgs) {
 
 // ------------------------------------------------------------------ path selection
 
+// Resolved dispatch: every network on the generic kernel, every network on the vertex
+// kernel, or the vertex bucket plus the generic kernel for the remaining networks.
+
+
 static int resolve_path(bp_context *c) {
     // BF16 contractions exist only on the vertex path: no silent FP32 substitute
-    if (c->dtype == BP_BF16) return c->vplan_ok && c->path != BP_PATH_GENERIC ? BP_PATH_VERTEX : -1;
-    if (c->path == BP_PATH_GENERIC) return BP_PATH_GENERIC;
-    if (c->path == BP_PATH_VERTEX) return c->vplan_ok ? BP_PATH_VERTEX : -1;
-    // AUTO: the vertex-centric MFMA path whenever the plan supports this network
-    return c->vplan_ok ? BP_PATH_VERTEX : BP_PATH_GENERIC;
+    if (c->dtype == BP_BF16)
+        return c->vplan_ok && c->vplan.all && c->path != BP_PATH_GENERIC ? BP_PATH_VERTEX : -1;
+    if (c->path == BP_PATH_GENERIC || !c->vplan_ok) return c->path == BP_PATH_VERTEX ? -1 : BP_PATH_GENERIC;
+    if (c->vplan.all) return BP_PATH_VERTEX;
+    // AUTO: vertex kernel for its bucket, generic kernel for the rest
+    return c->path == BP_PATH_VERTEX ? -1 : BP_PATH_MIXED;
 }
 
 // Allocate whatever workspace the next sweeps need, before any graph capture (no
 // allocation may happen inside a captured region).
 static int prepare_sweep(bp_context *c) {
     const int path = resolve_path(c);
-    if (path == BP_PATH_GENERIC || path < 0) {
+    if (path != BP_PATH_VERTEX) {
         const size_t es = c->dtype == BP_F64 ? sizeof(double) : sizeof(float);
         GenLaunch g = gen_launch(c, c->nports, es);
         RETCHK(ensure_ws(c, g, es));
@@ -415,10 +422,14 @@ static int sweep_once(bp_context *c, const NetCtl &done) {
     if (path < 0) return set_err(BP_ERR_UNSUPPORTED, "vertex path not available: %s",
                                  c->vplan.why.c_str());
     void *src = c->d_msg[c->cur], *dst = c->d_msg[c->cur ^ 1];
-    if (path == BP_PATH_VERTEX) {
+    if (path == BP_PATH_VERTEX || path == BP_PATH_MIXED) {
         RETCHK(vertex_sweep(&c->vplan, c->stream, c->topo(), c->d_vsites, src, dst, done,
                             c->d_res_fid, c->d_res_abs));
         HIPCHK(hipGetLastError());
+        if (path == BP_PATH_MIXED)  // FP32 only (BF16 never mixes)
+            RETCHK(launch_update_generic<float>(c, (const float *)c->d_sites, (const float *)src,
+                                                (float *)dst, done, 0, c->vplan.ngen,
+                                                c->vplan.d_gen_slots));
     } else if (c->dtype == BP_F64) {
         RETCHK(launch_update_generic<double>(c, (const double *)c->d_sites, (const double *)src,
                                              (double *)dst, done, 0, c->nports));
@@ -710,8 +721,8 @@ int bp_set_path(bp_context *c, int path) {
     if (!c) return set_err(BP_ERR_ARG, "ctx is NULL");
     if (path != BP_PATH_AUTO && path != BP_PATH_GENERIC && path != BP_PATH_VERTEX)
         return set_err(BP_ERR_ARG, "unknown path %d", path);
-    if (path == BP_PATH_VERTEX && !c->vplan_ok)
-        return set_err(BP_ERR_UNSUPPORTED, "vertex path not available: %s",
+    if (path == BP_PATH_VERTEX && !(c->vplan_ok && c->vplan.all))
+        return set_err(BP_ERR_UNSUPPORTED, "vertex path not available for every network: %s",
                        c->vplan.why.c_str());
     if (path == BP_PATH_GENERIC && c->dtype == BP_BF16)
         return set_err(BP_ERR_UNSUPPORTED, "BF16 contractions run on the vertex path only");

@@ -155,17 +155,20 @@ __device__ void load_incoming(const DevTopo &tp, int u, const VertexShape &vs, c
     }
 }
 
-// One directed message per workgroup iteration: update, normalise, residual.
+// One directed message per workgroup iteration: update, normalise, residual.  Work items
+// i in [item_begin, item_end) are message slots, or slot_list[i] when a list is given (the
+// slots of one dispatch bucket).
 template <typename T>
 __global__ __launch_bounds__(BP_BLOCK) void k_update_generic(
     DevTopo tp, const T *__restrict__ sites, const T *mold, T *mnew,
     NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs,
-    int slot_begin, int slot_end, int mats_elems, int out_elems, int red_elems,
-    long long buf_elems, int use_lds, T *ws, long long ws_stride) {
+    int item_begin, int item_end, const int *__restrict__ slot_list, int mats_elems,
+    int out_elems, int red_elems, long long buf_elems, int use_lds, T *ws, long long ws_stride) {
     extern __shared__ __align__(16) char smem[];
     GenSmem<T> sm = carve<T>(smem, mats_elems, out_elems, red_elems, buf_elems, use_lds, ws,
                              ws_stride);
-    for (int q = slot_begin + blockIdx.x; q < slot_end; q += gridDim.x) {
+    for (int it = item_begin + blockIdx.x; it < item_end; it += gridDim.x) {
+        const int q = slot_list ? slot_list[it] : it;
         const int u = tp.owner[q];
         const int base = tp.port_off[u];
         const int p = q - base;

@@ -37,14 +37,22 @@
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef short v4s __attribute__((ext_vector_type(4)));
 
+// The plan covers a dispatch bucket: every network whose vertices all have degree <= 4,
+// the same physical dimension d (<= 4) and bond dimension 8 on every edge.  Work item b
+// (one workgroup) is the b-th vertex of the bucket; the remaining networks' message slots
+// go to the generic kernel (gen_slots).
 struct VertexPlan {
     std::string why;
     int d = 0;                  // physical dimension (waves per workgroup)
-    int nv = 0;
+    int nv = 0;                 // vertices in the bucket
     int bf16 = 0;               // BF16 operands
-    int4 *d_vin = nullptr;      // per vertex: incoming message slot per padded port (-1: dummy)
-    int4 *d_vout = nullptr;     // per vertex: outgoing message slot per padded port (-1: dummy)
-    int *d_vnet = nullptr;      // network of each vertex
+    bool all = false;           // the bucket holds every network
+    int4 *d_vin = nullptr;      // per item: incoming message slot per padded port (-1: dummy)
+    int4 *d_vout = nullptr;     // per item: outgoing message slot per padded port (-1: dummy)
+    int *d_vnet = nullptr;      // network of each item
+    int *d_vlist = nullptr;     // vertex of each item
+    int *d_gen_slots = nullptr; // message slots outside the bucket
+    int ngen = 0;
     size_t lds_bytes = 0;
 };
 
@@ -56,8 +64,19 @@ static void vertex_plan_release(VertexPlan *p) {
     if (p->d_vin) (void)hipFree(p->d_vin);
     if (p->d_vout) (void)hipFree(p->d_vout);
     if (p->d_vnet) (void)hipFree(p->d_vnet);
+    if (p->d_vlist) (void)hipFree(p->d_vlist);
+    if (p->d_gen_slots) (void)hipFree(p->d_gen_slots);
     p->d_vin = p->d_vout = nullptr;
-    p->d_vnet = nullptr;
+    p->d_vnet = p->d_vlist = p->d_gen_slots = nullptr;
+    p->nv = p->ngen = 0;
+    p->all = false;
+}
+
+template <typename T>
+static int vx_upload(T **dst, const std::vector<T> &v) {
+    if (v.empty()) return 0;
+    if (hipMalloc((void **)dst, sizeof(T) * v.size()) != hipSuccess) return 5;
+    return hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice) == hipSuccess ? 0 : 2;
 }
 
 static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const int *port_off,
@@ -69,42 +88,60 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     p->why.clear();
     if (plain) { p->why = "plain (vector-message) networks use the generic path"; return 4; }
     if (dtype != 1 && dtype != 2) { p->why = "the vertex path computes in FP32 or BF16"; return 4; }
-    const int d = phys_dim[0];
-    for (int u = 0; u < nv; ++u) {
-        const int k = port_off[u + 1] - port_off[u];
-        if (k > 4) { p->why = "degree > 4"; return 4; }
-        if (phys_dim[u] != d) { p->why = "physical dimension differs between vertices"; return 4; }
-        for (int q = port_off[u]; q < port_off[u + 1]; ++q)
-            if (edge_dim[port_edge[q]] != VX_D) { p->why = "bond dimension is not 8"; return 4; }
+    // qualify whole networks (vertices of a network are contiguous)
+    int d = -1;
+    std::vector<int> vlist, gen_slots;
+    std::string last_reason = "no network has bond dimension 8 on every edge";
+    for (int u0 = 0; u0 < nv;) {
+        int u1 = u0;
+        while (u1 < nv && vnet[u1] == vnet[u0]) ++u1;
+        bool ok = true;
+        for (int u = u0; u < u1 && ok; ++u) {
+            const int k = port_off[u + 1] - port_off[u];
+            if (k > 4) { ok = false; last_reason = "degree > 4"; }
+            if (phys_dim[u] < 1 || phys_dim[u] > 4) { ok = false; last_reason = "physical dimension outside 1..4"; }
+            if (d >= 0 && phys_dim[u] != d) { ok = false; last_reason = "physical dimension differs between networks"; }
+            for (int q = port_off[u]; q < port_off[u + 1] && ok; ++q)
+                if (edge_dim[port_edge[q]] != VX_D) { ok = false; last_reason = "bond dimension is not 8"; }
+        }
+        if (ok && d < 0) d = phys_dim[u0];
+        for (int u = u0; u < u1; ++u) {
+            if (ok) vlist.push_back(u);
+            else for (int q = port_off[u]; q < port_off[u + 1]; ++q) gen_slots.push_back(q);
+        }
+        u0 = u1;
     }
-    if (d < 1 || d > 4) { p->why = "physical dimension outside 1..4"; return 4; }
-    std::vector<int4> vin(nv), vout(nv);
-    for (int u = 0; u < nv; ++u) {
+    if (vlist.empty()) { p->why = last_reason; return 4; }
+    const int nx = (int)vlist.size();
+    std::vector<int4> vin(nx), vout(nx);
+    std::vector<int> vn(nx);
+    for (int b = 0; b < nx; ++b) {
+        const int u = vlist[b];
         int in[4] = {-1, -1, -1, -1}, out[4] = {-1, -1, -1, -1};
         for (int j = 0; j < port_off[u + 1] - port_off[u]; ++j) {
             out[j] = port_off[u] + j;
             in[j] = port_rev[port_off[u] + j];
         }
-        vin[u] = make_int4(in[0], in[1], in[2], in[3]);
-        vout[u] = make_int4(out[0], out[1], out[2], out[3]);
+        vin[b] = make_int4(in[0], in[1], in[2], in[3]);
+        vout[b] = make_int4(out[0], out[1], out[2], out[3]);
+        vn[b] = vnet[u];
     }
-    if (hipMalloc((void **)&p->d_vin, sizeof(int4) * nv) != hipSuccess ||
-        hipMalloc((void **)&p->d_vout, sizeof(int4) * nv) != hipSuccess ||
-        hipMalloc((void **)&p->d_vnet, sizeof(int) * nv) != hipSuccess) {
+    int rc = vx_upload(&p->d_vin, vin);
+    if (!rc) rc = vx_upload(&p->d_vout, vout);
+    if (!rc) rc = vx_upload(&p->d_vnet, vn);
+    if (!rc) rc = vx_upload(&p->d_vlist, vlist);
+    if (!rc) rc = vx_upload(&p->d_gen_slots, gen_slots);
+    if (rc) {
         p->why = "allocation failed";
         vertex_plan_release(p);
-        return 5;
-    }
-    if (hipMemcpy(p->d_vin, vin.data(), sizeof(int4) * nv, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->d_vout, vout.data(), sizeof(int4) * nv, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->d_vnet, vnet, sizeof(int) * nv, hipMemcpyHostToDevice) != hipSuccess) {
-        p->why = "copy failed";
-        vertex_plan_release(p);
-        return 2;
+        return rc;
     }
     p->d = d;
-    p->nv = nv;
+    p->nv = nx;
+    p->ngen = (int)gen_slots.size();
+    p->all = gen_slots.empty();
     p->bf16 = dtype == 2;
+    if (!p->all) p->why = last_reason;  // why the rest is not covered
     // per wave: two transposable buffers; per workgroup: 4 messages + d x 4 x 64 partials
     p->lds_bytes = sizeof(float) * ((size_t)d * 2 * VX_BUF + 4 * 64 + (size_t)d * 4 * 64);
     return 0;
@@ -144,9 +181,10 @@ __device__ __forceinline__ void bf16_round2(float &a, float &b) {
 
 // Element e of the padded slice of u (layout X order, e = r * 64 + l).
 __device__ __forceinline__ float vx_pad_value(const double *sites64, const long long *site_off,
-                                              const int *port_off, int d, long long us, int e) {
+                                              const int *port_off, const int *vlist, int d,
+                                              long long us, int e) {
     const int s = (int)(us % d);
-    const int u = (int)(us / d);
+    const int u = vlist[us / d];
     const int k = port_off[u + 1] - port_off[u];
     const int idx[4] = {(e >> 9) & 7, (e >> 6) & 7, (e >> 3) & 7, e & 7};
     long long off = 0;
@@ -161,13 +199,14 @@ __device__ __forceinline__ float vx_pad_value(const double *sites64, const long 
 
 // FP32: vsites[u][s][r * 64 + l].  BF16: vsites[u][s][(r / 2) * 128 + 2 l + (r % 2)].
 __global__ void k_vertex_pad_sites(const double *sites64, const long long *site_off,
-                                   const int *port_off, int d, int nv, int bf16, void *vsites) {
+                                   const int *port_off, const int *vlist, int d, int nv, int bf16,
+                                   void *vsites) {
     const long long total = (long long)nv * d * 4096;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
          i += (long long)gridDim.x * blockDim.x) {
         const int e = (int)(i & 4095);
         const long long us = i >> 12;
-        const float v = vx_pad_value(sites64, site_off, port_off, d, us, e);
+        const float v = vx_pad_value(sites64, site_off, port_off, vlist, d, us, e);
         if (bf16) {
             const int r = e >> 6, l = e & 63;
             ((unsigned short *)vsites)[(us << 12) + (r >> 1) * 128 + 2 * l + (r & 1)] = bf16_rne(v);
@@ -185,7 +224,7 @@ static int vertex_upload_sites(VertexPlan *p, hipStream_t st, const double *d_si
     *d_vsites = nullptr;
     if (hipMalloc(d_vsites, bytes) != hipSuccess) return 5;
     hipLaunchKernelGGL(k_vertex_pad_sites, dim3(2048), dim3(256), 0, st, d_sites64, d_site_off,
-                       d_port_off, p->d, p->nv, p->bf16, *d_vsites);
+                       d_port_off, p->d_vlist, p->d, p->nv, p->bf16, *d_vsites);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
@@ -351,9 +390,10 @@ __global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mold,
                  float *__restrict__ mnew, const int4 *__restrict__ vin,
                  const int4 *__restrict__ vout, const int *__restrict__ vnet,
-                 NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs) {
+                 const long long *__restrict__ msg_off, NetCtl ctl,
+                 double *__restrict__ res_fid, double *__restrict__ res_abs) {
     extern __shared__ __align__(16) float vsm[];
-    const int u = blockIdx.x;
+    const int u = blockIdx.x;  // work item: the u-th vertex of the bucket
     const int w = threadIdx.x >> 6;     // wave = physical index s
     const int lane = threadIdx.x & 63;
     float *bufA = vsm + w * 2 * VX_BUF;
@@ -363,12 +403,19 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
     const int4 vi = vin[u], vo = vout[u];
     const int ins[4] = {vi.x, vi.y, vi.z, vi.w};
     const int outs[4] = {vo.x, vo.y, vo.z, vo.w};
+    // message offsets (other networks of a batch may have other bond dimensions)
+    long long ino[4], outo[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        ino[j] = ins[j] >= 0 ? msg_off[ins[j]] : 0;
+        outo[j] = outs[j] >= 0 ? msg_off[outs[j]] : 0;
+    }
 
     const int net = vnet[u];
     if (ctl.enabled && ctl.done[net]) {  // uniform over the workgroup
         for (int j = w; j < 4; j += NS) {
             if (outs[j] < 0) continue;
-            mnew[(long long)outs[j] * 64 + lane] = mold[(long long)outs[j] * 64 + lane];
+            mnew[outo[j] + lane] = mold[outo[j] + lane];
             if (lane == 0) { res_fid[outs[j]] = 0.0; res_abs[outs[j]] = 0.0; }
         }
         return;
@@ -376,7 +423,7 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
     // incoming messages (identity on padded ports); BF16 mode scales each by 1 / max|m|
     for (int i = threadIdx.x; i < 4 * 64; i += 64 * NS) {
         const int j = i >> 6, e = i & 63;
-        msm[i] = ins[j] >= 0 ? mold[(long long)ins[j] * 64 + e] : ((e >> 3) == (e & 7) ? 1.f : 0.f);
+        msm[i] = ins[j] >= 0 ? mold[ino[j] + e] : ((e >> 3) == (e & 7) ? 1.f : 0.f);
     }
     if constexpr (BF) {
         __syncthreads();
@@ -448,8 +495,8 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
         const double S = wave_sum((double)x);
         const double xn = S != 0.0 ? (double)x / S : (double)x;
         const float xf = (float)xn;
-        const double xo = (double)mold[(long long)q * 64 + lane];
-        mnew[(long long)q * 64 + lane] = xf;
+        const double xo = (double)mold[outo[j] + lane];
+        mnew[outo[j] + lane] = xf;
         const double dn = wave_sum((double)xf * xo), nn = wave_sum((double)xf * (double)xf),
                      oo = wave_sum(xo * xo);
         const double mx = wave_max(fabs((double)xf - xo));
@@ -471,7 +518,7 @@ static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *v
     case NS * 2 + BF:                                                                        \
         hipLaunchKernelGGL((k_vertex_d8<NS, BF>), dim3(p->nv), dim3(64 * NS), p->lds_bytes,  \
                            st, vsites, (const float *)src, (float *)dst, p->d_vin, p->d_vout, \
-                           p->d_vnet, ctl, res_fid, res_abs);                                \
+                           p->d_vnet, tp.msg_off, ctl, res_fid, res_abs);                    \
         break;
         VX_CASE(1, 0)
         VX_CASE(2, 0)

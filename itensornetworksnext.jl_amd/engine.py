@@ -119,7 +119,8 @@ class Engine:
     def path(self) -> str:
         p = C.c_int(0)
         L.check(self.lib.bp_get_path(self.h, C.byref(p)), "bp_get_path")
-        return {L.BP_PATH_GENERIC: "generic", L.BP_PATH_VERTEX: "vertex"}[p.value]
+        return {L.BP_PATH_GENERIC: "generic", L.BP_PATH_VERTEX: "vertex",
+                L.BP_PATH_MIXED: "mixed"}[p.value]
 
     # -- BP
     def sweep(self) -> None:

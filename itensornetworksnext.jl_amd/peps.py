@@ -57,6 +57,27 @@ def random_messages(topo: Topology, seed: int = 0, stream: int = 1) -> np.ndarra
     return philox(seed, stream).random(int(topo.msg_off[-1]))
 
 
+def uneven_batch(seed: int = 0, nnet: int = 64, L_range=(8, 48), D_choices=(4, 6, 8, 10, 12),
+                 d: int = 2):
+    """The batched uneven workload: `nnet` independent PEPS norm networks, L uniform on
+    L_range (inclusive), D uniform on D_choices, square or heavy-hex lattice chosen by the
+    seed stream (heavy-hex: max(2, L // 2) rows x L columns of degree-3 sites plus the
+    degree-2 bridge sites), sites iid N(0,1) / sqrt(d D^4).  Returns (topologies, sites)."""
+    from .graphs import build_topology, heavy_hex_lattice, named_grid
+
+    rng = philox(seed, 7)
+    tops, sites = [], []
+    for n in range(nnet):
+        L = int(rng.integers(L_range[0], L_range[1] + 1))
+        D = int(D_choices[int(rng.integers(0, len(D_choices)))])
+        hh = bool(rng.integers(0, 2))
+        g = heavy_hex_lattice(max(2, L // 2), L) if hh else named_grid((L, L))
+        t = build_topology(g, D, d)
+        tops.append(t)
+        sites.append(random_sites(t, seed * 1000003 + n, scale=peps_scale(d, D)))
+    return tops, sites
+
+
 def spin_ice_sites(topo: Topology) -> np.ndarray:
     """Spin-ice vertex tensors (test/test_beliefpropagation.jl:16-36): a degree-4 vertex
     carries t[i,j,k,l] = 1 when i+j+k+l == 2 (two in, two out), else 0."""

@@ -121,7 +121,7 @@ def test_autograd_function(pkg):
     e = _converged_engine(pkg, t, s, "f64", 1e-12)
     E2, g2, _, _ = e.tfim_energy(1.0, 3.0, grad=True, adj_tol=1e-10)
     e.close()
-    assert float(E) == pytest.approx(E2, rel=1e-10)
+    assert float(E.detach()) == pytest.approx(E2, rel=1e-10)
     np.testing.assert_allclose(g, g2, rtol=1e-8, atol=1e-10 * np.max(np.abs(g2)))
     d = np.random.default_rng(1).standard_normal(s.size) * 1e-6
     E3 = ag.bp_tfim_energy(torch.tensor(s + d), t, 1.0, 3.0)
