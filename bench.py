@@ -26,6 +26,12 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# HBM bytes per launch of the dominant kernel, measured with two rocprofv3 --pmc passes
+# (FETCH_SIZE, WRITE_SIZE) over this bench and scaled by a 256 MiB device copy of known
+# traffic (fetch x2.00, write x1.00); the summary and its commands are in
+# profiles/k_vertex_d8_32x32_D8_fp32_hbm_traffic.md.  Keyed by (path, dtype, L, D).
+MEASURED_TRAFFIC = {("vertex", "f32", 32, 8): 37.32e6}
+
 PEAK = {  # MI355X published dense peaks
     "hbm_GBps": 8000.0,
     "fp32_mfma_TFLOPs": 157.3,
@@ -97,6 +103,9 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a kernel of "
+                         "known traffic, used to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -119,6 +128,14 @@ def main(argv=None):
     byts = algorithmic_bytes_per_sweep(topo, elt)
 
     torch.cuda.init()
+    if args.copy_probe_mb > 0:
+        n = args.copy_probe_mb * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     eng = pkg.Engine(topo, sites, dtype=args.dtype, path=args.path)
     eng.init_messages("identity")
     path = eng.path()
@@ -151,7 +168,8 @@ def main(argv=None):
     else:
         achieved = flops / (kms * 1e-3) / 1e12
     roofline = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-                "frac": round(achieved / peak, 4), "traffic": None,
+                "frac": round(achieved / peak, 4),
+                "traffic": MEASURED_TRAFFIC.get((path, args.dtype, L, D)),
                 "kernel_ms": round(kms, 5), "algorithmic_flops_per_launch": flops,
                 "algorithmic_bytes_per_launch": byts}
 
