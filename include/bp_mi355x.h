@@ -158,6 +158,17 @@ int bp_grad_lognorm(bp_context *ctx, double *grad);
 int bp_tfim_energy(bp_context *ctx, double J, double h, double *energy, double *grad,
                    double adj_tol, int adj_maxiter, int *out_adj_iters, double *out_adj_res);
 
+/* bp_grad_lognorm and bp_tfim_energy with the gradient left in device memory: `grad` is a
+ * device buffer of nsite entries in the site layout, of the context's arithmetic type
+ * (double for BP_F64 contexts, float for BP_F32 and BP_BF16), written on the context's
+ * stream before the call returns.  No reference counterpart (the reference has no AD);
+ * this is the pullback an rrule or torch.autograd.Function hands to a device optimiser
+ * without a host round trip. */
+int bp_grad_lognorm_device(bp_context *ctx, void *grad);
+int bp_tfim_energy_device(bp_context *ctx, double J, double h, double *energy, void *grad,
+                          double adj_tol, int adj_maxiter, int *out_adj_iters,
+                          double *out_adj_res);
+
 /* Benchmark helper: launch `n` Jacobi sweeps back to back (no convergence bookkeeping,
  * messages ping-pong) and report the average device duration of one sweep's kernel
  * launches, timed with HIP events on the context's stream around each sweep. */

@@ -51,6 +51,9 @@ SIGNATURES = {
     "bp_grad_lognorm": (C.c_int, [_vp, _dp]),
     "bp_tfim_energy": (C.c_int, [_vp, C.c_double, C.c_double, _dp, _dp, C.c_double, C.c_int,
                                  _ip, _dp]),
+    "bp_grad_lognorm_device": (C.c_int, [_vp, _vp]),
+    "bp_tfim_energy_device": (C.c_int, [_vp, C.c_double, C.c_double, _dp, _vp, C.c_double,
+                                        C.c_int, _ip, _dp]),
     "bp_last_run_ms": (C.c_int, [_vp, _dp]),
     "bp_time_sweeps": (C.c_int, [_vp, C.c_int, _dp]),
 }
@@ -84,3 +87,64 @@ def check(rc: int, where: str) -> None:
     if rc != BP_OK:
         msg = load().bp_last_error().decode(errors="replace")
         raise BPError(rc, where, msg)
+
+
+# ---------------------------------------------------------------- device buffers
+# Device memory for the device-resident entry points comes from the HIP runtime that
+# libbpmi355x.so itself links (ROCm's libamdhip64.so.7).  PyTorch wheels bundle their own
+# HIP runtime; pointers are not exchanged between the two.
+_hip = None
+
+
+def hip_runtime():
+    global _hip
+    if _hip is None:
+        load()  # the engine first, so the same runtime instance is returned below
+        h = C.CDLL("libamdhip64.so.7")
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipMalloc.restype = C.c_int
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipFree.restype = C.c_int
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipMemcpy.restype = C.c_int
+        h.hipSetDevice.argtypes = [C.c_int]
+        h.hipSetDevice.restype = C.c_int
+        _hip = h
+    return _hip
+
+
+class DeviceArray:
+    """A 1-D device buffer of `n` float32 or float64 entries on HIP device `device`."""
+
+    def __init__(self, n: int, dtype, device: int = 0):
+        import numpy as np
+
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.device = int(device)
+        self.nbytes = self.n * self.dtype.itemsize
+        h = hip_runtime()
+        if h.hipSetDevice(self.device) != 0:
+            raise BPError(BP_ERR_HIP, "hipSetDevice", f"device {self.device}")
+        p = C.c_void_p()
+        rc = h.hipMalloc(C.byref(p), max(self.nbytes, 1))
+        if rc != 0:
+            raise BPError(BP_ERR_NOMEM, "hipMalloc", f"{self.nbytes} bytes (hip error {rc})")
+        self.ptr = p.value
+
+    def numpy(self):
+        import numpy as np
+
+        out = np.empty(self.n, dtype=self.dtype)
+        rc = hip_runtime().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2)  # to host
+        if rc != 0:
+            raise BPError(BP_ERR_HIP, "hipMemcpy", f"hip error {rc}")
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            hip_runtime().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()

@@ -375,31 +375,77 @@ __global__ __launch_bounds__(BP_BLOCK) void k_adj_maxdiff(const T *a, const T *b
     if (threadIdx.x == 0) out[0] = s > 0.0 ? m / s : m;
 }
 
-template <typename Tin>
-__global__ void k_adj_to_f64(const Tin *in, double *out, long long n) {
+template <typename Tin, typename Tout>
+__global__ void k_adj_cast(const Tin *in, Tout *out, long long n) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x)
-        out[i] = (double)in[i];
+        out[i] = (Tout)in[i];
+}
+
+// Where a gradient goes: FP64 host memory, or a device buffer of FP64 or FP32 entries.
+struct GradOut {
+    double *host = nullptr;
+    void *dev = nullptr;
+    int dev_f64 = 0;
+    bool any() const { return host != nullptr || dev != nullptr; }
+};
+
+// Writes the site cotangent `da` to its destination (g64: FP64 staging for host output).
+// The caller synchronises the stream.
+template <typename T>
+static int adj_emit_grad(hipStream_t st, const T *da, double *g64, long long nsite,
+                         const GradOut &go) {
+    if (go.dev) {
+        if (go.dev_f64)
+            hipLaunchKernelGGL((k_adj_cast<T, double>), dim3(1024), dim3(256), 0, st, da,
+                               (double *)go.dev, nsite);
+        else
+            hipLaunchKernelGGL((k_adj_cast<T, float>), dim3(1024), dim3(256), 0, st, da,
+                               (float *)go.dev, nsite);
+        return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
+    hipLaunchKernelGGL((k_adj_cast<T, double>), dim3(1024), dim3(256), 0, st, da, g64, nsite);
+    return hipMemcpyAsync(go.host, g64, sizeof(double) * nsite, hipMemcpyDeviceToHost, st) ==
+                   hipSuccess ? 0 : 2;
 }
 
 // ------------------------------------------------------------------ host side
 
+// Device buffers of the adjoint.  alloc() hands out the buffers of the previous call in the
+// same order (growing one when a call needs more), so repeated gradients on one topology
+// allocate nothing after the first; free_temps() only rewinds.  Every call ends with a
+// stream synchronisation, so a rewound buffer is idle when the next call reuses it.
 struct AdjointScratch {
     void *ws = nullptr;
     size_t ws_bytes = 0;
-    std::vector<void *> bufs;
+    std::vector<std::pair<void *, size_t>> pool;
+    size_t next = 0;
     void *alloc(size_t bytes) {
+        bytes = bytes ? (bytes + 255) & ~size_t(255) : 256;
+        if (next < pool.size()) {
+            auto &e = pool[next];
+            if (e.second < bytes) {
+                (void)hipFree(e.first);
+                e.first = nullptr;
+                e.second = 0;
+                if (hipMalloc(&e.first, bytes) != hipSuccess) return nullptr;
+                e.second = bytes;
+            }
+            ++next;
+            return e.first;
+        }
         void *p = nullptr;
-        if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) return nullptr;
-        bufs.push_back(p);
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        pool.push_back({p, bytes});
+        ++next;
         return p;
     }
-    void free_temps() {
-        for (void *p : bufs) (void)hipFree(p);
-        bufs.clear();
-    }
+    void free_temps() { next = 0; }
     void release() {
-        free_temps();
+        for (auto &e : pool)
+            if (e.first) (void)hipFree(e.first);
+        pool.clear();
+        next = 0;
         if (ws) (void)hipFree(ws);
         ws = nullptr;
         ws_bytes = 0;
@@ -454,7 +500,7 @@ static int adj_forward_opens(AdjointScratch *s, hipStream_t st, const DevTopo &t
 template <typename T>
 static int adj_grad_lognorm_t(AdjointScratch *s, hipStream_t st, const DevTopo &tp,
                               long long max_site, int max_mats, int maxD, const T *sites,
-                              const T *msgs, long long nsite, double *grad_host) {
+                              const T *msgs, long long nsite, const GradOut &go) {
     AdjLaunch L;
     if (adj_launch_cfg(s, tp, max_site, max_mats, maxD, sizeof(T), &L)) return 5;
     double *z = (double *)s->alloc(sizeof(double) * tp.nv);
@@ -473,18 +519,17 @@ static int adj_grad_lognorm_t(AdjointScratch *s, hipStream_t st, const DevTopo &
     hipLaunchKernelGGL(k_adj_sitevjp<T>, dim3(std::min(L.grid, tp.nv)), dim3(BP_BLOCK), L.lds, st,
                        tp, sites, msgs, (const T *)nullptr, 0, (const double *)nullptr,
                        (const double *)cv, (int)OP_I, abar, ADJ_ARGS(L, s));
-    hipLaunchKernelGGL(k_adj_to_f64<T>, dim3(1024), dim3(256), 0, st, abar, g64, nsite);
-    (void)hipMemcpyAsync(grad_host, g64, sizeof(double) * nsite, hipMemcpyDeviceToHost, st);
+    const int rc = adj_emit_grad(st, (const T *)abar, g64, nsite, go);
     const hipError_t e = hipStreamSynchronize(st);
     s->free_temps();
-    return e == hipSuccess ? 0 : 2;
+    return rc ? rc : (e == hipSuccess ? 0 : 2);
 }
 
 template <typename T>
 static int adj_tfim_t(AdjointScratch *s, hipStream_t st, const DevTopo &tp, int ne,
                       long long max_site, int max_mats, int maxD, const T *sites, const T *msgs,
                       long long nsite, long long nmsg, double J, double h, double *energy,
-                      double *grad_host, double adj_tol, int adj_maxiter, int *out_iters,
+                      const GradOut &go, double adj_tol, int adj_maxiter, int *out_iters,
                       double *out_res) {
     AdjLaunch L;
     if (adj_launch_cfg(s, tp, max_site, max_mats, maxD, sizeof(T), &L)) return 5;
@@ -531,7 +576,7 @@ static int adj_tfim_t(AdjointScratch *s, hipStream_t st, const DevTopo &tp, int 
         s->free_temps();
         return 2;
     }
-    if (!grad_host) {
+    if (!go.any()) {
         s->free_temps();
         return 0;
     }
@@ -617,37 +662,36 @@ static int adj_tfim_t(AdjointScratch *s, hipStream_t st, const DevTopo &tp, int 
                            (const T *)gg, 0, (const double *)ones, nullc, (int)OP_I, da,
                            ADJ_ARGS(L, s));
     }
-    hipLaunchKernelGGL(k_adj_to_f64<T>, dim3(1024), dim3(256), 0, st, (const T *)da, g64, nsite);
-    (void)hipMemcpyAsync(grad_host, g64, sizeof(double) * nsite, hipMemcpyDeviceToHost, st);
+    const int rc = adj_emit_grad(st, (const T *)da, g64, nsite, go);
     const hipError_t e = hipStreamSynchronize(st);
     if (out_iters) *out_iters = it;
     if (out_res) *out_res = r;
     s->free_temps();
-    return e == hipSuccess ? 0 : 2;
+    return rc ? rc : (e == hipSuccess ? 0 : 2);
 }
 
 #undef ADJ_ARGS
 
 static int adjoint_grad_lognorm(AdjointScratch *s, hipStream_t st, DevTopo tp, long long max_site,
                                 int max_mats, int maxD, int dtype, const void *sites,
-                                const void *msgs, long long nsite, double *grad) {
+                                const void *msgs, long long nsite, const GradOut &go) {
     if (dtype == 0)
         return adj_grad_lognorm_t<double>(s, st, tp, max_site, max_mats, maxD,
-                                          (const double *)sites, (const double *)msgs, nsite, grad);
+                                          (const double *)sites, (const double *)msgs, nsite, go);
     return adj_grad_lognorm_t<float>(s, st, tp, max_site, max_mats, maxD, (const float *)sites,
-                                     (const float *)msgs, nsite, grad);
+                                     (const float *)msgs, nsite, go);
 }
 
 static int adjoint_tfim(AdjointScratch *s, hipStream_t st, DevTopo tp, int ne, long long max_site,
                         int max_mats, int maxD, int dtype, const void *sites, const void *msgs,
                         long long nsite, long long nmsg, double J, double h, double *energy,
-                        double *grad, double adj_tol, int adj_maxiter, int *out_iters,
+                        const GradOut &go, double adj_tol, int adj_maxiter, int *out_iters,
                         double *out_res) {
     if (dtype == 0)
         return adj_tfim_t<double>(s, st, tp, ne, max_site, max_mats, maxD, (const double *)sites,
-                                  (const double *)msgs, nsite, nmsg, J, h, energy, grad, adj_tol,
+                                  (const double *)msgs, nsite, nmsg, J, h, energy, go, adj_tol,
                                   adj_maxiter, out_iters, out_res);
     return adj_tfim_t<float>(s, st, tp, ne, max_site, max_mats, maxD, (const float *)sites,
-                             (const float *)msgs, nsite, nmsg, J, h, energy, grad, adj_tol,
+                             (const float *)msgs, nsite, nmsg, J, h, energy, go, adj_tol,
                              adj_maxiter, out_iters, out_res);
 }

@@ -20,6 +20,7 @@
 #include "generic.h"
 #include "vertex.h"
 #include "adjoint.h"
+#include "vertex_adjoint.h"
 
 // ------------------------------------------------------------------ error state
 
@@ -892,6 +893,13 @@ static const void *current_msgs(bp_context *c, bool *is64) {
     return c->d_msg[c->cur];
 }
 
+// The adjoint runs on the vertex kernels when the FP32 bucket holds every network (the
+// generic kernels otherwise, and always for FP64 messages or a forced generic path).
+static bool vertex_adjoint_ok(const bp_context *c, bool is64) {
+    return !is64 && c->dtype == BP_F32 && c->vplan_ok && c->vplan.all && !c->vplan.bf16 &&
+           c->path != BP_PATH_GENERIC && c->d_vsites != nullptr;
+}
+
 // per-slot distances of the current messages against `other` (device buffer)
 static int diff_launch(bp_context *c, const void *other, bool other64) {
     bool is64 = false;
@@ -1098,23 +1106,27 @@ int bp_polish_fp64(bp_context *c, int nsweeps, double *out_delta) {
     return BP_OK;
 }
 
-int bp_grad_lognorm(bp_context *c, double *grad) {
+static int grad_lognorm_impl(bp_context *c, const GradOut &go) {
     RETCHK(ready(c));
-    if (!grad) return set_err(BP_ERR_ARG, "NULL argument");
     if (c->plain) return set_err(BP_ERR_UNSUPPORTED, "gradient needs a norm network");
     HIPCHK(hipSetDevice(c->device));
     bool is64 = false;
     const void *m = current_msgs(c, &is64);
+    if (vertex_adjoint_ok(c, is64)) {
+        const VdCtx vc{&c->vplan, c->stream, c->topo(), (const float *)c->d_vsites};
+        const int rv = vadj_grad_lognorm(&c->adj, vc, (const float *)m, c->nsite, go);
+        return rv == BP_OK ? BP_OK : set_err(rv, "log-norm gradient failed (status %d)", rv);
+    }
     const int rc = adjoint_grad_lognorm(&c->adj, c->stream, c->topo(), c->max_site, c->max_mats,
                                         c->maxD, is64 ? BP_F64 : c->dtype,
                                         is64 ? (const void *)c->d_sites64
                                              : (const void *)c->d_sites,
-                                        m, c->nsite, grad);
+                                        m, c->nsite, go);
     return rc == BP_OK ? BP_OK : set_err(rc, "log-norm gradient failed (status %d)", rc);
 }
 
-int bp_tfim_energy(bp_context *c, double J, double h, double *energy, double *grad,
-                   double adj_tol, int adj_maxiter, int *out_iters, double *out_res) {
+static int tfim_impl(bp_context *c, double J, double h, double *energy, const GradOut &go,
+                     double adj_tol, int adj_maxiter, int *out_iters, double *out_res) {
     RETCHK(ready(c));
     if (!energy) return set_err(BP_ERR_ARG, "NULL argument");
     if (c->plain) return set_err(BP_ERR_UNSUPPORTED, "energy needs a norm network");
@@ -1123,12 +1135,51 @@ int bp_tfim_energy(bp_context *c, double J, double h, double *energy, double *gr
     HIPCHK(hipSetDevice(c->device));
     bool is64 = false;
     const void *m = current_msgs(c, &is64);
+    if (vertex_adjoint_ok(c, is64)) {
+        const VdCtx vc{&c->vplan, c->stream, c->topo(), (const float *)c->d_vsites};
+        const int rv = vadj_tfim(&c->adj, vc, c->ne, (const float *)m, c->nsite, c->nmsg, J, h,
+                                 energy, go, adj_tol, adj_maxiter, out_iters, out_res);
+        return rv == BP_OK ? BP_OK : set_err(rv, "TFIM energy/gradient failed (status %d)", rv);
+    }
     const int rc = adjoint_tfim(&c->adj, c->stream, c->topo(), c->ne, c->max_site, c->max_mats,
                                 c->maxD, is64 ? BP_F64 : c->dtype,
                                 is64 ? (const void *)c->d_sites64 : (const void *)c->d_sites, m,
-                                c->nsite, c->nmsg, J, h, energy, grad, adj_tol, adj_maxiter,
+                                c->nsite, c->nmsg, J, h, energy, go, adj_tol, adj_maxiter,
                                 out_iters, out_res);
     return rc == BP_OK ? BP_OK : set_err(rc, "TFIM energy/gradient failed (status %d)", rc);
+}
+
+static GradOut device_grad(const bp_context *c, void *grad) {
+    GradOut go;
+    go.dev = grad;
+    go.dev_f64 = c->dtype == BP_F64;
+    return go;
+}
+
+int bp_grad_lognorm(bp_context *c, double *grad) {
+    if (!grad) return set_err(BP_ERR_ARG, "NULL argument");
+    GradOut go;
+    go.host = grad;
+    return grad_lognorm_impl(c, go);
+}
+
+int bp_grad_lognorm_device(bp_context *c, void *grad) {
+    if (!c || !grad) return set_err(BP_ERR_ARG, "NULL argument");
+    return grad_lognorm_impl(c, device_grad(c, grad));
+}
+
+int bp_tfim_energy(bp_context *c, double J, double h, double *energy, double *grad,
+                   double adj_tol, int adj_maxiter, int *out_iters, double *out_res) {
+    GradOut go;
+    go.host = grad;
+    return tfim_impl(c, J, h, energy, go, adj_tol, adj_maxiter, out_iters, out_res);
+}
+
+int bp_tfim_energy_device(bp_context *c, double J, double h, double *energy, void *grad,
+                          double adj_tol, int adj_maxiter, int *out_iters, double *out_res) {
+    if (!c || !grad) return set_err(BP_ERR_ARG, "NULL argument");
+    return tfim_impl(c, J, h, energy, device_grad(c, grad), adj_tol, adj_maxiter, out_iters,
+                     out_res);
 }
 
 }  // extern "C"

@@ -1,0 +1,643 @@
+// vertex_adjoint.h -- fixed-point adjoint and BP energy on the bond-dimension-8 vertex path
+// (FP32 MFMA), for buckets that hold every network.
+//
+// Every quantity the adjoint needs is one of two per-vertex contractions of the slice
+// layout of vertex.h, with the ket tensor op.A (op = I, X or Z on the physical index: a
+// signed permutation of the slices, so wave s reads ket slice pi(s) with sign sigma(s)):
+//
+//   message form, for every port j of u:
+//     R_j = cv * open(op.A, A, m, j) + sum_{t != j} open(op.A, A, {t <- W_t}, j)
+//   site form, for every slice s of u:
+//     R[s] = sigma(s) * ( cv * (A[pi(s)] x_all M) + sum_t A[pi(s)] x_all {M, t <- W_t} )
+//     with M the incoming messages (or their transposes, for the ket-side derivative).
+//
+// The sums over t are directional derivatives of multilinear maps and run as dual-number
+// absorptions: value V_{i+1} = V_i x_i m_i and tangent T_{i+1} = T_i x_i m_i + V_i x_i W_i,
+// sharing the partial products E01 / E23 exactly as the forward sweep kernel does.  The
+// value term folds into the tangent before the open contraction (open is linear), so a
+// message-form launch costs 18 mode products per slice against the forward's 8.
+//
+// Message form with cv = 1 and no W gives the unnormalised forward messages G (and with
+// op = X or Z the operator-inserted ones), their sums S and the vertex scalars
+// Z_u = <G_{u, port 0}, m_{in, port 0}>.  With W = the normalisation cotangent of lambda it
+// is one adjoint sweep lambda' = dE/dm + J_m^T lambda (every in-slot has exactly one
+// consumer vertex, so results are written, not accumulated).  The site form gives the
+// bra-side (M) and ket-side (M^T) derivatives of <op.A, A>_M.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "adjoint.h"
+#include "vertex.h"
+
+// Mode products that finish both output halves of a row/column before writing, so the
+// output may be the same array as the first input.
+template <bool HI>
+__device__ __forceinline__ void vd_mp1(const float (&T)[64], const float (&M)[16], float (&O)[64]) {
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        v4f acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const float b = HI ? T[a * 8 + x] : T[x * 8 + a];
+            acc0 = __builtin_amdgcn_mfma_f32_4x4x1f32(M[a * 2 + 0], b, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_4x4x1f32(M[a * 2 + 1], b, acc1, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (HI) { O[i * 8 + x] = acc0[i]; O[(4 + i) * 8 + x] = acc1[i]; }
+            else { O[x * 8 + i] = acc0[i]; O[x * 8 + 4 + i] = acc1[i]; }
+        }
+    }
+}
+
+// O = T1 x M1 + T2 x M2 on the same register port (O may be T1).
+template <bool HI>
+__device__ __forceinline__ void vd_mp2(const float (&T1)[64], const float (&M1)[16],
+                                       const float (&T2)[64], const float (&M2)[16],
+                                       float (&O)[64]) {
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        v4f acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const float b = HI ? T1[a * 8 + x] : T1[x * 8 + a];
+            acc0 = __builtin_amdgcn_mfma_f32_4x4x1f32(M1[a * 2 + 0], b, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_4x4x1f32(M1[a * 2 + 1], b, acc1, 0, 0, 0);
+        }
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const float b = HI ? T2[a * 8 + x] : T2[x * 8 + a];
+            acc0 = __builtin_amdgcn_mfma_f32_4x4x1f32(M2[a * 2 + 0], b, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_4x4x1f32(M2[a * 2 + 1], b, acc1, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (HI) { O[i * 8 + x] = acc0[i]; O[(4 + i) * 8 + x] = acc1[i]; }
+            else { O[x * 8 + i] = acc0[i]; O[x * 8 + 4 + i] = acc1[i]; }
+        }
+    }
+}
+
+// Register transposes through the wave's LDS buffer (row pitch 65: conflict-free).
+__device__ __forceinline__ void vd_x_to_y(float (&R)[64], float *buf, int lane) {
+#pragma unroll
+    for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = R[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) R[r] = buf[lane * VX_PITCH + r];
+    __syncthreads();
+}
+
+__device__ __forceinline__ void vd_y_to_x(float (&R)[64], float *buf, int lane) {
+#pragma unroll
+    for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = R[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) R[r] = buf[r * VX_PITCH + lane];
+    __syncthreads();
+}
+
+// Incoming messages (identity on padded ports) and overrides W at the out-slots (zero on
+// padded ports) into LDS, one port per wave in turn; `trans` stores every matrix transposed.
+// Port indices are compile-time so the slot arrays stay in registers.
+template <int NS>
+__device__ __forceinline__ void vd_load_mats(const float *msgs, const float *W, const int *ins,
+                                             const int *outs, const long long *ino,
+                                             const long long *outo, int trans, float *msm,
+                                             float *wsm) {
+    const int w = threadIdx.x >> 6, e = threadIdx.x & 63;
+    const int src = trans ? ((e & 7) * 8 + (e >> 3)) : e;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j % NS != w) continue;
+        msm[j * 64 + e] = ins[j] >= 0 ? msgs[ino[j] + src] : ((e >> 3) == (e & 7) ? 1.f : 0.f);
+        if (W) wsm[j * 64 + e] = outs[j] >= 0 ? W[outo[j] + src] : 0.f;
+    }
+}
+
+struct VdSlots {
+    int ins[4], outs[4];
+    long long ino[4], outo[4];
+};
+
+__device__ __forceinline__ VdSlots vd_slots(const int4 *vin, const int4 *vout,
+                                            const long long *msg_off, int u) {
+    VdSlots s;
+    const int4 vi = vin[u], vo = vout[u];
+    s.ins[0] = vi.x; s.ins[1] = vi.y; s.ins[2] = vi.z; s.ins[3] = vi.w;
+    s.outs[0] = vo.x; s.outs[1] = vo.y; s.outs[2] = vo.z; s.outs[3] = vo.w;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        s.ino[j] = s.ins[j] >= 0 ? msg_off[s.ins[j]] : 0;
+        s.outo[j] = s.outs[j] >= 0 ? msg_off[s.outs[j]] : 0;
+    }
+    return s;
+}
+
+// ---- message form.  to_in == 0: R_j goes to the out-slot of port j (raw, with its sum in
+// S_out and Z_u in z_out when given).  to_in == 1: add[in-slot] + R_j goes to the in-slot of
+// port j, with max|result - ref| and max|result| per slot in res_abs / res_ref when given.
+// `add` may be `out` (each in-slot is read and written by the same lane).
+template <int NS, bool HASW>
+__global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
+              const float *__restrict__ W, int op, double cv_const,
+              const double *__restrict__ cv_vertex, const int4 *__restrict__ vin,
+              const int4 *__restrict__ vout, const int *__restrict__ vlist,
+              const long long *__restrict__ msg_off, int to_in, const float *add, float *out,
+              const float *__restrict__ ref, double *__restrict__ S_out,
+              double *__restrict__ z_out, double *__restrict__ res_abs,
+              double *__restrict__ res_ref) {
+    extern __shared__ __align__(16) float vsm[];
+    const int u = blockIdx.x;
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    float *bufA = vsm + w * 2 * VX_BUF;
+    float *bufB = bufA + VX_BUF;
+    float *msm = vsm + NS * 2 * VX_BUF;
+    float *wsm = msm + 4 * 64;
+    float *part = wsm + 4 * 64;
+    const VdSlots sl = vd_slots(vin, vout, msg_off, u);
+    const int vid = vlist[u];
+    const float cv = (float)(cv_vertex ? cv_vertex[vid] : cv_const);
+    vd_load_mats<NS>(msgs, HASW ? W : nullptr, sl.ins, sl.outs, sl.ino, sl.outo, 0, msm, wsm);
+    const int ks = op == OP_X ? (w ^ 1) : w;
+    const float sg = (op == OP_Z && w == 1) ? -1.f : 1.f;
+    float R1[64], R2[64], R3[64];
+    {
+        const float *own = vsites + ((long long)u * NS + w) * 4096;
+        const float *kst = vsites + ((long long)u * NS + ks) * 4096;
+#pragma unroll
+        for (int r = 0; r < 64; ++r) bufA[r * VX_PITCH + lane] = own[r * 64 + lane];
+#pragma unroll
+        for (int r = 0; r < 64; ++r) R1[r] = kst[r * 64 + lane];
+    }
+    __syncthreads();
+    float M[16], Wm[16];
+    float o[4];
+
+    // ---- phase A: E01 (and its tangent) -> Y; out2 (open p2), out3 (open p3)
+    vx_load_m(msm + 0 * 64, M);
+    vd_mp1<true>(R1, M, R2);
+    if constexpr (HASW) { vx_load_m(wsm + 0 * 64, Wm); vd_mp1<true>(R1, Wm, R3); }
+    vx_load_m(msm + 1 * 64, M);
+    vd_mp1<false>(R2, M, R1);
+    if constexpr (HASW) { vx_load_m(wsm + 1 * 64, Wm); vd_mp2<false>(R3, M, R2, Wm, R3); }
+    vd_x_to_y(R1, bufB, lane);
+    if constexpr (HASW) {
+        vd_x_to_y(R3, bufB, lane);
+        if (cv != 0.f) {
+#pragma unroll
+            for (int r = 0; r < 64; ++r) R3[r] += cv * R1[r];
+        }
+    }
+    vx_load_m(msm + 3 * 64, M);
+    if constexpr (HASW) { vx_load_m(wsm + 3 * 64, Wm); vd_mp2<false>(R3, M, R1, Wm, R2); }
+    else vd_mp1<false>(R1, M, R2);
+    o[2] = vx_final<true, false, false>(R2, bufA);
+    vx_load_m(msm + 2 * 64, M);
+    if constexpr (HASW) { vx_load_m(wsm + 2 * 64, Wm); vd_mp2<true>(R3, M, R1, Wm, R2); }
+    else vd_mp1<true>(R1, M, R2);
+    o[3] = vx_final<false, false, false>(R2, bufA);
+
+    // ---- phase B: E23 (and its tangent) from the ket slice in Y -> X; out0, out1
+    {
+        const float *kbuf = vsm + ks * 2 * VX_BUF;
+#pragma unroll
+        for (int r = 0; r < 64; ++r) R1[r] = kbuf[lane * VX_PITCH + r];
+    }
+    vx_load_m(msm + 2 * 64, M);
+    vd_mp1<true>(R1, M, R2);
+    if constexpr (HASW) { vx_load_m(wsm + 2 * 64, Wm); vd_mp1<true>(R1, Wm, R3); }
+    vx_load_m(msm + 3 * 64, M);
+    vd_mp1<false>(R2, M, R1);
+    if constexpr (HASW) { vx_load_m(wsm + 3 * 64, Wm); vd_mp2<false>(R3, M, R2, Wm, R3); }
+    vd_y_to_x(R1, bufB, lane);
+    if constexpr (HASW) {
+        vd_y_to_x(R3, bufB, lane);
+        if (cv != 0.f) {
+#pragma unroll
+            for (int r = 0; r < 64; ++r) R3[r] += cv * R1[r];
+        }
+    }
+    vx_load_m(msm + 1 * 64, M);
+    if constexpr (HASW) { vx_load_m(wsm + 1 * 64, Wm); vd_mp2<false>(R3, M, R1, Wm, R2); }
+    else vd_mp1<false>(R1, M, R2);
+    o[0] = vx_final<true, true, false>(R2, bufA);
+    vx_load_m(msm + 0 * 64, M);
+    if constexpr (HASW) { vx_load_m(wsm + 0 * 64, Wm); vd_mp2<true>(R3, M, R1, Wm, R2); }
+    else vd_mp1<true>(R1, M, R2);
+    o[1] = vx_final<false, true, false>(R2, bufA);
+
+    // ---- sum over slices, write
+    const float scale = HASW ? sg : sg * cv;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) part[(w * 4 + j) * 64 + lane] = o[j] * scale;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j % NS != w) continue;
+        float x = 0.f;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) x += part[(s * 4 + j) * 64 + lane];
+        if (j == 0 && z_out) {
+            const double z = wave_sum((double)x * (double)msm[lane]);
+            if (lane == 0) z_out[vid] = z;
+        }
+        if (!to_in) {
+            if (sl.outs[j] < 0) continue;
+            if (out) out[sl.outo[j] + lane] = x;
+            if (S_out) {
+                const double S = wave_sum((double)x);
+                if (lane == 0) S_out[sl.outs[j]] = S;
+            }
+        } else {
+            if (sl.ins[j] < 0) continue;
+            const long long off = sl.ino[j] + lane;
+            const float v = x + (add ? add[off] : 0.f);
+            if (res_abs) {
+                const double dm = wave_max(fabs((double)v - (double)ref[off]));
+                const double rm = wave_max(fabs((double)v));
+                if (lane == 0) { res_abs[sl.ins[j]] = dm; res_ref[sl.ins[j]] = rm; }
+            }
+            out[off] = v;
+        }
+    }
+}
+
+// ---- site form: da[u][s] += R[s] at the real (unpadded) entries of u's site tensor.
+template <int NS, bool HASW>
+__global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_vd_site(const float *__restrict__ vsites, const float *__restrict__ msgs,
+               const float *__restrict__ W, int trans, int op, double cv_const,
+               const double *__restrict__ cv_vertex, const int4 *__restrict__ vin,
+               const int4 *__restrict__ vout, const int *__restrict__ vlist,
+               const long long *__restrict__ msg_off, const long long *__restrict__ site_off,
+               const int *__restrict__ port_off, float *__restrict__ da) {
+    extern __shared__ __align__(16) float vsm[];
+    const int u = blockIdx.x;
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    float *bufB = vsm + w * VX_BUF;
+    float *msm = vsm + NS * VX_BUF;
+    float *wsm = msm + 4 * 64;
+    const VdSlots sl = vd_slots(vin, vout, msg_off, u);
+    const int vid = vlist[u];
+    const float cv = (float)(cv_vertex ? cv_vertex[vid] : cv_const);
+    vd_load_mats<NS>(msgs, HASW ? W : nullptr, sl.ins, sl.outs, sl.ino, sl.outo, trans, msm, wsm);
+    const int ks = op == OP_X ? (w ^ 1) : w;
+    const float sg = (op == OP_Z && w == 1) ? -1.f : 1.f;
+    float R1[64], R2[64], R3[64];
+    {
+        const float *kst = vsites + ((long long)u * NS + ks) * 4096;
+#pragma unroll
+        for (int r = 0; r < 64; ++r) R1[r] = kst[r * 64 + lane];
+    }
+    __syncthreads();
+    float M[16], Wm[16];
+    vx_load_m(msm + 0 * 64, M);
+    vd_mp1<true>(R1, M, R2);                                                   // V1
+    if constexpr (HASW) { vx_load_m(wsm + 0 * 64, Wm); vd_mp1<true>(R1, Wm, R3); }
+    vx_load_m(msm + 1 * 64, M);
+    vd_mp1<false>(R2, M, R1);                                                  // V2
+    if constexpr (HASW) { vx_load_m(wsm + 1 * 64, Wm); vd_mp2<false>(R3, M, R2, Wm, R3); }
+    vd_x_to_y(R1, bufB, lane);
+    if constexpr (HASW) vd_x_to_y(R3, bufB, lane);
+    vx_load_m(msm + 2 * 64, M);
+    vd_mp1<true>(R1, M, R2);                                                   // V3
+    if constexpr (HASW) { vx_load_m(wsm + 2 * 64, Wm); vd_mp2<true>(R3, M, R1, Wm, R3); }
+    vx_load_m(msm + 3 * 64, M);
+    vd_mp1<false>(R2, M, R1);                                                  // V4
+    if constexpr (HASW) { vx_load_m(wsm + 3 * 64, Wm); vd_mp2<false>(R3, M, R2, Wm, R3); }
+    const float c = sg * cv;
+#pragma unroll
+    for (int r = 0; r < 64; ++r) R1[r] = HASW ? c * R1[r] + sg * R3[r] : c * R1[r];
+    vd_y_to_x(R1, bufB, lane);
+    // entry e = r * 64 + lane of the padded slice: ports (r >> 3, r & 7, lane >> 3, lane & 7)
+    const int k = port_off[vid + 1] - port_off[vid];
+    long long n = 1;
+    for (int j = 0; j < k; ++j) n *= 8;
+    // every load is issued before the first store (the stores may alias later loads as far
+    // as the compiler can tell, which would otherwise serialise the read-modify-writes)
+    float *dst = da + site_off[vid] + (long long)w * n;
+    // offset of padded entry (r, lane) in the real slice, or -1 for a padding entry
+    auto real_off = [&](int r) {
+        const int idx[4] = {r >> 3, r & 7, lane >> 3, lane & 7};
+        int off = 0;
+        bool real = true;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j < k) off = off * 8 + idx[j];
+            else real = real && idx[j] == 0;
+        }
+        return real ? off : -1;
+    };
+#pragma unroll
+    for (int r = 0; r < 64; ++r) {
+        const int off = real_off(r);
+        R2[r] = off >= 0 ? dst[off] : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < 64; ++r) {
+        const int off = real_off(r);
+        if (off >= 0) dst[off] = R2[r] + R1[r];
+    }
+}
+
+// ---- small kernels of the orchestration
+// dm[rev q] = cv1[owner q] * GX[q] + cv0[owner q] * G[q]   (vertex terms of the energy)
+__global__ void k_vd_vertex_terms(DevTopo tp, const float *G, const float *GX, const double *cv0,
+                                  const double *cv1, float *dm) {
+    const int q = blockIdx.x;
+    if (q >= tp.nports) return;
+    const int u = tp.owner[q];
+    const long long mo = tp.msg_off[q], ro = tp.msg_off[tp.port_rev[q]];
+    const int len = (int)(tp.msg_off[q + 1] - mo);
+    const float a = (float)cv1[u], b = (float)cv0[u];
+    for (int i = threadIdx.x; i < len; i += blockDim.x) dm[ro + i] = a * GX[mo + i] + b * G[mo + i];
+}
+
+// dst[q] = coef[q] * src[rev q]  (+ dst[q] when acc)
+__global__ void k_vd_scale_rev(DevTopo tp, const double *coef, const float *src, float *dst,
+                               int acc) {
+    const int q = blockIdx.x;
+    if (q >= tp.nports) return;
+    const long long mo = tp.msg_off[q], ro = tp.msg_off[tp.port_rev[q]];
+    const int len = (int)(tp.msg_off[q + 1] - mo);
+    const float c = (float)coef[q];
+    for (int i = threadIdx.x; i < len; i += blockDim.x)
+        dst[mo + i] = c * src[ro + i] + (acc ? dst[mo + i] : 0.f);
+}
+
+__global__ void k_vd_axpy(const float *a, float *b, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        b[i] += a[i];
+}
+
+__global__ void k_vd_inv(const double *z, double *out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = 1.0 / z[i];
+}
+
+// out[0] = max(ra) / max(rr) over n slots (one workgroup)
+__global__ __launch_bounds__(BP_BLOCK) void k_vd_res(const double *ra, const double *rr, int n,
+                                                      double *out) {
+    __shared__ double red[BP_BLOCK / 64];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double x = ra[i];
+        a = (x > a || x != x) ? x : a;
+        b = fmax(b, rr[i]);
+    }
+    a = block_max(a, red);
+    b = block_max(b, red);
+    if (threadIdx.x == 0) out[0] = b > 0.0 ? a / b : a;
+}
+
+// ---------------------------------------------------------------- host side
+
+static size_t vd_msg_lds(int ns) { return sizeof(float) * ((size_t)ns * 2 * VX_BUF + 512 + (size_t)ns * 256); }
+static size_t vd_site_lds(int ns) { return sizeof(float) * ((size_t)ns * VX_BUF + 512); }
+
+struct VdCtx {
+    const VertexPlan *vp;
+    hipStream_t st;
+    DevTopo tp;
+    const float *vsites;
+};
+
+static int vd_set_limits(int ns) {
+    static bool done[5] = {false, false, false, false, false};
+    if (ns < 1 || ns > 4) return 4;
+    if (done[ns]) return 0;
+    const void *fm[4][2] = {
+        {(const void *)k_vd_msg<1, false>, (const void *)k_vd_msg<1, true>},
+        {(const void *)k_vd_msg<2, false>, (const void *)k_vd_msg<2, true>},
+        {(const void *)k_vd_msg<3, false>, (const void *)k_vd_msg<3, true>},
+        {(const void *)k_vd_msg<4, false>, (const void *)k_vd_msg<4, true>}};
+    const void *fs[4][2] = {
+        {(const void *)k_vd_site<1, false>, (const void *)k_vd_site<1, true>},
+        {(const void *)k_vd_site<2, false>, (const void *)k_vd_site<2, true>},
+        {(const void *)k_vd_site<3, false>, (const void *)k_vd_site<3, true>},
+        {(const void *)k_vd_site<4, false>, (const void *)k_vd_site<4, true>}};
+    for (int h = 0; h < 2; ++h) {
+        if (hipFuncSetAttribute(fm[ns - 1][h], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)vd_msg_lds(ns)) != hipSuccess)
+            return 2;
+        if (hipFuncSetAttribute(fs[ns - 1][h], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)vd_site_lds(ns)) != hipSuccess)
+            return 2;
+    }
+    done[ns] = true;
+    return 0;
+}
+
+static int vd_msg(const VdCtx &c, const float *msgs, const float *W, int op, double cv_const,
+                  const double *cv_vertex, int to_in, const float *add, float *out,
+                  const float *ref, double *S_out, double *z_out, double *res_abs,
+                  double *res_ref) {
+    const VertexPlan &p = *c.vp;
+    const dim3 grid(p.nv), block(64 * p.d);
+    const size_t lds = vd_msg_lds(p.d);
+    const bool hw = W != nullptr;
+    if (op != OP_I && p.d != 2) return 4;  // X and Z pair the two slices of a qubit
+#define VD_MSG(NS, HW)                                                                         \
+    hipLaunchKernelGGL((k_vd_msg<NS, HW>), grid, block, lds, c.st, c.vsites, msgs, W, op,      \
+                       cv_const, cv_vertex, p.d_vin, p.d_vout, p.d_vlist, c.tp.msg_off, to_in, \
+                       add, out, ref, S_out, z_out, res_abs, res_ref)
+    switch (p.d * 2 + (hw ? 1 : 0)) {
+        case 2: VD_MSG(1, false); break;
+        case 3: VD_MSG(1, true); break;
+        case 4: VD_MSG(2, false); break;
+        case 5: VD_MSG(2, true); break;
+        case 6: VD_MSG(3, false); break;
+        case 7: VD_MSG(3, true); break;
+        case 8: VD_MSG(4, false); break;
+        case 9: VD_MSG(4, true); break;
+        default: return 4;
+    }
+#undef VD_MSG
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+static int vd_site(const VdCtx &c, const float *msgs, const float *W, int trans, int op,
+                   double cv_const, const double *cv_vertex, float *da) {
+    const VertexPlan &p = *c.vp;
+    const dim3 grid(p.nv), block(64 * p.d);
+    const size_t lds = vd_site_lds(p.d);
+    const bool hw = W != nullptr;
+    if (op != OP_I && p.d != 2) return 4;
+#define VD_SITE(NS, HW)                                                                         \
+    hipLaunchKernelGGL((k_vd_site<NS, HW>), grid, block, lds, c.st, c.vsites, msgs, W, trans,   \
+                       op, cv_const, cv_vertex, p.d_vin, p.d_vout, p.d_vlist, c.tp.msg_off,     \
+                       c.tp.site_off, c.tp.port_off, da)
+    switch (p.d * 2 + (hw ? 1 : 0)) {
+        case 2: VD_SITE(1, false); break;
+        case 3: VD_SITE(1, true); break;
+        case 4: VD_SITE(2, false); break;
+        case 5: VD_SITE(2, true); break;
+        case 6: VD_SITE(3, false); break;
+        case 7: VD_SITE(3, true); break;
+        case 8: VD_SITE(4, false); break;
+        case 9: VD_SITE(4, true); break;
+        default: return 4;
+    }
+#undef VD_SITE
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+#define VD_RET(x)                  \
+    do {                           \
+        const int rc_ = (x);       \
+        if (rc_) {                 \
+            s->free_temps();       \
+            return rc_;            \
+        }                          \
+    } while (0)
+
+// Gradient of log Z_BP: dlogZ/dA_u = (bra + ket derivative of Z_u) / Z_u at the fixed point.
+static int vadj_grad_lognorm(AdjointScratch *s, const VdCtx &c, const float *msgs, long long nsite,
+                             const GradOut &go) {
+    const int nv = c.tp.nv;
+    VD_RET(vd_set_limits(c.vp->d));
+    const bool direct = go.dev && !go.dev_f64;  // accumulate straight into the caller's buffer
+    double *z = (double *)s->alloc(sizeof(double) * nv);
+    double *iz = (double *)s->alloc(sizeof(double) * nv);
+    float *da = direct ? (float *)go.dev : (float *)s->alloc(sizeof(float) * nsite);
+    double *g64 = go.host ? (double *)s->alloc(sizeof(double) * nsite) : nullptr;
+    if (!z || !iz || !da || (go.host && !g64)) VD_RET(5);
+    VD_RET(vd_msg(c, msgs, nullptr, OP_I, 1.0, nullptr, 0, nullptr, nullptr, nullptr, nullptr, z,
+                  nullptr, nullptr));
+    hipLaunchKernelGGL(k_vd_inv, dim3((nv + 255) / 256), dim3(256), 0, c.st, z, iz, nv);
+    if (hipMemsetAsync(da, 0, sizeof(float) * nsite, c.st) != hipSuccess) VD_RET(2);
+    VD_RET(vd_site(c, msgs, nullptr, 0, OP_I, 0.0, iz, da));
+    VD_RET(vd_site(c, msgs, nullptr, 1, OP_I, 0.0, iz, da));
+    if (!direct) VD_RET(adj_emit_grad(c.st, (const float *)da, g64, nsite, go));
+    const hipError_t e = hipStreamSynchronize(c.st);
+    s->free_temps();
+    return e == hipSuccess ? 0 : 2;
+}
+
+// TFIM energy E = sum_<uv> -J <Z_u Z_v>_BP + sum_v -h <X_v>_BP and its gradient.
+static int vadj_tfim(AdjointScratch *s, const VdCtx &c, int ne, const float *msgs,
+                     long long nsite, long long nmsg, double J, double h, double *energy,
+                     const GradOut &go, double adj_tol, int adj_maxiter, int *out_iters,
+                     double *out_res) {
+    const DevTopo &tp = c.tp;
+    const int nv = tp.nv, np = tp.nports;
+    VD_RET(vd_set_limits(c.vp->d));
+    double *z = (double *)s->alloc(sizeof(double) * nv);
+    double *zx = (double *)s->alloc(sizeof(double) * nv);
+    float *G = (float *)s->alloc(sizeof(float) * nmsg);
+    float *GX = (float *)s->alloc(sizeof(float) * nmsg);
+    float *GZ = (float *)s->alloc(sizeof(float) * nmsg);
+    double *S = (double *)s->alloc(sizeof(double) * std::max(np, 1));
+    double *nz = (double *)s->alloc(sizeof(double) * std::max(ne, 1));
+    double *dd = (double *)s->alloc(sizeof(double) * std::max(ne, 1));
+    double *cv1 = (double *)s->alloc(sizeof(double) * nv);
+    double *cv0 = (double *)s->alloc(sizeof(double) * nv);
+    double *cq1 = (double *)s->alloc(sizeof(double) * std::max(np, 1));
+    double *cq0 = (double *)s->alloc(sizeof(double) * std::max(np, 1));
+    double *ev = (double *)s->alloc(sizeof(double) * nv);
+    double *eq = (double *)s->alloc(sizeof(double) * std::max(np, 1));
+    double *esum = (double *)s->alloc(sizeof(double));
+    if (!z || !zx || !G || !GX || !GZ || !S || !nz || !dd || !cv1 || !cv0 || !cq1 || !cq0 ||
+        !ev || !eq || !esum)
+        VD_RET(5);
+    // forward opens with I, X, Z on the ket: G (+ sums, Z_u), GX (+ Z_u^X), GZ
+    VD_RET(vd_msg(c, msgs, nullptr, OP_I, 1.0, nullptr, 0, nullptr, G, nullptr, S, z, nullptr,
+                  nullptr));
+    VD_RET(vd_msg(c, msgs, nullptr, OP_X, 1.0, nullptr, 0, nullptr, GX, nullptr, nullptr, zx,
+                  nullptr, nullptr));
+    VD_RET(vd_msg(c, msgs, nullptr, OP_Z, 1.0, nullptr, 0, nullptr, GZ, nullptr, nullptr, nullptr,
+                  nullptr, nullptr));
+    if (np > 0)
+        hipLaunchKernelGGL(k_adj_edge_dots<float>, dim3((np + 255) / 256), dim3(256), 0, c.st, tp,
+                           (const float *)G, (const float *)GZ, nz, dd);
+    const int nmax = std::max(nv, np);
+    hipLaunchKernelGGL(k_adj_tfim_coefs, dim3((nmax + 255) / 256), dim3(256), 0, c.st, tp, J, h,
+                       z, zx, nz, dd, cv1, cv0, cq1, cq0, ev, eq);
+    hipLaunchKernelGGL(k_adj_sum2, dim3(1), dim3(BP_BLOCK), 0, c.st, ev, nv, eq, np, esum);
+    if (hipMemcpyAsync(energy, esum, sizeof(double), hipMemcpyDeviceToHost, c.st) != hipSuccess ||
+        hipStreamSynchronize(c.st) != hipSuccess)
+        VD_RET(2);
+    if (!go.any()) {
+        s->free_temps();
+        return 0;
+    }
+    const bool direct = go.dev && !go.dev_f64;
+    float *dm = (float *)s->alloc(sizeof(float) * nmsg);
+    float *WI = (float *)s->alloc(sizeof(float) * nmsg);
+    float *WZ = (float *)s->alloc(sizeof(float) * nmsg);
+    float *lam = (float *)s->alloc(sizeof(float) * nmsg);
+    float *nxt = (float *)s->alloc(sizeof(float) * nmsg);
+    float *gg = (float *)s->alloc(sizeof(float) * nmsg);
+    float *da = direct ? (float *)go.dev : (float *)s->alloc(sizeof(float) * nsite);
+    double *ra = (double *)s->alloc(sizeof(double) * std::max(np, 1));
+    double *rr = (double *)s->alloc(sizeof(double) * std::max(np, 1));
+    double *res = (double *)s->alloc(sizeof(double));
+    double *g64 = go.host ? (double *)s->alloc(sizeof(double) * nsite) : nullptr;
+    if (!dm || !WI || !WZ || !lam || !nxt || !gg || !da || !ra || !rr || !res ||
+        (go.host && !g64))
+        VD_RET(5);
+    if (hipMemsetAsync(da, 0, sizeof(float) * nsite, c.st) != hipSuccess) VD_RET(2);
+    // explicit dE/dm: vertex terms, then the edge terms (port t carries the other side's G / GZ)
+    if (np > 0) {
+        hipLaunchKernelGGL(k_vd_vertex_terms, dim3(np), dim3(64), 0, c.st, tp, (const float *)G,
+                           (const float *)GX, (const double *)cv0, (const double *)cv1, dm);
+        hipLaunchKernelGGL(k_vd_scale_rev, dim3(np), dim3(64), 0, c.st, tp, (const double *)cq0,
+                           (const float *)G, WI, 0);
+        hipLaunchKernelGGL(k_vd_scale_rev, dim3(np), dim3(64), 0, c.st, tp, (const double *)cq1,
+                           (const float *)GZ, WZ, 0);
+        VD_RET(vd_msg(c, msgs, WI, OP_I, 0.0, nullptr, 1, dm, dm, nullptr, nullptr, nullptr,
+                      nullptr, nullptr));
+        VD_RET(vd_msg(c, msgs, WZ, OP_Z, 0.0, nullptr, 1, dm, dm, nullptr, nullptr, nullptr,
+                      nullptr, nullptr));
+    }
+    // adjoint sweeps lambda <- dE/dm + J_m^T lambda
+    const int cblocks = (int)std::max<long long>(1, std::min<long long>((nmsg + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_adj_axpy_copy<float>, dim3(cblocks), dim3(256), 0, c.st,
+                       (const float *)dm, lam, nmsg);
+    int it = 0;
+    double r = INFINITY;
+    while (np > 0 && it < adj_maxiter) {
+        hipLaunchKernelGGL(k_adj_gg<float>, dim3(np), dim3(64), 0, c.st, tp, (const float *)lam,
+                           (const float *)G, (const double *)S, gg);
+        VD_RET(vd_msg(c, msgs, gg, OP_I, 0.0, nullptr, 1, dm, nxt, lam, nullptr, nullptr, ra,
+                      rr));
+        hipLaunchKernelGGL(k_vd_res, dim3(1), dim3(BP_BLOCK), 0, c.st, (const double *)ra,
+                           (const double *)rr, np, res);
+        if (hipMemcpyAsync(&r, res, sizeof(double), hipMemcpyDeviceToHost, c.st) != hipSuccess ||
+            hipStreamSynchronize(c.st) != hipSuccess)
+            VD_RET(2);
+        std::swap(lam, nxt);
+        ++it;
+        if (r < adj_tol) break;
+    }
+    // grad = dE/dA + J_A^T lambda: op I carries W = cq0 G_rev + gg(lambda), op Z carries cq1 GZ_rev,
+    // op X and the value part of op I carry the vertex coefficients
+    if (np > 0) {
+        hipLaunchKernelGGL(k_adj_gg<float>, dim3(np), dim3(64), 0, c.st, tp, (const float *)lam,
+                           (const float *)G, (const double *)S, gg);
+        hipLaunchKernelGGL(k_vd_axpy, dim3(cblocks), dim3(256), 0, c.st, (const float *)gg, WI,
+                           nmsg);
+    }
+    const float *wi = np > 0 ? WI : nullptr, *wz = np > 0 ? WZ : nullptr;
+    for (int tr = 0; tr < 2; ++tr) {
+        VD_RET(vd_site(c, msgs, wi, tr, OP_I, 0.0, cv0, da));
+        if (wz) VD_RET(vd_site(c, msgs, wz, tr, OP_Z, 0.0, nullptr, da));
+        VD_RET(vd_site(c, msgs, nullptr, tr, OP_X, 0.0, cv1, da));
+    }
+    if (!direct) VD_RET(adj_emit_grad(c.st, (const float *)da, g64, nsite, go));
+    const hipError_t e = hipStreamSynchronize(c.st);
+    if (out_iters) *out_iters = it;
+    if (out_res) *out_res = r;
+    s->free_temps();
+    return e == hipSuccess ? 0 : 2;
+}
+
+#undef VD_RET

@@ -55,6 +55,7 @@ class Engine:
         self.lib = L.load()
         self.topo = topo
         self.dtype = dtype
+        self.device = int(device)
         h = C.c_void_p()
         L.check(self.lib.bp_create(C.byref(h), int(device), L.DTYPES[dtype]), "bp_create")
         self.h = h
@@ -205,17 +206,40 @@ class Engine:
         return out
 
     # -- gradients
-    def grad_lognorm(self) -> np.ndarray:
+    def _device_grad_buffer(self):
+        return L.DeviceArray(int(self.topo.site_off[-1]),
+                             np.float64 if self.dtype == "f64" else np.float32, self.device)
+
+    def grad_lognorm(self, device: bool = False, out=None):
+        """d log Z_BP / dA at the current fixed point: FP64 numpy array, or with
+        device=True a DeviceArray on the context's GPU in the arithmetic type (float64 for
+        f64 engines, float32 otherwise) that stays in device memory."""
+        if device:
+            out = out if out is not None else self._device_grad_buffer()
+            L.check(self.lib.bp_grad_lognorm_device(self.h, C.c_void_p(out.ptr)),
+                    "bp_grad_lognorm_device")
+            return out
         out = np.zeros(int(self.topo.site_off[-1]))
         L.check(self.lib.bp_grad_lognorm(self.h, _dp(out)), "bp_grad_lognorm")
         return out
 
     def tfim_energy(self, J: float = 1.0, h: float = 3.0, grad: bool = False,
-                    adj_tol: float = 1e-8, adj_maxiter: int = 1000):
+                    adj_tol: float = 1e-8, adj_maxiter: int = 1000, device: bool = False,
+                    out=None):
+        """BP energy of H = -J sum ZZ - h sum X; with grad=True also (gradient, adjoint
+        sweeps, adjoint residual).  device=True returns the gradient as a DeviceArray (see
+        grad_lognorm); `out` reuses a DeviceArray from an earlier call."""
         E = C.c_double(0)
-        g = np.zeros(int(self.topo.site_off[-1])) if grad else None
         it = C.c_int(0)
         res = C.c_double(0)
+        if grad and device:
+            g = out if out is not None else self._device_grad_buffer()
+            L.check(self.lib.bp_tfim_energy_device(self.h, float(J), float(h), C.byref(E),
+                                                   C.c_void_p(g.ptr), float(adj_tol),
+                                                   int(adj_maxiter), C.byref(it),
+                                                   C.byref(res)), "bp_tfim_energy_device")
+            return E.value, g, it.value, res.value
+        g = np.zeros(int(self.topo.site_off[-1])) if grad else None
         L.check(self.lib.bp_tfim_energy(self.h, float(J), float(h), C.byref(E), _dp(g),
                                         float(adj_tol), int(adj_maxiter), C.byref(it),
                                         C.byref(res)), "bp_tfim_energy")
