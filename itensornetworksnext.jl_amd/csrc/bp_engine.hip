@@ -243,6 +243,7 @@ struct bp_context {
     // vertex path plan
     VertexPlan vplan;
     bool vplan_ok = false;
+    int vx_min_d = 1;  // smallest bond dimension routed to the vertex kernels
     void *d_vsites = nullptr;  // site tensors re-laid out for the vertex kernel
 
     // adjoint scratch
@@ -621,7 +622,7 @@ int bp_set_topology(bp_context *c, int nv, int ne, const int *port_off, const in
     c->vplan_ok = vertex_plan_build(&c->vplan, c->dtype, c->plain, nv, c->port_off.data(),
                                     c->port_edge.data(), c->edge_dim.data(), c->phys_dim.data(),
                                     c->site_off.data(), c->msg_off.data(), c->port_rev.data(),
-                                    c->vnet.data()) == BP_OK;
+                                    c->vnet.data(), c->vx_min_d) == BP_OK;
     if (c->vplan_ok && vertex_set_lds_limits(&c->vplan) != BP_OK) {
         c->vplan_ok = false;
         c->vplan.why = "could not raise the LDS limit of the vertex kernel";
@@ -649,7 +650,7 @@ int bp_upload_sites(bp_context *c, const double *sites) {
     }
     if (c->vplan_ok) {
         if (vertex_upload_sites(&c->vplan, c->stream, c->d_sites64, &c->d_vsites, c->d_site_off,
-                                c->d_port_off) != BP_OK)
+                                c->d_port_off, c->d_port_edge, c->d_edge_dim) != BP_OK)
             return set_err(BP_ERR_HIP, "vertex-path site layout failed");
     }
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -18,7 +18,10 @@
 // out[a][a'] = sum E~[a..] A[a'..] runs per lane on the VALU (an 8x8x8 product) and is
 // reduced across the wave by a halving exchange, so lane l ends with entry l = (a, a').
 // Vertices of degree < 4 are padded with dimension-8 ports whose tensor entries vanish
-// beyond index 0 and whose message is the identity; the result is unchanged.
+// beyond index 0 and whose message is the identity, and bonds of dimension D < 8 are
+// zero-padded to 8 (site entries and incoming messages vanish beyond D, so the padded
+// entries of every product are exactly zero and only the D x D block is written back); the
+// result is unchanged.
 // The two LDS buffers per wave hold A (transposable, row pitch 65 floats: conflict-free
 // row and column access) and the transposed partial product.
 //
@@ -38,9 +41,9 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef short v4s __attribute__((ext_vector_type(4)));
 
 // The plan covers a dispatch bucket: every network whose vertices all have degree <= 4,
-// the same physical dimension d (<= 4) and bond dimension 8 on every edge.  Work item b
-// (one workgroup) is the b-th vertex of the bucket; the remaining networks' message slots
-// go to the generic kernel (gen_slots).
+// the same physical dimension d (<= 4) and bond dimensions in [min_d, 8] on every edge.
+// Work item b (one workgroup) is the b-th vertex of the bucket; the remaining networks'
+// message slots go to the generic kernel (gen_slots).
 struct VertexPlan {
     std::string why;
     int d = 0;                  // physical dimension (waves per workgroup)
@@ -49,6 +52,8 @@ struct VertexPlan {
     bool all = false;           // the bucket holds every network
     int4 *d_vin = nullptr;      // per item: incoming message slot per padded port (-1: dummy)
     int4 *d_vout = nullptr;     // per item: outgoing message slot per padded port (-1: dummy)
+    int4 *d_vdim = nullptr;     // per item: bond dimension per padded port (0: dummy)
+    int min_d = 1;              // smallest bond dimension the bucket takes
     int *d_vnet = nullptr;      // network of each item
     int *d_vlist = nullptr;     // vertex of each item
     int *d_gen_slots = nullptr; // message slots outside the bucket
@@ -63,10 +68,11 @@ static constexpr int VX_BUF = 64 * VX_PITCH;        // one transposable 64x64 bu
 static void vertex_plan_release(VertexPlan *p) {
     if (p->d_vin) (void)hipFree(p->d_vin);
     if (p->d_vout) (void)hipFree(p->d_vout);
+    if (p->d_vdim) (void)hipFree(p->d_vdim);
     if (p->d_vnet) (void)hipFree(p->d_vnet);
     if (p->d_vlist) (void)hipFree(p->d_vlist);
     if (p->d_gen_slots) (void)hipFree(p->d_gen_slots);
-    p->d_vin = p->d_vout = nullptr;
+    p->d_vin = p->d_vout = p->d_vdim = nullptr;
     p->d_vnet = p->d_vlist = p->d_gen_slots = nullptr;
     p->nv = p->ngen = 0;
     p->all = false;
@@ -82,7 +88,7 @@ static int vx_upload(T **dst, const std::vector<T> &v) {
 static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const int *port_off,
                              const int *port_edge, const int *edge_dim, const int *phys_dim,
                              const long long *site_off, const long long *msg_off,
-                             const int *port_rev, const int *vnet) {
+                             const int *port_rev, const int *vnet, int min_d) {
     (void)site_off;
     (void)msg_off;
     p->why.clear();
@@ -91,7 +97,9 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     // qualify whole networks (vertices of a network are contiguous)
     int d = -1;
     std::vector<int> vlist, gen_slots;
-    std::string last_reason = "no network has bond dimension 8 on every edge";
+    p->min_d = std::max(1, std::min(min_d, VX_D));
+    std::string last_reason = "no network has every bond dimension in [" +
+                              std::to_string(p->min_d) + ", 8]";
     for (int u0 = 0; u0 < nv;) {
         int u1 = u0;
         while (u1 < nv && vnet[u1] == vnet[u0]) ++u1;
@@ -101,8 +109,13 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
             if (k > 4) { ok = false; last_reason = "degree > 4"; }
             if (phys_dim[u] < 1 || phys_dim[u] > 4) { ok = false; last_reason = "physical dimension outside 1..4"; }
             if (d >= 0 && phys_dim[u] != d) { ok = false; last_reason = "physical dimension differs between networks"; }
-            for (int q = port_off[u]; q < port_off[u + 1] && ok; ++q)
-                if (edge_dim[port_edge[q]] != VX_D) { ok = false; last_reason = "bond dimension is not 8"; }
+            for (int q = port_off[u]; q < port_off[u + 1] && ok; ++q) {
+                const int D = edge_dim[port_edge[q]];
+                if (D > VX_D || D < p->min_d) {
+                    ok = false;
+                    last_reason = "bond dimension outside [" + std::to_string(p->min_d) + ", 8]";
+                }
+            }
         }
         if (ok && d < 0) d = phys_dim[u0];
         for (int u = u0; u < u1; ++u) {
@@ -113,21 +126,24 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     }
     if (vlist.empty()) { p->why = last_reason; return 4; }
     const int nx = (int)vlist.size();
-    std::vector<int4> vin(nx), vout(nx);
+    std::vector<int4> vin(nx), vout(nx), vdim(nx);
     std::vector<int> vn(nx);
     for (int b = 0; b < nx; ++b) {
         const int u = vlist[b];
-        int in[4] = {-1, -1, -1, -1}, out[4] = {-1, -1, -1, -1};
+        int in[4] = {-1, -1, -1, -1}, out[4] = {-1, -1, -1, -1}, dm[4] = {0, 0, 0, 0};
         for (int j = 0; j < port_off[u + 1] - port_off[u]; ++j) {
             out[j] = port_off[u] + j;
             in[j] = port_rev[port_off[u] + j];
+            dm[j] = edge_dim[port_edge[port_off[u] + j]];
         }
         vin[b] = make_int4(in[0], in[1], in[2], in[3]);
         vout[b] = make_int4(out[0], out[1], out[2], out[3]);
+        vdim[b] = make_int4(dm[0], dm[1], dm[2], dm[3]);
         vn[b] = vnet[u];
     }
     int rc = vx_upload(&p->d_vin, vin);
     if (!rc) rc = vx_upload(&p->d_vout, vout);
+    if (!rc) rc = vx_upload(&p->d_vdim, vdim);
     if (!rc) rc = vx_upload(&p->d_vnet, vn);
     if (!rc) rc = vx_upload(&p->d_vlist, vlist);
     if (!rc) rc = vx_upload(&p->d_gen_slots, gen_slots);
@@ -179,34 +195,42 @@ __device__ __forceinline__ void bf16_round2(float &a, float &b) {
     b = __uint_as_float(p & 0xffff0000u);
 }
 
-// Element e of the padded slice of u (layout X order, e = r * 64 + l).
+// Element e of the padded slice of u (layout X order, e = r * 64 + l): port j's index is
+// (e >> (9 - 3 j)) & 7; it is real below the port's bond dimension (padded ports: index 0).
 __device__ __forceinline__ float vx_pad_value(const double *sites64, const long long *site_off,
-                                              const int *port_off, const int *vlist, int d,
+                                              const int *port_off, const int *port_edge,
+                                              const int *edge_dim, const int *vlist, int d,
                                               long long us, int e) {
     const int s = (int)(us % d);
     const int u = vlist[us / d];
     const int k = port_off[u + 1] - port_off[u];
     const int idx[4] = {(e >> 9) & 7, (e >> 6) & 7, (e >> 3) & 7, e & 7};
-    long long off = 0;
+    long long off = 0, n = 1;
     for (int j = 0; j < 4; ++j) {
-        if (j < k) off = off * 8 + idx[j];
-        else if (idx[j] != 0) return 0.f;
+        if (j < k) {
+            const int D = edge_dim[port_edge[port_off[u] + j]];
+            if (idx[j] >= D) return 0.f;
+            off = off * D + idx[j];
+            n *= D;
+        } else if (idx[j] != 0) {
+            return 0.f;
+        }
     }
-    long long n = 1;
-    for (int j = 0; j < k; ++j) n *= 8;
     return (float)sites64[site_off[u] + s * n + off];
 }
 
 // FP32: vsites[u][s][r * 64 + l].  BF16: vsites[u][s][(r / 2) * 128 + 2 l + (r % 2)].
 __global__ void k_vertex_pad_sites(const double *sites64, const long long *site_off,
-                                   const int *port_off, const int *vlist, int d, int nv, int bf16,
+                                   const int *port_off, const int *port_edge,
+                                   const int *edge_dim, const int *vlist, int d, int nv, int bf16,
                                    void *vsites) {
     const long long total = (long long)nv * d * 4096;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
          i += (long long)gridDim.x * blockDim.x) {
         const int e = (int)(i & 4095);
         const long long us = i >> 12;
-        const float v = vx_pad_value(sites64, site_off, port_off, vlist, d, us, e);
+        const float v = vx_pad_value(sites64, site_off, port_off, port_edge, edge_dim, vlist, d,
+                                     us, e);
         if (bf16) {
             const int r = e >> 6, l = e & 63;
             ((unsigned short *)vsites)[(us << 12) + (r >> 1) * 128 + 2 * l + (r & 1)] = bf16_rne(v);
@@ -218,13 +242,15 @@ __global__ void k_vertex_pad_sites(const double *sites64, const long long *site_
 
 static int vertex_upload_sites(VertexPlan *p, hipStream_t st, const double *d_sites64,
                                void **d_vsites, const long long *d_site_off,
-                               const int *d_port_off) {
+                               const int *d_port_off, const int *d_port_edge,
+                               const int *d_edge_dim) {
     const size_t bytes = (p->bf16 ? 2 : 4) * (size_t)p->nv * p->d * 4096;
     if (*d_vsites) (void)hipFree(*d_vsites);
     *d_vsites = nullptr;
     if (hipMalloc(d_vsites, bytes) != hipSuccess) return 5;
     hipLaunchKernelGGL(k_vertex_pad_sites, dim3(2048), dim3(256), 0, st, d_sites64, d_site_off,
-                       d_port_off, p->d_vlist, p->d, p->nv, p->bf16, *d_vsites);
+                       d_port_off, d_port_edge, d_edge_dim, p->d_vlist, p->d, p->nv, p->bf16,
+                       *d_vsites);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
@@ -389,8 +415,8 @@ template <int NS, bool BF>
 __global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mold,
                  float *__restrict__ mnew, const int4 *__restrict__ vin,
-                 const int4 *__restrict__ vout, const int *__restrict__ vnet,
-                 const long long *__restrict__ msg_off, NetCtl ctl,
+                 const int4 *__restrict__ vout, const int4 *__restrict__ vdim,
+                 const int *__restrict__ vnet, const long long *__restrict__ msg_off, NetCtl ctl,
                  double *__restrict__ res_fid, double *__restrict__ res_abs) {
     extern __shared__ __align__(16) float vsm[];
     const int u = blockIdx.x;  // work item: the u-th vertex of the bucket
@@ -400,9 +426,10 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
     float *bufB = bufA + VX_BUF;
     float *msm = vsm + NS * 2 * VX_BUF;  // 4 x 64 incoming messages
     float *part = msm + 4 * 64;          // [s][port][64] per-slice results
-    const int4 vi = vin[u], vo = vout[u];
+    const int4 vi = vin[u], vo = vout[u], vdm = vdim[u];
     const int ins[4] = {vi.x, vi.y, vi.z, vi.w};
     const int outs[4] = {vo.x, vo.y, vo.z, vo.w};
+    const int dims[4] = {vdm.x, vdm.y, vdm.z, vdm.w};
     // message offsets (other networks of a batch may have other bond dimensions)
     long long ino[4], outo[4];
 #pragma unroll
@@ -410,28 +437,34 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
         ino[j] = ins[j] >= 0 ? msg_off[ins[j]] : 0;
         outo[j] = outs[j] >= 0 ? msg_off[outs[j]] : 0;
     }
+    // lane l holds entry (a, a') = (l / 8, l % 8) of an 8 x 8 block
+    const int la = lane >> 3, lb = lane & 7;
 
     const int net = vnet[u];
     if (ctl.enabled && ctl.done[net]) {  // uniform over the workgroup
-        for (int j = w; j < 4; j += NS) {
-            if (outs[j] < 0) continue;
-            mnew[outo[j] + lane] = mold[outo[j] + lane];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j % NS != w || outs[j] < 0) continue;
+            if (lane < dims[j] * dims[j]) mnew[outo[j] + lane] = mold[outo[j] + lane];
             if (lane == 0) { res_fid[outs[j]] = 0.0; res_abs[outs[j]] = 0.0; }
         }
         return;
     }
-    // incoming messages (identity on padded ports); BF16 mode scales each by 1 / max|m|
-    for (int i = threadIdx.x; i < 4 * 64; i += 64 * NS) {
-        const int j = i >> 6, e = i & 63;
-        msm[i] = ins[j] >= 0 ? mold[ino[j] + e] : ((e >> 3) == (e & 7) ? 1.f : 0.f);
-    }
-    if constexpr (BF) {
-        __syncthreads();
-        for (int j = w; j < 4; j += NS) {
-            const float x = msm[j * 64 + lane];
+    // incoming messages zero-padded to 8 x 8 (identity on padded ports); port indices are
+    // compile-time so the slot arrays stay in registers.  BF16 mode scales each message by
+    // 1 / max|m|.
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j % NS != w) continue;
+        const int D = dims[j];
+        float x;
+        if (ins[j] < 0) x = la == lb ? 1.f : 0.f;
+        else x = (la < D && lb < D) ? mold[ino[j] + la * D + lb] : 0.f;
+        if constexpr (BF) {
             const float mx = wave_max(fabsf(x));
-            msm[j * 64 + lane] = mx > 0.f ? x / mx : x;
+            x = mx > 0.f ? x / mx : x;
         }
+        msm[j * 64 + lane] = x;
     }
     // slice A[s] in layout X
     float T[64], E[64], G[64];
@@ -486,17 +519,21 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
 #pragma unroll
     for (int j = 0; j < 4; ++j) part[(w * 4 + j) * 64 + lane] = out[j];
     __syncthreads();
-    for (int j = w; j < 4; j += NS) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
         const int q = outs[j];
-        if (q < 0) continue;
+        if (j % NS != w || q < 0) continue;
         float x = 0.f;
 #pragma unroll
         for (int s = 0; s < NS; ++s) x += part[(s * 4 + j) * 64 + lane];
         const double S = wave_sum((double)x);
         const double xn = S != 0.0 ? (double)x / S : (double)x;
-        const float xf = (float)xn;
-        const double xo = (double)mold[outo[j] + lane];
-        mnew[outo[j] + lane] = xf;
+        const float xf = (float)xn;  // exactly 0 on padded entries
+        const int D = dims[j];
+        const bool real = la < D && lb < D;
+        const long long o = outo[j] + la * D + lb;
+        const double xo = real ? (double)mold[o] : 0.0;
+        if (real) mnew[o] = xf;
         const double dn = wave_sum((double)xf * xo), nn = wave_sum((double)xf * (double)xf),
                      oo = wave_sum(xo * xo);
         const double mx = wave_max(fabs((double)xf - xo));
@@ -518,7 +555,7 @@ static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *v
     case NS * 2 + BF:                                                                        \
         hipLaunchKernelGGL((k_vertex_d8<NS, BF>), dim3(p->nv), dim3(64 * NS), p->lds_bytes,  \
                            st, vsites, (const float *)src, (float *)dst, p->d_vin, p->d_vout, \
-                           p->d_vnet, tp.msg_off, ctl, res_fid, res_abs);                    \
+                           p->d_vdim, p->d_vnet, tp.msg_off, ctl, res_fid, res_abs);         \
         break;
         VX_CASE(1, 0)
         VX_CASE(2, 0)

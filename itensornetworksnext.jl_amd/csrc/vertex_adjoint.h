@@ -101,40 +101,43 @@ __device__ __forceinline__ void vd_y_to_x(float (&R)[64], float *buf, int lane) 
 }
 
 // Incoming messages (identity on padded ports) and overrides W at the out-slots (zero on
-// padded ports) into LDS, one port per wave in turn; `trans` stores every matrix transposed.
-// Port indices are compile-time so the slot arrays stay in registers.
-template <int NS>
-__device__ __forceinline__ void vd_load_mats(const float *msgs, const float *W, const int *ins,
-                                             const int *outs, const long long *ino,
-                                             const long long *outo, int trans, float *msm,
-                                             float *wsm) {
-    const int w = threadIdx.x >> 6, e = threadIdx.x & 63;
-    const int src = trans ? ((e & 7) * 8 + (e >> 3)) : e;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        if (j % NS != w) continue;
-        msm[j * 64 + e] = ins[j] >= 0 ? msgs[ino[j] + src] : ((e >> 3) == (e & 7) ? 1.f : 0.f);
-        if (W) wsm[j * 64 + e] = outs[j] >= 0 ? W[outo[j] + src] : 0.f;
-    }
-}
-
+// padded ports), zero-padded to 8 x 8, into LDS, one port per wave in turn; `trans` stores
+// every matrix transposed.  Port indices are compile-time so the slot arrays stay in
+// registers.
 struct VdSlots {
-    int ins[4], outs[4];
+    int ins[4], outs[4], dims[4];
     long long ino[4], outo[4];
 };
 
-__device__ __forceinline__ VdSlots vd_slots(const int4 *vin, const int4 *vout,
+__device__ __forceinline__ VdSlots vd_slots(const int4 *vin, const int4 *vout, const int4 *vdim,
                                             const long long *msg_off, int u) {
     VdSlots s;
-    const int4 vi = vin[u], vo = vout[u];
+    const int4 vi = vin[u], vo = vout[u], vd = vdim[u];
     s.ins[0] = vi.x; s.ins[1] = vi.y; s.ins[2] = vi.z; s.ins[3] = vi.w;
     s.outs[0] = vo.x; s.outs[1] = vo.y; s.outs[2] = vo.z; s.outs[3] = vo.w;
+    s.dims[0] = vd.x; s.dims[1] = vd.y; s.dims[2] = vd.z; s.dims[3] = vd.w;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         s.ino[j] = s.ins[j] >= 0 ? msg_off[s.ins[j]] : 0;
         s.outo[j] = s.outs[j] >= 0 ? msg_off[s.outs[j]] : 0;
     }
     return s;
+}
+
+template <int NS>
+__device__ __forceinline__ void vd_load_mats(const float *msgs, const float *W, const VdSlots &sl,
+                                             int trans, float *msm, float *wsm) {
+    const int w = threadIdx.x >> 6, e = threadIdx.x & 63;
+    const int a = trans ? (e & 7) : (e >> 3), b = trans ? (e >> 3) : (e & 7);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j % NS != w) continue;
+        const int D = sl.dims[j];
+        const bool real = a < D && b < D;
+        msm[j * 64 + e] = sl.ins[j] >= 0 ? (real ? msgs[sl.ino[j] + a * D + b] : 0.f)
+                                         : (a == b ? 1.f : 0.f);
+        if (W) wsm[j * 64 + e] = (sl.outs[j] >= 0 && real) ? W[sl.outo[j] + a * D + b] : 0.f;
+    }
 }
 
 // ---- message form.  to_in == 0: R_j goes to the out-slot of port j (raw, with its sum in
@@ -146,8 +149,9 @@ __global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
               const float *__restrict__ W, int op, double cv_const,
               const double *__restrict__ cv_vertex, const int4 *__restrict__ vin,
-              const int4 *__restrict__ vout, const int *__restrict__ vlist,
-              const long long *__restrict__ msg_off, int to_in, const float *add, float *out,
+              const int4 *__restrict__ vout, const int4 *__restrict__ vdim,
+              const int *__restrict__ vlist, const long long *__restrict__ msg_off, int to_in,
+              const float *add, float *out,
               const float *__restrict__ ref, double *__restrict__ S_out,
               double *__restrict__ z_out, double *__restrict__ res_abs,
               double *__restrict__ res_ref) {
@@ -160,10 +164,10 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
     float *msm = vsm + NS * 2 * VX_BUF;
     float *wsm = msm + 4 * 64;
     float *part = wsm + 4 * 64;
-    const VdSlots sl = vd_slots(vin, vout, msg_off, u);
+    const VdSlots sl = vd_slots(vin, vout, vdim, msg_off, u);
     const int vid = vlist[u];
     const float cv = (float)(cv_vertex ? cv_vertex[vid] : cv_const);
-    vd_load_mats<NS>(msgs, HASW ? W : nullptr, sl.ins, sl.outs, sl.ino, sl.outo, 0, msm, wsm);
+    vd_load_mats<NS>(msgs, HASW ? W : nullptr, sl, 0, msm, wsm);
     const int ks = op == OP_X ? (w ^ 1) : w;
     const float sg = (op == OP_Z && w == 1) ? -1.f : 1.f;
     float R1[64], R2[64], R3[64];
@@ -247,23 +251,27 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
             const double z = wave_sum((double)x * (double)msm[lane]);
             if (lane == 0) z_out[vid] = z;
         }
+        // lane = entry (a, a') of the padded 8 x 8 block; real below the bond dimension
+        const int D = sl.dims[j];
+        const bool real = (lane >> 3) < D && (lane & 7) < D;
+        const int ent = (lane >> 3) * D + (lane & 7);
         if (!to_in) {
             if (sl.outs[j] < 0) continue;
-            if (out) out[sl.outo[j] + lane] = x;
+            if (out && real) out[sl.outo[j] + ent] = x;
             if (S_out) {
                 const double S = wave_sum((double)x);
                 if (lane == 0) S_out[sl.outs[j]] = S;
             }
         } else {
             if (sl.ins[j] < 0) continue;
-            const long long off = sl.ino[j] + lane;
-            const float v = x + (add ? add[off] : 0.f);
+            const long long off = sl.ino[j] + ent;
+            const float v = real ? x + (add ? add[off] : 0.f) : 0.f;
             if (res_abs) {
-                const double dm = wave_max(fabs((double)v - (double)ref[off]));
+                const double dm = wave_max(real ? fabs((double)v - (double)ref[off]) : 0.0);
                 const double rm = wave_max(fabs((double)v));
                 if (lane == 0) { res_abs[sl.ins[j]] = dm; res_ref[sl.ins[j]] = rm; }
             }
-            out[off] = v;
+            if (real) out[off] = v;
         }
     }
 }
@@ -274,8 +282,9 @@ __global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_vd_site(const float *__restrict__ vsites, const float *__restrict__ msgs,
                const float *__restrict__ W, int trans, int op, double cv_const,
                const double *__restrict__ cv_vertex, const int4 *__restrict__ vin,
-               const int4 *__restrict__ vout, const int *__restrict__ vlist,
-               const long long *__restrict__ msg_off, const long long *__restrict__ site_off,
+               const int4 *__restrict__ vout, const int4 *__restrict__ vdim,
+               const int *__restrict__ vlist, const long long *__restrict__ msg_off,
+               const long long *__restrict__ site_off,
                const int *__restrict__ port_off, float *__restrict__ da) {
     extern __shared__ __align__(16) float vsm[];
     const int u = blockIdx.x;
@@ -284,10 +293,10 @@ void k_vd_site(const float *__restrict__ vsites, const float *__restrict__ msgs,
     float *bufB = vsm + w * VX_BUF;
     float *msm = vsm + NS * VX_BUF;
     float *wsm = msm + 4 * 64;
-    const VdSlots sl = vd_slots(vin, vout, msg_off, u);
+    const VdSlots sl = vd_slots(vin, vout, vdim, msg_off, u);
     const int vid = vlist[u];
     const float cv = (float)(cv_vertex ? cv_vertex[vid] : cv_const);
-    vd_load_mats<NS>(msgs, HASW ? W : nullptr, sl.ins, sl.outs, sl.ino, sl.outo, trans, msm, wsm);
+    vd_load_mats<NS>(msgs, HASW ? W : nullptr, sl, trans, msm, wsm);
     const int ks = op == OP_X ? (w ^ 1) : w;
     const float sg = (op == OP_Z && w == 1) ? -1.f : 1.f;
     float R1[64], R2[64], R3[64];
@@ -319,7 +328,9 @@ void k_vd_site(const float *__restrict__ vsites, const float *__restrict__ msgs,
     // entry e = r * 64 + lane of the padded slice: ports (r >> 3, r & 7, lane >> 3, lane & 7)
     const int k = port_off[vid + 1] - port_off[vid];
     long long n = 1;
-    for (int j = 0; j < k; ++j) n *= 8;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (j < k) n *= sl.dims[j];
     // every load is issued before the first store (the stores may alias later loads as far
     // as the compiler can tell, which would otherwise serialise the read-modify-writes)
     float *dst = da + site_off[vid] + (long long)w * n;
@@ -330,8 +341,12 @@ void k_vd_site(const float *__restrict__ vsites, const float *__restrict__ msgs,
         bool real = true;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            if (j < k) off = off * 8 + idx[j];
-            else real = real && idx[j] == 0;
+            if (j < k) {
+                off = off * sl.dims[j] + idx[j];
+                real = real && idx[j] < sl.dims[j];
+            } else {
+                real = real && idx[j] == 0;
+            }
         }
         return real ? off : -1;
     };
@@ -447,7 +462,8 @@ static int vd_msg(const VdCtx &c, const float *msgs, const float *W, int op, dou
     if (op != OP_I && p.d != 2) return 4;  // X and Z pair the two slices of a qubit
 #define VD_MSG(NS, HW)                                                                         \
     hipLaunchKernelGGL((k_vd_msg<NS, HW>), grid, block, lds, c.st, c.vsites, msgs, W, op,      \
-                       cv_const, cv_vertex, p.d_vin, p.d_vout, p.d_vlist, c.tp.msg_off, to_in, \
+                       cv_const, cv_vertex, p.d_vin, p.d_vout, p.d_vdim, p.d_vlist,            \
+                       c.tp.msg_off, to_in,                                                    \
                        add, out, ref, S_out, z_out, res_abs, res_ref)
     switch (p.d * 2 + (hw ? 1 : 0)) {
         case 2: VD_MSG(1, false); break;
@@ -473,7 +489,8 @@ static int vd_site(const VdCtx &c, const float *msgs, const float *W, int trans,
     if (op != OP_I && p.d != 2) return 4;
 #define VD_SITE(NS, HW)                                                                         \
     hipLaunchKernelGGL((k_vd_site<NS, HW>), grid, block, lds, c.st, c.vsites, msgs, W, trans,   \
-                       op, cv_const, cv_vertex, p.d_vin, p.d_vout, p.d_vlist, c.tp.msg_off,     \
+                       op, cv_const, cv_vertex, p.d_vin, p.d_vout, p.d_vdim, p.d_vlist,         \
+                       c.tp.msg_off,                                                            \
                        c.tp.site_off, c.tp.port_off, da)
     switch (p.d * 2 + (hw ? 1 : 0)) {
         case 2: VD_SITE(1, false); break;

@@ -312,10 +312,78 @@ def test_bf16_converged_then_fp64_polish(pkg, orc):
 
 
 def test_bf16_refuses_unsupported_shapes(pkg):
+    """BF16 runs only on the vertex kernels: a degree-6 cubic lattice is refused."""
     _say("case bf16 refusal")
-    t = _grid(pkg, 4, 3)
+    t = pkg.build_topology(pkg.named_grid((3, 3, 3)), 2, 2)
     s = pkg.peps.random_sites(t, 0)
     with pytest.raises(pkg._lib.BPError):
         with pkg.Engine(t, s, dtype="bf16") as e:
             e.init_messages("identity")
             e.sweep()
+
+
+# ---------------------------------------------------------------- bond dimensions below 8
+
+@pytest.mark.parametrize("shape,D,d", [((5, 4), 6, 2), ((4, 4), 3, 2), ((3, 5), 5, 3),
+                                       ((4, 3), 7, 1), ((3, 3), 1, 2)])
+def test_vertex_padded_bond_single_sweep(pkg, orc, shape, D, d):
+    """Bond dimensions below 8 run on the vertex kernels zero-padded to 8: one sweep from
+    random messages within the FP32 error bound, and identity messages to 1e-5."""
+    _say(f"case vertex padded D={D} {shape} d={d}")
+    t = pkg.build_topology(pkg.named_grid(shape), D, d)
+    s = pkg.peps.random_sites(t, 29, scale=pkg.peps.peps_scale(d, D))
+    m = pkg.peps.random_messages(t, 29)
+    with pkg.Engine(t, s, dtype="f32") as e:
+        assert e.path() == "vertex"
+        e.set_messages(m)
+        e.sweep()
+        got = e.messages()
+    assert _fp32_sweep_check(pkg, orc, t, s, m, got) < 1.0
+    mi = pkg.peps.identity_messages(t)
+    want_i, _, _ = orc.sweep_jacobi(t, s, mi)
+    with pkg.Engine(t, s, dtype="f32") as e:
+        e.set_messages(mi)
+        e.sweep()
+        assert _rel(e.messages(), want_i) < 1e-5
+
+
+def test_vertex_mixed_bond_dims_in_one_network(pkg, orc):
+    """Edges of one network with different bond dimensions (2..8)."""
+    _say("case vertex mixed bond dims")
+    g = pkg.named_grid((4, 5))
+    dims = {e: 2 + (i % 7) for i, e in enumerate(g.edges())}
+    t = pkg.build_topology(g, dims, 2)
+    s = pkg.peps.random_sites(t, 31)
+    m0 = pkg.peps.identity_messages(t)
+    want, sw, _, at = orc.bp(t, s, m0, maxiter=500, tol=1e-9, criterion="fidelity", nthreads=8)
+    with pkg.Engine(t, s, dtype="f32") as e:
+        assert e.path() == "vertex"
+        e.set_messages(m0)
+        r = e.run(maxiter=500, tol=1e-9, criterion="fidelity")
+        got = e.messages()
+        ln = e.lognorm()
+    assert r.converged[0]
+    assert _rel(got, want) < 1e-4
+    lw = orc.bethe_free_energy(t, s, want)
+    assert abs(ln - lw) / abs(lw) < 1e-5
+
+
+def test_vertex_padded_matches_generic_d6(pkg):
+    """The 16x16 D=6 differentiation lattice: vertex (padded) and generic FP32 paths agree
+    on the converged messages, the log-norm and the TFIM gradient."""
+    _say("case vertex vs generic 16x16 D=6")
+    t = _grid(pkg, 16, 6)
+    s = pkg.peps.random_sites(t, 33, scale=pkg.peps.peps_scale(2, 6))
+    out = {}
+    for path in ("generic", "auto"):
+        with pkg.Engine(t, s, dtype="f32", path=path) as e:
+            e.init_messages("identity")
+            r = e.run(maxiter=300, tol=1e-12, criterion="fidelity")
+            assert r.converged[0]
+            E, g, it, res = e.tfim_energy(1.0, 3.0, grad=True, adj_tol=1e-6, adj_maxiter=500)
+            out[path] = (e.path(), e.messages(), e.lognorm(), E, g)
+    assert out["auto"][0] == "vertex"
+    assert _rel(out["auto"][1], out["generic"][1]) < 1e-4
+    assert out["auto"][2] == pytest.approx(out["generic"][2], rel=1e-6)
+    assert out["auto"][3] == pytest.approx(out["generic"][3], rel=1e-6)
+    assert _rel(out["auto"][4], out["generic"][4]) < 2e-4
