@@ -73,6 +73,12 @@ __global__ void k_convert(const Tin *in, Tout *out, long long n) {
         out[i] = (Tout)in[i];
 }
 
+__global__ void k_round_bf16(const double *in, float *out, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        out[i] = bf16r((float)in[i]);
+}
+
 __global__ void k_reset_nets(int nnet, int *done, int *iter, double *delta, int *at) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nnet) return;
@@ -228,6 +234,7 @@ struct bp_context {
     // data
     double *d_sites64 = nullptr;  // FP64 master copy
     void *d_sites = nullptr;      // working dtype copy (FP32 for F32/BF16)
+    float *d_sites_bf = nullptr;  // BF16-rounded FP32 copy (BF16 contexts, generic kernel)
     void *d_msg[2] = {nullptr, nullptr};
     int cur = 0;
     double *d_msg64[2] = {nullptr, nullptr};  // FP64 polish buffers
@@ -294,7 +301,7 @@ static void free_all(bp_context *c) {
                     c->d_msg_off, c->d_sites64, c->d_sites, c->d_msg[0], c->d_msg[1],
                     c->d_msg64[0], c->d_msg64[1], c->d_res_fid, c->d_res_abs, c->d_zv, c->d_ze,
                     c->d_lre, c->d_lim, c->d_done, c->d_iter, c->d_at, c->d_delta, c->d_ws,
-                    c->d_vsites};
+                    c->d_vsites, c->d_sites_bf};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->adj.release();
@@ -340,14 +347,15 @@ static int ensure_ws(bp_context *c, const GenLaunch &g, size_t esz) {
 // Work items [q0, q1): message slots, or entries of `list` (a bucket's slots) when given.
 template <typename T>
 static int launch_update_generic(bp_context *c, const T *sites, const T *mold, T *mnew,
-                                 const NetCtl &ctl, int q0, int q1, const int *list = nullptr) {
+                                 const NetCtl &ctl, int q0, int q1, const int *list = nullptr,
+                                 int bf16 = 0) {
     if (q1 <= q0) return BP_OK;
     GenLaunch g = gen_launch(c, q1 - q0, sizeof(T));
     RETCHK(ensure_ws(c, g, sizeof(T)));
     hipLaunchKernelGGL(k_update_generic<T>, dim3(g.grid), dim3(BP_BLOCK), g.lds_bytes, c->stream,
                        c->topo(), sites, mold, mnew, ctl, c->d_res_fid, c->d_res_abs, q0, q1,
                        list, g.mats_elems, g.out_elems, g.red_elems, g.buf_elems, g.use_lds,
-                       (T *)c->d_ws, g.ws_stride);
+                       (T *)c->d_ws, g.ws_stride, bf16);
     HIPCHK(hipGetLastError());
     return BP_OK;
 }
@@ -380,9 +388,6 @@ static int launch_edge_scalars(bp_context *c, const T *msgs) {
 
 
 static int resolve_path(bp_context *c) {
-    // BF16 contractions exist only on the vertex path: no silent FP32 substitute
-    if (c->dtype == BP_BF16)
-        return c->vplan_ok && c->vplan.all && c->path != BP_PATH_GENERIC ? BP_PATH_VERTEX : -1;
     if (c->path == BP_PATH_GENERIC || !c->vplan_ok) return c->path == BP_PATH_VERTEX ? -1 : BP_PATH_GENERIC;
     if (c->vplan.all) return BP_PATH_VERTEX;
     // AUTO: vertex kernel for its bucket, generic kernel for the rest
@@ -418,6 +423,12 @@ static int launch_reduce(bp_context *c, int crit, int maxiter, double tol) {
     return BP_OK;
 }
 
+// Site operands of the generic message kernel: FP32, or BF16-rounded FP32 in BF16 contexts
+// (scalars and gradients keep the unrounded FP32 copy).
+static const float *generic_sites32(const bp_context *c) {
+    return (const float *)(c->dtype == BP_BF16 ? c->d_sites_bf : c->d_sites);
+}
+
 // One Jacobi sweep on the working dtype: reads d_msg[cur], writes d_msg[cur^1].
 static int sweep_once(bp_context *c, const NetCtl &done) {
     const int path = resolve_path(c);
@@ -428,16 +439,17 @@ static int sweep_once(bp_context *c, const NetCtl &done) {
         RETCHK(vertex_sweep(&c->vplan, c->stream, c->topo(), c->d_vsites, src, dst, done,
                             c->d_res_fid, c->d_res_abs));
         HIPCHK(hipGetLastError());
-        if (path == BP_PATH_MIXED)  // FP32 only (BF16 never mixes)
-            RETCHK(launch_update_generic<float>(c, (const float *)c->d_sites, (const float *)src,
+        if (path == BP_PATH_MIXED)
+            RETCHK(launch_update_generic<float>(c, generic_sites32(c), (const float *)src,
                                                 (float *)dst, done, 0, c->vplan.ngen,
-                                                c->vplan.d_gen_slots));
+                                                c->vplan.d_gen_slots, c->dtype == BP_BF16));
     } else if (c->dtype == BP_F64) {
         RETCHK(launch_update_generic<double>(c, (const double *)c->d_sites, (const double *)src,
                                              (double *)dst, done, 0, c->nports));
     } else {
-        RETCHK(launch_update_generic<float>(c, (const float *)c->d_sites, (const float *)src,
-                                            (float *)dst, done, 0, c->nports));
+        RETCHK(launch_update_generic<float>(c, generic_sites32(c), (const float *)src,
+                                            (float *)dst, done, 0, c->nports, nullptr,
+                                            c->dtype == BP_BF16));
     }
     c->cur ^= 1;
     return BP_OK;
@@ -613,6 +625,7 @@ int bp_set_topology(bp_context *c, int nv, int ne, const int *port_off, const in
     RETCHK(dev_alloc((void **)&c->d_delta, sizeof(double) * nnet));
     RETCHK(dev_alloc((void **)&c->d_sites64, sizeof(double) * c->nsite));
     RETCHK(dev_alloc(&c->d_sites, es * c->nsite));
+    if (c->dtype == BP_BF16) RETCHK(dev_alloc((void **)&c->d_sites_bf, sizeof(float) * c->nsite));
     if (c->d_msg64[0]) { (void)hipFree(c->d_msg64[0]); c->d_msg64[0] = nullptr; }
     if (c->d_msg64[1]) { (void)hipFree(c->d_msg64[1]); c->d_msg64[1] = nullptr; }
     c->fp64_active = false;
@@ -647,6 +660,11 @@ int bp_upload_sites(bp_context *c, const double *sites) {
         hipLaunchKernelGGL((k_convert<double, float>), dim3(std::max(blocks, 1)), dim3(256), 0,
                            c->stream, c->d_sites64, (float *)c->d_sites, c->nsite);
         HIPCHK(hipGetLastError());
+        if (c->dtype == BP_BF16) {
+            hipLaunchKernelGGL(k_round_bf16, dim3(std::max(blocks, 1)), dim3(256), 0, c->stream,
+                               c->d_sites64, c->d_sites_bf, c->nsite);
+            HIPCHK(hipGetLastError());
+        }
     }
     if (c->vplan_ok) {
         if (vertex_upload_sites(&c->vplan, c->stream, c->d_sites64, &c->d_vsites, c->d_site_off,
@@ -726,8 +744,6 @@ int bp_set_path(bp_context *c, int path) {
     if (path == BP_PATH_VERTEX && !(c->vplan_ok && c->vplan.all))
         return set_err(BP_ERR_UNSUPPORTED, "vertex path not available for every network: %s",
                        c->vplan.why.c_str());
-    if (path == BP_PATH_GENERIC && c->dtype == BP_BF16)
-        return set_err(BP_ERR_UNSUPPORTED, "BF16 contractions run on the vertex path only");
     c->path = path;
     return BP_OK;
 }
@@ -871,8 +887,9 @@ int bp_update_slot(bp_context *c, int slot) {
         RETCHK(launch_update_generic<double>(c, (const double *)c->d_sites, (const double *)m,
                                              (double *)m, make_ctl(c, false), slot, slot + 1));
     else
-        RETCHK(launch_update_generic<float>(c, (const float *)c->d_sites, (const float *)m,
-                                            (float *)m, make_ctl(c, false), slot, slot + 1));
+        RETCHK(launch_update_generic<float>(c, generic_sites32(c), (const float *)m,
+                                            (float *)m, make_ctl(c, false), slot, slot + 1,
+                                            nullptr, c->dtype == BP_BF16));
     HIPCHK(hipStreamSynchronize(c->stream));
     return BP_OK;
 }
@@ -982,8 +999,9 @@ int bp_run_sequential(bp_context *c, const int *order, int norder, int maxiter, 
                                                    (const double *)m, (double *)m, off, order[i],
                                                    order[i] + 1);
             else
-                rc = launch_update_generic<float>(c, (const float *)c->d_sites, (const float *)m,
-                                                  (float *)m, off, order[i], order[i] + 1);
+                rc = launch_update_generic<float>(c, generic_sites32(c), (const float *)m,
+                                                  (float *)m, off, order[i], order[i] + 1,
+                                                  nullptr, c->dtype == BP_BF16);
         }
         if (rc != BP_OK) break;
         ++it;

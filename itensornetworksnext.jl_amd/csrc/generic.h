@@ -137,6 +137,19 @@ __device__ GenSmem<T> carve(char *smem, int mats_elems, int out_elems, int red_e
     return s;
 }
 
+// x rounded to BF16 (round to nearest even; infinities and NaN pass through).
+__device__ __forceinline__ float bf16r(float x) {
+    unsigned u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return x;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xffff0000u);
+}
+
+template <typename T>
+__device__ __forceinline__ void block_round_bf16(T *x, long long n) {
+    for (long long i = threadIdx.x; i < n; i += blockDim.x) x[i] = (T)bf16r((float)x[i]);
+}
+
 // Load the incoming message of every port of u into LDS; returns per-port offsets.
 template <typename T>
 __device__ void load_incoming(const DevTopo &tp, int u, const VertexShape &vs, const T *msgs,
@@ -158,12 +171,17 @@ __device__ void load_incoming(const DevTopo &tp, int u, const VertexShape &vs, c
 // One directed message per workgroup iteration: update, normalise, residual.  Work items
 // i in [item_begin, item_end) are message slots, or slot_list[i] when a list is given (the
 // slots of one dispatch bucket).
+// bf16 != 0 (FP32 only): every product operand is rounded to BF16 and sums stay in FP32, as
+// on the vertex kernels: the sites arrive rounded, each incoming message is scaled by
+// 1 / max|m| (the scale cancels in the sum normalisation) and rounded, and every partial
+// product is rounded before it enters the next product.
 template <typename T>
 __global__ __launch_bounds__(BP_BLOCK) void k_update_generic(
     DevTopo tp, const T *__restrict__ sites, const T *mold, T *mnew,
     NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs,
     int item_begin, int item_end, const int *__restrict__ slot_list, int mats_elems,
-    int out_elems, int red_elems, long long buf_elems, int use_lds, T *ws, long long ws_stride) {
+    int out_elems, int red_elems, long long buf_elems, int use_lds, T *ws, long long ws_stride,
+    int bf16) {
     extern __shared__ __align__(16) char smem[];
     GenSmem<T> sm = carve<T>(smem, mats_elems, out_elems, red_elems, buf_elems, use_lds, ws,
                              ws_stride);
@@ -186,6 +204,21 @@ __global__ __launch_bounds__(BP_BLOCK) void k_update_generic(
         int moff[BP_MAXR];
         load_incoming(tp, u, vs, mold, sm.mats, moff, p);
         __syncthreads();
+        if (bf16) {
+            for (int j = 0; j < vs.k; ++j) {
+                if (j == p) continue;
+                const int D = vs.dims[1 + j];
+                const int mlen = tp.plain ? D : D * D;
+                double mx = 0.0;
+                for (int i = threadIdx.x; i < mlen; i += blockDim.x)
+                    mx = fmax(mx, fabs((double)sm.mats[moff[j] + i]));
+                mx = block_max(mx, sm.red);
+                const T sc = mx > 0.0 ? (T)(1.0 / mx) : T(1);
+                for (int i = threadIdx.x; i < mlen; i += blockDim.x)
+                    sm.mats[moff[j] + i] = (T)bf16r((float)(sm.mats[moff[j] + i] * sc));
+                __syncthreads();
+            }
+        }
         const T *A = sites + tp.site_off[u];
         const T *cur = A;
         T *bufs[2] = {sm.buf0, sm.buf1};
@@ -196,6 +229,12 @@ __global__ __launch_bounds__(BP_BLOCK) void k_update_generic(
             block_mode_product(cur, bufs[which], vs.dims, vs.r, 1 + j, sm.mats + moff[j], nb);
             if (tp.plain) vs.dims[1 + j] = 1;
             __syncthreads();
+            if (bf16) {
+                long long n = 1;
+                for (int i = 0; i < vs.r; ++i) n *= vs.dims[i];
+                block_round_bf16(bufs[which], n);
+                __syncthreads();
+            }
             cur = bufs[which];
             which ^= 1;
         }

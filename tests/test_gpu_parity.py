@@ -311,15 +311,57 @@ def test_bf16_converged_then_fp64_polish(pkg, orc):
     assert abs(ln_pol - lw) / abs(lw) < 1e-4
 
 
-def test_bf16_refuses_unsupported_shapes(pkg):
-    """BF16 runs only on the vertex kernels: a degree-6 cubic lattice is refused."""
-    _say("case bf16 refusal")
-    t = pkg.build_topology(pkg.named_grid((3, 3, 3)), 2, 2)
-    s = pkg.peps.random_sites(t, 0)
-    with pytest.raises(pkg._lib.BPError):
-        with pkg.Engine(t, s, dtype="bf16") as e:
-            e.init_messages("identity")
-            e.sweep()
+def test_bf16_generic_path_single_sweep(pkg, orc):
+    """BF16 operands on the generic kernel (bond dimension 10): one sweep from identity
+    messages within the BF16 error bound (gamma = 24 u, u = 2^-8)."""
+    _say("case bf16 generic single sweep D=10")
+    t = pkg.build_topology(pkg.named_grid((4, 3)), 10, 2)
+    s = pkg.peps.random_sites(t, 37, scale=pkg.peps.peps_scale(2, 10))
+    m = pkg.peps.identity_messages(t)
+    with pkg.Engine(t, s, dtype="bf16") as e:
+        assert e.path() == "generic"
+        e.set_messages(m)
+        e.sweep()
+        got = e.messages()
+    assert _fp32_sweep_check(pkg, orc, t, s, m, got, gamma=24 * 2.0 ** -8) < 1.0
+    # the same sweep in FP32 is more accurate than the BF16 one: the rounding is real
+    with pkg.Engine(t, s, dtype="f32") as e:
+        e.set_messages(m)
+        e.sweep()
+        got32 = e.messages()
+    want, _, _ = orc.sweep_jacobi(t, s, m)
+    assert _rel(got32, want) < _rel(got, want)
+
+
+def test_bf16_d16_converged_then_fp64_polish(pkg, orc):
+    """The large-bond mixed-precision workload at reduced lattice size (5x5, D = 16): BF16
+    operands with FP32 sums to convergence, then 3 FP64 polishing sweeps; errors against
+    the FP64 oracle at each stage (stated bounds: BF16 messages within 5e-2 and log-norm
+    within 1e-2 relative; polished messages at least 4x closer, log-norm within 1e-4)."""
+    _say("case bf16 D=16 converged + polish")
+    t = _grid(pkg, 5, 16)
+    s = pkg.peps.random_sites(t, 41, scale=pkg.peps.peps_scale(2, 16))
+    m0 = pkg.peps.identity_messages(t)
+    want, _, _, at = orc.bp(t, s, m0, maxiter=500, tol=1e-13, criterion="maxabs", nthreads=8)
+    assert at > 0
+    lw = orc.bethe_free_energy(t, s, want)
+    with pkg.Engine(t, s, dtype="bf16") as e:
+        e.set_messages(m0)
+        r = e.run(maxiter=200, tol=1e-5)
+        m_bf = e.messages()
+        ln_bf = e.lognorm()
+        e.polish_fp64(3)
+        m_pol = e.messages()
+        ln_pol = e.lognorm()
+    err_bf, err_pol = _rel(m_bf, want), _rel(m_pol, want)
+    _say(f"D=16 bf16 sweeps={r.sweeps[0]} msg err bf16={err_bf:.3e} polished={err_pol:.3e} "
+         f"lognorm err bf16={abs(ln_bf - lw) / abs(lw):.3e} "
+         f"polished={abs(ln_pol - lw) / abs(lw):.3e}")
+    assert r.converged[0]
+    assert err_bf < 5e-2
+    assert abs(ln_bf - lw) / abs(lw) < 1e-2
+    assert err_pol < err_bf / 4
+    assert abs(ln_pol - lw) / abs(lw) < 1e-4
 
 
 # ---------------------------------------------------------------- bond dimensions below 8
