@@ -169,6 +169,18 @@ int bp_tfim_energy_device(bp_context *ctx, double J, double h, double *energy, v
                           double adj_tol, int adj_maxiter, int *out_adj_iters,
                           double *out_adj_res);
 
+/* Autotuner.  Times, on the context's own sites and messages, (a) for every largest bond
+ * dimension D <= 8 present (FP32/BF16 norm networks, automatic path) whether its networks
+ * sweep faster on the vertex kernels (bonds padded to 8) or on the generic kernel, and
+ * (b) for every bucket of generic-kernel slots (owner's physical dimension and largest
+ * bond dimension) the workgroup size (128/256/512) and whether intermediates live in LDS
+ * (up to 64 KiB or up to 160 KiB) or in a global workspace.  mode 0: apply the choices found in `cache_path` only;
+ * 1: also tune what the cache lacks and rewrite it; 2: retune everything and rewrite it.
+ * cache_path may be NULL (no file).  The messages are restored afterwards.  out_tuned (may
+ * be NULL) receives the number of decisions timed.  No reference counterpart (the
+ * reference dispatches on Julia types, select_algorithm.jl:7-57, and has no kernels). */
+int bp_autotune(bp_context *ctx, const char *cache_path, int mode, int *out_tuned);
+
 /* Benchmark helper: launch `n` Jacobi sweeps back to back (no convergence bookkeeping,
  * messages ping-pong) and report the average device duration of one sweep's kernel
  * launches, timed with HIP events on the context's stream around each sweep. */

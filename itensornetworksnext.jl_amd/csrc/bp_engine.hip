@@ -9,6 +9,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <fstream>
+#include <map>
+#include <sstream>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -205,6 +208,30 @@ __global__ __launch_bounds__(BP_BLOCK) void k_lognorm_nets(DevTopo tp, const int
 
 // ------------------------------------------------------------------ context
 
+// Message slots run by the generic kernel, grouped by the shape of their owning vertex
+// (physical dimension d, largest bond dimension D; k is the largest degree in the bucket).
+// Each bucket is one launch with its own LDS / workspace sizing, workgroup size `bs` and
+// storage choice `lds`.  Degrees share a bucket: splitting them measured slower (more,
+// smaller launches per sweep).
+struct GenBucket {
+    int d = 0, k = 0, maxD = 0;
+    long long max_site = 0;
+    int max_mats = 0;
+    std::vector<int> slots;
+    int *d_slots = nullptr;
+    int bs = BP_BLOCK;  // 128, 256 or 512 threads
+    int lds = 1;        // intermediates: 0 global workspace, 1 LDS up to 64 KiB (more
+                        // workgroups per CU), 2 LDS up to 160 KiB
+    std::string key;
+};
+
+// One autotuner decision.  Generic buckets: workgroup size and storage.  Vertex mask
+// entries: whether networks of largest bond dimension D run on the vertex kernels.
+struct TuneEntry {
+    int block = BP_BLOCK, lds = 1, vertex = 1;
+    double us = 0.0, us_alt = 0.0;
+};
+
 struct bp_context {
     int device = 0;
     int dtype = BP_F64;
@@ -250,7 +277,12 @@ struct bp_context {
     // vertex path plan
     VertexPlan vplan;
     bool vplan_ok = false;
-    int vx_min_d = 1;  // smallest bond dimension routed to the vertex kernels
+    unsigned vx_mask = 0x1FEu;  // bit D: networks of largest bond dimension D may use it
+
+    // generic-kernel buckets of the resolved path (rebuilt when the path or plan changes)
+    std::vector<GenBucket> gb;
+    int gb_path = -1;
+    std::map<std::string, TuneEntry> tune;  // autotuner choices (cache)
     void *d_vsites = nullptr;  // site tensors re-laid out for the vertex kernel
 
     // adjoint scratch
@@ -306,6 +338,10 @@ static void free_all(bp_context *c) {
         if (p) (void)hipFree(p);
     c->adj.release();
     vertex_plan_release(&c->vplan);
+    for (auto &b : c->gb)
+        if (b.d_slots) (void)hipFree(b.d_slots);
+    c->gb.clear();
+    c->gb_path = -1;
 }
 
 // ------------------------------------------------------------------ generic launches
@@ -360,6 +396,156 @@ static int launch_update_generic(bp_context *c, const T *sites, const T *mold, T
     return BP_OK;
 }
 
+// ---- generic buckets
+
+static const char *dtype_name(int dt) { return dt == BP_F64 ? "f64" : dt == BP_F32 ? "f32" : "bf16"; }
+
+static constexpr size_t GEN_LDS_MAX = 160 * 1024;  // LDS per workgroup on gfx950
+static constexpr size_t GEN_LDS_DEFAULT = 64 * 1024;
+
+static GenLaunch gen_launch_bucket(const GenBucket &b, size_t esz) {
+    GenLaunch g;
+    g.mats_elems = std::max(b.max_mats, 1);
+    g.out_elems = std::max(b.maxD * b.maxD, 1);
+    g.red_elems = std::max(b.maxD * b.maxD, b.bs);
+    g.buf_elems = std::max(b.max_site, 1LL);
+    const size_t base = sizeof(double) * g.red_elems + esz * (g.mats_elems + g.out_elems) + 16;
+    const size_t with = base + 2 * esz * g.buf_elems;
+    g.use_lds = (b.lds == 2 && with <= GEN_LDS_MAX) || (b.lds == 1 && with <= GEN_LDS_DEFAULT);
+    g.lds_bytes = g.use_lds ? with : base;
+    g.grid = std::max(1, std::min((int)b.slots.size(), g.use_lds ? 65535 : 2048));
+    g.ws_stride = 2 * g.buf_elems;
+    return g;
+}
+
+static bool bucket_lds_fits(const GenBucket &b, size_t esz, int mode) {
+    GenBucket t = b;
+    t.lds = mode;
+    return gen_launch_bucket(t, esz).use_lds == 1;
+}
+
+// Raise the dynamic-LDS limit of every generic-kernel instance once per process.
+static int gen_set_lds_limits() {
+    static int done = 0;
+    if (done) return BP_OK;
+    const void *fns[6] = {(const void *)k_update_generic<float, 128>,
+                          (const void *)k_update_generic<float, 256>,
+                          (const void *)k_update_generic<float, 512>,
+                          (const void *)k_update_generic<double, 128>,
+                          (const void *)k_update_generic<double, 256>,
+                          (const void *)k_update_generic<double, 512>};
+    for (const void *f : fns)
+        HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)GEN_LDS_MAX));
+    done = 1;
+    return BP_OK;
+}
+
+static std::string bucket_key(int dtype, const GenBucket &b) {
+    std::ostringstream o;
+    o << "gen|" << dtype_name(dtype) << "|d" << b.d << "|D" << b.maxD;
+    return o.str();
+}
+
+static void free_buckets(bp_context *c) {
+    for (auto &b : c->gb)
+        if (b.d_slots) (void)hipFree(b.d_slots);
+    c->gb.clear();
+    c->gb_path = -1;
+}
+
+// Buckets of the generic slots of `path` (all slots, or the slots outside the vertex bucket).
+static int build_buckets(bp_context *c, int path) {
+    if (c->gb_path == path) return BP_OK;
+    free_buckets(c);
+    if (path == BP_PATH_VERTEX) { c->gb_path = path; return BP_OK; }
+    std::vector<int> all;
+    const std::vector<int> *slots = &c->vplan.gen_slots;
+    if (path != BP_PATH_MIXED) {
+        all.resize(c->nports);
+        for (int q = 0; q < c->nports; ++q) all[q] = q;
+        slots = &all;
+    }
+    std::map<std::pair<int, int>, int> index;
+    for (int q : *slots) {
+        const int u = c->owner[q];
+        const int k = c->port_off[u + 1] - c->port_off[u];
+        int maxD = 1, mats = 0;
+        for (int p = c->port_off[u]; p < c->port_off[u + 1]; ++p) {
+            const int D = c->edge_dim[c->port_edge[p]];
+            maxD = std::max(maxD, D);
+            mats += c->plain ? D : D * D;
+        }
+        const auto key = std::make_pair(c->plain ? 1 : c->phys_dim[u], maxD);
+        auto it = index.find(key);
+        if (it == index.end()) {
+            GenBucket b;
+            b.d = key.first;
+            b.maxD = maxD;
+            it = index.emplace(key, (int)c->gb.size()).first;
+            c->gb.push_back(b);
+        }
+        GenBucket &b = c->gb[it->second];
+        b.k = std::max(b.k, k);
+        b.slots.push_back(q);
+        b.max_site = std::max(b.max_site, c->site_off[u + 1] - c->site_off[u]);
+        b.max_mats = std::max(b.max_mats, mats);
+    }
+    for (auto &b : c->gb) {
+        b.key = bucket_key(c->dtype, b);
+        auto t = c->tune.find(b.key);
+        if (t != c->tune.end()) {
+            b.bs = t->second.block;
+            b.lds = t->second.lds;
+        }
+        RETCHK(upload(&b.d_slots, b.slots));
+    }
+    c->gb_path = path;
+    return BP_OK;
+}
+
+template <typename T>
+static int launch_bucket(bp_context *c, const GenBucket &b, const T *sites, const T *mold,
+                         T *mnew, const NetCtl &ctl, int bf16) {
+    if (b.slots.empty()) return BP_OK;
+    const GenLaunch g = gen_launch_bucket(b, sizeof(T));
+    const int n = (int)b.slots.size();
+#define GEN_B(BS)                                                                               \
+    hipLaunchKernelGGL((k_update_generic<T, BS>), dim3(g.grid), dim3(BS), g.lds_bytes,         \
+                       c->stream, c->topo(), sites, mold, mnew, ctl, c->d_res_fid,              \
+                       c->d_res_abs, 0, n, b.d_slots, g.mats_elems, g.out_elems, g.red_elems,   \
+                       g.buf_elems, g.use_lds, (T *)c->d_ws, g.ws_stride, bf16)
+    switch (b.bs) {
+        case 128: GEN_B(128); break;
+        case 512: GEN_B(512); break;
+        default: GEN_B(256); break;
+    }
+#undef GEN_B
+    HIPCHK(hipGetLastError());
+    return BP_OK;
+}
+
+// Workspace for every bucket (one buffer, buckets run in stream order).
+static int ensure_bucket_ws(bp_context *c, size_t esz) {
+    size_t need = 0;
+    for (const auto &b : c->gb) {
+        const GenLaunch g = gen_launch_bucket(b, esz);
+        if (!g.use_lds) need = std::max(need, esz * (size_t)g.ws_stride * (size_t)g.grid);
+    }
+    if (need > c->ws_bytes) {
+        RETCHK(dev_alloc(&c->d_ws, need));
+        c->ws_bytes = need;
+    }
+    return BP_OK;
+}
+
+template <typename T>
+static int launch_buckets(bp_context *c, const T *sites, const T *mold, T *mnew,
+                          const NetCtl &ctl, int bf16) {
+    for (const auto &b : c->gb) RETCHK(launch_bucket<T>(c, b, sites, mold, mnew, ctl, bf16));
+    return BP_OK;
+}
+
 template <typename T>
 static int launch_vertex_scalars(bp_context *c, const T *sites, const T *msgs) {
     GenLaunch g = gen_launch(c, c->nv, sizeof(T));
@@ -398,11 +584,11 @@ static int resolve_path(bp_context *c) {
 // allocation may happen inside a captured region).
 static int prepare_sweep(bp_context *c) {
     const int path = resolve_path(c);
-    if (path != BP_PATH_VERTEX) {
-        const size_t es = c->dtype == BP_F64 ? sizeof(double) : sizeof(float);
-        GenLaunch g = gen_launch(c, c->nports, es);
-        RETCHK(ensure_ws(c, g, es));
-    }
+    if (path < 0) return BP_OK;  // reported by sweep_once
+    RETCHK(gen_set_lds_limits());
+    RETCHK(build_buckets(c, path));
+    if (path != BP_PATH_VERTEX)
+        RETCHK(ensure_bucket_ws(c, c->dtype == BP_F64 ? sizeof(double) : sizeof(float)));
     return BP_OK;
 }
 
@@ -434,24 +620,45 @@ static int sweep_once(bp_context *c, const NetCtl &done) {
     const int path = resolve_path(c);
     if (path < 0) return set_err(BP_ERR_UNSUPPORTED, "vertex path not available: %s",
                                  c->vplan.why.c_str());
+    if (c->gb_path != path)  // normally done by prepare_sweep, outside any graph capture
+        RETCHK(prepare_sweep(c));
     void *src = c->d_msg[c->cur], *dst = c->d_msg[c->cur ^ 1];
     if (path == BP_PATH_VERTEX || path == BP_PATH_MIXED) {
         RETCHK(vertex_sweep(&c->vplan, c->stream, c->topo(), c->d_vsites, src, dst, done,
                             c->d_res_fid, c->d_res_abs));
         HIPCHK(hipGetLastError());
-        if (path == BP_PATH_MIXED)
-            RETCHK(launch_update_generic<float>(c, generic_sites32(c), (const float *)src,
-                                                (float *)dst, done, 0, c->vplan.ngen,
-                                                c->vplan.d_gen_slots, c->dtype == BP_BF16));
-    } else if (c->dtype == BP_F64) {
-        RETCHK(launch_update_generic<double>(c, (const double *)c->d_sites, (const double *)src,
-                                             (double *)dst, done, 0, c->nports));
-    } else {
-        RETCHK(launch_update_generic<float>(c, generic_sites32(c), (const float *)src,
-                                            (float *)dst, done, 0, c->nports, nullptr,
-                                            c->dtype == BP_BF16));
+    }
+    if (path != BP_PATH_VERTEX) {
+        if (c->dtype == BP_F64)
+            RETCHK(launch_buckets<double>(c, (const double *)c->d_sites, (const double *)src,
+                                          (double *)dst, done, 0));
+        else
+            RETCHK(launch_buckets<float>(c, generic_sites32(c), (const float *)src, (float *)dst,
+                                         done, c->dtype == BP_BF16));
     }
     c->cur ^= 1;
+    return BP_OK;
+}
+
+// Vertex-path plan for the current mask (may decline; the reason is kept for bp_set_path
+// errors), with the padded site layout when sites are present; generic buckets follow.
+static int rebuild_vertex_plan(bp_context *c) {
+    vertex_plan_release(&c->vplan);
+    free_buckets(c);
+    c->vplan_ok = vertex_plan_build(&c->vplan, c->dtype, c->plain, c->nv, c->port_off.data(),
+                                    c->port_edge.data(), c->edge_dim.data(), c->phys_dim.data(),
+                                    c->site_off.data(), c->msg_off.data(), c->port_rev.data(),
+                                    c->vnet.data(), c->vx_mask) == BP_OK;
+    if (c->vplan_ok && vertex_set_lds_limits(&c->vplan) != BP_OK) {
+        c->vplan_ok = false;
+        c->vplan.why = "could not raise the LDS limit of the vertex kernel";
+    }
+    if (c->vplan_ok && c->have_sites) {
+        if (vertex_upload_sites(&c->vplan, c->stream, c->d_sites64, &c->d_vsites, c->d_site_off,
+                                c->d_port_off, c->d_port_edge, c->d_edge_dim) != BP_OK)
+            return set_err(BP_ERR_HIP, "vertex-path site layout failed");
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     return BP_OK;
 }
 
@@ -630,16 +837,7 @@ int bp_set_topology(bp_context *c, int nv, int ne, const int *port_off, const in
     if (c->d_msg64[1]) { (void)hipFree(c->d_msg64[1]); c->d_msg64[1] = nullptr; }
     c->fp64_active = false;
     c->cur = 0;
-    // vertex-path plan (may decline; the reason is kept for bp_set_path errors)
-    vertex_plan_release(&c->vplan);
-    c->vplan_ok = vertex_plan_build(&c->vplan, c->dtype, c->plain, nv, c->port_off.data(),
-                                    c->port_edge.data(), c->edge_dim.data(), c->phys_dim.data(),
-                                    c->site_off.data(), c->msg_off.data(), c->port_rev.data(),
-                                    c->vnet.data(), c->vx_min_d) == BP_OK;
-    if (c->vplan_ok && vertex_set_lds_limits(&c->vplan) != BP_OK) {
-        c->vplan_ok = false;
-        c->vplan.why = "could not raise the LDS limit of the vertex kernel";
-    }
+    RETCHK(rebuild_vertex_plan(c));
     c->have_topo = true;
     c->have_sites = false;
     c->have_msgs = false;
@@ -841,6 +1039,219 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
         HIPCHK(hipMemcpy(out_delta, c->d_delta, sizeof(double) * c->nnet, hipMemcpyDeviceToHost));
     if (out_at)
         HIPCHK(hipMemcpy(out_at, c->d_at, sizeof(int) * c->nnet, hipMemcpyDeviceToHost));
+    return BP_OK;
+}
+
+// ------------------------------------------------------------------ autotuner
+//
+// Choices, each timed with HIP events on the context's own data (min of 3 launches after
+// a warm-up) and kept in a JSON cache, one entry per line:
+//   "vx|<dtype>|d<d>|D<D>": {"vertex": 0|1, "us": t_chosen, "us_alt": t_other}
+//       networks whose largest bond dimension is D <= 8 run on the vertex kernels (padded
+//       to 8) or on the generic kernel, whichever sweeps faster;
+//   "gen|<dtype>|d<d>|D<D>": {"block": 128|256|512, "lds": 0|1|2, "us": t, "us_alt": t0}
+//       workgroup size and intermediate storage (global, LDS up to 64 KiB, LDS up to
+//       160 KiB) of the generic kernel for the bucket of slots whose owner has physical
+//       dimension d and largest bond dimension D (t0: the default's time).
+
+static void tune_load(bp_context *c, const char *path) {
+    std::ifstream f(path);
+    if (!f) return;
+    std::string line;
+    while (std::getline(f, line)) {
+        const size_t a = line.find('"');
+        if (a == std::string::npos) continue;
+        const size_t b = line.find('"', a + 1);
+        if (b == std::string::npos) continue;
+        const std::string key = line.substr(a + 1, b - a - 1);
+        if (key.rfind("gen|", 0) != 0 && key.rfind("vx|", 0) != 0) continue;
+        TuneEntry e;
+        auto num = [&](const char *name, double *out) {
+            const std::string pat = std::string("\"") + name + "\":";
+            const size_t p = line.find(pat, b + 1);
+            if (p == std::string::npos) return false;
+            *out = atof(line.c_str() + p + pat.size());
+            return true;
+        };
+        double v = 0.0;
+        if (num("block", &v)) e.block = (int)v;
+        if (num("lds", &v)) e.lds = (int)v;
+        if (num("vertex", &v)) e.vertex = (int)v;
+        if (num("us", &v)) e.us = v;
+        if (num("us_alt", &v)) e.us_alt = v;
+        if (e.block != 128 && e.block != 256 && e.block != 512) e.block = BP_BLOCK;
+        c->tune[key] = e;
+    }
+}
+
+static int tune_save(const bp_context *c, const char *path) {
+    std::ofstream f(path);
+    if (!f) return set_err(BP_ERR_ARG, "cannot write the autotuner cache %s", path);
+    f << "{\n";
+    size_t i = 0;
+    for (const auto &kv : c->tune) {
+        const TuneEntry &e = kv.second;
+        f << "  \"" << kv.first << "\": {";
+        if (kv.first.rfind("gen|", 0) == 0) f << "\"block\": " << e.block << ", \"lds\": " << e.lds;
+        else f << "\"vertex\": " << e.vertex;
+        f << ", \"us\": " << e.us << ", \"us_alt\": " << e.us_alt << "}"
+          << (++i < c->tune.size() ? "," : "") << "\n";
+    }
+    f << "}\n";
+    f.close();
+    return f ? BP_OK : set_err(BP_ERR_ARG, "cannot write the autotuner cache %s", path);
+}
+
+// min over `reps` of one full plain sweep (every network), in ms; -1 on error
+static double tune_time_sweep(bp_context *c, int reps) {
+    if (prepare_sweep(c) != BP_OK) return -1.0;
+    const NetCtl ctl = make_ctl(c, false);
+    if (sweep_once(c, ctl) != BP_OK) return -1.0;
+    double best = 1e30;
+    for (int r = 0; r < reps; ++r) {
+        if (hipEventRecord(c->ev0, c->stream) != hipSuccess) return -1.0;
+        if (sweep_once(c, ctl) != BP_OK) return -1.0;
+        if (hipEventRecord(c->ev1, c->stream) != hipSuccess ||
+            hipEventSynchronize(c->ev1) != hipSuccess)
+            return -1.0;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        best = std::min(best, (double)ms);
+    }
+    return best;
+}
+
+// min over `reps` of one launch of bucket b (current messages in, the other buffer out)
+static double tune_time_bucket(bp_context *c, const GenBucket &b, int reps) {
+    const NetCtl ctl = make_ctl(c, false);
+    void *src = c->d_msg[c->cur], *dst = c->d_msg[c->cur ^ 1];
+    auto launch = [&]() {
+        if (c->dtype == BP_F64)
+            return launch_bucket<double>(c, b, (const double *)c->d_sites, (const double *)src,
+                                         (double *)dst, ctl, 0);
+        return launch_bucket<float>(c, b, generic_sites32(c), (const float *)src, (float *)dst,
+                                    ctl, c->dtype == BP_BF16);
+    };
+    if (launch() != BP_OK) return -1.0;
+    double best = 1e30;
+    for (int r = 0; r < reps; ++r) {
+        if (hipEventRecord(c->ev0, c->stream) != hipSuccess) return -1.0;
+        if (launch() != BP_OK) return -1.0;
+        if (hipEventRecord(c->ev1, c->stream) != hipSuccess ||
+            hipEventSynchronize(c->ev1) != hipSuccess)
+            return -1.0;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        best = std::min(best, (double)ms);
+    }
+    return best;
+}
+
+int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned) {
+    RETCHK(ready(c));
+    if (mode < 0 || mode > 2) return set_err(BP_ERR_ARG, "unknown autotune mode %d", mode);
+    HIPCHK(hipSetDevice(c->device));
+    if (cache_path && mode < 2) tune_load(c, cache_path);
+    if (mode == 2) c->tune.clear();
+    int tuned = 0;
+    const size_t es = c->elt();
+    // the timed sweeps advance the messages: keep a copy and restore it afterwards
+    void *snap = nullptr;
+    HIPCHK(hipMalloc(&snap, es * std::max<long long>(c->nmsg, 1)));
+    const int cur0 = c->cur;
+    const bool fp64_0 = c->fp64_active;
+    HIPCHK(hipMemcpyAsync(snap, c->d_msg[cur0], es * c->nmsg, hipMemcpyDeviceToDevice, c->stream));
+    c->fp64_active = false;
+    int rc = BP_OK;
+    // 1. vertex kernels or generic kernel, per largest bond dimension D <= 8
+    if ((c->dtype == BP_F32 || c->dtype == BP_BF16) && !c->plain && c->path == BP_PATH_AUTO) {
+        std::map<int, int> dmax_d;  // largest bond dimension -> physical dimension
+        for (int n = 0; n < c->nnet && rc == BP_OK; ++n) {
+            int dm = 0, deg = 0, d = c->phys_dim[c->vbeg[n]];
+            for (int u = c->vbeg[n]; u < c->vbeg[n + 1]; ++u) {
+                deg = std::max(deg, c->port_off[u + 1] - c->port_off[u]);
+                for (int q = c->port_off[u]; q < c->port_off[u + 1]; ++q)
+                    dm = std::max(dm, c->edge_dim[c->port_edge[q]]);
+            }
+            if (dm >= 1 && dm <= VX_D && deg <= 4 && d <= 4) dmax_d[dm] = d;
+        }
+        for (const auto &kv : dmax_d) {
+            const int D = kv.first;
+            std::ostringstream o;
+            o << "vx|" << dtype_name(c->dtype) << "|d" << kv.second << "|D" << D;
+            const std::string key = o.str();
+            auto it = c->tune.find(key);
+            if (it == c->tune.end() && mode >= 1) {
+                c->vx_mask |= 1u << D;
+                rc = rebuild_vertex_plan(c);
+                const double t_on = rc == BP_OK ? tune_time_sweep(c, 3) : -1.0;
+                c->vx_mask &= ~(1u << D);
+                if (rc == BP_OK) rc = rebuild_vertex_plan(c);
+                const double t_off = rc == BP_OK ? tune_time_sweep(c, 3) : -1.0;
+                if (rc != BP_OK || t_on < 0.0 || t_off < 0.0) {
+                    if (rc == BP_OK) rc = set_err(BP_ERR_HIP, "autotuner sweep failed");
+                    break;
+                }
+                TuneEntry e;
+                e.vertex = t_on <= t_off;
+                e.us = 1e3 * std::min(t_on, t_off);
+                e.us_alt = 1e3 * std::max(t_on, t_off);
+                it = c->tune.emplace(key, e).first;
+                ++tuned;
+            }
+            if (it != c->tune.end()) {
+                if (it->second.vertex) c->vx_mask |= 1u << D;
+                else c->vx_mask &= ~(1u << D);
+            }
+        }
+        if (rc == BP_OK) rc = rebuild_vertex_plan(c);
+    }
+    // 2. generic buckets: workgroup size and storage
+    const int path = resolve_path(c);
+    if (rc == BP_OK && path >= 0 && path != BP_PATH_VERTEX && mode >= 1) {
+        rc = prepare_sweep(c);
+        for (auto &b : c->gb) {
+            if (rc != BP_OK) break;
+            if (c->tune.count(b.key)) continue;  // applied by build_buckets
+            const int bs0 = b.bs, lds0 = b.lds;
+            double t0 = -1.0, best = 1e30;
+            int best_bs = bs0, best_lds = lds0;
+            for (int bs : {128, 256, 512}) {
+                for (int lds : {2, 1, 0}) {
+                    if (lds && !bucket_lds_fits(b, es, lds)) continue;
+                    if (lds == 2 && bucket_lds_fits(b, es, 1)) continue;  // same launch as 1
+                    b.bs = bs;
+                    b.lds = lds;
+                    rc = ensure_bucket_ws(c, es);
+                    if (rc != BP_OK) break;
+                    const double t = tune_time_bucket(c, b, 3);
+                    if (t < 0.0) { rc = set_err(BP_ERR_HIP, "autotuner launch failed"); break; }
+                    if (bs == BP_BLOCK && lds == (bucket_lds_fits(b, es, 1) ? 1 : 0)) t0 = t;
+                    if (t < best) { best = t; best_bs = bs; best_lds = lds; }
+                }
+                if (rc != BP_OK) break;
+            }
+            b.bs = best_bs;
+            b.lds = best_lds;
+            TuneEntry e;
+            e.block = best_bs;
+            e.lds = best_lds;
+            e.us = 1e3 * best;
+            e.us_alt = 1e3 * (t0 >= 0.0 ? t0 : best);
+            c->tune[b.key] = e;
+            ++tuned;
+        }
+        if (rc == BP_OK) rc = ensure_bucket_ws(c, es);
+    }
+    // restore the messages
+    (void)hipMemcpyAsync(c->d_msg[cur0], snap, es * c->nmsg, hipMemcpyDeviceToDevice, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(snap);
+    c->cur = cur0;
+    c->fp64_active = fp64_0;
+    if (rc != BP_OK) return rc;
+    if (cache_path && mode >= 1) RETCHK(tune_save(c, cache_path));
+    if (out_tuned) *out_tuned = tuned;
     return BP_OK;
 }
 

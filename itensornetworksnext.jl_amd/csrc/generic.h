@@ -175,8 +175,8 @@ __device__ void load_incoming(const DevTopo &tp, int u, const VertexShape &vs, c
 // on the vertex kernels: the sites arrive rounded, each incoming message is scaled by
 // 1 / max|m| (the scale cancels in the sum normalisation) and rounded, and every partial
 // product is rounded before it enters the next product.
-template <typename T>
-__global__ __launch_bounds__(BP_BLOCK) void k_update_generic(
+template <typename T, int BS = BP_BLOCK>
+__global__ __launch_bounds__(BS) void k_update_generic(
     DevTopo tp, const T *__restrict__ sites, const T *mold, T *mnew,
     NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs,
     int item_begin, int item_end, const int *__restrict__ slot_list, int mats_elems,

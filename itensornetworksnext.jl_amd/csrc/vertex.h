@@ -41,9 +41,11 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef short v4s __attribute__((ext_vector_type(4)));
 
 // The plan covers a dispatch bucket: every network whose vertices all have degree <= 4,
-// the same physical dimension d (<= 4) and bond dimensions in [min_d, 8] on every edge.
-// Work item b (one workgroup) is the b-th vertex of the bucket; the remaining networks'
-// message slots go to the generic kernel (gen_slots).
+// the same physical dimension d (<= 4) and bond dimensions <= 8 on every edge, with the
+// network's largest bond dimension D enabled in `mask` (bit D; the autotuner clears the
+// bit of a bond dimension that runs faster on the generic kernel).  Work item b (one
+// workgroup) is the b-th vertex of the bucket; the remaining networks' message slots go to
+// the generic kernel (gen_slots).
 struct VertexPlan {
     std::string why;
     int d = 0;                  // physical dimension (waves per workgroup)
@@ -53,10 +55,11 @@ struct VertexPlan {
     int4 *d_vin = nullptr;      // per item: incoming message slot per padded port (-1: dummy)
     int4 *d_vout = nullptr;     // per item: outgoing message slot per padded port (-1: dummy)
     int4 *d_vdim = nullptr;     // per item: bond dimension per padded port (0: dummy)
-    int min_d = 1;              // smallest bond dimension the bucket takes
+    unsigned mask = 0x1FEu;     // bit D: networks of largest bond dimension D may join
     int *d_vnet = nullptr;      // network of each item
     int *d_vlist = nullptr;     // vertex of each item
     int *d_gen_slots = nullptr; // message slots outside the bucket
+    std::vector<int> gen_slots; // the same on the host
     int ngen = 0;
     size_t lds_bytes = 0;
 };
@@ -74,6 +77,7 @@ static void vertex_plan_release(VertexPlan *p) {
     if (p->d_gen_slots) (void)hipFree(p->d_gen_slots);
     p->d_vin = p->d_vout = p->d_vdim = nullptr;
     p->d_vnet = p->d_vlist = p->d_gen_slots = nullptr;
+    p->gen_slots.clear();
     p->nv = p->ngen = 0;
     p->all = false;
 }
@@ -88,7 +92,7 @@ static int vx_upload(T **dst, const std::vector<T> &v) {
 static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const int *port_off,
                              const int *port_edge, const int *edge_dim, const int *phys_dim,
                              const long long *site_off, const long long *msg_off,
-                             const int *port_rev, const int *vnet, int min_d) {
+                             const int *port_rev, const int *vnet, unsigned mask) {
     (void)site_off;
     (void)msg_off;
     p->why.clear();
@@ -97,13 +101,13 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     // qualify whole networks (vertices of a network are contiguous)
     int d = -1;
     std::vector<int> vlist, gen_slots;
-    p->min_d = std::max(1, std::min(min_d, VX_D));
-    std::string last_reason = "no network has every bond dimension in [" +
-                              std::to_string(p->min_d) + ", 8]";
+    p->mask = mask;
+    std::string last_reason = "no network has every bond dimension <= 8";
     for (int u0 = 0; u0 < nv;) {
         int u1 = u0;
         while (u1 < nv && vnet[u1] == vnet[u0]) ++u1;
         bool ok = true;
+        int dmax = 0;
         for (int u = u0; u < u1 && ok; ++u) {
             const int k = port_off[u + 1] - port_off[u];
             if (k > 4) { ok = false; last_reason = "degree > 4"; }
@@ -111,11 +115,14 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
             if (d >= 0 && phys_dim[u] != d) { ok = false; last_reason = "physical dimension differs between networks"; }
             for (int q = port_off[u]; q < port_off[u + 1] && ok; ++q) {
                 const int D = edge_dim[port_edge[q]];
-                if (D > VX_D || D < p->min_d) {
-                    ok = false;
-                    last_reason = "bond dimension outside [" + std::to_string(p->min_d) + ", 8]";
-                }
+                dmax = std::max(dmax, D);
+                if (D > VX_D) { ok = false; last_reason = "bond dimension > 8"; }
             }
+        }
+        if (ok && dmax > 0 && !((mask >> dmax) & 1u)) {
+            ok = false;
+            last_reason = "bond dimension " + std::to_string(dmax) +
+                          " is routed to the generic kernel (autotuned)";
         }
         if (ok && d < 0) d = phys_dim[u0];
         for (int u = u0; u < u1; ++u) {
@@ -154,6 +161,7 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     }
     p->d = d;
     p->nv = nx;
+    p->gen_slots = gen_slots;
     p->ngen = (int)gen_slots.size();
     p->all = gen_slots.empty();
     p->bf16 = dtype == 2;

@@ -7,6 +7,7 @@ call lands in HIP kernels of libbpmi355x.so.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 from typing import Optional
 
@@ -206,6 +207,18 @@ class Engine:
         return out
 
     # -- gradients
+    def autotune(self, cache: str | None = None, mode: str = "missing") -> int:
+        """Time the kernel choices on this engine's data and apply the fastest (see
+        bp_autotune in include/bp_mi355x.h).  cache: JSON file of earlier decisions (read,
+        and rewritten unless mode == "cached").  mode: "cached" (apply the file only),
+        "missing" (tune what the file lacks) or "all" (retune).  Returns the number of
+        decisions timed; the messages are left as they were."""
+        modes = {"cached": 0, "missing": 1, "all": 2}
+        n = C.c_int(0)
+        path = None if cache is None else os.fsencode(cache)
+        L.check(self.lib.bp_autotune(self.h, path, modes[mode], C.byref(n)), "bp_autotune")
+        return n.value
+
     def _device_grad_buffer(self):
         return L.DeviceArray(int(self.topo.site_off[-1]),
                              np.float64 if self.dtype == "f64" else np.float32, self.device)
