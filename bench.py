@@ -141,6 +141,9 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--only", choices=["adjoint"], default=None,
+                    help="adjoint: run only the forward + adjoint workload (L x L, D; three "
+                         "energy + gradient calls) and print its timings, for profiling")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, used to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -156,6 +159,12 @@ def main(argv=None):
     import __graft_entry__ as ge
 
     pkg = ge.import_package()
+    if args.only == "adjoint":
+        torch.cuda.init()
+        runs = [adjoint_timing(pkg, torch, args.L, args.D, args.dtype, args.seed)
+                for _ in range(3)]
+        print(json.dumps({"adjoint_only": runs}), flush=True)
+        return 0
     L, D, d = args.L, args.D, 2
     g = pkg.named_grid((L, L))
     topo = pkg.build_topology(g, D, d)

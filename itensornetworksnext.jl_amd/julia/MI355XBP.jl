@@ -208,6 +208,37 @@ function device_tfim_energy_and_grad(ctx::DeviceContext, nsite::Integer; J = 1.0
     return E[], g
 end
 
+"""
+    device_tfim_energy_and_grad!(ctx, grad_ptr; J, h, adj_tol, adj_maxiter)
+
+The same with the gradient written into device memory at `grad_ptr` (`nsite` entries of
+Float32 for :f32/:bf16 contexts, Float64 for :f64), e.g. `pointer(g)` of an AMDGPU.jl
+`ROCArray` (ROCm's HIP runtime, the one the library links).  Nothing crosses PCIe.
+"""
+function device_tfim_energy_and_grad!(ctx::DeviceContext, grad_ptr::Ptr{Cvoid}; J = 1.0,
+        h = 3.0, adj_tol = 1.0e-8, adj_maxiter = 1000)
+    E = Ref{Float64}(0.0)
+    it = Ref{Cint}(0)
+    res = Ref{Float64}(0.0)
+    check(ccall((:bp_tfim_energy_device, libbp[]), Cint,
+        (Ptr{Cvoid}, Cdouble, Cdouble, Ref{Float64}, Ptr{Cvoid}, Cdouble, Cint, Ref{Cint},
+            Ref{Float64}), ctx.handle, J, h, E, grad_ptr, adj_tol, adj_maxiter, it, res),
+        "bp_tfim_energy_device")
+    return E[]
+end
+
+"""Autotune the kernel choices for this context's shapes; `mode` 0 applies the cache at
+`cache` only, 1 tunes what it lacks (and rewrites it), 2 retunes everything."""
+function autotune!(ctx::DeviceContext; cache::AbstractString = "", mode::Integer = 1)
+    n = Ref{Cint}(0)
+    GC.@preserve cache begin
+        path = isempty(cache) ? Ptr{UInt8}(C_NULL) : pointer(cache)
+        check(ccall((:bp_autotune, libbp[]), Cint, (Ptr{Cvoid}, Ptr{UInt8}, Cint, Ref{Cint}),
+            ctx.handle, path, mode, n), "bp_autotune")
+    end
+    return Int(n[])
+end
+
 # ChainRulesCore rrule: the pullback of the converged-BP energy with respect to the flat
 # site buffer is the adjoint gradient (implicit differentiation of the fixed point).
 function tfim_energy(ctx::DeviceContext, sites::AbstractVector{Float64}; J = 1.0, h = 3.0)
