@@ -185,6 +185,9 @@ def main(argv=None):
         del a, b
     eng = pkg.Engine(topo, sites, dtype=args.dtype, path=args.path)
     eng.init_messages("identity")
+    t_tune = time.perf_counter()
+    ntuned = eng.autotune(mode="all")  # kernel choices for this shape, outside the timed region
+    t_tune = time.perf_counter() - t_tune
     path = eng.path()
     # warmup
     if args.warmup > 0:
@@ -220,7 +223,16 @@ def main(argv=None):
                 "kernel_ms": round(kms, 5), "algorithmic_flops_per_launch": flops,
                 "algorithmic_bytes_per_launch": byts}
 
-    extras = {"path": path, "device_ms_per_step": round(res.ms / args.steps, 5)}
+    extras = {"path": path, "device_ms_per_step": round(res.ms / args.steps, 5),
+              "autotune": {"decisions": ntuned, "seconds": round(t_tune, 3)}}
+    if path == "vertex" and args.dtype == "f32":
+        # both FP32 sweep kernels, timed the same way (the run above used the tuned one)
+        for v in (0, 1):
+            e2 = pkg.Engine(topo, sites, dtype="f32")
+            e2.set_option("vertex_variant", v)
+            e2.init_messages("identity")
+            extras[f"vertex_variant_{v}_kernel_ms"] = e2.time_sweeps(100)
+            e2.close()
     if not args.no_extras and args.dtype != "f64":
         e64 = pkg.Engine(topo, sites, dtype="f64", path="generic")
         e64.init_messages("identity")

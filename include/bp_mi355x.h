@@ -169,10 +169,19 @@ int bp_tfim_energy_device(bp_context *ctx, double J, double h, double *energy, v
                           double adj_tol, int adj_maxiter, int *out_adj_iters,
                           double *out_adj_res);
 
+/* Kernel options (what the autotuner decides, settable by hand):
+ *   "vertex_variant"   0: FP32 vertex sweep kernel with two LDS buffers per wave (default),
+ *                      1: one buffer per wave and two waves per SIMD;
+ *   "vertex_bond_mask" bit D set: networks whose largest bond dimension is D may run on the
+ *                      vertex kernels (D 1..8; 9..16 in BF16 contexts; default 0x1FFFE).
+ * No reference counterpart. */
+int bp_set_option(bp_context *ctx, const char *name, int value);
+
 /* Autotuner.  Times, on the context's own sites and messages, (a) for every largest bond
  * dimension D <= 8 present (FP32/BF16 norm networks, automatic path) whether its networks
  * sweep faster on the vertex kernels (bonds padded to 8) or on the generic kernel, and
- * (b) for every bucket of generic-kernel slots (owner's physical dimension and largest
+ * (b) the FP32 vertex sweep kernel variant ("vertex_variant" of bp_set_option), and
+ * (c) for every bucket of generic-kernel slots (owner's physical dimension and largest
  * bond dimension) the workgroup size (128/256/512) and whether intermediates live in LDS
  * (up to 64 KiB or up to 160 KiB) or in a global workspace.  mode 0: apply the choices found in `cache_path` only;
  * 1: also tune what the cache lacks and rewrite it; 2: retune everything and rewrite it.
@@ -181,9 +190,10 @@ int bp_tfim_energy_device(bp_context *ctx, double J, double h, double *energy, v
  * reference dispatches on Julia types, select_algorithm.jl:7-57, and has no kernels). */
 int bp_autotune(bp_context *ctx, const char *cache_path, int mode, int *out_tuned);
 
-/* Benchmark helper: launch `n` Jacobi sweeps back to back (no convergence bookkeeping,
- * messages ping-pong) and report the average device duration of one sweep's kernel
- * launches, timed with HIP events on the context's stream around each sweep. */
+/* Benchmark helper: launch one untimed and then `n` timed Jacobi sweeps back to back (no
+ * convergence bookkeeping, messages ping-pong) and report the average device duration of
+ * one sweep's kernel launches, timed with HIP events on the context's stream around each
+ * sweep. */
 int bp_time_sweeps(bp_context *ctx, int n, double *avg_ms);
 
 /* Elapsed milliseconds of the last bp_run, measured with HIP events on the context's

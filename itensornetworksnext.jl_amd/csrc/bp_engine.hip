@@ -22,6 +22,7 @@
 #include "common.h"
 #include "generic.h"
 #include "vertex.h"
+#include "vertex16.h"
 #include "adjoint.h"
 #include "vertex_adjoint.h"
 
@@ -277,7 +278,9 @@ struct bp_context {
     // vertex path plan
     VertexPlan vplan;
     bool vplan_ok = false;
-    unsigned vx_mask = 0x1FEu;  // bit D: networks of largest bond dimension D may use it
+    unsigned vx_mask = 0x1FFFEu;  // bit D: networks of largest bond dimension D may use it
+    int vx_variant = 0;         // FP32 vertex sweep kernel (0: two LDS buffers per wave,
+                                // 1: one buffer per wave, two waves per SIMD)
 
     // generic-kernel buckets of the resolved path (rebuilt when the path or plan changes)
     std::vector<GenBucket> gb;
@@ -626,6 +629,8 @@ static int sweep_once(bp_context *c, const NetCtl &done) {
     if (path == BP_PATH_VERTEX || path == BP_PATH_MIXED) {
         RETCHK(vertex_sweep(&c->vplan, c->stream, c->topo(), c->d_vsites, src, dst, done,
                             c->d_res_fid, c->d_res_abs));
+        RETCHK(vertex16_sweep(&c->vplan, c->stream, c->topo(), src, dst, done, c->d_res_fid,
+                              c->d_res_abs));
         HIPCHK(hipGetLastError());
     }
     if (path != BP_PATH_VERTEX) {
@@ -640,6 +645,17 @@ static int sweep_once(bp_context *c, const NetCtl &done) {
     return BP_OK;
 }
 
+// Padded site layouts of both vertex buckets (bond dimension <= 8, and 9..16 in BF16).
+static int upload_vertex_sites(bp_context *c) {
+    if (vertex_upload_sites(&c->vplan, c->stream, c->d_sites64, &c->d_vsites, c->d_site_off,
+                            c->d_port_off, c->d_port_edge, c->d_edge_dim) != BP_OK ||
+        vertex16_upload_sites(&c->vplan, c->stream, c->d_sites64, c->d_site_off, c->d_port_off,
+                              c->d_port_edge, c->d_edge_dim) != BP_OK)
+        return set_err(BP_ERR_HIP, "vertex-path site layout failed");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BP_OK;
+}
+
 // Vertex-path plan for the current mask (may decline; the reason is kept for bp_set_path
 // errors), with the padded site layout when sites are present; generic buckets follow.
 static int rebuild_vertex_plan(bp_context *c) {
@@ -649,16 +665,13 @@ static int rebuild_vertex_plan(bp_context *c) {
                                     c->port_edge.data(), c->edge_dim.data(), c->phys_dim.data(),
                                     c->site_off.data(), c->msg_off.data(), c->port_rev.data(),
                                     c->vnet.data(), c->vx_mask) == BP_OK;
-    if (c->vplan_ok && vertex_set_lds_limits(&c->vplan) != BP_OK) {
+    if (c->vplan_ok && (vertex_set_lds_limits(&c->vplan) != BP_OK ||
+                        (c->vplan.nv16 > 0 && vertex16_set_lds_limit() != BP_OK))) {
         c->vplan_ok = false;
         c->vplan.why = "could not raise the LDS limit of the vertex kernel";
     }
-    if (c->vplan_ok && c->have_sites) {
-        if (vertex_upload_sites(&c->vplan, c->stream, c->d_sites64, &c->d_vsites, c->d_site_off,
-                                c->d_port_off, c->d_port_edge, c->d_edge_dim) != BP_OK)
-            return set_err(BP_ERR_HIP, "vertex-path site layout failed");
-        HIPCHK(hipStreamSynchronize(c->stream));
-    }
+    c->vplan.variant = c->vx_variant;
+    if (c->vplan_ok && c->have_sites) RETCHK(upload_vertex_sites(c));
     return BP_OK;
 }
 
@@ -864,11 +877,7 @@ int bp_upload_sites(bp_context *c, const double *sites) {
             HIPCHK(hipGetLastError());
         }
     }
-    if (c->vplan_ok) {
-        if (vertex_upload_sites(&c->vplan, c->stream, c->d_sites64, &c->d_vsites, c->d_site_off,
-                                c->d_port_off, c->d_port_edge, c->d_edge_dim) != BP_OK)
-            return set_err(BP_ERR_HIP, "vertex-path site layout failed");
-    }
+    if (c->vplan_ok) RETCHK(upload_vertex_sites(c));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_sites = true;
     return BP_OK;
@@ -944,6 +953,22 @@ int bp_set_path(bp_context *c, int path) {
                        c->vplan.why.c_str());
     c->path = path;
     return BP_OK;
+}
+
+int bp_set_option(bp_context *c, const char *name, int value) {
+    if (!c || !name) return set_err(BP_ERR_ARG, "NULL argument");
+    if (strcmp(name, "vertex_variant") == 0) {
+        if (value != 0 && value != 1) return set_err(BP_ERR_ARG, "vertex_variant is 0 or 1");
+        c->vx_variant = value;
+        c->vplan.variant = value;
+        return BP_OK;
+    }
+    if (strcmp(name, "vertex_bond_mask") == 0) {
+        c->vx_mask = (unsigned)value & 0x1FFFEu;
+        if (c->have_topo) RETCHK(rebuild_vertex_plan(c));
+        return BP_OK;
+    }
+    return set_err(BP_ERR_ARG, "unknown option %s", name);
 }
 
 int bp_get_path(bp_context *c, int *path) {
@@ -1049,6 +1074,9 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
 //   "vx|<dtype>|d<d>|D<D>": {"vertex": 0|1, "us": t_chosen, "us_alt": t_other}
 //       networks whose largest bond dimension is D <= 8 run on the vertex kernels (padded
 //       to 8) or on the generic kernel, whichever sweeps faster;
+//   "vxk|f32|d<d>": {"variant": 0|1, "us": t_chosen, "us_alt": t_other}
+//       the FP32 vertex sweep kernel: two LDS buffers per wave, or one buffer per wave
+//       with two waves per SIMD;
 //   "gen|<dtype>|d<d>|D<D>": {"block": 128|256|512, "lds": 0|1|2, "us": t, "us_alt": t0}
 //       workgroup size and intermediate storage (global, LDS up to 64 KiB, LDS up to
 //       160 KiB) of the generic kernel for the bucket of slots whose owner has physical
@@ -1064,7 +1092,8 @@ static void tune_load(bp_context *c, const char *path) {
         const size_t b = line.find('"', a + 1);
         if (b == std::string::npos) continue;
         const std::string key = line.substr(a + 1, b - a - 1);
-        if (key.rfind("gen|", 0) != 0 && key.rfind("vx|", 0) != 0) continue;
+        if (key.rfind("gen|", 0) != 0 && key.rfind("vx|", 0) != 0 && key.rfind("vxk|", 0) != 0)
+            continue;
         TuneEntry e;
         auto num = [&](const char *name, double *out) {
             const std::string pat = std::string("\"") + name + "\":";
@@ -1077,6 +1106,7 @@ static void tune_load(bp_context *c, const char *path) {
         if (num("block", &v)) e.block = (int)v;
         if (num("lds", &v)) e.lds = (int)v;
         if (num("vertex", &v)) e.vertex = (int)v;
+        if (num("variant", &v)) e.vertex = (int)v;
         if (num("us", &v)) e.us = v;
         if (num("us_alt", &v)) e.us_alt = v;
         if (e.block != 128 && e.block != 256 && e.block != 512) e.block = BP_BLOCK;
@@ -1093,6 +1123,7 @@ static int tune_save(const bp_context *c, const char *path) {
         const TuneEntry &e = kv.second;
         f << "  \"" << kv.first << "\": {";
         if (kv.first.rfind("gen|", 0) == 0) f << "\"block\": " << e.block << ", \"lds\": " << e.lds;
+        else if (kv.first.rfind("vxk|", 0) == 0) f << "\"variant\": " << e.vertex;
         else f << "\"vertex\": " << e.vertex;
         f << ", \"us\": " << e.us << ", \"us_alt\": " << e.us_alt << "}"
           << (++i < c->tune.size() ? "," : "") << "\n";
@@ -1173,7 +1204,8 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
                 for (int q = c->port_off[u]; q < c->port_off[u + 1]; ++q)
                     dm = std::max(dm, c->edge_dim[c->port_edge[q]]);
             }
-            if (dm >= 1 && dm <= VX_D && deg <= 4 && d <= 4) dmax_d[dm] = d;
+            const int dlim = c->dtype == BP_BF16 ? 16 : VX_D;
+            if (dm >= 1 && dm <= dlim && deg <= 4 && d <= 4) dmax_d[dm] = d;
         }
         for (const auto &kv : dmax_d) {
             const int D = kv.first;
@@ -1205,6 +1237,31 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
             }
         }
         if (rc == BP_OK) rc = rebuild_vertex_plan(c);
+    }
+    // 1b. FP32 vertex sweep kernel variant, timed on full sweeps
+    if (rc == BP_OK && c->dtype == BP_F32 && c->vplan_ok && c->path != BP_PATH_GENERIC) {
+        std::ostringstream o;
+        o << "vxk|f32|d" << c->vplan.d;
+        const std::string key = o.str();
+        auto it = c->tune.find(key);
+        if (it == c->tune.end() && mode >= 1) {
+            double t[2] = {-1.0, -1.0};
+            for (int v = 0; v < 2 && rc == BP_OK; ++v) {
+                c->vplan.variant = v;
+                t[v] = tune_time_sweep(c, 3);
+                if (t[v] < 0.0) rc = set_err(BP_ERR_HIP, "autotuner sweep failed");
+            }
+            if (rc == BP_OK) {
+                TuneEntry e;
+                e.vertex = t[1] < t[0] ? 1 : 0;  // the chosen variant
+                e.us = 1e3 * std::min(t[0], t[1]);
+                e.us_alt = 1e3 * std::max(t[0], t[1]);
+                it = c->tune.emplace(key, e).first;
+                ++tuned;
+            }
+        }
+        if (it != c->tune.end()) c->vx_variant = it->second.vertex;
+        c->vplan.variant = c->vx_variant;
     }
     // 2. generic buckets: workgroup size and storage
     const int path = resolve_path(c);
@@ -1263,7 +1320,8 @@ int bp_time_sweeps(bp_context *c, int n, double *avg_ms) {
     RETCHK(prepare_sweep(c));
     std::vector<hipEvent_t> ev(2 * n);
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
-    int rc = BP_OK;
+    // one untimed sweep first: a kernel's first launch may also load its code object
+    int rc = sweep_once(c, make_ctl(c, false));
     for (int i = 0; i < n && rc == BP_OK; ++i) {
         (void)hipEventRecord(ev[2 * i], c->stream);
         rc = sweep_once(c, make_ctl(c, false));

@@ -62,6 +62,13 @@ struct VertexPlan {
     std::vector<int> gen_slots; // the same on the host
     int ngen = 0;
     size_t lds_bytes = 0;
+    int variant = 0;            // FP32 sweep kernel: 0 k_vertex_d8, 1 k_vertex_d8_lo
+    // BF16 networks whose largest bond dimension is 9..16 (k_vertex_d16, vertex16.h)
+    int nv16 = 0;
+    int4 *d_v16in = nullptr, *d_v16out = nullptr, *d_v16dim = nullptr;
+    int *d_v16net = nullptr, *d_v16list = nullptr;
+    unsigned short *d_v16sites = nullptr;  // padded BF16 slices, 16^4 entries each
+    float *d_v16part = nullptr;            // per-slice unnormalised messages
 };
 
 static constexpr int VX_D = 8;
@@ -72,6 +79,15 @@ static void vertex_plan_release(VertexPlan *p) {
     if (p->d_vin) (void)hipFree(p->d_vin);
     if (p->d_vout) (void)hipFree(p->d_vout);
     if (p->d_vdim) (void)hipFree(p->d_vdim);
+    void *v16[7] = {p->d_v16in, p->d_v16out, p->d_v16dim, p->d_v16net, p->d_v16list,
+                    p->d_v16sites, p->d_v16part};
+    for (void *x : v16)
+        if (x) (void)hipFree(x);
+    p->d_v16in = p->d_v16out = p->d_v16dim = nullptr;
+    p->d_v16net = p->d_v16list = nullptr;
+    p->d_v16sites = nullptr;
+    p->d_v16part = nullptr;
+    p->nv16 = 0;
     if (p->d_vnet) (void)hipFree(p->d_vnet);
     if (p->d_vlist) (void)hipFree(p->d_vlist);
     if (p->d_gen_slots) (void)hipFree(p->d_gen_slots);
@@ -100,7 +116,7 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     if (dtype != 1 && dtype != 2) { p->why = "the vertex path computes in FP32 or BF16"; return 4; }
     // qualify whole networks (vertices of a network are contiguous)
     int d = -1;
-    std::vector<int> vlist, gen_slots;
+    std::vector<int> vlist, vlist16, gen_slots;
     p->mask = mask;
     std::string last_reason = "no network has every bond dimension <= 8";
     for (int u0 = 0; u0 < nv;) {
@@ -116,7 +132,10 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
             for (int q = port_off[u]; q < port_off[u + 1] && ok; ++q) {
                 const int D = edge_dim[port_edge[q]];
                 dmax = std::max(dmax, D);
-                if (D > VX_D) { ok = false; last_reason = "bond dimension > 8"; }
+                if (D > 16 || (D > VX_D && dtype != 2)) {
+                    ok = false;
+                    last_reason = dtype == 2 ? "bond dimension > 16" : "bond dimension > 8";
+                }
             }
         }
         if (ok && dmax > 0 && !((mask >> dmax) & 1u)) {
@@ -126,12 +145,13 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
         }
         if (ok && d < 0) d = phys_dim[u0];
         for (int u = u0; u < u1; ++u) {
-            if (ok) vlist.push_back(u);
+            if (ok && dmax <= VX_D) vlist.push_back(u);
+            else if (ok) vlist16.push_back(u);
             else for (int q = port_off[u]; q < port_off[u + 1]; ++q) gen_slots.push_back(q);
         }
         u0 = u1;
     }
-    if (vlist.empty()) { p->why = last_reason; return 4; }
+    if (vlist.empty() && vlist16.empty()) { p->why = last_reason; return 4; }
     const int nx = (int)vlist.size();
     std::vector<int4> vin(nx), vout(nx), vdim(nx);
     std::vector<int> vn(nx);
@@ -148,9 +168,34 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
         vdim[b] = make_int4(dm[0], dm[1], dm[2], dm[3]);
         vn[b] = vnet[u];
     }
+    // the bond-dimension-16 bucket, laid out the same way
+    const int n16 = (int)vlist16.size();
+    std::vector<int4> vin16(n16), vout16(n16), vdim16(n16);
+    std::vector<int> vn16(n16);
+    for (int b = 0; b < n16; ++b) {
+        const int u = vlist16[b];
+        int in[4] = {-1, -1, -1, -1}, out[4] = {-1, -1, -1, -1}, dm[4] = {0, 0, 0, 0};
+        for (int j = 0; j < port_off[u + 1] - port_off[u]; ++j) {
+            out[j] = port_off[u] + j;
+            in[j] = port_rev[port_off[u] + j];
+            dm[j] = edge_dim[port_edge[port_off[u] + j]];
+        }
+        vin16[b] = make_int4(in[0], in[1], in[2], in[3]);
+        vout16[b] = make_int4(out[0], out[1], out[2], out[3]);
+        vdim16[b] = make_int4(dm[0], dm[1], dm[2], dm[3]);
+        vn16[b] = vnet[u];
+    }
     int rc = vx_upload(&p->d_vin, vin);
     if (!rc) rc = vx_upload(&p->d_vout, vout);
     if (!rc) rc = vx_upload(&p->d_vdim, vdim);
+    if (!rc) rc = vx_upload(&p->d_v16in, vin16);
+    if (!rc) rc = vx_upload(&p->d_v16out, vout16);
+    if (!rc) rc = vx_upload(&p->d_v16dim, vdim16);
+    if (!rc) rc = vx_upload(&p->d_v16net, vn16);
+    if (!rc) rc = vx_upload(&p->d_v16list, vlist16);
+    if (!rc && n16 > 0 &&
+        hipMalloc((void **)&p->d_v16part, sizeof(float) * (size_t)n16 * d * 4 * 256) != hipSuccess)
+        rc = 5;
     if (!rc) rc = vx_upload(&p->d_vnet, vn);
     if (!rc) rc = vx_upload(&p->d_vlist, vlist);
     if (!rc) rc = vx_upload(&p->d_gen_slots, gen_slots);
@@ -161,6 +206,7 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     }
     p->d = d;
     p->nv = nx;
+    p->nv16 = n16;
     p->gen_slots = gen_slots;
     p->ngen = (int)gen_slots.size();
     p->all = gen_slots.empty();
@@ -255,6 +301,7 @@ static int vertex_upload_sites(VertexPlan *p, hipStream_t st, const double *d_si
     const size_t bytes = (p->bf16 ? 2 : 4) * (size_t)p->nv * p->d * 4096;
     if (*d_vsites) (void)hipFree(*d_vsites);
     *d_vsites = nullptr;
+    if (p->nv == 0) return 0;
     if (hipMalloc(d_vsites, bytes) != hipSuccess) return 5;
     hipLaunchKernelGGL(k_vertex_pad_sites, dim3(2048), dim3(256), 0, st, d_sites64, d_site_off,
                        d_port_off, d_port_edge, d_edge_dim, p->d_vlist, p->d, p->nv, p->bf16,
@@ -391,22 +438,15 @@ __device__ __forceinline__ float vx_reduce64(float (&v)[64]) {
 //   LO (open port = register lo index): out[a][a'] = sum_hi G[hi*8+a] A[hi*8+a']
 // XL: the A buffer is read in layout X (else Y).  Returns entry (lane/8, lane%8).  In BF16
 // mode the partial product G is rounded to BF16 first (the buffer already holds BF16 values).
-template <bool HI, bool XL, bool BF>
-__device__ __forceinline__ float vx_final(const float (&Gin)[64], const float *bufA) {
+template <bool HI, bool XL, int PITCH>
+__device__ __forceinline__ float vx_final_acc(const float (&G)[64], const float *bufA) {
     const int lane = threadIdx.x & 63;
-    float G[64];
-#pragma unroll
-    for (int i = 0; i < 64; ++i) G[i] = Gin[i];
-    if constexpr (BF) {
-#pragma unroll
-        for (int i = 0; i < 64; i += 2) bf16_round2(G[i], G[i + 1]);
-    }
     float acc[64];
 #pragma unroll
     for (int i = 0; i < 64; ++i) acc[i] = 0.f;
 #pragma unroll
     for (int r = 0; r < 64; ++r) {
-        const float av = XL ? bufA[r * VX_PITCH + lane] : bufA[lane * VX_PITCH + r];
+        const float av = XL ? bufA[r * PITCH + lane] : bufA[lane * PITCH + r];
         const int hi = r >> 3, lo = r & 7;
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
@@ -415,6 +455,22 @@ __device__ __forceinline__ float vx_final(const float (&Gin)[64], const float *b
         }
     }
     return vx_reduce64(acc);
+}
+
+template <bool HI, bool XL, bool BF, int PITCH = VX_PITCH>
+__device__ __forceinline__ float vx_final(const float (&Gin)[64], const float *bufA) {
+    if constexpr (BF) {
+        float G[64];
+#pragma unroll
+        for (int i = 0; i < 64; i += 2) {
+            G[i] = Gin[i];
+            G[i + 1] = Gin[i + 1];
+            bf16_round2(G[i], G[i + 1]);
+        }
+        return vx_final_acc<HI, XL, PITCH>(G, bufA);
+    } else {
+        return vx_final_acc<HI, XL, PITCH>(Gin, bufA);
+    }
 }
 
 // ---------------------------------------------------------------- the sweep kernel
@@ -553,10 +609,159 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
     }
 }
 
+// ---------------------------------------------------------------- low-LDS variant (FP32)
+//
+// The same sweep with one LDS buffer per wave instead of two (36 KB per two-wave workgroup,
+// so four workgroups fit a CU) and a 256-register budget, i.e. two waves per SIMD: the
+// buffer first carries the E01 transpose, then a copy of A for phase A's bra (layout Y) and
+// phase B's ket (layout Y), then the E23 transpose; phase B's bra (layout X) is read from
+// the site array itself, one coalesced 256-byte row per register, mostly from L2.
+template <int NS>
+__global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__ mold,
+                    float *__restrict__ mnew, const int4 *__restrict__ vin,
+                    const int4 *__restrict__ vout, const int4 *__restrict__ vdim,
+                    const int *__restrict__ vnet, const long long *__restrict__ msg_off,
+                    NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs) {
+    extern __shared__ __align__(16) float vsm[];
+    const int u = blockIdx.x;
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    float *buf = vsm + w * VX_BUF;
+    float *msm = vsm + NS * VX_BUF;      // 4 x 64 incoming messages
+    float *part = msm + 4 * 64;          // [s][port][64] per-slice results
+    const int4 vi = vin[u], vo = vout[u], vdm = vdim[u];
+    const int ins[4] = {vi.x, vi.y, vi.z, vi.w};
+    const int outs[4] = {vo.x, vo.y, vo.z, vo.w};
+    const int dims[4] = {vdm.x, vdm.y, vdm.z, vdm.w};
+    long long ino[4], outo[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        ino[j] = ins[j] >= 0 ? msg_off[ins[j]] : 0;
+        outo[j] = outs[j] >= 0 ? msg_off[outs[j]] : 0;
+    }
+    const int la = lane >> 3, lb = lane & 7;
+    const int net = vnet[u];
+    if (ctl.enabled && ctl.done[net]) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j % NS != w || outs[j] < 0) continue;
+            if (lane < dims[j] * dims[j]) mnew[outo[j] + lane] = mold[outo[j] + lane];
+            if (lane == 0) { res_fid[outs[j]] = 0.0; res_abs[outs[j]] = 0.0; }
+        }
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j % NS != w) continue;
+        const int D = dims[j];
+        float x;
+        if (ins[j] < 0) x = la == lb ? 1.f : 0.f;
+        else x = (la < D && lb < D) ? mold[ino[j] + la * D + lb] : 0.f;
+        msm[j * 64 + lane] = x;
+    }
+    const float *src = (const float *)vsites_v + ((long long)u * NS + w) * 4096;
+    float T[64], E[64], G[64];
+#pragma unroll
+    for (int r = 0; r < 64; ++r) T[r] = src[r * 64 + lane];
+    __syncthreads();
+    float out[4];
+
+    // ---- phase A: E01 (X) -> Y through the buffer; then A into the buffer for the bra
+    vx_mode<true, false>(T, E, msm + 0 * 64);
+    vx_mode<false, false>(E, G, msm + 1 * 64);
+#pragma unroll
+    for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = G[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) E[r] = buf[lane * VX_PITCH + r];  // E01 in Y
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = T[r];  // A, readable as X or Y
+    __syncthreads();
+    vx_mode<false, false>(E, G, msm + 3 * 64);
+    out[2] = vx_final<true, false, false>(G, buf);
+    vx_mode<true, false>(E, G, msm + 2 * 64);
+    out[3] = vx_final<false, false, false>(G, buf);
+
+    // ---- phase B: E23 (Y) -> X through the buffer; bra in X from the site array
+#pragma unroll
+    for (int r = 0; r < 64; ++r) T[r] = buf[lane * VX_PITCH + r];  // A in Y
+    vx_mode<true, false>(T, E, msm + 2 * 64);
+    vx_mode<false, false>(E, G, msm + 3 * 64);
+    __syncthreads();  // every lane has finished reading A from the buffer
+#pragma unroll
+    for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = G[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) E[r] = buf[r * VX_PITCH + lane];  // E23 in X
+    vx_mode<false, false>(E, G, msm + 1 * 64);
+    out[0] = vx_final<true, true, false, 64>(G, src);
+    vx_mode<true, false>(E, G, msm + 0 * 64);
+    out[1] = vx_final<false, true, false, 64>(G, src);
+
+    // ---- sum over s, normalise, residual, write (as k_vertex_d8); the slot data are read
+    // again here rather than kept live through the contractions
+#pragma unroll
+    for (int j = 0; j < 4; ++j) part[(w * 4 + j) * 64 + lane] = out[j];
+    __syncthreads();
+    const int4 vo2 = vout[u], vd2 = vdim[u];
+    const int outs2[4] = {vo2.x, vo2.y, vo2.z, vo2.w};
+    const int dims2[4] = {vd2.x, vd2.y, vd2.z, vd2.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int q = outs2[j];
+        if (j % NS != w || q < 0) continue;
+        float x = 0.f;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) x += part[(s * 4 + j) * 64 + lane];
+        const double S = wave_sum((double)x);
+        const double xn = S != 0.0 ? (double)x / S : (double)x;
+        const float xf = (float)xn;
+        const int D = dims2[j];
+        const bool real = la < D && lb < D;
+        const long long o = msg_off[q] + la * D + lb;
+        const double xo = real ? (double)mold[o] : 0.0;
+        if (real) mnew[o] = xf;
+        const double dn = wave_sum((double)xf * xo), nn = wave_sum((double)xf * (double)xf),
+                     oo = wave_sum(xo * xo);
+        const double mx = wave_max(fabs((double)xf - xo));
+        if (lane == 0) {
+            const double c = dn / sqrt(nn * oo);
+            res_fid[q] = 1.0 - c * c;
+            res_abs[q] = mx;
+        }
+    }
+}
+
+static size_t vertex_lo_lds(int ns) {
+    return sizeof(float) * ((size_t)ns * VX_BUF + 4 * 64 + (size_t)ns * 4 * 64);
+}
+
 static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *vsites,
                         const void *src, void *dst, const NetCtl &ctl, double *res_fid,
                         double *res_abs) {
     (void)tp;
+    if (p->nv == 0) return 0;
+    if (p->variant == 1 && !p->bf16) {
+        const size_t lds = vertex_lo_lds(p->d);
+        switch (p->d) {
+#define VX_LO(NS)                                                                              \
+    case NS:                                                                                   \
+        hipLaunchKernelGGL((k_vertex_d8_lo<NS>), dim3(p->nv), dim3(64 * NS), lds, st, vsites,  \
+                           (const float *)src, (float *)dst, p->d_vin, p->d_vout, p->d_vdim,   \
+                           p->d_vnet, tp.msg_off, ctl, res_fid, res_abs);                      \
+        break;
+            VX_LO(1)
+            VX_LO(2)
+            VX_LO(3)
+            VX_LO(4)
+#undef VX_LO
+            default:
+                return 4;
+        }
+        return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
     const int key = p->d * 2 + (p->bf16 ? 1 : 0);
     switch (key) {
 #define VX_CASE(NS, BF)                                                                      \

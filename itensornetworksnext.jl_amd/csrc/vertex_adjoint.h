@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "adjoint.h"
@@ -124,9 +125,14 @@ __device__ __forceinline__ VdSlots vd_slots(const int4 *vin, const int4 *vout, c
     return s;
 }
 
+// With Gn != nullptr, W holds lambda and the override is its cotangent through the sum
+// normalisation, (lambda - <lambda, G>/S) / S (G the unnormalised forward message, S its
+// sum; lambda unchanged where S = 0).
 template <int NS>
 __device__ __forceinline__ void vd_load_mats(const float *msgs, const float *W, const VdSlots &sl,
-                                             int trans, float *msm, float *wsm) {
+                                             int trans, float *msm, float *wsm,
+                                             const float *Gn = nullptr,
+                                             const double *Sn = nullptr) {
     const int w = threadIdx.x >> 6, e = threadIdx.x & 63;
     const int a = trans ? (e & 7) : (e >> 3), b = trans ? (e >> 3) : (e & 7);
 #pragma unroll
@@ -136,14 +142,28 @@ __device__ __forceinline__ void vd_load_mats(const float *msgs, const float *W, 
         const bool real = a < D && b < D;
         msm[j * 64 + e] = sl.ins[j] >= 0 ? (real ? msgs[sl.ino[j] + a * D + b] : 0.f)
                                          : (a == b ? 1.f : 0.f);
-        if (W) wsm[j * 64 + e] = (sl.outs[j] >= 0 && real) ? W[sl.outo[j] + a * D + b] : 0.f;
+        if (W) {
+            const bool have = sl.outs[j] >= 0 && real;
+            float x = have ? W[sl.outo[j] + a * D + b] : 0.f;
+            if (Gn) {
+                const float g = have ? Gn[sl.outo[j] + a * D + b] : 0.f;
+                const double dot = wave_sum((double)x * (double)g);
+                const double S = sl.outs[j] >= 0 ? Sn[sl.outs[j]] : 0.0;
+                if (S != 0.0) x = (float)(((double)x - dot / S) / S);
+            }
+            wsm[j * 64 + e] = x;
+        }
     }
 }
 
 // ---- message form.  to_in == 0: R_j goes to the out-slot of port j (raw, with its sum in
 // S_out and Z_u in z_out when given).  to_in == 1: add[in-slot] + R_j goes to the in-slot of
-// port j, with max|result - ref| and max|result| per slot in res_abs / res_ref when given.
-// `add` may be `out` (each in-slot is read and written by the same lane).
+// port j.  `add` may be `out` (each in-slot is read and written by the same lane).
+// Gn / Sn: W is lambda, overrides are its normalisation cotangent (see vd_load_mats).
+// resacc (adjoint sweeps): max|result - ref| and max|result| of the workgroup go, as the
+// bit patterns of non-negative doubles, into resacc[rslot][0|1][blockIdx % 64] by atomic
+// max; workgroup 0 clears the other slot for the next sweep (its values were read by the
+// host before this launch).
 template <int NS, bool HASW>
 __global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
@@ -153,8 +173,9 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
               const int *__restrict__ vlist, const long long *__restrict__ msg_off, int to_in,
               const float *add, float *out,
               const float *__restrict__ ref, double *__restrict__ S_out,
-              double *__restrict__ z_out, double *__restrict__ res_abs,
-              double *__restrict__ res_ref) {
+              double *__restrict__ z_out, const float *__restrict__ Gn,
+              const double *__restrict__ Sn, unsigned long long *__restrict__ resacc,
+              int rslot) {
     extern __shared__ __align__(16) float vsm[];
     const int u = blockIdx.x;
     const int w = threadIdx.x >> 6;
@@ -167,7 +188,8 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
     const VdSlots sl = vd_slots(vin, vout, vdim, msg_off, u);
     const int vid = vlist[u];
     const float cv = (float)(cv_vertex ? cv_vertex[vid] : cv_const);
-    vd_load_mats<NS>(msgs, HASW ? W : nullptr, sl, 0, msm, wsm);
+    vd_load_mats<NS>(msgs, HASW ? W : nullptr, sl, 0, msm, wsm, Gn, Sn);
+    if (resacc && blockIdx.x == 0 && threadIdx.x < 128) resacc[(rslot ^ 1) * 128 + threadIdx.x] = 0ull;
     const int ks = op == OP_X ? (w ^ 1) : w;
     const float sg = (op == OP_Z && w == 1) ? -1.f : 1.f;
     float R1[64], R2[64], R3[64];
@@ -198,14 +220,22 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
             for (int r = 0; r < 64; ++r) R3[r] += cv * R1[r];
         }
     }
+    // both operands of the open contractions first, so E01 and its tangent die before
+    // the (register-hungry) contractions: G2 -> R2, G3 -> R3 (or R1 without overrides)
     vx_load_m(msm + 3 * 64, M);
     if constexpr (HASW) { vx_load_m(wsm + 3 * 64, Wm); vd_mp2<false>(R3, M, R1, Wm, R2); }
     else vd_mp1<false>(R1, M, R2);
-    o[2] = vx_final<true, false, false>(R2, bufA);
     vx_load_m(msm + 2 * 64, M);
-    if constexpr (HASW) { vx_load_m(wsm + 2 * 64, Wm); vd_mp2<true>(R3, M, R1, Wm, R2); }
-    else vd_mp1<true>(R1, M, R2);
-    o[3] = vx_final<false, false, false>(R2, bufA);
+    if constexpr (HASW) {
+        vx_load_m(wsm + 2 * 64, Wm);
+        vd_mp2<true>(R3, M, R1, Wm, R3);
+        o[2] = vx_final<true, false, false>(R2, bufA);
+        o[3] = vx_final<false, false, false>(R3, bufA);
+    } else {
+        vd_mp1<true>(R1, M, R1);
+        o[2] = vx_final<true, false, false>(R2, bufA);
+        o[3] = vx_final<false, false, false>(R1, bufA);
+    }
 
     // ---- phase B: E23 (and its tangent) from the ket slice in Y -> X; out0, out1
     {
@@ -230,17 +260,24 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
     vx_load_m(msm + 1 * 64, M);
     if constexpr (HASW) { vx_load_m(wsm + 1 * 64, Wm); vd_mp2<false>(R3, M, R1, Wm, R2); }
     else vd_mp1<false>(R1, M, R2);
-    o[0] = vx_final<true, true, false>(R2, bufA);
     vx_load_m(msm + 0 * 64, M);
-    if constexpr (HASW) { vx_load_m(wsm + 0 * 64, Wm); vd_mp2<true>(R3, M, R1, Wm, R2); }
-    else vd_mp1<true>(R1, M, R2);
-    o[1] = vx_final<false, true, false>(R2, bufA);
+    if constexpr (HASW) {
+        vx_load_m(wsm + 0 * 64, Wm);
+        vd_mp2<true>(R3, M, R1, Wm, R3);
+        o[0] = vx_final<true, true, false>(R2, bufA);
+        o[1] = vx_final<false, true, false>(R3, bufA);
+    } else {
+        vd_mp1<true>(R1, M, R1);
+        o[0] = vx_final<true, true, false>(R2, bufA);
+        o[1] = vx_final<false, true, false>(R1, bufA);
+    }
 
     // ---- sum over slices, write
     const float scale = HASW ? sg : sg * cv;
 #pragma unroll
     for (int j = 0; j < 4; ++j) part[(w * 4 + j) * 64 + lane] = o[j] * scale;
     __syncthreads();
+    double rd = 0.0, rv = 0.0;  // this wave's residual maxima (resacc)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         if (j % NS != w) continue;
@@ -266,12 +303,29 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
             if (sl.ins[j] < 0) continue;
             const long long off = sl.ino[j] + ent;
             const float v = real ? x + (add ? add[off] : 0.f) : 0.f;
-            if (res_abs) {
-                const double dm = wave_max(real ? fabs((double)v - (double)ref[off]) : 0.0);
-                const double rm = wave_max(fabs((double)v));
-                if (lane == 0) { res_abs[sl.ins[j]] = dm; res_ref[sl.ins[j]] = rm; }
+            if (resacc) {
+                const double dm = real ? fabs((double)v - (double)ref[off]) : 0.0;
+                rd = (dm > rd || dm != dm) ? dm : rd;
+                rv = fmax(rv, fabs((double)v));
             }
             if (real) out[off] = v;
+        }
+    }
+    if (resacc) {  // uniform over the workgroup
+        rd = wave_max(rd);
+        rv = wave_max(rv);
+        __syncthreads();  // part[] is free again
+        double *red = reinterpret_cast<double *>(part);
+        if (lane == 0) { red[2 * w] = rd; red[2 * w + 1] = rv; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 1; i < NS; ++i) {
+                rd = (red[2 * i] > rd || red[2 * i] != red[2 * i]) ? red[2 * i] : rd;
+                rv = fmax(rv, red[2 * i + 1]);
+            }
+            unsigned long long *slot = resacc + rslot * 128 + (blockIdx.x & 63);
+            atomicMax(slot, (unsigned long long)__double_as_longlong(rd));
+            atomicMax(slot + 64, (unsigned long long)__double_as_longlong(rv));
         }
     }
 }
@@ -398,21 +452,6 @@ __global__ void k_vd_inv(const double *z, double *out, int n) {
     if (i < n) out[i] = 1.0 / z[i];
 }
 
-// out[0] = max(ra) / max(rr) over n slots (one workgroup)
-__global__ __launch_bounds__(BP_BLOCK) void k_vd_res(const double *ra, const double *rr, int n,
-                                                      double *out) {
-    __shared__ double red[BP_BLOCK / 64];
-    double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const double x = ra[i];
-        a = (x > a || x != x) ? x : a;
-        b = fmax(b, rr[i]);
-    }
-    a = block_max(a, red);
-    b = block_max(b, red);
-    if (threadIdx.x == 0) out[0] = b > 0.0 ? a / b : a;
-}
-
 // ---------------------------------------------------------------- host side
 
 static size_t vd_msg_lds(int ns) { return sizeof(float) * ((size_t)ns * 2 * VX_BUF + 512 + (size_t)ns * 256); }
@@ -453,8 +492,9 @@ static int vd_set_limits(int ns) {
 
 static int vd_msg(const VdCtx &c, const float *msgs, const float *W, int op, double cv_const,
                   const double *cv_vertex, int to_in, const float *add, float *out,
-                  const float *ref, double *S_out, double *z_out, double *res_abs,
-                  double *res_ref) {
+                  const float *ref, double *S_out, double *z_out, const float *Gn = nullptr,
+                  const double *Sn = nullptr, unsigned long long *resacc = nullptr,
+                  int rslot = 0) {
     const VertexPlan &p = *c.vp;
     const dim3 grid(p.nv), block(64 * p.d);
     const size_t lds = vd_msg_lds(p.d);
@@ -464,7 +504,7 @@ static int vd_msg(const VdCtx &c, const float *msgs, const float *W, int op, dou
     hipLaunchKernelGGL((k_vd_msg<NS, HW>), grid, block, lds, c.st, c.vsites, msgs, W, op,      \
                        cv_const, cv_vertex, p.d_vin, p.d_vout, p.d_vdim, p.d_vlist,            \
                        c.tp.msg_off, to_in,                                                    \
-                       add, out, ref, S_out, z_out, res_abs, res_ref)
+                       add, out, ref, S_out, z_out, Gn, Sn, resacc, rslot)
     switch (p.d * 2 + (hw ? 1 : 0)) {
         case 2: VD_MSG(1, false); break;
         case 3: VD_MSG(1, true); break;
@@ -594,13 +634,9 @@ static int vadj_tfim(AdjointScratch *s, const VdCtx &c, int ne, const float *msg
     float *nxt = (float *)s->alloc(sizeof(float) * nmsg);
     float *gg = (float *)s->alloc(sizeof(float) * nmsg);
     float *da = direct ? (float *)go.dev : (float *)s->alloc(sizeof(float) * nsite);
-    double *ra = (double *)s->alloc(sizeof(double) * std::max(np, 1));
-    double *rr = (double *)s->alloc(sizeof(double) * std::max(np, 1));
-    double *res = (double *)s->alloc(sizeof(double));
+    unsigned long long *acc = (unsigned long long *)s->alloc(sizeof(unsigned long long) * 256);
     double *g64 = go.host ? (double *)s->alloc(sizeof(double) * nsite) : nullptr;
-    if (!dm || !WI || !WZ || !lam || !nxt || !gg || !da || !ra || !rr || !res ||
-        (go.host && !g64))
-        VD_RET(5);
+    if (!dm || !WI || !WZ || !lam || !nxt || !gg || !da || !acc || (go.host && !g64)) VD_RET(5);
     if (hipMemsetAsync(da, 0, sizeof(float) * nsite, c.st) != hipSuccess) VD_RET(2);
     // explicit dE/dm: vertex terms, then the edge terms (port t carries the other side's G / GZ)
     if (np > 0) {
@@ -621,16 +657,26 @@ static int vadj_tfim(AdjointScratch *s, const VdCtx &c, int ne, const float *msg
                        (const float *)dm, lam, nmsg);
     int it = 0;
     double r = INFINITY;
+    if (hipMemsetAsync(acc, 0, sizeof(unsigned long long) * 256, c.st) != hipSuccess) VD_RET(2);
+    std::vector<unsigned long long> hacc(128);
     while (np > 0 && it < adj_maxiter) {
-        hipLaunchKernelGGL(k_adj_gg<float>, dim3(np), dim3(64), 0, c.st, tp, (const float *)lam,
-                           (const float *)G, (const double *)S, gg);
-        VD_RET(vd_msg(c, msgs, gg, OP_I, 0.0, nullptr, 1, dm, nxt, lam, nullptr, nullptr, ra,
-                      rr));
-        hipLaunchKernelGGL(k_vd_res, dim3(1), dim3(BP_BLOCK), 0, c.st, (const double *)ra,
-                           (const double *)rr, np, res);
-        if (hipMemcpyAsync(&r, res, sizeof(double), hipMemcpyDeviceToHost, c.st) != hipSuccess ||
+        const int slot = it & 1;
+        // lambda' = dE/dm + J_m^T lambda, the normalisation cotangent formed in the kernel
+        VD_RET(vd_msg(c, msgs, lam, OP_I, 0.0, nullptr, 1, dm, nxt, lam, nullptr, nullptr, G, S,
+                      acc, slot));
+        if (hipMemcpyAsync(hacc.data(), acc + slot * 128, sizeof(unsigned long long) * 128,
+                           hipMemcpyDeviceToHost, c.st) != hipSuccess ||
             hipStreamSynchronize(c.st) != hipSuccess)
             VD_RET(2);
+        double ra = 0.0, rr = 0.0;
+        for (int i = 0; i < 64; ++i) {
+            double x, y;
+            std::memcpy(&x, &hacc[i], sizeof(double));
+            std::memcpy(&y, &hacc[64 + i], sizeof(double));
+            ra = (x > ra || x != x) ? x : ra;
+            rr = std::max(rr, y);
+        }
+        r = rr > 0.0 ? ra / rr : ra;
         std::swap(lam, nxt);
         ++it;
         if (r < adj_tol) break;

@@ -207,6 +207,12 @@ class Engine:
         return out
 
     # -- gradients
+    def set_option(self, name: str, value: int) -> None:
+        """Kernel options (bp_set_option): "vertex_variant" (0 or 1) and
+        "vertex_bond_mask" (bit D lets networks of largest bond dimension D use the vertex
+        kernels)."""
+        L.check(self.lib.bp_set_option(self.h, name.encode(), int(value)), "bp_set_option")
+
     def autotune(self, cache: str | None = None, mode: str = "missing") -> int:
         """Time the kernel choices on this engine's data and apply the fastest (see
         bp_autotune in include/bp_mi355x.h).  cache: JSON file of earlier decisions (read,

@@ -318,7 +318,7 @@ def test_bf16_generic_path_single_sweep(pkg, orc):
     t = pkg.build_topology(pkg.named_grid((4, 3)), 10, 2)
     s = pkg.peps.random_sites(t, 37, scale=pkg.peps.peps_scale(2, 10))
     m = pkg.peps.identity_messages(t)
-    with pkg.Engine(t, s, dtype="bf16") as e:
+    with pkg.Engine(t, s, dtype="bf16", path="generic") as e:
         assert e.path() == "generic"
         e.set_messages(m)
         e.sweep()
@@ -346,6 +346,7 @@ def test_bf16_d16_converged_then_fp64_polish(pkg, orc):
     assert at > 0
     lw = orc.bethe_free_energy(t, s, want)
     with pkg.Engine(t, s, dtype="bf16") as e:
+        assert e.path() == "vertex"  # the bond-dimension-16 MFMA kernel
         e.set_messages(m0)
         r = e.run(maxiter=200, tol=1e-5)
         m_bf = e.messages()
@@ -429,3 +430,92 @@ def test_vertex_padded_matches_generic_d6(pkg):
     assert out["auto"][2] == pytest.approx(out["generic"][2], rel=1e-6)
     assert out["auto"][3] == pytest.approx(out["generic"][3], rel=1e-6)
     assert _rel(out["auto"][4], out["generic"][4]) < 2e-4
+
+
+@pytest.mark.parametrize("shape,D,d", [((6, 5), 8, 2), ((4, 4), 8, 1), ((3, 5), 6, 3),
+                                       ((5, 4), 8, 4)])
+def test_vertex_low_lds_variant(pkg, orc, shape, D, d):
+    """The one-buffer-per-wave FP32 vertex kernel (two waves per SIMD) computes what the
+    default kernel computes: one sweep within the FP32 bound and 20 sweeps agreeing."""
+    _say(f"case vertex low-LDS variant {shape} D={D} d={d}")
+    t = pkg.build_topology(pkg.named_grid(shape), D, d)
+    s = pkg.peps.random_sites(t, 43, scale=pkg.peps.peps_scale(d, D))
+    m = pkg.peps.random_messages(t, 43)
+    res = {}
+    for v in (0, 1):
+        with pkg.Engine(t, s, dtype="f32") as e:
+            e.set_option("vertex_variant", v)
+            e.set_messages(m)
+            e.sweep()
+            one = e.messages()
+            assert _fp32_sweep_check(pkg, orc, t, s, m, one) < 1.0
+            e.init_messages("identity")
+            e.run(maxiter=20)
+            res[v] = (one, e.messages())
+    assert _rel(res[1][0], res[0][0]) < 1e-5
+    assert _rel(res[1][1], res[0][1]) < 1e-5
+
+
+# ---------------------------------------------------------------- bond dimensions 9..16 (BF16)
+
+@pytest.mark.parametrize("shape,D,d", [((4, 3), 16, 2), ((3, 3), 12, 2), ((3, 4), 10, 1),
+                                       ((3, 3), 16, 3)])
+def test_vertex16_single_sweep_within_bf16_bound(pkg, orc, shape, D, d):
+    """The BF16 MFMA kernel for bond dimensions 9..16 (padded to 16): one sweep from identity
+    messages within the BF16 error bound (gamma = 24 u, u = 2^-8)."""
+    _say(f"case vertex16 single sweep {shape} D={D} d={d}")
+    t = pkg.build_topology(pkg.named_grid(shape), D, d)
+    s = pkg.peps.random_sites(t, 47, scale=pkg.peps.peps_scale(d, D))
+    m = pkg.peps.identity_messages(t)
+    with pkg.Engine(t, s, dtype="bf16") as e:
+        assert e.path() == "vertex"
+        e.set_messages(m)
+        e.sweep()
+        got = e.messages()
+    assert _fp32_sweep_check(pkg, orc, t, s, m, got, gamma=24 * 2.0 ** -8) < 1.0
+    want, _, _ = orc.sweep_jacobi(t, s, m)
+    _say(f"D={D}: rel err {_rel(got, want):.2e}")
+
+
+def test_vertex16_random_messages_and_mixed_dims(pkg, orc):
+    """Non-symmetric random incoming messages and bond dimensions 9..16 within one network."""
+    _say("case vertex16 random messages, mixed bond dims")
+    g = pkg.named_grid((3, 4))
+    dims = {e: 9 + (i % 8) for i, e in enumerate(g.edges())}
+    t = pkg.build_topology(g, dims, 2)
+    s = pkg.peps.random_sites(t, 53)
+    m = pkg.peps.random_messages(t, 53)
+    with pkg.Engine(t, s, dtype="bf16") as e:
+        assert e.path() == "vertex"
+        e.set_messages(m)
+        e.sweep()
+        got = e.messages()
+    assert _fp32_sweep_check(pkg, orc, t, s, m, got, gamma=24 * 2.0 ** -8) < 1.0
+
+
+def test_bf16_batch_all_kernels(pkg, orc):
+    """One BF16 batch on the three kernels at once: D = 8 (vertex), D = 16 (vertex16) and
+    D = 20 (generic), converged with per-network masks against the FP64 oracle."""
+    _say("case bf16 batch across kernels")
+    specs = [((4, 4), 8), ((3, 3), 16), ((2, 3), 20)]
+    tops, sites = [], []
+    for k, (shape, D) in enumerate(specs):
+        t = pkg.build_topology(pkg.named_grid(shape), D, 2)
+        tops.append(t)
+        sites.append(pkg.peps.random_sites(t, 90 + k, scale=pkg.peps.peps_scale(2, D)))
+    b = pkg.concat_topologies(tops)
+    with pkg.Engine(b, np.concatenate(sites), dtype="bf16") as e:
+        assert e.path() == "mixed"
+        e.init_messages("identity")
+        r = e.run(maxiter=300, tol=1e-5, criterion="fidelity")
+        mb = e.messages()
+    off = 0
+    for i, (t, s) in enumerate(zip(tops, sites)):
+        want, _, _, at = orc.bp(t, s, pkg.peps.identity_messages(t), maxiter=500, tol=1e-13,
+                                criterion="maxabs", nthreads=8)
+        n = int(t.msg_off[-1])
+        assert r.converged[i]
+        err = _rel(mb[off:off + n], want)
+        _say(f"network {i} D={specs[i][1]}: sweeps {r.sweeps[i]} rel err {err:.2e}")
+        assert err < 5e-2
+        off += n
