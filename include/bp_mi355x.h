@@ -172,6 +172,8 @@ int bp_tfim_energy_device(bp_context *ctx, double J, double h, double *energy, v
 /* Kernel options (what the autotuner decides, settable by hand):
  *   "vertex_variant"   0: FP32 vertex sweep kernel with two LDS buffers per wave (default),
  *                      1: one buffer per wave and two waves per SIMD;
+ *   "vertex16_waves"   waves per workgroup of the BF16 kernel for bond dimensions 9..16
+ *                      (4, 8 or 16; default 16);
  *   "vertex_bond_mask" bit D set: networks whose largest bond dimension is D may run on the
  *                      vertex kernels (D 1..8; 9..16 in BF16 contexts; default 0x1FFFE).
  * No reference counterpart. */
@@ -180,7 +182,8 @@ int bp_set_option(bp_context *ctx, const char *name, int value);
 /* Autotuner.  Times, on the context's own sites and messages, (a) for every largest bond
  * dimension D <= 8 present (FP32/BF16 norm networks, automatic path) whether its networks
  * sweep faster on the vertex kernels (bonds padded to 8) or on the generic kernel, and
- * (b) the FP32 vertex sweep kernel variant ("vertex_variant" of bp_set_option), and
+ * (b) the FP32 vertex sweep kernel variant ("vertex_variant" of bp_set_option) and the
+ * waves per workgroup of the bond-dimension-16 kernel ("vertex16_waves"), and
  * (c) for every bucket of generic-kernel slots (owner's physical dimension and largest
  * bond dimension) the workgroup size (128/256/512) and whether intermediates live in LDS
  * (up to 64 KiB or up to 160 KiB) or in a global workspace.  mode 0: apply the choices found in `cache_path` only;

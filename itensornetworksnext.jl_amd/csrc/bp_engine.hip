@@ -281,6 +281,7 @@ struct bp_context {
     unsigned vx_mask = 0x1FFFEu;  // bit D: networks of largest bond dimension D may use it
     int vx_variant = 0;         // FP32 vertex sweep kernel (0: two LDS buffers per wave,
                                 // 1: one buffer per wave, two waves per SIMD)
+    int v16_waves = 16;         // waves per slice workgroup of the bond-dimension-16 kernel
 
     // generic-kernel buckets of the resolved path (rebuilt when the path or plan changes)
     std::vector<GenBucket> gb;
@@ -671,6 +672,7 @@ static int rebuild_vertex_plan(bp_context *c) {
         c->vplan.why = "could not raise the LDS limit of the vertex kernel";
     }
     c->vplan.variant = c->vx_variant;
+    c->vplan.v16_waves = c->v16_waves;
     if (c->vplan_ok && c->have_sites) RETCHK(upload_vertex_sites(c));
     return BP_OK;
 }
@@ -963,6 +965,13 @@ int bp_set_option(bp_context *c, const char *name, int value) {
         c->vplan.variant = value;
         return BP_OK;
     }
+    if (strcmp(name, "vertex16_waves") == 0) {
+        if (value != 4 && value != 8 && value != 16)
+            return set_err(BP_ERR_ARG, "vertex16_waves is 4, 8 or 16");
+        c->v16_waves = value;
+        c->vplan.v16_waves = value;
+        return BP_OK;
+    }
     if (strcmp(name, "vertex_bond_mask") == 0) {
         c->vx_mask = (unsigned)value & 0x1FFFEu;
         if (c->have_topo) RETCHK(rebuild_vertex_plan(c));
@@ -1077,6 +1086,8 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
 //   "vxk|f32|d<d>": {"variant": 0|1, "us": t_chosen, "us_alt": t_other}
 //       the FP32 vertex sweep kernel: two LDS buffers per wave, or one buffer per wave
 //       with two waves per SIMD;
+//   "v16w|bf16|d<d>": {"waves": 4|8|16, "us": t_best, "us_alt": t_worst}
+//       waves per slice workgroup of the bond-dimension-16 kernel;
 //   "gen|<dtype>|d<d>|D<D>": {"block": 128|256|512, "lds": 0|1|2, "us": t, "us_alt": t0}
 //       workgroup size and intermediate storage (global, LDS up to 64 KiB, LDS up to
 //       160 KiB) of the generic kernel for the bucket of slots whose owner has physical
@@ -1092,7 +1103,8 @@ static void tune_load(bp_context *c, const char *path) {
         const size_t b = line.find('"', a + 1);
         if (b == std::string::npos) continue;
         const std::string key = line.substr(a + 1, b - a - 1);
-        if (key.rfind("gen|", 0) != 0 && key.rfind("vx|", 0) != 0 && key.rfind("vxk|", 0) != 0)
+        if (key.rfind("gen|", 0) != 0 && key.rfind("vx|", 0) != 0 &&
+            key.rfind("vxk|", 0) != 0 && key.rfind("v16w|", 0) != 0)
             continue;
         TuneEntry e;
         auto num = [&](const char *name, double *out) {
@@ -1104,12 +1116,17 @@ static void tune_load(bp_context *c, const char *path) {
         };
         double v = 0.0;
         if (num("block", &v)) e.block = (int)v;
+        if (num("waves", &v)) e.block = (int)v;
         if (num("lds", &v)) e.lds = (int)v;
         if (num("vertex", &v)) e.vertex = (int)v;
         if (num("variant", &v)) e.vertex = (int)v;
         if (num("us", &v)) e.us = v;
         if (num("us_alt", &v)) e.us_alt = v;
-        if (e.block != 128 && e.block != 256 && e.block != 512) e.block = BP_BLOCK;
+        if (key.rfind("v16w|", 0) == 0) {
+            if (e.block != 4 && e.block != 8 && e.block != 16) e.block = 8;
+        } else if (e.block != 128 && e.block != 256 && e.block != 512) {
+            e.block = BP_BLOCK;
+        }
         c->tune[key] = e;
     }
 }
@@ -1124,6 +1141,7 @@ static int tune_save(const bp_context *c, const char *path) {
         f << "  \"" << kv.first << "\": {";
         if (kv.first.rfind("gen|", 0) == 0) f << "\"block\": " << e.block << ", \"lds\": " << e.lds;
         else if (kv.first.rfind("vxk|", 0) == 0) f << "\"variant\": " << e.vertex;
+        else if (kv.first.rfind("v16w|", 0) == 0) f << "\"waves\": " << e.block;
         else f << "\"vertex\": " << e.vertex;
         f << ", \"us\": " << e.us << ", \"us_alt\": " << e.us_alt << "}"
           << (++i < c->tune.size() ? "," : "") << "\n";
@@ -1262,6 +1280,35 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
         }
         if (it != c->tune.end()) c->vx_variant = it->second.vertex;
         c->vplan.variant = c->vx_variant;
+    }
+    // 1c. waves per slice workgroup of the bond-dimension-16 kernel (BF16)
+    if (rc == BP_OK && c->vplan_ok && c->vplan.nv16 > 0 && c->path != BP_PATH_GENERIC) {
+        std::ostringstream o;
+        o << "v16w|bf16|d" << c->vplan.d;
+        const std::string key = o.str();
+        auto it = c->tune.find(key);
+        if (it == c->tune.end() && mode >= 1) {
+            const int cand[3] = {4, 8, 16};
+            double best = 1e30, worst = 0.0;
+            int bw = c->v16_waves;
+            for (int k = 0; k < 3 && rc == BP_OK; ++k) {
+                c->vplan.v16_waves = cand[k];
+                const double t = tune_time_sweep(c, 3);
+                if (t < 0.0) { rc = set_err(BP_ERR_HIP, "autotuner sweep failed"); break; }
+                if (t < best) { best = t; bw = cand[k]; }
+                worst = std::max(worst, t);
+            }
+            if (rc == BP_OK) {
+                TuneEntry e;
+                e.block = bw;
+                e.us = 1e3 * best;
+                e.us_alt = 1e3 * worst;
+                it = c->tune.emplace(key, e).first;
+                ++tuned;
+            }
+        }
+        if (it != c->tune.end()) c->v16_waves = it->second.block;
+        c->vplan.v16_waves = c->v16_waves;
     }
     // 2. generic buckets: workgroup size and storage
     const int path = resolve_path(c);

@@ -69,6 +69,7 @@ struct VertexPlan {
     int *d_v16net = nullptr, *d_v16list = nullptr;
     unsigned short *d_v16sites = nullptr;  // padded BF16 slices, 16^4 entries each
     float *d_v16part = nullptr;            // per-slice unnormalised messages
+    int v16_waves = 16;                    // waves per slice workgroup: 4, 8 or 16
 };
 
 static constexpr int VX_D = 8;

@@ -1,6 +1,7 @@
 // vertex16.h -- BF16 MFMA sweep for bond dimensions 9..16 (padded to 16) on gfx950.
 //
-// One workgroup of four waves per (vertex, physical slice s).  The slice A[s] (four ports of
+// One workgroup of NW waves (4, 8 or 16; 16 by default, the autotuner's choice on the
+// 24x24 D=16 lattice) per (vertex, physical slice s).  The slice A[s] (four ports of
 // dimension 16, 16^4 entries) lives in LDS as BF16 (128 KiB), rows of 16 entries along the
 // last port, row r stored at row r ^ ((r>>3 ^ r>>6 ^ r>>9) & 7) so that accesses strided
 // along any port spread over the banks.
@@ -34,7 +35,7 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef short v8s __attribute__((ext_vector_type(8)));
 
 static constexpr int V16_ELEMS = 65536;                            // 16^4 per slice
-static constexpr size_t V16_LDS = V16_ELEMS * 2 + 4 * 256 * 4;      // slice + per-wave partials
+static constexpr size_t v16_lds(int nw) { return V16_ELEMS * 2 + (size_t)nw * 256 * 4; }
 
 __device__ __forceinline__ int v16_row(int r) { return r ^ (((r >> 3) ^ (r >> 6) ^ (r >> 9)) & 7); }
 
@@ -42,6 +43,10 @@ __device__ __forceinline__ int v16_row(int r) { return r ^ (((r >> 3) ^ (r >> 6)
 __device__ __forceinline__ int v16_off(int p0, int p1, int p2, int p3) {
     return v16_row((p0 * 16 + p1) * 16 + p2) * 32 + p3 * 2;
 }
+
+// v16_row is linear over xor, and a row index (p0 << 8 | p1 << 4 | p2) splits into fields
+// that depend on the lane and fields that depend only on the loop iteration, so
+// v16_off = v16_off(lane fields) ^ v16_off(iteration fields): one xor per access.
 
 __device__ __forceinline__ unsigned short v16_ld(const unsigned char *S, int off) {
     return *reinterpret_cast<const unsigned short *>(S + off);
@@ -60,24 +65,29 @@ template <int J> struct V16Ports {
 };
 
 // Mode product on port J of the slice in LDS, in place.  Am: the message as A operand.
-template <int J>
+template <int J, int NW>
 __device__ __forceinline__ void v16_mode(unsigned char *S, v4s Am) {
     using PT = V16Ports<J>;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = lane & 15, kg = lane >> 4;
     const v4f zero = {0.f, 0.f, 0.f, 0.f};
+    int loff[4];  // lane fields: port J = 4 kg + t, port R = n
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        int p[4] = {0, 0, 0, 0};
+        p[J] = 4 * kg + t;
+        p[PT::R] = n;
+        loff[t] = v16_off(p[0], p[1], p[2], p[3]);
+    }
 #pragma unroll 4
-    for (int g = w; g < 256; g += 4) {
+    for (int g = w; g < 256; g += NW) {
+        int q[4] = {0, 0, 0, 0};  // iteration fields: ports P, Q (never the last port)
+        q[PT::P] = g >> 4;
+        q[PT::Q] = g & 15;
+        const int goff = v16_off(q[0], q[1], q[2], 0);
         int off[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            int p[4];
-            p[J] = 4 * kg + t;
-            p[PT::R] = n;
-            p[PT::P] = g >> 4;
-            p[PT::Q] = g & 15;
-            off[t] = v16_off(p[0], p[1], p[2], p[3]);
-        }
+        for (int t = 0; t < 4; ++t) off[t] = loff[t] ^ goff;
         v4s b;
         if constexpr (J == 3) {  // the four entries are adjacent in one row
             const unsigned long long x = *reinterpret_cast<const unsigned long long *>(S + off[0]);
@@ -106,15 +116,16 @@ __device__ __forceinline__ void v16_mode(unsigned char *S, v4s Am) {
 // 16 values of p3, read from the natural layout `bra`; for J = 3, (p0, p1) pairs x all 16
 // values of p2, read from the copy `bra3` whose last two ports are swapped, so that each
 // lane's 8 K-values are 16 contiguous bytes in both cases.
-template <int J>
+template <int J, int NW>
 __device__ __forceinline__ void v16_bra_prefetch(const unsigned short *bra,
-                                                 const unsigned short *bra3, uint4 (&pre)[32]) {
+                                                 const unsigned short *bra3,
+                                                 uint4 (&pre)[128 / NW]) {
     using PT = V16Ports<J>;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = lane & 15, kg = lane >> 4;
 #pragma unroll
-    for (int i = 0; i < 32; ++i) {
-        const int t = w + 4 * i;
+    for (int i = 0; i < 128 / NW; ++i) {
+        const int t = w + NW * i;
         const int hi = 2 * t + (kg >> 1), lo8 = 8 * (kg & 1);
         int e;
         if constexpr (J == 3) {
@@ -133,30 +144,40 @@ __device__ __forceinline__ void v16_bra_prefetch(const unsigned short *bra,
 
 // This wave's share of out_J[a][a'] = sum_K G[a, K] A[a', K] with the bra operands of
 // v16_bra_prefetch; lane l returns entries (4(l/16)+i, l%16).
-template <int J>
-__device__ __forceinline__ v4f v16_open(const unsigned char *S, const uint4 (&pre)[32]) {
+template <int J, int NW>
+__device__ __forceinline__ v4f v16_open(const unsigned char *S, const uint4 (&pre)[128 / NW]) {
     using PT = V16Ports<J>;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = lane & 15, kg = lane >> 4;
     v4f acc = {0.f, 0.f, 0.f, 0.f};
+    // step t = w + NW i covers hi = 2 t + (kg >> 1) = 2 NW i | 2 w | (kg >> 1) (disjoint
+    // bits): lane fields kg, n; iteration fields i, w
+    int loff;
+    if constexpr (J == 3) {
+        loff = v16_off(0, kg >> 1, 8 * (kg & 1), n);
+    } else {
+        int p[4] = {0, 0, 0, 0};
+        p[J] = n;
+        p[PT::Q] = kg >> 1;
+        p[3] = 8 * (kg & 1);
+        loff = v16_off(p[0], p[1], p[2], p[3]);
+    }
 #pragma unroll
-    for (int i = 0; i < 32; ++i) {
-        const int t = w + 4 * i;
-        const int hi = 2 * t + (kg >> 1), lo8 = 8 * (kg & 1);
+    for (int i = 0; i < 128 / NW; ++i) {
+        const int it = 2 * NW * i + 2 * w;  // iteration part of hi
         uint4 ga;
         if constexpr (J == 3) {
             v8s a8;
-            const int p0 = hi >> 4, p1 = hi & 15;
+            const int ioff = v16_off(it >> 4, it & 15, 0, 0);
 #pragma unroll
-            for (int u = 0; u < 8; ++u) a8[u] = (short)v16_ld(S, v16_off(p0, p1, lo8 + u, n));
+            for (int u = 0; u < 8; ++u)
+                a8[u] = (short)v16_ld(S, (loff ^ ioff) ^ v16_off(0, 0, u, 0));
             ga = __builtin_bit_cast(uint4, a8);
         } else {
-            int p[4];
-            p[J] = n;
-            p[PT::P] = hi >> 4;
-            p[PT::Q] = hi & 15;
-            p[3] = lo8;
-            ga = *reinterpret_cast<const uint4 *>(S + v16_off(p[0], p[1], p[2], lo8));
+            int q[4] = {0, 0, 0, 0};
+            q[PT::P] = it >> 4;
+            q[PT::Q] = it & 15;
+            ga = *reinterpret_cast<const uint4 *>(S + (loff ^ v16_off(q[0], q[1], q[2], 0)));
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, ga),
                                                       __builtin_bit_cast(v8bf, pre[i]), acc, 0,
@@ -167,26 +188,27 @@ __device__ __forceinline__ v4f v16_open(const unsigned char *S, const uint4 (&pr
 
 // Copy the slice's padded BF16 entries into LDS (16-byte chunks, half a row each), eight
 // loads in flight per thread.
+template <int NW>
 __device__ __forceinline__ void v16_load_slice(unsigned char *S, const unsigned short *src) {
     const uint4 *g = reinterpret_cast<const uint4 *>(src);
 #pragma unroll 8
-    for (int i = 0; i < 32; ++i) {
-        const int c = threadIdx.x + 256 * i;
+    for (int i = 0; i < V16_ELEMS / 8 / (64 * NW); ++i) {
+        const int c = threadIdx.x + 64 * NW * i;
         *reinterpret_cast<uint4 *>(S + v16_row(c >> 1) * 32 + (c & 1) * 16) = g[c];
     }
 }
 
-template <int J>
+template <int J, int NW>
 __device__ __forceinline__ void v16_mode_if(unsigned char *S, const int (&ins)[4],
                                             const v4s (&Am)[4]) {
     if (ins[J] < 0) return;  // identity message on a padded port (uniform)
-    v16_mode<J>(S, Am[J]);
+    v16_mode<J, NW>(S, Am[J]);
     __syncthreads();
 }
 
 // One outgoing port O of the slice: reload, absorb ports Q0, Q1, Q2, open, reduce the four
 // waves' shares, store the unnormalised 16 x 16 result.
-template <int O, int Q0, int Q1, int Q2>
+template <int NW, int O, int Q0, int Q1, int Q2>
 __device__ __forceinline__ void v16_out(unsigned char *S, float *red, const unsigned short *src,
                                         const unsigned short *src_t, float *dst,
                                         const int (&ins)[4], const int (&outs)[4],
@@ -194,25 +216,30 @@ __device__ __forceinline__ void v16_out(unsigned char *S, float *red, const unsi
     if (outs[O] < 0) return;  // uniform over the workgroup
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = lane & 15, kg = lane >> 4;
-    uint4 pre[32];
-    v16_bra_prefetch<O>(src, src_t, pre);
+    uint4 pre[128 / NW];
+    v16_bra_prefetch<O, NW>(src, src_t, pre);
     __syncthreads();  // the previous open has finished reading the slice
-    v16_load_slice(S, src);
+    v16_load_slice<NW>(S, src);
     __syncthreads();
-    v16_mode_if<Q0>(S, ins, Am);
-    v16_mode_if<Q1>(S, ins, Am);
-    v16_mode_if<Q2>(S, ins, Am);
-    const v4f acc = v16_open<O>(S, pre);
+    v16_mode_if<Q0, NW>(S, ins, Am);
+    v16_mode_if<Q1, NW>(S, ins, Am);
+    v16_mode_if<Q2, NW>(S, ins, Am);
+    const v4f acc = v16_open<O, NW>(S, pre);
 #pragma unroll
     for (int i = 0; i < 4; ++i) red[w * 256 + (4 * kg + i) * 16 + n] = acc[i];
     __syncthreads();
-    for (int e = threadIdx.x; e < 256; e += blockDim.x)
-        dst[O * 256 + e] = red[e] + red[256 + e] + red[512 + e] + red[768 + e];
+    for (int e = threadIdx.x; e < 256; e += blockDim.x) {
+        float x = 0.f;
+#pragma unroll
+        for (int v = 0; v < NW; ++v) x += red[v * 256 + e];
+        dst[O * 256 + e] = x;
+    }
 }
 
-// grid: items x d (one workgroup per slice), 256 threads.  part[((item * d + s) * 4 + j) * 256
+// grid: items x d (one workgroup per slice), 64 NW threads.  part[((item * d + s) * 4 + j) * 256
 // + a * 16 + a'] receives slice s's unnormalised out_j.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+template <int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 4, NW / 4)))
 void k_vertex_d16(const unsigned short *__restrict__ vsites16,
                   const unsigned short *__restrict__ vsites16t, const float *__restrict__ mold,
                   const int4 *__restrict__ vin, const int4 *__restrict__ vout,
@@ -253,10 +280,10 @@ void k_vertex_d16(const unsigned short *__restrict__ vsites16,
     float *dst = part + ((long long)item * d + s) * 4 * 256;
     // outgoing port O: absorb the other three ports' messages (E01 order for O = 2, 3, E23
     // order for O = 0, 1), then open O
-    v16_out<0, 2, 3, 1>(S, red, src, src_t, dst, ins, outs, Am);
-    v16_out<1, 2, 3, 0>(S, red, src, src_t, dst, ins, outs, Am);
-    v16_out<2, 0, 1, 3>(S, red, src, src_t, dst, ins, outs, Am);
-    v16_out<3, 0, 1, 2>(S, red, src, src_t, dst, ins, outs, Am);
+    v16_out<NW, 0, 2, 3, 1>(S, red, src, src_t, dst, ins, outs, Am);
+    v16_out<NW, 1, 2, 3, 0>(S, red, src, src_t, dst, ins, outs, Am);
+    v16_out<NW, 2, 0, 1, 3>(S, red, src, src_t, dst, ins, outs, Am);
+    v16_out<NW, 3, 0, 1, 2>(S, red, src, src_t, dst, ins, outs, Am);
 }
 
 // grid: items, 256 threads (wave j = port j): sum the slices, normalise, residuals, write.
@@ -370,9 +397,13 @@ static int vertex16_upload_sites(VertexPlan *p, hipStream_t st, const double *d_
 static int vertex16_set_lds_limit() {
     static int done = 0;
     if (done) return 0;
-    if (hipFuncSetAttribute((const void *)k_vertex_d16, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)V16_LDS) != hipSuccess)
-        return 2;
+    const void *f[3] = {(const void *)k_vertex_d16<4>, (const void *)k_vertex_d16<8>,
+                        (const void *)k_vertex_d16<16>};
+    const int nw[3] = {4, 8, 16};
+    for (int i = 0; i < 3; ++i)
+        if (hipFuncSetAttribute(f[i], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)v16_lds(nw[i])) != hipSuccess)
+            return 2;
     done = 1;
     return 0;
 }
@@ -382,11 +413,17 @@ static int vertex16_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void 
                           const NetCtl &ctl, double *res_fid, double *res_abs) {
     if (p->nv16 == 0) return 0;
     const size_t half = (size_t)p->nv16 * p->d * V16_ELEMS;
-    hipLaunchKernelGGL(k_vertex_d16, dim3(p->nv16 * p->d), dim3(256), V16_LDS, st,
-                       (const unsigned short *)p->d_v16sites,
-                       (const unsigned short *)p->d_v16sites + half, (const float *)src, p->d_v16in,
-                       p->d_v16out, p->d_v16dim, p->d_v16net, tp.msg_off, ctl, p->d,
-                       p->d_v16part);
+    const unsigned short *s0 = p->d_v16sites, *s1 = p->d_v16sites + half;
+#define V16_LAUNCH(NW)                                                                         \
+    hipLaunchKernelGGL((k_vertex_d16<NW>), dim3(p->nv16 * p->d), dim3(64 * NW), v16_lds(NW),    \
+                       st, s0, s1, (const float *)src, p->d_v16in, p->d_v16out, p->d_v16dim,   \
+                       p->d_v16net, tp.msg_off, ctl, p->d, p->d_v16part)
+    switch (p->v16_waves) {
+        case 4: V16_LAUNCH(4); break;
+        case 8: V16_LAUNCH(8); break;
+        default: V16_LAUNCH(16); break;
+    }
+#undef V16_LAUNCH
     hipLaunchKernelGGL(k_v16_finish, dim3(p->nv16), dim3(256), 0, st, (const float *)p->d_v16part,
                        (const float *)src, (float *)dst, p->d_v16out, p->d_v16dim, p->d_v16net,
                        tp.msg_off, ctl, p->d, res_fid, res_abs);
