@@ -128,6 +128,51 @@ def uneven_batch_timing(pkg, torch, seed, sweeps=20):
     return out
 
 
+def large_bond_timing(pkg, torch, seed, L=24, D=16):
+    """The mixed-precision workload at large bond dimension: BF16 MFMA sweeps (FP32 sums,
+    per-message max-norm scaling) to fidelity 1e-5, then 3 FP64 polishing sweeps; the
+    message and log-norm change from each stage to the next is reported (the oracle check
+    of both stages against FP64 runs on 5x5 at this bond dimension in the tests)."""
+    import time
+
+    t = pkg.build_topology(pkg.named_grid((L, L)), D, 2)
+    s = pkg.peps.random_sites(t, seed, scale=pkg.peps.peps_scale(2, D))
+    e = pkg.Engine(t, s, dtype="bf16")
+    e.init_messages("identity")
+    e.autotune(mode="all")
+    kms = e.time_sweeps(10)
+    e.init_messages("identity")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = e.run(maxiter=200, tol=1e-5)
+    torch.cuda.synchronize()
+    t_bf = time.perf_counter() - t0
+    m_bf, ln_bf = e.messages(), e.lognorm()
+    t1 = time.perf_counter()
+    e.polish_fp64(3)
+    t_pol = time.perf_counter() - t1
+    m_pol, ln_pol = e.messages(), e.lognorm()
+    e.polish_fp64(3)
+    m_pol2, ln_pol2 = e.messages(), e.lognorm()
+    flops = algorithmic_flops_per_sweep(t)
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    out = {"workload": f"{L}x{L} PEPS norm network, d=2, D={D}, BF16 MFMA operands / FP32 sums, "
+                       f"then 3 FP64 polishing sweeps",
+           "path": e.path(), "kernel_ms_per_sweep": kms,
+           "msg_per_s_kernel": t.nports / (kms * 1e-3),
+           "algorithmic_TFLOPs": flops / (kms * 1e-3) / 1e12,
+           "bf16_sweeps": int(r.sweeps[0]), "bf16_s": t_bf, "polish3_s": t_pol,
+           "msg_change_bf16_to_polished": rel(m_bf, m_pol),
+           "msg_change_polished_3_to_6": rel(m_pol, m_pol2),
+           "lognorm_change_bf16_to_polished": abs(ln_bf - ln_pol) / abs(ln_pol),
+           "lognorm_change_polished_3_to_6": abs(ln_pol - ln_pol2) / abs(ln_pol2)}
+    e.close()
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,9 +186,10 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--only", choices=["adjoint"], default=None,
+    ap.add_argument("--only", choices=["adjoint", "large_bond"], default=None,
                     help="adjoint: run only the forward + adjoint workload (L x L, D; three "
-                         "energy + gradient calls) and print its timings, for profiling")
+                         "energy + gradient calls); large_bond: only the 24x24 D=16 BF16 + "
+                         "FP64-polish workload; print the timings, for profiling")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, used to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -164,6 +210,11 @@ def main(argv=None):
         runs = [adjoint_timing(pkg, torch, args.L, args.D, args.dtype, args.seed)
                 for _ in range(3)]
         print(json.dumps({"adjoint_only": runs}), flush=True)
+        return 0
+    if args.only == "large_bond":
+        torch.cuda.init()
+        print(json.dumps({"large_bond_only": large_bond_timing(pkg, torch, args.seed)}),
+              flush=True)
         return 0
     L, D, d = args.L, args.D, 2
     g = pkg.named_grid((L, L))
@@ -269,6 +320,7 @@ def main(argv=None):
 
     if not args.no_extras and args.dtype == "f32":
         extras["uneven_batch"] = uneven_batch_timing(pkg, torch, args.seed)
+        extras["large_bond_bf16"] = large_bond_timing(pkg, torch, args.seed)
 
     cpu = None
     if not args.no_cpu:
