@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <fstream>
 #include <map>
@@ -25,6 +26,11 @@
 #include "vertex16.h"
 #include "adjoint.h"
 #include "vertex_adjoint.h"
+
+// Process-wide plan epoch: moves on every device (re)allocation and every change of a
+// kernel choice, so that a cached sweep graph never replays stale pointers.
+static std::atomic<unsigned> g_epoch{1};
+static void bump_epoch() { g_epoch.fetch_add(1); }
 
 // ------------------------------------------------------------------ error state
 
@@ -96,12 +102,24 @@ __global__ void k_reset_nets(int nnet, int *done, int *iter, double *delta, int 
 // network): StopWhenConverged on `res` (fidelity or max-abs per slot) and
 // StopAfterIteration(maxiter) (AlgorithmsInterfaceExtensions.jl:84-109,
 // beliefpropagation.jl:27-55).
-__global__ __launch_bounds__(1024) void k_reduce_nets(const int *pbeg, const double *res,
-                                                      int maxiter, double tol, int *done,
-                                                      int *iter, double *delta, int *at) {
+// Stopping parameters of a run, read from device memory so that a captured sweep sequence
+// serves every (maxiter, tol, criterion).
+struct RunParams {
+    int maxiter;
+    int crit;  // BP_CRIT_FIDELITY or BP_CRIT_MAXABS
+    double tol;
+};
+
+__global__ __launch_bounds__(1024) void k_reduce_nets(const int *pbeg, const double *res_fid,
+                                                      const double *res_abs,
+                                                      const RunParams *rp, int *done, int *iter,
+                                                      double *delta, int *at) {
     __shared__ double red[16];
     const int n = blockIdx.x;
     if (done[n]) return;  // uniform over the workgroup
+    const int maxiter = rp->maxiter;
+    const double tol = rp->tol;
+    const double *res = rp->crit == BP_CRIT_FIDELITY ? res_fid : res_abs;
     double m = 0.0;
     for (int q = pbeg[n] + threadIdx.x; q < pbeg[n + 1]; q += blockDim.x) {
         const double r = res[q];
@@ -283,6 +301,14 @@ struct bp_context {
                                 // 1: one buffer per wave, two waves per SIMD)
     int v16_waves = 16;         // waves per slice workgroup of the bond-dimension-16 kernel
 
+    // cached sweep graphs (16 sweeps + bookkeeping), one per starting buffer parity; valid
+    // while graph_epoch equals the process-wide plan epoch (which moves whenever a captured
+    // pointer or kernel choice may have changed)
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    unsigned graph_epoch = 0;
+    RunParams *d_runp = nullptr;
+    RunParams runp_host{};
+
     // generic-kernel buckets of the resolved path (rebuilt when the path or plan changes)
     std::vector<GenBucket> gb;
     int gb_path = -1;
@@ -324,6 +350,7 @@ static int upload(T **dst, const std::vector<T> &v) {
 }
 
 static int dev_alloc(void **p, size_t bytes) {
+    bump_epoch();
     if (*p) HIPCHK(hipFree(*p));
     *p = nullptr;
     if (bytes == 0) bytes = 16;
@@ -331,13 +358,24 @@ static int dev_alloc(void **p, size_t bytes) {
     return BP_OK;
 }
 
+static size_t run_state_bytes(int nnet) { return (sizeof(double) + 3 * sizeof(int)) * nnet; }
+
+static void graphs_release(bp_context *c) {
+    for (auto &g : c->gexec) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    c->graph_epoch = 0;
+}
+
 static void free_all(bp_context *c) {
+    graphs_release(c);
     void *ptrs[] = {c->d_port_off, c->d_port_nbr, c->d_port_edge, c->d_port_rev, c->d_edge_dim,
                     c->d_phys_dim, c->d_owner, c->d_vnet, c->d_vbeg, c->d_pbeg, c->d_site_off,
                     c->d_msg_off, c->d_sites64, c->d_sites, c->d_msg[0], c->d_msg[1],
                     c->d_msg64[0], c->d_msg64[1], c->d_res_fid, c->d_res_abs, c->d_zv, c->d_ze,
-                    c->d_lre, c->d_lim, c->d_done, c->d_iter, c->d_at, c->d_delta, c->d_ws,
-                    c->d_vsites, c->d_sites_bf};
+                    c->d_lre, c->d_lim, c->d_delta, c->d_ws,
+                    c->d_vsites, c->d_sites_bf, c->d_runp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->adj.release();
@@ -452,6 +490,7 @@ static std::string bucket_key(int dtype, const GenBucket &b) {
 }
 
 static void free_buckets(bp_context *c) {
+    bump_epoch();
     for (auto &b : c->gb)
         if (b.d_slots) (void)hipFree(b.d_slots);
     c->gb.clear();
@@ -605,10 +644,10 @@ static NetCtl make_ctl(bp_context *c, bool enabled) {
 }
 
 // Per-network bookkeeping after a sweep: one workgroup of 1024 threads per network.
-static int launch_reduce(bp_context *c, int crit, int maxiter, double tol) {
-    const double *res = crit == BP_CRIT_FIDELITY ? c->d_res_fid : c->d_res_abs;
-    hipLaunchKernelGGL(k_reduce_nets, dim3(c->nnet), dim3(1024), 0, c->stream, c->d_pbeg, res,
-                       maxiter, tol, c->d_done, c->d_iter, c->d_delta, c->d_at);
+static int launch_reduce(bp_context *c) {
+    hipLaunchKernelGGL(k_reduce_nets, dim3(c->nnet), dim3(1024), 0, c->stream, c->d_pbeg,
+                       c->d_res_fid, c->d_res_abs, c->d_runp, c->d_done, c->d_iter, c->d_delta,
+                       c->d_at);
     HIPCHK(hipGetLastError());
     return BP_OK;
 }
@@ -648,6 +687,7 @@ static int sweep_once(bp_context *c, const NetCtl &done) {
 
 // Padded site layouts of both vertex buckets (bond dimension <= 8, and 9..16 in BF16).
 static int upload_vertex_sites(bp_context *c) {
+    bump_epoch();
     if (vertex_upload_sites(&c->vplan, c->stream, c->d_sites64, &c->d_vsites, c->d_site_off,
                             c->d_port_off, c->d_port_edge, c->d_edge_dim) != BP_OK ||
         vertex16_upload_sites(&c->vplan, c->stream, c->d_sites64, c->d_site_off, c->d_port_off,
@@ -660,6 +700,7 @@ static int upload_vertex_sites(bp_context *c) {
 // Vertex-path plan for the current mask (may decline; the reason is kept for bp_set_path
 // errors), with the padded site layout when sites are present; generic buckets follow.
 static int rebuild_vertex_plan(bp_context *c) {
+    bump_epoch();
     vertex_plan_release(&c->vplan);
     free_buckets(c);
     c->vplan_ok = vertex_plan_build(&c->vplan, c->dtype, c->plain, c->nv, c->port_off.data(),
@@ -703,6 +744,11 @@ int bp_create(bp_context **out, int device, int dtype) {
         return set_err(BP_ERR_HIP, "stream/event creation failed");
     }
     c->stream = c->own_stream;
+    if (dev_alloc((void **)&c->d_runp, sizeof(RunParams)) != BP_OK) {
+        const std::string e = g_err;
+        (void)bp_destroy(c);
+        return set_err(BP_ERR_HIP, "%s", e.c_str());
+    }
     *out = c;
     return BP_OK;
 }
@@ -841,10 +887,11 @@ int bp_set_topology(bp_context *c, int nv, int ne, const int *port_off, const in
     RETCHK(dev_alloc((void **)&c->d_ze, sizeof(double) * std::max(ne, 1)));
     RETCHK(dev_alloc((void **)&c->d_lre, sizeof(double) * nnet));
     RETCHK(dev_alloc((void **)&c->d_lim, sizeof(double) * nnet));
-    RETCHK(dev_alloc((void **)&c->d_done, sizeof(int) * nnet));
-    RETCHK(dev_alloc((void **)&c->d_iter, sizeof(int) * nnet));
-    RETCHK(dev_alloc((void **)&c->d_at, sizeof(int) * nnet));
-    RETCHK(dev_alloc((void **)&c->d_delta, sizeof(double) * nnet));
+    // per-network run state in one block (one copy back per run): delta, done, iter, at
+    RETCHK(dev_alloc((void **)&c->d_delta, run_state_bytes(nnet)));
+    c->d_done = (int *)(c->d_delta + nnet);
+    c->d_iter = c->d_done + nnet;
+    c->d_at = c->d_iter + nnet;
     RETCHK(dev_alloc((void **)&c->d_sites64, sizeof(double) * c->nsite));
     RETCHK(dev_alloc(&c->d_sites, es * c->nsite));
     if (c->dtype == BP_BF16) RETCHK(dev_alloc((void **)&c->d_sites_bf, sizeof(float) * c->nsite));
@@ -953,12 +1000,14 @@ int bp_set_path(bp_context *c, int path) {
     if (path == BP_PATH_VERTEX && !(c->vplan_ok && c->vplan.all))
         return set_err(BP_ERR_UNSUPPORTED, "vertex path not available for every network: %s",
                        c->vplan.why.c_str());
+    bump_epoch();
     c->path = path;
     return BP_OK;
 }
 
 int bp_set_option(bp_context *c, const char *name, int value) {
     if (!c || !name) return set_err(BP_ERR_ARG, "NULL argument");
+    bump_epoch();
     if (strcmp(name, "vertex_variant") == 0) {
         if (value != 0 && value != 1) return set_err(BP_ERR_ARG, "vertex_variant is 0 or 1");
         c->vx_variant = value;
@@ -1016,32 +1065,52 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
     c->fp64_active = false;
     RETCHK(prepare_sweep(c));
     const int cap = maxiter > 0 ? maxiter : 100000;  // tol-only runs are capped
+    {
+        RunParams hp;
+        hp.maxiter = cap;
+        hp.crit = criterion;
+        hp.tol = tol;
+        c->runp_host = hp;  // stays alive until the copy below has run (context lifetime)
+        HIPCHK(hipMemcpyAsync(c->d_runp, &c->runp_host, sizeof(RunParams),
+                              hipMemcpyHostToDevice, c->stream));
+    }
     hipLaunchKernelGGL(k_reset_nets, dim3((c->nnet + 255) / 256), dim3(256), 0, c->stream,
                        c->nnet, c->d_done, c->d_iter, c->d_delta, c->d_at);
     const NetCtl ctl = make_ctl(c, true);
     HIPCHK(hipGetLastError());
     const int chunk = 16;  // sweeps between host checks of the done flags (even)
-    std::vector<int> done(c->nnet);
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
+    // run state copied back after every chunk: delta, done, iter, at (one block)
+    const int nn = c->nnet;
+    std::vector<char> st(run_state_bytes(nn));
+    const int *done = (const int *)(st.data() + sizeof(double) * nn);
+    if (c->graph_epoch != g_epoch.load()) graphs_release(c);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     int launched = 0;
     while (launched < cap) {
         const int n = std::min(chunk, cap - launched);
         if (use_graph && n == chunk) {
+            hipGraphExec_t &gexec = c->gexec[c->cur];
             if (!gexec) {
+                hipGraph_t graph = nullptr;
                 const int cur0 = c->cur;
                 HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                 int rc = BP_OK;
                 for (int i = 0; i < n && rc == BP_OK; ++i) {
                     rc = sweep_once(c, ctl);
-                    if (rc == BP_OK) rc = launch_reduce(c, criterion, cap, tol);
+                    if (rc == BP_OK) rc = launch_reduce(c);
                 }
                 hipError_t e = hipStreamEndCapture(c->stream, &graph);
                 if (rc != BP_OK) return rc;
                 if (e != hipSuccess) return set_err(BP_ERR_HIP, "graph capture failed: %s",
                                                     hipGetErrorString(e));
-                HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+                const hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (ie != hipSuccess) {
+                    gexec = nullptr;
+                    return set_err(BP_ERR_HIP, "graph instantiate failed: %s",
+                                   hipGetErrorString(ie));
+                }
+                c->graph_epoch = g_epoch.load();
                 c->cur = cur0;  // capture does not execute; replay flips parity n times
             }
             HIPCHK(hipGraphLaunch(gexec, c->stream));
@@ -1049,30 +1118,31 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
         } else {
             for (int i = 0; i < n; ++i) {
                 RETCHK(sweep_once(c, ctl));
-                RETCHK(launch_reduce(c, criterion, cap, tol));
+                RETCHK(launch_reduce(c));
             }
         }
         launched += n;
-        HIPCHK(hipMemcpyAsync(done.data(), c->d_done, sizeof(int) * c->nnet,
-                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipEventRecord(c->ev1, c->stream));  // the last record ends the timed span
+        HIPCHK(hipMemcpyAsync(st.data(), c->d_delta, st.size(), hipMemcpyDeviceToHost,
+                              c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         bool all = true;
-        for (int x : done) all = all && x;
+        for (int i = 0; i < nn; ++i) all = all && done[i];
         if (all) break;
     }
-    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    if (launched == 0) HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipEventSynchronize(c->ev1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
-    if (out_sweeps)
-        HIPCHK(hipMemcpy(out_sweeps, c->d_iter, sizeof(int) * c->nnet, hipMemcpyDeviceToHost));
-    if (out_delta)
-        HIPCHK(hipMemcpy(out_delta, c->d_delta, sizeof(double) * c->nnet, hipMemcpyDeviceToHost));
-    if (out_at)
-        HIPCHK(hipMemcpy(out_at, c->d_at, sizeof(int) * c->nnet, hipMemcpyDeviceToHost));
+    if (launched == 0)
+        HIPCHK(hipMemcpy(st.data(), c->d_delta, st.size(), hipMemcpyDeviceToHost));
+    const char *p = st.data();
+    if (out_delta) std::memcpy(out_delta, p, sizeof(double) * nn);
+    p += sizeof(double) * nn + sizeof(int) * nn;  // past delta and done
+    if (out_sweeps) std::memcpy(out_sweeps, p, sizeof(int) * nn);
+    p += sizeof(int) * nn;
+    if (out_at) std::memcpy(out_at, p, sizeof(int) * nn);
     return BP_OK;
 }
 
@@ -1353,6 +1423,7 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
     (void)hipFree(snap);
     c->cur = cur0;
     c->fp64_active = fp64_0;
+    bump_epoch();  // kernel choices may have changed
     if (rc != BP_OK) return rc;
     if (cache_path && mode >= 1) RETCHK(tune_save(c, cache_path));
     if (out_tuned) *out_tuned = tuned;

@@ -35,6 +35,16 @@ struct NetCtl {
     int enabled;   // 0: plain sweep, every network updated
 };
 
+// Ordering of LDS traffic within one wave: lane A's store is seen by lane B's later load
+// once the wave's LDS operations have completed (s_waitcnt lgkmcnt(0)); the empty asm keeps
+// the compiler from moving LDS accesses across.  Used instead of __syncthreads where the
+// buffer belongs to the wave alone, so the waves of a workgroup do not wait for each other.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt, expcnt unrestricted
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+
 // wave64 reductions (DPP/permute through the compiler's __shfl_xor lowering)
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {

@@ -556,7 +556,7 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
     vx_mode<false, BF>(E, G, msm + 1 * 64);
 #pragma unroll
     for (int r = 0; r < 64; ++r) bufB[r * VX_PITCH + lane] = G[r];
-    __syncthreads();
+    wave_lds_sync();  // bufB belongs to this wave
 #pragma unroll
     for (int r = 0; r < 64; ++r) E[r] = bufB[lane * VX_PITCH + r];  // E01 in Y
     vx_mode<false, BF>(E, G, msm + 3 * 64);
@@ -569,10 +569,10 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
     for (int r = 0; r < 64; ++r) T[r] = bufA[lane * VX_PITCH + r];  // A in Y
     vx_mode<true, BF>(T, E, msm + 2 * 64);
     vx_mode<false, BF>(E, G, msm + 3 * 64);
-    __syncthreads();  // every lane has finished reading bufB (phase A)
+    wave_lds_sync();  // every lane of the wave has finished reading bufB (phase A)
 #pragma unroll
     for (int r = 0; r < 64; ++r) bufB[lane * VX_PITCH + r] = G[r];  // write from Y
-    __syncthreads();
+    wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < 64; ++r) E[r] = bufB[r * VX_PITCH + lane];  // E23 in X
     vx_mode<false, BF>(E, G, msm + 1 * 64);

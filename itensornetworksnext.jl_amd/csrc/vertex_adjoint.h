@@ -82,23 +82,24 @@ __device__ __forceinline__ void vd_mp2(const float (&T1)[64], const float (&M1)[
     }
 }
 
-// Register transposes through the wave's LDS buffer (row pitch 65: conflict-free).
+// Register transposes through the wave's own LDS buffer (row pitch 65: conflict-free);
+// only the wave's lanes touch it, so wave-level ordering suffices.
 __device__ __forceinline__ void vd_x_to_y(float (&R)[64], float *buf, int lane) {
 #pragma unroll
     for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = R[r];
-    __syncthreads();
+    wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < 64; ++r) R[r] = buf[lane * VX_PITCH + r];
-    __syncthreads();
+    wave_lds_sync();
 }
 
 __device__ __forceinline__ void vd_y_to_x(float (&R)[64], float *buf, int lane) {
 #pragma unroll
     for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = R[r];
-    __syncthreads();
+    wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < 64; ++r) R[r] = buf[r * VX_PITCH + lane];
-    __syncthreads();
+    wave_lds_sync();
 }
 
 // Incoming messages (identity on padded ports) and overrides W at the out-slots (zero on
@@ -156,14 +157,56 @@ __device__ __forceinline__ void vd_load_mats(const float *msgs, const float *W, 
     }
 }
 
+// Convergence bookkeeping of the adjoint sweeps, kept on the device so that sweeps are
+// queued in chunks without a host round trip per sweep.  Sweep j accumulates its maxima in
+// slot j % 3 of the accumulator (3 x 128 words); the check of sweep j runs at the start of
+// sweep j + 1 (every workgroup forms the same r from the same words) or in k_vd_adj_check.
+struct VdLoopState {
+    int stop;  // set once the relative change fell below tol
+    int it;    // sweeps checked (= sweeps that ran, once the loop has ended)
+    double r;  // relative change of the last checked sweep
+};
+
+// r = max|lambda' - lambda| / max|lambda'| of sweep j from the workgroup maxima in slot j % 3
+// (NaN in the change propagates; NaN in the magnitude is ignored).  One wave, uniform result.
+__device__ __forceinline__ double vd_adj_r(const unsigned long long *acc, int j, int lane) {
+    const unsigned long long *a = acc + (j % 3) * 128;
+    const double x = __longlong_as_double((long long)a[lane]);
+    double y = __longlong_as_double((long long)a[64 + lane]);
+    if (y != y) y = 0.0;
+    const double ra = wave_max(x), rr = wave_max(y);
+    return rr > 0.0 ? ra / rr : ra;
+}
+
+// Check of sweep j (thread 0 of the grid records it); returns whether sweeps go on.
+__device__ __forceinline__ bool vd_adj_check(const unsigned long long *acc, int j, double tol,
+                                             VdLoopState *st, int lane, bool record) {
+    const double r = vd_adj_r(acc, j, lane);
+    const bool go = !(r < tol);
+    if (record && lane == 0) {
+        st->it = j + 1;
+        st->r = r;
+        if (!go) st->stop = 1;
+    }
+    return go;
+}
+
+__global__ __launch_bounds__(64) void k_vd_adj_check(const unsigned long long *__restrict__ acc,
+                                                     int j, double tol, VdLoopState *st) {
+    if (st->stop) return;  // uniform
+    (void)vd_adj_check(acc, j, tol, st, threadIdx.x, true);
+}
+
 // ---- message form.  to_in == 0: R_j goes to the out-slot of port j (raw, with its sum in
 // S_out and Z_u in z_out when given).  to_in == 1: add[in-slot] + R_j goes to the in-slot of
 // port j.  `add` may be `out` (each in-slot is read and written by the same lane).
 // Gn / Sn: W is lambda, overrides are its normalisation cotangent (see vd_load_mats).
 // resacc (adjoint sweeps): max|result - ref| and max|result| of the workgroup go, as the
 // bit patterns of non-negative doubles, into resacc[rslot][0|1][blockIdx % 64] by atomic
-// max; workgroup 0 clears the other slot for the next sweep (its values were read by the
-// host before this launch).
+// max, rslot = sweep % 3; workgroup 0 clears the slot of the next sweep (written two sweeps
+// ago and checked one sweep ago).  lst (adjoint sweeps): the launch does nothing once
+// lst->stop is set; when chk >= 0 it first checks sweep chk (the previous one) against tol
+// and does nothing if that sweep met it.
 template <int NS, bool HASW>
 __global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
@@ -175,11 +218,21 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
               const float *__restrict__ ref, double *__restrict__ S_out,
               double *__restrict__ z_out, const float *__restrict__ Gn,
               const double *__restrict__ Sn, unsigned long long *__restrict__ resacc,
-              int rslot) {
+              int rslot, VdLoopState *lst, double tol, int chk) {
     extern __shared__ __align__(16) float vsm[];
     const int u = blockIdx.x;
     const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
+    if (lst) {  // adjoint loop control, uniform over the grid
+        __shared__ int s_go;
+        if (w == 0) {
+            bool go = lst->stop == 0;
+            if (go && chk >= 0) go = vd_adj_check(resacc, chk, tol, lst, lane, u == 0);
+            if (lane == 0) s_go = go ? 1 : 0;
+        }
+        __syncthreads();
+        if (!s_go) return;
+    }
     float *bufA = vsm + w * 2 * VX_BUF;
     float *bufB = bufA + VX_BUF;
     float *msm = vsm + NS * 2 * VX_BUF;
@@ -189,7 +242,8 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
     const int vid = vlist[u];
     const float cv = (float)(cv_vertex ? cv_vertex[vid] : cv_const);
     vd_load_mats<NS>(msgs, HASW ? W : nullptr, sl, 0, msm, wsm, Gn, Sn);
-    if (resacc && blockIdx.x == 0 && threadIdx.x < 128) resacc[(rslot ^ 1) * 128 + threadIdx.x] = 0ull;
+    if (resacc && blockIdx.x == 0 && threadIdx.x < 128)
+        resacc[((rslot + 1) % 3) * 128 + threadIdx.x] = 0ull;
     const int ks = op == OP_X ? (w ^ 1) : w;
     const float sg = (op == OP_Z && w == 1) ? -1.f : 1.f;
     float R1[64], R2[64], R3[64];
@@ -494,7 +548,8 @@ static int vd_msg(const VdCtx &c, const float *msgs, const float *W, int op, dou
                   const double *cv_vertex, int to_in, const float *add, float *out,
                   const float *ref, double *S_out, double *z_out, const float *Gn = nullptr,
                   const double *Sn = nullptr, unsigned long long *resacc = nullptr,
-                  int rslot = 0) {
+                  int rslot = 0, VdLoopState *lst = nullptr, double tol = 0.0,
+                  int chk = -1) {
     const VertexPlan &p = *c.vp;
     const dim3 grid(p.nv), block(64 * p.d);
     const size_t lds = vd_msg_lds(p.d);
@@ -504,7 +559,7 @@ static int vd_msg(const VdCtx &c, const float *msgs, const float *W, int op, dou
     hipLaunchKernelGGL((k_vd_msg<NS, HW>), grid, block, lds, c.st, c.vsites, msgs, W, op,      \
                        cv_const, cv_vertex, p.d_vin, p.d_vout, p.d_vdim, p.d_vlist,            \
                        c.tp.msg_off, to_in,                                                    \
-                       add, out, ref, S_out, z_out, Gn, Sn, resacc, rslot)
+                       add, out, ref, S_out, z_out, Gn, Sn, resacc, rslot, lst, tol, chk)
     switch (p.d * 2 + (hw ? 1 : 0)) {
         case 2: VD_MSG(1, false); break;
         case 3: VD_MSG(1, true); break;
@@ -619,24 +674,32 @@ static int vadj_tfim(AdjointScratch *s, const VdCtx &c, int ne, const float *msg
     hipLaunchKernelGGL(k_adj_tfim_coefs, dim3((nmax + 255) / 256), dim3(256), 0, c.st, tp, J, h,
                        z, zx, nz, dd, cv1, cv0, cq1, cq0, ev, eq);
     hipLaunchKernelGGL(k_adj_sum2, dim3(1), dim3(BP_BLOCK), 0, c.st, ev, nv, eq, np, esum);
-    if (hipMemcpyAsync(energy, esum, sizeof(double), hipMemcpyDeviceToHost, c.st) != hipSuccess ||
-        hipStreamSynchronize(c.st) != hipSuccess)
-        VD_RET(2);
     if (!go.any()) {
+        if (hipMemcpyAsync(energy, esum, sizeof(double), hipMemcpyDeviceToHost, c.st) !=
+                hipSuccess ||
+            hipStreamSynchronize(c.st) != hipSuccess)
+            VD_RET(2);
         s->free_temps();
         return 0;
     }
+    // with a gradient, the energy is read back with the loop state (esum stays untouched)
     const bool direct = go.dev && !go.dev_f64;
     float *dm = (float *)s->alloc(sizeof(float) * nmsg);
     float *WI = (float *)s->alloc(sizeof(float) * nmsg);
     float *WZ = (float *)s->alloc(sizeof(float) * nmsg);
-    float *lam = (float *)s->alloc(sizeof(float) * nmsg);
-    float *nxt = (float *)s->alloc(sizeof(float) * nmsg);
+    float *lbuf[2] = {(float *)s->alloc(sizeof(float) * nmsg),
+                      (float *)s->alloc(sizeof(float) * nmsg)};
     float *gg = (float *)s->alloc(sizeof(float) * nmsg);
     float *da = direct ? (float *)go.dev : (float *)s->alloc(sizeof(float) * nsite);
-    unsigned long long *acc = (unsigned long long *)s->alloc(sizeof(unsigned long long) * 256);
+    // residual accumulator (3 x 128 words) followed by the loop state, cleared together
+    const size_t acc_bytes = sizeof(unsigned long long) * 384;
+    unsigned long long *acc =
+        (unsigned long long *)s->alloc(acc_bytes + sizeof(VdLoopState));
+    VdLoopState *lst = acc ? (VdLoopState *)(acc + 384) : nullptr;
     double *g64 = go.host ? (double *)s->alloc(sizeof(double) * nsite) : nullptr;
-    if (!dm || !WI || !WZ || !lam || !nxt || !gg || !da || !acc || (go.host && !g64)) VD_RET(5);
+    if (!dm || !WI || !WZ || !lbuf[0] || !lbuf[1] || !gg || !da || !acc || !lst ||
+        (go.host && !g64))
+        VD_RET(5);
     if (hipMemsetAsync(da, 0, sizeof(float) * nsite, c.st) != hipSuccess) VD_RET(2);
     // explicit dE/dm: vertex terms, then the edge terms (port t carries the other side's G / GZ)
     if (np > 0) {
@@ -654,33 +717,38 @@ static int vadj_tfim(AdjointScratch *s, const VdCtx &c, int ne, const float *msg
     // adjoint sweeps lambda <- dE/dm + J_m^T lambda
     const int cblocks = (int)std::max<long long>(1, std::min<long long>((nmsg + 255) / 256, 4096));
     hipLaunchKernelGGL(k_adj_axpy_copy<float>, dim3(cblocks), dim3(256), 0, c.st,
-                       (const float *)dm, lam, nmsg);
-    int it = 0;
-    double r = INFINITY;
-    if (hipMemsetAsync(acc, 0, sizeof(unsigned long long) * 256, c.st) != hipSuccess) VD_RET(2);
-    std::vector<unsigned long long> hacc(128);
-    while (np > 0 && it < adj_maxiter) {
-        const int slot = it & 1;
-        // lambda' = dE/dm + J_m^T lambda, the normalisation cotangent formed in the kernel
-        VD_RET(vd_msg(c, msgs, lam, OP_I, 0.0, nullptr, 1, dm, nxt, lam, nullptr, nullptr, G, S,
-                      acc, slot));
-        if (hipMemcpyAsync(hacc.data(), acc + slot * 128, sizeof(unsigned long long) * 128,
-                           hipMemcpyDeviceToHost, c.st) != hipSuccess ||
+                       (const float *)dm, lbuf[0], nmsg);
+    // sweep i reads lbuf[i & 1] and writes lbuf[(i + 1) & 1]; sweeps queued after the
+    // stop flag was set return at once, so lambda sits in lbuf[it & 1] for the `it` sweeps
+    // that ran
+    VdLoopState hst{0, 0, INFINITY};
+    double hE = 0.0;
+    if (hipMemsetAsync(acc, 0, acc_bytes + sizeof(VdLoopState), c.st) != hipSuccess) VD_RET(2);
+    // sweeps queued between host reads of the loop state: 16, then doubling up to 64 (a
+    // sweep queued after convergence costs one empty launch, a host read one round trip)
+    int chunk = 16;
+    int queued = 0;
+    while (np > 0 && queued < adj_maxiter) {
+        const int n = std::min(chunk, adj_maxiter - queued);
+        chunk = std::min(2 * chunk, 64);
+        for (int i = queued; i < queued + n; ++i) {
+            float *lam = lbuf[i & 1], *nxt = lbuf[(i + 1) & 1];
+            // lambda' = dE/dm + J_m^T lambda, the normalisation cotangent formed in the kernel
+            VD_RET(vd_msg(c, msgs, lam, OP_I, 0.0, nullptr, 1, dm, nxt, lam, nullptr, nullptr, G,
+                          S, acc, i % 3, lst, adj_tol, i - 1));
+        }
+        queued += n;
+        hipLaunchKernelGGL(k_vd_adj_check, dim3(1), dim3(64), 0, c.st,
+                           (const unsigned long long *)acc, queued - 1, adj_tol, lst);
+        if (hipMemcpyAsync(&hst, lst, sizeof(VdLoopState), hipMemcpyDeviceToHost, c.st) !=
+                hipSuccess ||
             hipStreamSynchronize(c.st) != hipSuccess)
             VD_RET(2);
-        double ra = 0.0, rr = 0.0;
-        for (int i = 0; i < 64; ++i) {
-            double x, y;
-            std::memcpy(&x, &hacc[i], sizeof(double));
-            std::memcpy(&y, &hacc[64 + i], sizeof(double));
-            ra = (x > ra || x != x) ? x : ra;
-            rr = std::max(rr, y);
-        }
-        r = rr > 0.0 ? ra / rr : ra;
-        std::swap(lam, nxt);
-        ++it;
-        if (r < adj_tol) break;
+        if (hst.stop) break;
     }
+    const int it = hst.it;
+    const double r = np > 0 ? hst.r : INFINITY;
+    const float *lam = lbuf[it & 1];
     // grad = dE/dA + J_A^T lambda: op I carries W = cq0 G_rev + gg(lambda), op Z carries cq1 GZ_rev,
     // op X and the value part of op I carry the vertex coefficients
     if (np > 0) {
@@ -696,7 +764,10 @@ static int vadj_tfim(AdjointScratch *s, const VdCtx &c, int ne, const float *msg
         VD_RET(vd_site(c, msgs, nullptr, tr, OP_X, 0.0, cv1, da));
     }
     if (!direct) VD_RET(adj_emit_grad(c.st, (const float *)da, g64, nsite, go));
+    if (hipMemcpyAsync(&hE, esum, sizeof(double), hipMemcpyDeviceToHost, c.st) != hipSuccess)
+        VD_RET(2);
     const hipError_t e = hipStreamSynchronize(c.st);
+    *energy = hE;
     if (out_iters) *out_iters = it;
     if (out_res) *out_res = r;
     s->free_temps();
