@@ -129,13 +129,29 @@ __device__ __forceinline__ VdSlots vd_slots(const int4 *vin, const int4 *vout, c
 // With Gn != nullptr, W holds lambda and the override is its cotangent through the sum
 // normalisation, (lambda - <lambda, G>/S) / S (G the unnormalised forward message, S its
 // sum; lambda unchanged where S = 0).
+// With W2 != nullptr (two slices, d = 2), wave w gets its own overrides W + sigma_w W2 at
+// wsm + 64 * 4 * w, sigma = (+1, -1): the Z-inserted launch with overrides W2 folded into
+// the plain launch with overrides W (both results are linear in the overrides, and Z only
+// flips the sign of slice 1).  Otherwise the overrides sit at wsm, shared by the waves.
 template <int NS>
 __device__ __forceinline__ void vd_load_mats(const float *msgs, const float *W, const VdSlots &sl,
                                              int trans, float *msm, float *wsm,
                                              const float *Gn = nullptr,
-                                             const double *Sn = nullptr) {
+                                             const double *Sn = nullptr,
+                                             const float *W2 = nullptr) {
     const int w = threadIdx.x >> 6, e = threadIdx.x & 63;
     const int a = trans ? (e & 7) : (e >> 3), b = trans ? (e >> 3) : (e & 7);
+    if (W && W2) {
+        const float sgn = w == 1 ? -1.f : 1.f;
+        float *ws = wsm + w * 256;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int D = sl.dims[j];
+            const bool have = sl.outs[j] >= 0 && a < D && b < D;
+            const long long off = sl.outo[j] + a * D + b;
+            ws[j * 64 + e] = have ? W[off] + sgn * W2[off] : 0.f;
+        }
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         if (j % NS != w) continue;
@@ -143,7 +159,7 @@ __device__ __forceinline__ void vd_load_mats(const float *msgs, const float *W, 
         const bool real = a < D && b < D;
         msm[j * 64 + e] = sl.ins[j] >= 0 ? (real ? msgs[sl.ino[j] + a * D + b] : 0.f)
                                          : (a == b ? 1.f : 0.f);
-        if (W) {
+        if (W && !W2) {
             const bool have = sl.outs[j] >= 0 && real;
             float x = have ? W[sl.outo[j] + a * D + b] : 0.f;
             if (Gn) {
@@ -218,7 +234,8 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
               const float *__restrict__ ref, double *__restrict__ S_out,
               double *__restrict__ z_out, const float *__restrict__ Gn,
               const double *__restrict__ Sn, unsigned long long *__restrict__ resacc,
-              int rslot, VdLoopState *lst, double tol, int chk) {
+              int rslot, VdLoopState *lst, double tol, int chk,
+              const float *__restrict__ W2) {
     extern __shared__ __align__(16) float vsm[];
     const int u = blockIdx.x;
     const int w = threadIdx.x >> 6;
@@ -236,12 +253,13 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
     float *bufA = vsm + w * 2 * VX_BUF;
     float *bufB = bufA + VX_BUF;
     float *msm = vsm + NS * 2 * VX_BUF;
-    float *wsm = msm + 4 * 64;
-    float *part = wsm + 4 * 64;
+    float *wsm0 = msm + 4 * 64;
+    float *part = wsm0 + NS * 4 * 64;
     const VdSlots sl = vd_slots(vin, vout, vdim, msg_off, u);
     const int vid = vlist[u];
     const float cv = (float)(cv_vertex ? cv_vertex[vid] : cv_const);
-    vd_load_mats<NS>(msgs, HASW ? W : nullptr, sl, 0, msm, wsm, Gn, Sn);
+    vd_load_mats<NS>(msgs, HASW ? W : nullptr, sl, 0, msm, wsm0, Gn, Sn, HASW ? W2 : nullptr);
+    const float *wsm = wsm0 + (W2 ? w * 256 : 0);
     if (resacc && blockIdx.x == 0 && threadIdx.x < 128)
         resacc[((rslot + 1) % 3) * 128 + threadIdx.x] = 0ull;
     const int ks = op == OP_X ? (w ^ 1) : w;
@@ -393,28 +411,55 @@ void k_vd_site(const float *__restrict__ vsites, const float *__restrict__ msgs,
                const int4 *__restrict__ vout, const int4 *__restrict__ vdim,
                const int *__restrict__ vlist, const long long *__restrict__ msg_off,
                const long long *__restrict__ site_off,
-               const int *__restrict__ port_off, float *__restrict__ da) {
+               const int *__restrict__ port_off, float *__restrict__ da,
+               const float *__restrict__ W2, const double *__restrict__ cv_swap) {
     extern __shared__ __align__(16) float vsm[];
     const int u = blockIdx.x;
     const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     float *bufB = vsm + w * VX_BUF;
     float *msm = vsm + NS * VX_BUF;
-    float *wsm = msm + 4 * 64;
+    float *wsm0 = msm + 4 * 64;
     const VdSlots sl = vd_slots(vin, vout, vdim, msg_off, u);
     const int vid = vlist[u];
     const float cv = (float)(cv_vertex ? cv_vertex[vid] : cv_const);
-    vd_load_mats<NS>(msgs, HASW ? W : nullptr, sl, trans, msm, wsm);
+    vd_load_mats<NS>(msgs, HASW ? W : nullptr, sl, trans, msm, wsm0, nullptr, nullptr,
+                     HASW ? W2 : nullptr);
+    const float *wsm = wsm0 + (W2 ? w * 256 : 0);
     const int ks = op == OP_X ? (w ^ 1) : w;
     const float sg = (op == OP_Z && w == 1) ? -1.f : 1.f;
     float R1[64], R2[64], R3[64];
+    // cv_swap given (d = 2): the X-inserted value term cv_swap A[w ^ 1] x_all M of this
+    // slice runs first, into a wave-private LDS block (entry r * 64 + lane), and joins the
+    // result at the end; the tangent chain needs the value chain of A[w] alone
+    const bool fold = cv_swap != nullptr;
+    float *xbuf = wsm0 + NS * 4 * 64 + w * 4096;
     {
         const float *kst = vsites + ((long long)u * NS + ks) * 4096;
+        if (fold) {
+            const float cx = (float)cv_swap[vid];
+            const float *xst = vsites + ((long long)u * NS + (w ^ 1)) * 4096;
+#pragma unroll
+            for (int r = 0; r < 64; ++r) R2[r] = cx * xst[r * 64 + lane];
+        }
 #pragma unroll
         for (int r = 0; r < 64; ++r) R1[r] = kst[r * 64 + lane];
     }
     __syncthreads();
     float M[16], Wm[16];
+    if (fold) {
+        vx_load_m(msm + 0 * 64, M);
+        vd_mp1<true>(R2, M, R2);
+        vx_load_m(msm + 1 * 64, M);
+        vd_mp1<false>(R2, M, R2);
+        vd_x_to_y(R2, bufB, lane);
+        vx_load_m(msm + 2 * 64, M);
+        vd_mp1<true>(R2, M, R2);
+        vx_load_m(msm + 3 * 64, M);
+        vd_mp1<false>(R2, M, R2);
+#pragma unroll
+        for (int r = 0; r < 64; ++r) xbuf[r * 64 + lane] = R2[r];
+    }
     vx_load_m(msm + 0 * 64, M);
     vd_mp1<true>(R1, M, R2);                                                   // V1
     if constexpr (HASW) { vx_load_m(wsm + 0 * 64, Wm); vd_mp1<true>(R1, Wm, R3); }
@@ -432,6 +477,10 @@ void k_vd_site(const float *__restrict__ vsites, const float *__restrict__ msgs,
     const float c = sg * cv;
 #pragma unroll
     for (int r = 0; r < 64; ++r) R1[r] = HASW ? c * R1[r] + sg * R3[r] : c * R1[r];
+    if (fold) {
+#pragma unroll
+        for (int r = 0; r < 64; ++r) R1[r] += xbuf[r * 64 + lane];
+    }
     vd_y_to_x(R1, bufB, lane);
     // entry e = r * 64 + lane of the padded slice: ports (r >> 3, r & 7, lane >> 3, lane & 7)
     const int k = port_off[vid + 1] - port_off[vid];
@@ -508,8 +557,14 @@ __global__ void k_vd_inv(const double *z, double *out, int n) {
 
 // ---------------------------------------------------------------- host side
 
-static size_t vd_msg_lds(int ns) { return sizeof(float) * ((size_t)ns * 2 * VX_BUF + 512 + (size_t)ns * 256); }
-static size_t vd_site_lds(int ns) { return sizeof(float) * ((size_t)ns * VX_BUF + 512); }
+// LDS: slice buffers, messages (4 x 64), overrides (one set per wave), partial results
+static size_t vd_msg_lds(int ns) {
+    return sizeof(float) * ((size_t)ns * 2 * VX_BUF + 256 + (size_t)ns * 256 + (size_t)ns * 256);
+}
+static size_t vd_site_lds(int ns, bool fold = true) {
+    return sizeof(float) * ((size_t)ns * VX_BUF + 256 + (size_t)ns * 256 +
+                            (fold ? (size_t)ns * 4096 : 0));
+}
 
 struct VdCtx {
     const VertexPlan *vp;
@@ -549,17 +604,18 @@ static int vd_msg(const VdCtx &c, const float *msgs, const float *W, int op, dou
                   const float *ref, double *S_out, double *z_out, const float *Gn = nullptr,
                   const double *Sn = nullptr, unsigned long long *resacc = nullptr,
                   int rslot = 0, VdLoopState *lst = nullptr, double tol = 0.0,
-                  int chk = -1) {
+                  int chk = -1, const float *W2 = nullptr) {
     const VertexPlan &p = *c.vp;
     const dim3 grid(p.nv), block(64 * p.d);
     const size_t lds = vd_msg_lds(p.d);
     const bool hw = W != nullptr;
     if (op != OP_I && p.d != 2) return 4;  // X and Z pair the two slices of a qubit
+    if (W2 && (p.d != 2 || op != OP_I || !W || Gn)) return 4;
 #define VD_MSG(NS, HW)                                                                         \
     hipLaunchKernelGGL((k_vd_msg<NS, HW>), grid, block, lds, c.st, c.vsites, msgs, W, op,      \
                        cv_const, cv_vertex, p.d_vin, p.d_vout, p.d_vdim, p.d_vlist,            \
                        c.tp.msg_off, to_in,                                                    \
-                       add, out, ref, S_out, z_out, Gn, Sn, resacc, rslot, lst, tol, chk)
+                       add, out, ref, S_out, z_out, Gn, Sn, resacc, rslot, lst, tol, chk, W2)
     switch (p.d * 2 + (hw ? 1 : 0)) {
         case 2: VD_MSG(1, false); break;
         case 3: VD_MSG(1, true); break;
@@ -575,18 +631,23 @@ static int vd_msg(const VdCtx &c, const float *msgs, const float *W, int op, dou
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
+// W2 (Z-inserted overrides) and cv_swap (X-inserted value coefficients) fold the Z and X
+// launches of a d = 2 vertex into this op = I launch.
 static int vd_site(const VdCtx &c, const float *msgs, const float *W, int trans, int op,
-                   double cv_const, const double *cv_vertex, float *da) {
+                   double cv_const, const double *cv_vertex, float *da,
+                   const float *W2 = nullptr, const double *cv_swap = nullptr) {
     const VertexPlan &p = *c.vp;
     const dim3 grid(p.nv), block(64 * p.d);
-    const size_t lds = vd_site_lds(p.d);
+    const size_t lds = vd_site_lds(p.d, cv_swap != nullptr);
     const bool hw = W != nullptr;
     if (op != OP_I && p.d != 2) return 4;
+    if ((W2 || cv_swap) && (p.d != 2 || op != OP_I)) return 4;
+    if (W2 && !W) return 4;
 #define VD_SITE(NS, HW)                                                                         \
     hipLaunchKernelGGL((k_vd_site<NS, HW>), grid, block, lds, c.st, c.vsites, msgs, W, trans,   \
                        op, cv_const, cv_vertex, p.d_vin, p.d_vout, p.d_vdim, p.d_vlist,         \
                        c.tp.msg_off,                                                            \
-                       c.tp.site_off, c.tp.port_off, da)
+                       c.tp.site_off, c.tp.port_off, da, W2, cv_swap)
     switch (p.d * 2 + (hw ? 1 : 0)) {
         case 2: VD_SITE(1, false); break;
         case 3: VD_SITE(1, true); break;
@@ -709,10 +770,9 @@ static int vadj_tfim(AdjointScratch *s, const VdCtx &c, int ne, const float *msg
                            (const float *)G, WI, 0);
         hipLaunchKernelGGL(k_vd_scale_rev, dim3(np), dim3(64), 0, c.st, tp, (const double *)cq1,
                            (const float *)GZ, WZ, 0);
+        // the I- and Z-inserted edge terms in one launch (overrides WI + sigma WZ per slice)
         VD_RET(vd_msg(c, msgs, WI, OP_I, 0.0, nullptr, 1, dm, dm, nullptr, nullptr, nullptr,
-                      nullptr, nullptr));
-        VD_RET(vd_msg(c, msgs, WZ, OP_Z, 0.0, nullptr, 1, dm, dm, nullptr, nullptr, nullptr,
-                      nullptr, nullptr));
+                      nullptr, nullptr, nullptr, 0, nullptr, 0.0, -1, WZ));
     }
     // adjoint sweeps lambda <- dE/dm + J_m^T lambda
     const int cblocks = (int)std::max<long long>(1, std::min<long long>((nmsg + 255) / 256, 4096));
@@ -758,11 +818,9 @@ static int vadj_tfim(AdjointScratch *s, const VdCtx &c, int ne, const float *msg
                            nmsg);
     }
     const float *wi = np > 0 ? WI : nullptr, *wz = np > 0 ? WZ : nullptr;
-    for (int tr = 0; tr < 2; ++tr) {
-        VD_RET(vd_site(c, msgs, wi, tr, OP_I, 0.0, cv0, da));
-        if (wz) VD_RET(vd_site(c, msgs, wz, tr, OP_Z, 0.0, nullptr, da));
-        VD_RET(vd_site(c, msgs, nullptr, tr, OP_X, 0.0, cv1, da));
-    }
+    // per side: the I (overrides wi, value cv0), Z (overrides wz) and X (value cv1) terms
+    // in one launch
+    for (int tr = 0; tr < 2; ++tr) VD_RET(vd_site(c, msgs, wi, tr, OP_I, 0.0, cv0, da, wz, cv1));
     if (!direct) VD_RET(adj_emit_grad(c.st, (const float *)da, g64, nsite, go));
     if (hipMemcpyAsync(&hE, esum, sizeof(double), hipMemcpyDeviceToHost, c.st) != hipSuccess)
         VD_RET(2);
