@@ -1282,7 +1282,7 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
     HIPCHK(hipMemcpyAsync(snap, c->d_msg[cur0], es * c->nmsg, hipMemcpyDeviceToDevice, c->stream));
     c->fp64_active = false;
     int rc = BP_OK;
-    // 1. vertex kernels or generic kernel, per largest bond dimension D <= 8
+    // 1. vertex kernels or generic kernel, per largest bond dimension (<= 12 FP32, <= 16 BF16)
     if ((c->dtype == BP_F32 || c->dtype == BP_BF16) && !c->plain && c->path == BP_PATH_AUTO) {
         std::map<int, int> dmax_d;  // largest bond dimension -> physical dimension
         for (int n = 0; n < c->nnet && rc == BP_OK; ++n) {
@@ -1292,7 +1292,7 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
                 for (int q = c->port_off[u]; q < c->port_off[u + 1]; ++q)
                     dm = std::max(dm, c->edge_dim[c->port_edge[q]]);
             }
-            const int dlim = c->dtype == BP_BF16 ? 16 : VX_D;
+            const int dlim = c->dtype == BP_BF16 ? 16 : 12;
             if (dm >= 1 && dm <= dlim && deg <= 4 && d <= 4) dmax_d[dm] = d;
         }
         for (const auto &kv : dmax_d) {
@@ -1352,7 +1352,8 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
         c->vplan.variant = c->vx_variant;
     }
     // 1c. waves per slice workgroup of the bond-dimension-16 kernel (BF16)
-    if (rc == BP_OK && c->vplan_ok && c->vplan.nv16 > 0 && c->path != BP_PATH_GENERIC) {
+    if (rc == BP_OK && c->vplan_ok && c->vplan.nv16 > 0 && c->vplan.bf16 &&
+        c->path != BP_PATH_GENERIC) {
         std::ostringstream o;
         o << "v16w|bf16|d" << c->vplan.d;
         const std::string key = o.str();
@@ -1502,7 +1503,7 @@ static const void *current_msgs(bp_context *c, bool *is64) {
 // generic kernels otherwise, and always for FP64 messages or a forced generic path).
 static bool vertex_adjoint_ok(const bp_context *c, bool is64) {
     return !is64 && c->dtype == BP_F32 && c->vplan_ok && c->vplan.all && !c->vplan.bf16 &&
-           c->path != BP_PATH_GENERIC && c->d_vsites != nullptr;
+           c->vplan.nv16 == 0 && c->path != BP_PATH_GENERIC && c->d_vsites != nullptr;
 }
 
 // per-slot distances of the current messages against `other` (device buffer)

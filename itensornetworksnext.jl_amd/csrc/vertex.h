@@ -63,11 +63,13 @@ struct VertexPlan {
     int ngen = 0;
     size_t lds_bytes = 0;
     int variant = 0;            // FP32 sweep kernel: 0 k_vertex_d8, 1 k_vertex_d8_lo
-    // BF16 networks whose largest bond dimension is 9..16 (k_vertex_d16, vertex16.h)
+    // networks whose largest bond dimension is 9..16 in BF16 (k_vertex_d16, vertex16.h) or
+    // 9..12 in FP32 (k_vertex_d12, vertex12.h): the large-bond bucket
     int nv16 = 0;
     int4 *d_v16in = nullptr, *d_v16out = nullptr, *d_v16dim = nullptr;
     int *d_v16net = nullptr, *d_v16list = nullptr;
-    unsigned short *d_v16sites = nullptr;  // padded BF16 slices, 16^4 entries each
+    unsigned short *d_v16sites = nullptr;  // BF16: padded slices, 16^4 entries each
+    float *d_v12sites = nullptr;           // FP32: padded slices, 12^4 entries each (2 layouts)
     float *d_v16part = nullptr;            // per-slice unnormalised messages
     int v16_waves = 16;                    // waves per slice workgroup: 4, 8 or 16
 };
@@ -80,13 +82,14 @@ static void vertex_plan_release(VertexPlan *p) {
     if (p->d_vin) (void)hipFree(p->d_vin);
     if (p->d_vout) (void)hipFree(p->d_vout);
     if (p->d_vdim) (void)hipFree(p->d_vdim);
-    void *v16[7] = {p->d_v16in, p->d_v16out, p->d_v16dim, p->d_v16net, p->d_v16list,
-                    p->d_v16sites, p->d_v16part};
+    void *v16[8] = {p->d_v16in, p->d_v16out, p->d_v16dim, p->d_v16net, p->d_v16list,
+                    p->d_v16sites, p->d_v16part, p->d_v12sites};
     for (void *x : v16)
         if (x) (void)hipFree(x);
     p->d_v16in = p->d_v16out = p->d_v16dim = nullptr;
     p->d_v16net = p->d_v16list = nullptr;
     p->d_v16sites = nullptr;
+    p->d_v12sites = nullptr;
     p->d_v16part = nullptr;
     p->nv16 = 0;
     if (p->d_vnet) (void)hipFree(p->d_vnet);
@@ -115,7 +118,10 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     p->why.clear();
     if (plain) { p->why = "plain (vector-message) networks use the generic path"; return 4; }
     if (dtype != 1 && dtype != 2) { p->why = "the vertex path computes in FP32 or BF16"; return 4; }
-    // qualify whole networks (vertices of a network are contiguous)
+    // qualify whole networks (vertices of a network are contiguous); bond dimensions up to 8
+    // go to the D <= 8 kernels, larger ones up to 16 (BF16) or 12 (FP32) to the large-bond
+    // bucket
+    const int dlim = dtype == 2 ? 16 : 12;
     int d = -1;
     std::vector<int> vlist, vlist16, gen_slots;
     p->mask = mask;
@@ -133,9 +139,9 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
             for (int q = port_off[u]; q < port_off[u + 1] && ok; ++q) {
                 const int D = edge_dim[port_edge[q]];
                 dmax = std::max(dmax, D);
-                if (D > 16 || (D > VX_D && dtype != 2)) {
+                if (D > dlim) {
                     ok = false;
-                    last_reason = dtype == 2 ? "bond dimension > 16" : "bond dimension > 8";
+                    last_reason = "bond dimension > " + std::to_string(dlim);
                 }
             }
         }
@@ -169,7 +175,7 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
         vdim[b] = make_int4(dm[0], dm[1], dm[2], dm[3]);
         vn[b] = vnet[u];
     }
-    // the bond-dimension-16 bucket, laid out the same way
+    // the large-bond bucket, laid out the same way
     const int n16 = (int)vlist16.size();
     std::vector<int4> vin16(n16), vout16(n16), vdim16(n16);
     std::vector<int> vn16(n16);

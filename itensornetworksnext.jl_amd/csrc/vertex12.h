@@ -1,0 +1,250 @@
+// vertex12.h -- FP32 MFMA sweep for bond dimensions 9..12 (padded to 12) on gfx950.
+//
+// The FP32 counterpart of vertex16.h: one workgroup of 12 waves per (vertex, physical slice
+// s), the slice A[s] (four ports of dimension 12, 12^4 entries) in LDS as FP32 (82 KiB),
+// with rows of 144 (p2, p3) entries padded to 145 and blocks of 12 rows padded by one entry,
+// so that the stride along every port is odd or 12.
+//
+// Mode product on port J ("T x_J m_J", in place) runs on v_mfma_f32_16x16x4_f32: A operand
+// = m_J transposed (lane l holds m_J[4 kc + l/16][l%16], kc = 0..2 for the 12 values of the
+// contracted index), B operand = 16 fibres along J (lane l: entry J = 4 kc + l/16 of fibre
+// l%16).  The fibres of a tile run over the two later ports of the other three, flattened
+// (144 = 9 tiles of 16); wave w takes the 9 tiles whose first other port is w.  The 16 x 16
+// result holds rows J = 0..11 of the tile (rows 12..15 are the zero rows of the padded A
+// operand and are dropped); every tile is read whole before it is written, and tiles of
+// different waves are disjoint, so the product is in place.
+//
+// Open contraction out_O[a][a'] = sum_K G[a, K] A[a', K] (K: the other three ports, 1728
+// values) runs on the same MFMA: G from LDS, the bra A from HBM (L2 / Infinity Cache),
+// prefetched as 16-byte loads before the mode products.  Wave w takes the K values whose
+// outer other port is w; for O = 0, 1 the bra rows come from the natural copy
+// [p0][p1][p2][p3], for O = 2, 3 from a copy laid out [p2][p3][p0][p1], so that each lane's
+// four K values are contiguous in both.  The twelve waves' 16 x 16 partials are summed
+// through LDS into the same per-slice buffer as the BF16 kernel, and k_v16_finish sums the
+// slices, normalises, writes and forms the residuals.
+//
+// Per slice: for each of the four outgoing ports, reload the slice and absorb the other
+// three ports' messages (12 mode products), then open.  Ports of a padded (dummy) slot are
+// skipped (identity message).  Arithmetic: FP32 operands and accumulation throughout.
+#pragma once
+
+#include "vertex.h"
+
+static constexpr int V12_ELEMS = 20736;  // 12^4 per slice (HBM copies, unpadded)
+static constexpr int V12_NW = 12;        // waves per slice workgroup
+// LDS strides of (p0, p1, p2, p3)
+static constexpr int V12_S0 = 1741, V12_S1 = 145, V12_S2 = 12, V12_S3 = 1;
+static constexpr int V12_LDS_ELEMS = 12 * V12_S0;
+static constexpr size_t v12_lds() { return sizeof(float) * (V12_LDS_ELEMS + V12_NW * 256); }
+
+__host__ __device__ constexpr int v12_str(int j) {
+    return j == 0 ? V12_S0 : (j == 1 ? V12_S1 : (j == 2 ? V12_S2 : V12_S3));
+}
+
+// The other three ports of J in ascending order: tiles are indexed by P (one per wave),
+// fibres by (Q, R) flattened.
+template <int J> struct V12Ports {
+    static constexpr int P = J == 0 ? 1 : 0;
+    static constexpr int Q = J <= 1 ? 2 : 1;
+    static constexpr int R = J == 3 ? 2 : 3;
+};
+
+// Mode product on port J of the slice in LDS, in place.  Am: the message as A operand.
+template <int J>
+__device__ __forceinline__ void v12_mode(float *S, const float (&Am)[3]) {
+    using PT = V12Ports<J>;
+    constexpr int SJ = v12_str(J), SP = v12_str(PT::P), SQ = v12_str(PT::Q),
+                  SR = v12_str(PT::R);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int n = lane & 15, kk = lane >> 4;
+    const v4f zero = {0.f, 0.f, 0.f, 0.f};
+    const int pbase = w * SP;
+#pragma unroll
+    for (int jf = 0; jf < 9; ++jf) {
+        const int f = 16 * jf + n;  // fibre over (Q, R): 0..143
+        const int q = f / 12, r = f - 12 * q;
+        const int base = pbase + q * SQ + r * SR;
+        float b[3];
+#pragma unroll
+        for (int kc = 0; kc < 3; ++kc) b[kc] = S[base + (4 * kc + kk) * SJ];
+        v4f c = zero;
+#pragma unroll
+        for (int kc = 0; kc < 3; ++kc)
+            c = __builtin_amdgcn_mfma_f32_16x16x4f32(Am[kc], b[kc], c, 0, 0, 0);
+        if (kk < 3) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) S[base + (4 * kk + t) * SJ] = c[t];
+        }
+    }
+}
+
+// Bra operands of this wave's 36 MFMA steps of the open contraction on port O: step
+// (jm, t) of lane (n, kk) takes K = (hi = w, lo = 16 jm + 4 kk + t), a' = n.
+template <int O>
+__device__ __forceinline__ void v12_bra_prefetch(const float *natural, const float *swapped,
+                                                 float4 (&pre)[9]) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int n = lane & 15, kk = lane >> 4;
+    const float *src = O < 2 ? natural : swapped;
+    const int row = (O == 0 || O == 2) ? n * 1728 + 144 * w : w * 1728 + n * 144;
+#pragma unroll
+    for (int jm = 0; jm < 9; ++jm) {
+        if (n < 12) pre[jm] = *reinterpret_cast<const float4 *>(src + row + 16 * jm + 4 * kk);
+        else pre[jm] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// This wave's share of out_O[a][a'] = sum_K G[a, K] A[a', K]; lane l returns entries
+// (4(l/16)+i, l%16).  K = (hi, lo) maps to ports as: O = 0: (p1, p2, p3), O = 1: (p0, p2,
+// p3), O = 2: (p3, p0, p1), O = 3: (p2, p0, p1), with lo = 12 x + y over the last two.
+template <int O>
+__device__ __forceinline__ v4f v12_open(const float *S, const float4 (&pre)[9]) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int n = lane & 15, kk = lane >> 4;
+    const bool live = n < 12;
+    const int a = live ? n : 0;
+    int base;
+    if constexpr (O == 0) base = a * V12_S0 + w * V12_S1;
+    else if constexpr (O == 1) base = w * V12_S0 + a * V12_S1;
+    else if constexpr (O == 2) base = a * V12_S2 + w * V12_S3;
+    else base = w * V12_S2 + a * V12_S3;
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int jm = 0; jm < 9; ++jm) {
+        const float pv[4] = {pre[jm].x, pre[jm].y, pre[jm].z, pre[jm].w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int lo = 16 * jm + 4 * kk + t;
+            const int x = lo / 12, y = lo - 12 * x;
+            const int off = O < 2 ? x * V12_S2 + y * V12_S3 : x * V12_S0 + y * V12_S1;
+            const float g = live ? S[base + off] : 0.f;
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(g, pv[t], acc, 0, 0, 0);
+        }
+    }
+    return acc;
+}
+
+// Copy the slice's natural-layout entries into the padded LDS layout (16-byte loads).
+__device__ __forceinline__ void v12_load_slice(float *S, const float *src) {
+    const float4 *g = reinterpret_cast<const float4 *>(src);
+    for (int c = threadIdx.x; c < V12_ELEMS / 4; c += 64 * V12_NW) {
+        const float4 v = g[c];
+        const int e = 4 * c;
+        const int row = e / 144, col = e - 144 * row;  // row = 12 p0 + p1, col = 12 p2 + p3
+        const int p0 = row / 12, p1 = row - 12 * p0;
+        float *d = S + p0 * V12_S0 + p1 * V12_S1 + col;
+        d[0] = v.x;
+        d[1] = v.y;
+        d[2] = v.z;
+        d[3] = v.w;
+    }
+}
+
+template <int J>
+__device__ __forceinline__ void v12_mode_if(float *S, const int (&ins)[4],
+                                            const float (&Am)[4][3]) {
+    if (ins[J] < 0) return;  // identity message on a padded port (uniform)
+    v12_mode<J>(S, Am[J]);
+    __syncthreads();
+}
+
+// One outgoing port O of the slice: reload, absorb ports Q0, Q1, Q2, open, reduce the
+// waves' shares, store the unnormalised 16 x 16 block.
+template <int O, int Q0, int Q1, int Q2>
+__device__ __forceinline__ void v12_out(float *S, float *red, const float *src,
+                                        const float *src_w, float *dst, const int (&ins)[4],
+                                        const int (&outs)[4], const float (&Am)[4][3]) {
+    if (outs[O] < 0) return;  // uniform over the workgroup
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int n = lane & 15, kk = lane >> 4;
+    float4 pre[9];
+    v12_bra_prefetch<O>(src, src_w, pre);
+    __syncthreads();  // the previous port has finished with the slice and the partials
+    v12_load_slice(S, src);
+    __syncthreads();
+    v12_mode_if<Q0>(S, ins, Am);
+    v12_mode_if<Q1>(S, ins, Am);
+    v12_mode_if<Q2>(S, ins, Am);
+    const v4f acc = v12_open<O>(S, pre);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[w * 256 + (4 * kk + i) * 16 + n] = acc[i];
+    __syncthreads();
+    for (int e = threadIdx.x; e < 256; e += 64 * V12_NW) {
+        float x = 0.f;
+#pragma unroll
+        for (int v = 0; v < V12_NW; ++v) x += red[v * 256 + e];
+        dst[O * 256 + e] = x;
+    }
+}
+
+// grid: items x d (one workgroup per slice), 768 threads.  part[((item * d + s) * 4 + j) *
+// 256 + a * 16 + a'] receives slice s's unnormalised out_j (the layout k_v16_finish reads).
+__global__ __launch_bounds__(64 * V12_NW) void k_vertex_d12(
+    const float *__restrict__ vsites_n, const float *__restrict__ vsites_w,
+    const float *__restrict__ mold, const int4 *__restrict__ vin, const int4 *__restrict__ vout,
+    const int4 *__restrict__ vdim, const int *__restrict__ vnet,
+    const long long *__restrict__ msg_off, NetCtl ctl, int d, float *__restrict__ part) {
+    extern __shared__ __align__(16) float v12sm[];
+    float *S = v12sm;
+    float *red = v12sm + V12_LDS_ELEMS;
+    const int item = blockIdx.x / d, s = blockIdx.x - (blockIdx.x / d) * d;
+    const int lane = threadIdx.x & 63;
+    const int n = lane & 15, kk = lane >> 4;
+    if (ctl.enabled && ctl.done[vnet[item]]) return;  // k_v16_finish carries the messages
+    const int4 vi = vin[item], vo = vout[item], vd = vdim[item];
+    const int ins[4] = {vi.x, vi.y, vi.z, vi.w};
+    const int outs[4] = {vo.x, vo.y, vo.z, vo.w};
+    const int dims[4] = {vd.x, vd.y, vd.z, vd.w};
+    // incoming messages as MFMA A operands: lane l holds m[4 kc + l/16][l%16] (zero beyond
+    // the bond dimension; padded ports are never absorbed)
+    float Am[4][3];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int kc = 0; kc < 3; ++kc) {
+            const int k = 4 * kc + kk, D = dims[j];
+            Am[j][kc] = (ins[j] >= 0 && k < D && n < D) ? mold[msg_off[ins[j]] + k * D + n]
+                                                         : 0.f;
+        }
+    }
+    const long long so = ((long long)item * d + s) * V12_ELEMS;
+    const float *src = vsites_n + so, *src_w = vsites_w + so;
+    float *dst = part + ((long long)item * d + s) * 4 * 256;
+    v12_out<0, 2, 3, 1>(S, red, src, src_w, dst, ins, outs, Am);
+    v12_out<1, 2, 3, 0>(S, red, src, src_w, dst, ins, outs, Am);
+    v12_out<2, 0, 1, 3>(S, red, src, src_w, dst, ins, outs, Am);
+    v12_out<3, 0, 1, 2>(S, red, src, src_w, dst, ins, outs, Am);
+}
+
+// Padded FP32 site entries of the bucket: vsites_n[item][s][p0][p1][p2][p3] (12 each) and
+// vsites_w[item][s][p2][p3][p0][p1]; missing ports keep index 0 only, bond indices beyond D
+// are zero.
+__global__ void k_v12_pad_sites(const double *sites64, const long long *site_off,
+                                const int *port_off, const int *port_edge, const int *edge_dim,
+                                const int *vlist, int d, int nv, float *out_n, float *out_w) {
+    const long long total = (long long)nv * d * V12_ELEMS;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int e = (int)(i % V12_ELEMS);
+        const long long us = i / V12_ELEMS;
+        const int s = (int)(us % d);
+        const int u = vlist[us / d];
+        const int k = port_off[u + 1] - port_off[u];
+        const int idx[4] = {e / 1728, (e / 144) % 12, (e / 12) % 12, e % 12};
+        long long off = 0, cnt = 1;
+        bool real = true;
+        for (int j = 0; j < 4; ++j) {
+            if (j < k) {
+                const int D = edge_dim[port_edge[port_off[u] + j]];
+                if (idx[j] >= D) real = false;
+                off = off * D + idx[j];
+                cnt *= D;
+            } else if (idx[j] != 0) {
+                real = false;
+            }
+        }
+        const float v = real ? (float)sites64[site_off[u] + s * cnt + off] : 0.f;
+        out_n[i] = v;
+        out_w[us * V12_ELEMS + ((idx[2] * 12 + idx[3]) * 12 + idx[0]) * 12 + idx[1]] = v;
+    }
+}

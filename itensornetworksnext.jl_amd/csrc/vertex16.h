@@ -30,6 +30,7 @@
 #pragma once
 
 #include "vertex.h"
+#include "vertex12.h"
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef short v8s __attribute__((ext_vector_type(8)));
@@ -384,6 +385,16 @@ static int vertex16_upload_sites(VertexPlan *p, hipStream_t st, const double *d_
                                  const long long *d_site_off, const int *d_port_off,
                                  const int *d_port_edge, const int *d_edge_dim) {
     if (p->nv16 == 0) return 0;
+    if (!p->bf16) {  // FP32: the bond-dimension-12 kernel's two layouts
+        const size_t n = (size_t)p->nv16 * p->d * V12_ELEMS;
+        if (p->d_v12sites) (void)hipFree(p->d_v12sites);
+        p->d_v12sites = nullptr;
+        if (hipMalloc((void **)&p->d_v12sites, 2 * n * sizeof(float)) != hipSuccess) return 5;
+        hipLaunchKernelGGL(k_v12_pad_sites, dim3(4096), dim3(256), 0, st, d_sites64, d_site_off,
+                           d_port_off, d_port_edge, d_edge_dim, p->d_v16list, p->d, p->nv16,
+                           p->d_v12sites, p->d_v12sites + n);
+        return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
     const size_t bytes = 2 * (size_t)p->nv16 * p->d * V16_ELEMS;
     if (p->d_v16sites) (void)hipFree(p->d_v16sites);
     p->d_v16sites = nullptr;
@@ -404,14 +415,29 @@ static int vertex16_set_lds_limit() {
         if (hipFuncSetAttribute(f[i], hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)v16_lds(nw[i])) != hipSuccess)
             return 2;
+    if (hipFuncSetAttribute((const void *)k_vertex_d12, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)v12_lds()) != hipSuccess)
+        return 2;
     done = 1;
     return 0;
 }
 
-// The bond-dimension-16 bucket's part of a sweep: slices, then the per-message finish.
+// The large-bond bucket's part of a sweep: slices, then the per-message finish.
 static int vertex16_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *src, void *dst,
                           const NetCtl &ctl, double *res_fid, double *res_abs) {
     if (p->nv16 == 0) return 0;
+    if (!p->bf16) {
+        const size_t n = (size_t)p->nv16 * p->d * V12_ELEMS;
+        hipLaunchKernelGGL(k_vertex_d12, dim3(p->nv16 * p->d), dim3(64 * V12_NW), v12_lds(), st,
+                           (const float *)p->d_v12sites, (const float *)(p->d_v12sites + n),
+                           (const float *)src, p->d_v16in, p->d_v16out, p->d_v16dim,
+                           p->d_v16net, tp.msg_off, ctl, p->d, p->d_v16part);
+        hipLaunchKernelGGL(k_v16_finish, dim3(p->nv16), dim3(256), 0, st,
+                           (const float *)p->d_v16part, (const float *)src, (float *)dst,
+                           p->d_v16out, p->d_v16dim, p->d_v16net, tp.msg_off, ctl, p->d,
+                           res_fid, res_abs);
+        return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
     const size_t half = (size_t)p->nv16 * p->d * V16_ELEMS;
     const unsigned short *s0 = p->d_v16sites, *s1 = p->d_v16sites + half;
 #define V16_LAUNCH(NW)                                                                         \

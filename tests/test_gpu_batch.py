@@ -1,6 +1,6 @@
 """Batched uneven networks on the GPU: per-network convergence masks, mixed dispatch
-(vertex kernels for the networks with every bond dimension <= 8, generic kernel for the
-rest, in the same sweep), against the FP64 oracle per network."""
+(vertex kernels for the networks with every bond dimension <= 12 in FP32, generic kernel
+for the rest, in the same sweep), against the FP64 oracle per network."""
 import numpy as np
 import pytest
 
@@ -41,8 +41,9 @@ def _check_batch(pkg, orc, tops, sites, dtype, mtol, ltol, maxiter=400, tol=1e-1
 
 def test_mixed_dispatch_batch(pkg, orc):
     _say("case mixed dispatch batch")
-    # bond dimensions <= 8 run on the vertex kernels, D = 10 on the generic kernel
-    specs = [(5, 8, False), (4, 3, False), (3, 10, False), (3, 5, True)]
+    # bond dimensions <= 8 run on the D <= 8 vertex kernels, D = 10 on the FP32 large-bond
+    # kernel (9..12), D = 13 on the generic kernel
+    specs = [(5, 8, False), (4, 3, False), (3, 10, False), (3, 13, False), (3, 5, True)]
     tops, sites = [], []
     for k, (L, D, hh) in enumerate(specs):
         g = pkg.heavy_hex_lattice(2, L) if hh else pkg.named_grid((L, L))

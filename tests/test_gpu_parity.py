@@ -456,6 +456,81 @@ def test_vertex_low_lds_variant(pkg, orc, shape, D, d):
     assert _rel(res[1][1], res[0][1]) < 1e-5
 
 
+# ---------------------------------------------------------------- bond dimensions 9..12 (FP32)
+
+@pytest.mark.parametrize("shape,D,d", [((4, 3), 12, 2), ((3, 3), 10, 2), ((3, 4), 9, 1),
+                                       ((3, 3), 12, 3), ((2, 5), 11, 4)])
+def test_vertex12_single_sweep_fp32(pkg, orc, shape, D, d):
+    """The FP32 MFMA kernel for bond dimensions 9..12 (padded to 12): one sweep from random
+    non-symmetric messages within the FP32 error bound, and from identity messages within
+    1e-5 relative of the FP64 oracle."""
+    _say(f"case vertex12 single sweep {shape} D={D} d={d}")
+    t = pkg.build_topology(pkg.named_grid(shape), D, d)
+    s = pkg.peps.random_sites(t, 61, scale=pkg.peps.peps_scale(d, D))
+    m = pkg.peps.random_messages(t, 61)
+    with pkg.Engine(t, s, dtype="f32") as e:
+        assert e.path() == "vertex"
+        e.set_messages(m)
+        e.sweep()
+        got = e.messages()
+    assert _fp32_sweep_check(pkg, orc, t, s, m, got) < 1.0
+    mi = pkg.peps.identity_messages(t)
+    want_i, _, _ = orc.sweep_jacobi(t, s, mi)
+    with pkg.Engine(t, s, dtype="f32") as e:
+        e.set_messages(mi)
+        e.sweep()
+        err = _rel(e.messages(), want_i)
+    _say(f"D={D} d={d}: identity-message sweep rel err {err:.2e}")
+    assert err < 1e-5
+
+
+def test_vertex12_mixed_dims_and_generic_agree(pkg, orc):
+    """Bond dimensions 9..12 within one network: within the FP32 bound, and the MFMA kernel
+    and the generic FP32 kernel agree after 20 sweeps."""
+    _say("case vertex12 mixed bond dims")
+    g = pkg.named_grid((3, 4))
+    dims = {e: 9 + (i % 4) for i, e in enumerate(g.edges())}
+    t = pkg.build_topology(g, dims, 2)
+    s = pkg.peps.random_sites(t, 62)
+    m = pkg.peps.random_messages(t, 62)
+    with pkg.Engine(t, s, dtype="f32") as e:
+        assert e.path() == "vertex"
+        e.set_messages(m)
+        e.sweep()
+        assert _fp32_sweep_check(pkg, orc, t, s, m, e.messages()) < 1.0
+    res = {}
+    for path in ("generic", "auto"):
+        with pkg.Engine(t, s, dtype="f32", path=path) as e:
+            e.init_messages("identity")
+            e.run(maxiter=20)
+            res[path] = (e.messages(), e.lognorm())
+    assert _rel(res["auto"][0], res["generic"][0]) < 1e-5
+    assert res["auto"][1] == pytest.approx(res["generic"][1], rel=1e-6)
+
+
+def test_vertex12_converged_fp32_vs_oracle(pkg, orc):
+    """A 5x5 D=12 lattice converged in FP32 on the large-bond kernel against the FP64
+    oracle: messages within 1e-4 relative, log-norm within 1e-5."""
+    _say("case vertex12 converged 5x5 D=12")
+    t = _grid(pkg, 5, 12)
+    s = pkg.peps.random_sites(t, 63, scale=pkg.peps.peps_scale(2, 12))
+    m0 = pkg.peps.identity_messages(t)
+    want, sw, _, _ = orc.bp(t, s, m0, maxiter=400, tol=1e-12, criterion="maxabs", nthreads=8)
+    with pkg.Engine(t, s, dtype="f32") as e:
+        assert e.path() == "vertex"
+        e.set_messages(m0)
+        r = e.run(maxiter=400, tol=1e-10, criterion="fidelity")
+        got = e.messages()
+        ln = e.lognorm()
+    assert r.converged[0]
+    err = _rel(got, want)
+    lw = orc.bethe_free_energy(t, s, want)
+    _say(f"sweeps {r.sweeps[0]} (oracle {sw}), msg rel err {err:.2e}, "
+         f"log-norm rel err {abs(ln - lw) / abs(lw):.2e}")
+    assert err < 1e-4
+    assert abs(ln - lw) / abs(lw) < 1e-5
+
+
 # ---------------------------------------------------------------- bond dimensions 9..16 (BF16)
 
 @pytest.mark.parametrize("shape,D,d", [((4, 3), 16, 2), ((3, 3), 12, 2), ((3, 4), 10, 1),
