@@ -114,7 +114,7 @@ def uneven_batch_timing(pkg, torch, seed, sweeps=20):
     t0 = time.perf_counter()
     ntuned = e.autotune(mode="all")
     t_tune = time.perf_counter() - t0
-    e.run(maxiter=2, use_graph=False)  # warm up
+    e.run(maxiter=sweeps, use_graph=True)  # warm up (captures the sweep graph)
     e.init_messages("identity")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -187,10 +187,11 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--only", choices=["adjoint", "large_bond"], default=None,
+    ap.add_argument("--only", choices=["adjoint", "large_bond", "uneven"], default=None,
                     help="adjoint: run only the forward + adjoint workload (L x L, D; three "
                          "energy + gradient calls); large_bond: only the 24x24 D=16 BF16 + "
-                         "FP64-polish workload; print the timings, for profiling")
+                         "FP64-polish workload; uneven: only the 64-network batched "
+                         "workload; print the timings, for profiling")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, used to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -211,6 +212,11 @@ def main(argv=None):
         runs = [adjoint_timing(pkg, torch, args.L, args.D, args.dtype, args.seed)
                 for _ in range(3)]
         print(json.dumps({"adjoint_only": runs}), flush=True)
+        return 0
+    if args.only == "uneven":
+        torch.cuda.init()
+        print(json.dumps({"uneven_only": uneven_batch_timing(pkg, torch, args.seed)}),
+              flush=True)
         return 0
     if args.only == "large_bond":
         torch.cuda.init()

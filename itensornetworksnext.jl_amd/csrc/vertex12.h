@@ -23,9 +23,11 @@
 // through LDS into the same per-slice buffer as the BF16 kernel, and k_v16_finish sums the
 // slices, normalises, writes and forms the residuals.
 //
-// Per slice: for each of the four outgoing ports, reload the slice and absorb the other
-// three ports' messages (12 mode products), then open.  Ports of a padded (dummy) slot are
-// skipped (identity message).  Arithmetic: FP32 operands and accumulation throughout.
+// Per slice: the partial product E23 (ports 2, 3 absorbed) serves outgoing ports 0 and 1,
+// E01 serves 2 and 3; while the first of each pair absorbs its third message in LDS, the
+// partial waits in registers (28 words per thread), so a slice costs 8 mode products and
+// two loads from HBM.  Ports of a padded (dummy) slot are skipped (identity message).
+// Arithmetic: FP32 operands and accumulation throughout.
 #pragma once
 
 #include "vertex.h"
@@ -148,23 +150,13 @@ __device__ __forceinline__ void v12_mode_if(float *S, const int (&ins)[4],
     __syncthreads();
 }
 
-// One outgoing port O of the slice: reload, absorb ports Q0, Q1, Q2, open, reduce the
-// waves' shares, store the unnormalised 16 x 16 block.
-template <int O, int Q0, int Q1, int Q2>
-__device__ __forceinline__ void v12_out(float *S, float *red, const float *src,
-                                        const float *src_w, float *dst, const int (&ins)[4],
-                                        const int (&outs)[4], const float (&Am)[4][3]) {
-    if (outs[O] < 0) return;  // uniform over the workgroup
+// Open on port O with the bra operands in `pre`, reduce the waves' shares through LDS,
+// store the unnormalised 16 x 16 block.
+template <int O>
+__device__ __forceinline__ void v12_open_store(const float *S, float *red, float *dst,
+                                               const float4 (&pre)[9]) {
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = lane & 15, kk = lane >> 4;
-    float4 pre[9];
-    v12_bra_prefetch<O>(src, src_w, pre);
-    __syncthreads();  // the previous port has finished with the slice and the partials
-    v12_load_slice(S, src);
-    __syncthreads();
-    v12_mode_if<Q0>(S, ins, Am);
-    v12_mode_if<Q1>(S, ins, Am);
-    v12_mode_if<Q2>(S, ins, Am);
     const v4f acc = v12_open<O>(S, pre);
 #pragma unroll
     for (int i = 0; i < 4; ++i) red[w * 256 + (4 * kk + i) * 16 + n] = acc[i];
@@ -174,6 +166,63 @@ __device__ __forceinline__ void v12_out(float *S, float *red, const float *src,
 #pragma unroll
         for (int v = 0; v < V12_NW; ++v) x += red[v * 256 + e];
         dst[O * 256 + e] = x;
+    }
+}
+
+// Raw LDS words of the slice buffer held by this thread while the buffer is reused
+// (20892 words over 768 threads: 28 each, the last partly).
+static constexpr int V12_PARK = (V12_LDS_ELEMS + 64 * V12_NW - 1) / (64 * V12_NW);
+
+__device__ __forceinline__ void v12_park(const float *S, float (&pk)[V12_PARK]) {
+#pragma unroll
+    for (int i = 0; i < V12_PARK; ++i) {
+        const int a = threadIdx.x + 64 * V12_NW * i;
+        pk[i] = a < V12_LDS_ELEMS ? S[a] : 0.f;
+    }
+}
+
+__device__ __forceinline__ void v12_unpark(float *S, const float (&pk)[V12_PARK]) {
+#pragma unroll
+    for (int i = 0; i < V12_PARK; ++i) {
+        const int a = threadIdx.x + 64 * V12_NW * i;
+        if (a < V12_LDS_ELEMS) S[a] = pk[i];
+    }
+}
+
+// Outgoing ports O1 and O2 of the slice from one shared partial product: reload the slice,
+// absorb ports X0, X1 (E23 for O = 0, 1; E01 for O = 2, 3), keep the partial in registers,
+// then absorb Y1 and open O1, restore the partial, absorb Y2 and open O2.
+template <int O1, int O2, int X0, int X1, int Y1, int Y2>
+__device__ __forceinline__ void v12_pair(float *S, float *red, const float *src,
+                                         const float *src_w, float *dst, const int (&ins)[4],
+                                         const int (&outs)[4], const float (&Am)[4][3]) {
+    const bool first = outs[O1] >= 0, second = outs[O2] >= 0;  // uniform
+    if (!first && !second) return;
+    float4 pre[9];
+    if (first) v12_bra_prefetch<O1>(src, src_w, pre);
+    else v12_bra_prefetch<O2>(src, src_w, pre);
+    __syncthreads();  // earlier opens have finished with the slice and the partials
+    v12_load_slice(S, src);
+    __syncthreads();
+    v12_mode_if<X0>(S, ins, Am);
+    v12_mode_if<X1>(S, ins, Am);
+    float pk[V12_PARK];
+    if (first && second) {
+        v12_park(S, pk);
+        __syncthreads();
+    }
+    if (first) {
+        v12_mode_if<Y1>(S, ins, Am);
+        v12_open_store<O1>(S, red, dst, pre);
+    }
+    if (second) {
+        if (first) {
+            v12_bra_prefetch<O2>(src, src_w, pre);
+            v12_unpark(S, pk);  // the open above has passed its barrier
+            __syncthreads();
+        }
+        v12_mode_if<Y2>(S, ins, Am);
+        v12_open_store<O2>(S, red, dst, pre);
     }
 }
 
@@ -210,10 +259,9 @@ __global__ __launch_bounds__(64 * V12_NW) void k_vertex_d12(
     const long long so = ((long long)item * d + s) * V12_ELEMS;
     const float *src = vsites_n + so, *src_w = vsites_w + so;
     float *dst = part + ((long long)item * d + s) * 4 * 256;
-    v12_out<0, 2, 3, 1>(S, red, src, src_w, dst, ins, outs, Am);
-    v12_out<1, 2, 3, 0>(S, red, src, src_w, dst, ins, outs, Am);
-    v12_out<2, 0, 1, 3>(S, red, src, src_w, dst, ins, outs, Am);
-    v12_out<3, 0, 1, 2>(S, red, src, src_w, dst, ins, outs, Am);
+    // E23 serves outgoing ports 0 (absorb 1) and 1 (absorb 0); E01 serves 2 and 3
+    v12_pair<0, 1, 2, 3, 1, 0>(S, red, src, src_w, dst, ins, outs, Am);
+    v12_pair<2, 3, 0, 1, 3, 2>(S, red, src, src_w, dst, ins, outs, Am);
 }
 
 // Padded FP32 site entries of the bucket: vsites_n[item][s][p0][p1][p2][p3] (12 each) and
