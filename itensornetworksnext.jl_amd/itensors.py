@@ -73,11 +73,14 @@ def random_itensor(inds: Sequence[Index], rng=None, dtype=np.float64) -> ITensor
 
 
 class ITensorNetwork:
-    """vertex -> ITensor with edges inferred from shared index names."""
+    """vertex -> ITensor with edges inferred from shared index names.  Edges are added in
+    the order the tensors and their indices arrive (never in hash order), so the same
+    construction gives the same edge order, and with it the same default BP schedule."""
 
     def __init__(self, tensors: Dict[Hashable, ITensor] | None = None):
         self.tensors: Dict[Hashable, ITensor] = {}
-        self.dimname_vertices: Dict[Index, set] = {}
+        # index -> vertices holding it, in insertion order (a dict used as an ordered set)
+        self.dimname_vertices: Dict[Index, dict] = {}
         self.graph = NamedGraph()
         for v, t in (tensors or {}).items():
             self[v] = t
@@ -86,21 +89,25 @@ class ITensorNetwork:
     def __setitem__(self, v, t: ITensor):
         if not isinstance(t, ITensor):
             raise TypeError("ITensorNetwork holds ITensors")
-        old = set(self.tensors[v].inds) if v in self.tensors else set()
-        new = set(t.inds)
+        old = list(self.tensors[v].inds) if v in self.tensors else []
+        new = list(t.inds)
         self.graph.add_vertex(v)
-        for ind in old - new:
-            vs = self.dimname_vertices.get(ind, set())
-            vs.discard(v)
+        for ind in old:
+            if ind in new:
+                continue
+            vs = self.dimname_vertices.get(ind, {})
+            vs.pop(v, None)
             if not vs:
                 self.dimname_vertices.pop(ind, None)
             self._rebuild_edges()
-        for ind in new - old:
-            vs = self.dimname_vertices.setdefault(ind, set())
+        for ind in new:
+            if ind in old:
+                continue
+            vs = self.dimname_vertices.setdefault(ind, {})
             if len(vs) > 1:
                 raise ValueError(f"index {ind} can appear in at most one existing tensor, "
                                  f"got {len(vs)}.")
-            vs.add(v)
+            vs[v] = None
             if len(vs) == 2:
                 a, b = tuple(vs)
                 if not self.graph.has_edge(a, b):

@@ -57,7 +57,8 @@ def test_algorithm_chain_and_tree_maxiter1(pkg, dtype, rtol):
 @pytest.mark.parametrize("n", [3, 4, 5])
 def test_spin_ice_analytical(pkg, n):
     """exp(bethe_free_energy) == 1.5^(n^2) on the periodic n x n spin-ice network
-    (test_beliefpropagation.jl:204-225), default strategy, maxiter 10, tol 1e-10."""
+    (test_beliefpropagation.jl:204-225), default strategy, maxiter 10, tol 1e-10; the
+    reference's `≈` uses Julia's default rtol, sqrt(eps(Float64))."""
     _say(f"case api spin ice {n}")
     g = pkg.named_grid((n, n), periodic=True)
     l = _links(pkg, g, 2)
@@ -70,7 +71,8 @@ def test_spin_ice_analytical(pkg, n):
     rng = np.random.default_rng(7)
     msgs = {e: rng.random(2) for e in g.all_edges()}
     cache = pkg.beliefpropagation(tn, msgs, stopping_criterion={"maxiter": 10, "tol": 1e-10})
-    assert math.exp(pkg.bethe_free_energy(tn, cache)) == pytest.approx(1.5 ** (n * n), rel=1e-8)
+    assert math.exp(pkg.bethe_free_energy(tn, cache)) == pytest.approx(
+        1.5 ** (n * n), rel=math.sqrt(np.finfo(np.float64).eps))
 
 
 def _peps(pkg, L, D, seed):

@@ -1,5 +1,6 @@
 """Autotuner: timed choices on the engine's own data (vertex kernels or generic kernel per
-largest bond dimension <= 8; workgroup size and storage per generic bucket), persisted to
+largest bond dimension <= 12 in FP32; workgroup size and storage per generic bucket, here
+the D = 13 networks), persisted to
 a JSON cache and applied again from it.  Tuning changes how a sweep is scheduled, never what
 it computes: results after tuning match the untuned engine within the FP32 bound."""
 import json
@@ -20,7 +21,7 @@ def _rel(a, b):
 
 def _batch(pkg):
     tops, sites = pkg.peps.uneven_batch(seed=5, nnet=8, L_range=(3, 6),
-                                        D_choices=(3, 6, 8, 10, 12))
+                                        D_choices=(3, 6, 8, 12, 13))
     return pkg.concat_topologies(tops), np.concatenate(sites), tops
 
 
