@@ -32,6 +32,7 @@
 // are stored as BF16 pairs (r, r+1) per lane so each load is a coalesced 256-byte row.
 #pragma once
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -70,6 +71,9 @@ struct VertexPlan {
     int *d_v16net = nullptr, *d_v16list = nullptr;
     unsigned short *d_v16sites = nullptr;  // BF16: padded slices, 16^4 entries each
     float *d_v12sites = nullptr;           // FP32: padded slices, 12^4 entries each (2 layouts)
+    int nv16_k4 = 0;                       // FP32: items [0, nv16_k4) have degree 4, the
+                                           // rest degree <= 3 (k_vertex_d12_k3)
+    float *d_v3sites = nullptr;            // FP32, degree <= 3: 12^3 slices (3 layouts)
     float *d_v16part = nullptr;            // per-slice unnormalised messages
     int v16_waves = 16;                    // waves per slice workgroup: 4, 8 or 16
 };
@@ -82,14 +86,16 @@ static void vertex_plan_release(VertexPlan *p) {
     if (p->d_vin) (void)hipFree(p->d_vin);
     if (p->d_vout) (void)hipFree(p->d_vout);
     if (p->d_vdim) (void)hipFree(p->d_vdim);
-    void *v16[8] = {p->d_v16in, p->d_v16out, p->d_v16dim, p->d_v16net, p->d_v16list,
-                    p->d_v16sites, p->d_v16part, p->d_v12sites};
+    void *v16[9] = {p->d_v16in, p->d_v16out, p->d_v16dim, p->d_v16net, p->d_v16list,
+                    p->d_v16sites, p->d_v16part, p->d_v12sites, p->d_v3sites};
     for (void *x : v16)
         if (x) (void)hipFree(x);
     p->d_v16in = p->d_v16out = p->d_v16dim = nullptr;
     p->d_v16net = p->d_v16list = nullptr;
     p->d_v16sites = nullptr;
     p->d_v12sites = nullptr;
+    p->d_v3sites = nullptr;
+    p->nv16_k4 = 0;
     p->d_v16part = nullptr;
     p->nv16 = 0;
     if (p->d_vnet) (void)hipFree(p->d_vnet);
@@ -175,7 +181,15 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
         vdim[b] = make_int4(dm[0], dm[1], dm[2], dm[3]);
         vn[b] = vnet[u];
     }
-    // the large-bond bucket, laid out the same way
+    // the large-bond bucket, laid out the same way; in FP32 the degree-4 items first, then
+    // the degree <= 3 items (their own kernel)
+    int n16_k4 = (int)vlist16.size();
+    if (dtype == 1) {
+        std::stable_partition(vlist16.begin(), vlist16.end(),
+                              [&](int u) { return port_off[u + 1] - port_off[u] == 4; });
+        n16_k4 = 0;
+        for (int u : vlist16) n16_k4 += port_off[u + 1] - port_off[u] == 4;
+    }
     const int n16 = (int)vlist16.size();
     std::vector<int4> vin16(n16), vout16(n16), vdim16(n16);
     std::vector<int> vn16(n16);
@@ -214,6 +228,7 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     p->d = d;
     p->nv = nx;
     p->nv16 = n16;
+    p->nv16_k4 = n16_k4;
     p->gen_slots = gen_slots;
     p->ngen = (int)gen_slots.size();
     p->all = gen_slots.empty();

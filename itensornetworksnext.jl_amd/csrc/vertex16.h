@@ -385,14 +385,26 @@ static int vertex16_upload_sites(VertexPlan *p, hipStream_t st, const double *d_
                                  const long long *d_site_off, const int *d_port_off,
                                  const int *d_port_edge, const int *d_edge_dim) {
     if (p->nv16 == 0) return 0;
-    if (!p->bf16) {  // FP32: the bond-dimension-12 kernel's two layouts
-        const size_t n = (size_t)p->nv16 * p->d * V12_ELEMS;
+    if (!p->bf16) {  // FP32: the bond-dimension-12 kernels' layouts (degree 4, degree <= 3)
+        const int n4 = p->nv16_k4, n3 = p->nv16 - p->nv16_k4;
         if (p->d_v12sites) (void)hipFree(p->d_v12sites);
+        if (p->d_v3sites) (void)hipFree(p->d_v3sites);
         p->d_v12sites = nullptr;
-        if (hipMalloc((void **)&p->d_v12sites, 2 * n * sizeof(float)) != hipSuccess) return 5;
-        hipLaunchKernelGGL(k_v12_pad_sites, dim3(4096), dim3(256), 0, st, d_sites64, d_site_off,
-                           d_port_off, d_port_edge, d_edge_dim, p->d_v16list, p->d, p->nv16,
-                           p->d_v12sites, p->d_v12sites + n);
+        p->d_v3sites = nullptr;
+        if (n4 > 0) {
+            const size_t n = (size_t)n4 * p->d * V12_ELEMS;
+            if (hipMalloc((void **)&p->d_v12sites, 2 * n * sizeof(float)) != hipSuccess) return 5;
+            hipLaunchKernelGGL(k_v12_pad_sites, dim3(4096), dim3(256), 0, st, d_sites64,
+                               d_site_off, d_port_off, d_port_edge, d_edge_dim, p->d_v16list,
+                               p->d, n4, p->d_v12sites, p->d_v12sites + n);
+        }
+        if (n3 > 0) {
+            const size_t n = (size_t)n3 * p->d * 3 * V3_ELEMS;
+            if (hipMalloc((void **)&p->d_v3sites, n * sizeof(float)) != hipSuccess) return 5;
+            hipLaunchKernelGGL(k_v3_pad_sites, dim3(4096), dim3(256), 0, st, d_sites64,
+                               d_site_off, d_port_off, d_port_edge, d_edge_dim,
+                               p->d_v16list + n4, p->d, n3, p->d_v3sites);
+        }
         return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     const size_t bytes = 2 * (size_t)p->nv16 * p->d * V16_ELEMS;
@@ -427,11 +439,22 @@ static int vertex16_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void 
                           const NetCtl &ctl, double *res_fid, double *res_abs) {
     if (p->nv16 == 0) return 0;
     if (!p->bf16) {
-        const size_t n = (size_t)p->nv16 * p->d * V12_ELEMS;
-        hipLaunchKernelGGL(k_vertex_d12, dim3(p->nv16 * p->d), dim3(64 * V12_NW), v12_lds(), st,
-                           (const float *)p->d_v12sites, (const float *)(p->d_v12sites + n),
-                           (const float *)src, p->d_v16in, p->d_v16out, p->d_v16dim,
-                           p->d_v16net, tp.msg_off, ctl, p->d, p->d_v16part);
+        const int n4 = p->nv16_k4, n3 = p->nv16 - p->nv16_k4;
+        if (n4 > 0) {
+            const size_t n = (size_t)n4 * p->d * V12_ELEMS;
+            hipLaunchKernelGGL(k_vertex_d12, dim3(n4 * p->d), dim3(64 * V12_NW), v12_lds(), st,
+                               (const float *)p->d_v12sites,
+                               (const float *)(p->d_v12sites + n), (const float *)src,
+                               p->d_v16in, p->d_v16out, p->d_v16dim, p->d_v16net, tp.msg_off,
+                               ctl, p->d, p->d_v16part);
+        }
+        if (n3 > 0) {
+            const int nsl = n3 * p->d;
+            hipLaunchKernelGGL(k_vertex_d12_k3, dim3((nsl + V3_WAVES - 1) / V3_WAVES),
+                               dim3(64 * V3_WAVES), 0, st, (const float *)p->d_v3sites,
+                               (const float *)src, p->d_v16in, p->d_v16out, p->d_v16dim,
+                               p->d_v16net, tp.msg_off, ctl, p->d, n4, n3, p->d_v16part);
+        }
         hipLaunchKernelGGL(k_v16_finish, dim3(p->nv16), dim3(256), 0, st,
                            (const float *)p->d_v16part, (const float *)src, (float *)dst,
                            p->d_v16out, p->d_v16dim, p->d_v16net, tp.msg_off, ctl, p->d,

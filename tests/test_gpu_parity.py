@@ -484,6 +484,30 @@ def test_vertex12_single_sweep_fp32(pkg, orc, shape, D, d):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("kind,D", [("heavyhex", 12), ("heavyhex", 10), ("path", 11)])
+def test_vertex12_low_degree_single_sweep(pkg, orc, kind, D):
+    """Degree <= 3 vertices of the FP32 large-bond bucket (their own 12^3-slice kernel):
+    heavy-hex lattices (degrees 2 and 3) and a path graph (degrees 1 and 2), one sweep from
+    random messages within the FP32 bound and from identity messages within 1e-5."""
+    _say(f"case vertex12 low degree {kind} D={D}")
+    g = pkg.heavy_hex_lattice(3, 3) if kind == "heavyhex" else pkg.named_path_graph(5)
+    t = pkg.build_topology(g, D, 2)
+    s = pkg.peps.random_sites(t, 64, scale=pkg.peps.peps_scale(2, D))
+    m = pkg.peps.random_messages(t, 64)
+    with pkg.Engine(t, s, dtype="f32") as e:
+        assert e.path() == "vertex"
+        e.set_messages(m)
+        e.sweep()
+        got = e.messages()
+    assert _fp32_sweep_check(pkg, orc, t, s, m, got) < 1.0
+    mi = pkg.peps.identity_messages(t)
+    want_i, _, _ = orc.sweep_jacobi(t, s, mi)
+    with pkg.Engine(t, s, dtype="f32") as e:
+        e.set_messages(mi)
+        e.sweep()
+        assert _rel(e.messages(), want_i) < 1e-5
+
+
 def test_vertex12_mixed_dims_and_generic_agree(pkg, orc):
     """Bond dimensions 9..12 within one network: within the FP32 bound, and the MFMA kernel
     and the generic FP32 kernel agree after 20 sweeps."""
