@@ -22,8 +22,9 @@
 // does what the D <= 8 kernels do at their end (normalisation, residuals, write-back of the
 // D x D block).
 //
-// Per slice: for each of the four outgoing ports, reload the slice and absorb the three
-// other ports' messages (12 mode products), then open.  Ports of a padded (dummy) slot are
+// Per slice: E23 (ports 2, 3 absorbed) serves outgoing ports 0 and 1, E01 serves 2 and 3;
+// while the first port of a pair absorbs its third message in LDS the partial waits in
+// registers (8 mode products, two slice loads; with 4 waves per slice, 12 and four).  Ports of a padded (dummy) slot are
 // skipped: their message is the identity.  Arithmetic: BF16 operands (site entries,
 // messages scaled by 1 / max|m|, and every partial product), FP32 accumulation, as the
 // D <= 8 BF16 kernel.
@@ -237,6 +238,82 @@ __device__ __forceinline__ void v16_out(unsigned char *S, float *red, const unsi
     }
 }
 
+// Outgoing ports O1 and O2 from one shared partial product: reload the slice, absorb X0, X1
+// (E23 for O = 0, 1; E01 for O = 2, 3), keep the partial in registers (the raw LDS bytes,
+// 128 / NW 16-byte chunks per thread), absorb Y1 and open O1, restore, absorb Y2 and open
+// O2: 8 mode products and two slice loads per slice instead of 12 and four.  With 4 waves
+// the partial would take 128 registers per thread, so that shape keeps the per-port path.
+template <int NW, int O1, int O2, int X0, int X1, int Y1, int Y2>
+__device__ __forceinline__ void v16_pair(unsigned char *S, float *red, const unsigned short *src,
+                                         const unsigned short *src_t, float *dst,
+                                         const int (&ins)[4], const int (&outs)[4],
+                                         const v4s (&Am)[4]) {
+    if constexpr (NW < 8) {
+        v16_out<NW, O1, X0, X1, Y1>(S, red, src, src_t, dst, ins, outs, Am);
+        v16_out<NW, O2, X0, X1, Y2>(S, red, src, src_t, dst, ins, outs, Am);
+    } else {
+        const bool first = outs[O1] >= 0, second = outs[O2] >= 0;  // uniform
+        if (!first && !second) return;
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int n = lane & 15, kg = lane >> 4;
+        constexpr int NP = V16_ELEMS * 2 / 16 / (64 * NW);
+        uint4 pre[128 / NW];
+        if (first) v16_bra_prefetch<O1, NW>(src, src_t, pre);
+        else v16_bra_prefetch<O2, NW>(src, src_t, pre);
+        __syncthreads();  // earlier opens have finished with the slice and the partials
+        v16_load_slice<NW>(S, src);
+        __syncthreads();
+        v16_mode_if<X0, NW>(S, ins, Am);
+        v16_mode_if<X1, NW>(S, ins, Am);
+        unsigned pk[4 * NP];  // scalars, so the partial stays in registers
+        if (first && second) {
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                const uint4 v = reinterpret_cast<const uint4 *>(S)[threadIdx.x + 64 * NW * i];
+                pk[4 * i] = v.x;
+                pk[4 * i + 1] = v.y;
+                pk[4 * i + 2] = v.z;
+                pk[4 * i + 3] = v.w;
+            }
+            __syncthreads();
+        }
+        if (first) {
+            v16_mode_if<Y1, NW>(S, ins, Am);
+            const v4f acc = v16_open<O1, NW>(S, pre);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) red[w * 256 + (4 * kg + i) * 16 + n] = acc[i];
+            __syncthreads();
+            for (int e = threadIdx.x; e < 256; e += blockDim.x) {
+                float x = 0.f;
+#pragma unroll
+                for (int v = 0; v < NW; ++v) x += red[v * 256 + e];
+                dst[O1 * 256 + e] = x;
+            }
+        }
+        if (second) {
+            if (first) {
+                v16_bra_prefetch<O2, NW>(src, src_t, pre);
+#pragma unroll
+                for (int i = 0; i < NP; ++i)
+                    reinterpret_cast<uint4 *>(S)[threadIdx.x + 64 * NW * i] =
+                        make_uint4(pk[4 * i], pk[4 * i + 1], pk[4 * i + 2], pk[4 * i + 3]);
+                __syncthreads();
+            }
+            v16_mode_if<Y2, NW>(S, ins, Am);
+            const v4f acc = v16_open<O2, NW>(S, pre);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) red[w * 256 + (4 * kg + i) * 16 + n] = acc[i];
+            __syncthreads();
+            for (int e = threadIdx.x; e < 256; e += blockDim.x) {
+                float x = 0.f;
+#pragma unroll
+                for (int v = 0; v < NW; ++v) x += red[v * 256 + e];
+                dst[O2 * 256 + e] = x;
+            }
+        }
+    }
+}
+
 // grid: items x d (one workgroup per slice), 64 NW threads.  part[((item * d + s) * 4 + j) * 256
 // + a * 16 + a'] receives slice s's unnormalised out_j.
 template <int NW>
@@ -279,12 +356,9 @@ void k_vertex_d16(const unsigned short *__restrict__ vsites16,
     const unsigned short *src = vsites16 + ((long long)item * d + s) * V16_ELEMS;
     const unsigned short *src_t = vsites16t + ((long long)item * d + s) * V16_ELEMS;
     float *dst = part + ((long long)item * d + s) * 4 * 256;
-    // outgoing port O: absorb the other three ports' messages (E01 order for O = 2, 3, E23
-    // order for O = 0, 1), then open O
-    v16_out<NW, 0, 2, 3, 1>(S, red, src, src_t, dst, ins, outs, Am);
-    v16_out<NW, 1, 2, 3, 0>(S, red, src, src_t, dst, ins, outs, Am);
-    v16_out<NW, 2, 0, 1, 3>(S, red, src, src_t, dst, ins, outs, Am);
-    v16_out<NW, 3, 0, 1, 2>(S, red, src, src_t, dst, ins, outs, Am);
+    // E23 serves outgoing ports 0 (absorb 1) and 1 (absorb 0); E01 serves 2 and 3
+    v16_pair<NW, 0, 1, 2, 3, 1, 0>(S, red, src, src_t, dst, ins, outs, Am);
+    v16_pair<NW, 2, 3, 0, 1, 3, 2>(S, red, src, src_t, dst, ins, outs, Am);
 }
 
 // grid: items, 256 threads (wave j = port j): sum the slices, normalise, residuals, write.
