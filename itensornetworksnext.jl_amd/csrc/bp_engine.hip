@@ -669,6 +669,8 @@ static int sweep_once(bp_context *c, const NetCtl &done) {
     if (path == BP_PATH_VERTEX || path == BP_PATH_MIXED) {
         RETCHK(vertex_sweep(&c->vplan, c->stream, c->topo(), c->d_vsites, src, dst, done,
                             c->d_res_fid, c->d_res_abs));
+        RETCHK(vertex_k3_sweep(&c->vplan, c->stream, c->topo(), src, dst, done,
+                               c->d_res_fid, c->d_res_abs));
         RETCHK(vertex16_sweep(&c->vplan, c->stream, c->topo(), src, dst, done, c->d_res_fid,
                               c->d_res_abs));
         HIPCHK(hipGetLastError());
@@ -690,6 +692,8 @@ static int upload_vertex_sites(bp_context *c) {
     bump_epoch();
     if (vertex_upload_sites(&c->vplan, c->stream, c->d_sites64, &c->d_vsites, c->d_site_off,
                             c->d_port_off, c->d_port_edge, c->d_edge_dim) != BP_OK ||
+        vertex_k3_upload_sites(&c->vplan, c->stream, c->d_sites64, c->d_site_off,
+                               c->d_port_off, c->d_port_edge, c->d_edge_dim) != BP_OK ||
         vertex16_upload_sites(&c->vplan, c->stream, c->d_sites64, c->d_site_off, c->d_port_off,
                               c->d_port_edge, c->d_edge_dim) != BP_OK)
         return set_err(BP_ERR_HIP, "vertex-path site layout failed");

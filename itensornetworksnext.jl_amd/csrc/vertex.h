@@ -74,6 +74,11 @@ struct VertexPlan {
     int nv16_k4 = 0;                       // FP32: items [0, nv16_k4) have degree 4, the
                                            // rest degree <= 3 (k_vertex_d12_k3)
     float *d_v3sites = nullptr;            // FP32, degree <= 3: 12^3 slices (3 layouts)
+    // FP32 networks of bond dimension <= 8 with every vertex of degree <= 3 (heavy-hex
+    // lattices, trees): the last nv_k3 items of the D <= 8 bucket, swept by k_vertex_k3<8>
+    int nv_k3 = 0;
+    float *d_vk3sites = nullptr;           // 8^3 slices, 3 layouts
+    float *d_vk3part = nullptr;            // per-slice unnormalised messages
     float *d_v16part = nullptr;            // per-slice unnormalised messages
     int v16_waves = 16;                    // waves per slice workgroup: 4, 8 or 16
 };
@@ -86,8 +91,9 @@ static void vertex_plan_release(VertexPlan *p) {
     if (p->d_vin) (void)hipFree(p->d_vin);
     if (p->d_vout) (void)hipFree(p->d_vout);
     if (p->d_vdim) (void)hipFree(p->d_vdim);
-    void *v16[9] = {p->d_v16in, p->d_v16out, p->d_v16dim, p->d_v16net, p->d_v16list,
-                    p->d_v16sites, p->d_v16part, p->d_v12sites, p->d_v3sites};
+    void *v16[11] = {p->d_v16in, p->d_v16out, p->d_v16dim, p->d_v16net, p->d_v16list,
+                     p->d_v16sites, p->d_v16part, p->d_v12sites, p->d_v3sites,
+                     p->d_vk3sites, p->d_vk3part};
     for (void *x : v16)
         if (x) (void)hipFree(x);
     p->d_v16in = p->d_v16out = p->d_v16dim = nullptr;
@@ -96,6 +102,8 @@ static void vertex_plan_release(VertexPlan *p) {
     p->d_v12sites = nullptr;
     p->d_v3sites = nullptr;
     p->nv16_k4 = 0;
+    p->d_vk3sites = p->d_vk3part = nullptr;
+    p->nv_k3 = 0;
     p->d_v16part = nullptr;
     p->nv16 = 0;
     if (p->d_vnet) (void)hipFree(p->d_vnet);
@@ -129,16 +137,17 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     // bucket
     const int dlim = dtype == 2 ? 16 : 12;
     int d = -1;
-    std::vector<int> vlist, vlist16, gen_slots;
+    std::vector<int> vlist, vlist16, gen_slots, vlist_k3;
     p->mask = mask;
     std::string last_reason = "no network has every bond dimension <= 8";
     for (int u0 = 0; u0 < nv;) {
         int u1 = u0;
         while (u1 < nv && vnet[u1] == vnet[u0]) ++u1;
         bool ok = true;
-        int dmax = 0;
+        int dmax = 0, kmax = 0;
         for (int u = u0; u < u1 && ok; ++u) {
             const int k = port_off[u + 1] - port_off[u];
+            kmax = std::max(kmax, k);
             if (k > 4) { ok = false; last_reason = "degree > 4"; }
             if (phys_dim[u] < 1 || phys_dim[u] > 4) { ok = false; last_reason = "physical dimension outside 1..4"; }
             if (d >= 0 && phys_dim[u] != d) { ok = false; last_reason = "physical dimension differs between networks"; }
@@ -157,13 +166,18 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
                           " is routed to the generic kernel (autotuned)";
         }
         if (ok && d < 0) d = phys_dim[u0];
+        // FP32 networks of degree <= 3 throughout: their own kernel, at the bucket's end
+        const bool k3 = ok && dtype == 1 && dmax <= VX_D && kmax <= 3;
         for (int u = u0; u < u1; ++u) {
-            if (ok && dmax <= VX_D) vlist.push_back(u);
+            if (k3) vlist_k3.push_back(u);
+            else if (ok && dmax <= VX_D) vlist.push_back(u);
             else if (ok) vlist16.push_back(u);
             else for (int q = port_off[u]; q < port_off[u + 1]; ++q) gen_slots.push_back(q);
         }
         u0 = u1;
     }
+    const int nk3 = (int)vlist_k3.size();
+    vlist.insert(vlist.end(), vlist_k3.begin(), vlist_k3.end());
     if (vlist.empty() && vlist16.empty()) { p->why = last_reason; return 4; }
     const int nx = (int)vlist.size();
     std::vector<int4> vin(nx), vout(nx), vdim(nx);
@@ -217,6 +231,9 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     if (!rc && n16 > 0 &&
         hipMalloc((void **)&p->d_v16part, sizeof(float) * (size_t)n16 * d * 4 * 256) != hipSuccess)
         rc = 5;
+    if (!rc && nk3 > 0 &&
+        hipMalloc((void **)&p->d_vk3part, sizeof(float) * (size_t)nk3 * d * 4 * 256) != hipSuccess)
+        rc = 5;
     if (!rc) rc = vx_upload(&p->d_vnet, vn);
     if (!rc) rc = vx_upload(&p->d_vlist, vlist);
     if (!rc) rc = vx_upload(&p->d_gen_slots, gen_slots);
@@ -229,6 +246,7 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     p->nv = nx;
     p->nv16 = n16;
     p->nv16_k4 = n16_k4;
+    p->nv_k3 = nk3;
     p->gen_slots = gen_slots;
     p->ngen = (int)gen_slots.size();
     p->all = gen_slots.empty();
@@ -764,13 +782,14 @@ static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *v
                         const void *src, void *dst, const NetCtl &ctl, double *res_fid,
                         double *res_abs) {
     (void)tp;
-    if (p->nv == 0) return 0;
+    const int n4 = p->nv - p->nv_k3;  // items on the four-port kernels
+    if (n4 == 0) return 0;
     if (p->variant == 1 && !p->bf16) {
         const size_t lds = vertex_lo_lds(p->d);
         switch (p->d) {
 #define VX_LO(NS)                                                                              \
     case NS:                                                                                   \
-        hipLaunchKernelGGL((k_vertex_d8_lo<NS>), dim3(p->nv), dim3(64 * NS), lds, st, vsites,  \
+        hipLaunchKernelGGL((k_vertex_d8_lo<NS>), dim3(n4), dim3(64 * NS), lds, st, vsites,     \
                            (const float *)src, (float *)dst, p->d_vin, p->d_vout, p->d_vdim,   \
                            p->d_vnet, tp.msg_off, ctl, res_fid, res_abs);                      \
         break;
@@ -788,7 +807,7 @@ static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *v
     switch (key) {
 #define VX_CASE(NS, BF)                                                                      \
     case NS * 2 + BF:                                                                        \
-        hipLaunchKernelGGL((k_vertex_d8<NS, BF>), dim3(p->nv), dim3(64 * NS), p->lds_bytes,  \
+        hipLaunchKernelGGL((k_vertex_d8<NS, BF>), dim3(n4), dim3(64 * NS), p->lds_bytes,     \
                            st, vsites, (const float *)src, (float *)dst, p->d_vin, p->d_vout, \
                            p->d_vdim, p->d_vnet, tp.msg_off, ctl, res_fid, res_abs);         \
         break;

@@ -299,48 +299,54 @@ __global__ void k_v12_pad_sites(const double *sites64, const long long *site_off
 
 // ---------------------------------------------------------------- degree <= 3 vertices
 //
-// A vertex with at most three ports has a 12^3 slice (6.9 KiB): one wave owns one slice
-// (four slices per workgroup, no workgroup barriers), with the same MFMA tiles as above:
-// fibres over the two other ports flattened (144 = 9 tiles of 16), contraction index in
-// three chunks of 4.  Per slice: E2 (port 2 absorbed) serves outgoing ports 0 and 1 (the
-// partial parked in registers meanwhile), port 2 absorbs 0 and 1 from a reload: 4 mode
-// products, 3 opens of K = 144.  The bra of the open on port O comes from a copy of the
-// slice with port O outermost, so each lane's four K values are 16 contiguous bytes.  A
-// degree-2 (or 1) vertex runs the same code with the missing ports skipped.
-static constexpr int V3_ELEMS = 1728;  // 12^3 per slice (HBM copies, unpadded)
-// LDS strides of (p0, p1, p2): blocks of 144 padded to 145
-static constexpr int V3_S0 = 145, V3_S1 = 12, V3_S2 = 1;
-static constexpr int V3_LDS = 12 * V3_S0;     // words per wave
-static constexpr int V3_WAVES = 4;             // slices per workgroup
-static constexpr int V3_PARK = (V3_LDS + 63) / 64;
-
-__host__ __device__ constexpr int v3_str(int j) { return j == 0 ? V3_S0 : (j == 1 ? V3_S1 : V3_S2); }
+// A vertex with at most three ports has a DP^3 slice (DP = 12: 6.9 KiB; DP = 8: 2 KiB, for
+// networks of bond dimension <= 8 whose vertices all have degree <= 3, such as heavy-hex
+// lattices): one wave owns one slice (four slices per workgroup, no workgroup barriers),
+// with the same MFMA tiles as above: fibres over the two other ports flattened (DP^2 / 16
+// tiles of 16), contraction index in DP / 4 chunks of 4.  Per slice: E2 (port 2 absorbed)
+// serves outgoing ports 0 and 1 (the partial parked in registers meanwhile), port 2 absorbs
+// 0 and 1 from a reload: 4 mode products, 3 opens of K = DP^2.  The bra of the open on
+// port O comes from a copy of the slice with port O outermost, so each lane's four K
+// values are 16 contiguous bytes.  A degree-2 (or 1) vertex runs the same code with the
+// missing ports skipped.  Results go to per-slice 16 x 16 partials that k_v16_finish
+// completes.
+template <int DP> struct V3 {
+    static constexpr int ELEMS = DP * DP * DP;     // per slice (HBM copies, unpadded)
+    static constexpr int S0 = DP * DP + 1, S1 = DP, S2 = 1;  // LDS strides of (p0, p1, p2)
+    static constexpr int LDS = DP * S0;            // words per wave
+    static constexpr int TILES = DP * DP / 16;     // fibre tiles (and K chunks of 16)
+    static constexpr int KC = DP / 4;              // contraction chunks of 4
+    static constexpr int PARK = (LDS + 63) / 64;
+    static constexpr int str(int j) { return j == 0 ? S0 : (j == 1 ? S1 : S2); }
+};
+static constexpr int V3_WAVES = 4;  // slices per workgroup
 
 template <int J> struct V3Ports {
     static constexpr int P = J == 0 ? 1 : 0;
     static constexpr int Q = J == 2 ? 1 : 2;
 };
 
-template <int J>
+template <int DP, int J>
 __device__ __forceinline__ void v3_mode(float *S, const float (&Am)[3]) {
+    using C = V3<DP>;
     using PT = V3Ports<J>;
-    constexpr int SJ = v3_str(J), SP = v3_str(PT::P), SQ = v3_str(PT::Q);
+    constexpr int SJ = C::str(J), SP = C::str(PT::P), SQ = C::str(PT::Q);
     const int lane = threadIdx.x & 63;
     const int n = lane & 15, kk = lane >> 4;
     const v4f zero = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int jf = 0; jf < 9; ++jf) {
+    for (int jf = 0; jf < C::TILES; ++jf) {
         const int f = 16 * jf + n;  // fibre over (P, Q)
-        const int p = f / 12, q = f - 12 * p;
+        const int p = f / DP, q = f - DP * p;
         const int base = p * SP + q * SQ;
-        float b[3];
+        float b[C::KC];
 #pragma unroll
-        for (int kc = 0; kc < 3; ++kc) b[kc] = S[base + (4 * kc + kk) * SJ];
+        for (int kc = 0; kc < C::KC; ++kc) b[kc] = S[base + (4 * kc + kk) * SJ];
         v4f c = zero;
 #pragma unroll
-        for (int kc = 0; kc < 3; ++kc)
+        for (int kc = 0; kc < C::KC; ++kc)
             c = __builtin_amdgcn_mfma_f32_16x16x4f32(Am[kc], b[kc], c, 0, 0, 0);
-        if (kk < 3) {
+        if (kk < C::KC) {
 #pragma unroll
             for (int t = 0; t < 4; ++t) S[base + (4 * kk + t) * SJ] = c[t];
         }
@@ -348,41 +354,43 @@ __device__ __forceinline__ void v3_mode(float *S, const float (&Am)[3]) {
     wave_lds_sync();  // the next product reads other lanes' rows
 }
 
-template <int J>
+template <int DP, int J>
 __device__ __forceinline__ void v3_mode_if(float *S, const int (&ins)[4],
                                            const float (&Am)[4][3]) {
     if (ins[J] < 0) return;  // missing or padded port (uniform over the wave)
-    v3_mode<J>(S, Am[J]);
+    v3_mode<DP, J>(S, Am[J]);
 }
 
 // bra rows of the open on port O: copy O of the slice, [O][the other two ascending]
-__device__ __forceinline__ void v3_bra_prefetch(const float *bra, float4 (&pre)[9]) {
+template <int DP>
+__device__ __forceinline__ void v3_bra_prefetch(const float *bra, float4 (&pre)[V3<DP>::TILES]) {
     const int lane = threadIdx.x & 63;
     const int n = lane & 15, kk = lane >> 4;
 #pragma unroll
-    for (int j = 0; j < 9; ++j) {
-        if (n < 12) pre[j] = *reinterpret_cast<const float4 *>(bra + n * 144 + 16 * j + 4 * kk);
+    for (int j = 0; j < V3<DP>::TILES; ++j) {
+        if (n < DP) pre[j] = *reinterpret_cast<const float4 *>(bra + n * DP * DP + 16 * j + 4 * kk);
         else pre[j] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
 
-// out_O[a][a'] over K = the two other ports (x, y) = divmod(K, 12); lane l returns entries
+// out_O[a][a'] over K = the two other ports (x, y) = divmod(K, DP); lane l returns entries
 // (4(l/16)+i, l%16)
-template <int O>
-__device__ __forceinline__ v4f v3_open(const float *S, const float4 (&pre)[9]) {
-    constexpr int SO = v3_str(O), SX = v3_str(O == 0 ? 1 : 0), SY = v3_str(O == 2 ? 1 : 2);
+template <int DP, int O>
+__device__ __forceinline__ v4f v3_open(const float *S, const float4 (&pre)[V3<DP>::TILES]) {
+    using C = V3<DP>;
+    constexpr int SO = C::str(O), SX = C::str(O == 0 ? 1 : 0), SY = C::str(O == 2 ? 1 : 2);
     const int lane = threadIdx.x & 63;
     const int n = lane & 15, kk = lane >> 4;
-    const bool live = n < 12;
+    const bool live = n < DP;
     const int base = (live ? n : 0) * SO;
     v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int j = 0; j < 9; ++j) {
+    for (int j = 0; j < C::TILES; ++j) {
         const float pv[4] = {pre[j].x, pre[j].y, pre[j].z, pre[j].w};
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int K = 16 * j + 4 * kk + t;
-            const int x = K / 12, y = K - 12 * x;
+            const int x = K / DP, y = K - DP * x;
             const float g = live ? S[base + x * SX + y * SY] : 0.f;
             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(g, pv[t], acc, 0, 0, 0);
         }
@@ -390,14 +398,16 @@ __device__ __forceinline__ v4f v3_open(const float *S, const float4 (&pre)[9]) {
     return acc;
 }
 
+template <int DP>
 __device__ __forceinline__ void v3_load_slice(float *S, const float *src) {
+    using C = V3<DP>;
     const int lane = threadIdx.x & 63;
     const float4 *g = reinterpret_cast<const float4 *>(src);
-    for (int c = lane; c < V3_ELEMS / 4; c += 64) {
+    for (int c = lane; c < C::ELEMS / 4; c += 64) {
         const float4 v = g[c];
         const int e = 4 * c;
-        const int p0 = e / 144, rem = e - 144 * p0;  // rem = 12 p1 + p2
-        float *d = S + p0 * V3_S0 + rem;
+        const int p0 = e / (DP * DP), rem = e - DP * DP * p0;  // rem = DP p1 + p2
+        float *d = S + p0 * C::S0 + rem;
         d[0] = v.x;
         d[1] = v.y;
         d[2] = v.z;
@@ -406,25 +416,28 @@ __device__ __forceinline__ void v3_load_slice(float *S, const float *src) {
     wave_lds_sync();
 }
 
-template <int O>
-__device__ __forceinline__ void v3_open_store(const float *S, const float4 (&pre)[9],
+template <int DP, int O>
+__device__ __forceinline__ void v3_open_store(const float *S, const float4 (&pre)[V3<DP>::TILES],
                                               float *dst) {
     const int lane = threadIdx.x & 63;
     const int n = lane & 15, kk = lane >> 4;
-    const v4f acc = v3_open<O>(S, pre);
+    const v4f acc = v3_open<DP, O>(S, pre);
 #pragma unroll
     for (int i = 0; i < 4; ++i) dst[O * 256 + (4 * kk + i) * 16 + n] = acc[i];
 }
 
-// grid: ceil(items * d / 4) workgroups of 4 waves; wave w takes slice 4 blockIdx + w of the
-// degree <= 3 items [item0, item0 + nitems) of the large-bond bucket.  vs3[(i * d + s) * 3 +
-// O] is slice s of the i-th such item with port O outermost.
-__global__ __launch_bounds__(64 * V3_WAVES) void k_vertex_d12_k3(
+// grid: ceil(nitems * d / 4) workgroups of 4 waves; wave w takes slice 4 blockIdx + w of the
+// items [item0, item0 + nitems) (for vin / vout / vdim / vnet); vs3[(i * d + s) * 3 + O]
+// is slice s of the i-th item with port O outermost, and part[(i * d + s) * 4 + j] receives
+// its unnormalised out_j.
+template <int DP>
+__global__ __launch_bounds__(64 * V3_WAVES) void k_vertex_k3(
     const float *__restrict__ vs3, const float *__restrict__ mold, const int4 *__restrict__ vin,
     const int4 *__restrict__ vout, const int4 *__restrict__ vdim, const int *__restrict__ vnet,
     const long long *__restrict__ msg_off, NetCtl ctl, int d, int item0, int nitems,
     float *__restrict__ part) {
-    __shared__ float v3sm[V3_WAVES * V3_LDS];
+    using C = V3<DP>;
+    __shared__ float v3sm[V3_WAVES * C::LDS];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = lane & 15, kk = lane >> 4;
     const int slice = blockIdx.x * V3_WAVES + w;
@@ -432,83 +445,83 @@ __global__ __launch_bounds__(64 * V3_WAVES) void k_vertex_d12_k3(
     const int i = slice / d, s = slice - i * d;
     const int item = item0 + i;
     if (ctl.enabled && ctl.done[vnet[item]]) return;
-    float *S = v3sm + w * V3_LDS;
+    float *S = v3sm + w * C::LDS;
     const int4 vi = vin[item], vo = vout[item], vd = vdim[item];
     const int ins[4] = {vi.x, vi.y, vi.z, vi.w};
     const int outs[4] = {vo.x, vo.y, vo.z, vo.w};
     const int dims[4] = {vd.x, vd.y, vd.z, vd.w};
     float Am[4][3];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
+    for (int j = 0; j < 4; ++j) {
 #pragma unroll
         for (int kc = 0; kc < 3; ++kc) {
             const int k = 4 * kc + kk, D = dims[j];
-            Am[j][kc] = (ins[j] >= 0 && k < D && n < D) ? mold[msg_off[ins[j]] + k * D + n]
-                                                         : 0.f;
+            Am[j][kc] = (j < 3 && kc < C::KC && ins[j] >= 0 && k < D && n < D)
+                            ? mold[msg_off[ins[j]] + k * D + n]
+                            : 0.f;
         }
     }
-#pragma unroll
-    for (int kc = 0; kc < 3; ++kc) Am[3][kc] = 0.f;
-    const float *src = vs3 + ((long long)i * d + s) * 3 * V3_ELEMS;
-    float *dst = part + ((long long)item * d + s) * 4 * 256;
-    float4 pre[9];
-    v3_load_slice(S, src);
+    const float *src = vs3 + ((long long)i * d + s) * 3 * C::ELEMS;
+    float *dst = part + ((long long)i * d + s) * 4 * 256;
+    float4 pre[C::TILES];
+    v3_load_slice<DP>(S, src);
     // E2 -> outgoing ports 0 (absorb 1) and 1 (absorb 0)
     const bool o0 = outs[0] >= 0, o1 = outs[1] >= 0;
-    v3_mode_if<2>(S, ins, Am);
-    float pk[V3_PARK];
+    v3_mode_if<DP, 2>(S, ins, Am);
+    float pk[C::PARK];
     if (o0 && o1) {
 #pragma unroll
-        for (int r = 0; r < V3_PARK; ++r) {
+        for (int r = 0; r < C::PARK; ++r) {
             const int a = lane + 64 * r;
-            pk[r] = a < V3_LDS ? S[a] : 0.f;
+            pk[r] = a < C::LDS ? S[a] : 0.f;
         }
     }
     if (o0) {
-        v3_bra_prefetch(src + 0 * V3_ELEMS, pre);
-        v3_mode_if<1>(S, ins, Am);
-        v3_open_store<0>(S, pre, dst);
+        v3_bra_prefetch<DP>(src + 0 * C::ELEMS, pre);
+        v3_mode_if<DP, 1>(S, ins, Am);
+        v3_open_store<DP, 0>(S, pre, dst);
     }
     if (o1) {
-        v3_bra_prefetch(src + 1 * V3_ELEMS, pre);
+        v3_bra_prefetch<DP>(src + 1 * C::ELEMS, pre);
         if (o0) {
             wave_lds_sync();  // the open above has finished reading the slice
 #pragma unroll
-            for (int r = 0; r < V3_PARK; ++r) {
+            for (int r = 0; r < C::PARK; ++r) {
                 const int a = lane + 64 * r;
-                if (a < V3_LDS) S[a] = pk[r];
+                if (a < C::LDS) S[a] = pk[r];
             }
             wave_lds_sync();
         }
-        v3_mode_if<0>(S, ins, Am);
-        v3_open_store<1>(S, pre, dst);
+        v3_mode_if<DP, 0>(S, ins, Am);
+        v3_open_store<DP, 1>(S, pre, dst);
     }
     // outgoing port 2: absorb 0 and 1 from a reload
     if (outs[2] >= 0) {
-        v3_bra_prefetch(src + 2 * V3_ELEMS, pre);
+        v3_bra_prefetch<DP>(src + 2 * C::ELEMS, pre);
         wave_lds_sync();
-        v3_load_slice(S, src);
-        v3_mode_if<0>(S, ins, Am);
-        v3_mode_if<1>(S, ins, Am);
-        v3_open_store<2>(S, pre, dst);
+        v3_load_slice<DP>(S, src);
+        v3_mode_if<DP, 0>(S, ins, Am);
+        v3_mode_if<DP, 1>(S, ins, Am);
+        v3_open_store<DP, 2>(S, pre, dst);
     }
 }
 
-// Site copies of the degree <= 3 items: vs3[(i * d + s) * 3 + O][a'][x][y], port O first,
-// then the other two ascending; missing ports keep index 0 only, bond indices beyond D are
-// zero.
+// Site copies of degree <= 3 items: vs3[(i * d + s) * 3 + O][a'][x][y], port O first, then
+// the other two ascending; missing ports keep index 0 only, bond indices beyond D are zero.
+template <int DP>
 __global__ void k_v3_pad_sites(const double *sites64, const long long *site_off,
                                const int *port_off, const int *port_edge, const int *edge_dim,
                                const int *vlist, int d, int nitems, float *out) {
-    const long long total = (long long)nitems * d * V3_ELEMS;
+    constexpr int E = V3<DP>::ELEMS;
+    const long long total = (long long)nitems * d * E;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
          i += (long long)gridDim.x * blockDim.x) {
-        const int e = (int)(i % V3_ELEMS);
-        const long long us = i / V3_ELEMS;
+        const int e = (int)(i % E);
+        const long long us = i / E;
         const int s = (int)(us % d);
         const int u = vlist[us / d];
         const int k = port_off[u + 1] - port_off[u];
-        const int idx[3] = {e / 144, (e / 12) % 12, e % 12};
+        const int idx[3] = {e / (DP * DP), (e / DP) % DP, e % DP};
         long long off = 0, cnt = 1;
         bool real = true;
         for (int j = 0; j < 3; ++j) {
@@ -522,9 +535,10 @@ __global__ void k_v3_pad_sites(const double *sites64, const long long *site_off,
             }
         }
         const float v = real ? (float)sites64[site_off[u] + s * cnt + off] : 0.f;
-        float *o = out + us * 3 * V3_ELEMS;
-        o[e] = v;                                                   // [p0][p1][p2]
-        o[V3_ELEMS + (idx[1] * 12 + idx[0]) * 12 + idx[2]] = v;     // [p1][p0][p2]
-        o[2 * V3_ELEMS + (idx[2] * 12 + idx[0]) * 12 + idx[1]] = v; // [p2][p0][p1]
+        float *o = out + us * 3 * E;
+        o[e] = v;                                                 // [p0][p1][p2]
+        o[E + (idx[1] * DP + idx[0]) * DP + idx[2]] = v;          // [p1][p0][p2]
+        o[2 * E + (idx[2] * DP + idx[0]) * DP + idx[1]] = v;      // [p2][p0][p1]
     }
 }
+

@@ -473,9 +473,9 @@ static int vertex16_upload_sites(VertexPlan *p, hipStream_t st, const double *d_
                                p->d, n4, p->d_v12sites, p->d_v12sites + n);
         }
         if (n3 > 0) {
-            const size_t n = (size_t)n3 * p->d * 3 * V3_ELEMS;
+            const size_t n = (size_t)n3 * p->d * 3 * V3<12>::ELEMS;
             if (hipMalloc((void **)&p->d_v3sites, n * sizeof(float)) != hipSuccess) return 5;
-            hipLaunchKernelGGL(k_v3_pad_sites, dim3(4096), dim3(256), 0, st, d_sites64,
+            hipLaunchKernelGGL(k_v3_pad_sites<12>, dim3(4096), dim3(256), 0, st, d_sites64,
                                d_site_off, d_port_off, d_port_edge, d_edge_dim,
                                p->d_v16list + n4, p->d, n3, p->d_v3sites);
         }
@@ -524,10 +524,11 @@ static int vertex16_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void 
         }
         if (n3 > 0) {
             const int nsl = n3 * p->d;
-            hipLaunchKernelGGL(k_vertex_d12_k3, dim3((nsl + V3_WAVES - 1) / V3_WAVES),
+            hipLaunchKernelGGL(k_vertex_k3<12>, dim3((nsl + V3_WAVES - 1) / V3_WAVES),
                                dim3(64 * V3_WAVES), 0, st, (const float *)p->d_v3sites,
                                (const float *)src, p->d_v16in, p->d_v16out, p->d_v16dim,
-                               p->d_v16net, tp.msg_off, ctl, p->d, n4, n3, p->d_v16part);
+                               p->d_v16net, tp.msg_off, ctl, p->d, n4, n3,
+                               p->d_v16part + (size_t)n4 * p->d * 4 * 256);
         }
         hipLaunchKernelGGL(k_v16_finish, dim3(p->nv16), dim3(256), 0, st,
                            (const float *)p->d_v16part, (const float *)src, (float *)dst,
@@ -550,5 +551,39 @@ static int vertex16_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void 
     hipLaunchKernelGGL(k_v16_finish, dim3(p->nv16), dim3(256), 0, st, (const float *)p->d_v16part,
                        (const float *)src, (float *)dst, p->d_v16out, p->d_v16dim, p->d_v16net,
                        tp.msg_off, ctl, p->d, res_fid, res_abs);
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+// ---------------------------------------------------------------- host side (D <= 8 bucket)
+
+// FP32 networks of bond dimension <= 8 whose vertices all have degree <= 3 sit at the end of
+// the D <= 8 bucket (items [nv - nv_k3, nv)); they sweep on k_vertex_k3<8> and the finisher
+// instead of the four-port kernel (their padded four-port layout stays, for the adjoint).
+static int vertex_k3_upload_sites(VertexPlan *p, hipStream_t st, const double *d_sites64,
+                                  const long long *d_site_off, const int *d_port_off,
+                                  const int *d_port_edge, const int *d_edge_dim) {
+    if (p->d_vk3sites) (void)hipFree(p->d_vk3sites);
+    p->d_vk3sites = nullptr;
+    if (p->nv_k3 == 0) return 0;
+    const size_t n = (size_t)p->nv_k3 * p->d * 3 * V3<8>::ELEMS;
+    if (hipMalloc((void **)&p->d_vk3sites, n * sizeof(float)) != hipSuccess) return 5;
+    hipLaunchKernelGGL(k_v3_pad_sites<8>, dim3(2048), dim3(256), 0, st, d_sites64, d_site_off,
+                       d_port_off, d_port_edge, d_edge_dim, p->d_vlist + (p->nv - p->nv_k3),
+                       p->d, p->nv_k3, p->d_vk3sites);
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+static int vertex_k3_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *src,
+                           void *dst, const NetCtl &ctl, double *res_fid, double *res_abs) {
+    if (p->nv_k3 == 0) return 0;
+    const int i0 = p->nv - p->nv_k3, nsl = p->nv_k3 * p->d;
+    hipLaunchKernelGGL(k_vertex_k3<8>, dim3((nsl + V3_WAVES - 1) / V3_WAVES),
+                       dim3(64 * V3_WAVES), 0, st, (const float *)p->d_vk3sites,
+                       (const float *)src, p->d_vin, p->d_vout, p->d_vdim, p->d_vnet, tp.msg_off,
+                       ctl, p->d, i0, p->nv_k3, p->d_vk3part);
+    hipLaunchKernelGGL(k_v16_finish, dim3(p->nv_k3), dim3(256), 0, st,
+                       (const float *)p->d_vk3part, (const float *)src, (float *)dst,
+                       p->d_vout + i0, p->d_vdim + i0, p->d_vnet + i0, tp.msg_off, ctl, p->d,
+                       res_fid, res_abs);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }

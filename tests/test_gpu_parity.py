@@ -508,6 +508,39 @@ def test_vertex12_low_degree_single_sweep(pkg, orc, kind, D):
         assert _rel(e.messages(), want_i) < 1e-5
 
 
+@pytest.mark.parametrize("kind,D,d", [("heavyhex", 8, 2), ("heavyhex", 5, 2), ("path", 8, 3),
+                                       ("heavyhex", 7, 1)])
+def test_vertex_k3_d8_single_sweep(pkg, orc, kind, D, d):
+    """FP32 networks of bond dimension <= 8 whose vertices all have degree <= 3 (heavy-hex
+    lattices, paths) sweep on the 8^3-slice kernel: one sweep from random messages within
+    the FP32 bound, from identity messages within 1e-5, and 30 sweeps agree with the
+    generic FP32 kernel within 1e-5 (log-norm 1e-6)."""
+    _say(f"case vertex k3 D<=8 {kind} D={D} d={d}")
+    g = pkg.heavy_hex_lattice(3, 4) if kind == "heavyhex" else pkg.named_path_graph(6)
+    t = pkg.build_topology(g, D, d)
+    s = pkg.peps.random_sites(t, 65, scale=pkg.peps.peps_scale(d, D))
+    m = pkg.peps.random_messages(t, 65)
+    with pkg.Engine(t, s, dtype="f32") as e:
+        assert e.path() == "vertex"
+        e.set_messages(m)
+        e.sweep()
+        got = e.messages()
+    assert _fp32_sweep_check(pkg, orc, t, s, m, got) < 1.0
+    mi = pkg.peps.identity_messages(t)
+    want_i, _, _ = orc.sweep_jacobi(t, s, mi)
+    res = {}
+    for path in ("auto", "generic"):
+        with pkg.Engine(t, s, dtype="f32", path=path) as e:
+            e.set_messages(mi)
+            e.sweep()
+            if path == "auto":
+                assert _rel(e.messages(), want_i) < 1e-5
+            e.run(maxiter=29)
+            res[path] = (e.messages(), e.lognorm())
+    assert _rel(res["auto"][0], res["generic"][0]) < 1e-5
+    assert res["auto"][1] == pytest.approx(res["generic"][1], rel=1e-6)
+
+
 def test_vertex12_mixed_dims_and_generic_agree(pkg, orc):
     """Bond dimensions 9..12 within one network: within the FP32 bound, and the MFMA kernel
     and the generic FP32 kernel agree after 20 sweeps."""
