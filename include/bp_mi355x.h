@@ -169,19 +169,38 @@ int bp_tfim_energy_device(bp_context *ctx, double J, double h, double *energy, v
                           double adj_tol, int adj_maxiter, int *out_adj_iters,
                           double *out_adj_res);
 
+/* BP expectation values at the current messages, with arbitrary real operators on the
+ * physical index (acting on the ket layer: (O.A)[s', X] = sum_s O[s', s] A[s, X]).
+ * Operators are ld x ld row-major blocks (ld >= every physical dimension; a vertex of
+ * physical dimension d uses the top-left d x d corner).
+ *   bp_expect_vertex: ops[v] for every vertex v; out[v] = Z_v[O_v] / Z_v, the operator-
+ *     inserted vertex scalar over the plain one (messagecache.jl:139-143 region scalar
+ *     with an insertion).
+ *   bp_expect_edge: per edge e (edge ids in the order of the topology's edges; the edge's
+ *     first vertex owns its lower message slot) ops_first[e] on the first vertex and
+ *     ops_second[e] on the second; out[e] = <g^A_{u->v}, g^B_{v->u}> / <g_{u->v}, g_{v->u}>
+ *     with g^O the unnormalised message of the sender with O inserted (the two-site region
+ *     {u, v} with its boundary messages, messagecache.jl:134-137 incoming_edge_data).
+ * Norm networks only.  The reference has no expectation API (its region scalars are the
+ * operator-free case); FP64 results, contractions in the context's arithmetic type. */
+int bp_expect_vertex(bp_context *ctx, const double *ops, int ld, double *out);
+int bp_expect_edge(bp_context *ctx, const double *ops_first, const double *ops_second, int ld,
+                   double *out);
+
 /* Kernel options (what the autotuner decides, settable by hand):
  *   "vertex_variant"   0: FP32 vertex sweep kernel with two LDS buffers per wave (default),
  *                      1: one buffer per wave and two waves per SIMD;
  *   "vertex16_waves"   waves per workgroup of the BF16 kernel for bond dimensions 9..16
  *                      (4, 8 or 16; default 16);
  *   "vertex_bond_mask" bit D set: networks whose largest bond dimension is D may run on the
- *                      vertex kernels (D 1..8; 9..16 in BF16 contexts; default 0x1FFFE).
+ *                      vertex kernels (D 1..12 in FP32, 1..16 in BF16; default 0x1FFFE).
  * No reference counterpart. */
 int bp_set_option(bp_context *ctx, const char *name, int value);
 
 /* Autotuner.  Times, on the context's own sites and messages, (a) for every largest bond
- * dimension D <= 8 present (FP32/BF16 norm networks, automatic path) whether its networks
- * sweep faster on the vertex kernels (bonds padded to 8) or on the generic kernel, and
+ * dimension D present that the vertex kernels take (<= 12 FP32, <= 16 BF16; norm networks,
+ * automatic path) whether its networks sweep faster on the vertex kernels (bonds padded to
+ * 8, 12 or 16) or on the generic kernel, and
  * (b) the FP32 vertex sweep kernel variant ("vertex_variant" of bp_set_option) and the
  * waves per workgroup of the bond-dimension-16 kernel ("vertex16_waves"), and
  * (c) for every bucket of generic-kernel slots (owner's physical dimension and largest

@@ -22,7 +22,8 @@ from .itensors import (Index, ITensor, ITensorNetwork, NormNetwork, linkinds,  #
                        normnetwork, random_itensor, siteinds, tensornetwork, to_topology,
                        uniquename)
 from .messagecache import (MessageCache, bethe_free_energy, edge_scalar,  # noqa: F401
-                           edge_scalars, incoming_edge_data, incoming_messages, iterate_diff,
+                           edge_scalars, expect, expect2, incoming_edge_data,
+                           incoming_messages, iterate_diff,
                            message_environment, messagecache, one, region_scalar,
                            similar_message_environment, vertex_scalar, vertex_scalars)
 
@@ -32,7 +33,7 @@ __all__ = [
     "NormNetwork", "RunResult", "SimpleMessageUpdate", "StopAfterIteration", "StopWhenAny",
     "StopWhenConverged", "Topology", "beliefpropagation", "bethe_free_energy",
     "build_topology", "concat_topologies", "default_algorithm",
-    "default_beliefpropagation_edges", "edge_scalar", "edge_scalars",
+    "default_beliefpropagation_edges", "edge_scalar", "edge_scalars", "expect", "expect2",
     "forest_cover_edge_sequence", "heavy_hex_lattice", "incoming_edge_data",
     "incoming_messages", "iterate_diff", "linkinds", "message_environment", "message_update_",
     "messagecache", "named_comb_tree", "named_grid", "named_path_graph", "normnetwork", "one",

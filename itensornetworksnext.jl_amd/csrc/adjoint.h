@@ -161,6 +161,90 @@ __global__ __launch_bounds__(BP_BLOCK) void k_adj_vscalar(DevTopo tp, const T *s
     }
 }
 
+// ---- expectation values with arbitrary operators (ld x ld row-major blocks, the top-left
+// d x d corner used): ket = O.A, out[s', X] = sum_s O[s', s] A[s, X], in b2
+template <typename T>
+__device__ const T *ket_tensor_m(const T *A, const VertexShape &vs, const double *opm, int ld,
+                                 T *b2) {
+    const int d = vs.dims[0];
+    const long long rest = vs.n / d;
+    for (long long i = threadIdx.x; i < vs.n; i += blockDim.x) {
+        const int sp = (int)(i / rest);
+        const long long x = i - sp * rest;
+        double acc = 0.0;
+        for (int t = 0; t < d; ++t) acc += opm[sp * ld + t] * (double)A[t * rest + x];
+        b2[i] = (T)acc;
+    }
+    __syncthreads();
+    return b2;
+}
+
+// zo[u] = open(O_u.A, A, m_in, -1): the operator-inserted vertex scalar
+template <typename T>
+__global__ __launch_bounds__(BP_BLOCK) void k_expect_vscalar(DevTopo tp, const T *sites,
+                                                              const T *msgs, const double *ops,
+                                                              int ld, double *zo, int mats_elems,
+                                                              int ov_elems, int red_elems,
+                                                              long long buf_elems, int use_lds,
+                                                              T *ws, long long ws_stride) {
+    extern __shared__ __align__(16) char smem[];
+    AdjSm<T> sm = adj_carve<T>(smem, mats_elems, ov_elems, red_elems, buf_elems, use_lds, ws,
+                               ws_stride);
+    for (int u = blockIdx.x; u < tp.nv; u += gridDim.x) {
+        VertexShape vs = vertex_shape(tp, u);
+        int moff[BP_MAXR];
+        load_incoming(tp, u, vs, msgs, sm.mats, moff, -1);
+        __syncthreads();
+        const T *A = sites + tp.site_off[u];
+        const T *ket = ket_tensor_m(A, vs, ops + (long long)u * ld * ld, ld, sm.b2);
+        const T *cur = absorb_ports(ket, vs, sm.mats, moff, -1, sm.ov, -1, 0, sm.b0, sm.b1);
+        double acc = 0.0;
+        for (long long i = threadIdx.x; i < vs.n; i += blockDim.x)
+            acc += (double)cur[i] * (double)A[i];
+        acc = block_sum(acc, sm.red);
+        if (threadIdx.x == 0) zo[u] = acc;
+        __syncthreads();
+    }
+}
+
+// G[q] = open(O.A, A, m_in, p) for every slot q (u -> v through port p of u), unnormalised,
+// with O = ops_first[e] when q is the lower slot of its edge e, else ops_second[e]
+template <typename T>
+__global__ __launch_bounds__(BP_BLOCK) void k_expect_open(DevTopo tp, const T *sites,
+                                                           const T *msgs, const double *ops_first,
+                                                           const double *ops_second, int ld,
+                                                           T *G, int mats_elems, int ov_elems,
+                                                           int red_elems, long long buf_elems,
+                                                           int use_lds, T *ws,
+                                                           long long ws_stride) {
+    extern __shared__ __align__(16) char smem[];
+    AdjSm<T> sm = adj_carve<T>(smem, mats_elems, ov_elems, red_elems, buf_elems, use_lds, ws,
+                               ws_stride);
+    for (int q = blockIdx.x; q < tp.nports; q += gridDim.x) {
+        const int u = tp.owner[q];
+        const int p = q - tp.port_off[u];
+        const int e = tp.port_edge[q];
+        const double *opm = (q < tp.port_rev[q] ? ops_first : ops_second) + (long long)e * ld * ld;
+        VertexShape vs = vertex_shape(tp, u);
+        int moff[BP_MAXR];
+        load_incoming(tp, u, vs, msgs, sm.mats, moff, p);
+        __syncthreads();
+        const T *A = sites + tp.site_off[u];
+        const T *ket = ket_tensor_m(A, vs, opm, ld, sm.b2);
+        const T *cur = absorb_ports(ket, vs, sm.mats, moff, -1, sm.ov, p, 0, sm.b0, sm.b1);
+        block_open_contract(cur, A, vs.dims, vs.r, 1 + p, sm.red, sm.out);
+        const long long mo = tp.msg_off[q];
+        const int len = (int)(tp.msg_off[q + 1] - mo);
+        for (int i = threadIdx.x; i < len; i += blockDim.x) G[mo + i] = sm.out[i];
+        __syncthreads();
+    }
+}
+
+__global__ void k_expect_ratio(const double *num, const double *den, int n, double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = num[i] / den[i];
+}
+
 // ---- cotangent through the sum normalisation f = g / S:  gg = (lam - <lam, f>) / S
 template <typename T>
 __global__ void k_adj_gg(DevTopo tp, const T *lam, const T *G, const double *S, T *gg) {
@@ -670,7 +754,6 @@ static int adj_tfim_t(AdjointScratch *s, hipStream_t st, const DevTopo &tp, int 
     return rc ? rc : (e == hipSuccess ? 0 : 2);
 }
 
-#undef ADJ_ARGS
 
 static int adjoint_grad_lognorm(AdjointScratch *s, hipStream_t st, DevTopo tp, long long max_site,
                                 int max_mats, int maxD, int dtype, const void *sites,
@@ -681,6 +764,90 @@ static int adjoint_grad_lognorm(AdjointScratch *s, hipStream_t st, DevTopo tp, l
     return adj_grad_lognorm_t<float>(s, st, tp, max_site, max_mats, maxD, (const float *)sites,
                                      (const float *)msgs, nsite, go);
 }
+
+// BP expectation values: <O_v> per vertex (ops_v != nullptr) or <A_e B_e> per edge
+// (ops_a, ops_b != nullptr), operators as host arrays of ld x ld blocks; results (FP64)
+// into out (host, nv or ne entries).  nmsg: entries of the message buffer.
+template <typename T>
+static int expect_t(AdjointScratch *s, hipStream_t st, const DevTopo &tp, int ne,
+                    long long nmsg, long long max_site, int max_mats, int maxD, const T *sites,
+                    const T *msgs, const double *ops_v, const double *ops_a,
+                    const double *ops_b, int ld, double *out) {
+    AdjLaunch L;
+    if (adj_launch_cfg(s, tp, max_site, max_mats, maxD, sizeof(T), &L)) return 5;
+    const size_t blk = sizeof(double) * (size_t)ld * ld;
+    int rc = 0;
+    if (ops_v) {
+        const int nv = tp.nv;
+        double *dops = (double *)s->alloc(blk * std::max(nv, 1));
+        double *z = (double *)s->alloc(sizeof(double) * std::max(nv, 1));
+        double *zo = (double *)s->alloc(sizeof(double) * std::max(nv, 1));
+        double *r = (double *)s->alloc(sizeof(double) * std::max(nv, 1));
+        if (!dops || !z || !zo || !r) { s->free_temps(); return 5; }
+        if (hipMemcpyAsync(dops, ops_v, blk * nv, hipMemcpyHostToDevice, st) != hipSuccess) rc = 2;
+        const int grid = std::min(L.grid, std::max(nv, 1));
+        if (!rc) {
+            hipLaunchKernelGGL(k_adj_vscalar<T>, dim3(grid), dim3(BP_BLOCK), L.lds, st, tp, sites,
+                               msgs, (int)OP_I, z, ADJ_ARGS(L, s));
+            hipLaunchKernelGGL(k_expect_vscalar<T>, dim3(grid), dim3(BP_BLOCK), L.lds, st, tp,
+                               sites, msgs, (const double *)dops, ld, zo, ADJ_ARGS(L, s));
+            hipLaunchKernelGGL(k_expect_ratio, dim3((nv + 255) / 256), dim3(256), 0, st,
+                               (const double *)zo, (const double *)z, nv, r);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(out, r, sizeof(double) * nv, hipMemcpyDeviceToHost, st) !=
+                    hipSuccess)
+                rc = 2;
+        }
+    } else {
+        const int np = tp.nports;
+        const long long msz = std::max(nmsg, 1LL);
+        double *da = (double *)s->alloc(blk * std::max(ne, 1));
+        double *db = (double *)s->alloc(blk * std::max(ne, 1));
+        T *G = (T *)s->alloc(sizeof(T) * msz);
+        T *GO = (T *)s->alloc(sizeof(T) * msz);
+        double *S = (double *)s->alloc(sizeof(double) * std::max(np, 1));
+        double *nz = (double *)s->alloc(sizeof(double) * std::max(ne, 1));
+        double *dd = (double *)s->alloc(sizeof(double) * std::max(ne, 1));
+        double *r = (double *)s->alloc(sizeof(double) * std::max(ne, 1));
+        if (!da || !db || !G || !GO || !S || !nz || !dd || !r) { s->free_temps(); return 5; }
+        if (hipMemcpyAsync(da, ops_a, blk * ne, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(db, ops_b, blk * ne, hipMemcpyHostToDevice, st) != hipSuccess)
+            rc = 2;
+        if (!rc && np > 0) {
+            const int grid = std::min(L.grid, np);
+            hipLaunchKernelGGL(k_adj_open<T>, dim3(grid), dim3(BP_BLOCK), L.lds, st, tp, sites,
+                               msgs, (int)OP_I, G, S, ADJ_ARGS(L, s));
+            hipLaunchKernelGGL(k_expect_open<T>, dim3(grid), dim3(BP_BLOCK), L.lds, st, tp,
+                               sites, msgs, (const double *)da, (const double *)db, ld, GO,
+                               ADJ_ARGS(L, s));
+            hipLaunchKernelGGL(k_adj_edge_dots<T>, dim3((np + 255) / 256), dim3(256), 0, st, tp,
+                               (const T *)G, (const T *)GO, nz, dd);
+            hipLaunchKernelGGL(k_expect_ratio, dim3((ne + 255) / 256), dim3(256), 0, st,
+                               (const double *)nz, (const double *)dd, ne, r);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(out, r, sizeof(double) * ne, hipMemcpyDeviceToHost, st) !=
+                    hipSuccess)
+                rc = 2;
+        }
+    }
+    const hipError_t e = hipStreamSynchronize(st);
+    s->free_temps();
+    return rc ? rc : (e == hipSuccess ? 0 : 2);
+}
+
+static int bp_expect(AdjointScratch *s, hipStream_t st, DevTopo tp, int ne, long long nmsg,
+                     long long max_site, int max_mats, int maxD, int dtype, const void *sites,
+                     const void *msgs, const double *ops_v, const double *ops_a,
+                     const double *ops_b, int ld, double *out) {
+    if (dtype == 0)
+        return expect_t<double>(s, st, tp, ne, nmsg, max_site, max_mats, maxD,
+                                (const double *)sites, (const double *)msgs, ops_v, ops_a, ops_b,
+                                ld, out);
+    return expect_t<float>(s, st, tp, ne, nmsg, max_site, max_mats, maxD, (const float *)sites,
+                           (const float *)msgs, ops_v, ops_a, ops_b, ld, out);
+}
+
+#undef ADJ_ARGS
 
 static int adjoint_tfim(AdjointScratch *s, hipStream_t st, DevTopo tp, int ne, long long max_site,
                         int max_mats, int maxD, int dtype, const void *sites, const void *msgs,

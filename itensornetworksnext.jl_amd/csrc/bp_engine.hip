@@ -1760,6 +1760,36 @@ static int tfim_impl(bp_context *c, double J, double h, double *energy, const Gr
     return rc == BP_OK ? BP_OK : set_err(rc, "TFIM energy/gradient failed (status %d)", rc);
 }
 
+// operators: ld x ld blocks, ld >= every physical dimension
+static int expect_impl(bp_context *c, const double *ops_v, const double *ops_a,
+                       const double *ops_b, int ld, double *out) {
+    RETCHK(ready(c));
+    if (!out || (!ops_v && (!ops_a || !ops_b))) return set_err(BP_ERR_ARG, "NULL argument");
+    if (c->plain) return set_err(BP_ERR_UNSUPPORTED, "expectation values need a norm network");
+    for (int u = 0; u < c->nv; ++u)
+        if (c->phys_dim[u] > ld)
+            return set_err(BP_ERR_ARG, "operator block %d smaller than physical dimension %d", ld,
+                           c->phys_dim[u]);
+    HIPCHK(hipSetDevice(c->device));
+    bool is64 = false;
+    const void *m = current_msgs(c, &is64);
+    const int rc = bp_expect(&c->adj, c->stream, c->topo(), c->ne, c->nmsg, c->max_site,
+                             c->max_mats, c->maxD, is64 ? BP_F64 : c->dtype,
+                             is64 ? (const void *)c->d_sites64 : (const void *)c->d_sites, m,
+                             ops_v, ops_a, ops_b, ld, out);
+    return rc == BP_OK ? BP_OK : set_err(rc, "expectation values failed (status %d)", rc);
+}
+
+int bp_expect_vertex(bp_context *c, const double *ops, int ld, double *out) {
+    if (!ops) return set_err(BP_ERR_ARG, "NULL argument");
+    return expect_impl(c, ops, nullptr, nullptr, ld, out);
+}
+
+int bp_expect_edge(bp_context *c, const double *ops_first, const double *ops_second, int ld,
+                   double *out) {
+    return expect_impl(c, nullptr, ops_first, ops_second, ld, out);
+}
+
 static GradOut device_grad(const bp_context *c, void *grad) {
     GradOut go;
     go.dev = grad;

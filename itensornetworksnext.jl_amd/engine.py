@@ -207,6 +207,29 @@ class Engine:
         return out
 
     # -- gradients
+    def expect_vertex(self, ops) -> np.ndarray:
+        """<O_v>_BP for every vertex at the current messages; ops: (nv, ld, ld) real
+        operators on the physical index (top-left d x d block used)."""
+        o = _f64(ops)
+        if o.ndim != 3 or o.shape[0] != self.topo.nv or o.shape[1] != o.shape[2]:
+            raise ValueError("ops must have shape (nv, ld, ld)")
+        out = np.zeros(self.topo.nv)
+        L.check(self.lib.bp_expect_vertex(self.h, _dp(o), int(o.shape[1]), _dp(out)),
+                "bp_expect_vertex")
+        return out
+
+    def expect_edge(self, ops_first, ops_second) -> np.ndarray:
+        """<A_e B_e>_BP for every topology edge e: ops_first[e] on the owner of the edge's
+        lower message slot, ops_second[e] on the other end; shapes (ne, ld, ld)."""
+        a, b = _f64(ops_first), _f64(ops_second)
+        ne = self.topo.ne
+        if a.shape != b.shape or a.ndim != 3 or a.shape[0] != ne or a.shape[1] != a.shape[2]:
+            raise ValueError("ops must have shape (ne, ld, ld)")
+        out = np.zeros(ne)
+        L.check(self.lib.bp_expect_edge(self.h, _dp(a), _dp(b), int(a.shape[1]), _dp(out)),
+                "bp_expect_edge")
+        return out
+
     def set_option(self, name: str, value: int) -> None:
         """Kernel options (bp_set_option): "vertex_variant" (0 or 1) and
         "vertex_bond_mask" (bit D lets networks of largest bond dimension D use the vertex

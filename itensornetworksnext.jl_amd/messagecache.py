@@ -230,6 +230,76 @@ def bethe_free_energy(factors, messages):
     return v
 
 
+def expect(factors, messages, ops) -> dict:
+    """BP expectation values <O_v>_BP = Z_v[O_v] / Z_v of one-site operators.
+
+    `ops` maps vertices to real d x d matrices acting on the physical index of the ket
+    layer; returns {vertex: value} for the vertices in `ops`.  The region scalar of
+    messagecache.jl:139-143 with an operator inserted (the reference itself has no
+    expectation API); contractions on the device in FP64."""
+    fl, e = _engine(factors, messages)
+    try:
+        t = fl.topo
+        ld = max(int(np.max(t.phys_dim)), max((np.shape(o)[0] for o in ops.values()), default=1))
+        blk = np.zeros((t.nv, ld, ld))
+        for u in range(t.nv):
+            blk[u] = np.eye(ld)
+        index = {v: i for i, v in enumerate(t.vertices)}
+        for v, o in ops.items():
+            o = np.asarray(o, dtype=np.float64)
+            d = int(t.phys_dim[index[v]])
+            if o.shape != (d, d):
+                raise ValueError(f"operator on {v!r} must be {d} x {d}")
+            blk[index[v]] = 0.0
+            blk[index[v], :d, :d] = o
+        vals = e.expect_vertex(blk)
+    finally:
+        e.close()
+    return {v: float(vals[index[v]]) for v in ops}
+
+
+def expect2(factors, messages, ops) -> dict:
+    """BP expectation values <O_u O_v>_BP of nearest-neighbour two-site operators: the
+    two-site region {u, v} with its boundary messages (messagecache.jl:134-137),
+    <g^{O_u}_{u->v}, g^{O_v}_{v->u}> / <g_{u->v}, g_{v->u}> with g^O the unnormalised
+    message of the sender with O inserted.  `ops` maps edges (u, v) to pairs (O_u, O_v);
+    returns {edge: value}.  FP64 on the device."""
+    fl, e = _engine(factors, messages)
+    try:
+        t = fl.topo
+        ld = int(np.max(t.phys_dim))
+        a = np.broadcast_to(np.eye(ld), (t.ne, ld, ld)).copy()
+        b = a.copy()
+        index = {v: i for i, v in enumerate(t.vertices)}
+        own = t.port_owner()
+        first = {}
+        for q in range(t.nports):
+            if q < int(t.port_rev[q]):
+                first[int(t.port_edge[q])] = int(own[q])
+        eid = {}
+        for q in range(t.nports):
+            eid[(int(own[q]), int(t.port_nbr[q]))] = int(t.port_edge[q])
+        keys = []
+        for (u, v), (ou, ov) in ops.items():
+            iu, iv = index[u], index[v]
+            if (iu, iv) not in eid:
+                raise ValueError(f"({u!r}, {v!r}) is not an edge")
+            k = eid[(iu, iv)]
+            ou, ov = np.asarray(ou, dtype=np.float64), np.asarray(ov, dtype=np.float64)
+            if first[k] != iu:
+                ou, ov = ov, ou
+                iu, iv = iv, iu
+            a[k] = 0.0
+            b[k] = 0.0
+            a[k, :ou.shape[0], :ou.shape[1]] = ou
+            b[k, :ov.shape[0], :ov.shape[1]] = ov
+            keys.append(((u, v), k))
+        vals = e.expect_edge(a, b)
+    finally:
+        e.close()
+    return {uv: float(vals[k]) for uv, k in keys}
+
+
 def iterate_diff(cache1: MessageCache, cache2: MessageCache) -> float:
     """max over directed edges of 1 - |<m1^, m2^>|^2 (beliefpropagation.jl:261-267)."""
     topo, e = _cache_engine(cache1)

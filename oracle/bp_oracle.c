@@ -836,6 +836,78 @@ int orc_tfim_energy(const orc_graph *g, const double *sites, const double *msgs,
     return 0;
 }
 
+/* top-left d x d block of an ld x ld operator */
+static void op_block(const double *op, int ld, int d, double *out) {
+    for (int a = 0; a < d; ++a)
+        for (int b = 0; b < d; ++b) out[a * d + b] = op[a * ld + b];
+}
+
+int orc_expect_vertex(const orc_graph *g, const double *sites, const double *msgs,
+                      const double *ops, int ld, double *out) {
+    long long wn = max_site_size(g);
+    double *w0 = (double *)malloc(sizeof(double) * wn);
+    double *w1 = (double *)malloc(sizeof(double) * wn);
+    double *opA = (double *)malloc(sizeof(double) * wn);
+    double *ob = (double *)malloc(sizeof(double) * ld * ld);
+    if (!w0 || !w1 || !opA || !ob) return 1;
+    for (int u = 0; u < g->nv; ++u) {
+        int base = g->port_off[u], k = vdeg(g, u), d = g->phys_dim[u];
+        if (d > ld) return 5;
+        const double *A = sites + g->site_off[u];
+        const double *mats[MAXR];
+        for (int j = 0; j < k; ++j) mats[j] = msgs + g->msg_off[g->port_rev[base + j]];
+        op_block(ops + (long long)u * ld * ld, ld, d, ob);
+        apply_phys(ob, A, d, site_size(g, u) / d, opA);
+        double Z, Zo;
+        vertex_open_w(g, u, A, A, mats, -1, &Z, w0, w1);
+        vertex_open_w(g, u, opA, A, mats, -1, &Zo, w0, w1);
+        out[u] = Zo / Z;
+    }
+    free(w0); free(w1); free(opA); free(ob);
+    return 0;
+}
+
+int orc_expect_edge(const orc_graph *g, const double *sites, const double *msgs,
+                    const double *ops_first, const double *ops_second, int ld, double *out) {
+    long long wn = max_site_size(g);
+    int nports = g->port_off[g->nv];
+    double *w0 = (double *)malloc(sizeof(double) * wn);
+    double *w1 = (double *)malloc(sizeof(double) * wn);
+    double *opA = (double *)malloc(sizeof(double) * wn);
+    double *opB = (double *)malloc(sizeof(double) * wn);
+    double *ob = (double *)malloc(sizeof(double) * ld * ld);
+    long long ml = 1;
+    for (int q = 0; q < nports; ++q) ml = msg_len(g, q) > ml ? msg_len(g, q) : ml;
+    double *gu = (double *)malloc(sizeof(double) * ml), *gv = (double *)malloc(sizeof(double) * ml);
+    double *hu = (double *)malloc(sizeof(double) * ml), *hv = (double *)malloc(sizeof(double) * ml);
+    if (!w0 || !w1 || !opA || !opB || !ob || !gu || !gv || !hu || !hv) return 1;
+    for (int q = 0; q < nports; ++q) {
+        int r = g->port_rev[q];
+        if (q > r) continue;
+        int e = g->port_edge[q];
+        int u = port_owner(g, q), v = port_owner(g, r);
+        int bu = g->port_off[u], ku = vdeg(g, u), bv = g->port_off[v], kv = vdeg(g, v);
+        int du = g->phys_dim[u], dv = g->phys_dim[v];
+        if (du > ld || dv > ld) return 5;
+        const double *Au = sites + g->site_off[u], *Av = sites + g->site_off[v];
+        const double *mu[MAXR], *mv[MAXR];
+        for (int j = 0; j < ku; ++j) mu[j] = msgs + g->msg_off[g->port_rev[bu + j]];
+        for (int j = 0; j < kv; ++j) mv[j] = msgs + g->msg_off[g->port_rev[bv + j]];
+        op_block(ops_first + (long long)e * ld * ld, ld, du, ob);
+        apply_phys(ob, Au, du, site_size(g, u) / du, opA);
+        op_block(ops_second + (long long)e * ld * ld, ld, dv, ob);
+        apply_phys(ob, Av, dv, site_size(g, v) / dv, opB);
+        vertex_open_w(g, u, Au, Au, mu, q - bu, gu, w0, w1);
+        vertex_open_w(g, v, Av, Av, mv, r - bv, gv, w0, w1);
+        vertex_open_w(g, u, opA, Au, mu, q - bu, hu, w0, w1);
+        vertex_open_w(g, v, opB, Av, mv, r - bv, hv, w0, w1);
+        long long n = msg_len(g, q);
+        out[e] = dot(hu, hv, n) / dot(gu, gv, n);
+    }
+    free(w0); free(w1); free(opA); free(opB); free(ob); free(gu); free(gv); free(hu); free(hv);
+    return 0;
+}
+
 int orc_fixed_point_gradient(const orc_graph *g, const double *sites, const double *msgs,
                              const double *dL_dm, const double *dL_dA, int maxiter, double tol,
                              double *grad, int *out_iters, double *out_res) {

@@ -138,6 +138,21 @@ int orc_grad_lognorm_explicit(const orc_graph *g, const double *sites, const dou
 int orc_tfim_energy(const orc_graph *g, const double *sites, const double *msgs, double J,
                     double h, double *E, double *dE_dm, double *dE_dA);
 
+/* ---- BP expectation values with arbitrary physical operators (the operator acts on the
+ * ket layer: (O.A)[s', X] = sum_s O[s', s] A[s, X]).  Operators are ld x ld row-major
+ * blocks; a vertex of physical dimension d uses the top-left d x d corner.
+ *   vertex v:  out[v] = Z_v[O_v] / Z_v  (incoming messages on every port)
+ *   edge e = {u, v} (u owns the lower port of e):
+ *     out[e] = <g^{A_e}_{u->v}, g^{B_e}_{v->u}> / <g_{u->v}, g_{v->u}>
+ *   with g^{O}_{u->v} the unnormalised message of u with O inserted, ops_first[e] = A_e on
+ *   u, ops_second[e] = B_e on v.  Real arithmetic; no reference counterpart (the
+ *   reference has no expectation API): the ratios restate its region scalars
+ *   (messagecache.jl:139-151) with an operator inserted. */
+int orc_expect_vertex(const orc_graph *g, const double *sites, const double *msgs,
+                      const double *ops, int ld, double *out);
+int orc_expect_edge(const orc_graph *g, const double *sites, const double *msgs,
+                    const double *ops_first, const double *ops_second, int ld, double *out);
+
 /* ---- full implicit-differentiation gradient of an objective L(m*, A) at the fixed
  * point m*: solves lambda = J_m^T lambda + dL/dm (projected onto the sum-normalised
  * tangent space) by adjoint sweeps to tolerance tol on the relative change

@@ -70,12 +70,15 @@ def _declare(L):
     L.orc_norm_vjp.argtypes = [G, _dp, _dp, _dp, _dp, _dp]
     L.orc_grad_lognorm_explicit.argtypes = [G, _dp, _dp, _dp]
     L.orc_tfim_energy.argtypes = [G, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp]
+    L.orc_expect_vertex.argtypes = [G, _dp, _dp, _dp, C.c_int, _dp]
+    L.orc_expect_edge.argtypes = [G, _dp, _dp, _dp, _dp, C.c_int, _dp]
     L.orc_fixed_point_gradient.argtypes = [G, _dp, _dp, _dp, _dp, C.c_int, C.c_double, _dp,
                                            _ip, _dp]
     for name in ("orc_norm_message", "orc_plain_message", "orc_sweep_jacobi",
                  "orc_sweep_sequential", "orc_bp", "orc_vertex_scalars", "orc_edge_scalars",
                  "orc_bethe_free_energy", "orc_contract_labeled", "orc_norm_vjp",
-                 "orc_grad_lognorm_explicit", "orc_tfim_energy", "orc_fixed_point_gradient"):
+                 "orc_grad_lognorm_explicit", "orc_tfim_energy", "orc_fixed_point_gradient",
+              "orc_expect_vertex", "orc_expect_edge"):
         getattr(L, name).restype = C.c_int
 
 
@@ -292,6 +295,27 @@ def tfim_energy(topo, sites, msgs, J=1.0, h=3.0, partials=False):
     _check(lib().orc_tfim_energy(g.ref, _p(_f64(sites)), _p(_f64(msgs)), float(J), float(h),
                                  C.byref(E), _p(dm), _p(da)), "tfim_energy")
     return (E.value, dm, da) if partials else E.value
+
+
+def expect_vertex(topo, sites, msgs, ops):
+    """<O_v>_BP for every vertex; ops: (nv, ld, ld) operators on the physical index."""
+    g = OracleGraph(topo)
+    ops = _f64(ops)
+    out = np.zeros(topo.nv)
+    _check(lib().orc_expect_vertex(g.ref, _p(_f64(sites)), _p(_f64(msgs)), _p(ops),
+                                   int(ops.shape[-1]), _p(out)), "expect_vertex")
+    return out
+
+
+def expect_edge(topo, sites, msgs, ops_first, ops_second):
+    """<A_e B_e>_BP for every edge (topology edge id); ops_first[e] acts on the owner of
+    the edge's lower port, ops_second[e] on the other end."""
+    g = OracleGraph(topo)
+    a, b = _f64(ops_first), _f64(ops_second)
+    out = np.zeros(int(topo.ne))
+    _check(lib().orc_expect_edge(g.ref, _p(_f64(sites)), _p(_f64(msgs)), _p(a), _p(b),
+                                 int(a.shape[-1]), _p(out)), "expect_edge")
+    return out
 
 
 def fixed_point_gradient(topo, sites, msgs, dL_dm, dL_dA, maxiter=1000, tol=1e-12):

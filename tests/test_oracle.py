@@ -281,3 +281,85 @@ def test_tfim_product_state_closed_form(pkg, orc):
     for (x, y) in t.edges:
         want -= J * ez[t.vertices.index(x)] * ez[t.vertices.index(y)]
     assert orc.tfim_energy(t, s, m, J, h) == pytest.approx(want, rel=1e-13)
+
+
+# ---------------------------------------------------------------- expectation values
+
+def _state_vector(t, s):
+    """psi[s_0, ..., s_{nv-1}] of a small network by direct contraction of its sites."""
+    args = []
+    for u in range(t.nv):
+        k = t.degree(u)
+        ports = [int(t.port_off[u]) + j for j in range(k)]
+        dims = [int(t.phys_dim[u])] + [int(t.edge_dim[t.port_edge[q]]) for q in ports]
+        args += [s[t.site_off[u]:t.site_off[u + 1]].reshape(dims),
+                 [u] + [t.nv + int(t.port_edge[q]) for q in ports]]
+    return np.einsum(*args, list(range(t.nv)))
+
+
+def _apply(op, psi, axis):
+    return np.moveaxis(np.tensordot(op, psi, axes=([1], [axis])), 0, axis)
+
+
+def _edge_ends(t):
+    """(first, second) vertex of every topology edge: first owns the edge's lower port."""
+    own = t.port_owner()
+    ends = {}
+    for q in range(t.nports):
+        r = int(t.port_rev[q])
+        if q < r:
+            ends[int(t.port_edge[q])] = (int(own[q]), int(own[r]))
+    return [ends[e] for e in range(t.ne)]
+
+
+@pytest.mark.parametrize("kind", ["path", "comb"])
+def test_expect_tree_exact(pkg, orc, kind):
+    """On a tree BP is exact: <O_v>_BP and <A_e B_e>_BP (adjacent sites) equal the exact
+    <psi|O|psi>/<psi|psi> for random, non-symmetric operators."""
+    g = pkg.named_path_graph(5) if kind == "path" else pkg.named_comb_tree((3, 2))
+    t = pkg.build_topology(g, 2, 2)
+    s = pkg.peps.random_sites(t, 31)
+    m, _, _, at = orc.bp(t, s, pkg.peps.identity_messages(t), maxiter=200, tol=1e-15,
+                         criterion="maxabs")
+    assert at > 0
+    rng = np.random.default_rng(5)
+    psi = _state_vector(t, s)
+    nn = float(np.sum(psi * psi))
+    ops = rng.standard_normal((t.nv, 2, 2))
+    got = orc.expect_vertex(t, s, m, ops)
+    for u in range(t.nv):
+        want = float(np.sum(psi * _apply(ops[u], psi, u))) / nn
+        assert got[u] == pytest.approx(want, rel=1e-11, abs=1e-13)
+    a = rng.standard_normal((t.ne, 2, 2))
+    b = rng.standard_normal((t.ne, 2, 2))
+    got_e = orc.expect_edge(t, s, m, a, b)
+    for e, (u, v) in enumerate(_edge_ends(t)):
+        want = float(np.sum(psi * _apply(a[e], _apply(b[e], psi, v), u))) / nn
+        assert got_e[e] == pytest.approx(want, rel=1e-11, abs=1e-13)
+
+
+def test_expect_identity_and_linearity(pkg, orc):
+    """Identity operators give exactly 1; the estimates are linear in the operator."""
+    t, s, m = _converged(pkg, orc, 4, 3, 32)
+    one = np.broadcast_to(np.eye(2), (t.nv, 2, 2))
+    assert np.allclose(orc.expect_vertex(t, s, m, one), 1.0, rtol=0, atol=1e-13)
+    onee = np.broadcast_to(np.eye(2), (t.ne, 2, 2))
+    assert np.allclose(orc.expect_edge(t, s, m, onee, onee), 1.0, rtol=0, atol=1e-13)
+    rng = np.random.default_rng(6)
+    o1, o2 = rng.standard_normal((2, t.nv, 2, 2))
+    lhs = orc.expect_vertex(t, s, m, 2.0 * o1 - 3.0 * o2)
+    rhs = 2.0 * orc.expect_vertex(t, s, m, o1) - 3.0 * orc.expect_vertex(t, s, m, o2)
+    assert np.allclose(lhs, rhs, rtol=1e-12, atol=1e-12)
+
+
+def test_expect_reproduces_tfim_energy(pkg, orc):
+    """sum_e -J <Z Z>_BP + sum_v -h <X>_BP equals the (separately pinned) TFIM energy."""
+    t, s, m = _converged(pkg, orc, 4, 3, 33)
+    X = np.array([[0.0, 1.0], [1.0, 0.0]])
+    Z = np.array([[1.0, 0.0], [0.0, -1.0]])
+    J, h = 1.0, 3.0
+    ev = orc.expect_vertex(t, s, m, np.broadcast_to(X, (t.nv, 2, 2)))
+    ee = orc.expect_edge(t, s, m, np.broadcast_to(Z, (t.ne, 2, 2)),
+                         np.broadcast_to(Z, (t.ne, 2, 2)))
+    assert -h * ev.sum() - J * ee.sum() == pytest.approx(orc.tfim_energy(t, s, m, J, h),
+                                                          rel=1e-13)
