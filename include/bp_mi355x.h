@@ -187,6 +187,22 @@ int bp_expect_vertex(bp_context *ctx, const double *ops, int ld, double *out);
 int bp_expect_edge(bp_context *ctx, const double *ops_first, const double *ops_second, int ld,
                    double *out);
 
+/* Local energy E = sum_v <O_v>_BP + sum_e <A_e B_e>_BP (the expectation values of
+ * bp_expect_vertex / bp_expect_edge, same operator layout; ops_v, or the ops_first /
+ * ops_second pair, may be NULL to drop that sum) at the current messages, and optionally
+ * its gradient with respect to every site tensor by fixed-point adjoint sweeps (grad may be
+ * NULL; FP64 host array in the site layout, or, for the _device variant, a device buffer
+ * as bp_tfim_energy_device takes).  The TFIM energy is the case O_v = -h X,
+ * A_e = -J Z, B_e = Z.  Generic (VALU) kernels in the context's arithmetic type.  No
+ * reference counterpart (the reference has no AD or expectation API). */
+int bp_local_energy(bp_context *ctx, const double *ops_v, const double *ops_first,
+                    const double *ops_second, int ld, double *energy, double *grad,
+                    double adj_tol, int adj_maxiter, int *out_adj_iters, double *out_adj_res);
+int bp_local_energy_device(bp_context *ctx, const double *ops_v, const double *ops_first,
+                           const double *ops_second, int ld, double *energy, void *grad,
+                           double adj_tol, int adj_maxiter, int *out_adj_iters,
+                           double *out_adj_res);
+
 /* Kernel options (what the autotuner decides, settable by hand):
  *   "vertex_variant"   0: FP32 vertex sweep kernel with two LDS buffers per wave (default),
  *                      1: one buffer per wave and two waves per SIMD;

@@ -56,6 +56,10 @@ SIGNATURES = {
                                         C.c_int, _ip, _dp]),
     "bp_expect_vertex": (C.c_int, [_vp, _dp, C.c_int, _dp]),
     "bp_expect_edge": (C.c_int, [_vp, _dp, _dp, C.c_int, _dp]),
+    "bp_local_energy": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int, _dp, _dp, C.c_double, C.c_int,
+                                  _ip, _dp]),
+    "bp_local_energy_device": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int, _dp, _vp, C.c_double,
+                                         C.c_int, _ip, _dp]),
     "bp_autotune": (C.c_int, [_vp, C.c_char_p, C.c_int, _ip]),
     "bp_set_option": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "bp_last_run_ms": (C.c_int, [_vp, _dp]),

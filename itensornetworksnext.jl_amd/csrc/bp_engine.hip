@@ -1790,11 +1790,52 @@ int bp_expect_edge(bp_context *c, const double *ops_first, const double *ops_sec
     return expect_impl(c, nullptr, ops_first, ops_second, ld, out);
 }
 
+static int local_impl(bp_context *c, const double *ops_v, const double *ops_first,
+                      const double *ops_second, int ld, double *energy, const GradOut &go,
+                      double adj_tol, int adj_maxiter, int *out_iters, double *out_res) {
+    RETCHK(ready(c));
+    if (!energy || (!ops_v && (!ops_first || !ops_second)))
+        return set_err(BP_ERR_ARG, "NULL argument");
+    if ((ops_first == nullptr) != (ops_second == nullptr))
+        return set_err(BP_ERR_ARG, "edge operators come in (first, second) pairs");
+    if (c->plain) return set_err(BP_ERR_UNSUPPORTED, "energies need a norm network");
+    for (int u = 0; u < c->nv; ++u)
+        if (c->phys_dim[u] > ld)
+            return set_err(BP_ERR_ARG, "operator block %d smaller than physical dimension %d", ld,
+                           c->phys_dim[u]);
+    HIPCHK(hipSetDevice(c->device));
+    bool is64 = false;
+    const void *m = current_msgs(c, &is64);
+    const int rc = adjoint_local(&c->adj, c->stream, c->topo(), c->ne, c->max_site, c->max_mats,
+                                 c->maxD, is64 ? BP_F64 : c->dtype,
+                                 is64 ? (const void *)c->d_sites64 : (const void *)c->d_sites, m,
+                                 c->nsite, c->nmsg, ops_v, ops_first, ops_second, ld, energy, go,
+                                 adj_tol, adj_maxiter, out_iters, out_res);
+    return rc == BP_OK ? BP_OK : set_err(rc, "local energy/gradient failed (status %d)", rc);
+}
+
 static GradOut device_grad(const bp_context *c, void *grad) {
     GradOut go;
     go.dev = grad;
     go.dev_f64 = c->dtype == BP_F64;
     return go;
+}
+
+int bp_local_energy(bp_context *c, const double *ops_v, const double *ops_first,
+                    const double *ops_second, int ld, double *energy, double *grad,
+                    double adj_tol, int adj_maxiter, int *out_iters, double *out_res) {
+    GradOut go;
+    go.host = grad;
+    return local_impl(c, ops_v, ops_first, ops_second, ld, energy, go, adj_tol, adj_maxiter,
+                      out_iters, out_res);
+}
+
+int bp_local_energy_device(bp_context *c, const double *ops_v, const double *ops_first,
+                           const double *ops_second, int ld, double *energy, void *grad,
+                           double adj_tol, int adj_maxiter, int *out_iters, double *out_res) {
+    if (!c || !grad) return set_err(BP_ERR_ARG, "NULL argument");
+    return local_impl(c, ops_v, ops_first, ops_second, ld, energy, device_grad(c, grad), adj_tol,
+                      adj_maxiter, out_iters, out_res);
 }
 
 int bp_grad_lognorm(bp_context *c, double *grad) {

@@ -265,6 +265,43 @@ class Engine:
         L.check(self.lib.bp_grad_lognorm(self.h, _dp(out)), "bp_grad_lognorm")
         return out
 
+    def local_energy(self, ops_v=None, ops_first=None, ops_second=None, grad: bool = False,
+                     adj_tol: float = 1e-8, adj_maxiter: int = 1000, device: bool = False,
+                     out=None):
+        """E = sum_v <O_v>_BP + sum_e <A_e B_e>_BP with operator arrays as expect_vertex /
+        expect_edge take (either sum may be omitted); with grad=True also (gradient,
+        adjoint sweeps, adjoint residual), on the device with device=True."""
+        if ops_v is None and (ops_first is None or ops_second is None):
+            raise ValueError("give vertex operators, edge operator pairs, or both")
+        ov = _f64(ops_v) if ops_v is not None else None
+        oa = _f64(ops_first) if ops_first is not None else None
+        ob = _f64(ops_second) if ops_second is not None else None
+        ref = ov if ov is not None else oa
+        ld = int(ref.shape[-1])
+        if ov is not None and ov.shape != (self.topo.nv, ld, ld):
+            raise ValueError("ops_v must have shape (nv, ld, ld)")
+        if oa is not None and (oa.shape != (self.topo.ne, ld, ld) or ob is None or
+                               ob.shape != oa.shape):
+            raise ValueError("edge operators must both have shape (ne, ld, ld)")
+        E = C.c_double(0)
+        it = C.c_int(0)
+        res = C.c_double(0)
+        if grad and device:
+            g = out if out is not None else self._device_grad_buffer()
+            L.check(self.lib.bp_local_energy_device(self.h, _dp(ov), _dp(oa), _dp(ob), ld,
+                                                    C.byref(E), C.c_void_p(g.ptr),
+                                                    float(adj_tol), int(adj_maxiter),
+                                                    C.byref(it), C.byref(res)),
+                    "bp_local_energy_device")
+            return E.value, g, it.value, res.value
+        g = np.zeros(int(self.topo.site_off[-1])) if grad else None
+        L.check(self.lib.bp_local_energy(self.h, _dp(ov), _dp(oa), _dp(ob), ld, C.byref(E),
+                                         _dp(g), float(adj_tol), int(adj_maxiter),
+                                         C.byref(it), C.byref(res)), "bp_local_energy")
+        if grad:
+            return E.value, g, it.value, res.value
+        return E.value
+
     def tfim_energy(self, J: float = 1.0, h: float = 3.0, grad: bool = False,
                     adj_tol: float = 1e-8, adj_maxiter: int = 1000, device: bool = False,
                     out=None):

@@ -908,6 +908,130 @@ int orc_expect_edge(const orc_graph *g, const double *sites, const double *msgs,
     return 0;
 }
 
+int orc_local_energy(const orc_graph *g, const double *sites, const double *msgs,
+                     const double *ops_v, const double *ops_first, const double *ops_second,
+                     int ld, double *E, double *dE_dm, double *dE_dA) {
+    long long wn = max_site_size(g);
+    long long ms = max_mat_store(g);
+    int nports = g->port_off[g->nv];
+    long long ml = 1;
+    for (int q = 0; q < nports; ++q) ml = msg_len(g, q) > ml ? msg_len(g, q) : ml;
+    double *w0 = (double *)malloc(sizeof(double) * wn);
+    double *w1 = (double *)malloc(sizeof(double) * wn);
+    double *w2 = (double *)malloc(sizeof(double) * wn);
+    double *opA = (double *)malloc(sizeof(double) * wn);
+    double *opB = (double *)malloc(sizeof(double) * wn);
+    double *mt = (double *)malloc(sizeof(double) * ms);
+    double *oa = (double *)malloc(sizeof(double) * ld * ld);
+    double *ob = (double *)malloc(sizeof(double) * ld * ld);
+    double *gu = (double *)malloc(sizeof(double) * ml), *gv = (double *)malloc(sizeof(double) * ml);
+    double *hu = (double *)malloc(sizeof(double) * ml), *hv = (double *)malloc(sizeof(double) * ml);
+    double *tmp = (double *)malloc(sizeof(double) * ml);
+    if (!w0 || !w1 || !w2 || !opA || !opB || !mt || !oa || !ob || !gu || !gv || !hu || !hv || !tmp)
+        return 1;
+    if (dE_dm) memset(dE_dm, 0, sizeof(double) * g->msg_off[nports]);
+    if (dE_dA) memset(dE_dA, 0, sizeof(double) * g->site_off[g->nv]);
+    ksum acc = {0, 0};
+    /* vertex terms <O_v> = Zo / Z */
+    for (int u = 0; ops_v && u < g->nv; ++u) {
+        int base = g->port_off[u], k = vdeg(g, u), d = g->phys_dim[u];
+        if (d > ld) return 5;
+        const double *A = sites + g->site_off[u];
+        const double *mats[MAXR];
+        for (int j = 0; j < k; ++j) mats[j] = msgs + g->msg_off[g->port_rev[base + j]];
+        op_block(ops_v + (long long)u * ld * ld, ld, d, oa);
+        apply_phys(oa, A, d, site_size(g, u) / d, opA);
+        double Z, Zo;
+        vertex_open_w(g, u, A, A, mats, -1, &Z, w0, w1);
+        vertex_open_w(g, u, opA, A, mats, -1, &Zo, w0, w1);
+        double ex = Zo / Z, c = 1.0 / Z;
+        kadd(&acc, ex);
+        if (dE_dm) {
+            for (int j = 0; j < k; ++j) {
+                int qin = g->port_rev[base + j];
+                long long n = msg_len(g, qin);
+                double *dm = dE_dm + g->msg_off[qin];
+                vertex_open_w(g, u, opA, A, mats, j, tmp, w0, w1);
+                for (long long i = 0; i < n; ++i) dm[i] += c * tmp[i];
+                vertex_open_w(g, u, A, A, mats, j, tmp, w0, w1);
+                for (long long i = 0; i < n; ++i) dm[i] -= c * ex * tmp[i];
+            }
+        }
+        if (dE_dA) {
+            double *da = dE_dA + g->site_off[u];
+            accum_site_grad(g, u, opA, A, mats, c, 0.0, oa, da, w0, w2, mt);
+            accum_site_grad(g, u, opA, A, mats, 0.0, c, NULL, da, w0, w2, mt);
+            accum_site_grad(g, u, A, A, mats, -c * ex, -c * ex, NULL, da, w0, w2, mt);
+        }
+    }
+    /* edge terms <A_e B_e> = <h_uv, h_vu> / <g_uv, g_vu> */
+    for (int q = 0; ops_first && ops_second && q < nports; ++q) {
+        int r = g->port_rev[q];
+        if (q > r) continue;
+        int e = g->port_edge[q];
+        int u = port_owner(g, q), v = port_owner(g, r);
+        int bu = g->port_off[u], ku = vdeg(g, u), pu = q - bu;
+        int bv = g->port_off[v], kv = vdeg(g, v), pv = r - bv;
+        int du = g->phys_dim[u], dv = g->phys_dim[v];
+        if (du > ld || dv > ld) return 5;
+        const double *Au = sites + g->site_off[u], *Av = sites + g->site_off[v];
+        const double *mu[MAXR], *mv[MAXR];
+        for (int j = 0; j < ku; ++j) mu[j] = msgs + g->msg_off[g->port_rev[bu + j]];
+        for (int j = 0; j < kv; ++j) mv[j] = msgs + g->msg_off[g->port_rev[bv + j]];
+        op_block(ops_first + (long long)e * ld * ld, ld, du, oa);
+        op_block(ops_second + (long long)e * ld * ld, ld, dv, ob);
+        apply_phys(oa, Au, du, site_size(g, u) / du, opA);
+        apply_phys(ob, Av, dv, site_size(g, v) / dv, opB);
+        vertex_open_w(g, u, Au, Au, mu, pu, gu, w0, w1);
+        vertex_open_w(g, v, Av, Av, mv, pv, gv, w0, w1);
+        vertex_open_w(g, u, opA, Au, mu, pu, hu, w0, w1);
+        vertex_open_w(g, v, opB, Av, mv, pv, hv, w0, w1);
+        long long n = msg_len(g, q);
+        double Dn = dot(gu, gv, n), Nn = dot(hu, hv, n);
+        double ee = Nn / Dn, c = 1.0 / Dn;
+        kadd(&acc, ee);
+        if (!dE_dm && !dE_dA) continue;
+        const double *mw2[MAXR];
+        for (int side = 0; side < 2; ++side) {
+            int w = side == 0 ? u : v;
+            int bw = side == 0 ? bu : bv, kw = side == 0 ? ku : kv, pw = side == 0 ? pu : pv;
+            const double *Aw = side == 0 ? Au : Av;
+            const double *opw = side == 0 ? opA : opB;
+            const double *om = side == 0 ? oa : ob;
+            const double *const *mw = side == 0 ? mu : mv;
+            const double *gN = side == 0 ? hv : hu; /* other side's inserted env */
+            const double *gD = side == 0 ? gv : gu;
+            for (int j = 0; j < kw; ++j) mw2[j] = mw[j];
+            if (dE_dm) {
+                for (int j = 0; j < kw; ++j) {
+                    if (j == pw) continue;
+                    int qin = g->port_rev[bw + j];
+                    long long nj = msg_len(g, qin);
+                    double *dm = dE_dm + g->msg_off[qin];
+                    mw2[pw] = gN;
+                    vertex_open_w(g, w, opw, Aw, mw2, j, tmp, w0, w1);
+                    for (long long i = 0; i < nj; ++i) dm[i] += c * tmp[i];
+                    mw2[pw] = gD;
+                    vertex_open_w(g, w, Aw, Aw, mw2, j, tmp, w0, w1);
+                    for (long long i = 0; i < nj; ++i) dm[i] -= c * ee * tmp[i];
+                }
+            }
+            if (dE_dA) {
+                double *da = dE_dA + g->site_off[w];
+                mw2[pw] = gN;
+                accum_site_grad(g, w, opw, Aw, mw2, c, 0.0, om, da, w0, w2, mt);
+                accum_site_grad(g, w, opw, Aw, mw2, 0.0, c, NULL, da, w0, w2, mt);
+                mw2[pw] = gD;
+                accum_site_grad(g, w, Aw, Aw, mw2, -c * ee, -c * ee, NULL, da, w0, w2, mt);
+            }
+        }
+    }
+    *E = kval(&acc);
+    free(w0); free(w1); free(w2); free(opA); free(opB); free(mt); free(oa); free(ob);
+    free(gu); free(gv); free(hu); free(hv); free(tmp);
+    return 0;
+}
+
 int orc_fixed_point_gradient(const orc_graph *g, const double *sites, const double *msgs,
                              const double *dL_dm, const double *dL_dA, int maxiter, double tol,
                              double *grad, int *out_iters, double *out_res) {

@@ -153,6 +153,14 @@ int orc_expect_vertex(const orc_graph *g, const double *sites, const double *msg
 int orc_expect_edge(const orc_graph *g, const double *sites, const double *msgs,
                     const double *ops_first, const double *ops_second, int ld, double *out);
 
+/* ---- BP-estimated local energy with arbitrary operators (the generalisation of the TFIM
+ * energy): E = sum_v <O_v>_BP + sum_e <A_e B_e>_BP with the expectation values of
+ * orc_expect_vertex / orc_expect_edge (ops_v, or the ops_first / ops_second pair, may be
+ * NULL to drop that sum); optional explicit partials as orc_tfim_energy. */
+int orc_local_energy(const orc_graph *g, const double *sites, const double *msgs,
+                     const double *ops_v, const double *ops_first, const double *ops_second,
+                     int ld, double *E, double *dE_dm, double *dE_dA);
+
 /* ---- full implicit-differentiation gradient of an objective L(m*, A) at the fixed
  * point m*: solves lambda = J_m^T lambda + dL/dm (projected onto the sum-normalised
  * tangent space) by adjoint sweeps to tolerance tol on the relative change

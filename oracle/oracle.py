@@ -72,13 +72,14 @@ def _declare(L):
     L.orc_tfim_energy.argtypes = [G, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp]
     L.orc_expect_vertex.argtypes = [G, _dp, _dp, _dp, C.c_int, _dp]
     L.orc_expect_edge.argtypes = [G, _dp, _dp, _dp, _dp, C.c_int, _dp]
+    L.orc_local_energy.argtypes = [G, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp]
     L.orc_fixed_point_gradient.argtypes = [G, _dp, _dp, _dp, _dp, C.c_int, C.c_double, _dp,
                                            _ip, _dp]
     for name in ("orc_norm_message", "orc_plain_message", "orc_sweep_jacobi",
                  "orc_sweep_sequential", "orc_bp", "orc_vertex_scalars", "orc_edge_scalars",
                  "orc_bethe_free_energy", "orc_contract_labeled", "orc_norm_vjp",
                  "orc_grad_lognorm_explicit", "orc_tfim_energy", "orc_fixed_point_gradient",
-              "orc_expect_vertex", "orc_expect_edge"):
+              "orc_expect_vertex", "orc_expect_edge", "orc_local_energy"):
         getattr(L, name).restype = C.c_int
 
 
@@ -316,6 +317,24 @@ def expect_edge(topo, sites, msgs, ops_first, ops_second):
     _check(lib().orc_expect_edge(g.ref, _p(_f64(sites)), _p(_f64(msgs)), _p(a), _p(b),
                                  int(a.shape[-1]), _p(out)), "expect_edge")
     return out
+
+
+def local_energy(topo, sites, msgs, ops_v=None, ops_first=None, ops_second=None,
+                 partials=False):
+    """E = sum_v <O_v>_BP + sum_e <A_e B_e>_BP (either sum may be omitted), optionally with
+    the explicit partials (dE/dm, dE/dA)."""
+    g = OracleGraph(topo)
+    ref = ops_v if ops_v is not None else ops_first
+    ld = int(np.shape(ref)[-1])
+    E = C.c_double(0)
+    dm = np.zeros(int(topo.msg_off[-1])) if partials else None
+    da = np.zeros(int(topo.site_off[-1])) if partials else None
+    ov = _f64(ops_v) if ops_v is not None else None
+    oa = _f64(ops_first) if ops_first is not None else None
+    ob = _f64(ops_second) if ops_second is not None else None
+    _check(lib().orc_local_energy(g.ref, _p(_f64(sites)), _p(_f64(msgs)), _p(ov), _p(oa), _p(ob),
+                                  ld, C.byref(E), _p(dm), _p(da)), "local_energy")
+    return (E.value, dm, da) if partials else E.value
 
 
 def fixed_point_gradient(topo, sites, msgs, dL_dm, dL_dA, maxiter=1000, tol=1e-12):

@@ -1,10 +1,12 @@
-"""BP expectation values with arbitrary operators on the device (bp_expect_vertex /
-bp_expect_edge) against the FP64 oracle, whose restatement is pinned by exact
-tree expectations, identity and linearity checks and the TFIM energy (test_oracle.py).
+"""BP expectation values and local energies with arbitrary operators on the device
+(bp_expect_vertex / bp_expect_edge / bp_local_energy) against the FP64 oracle, whose
+restatement is pinned by exact tree expectations, identity and linearity checks, the TFIM
+energy and finite differences of the gradient (test_oracle.py).
 
-Tolerances: FP64 contexts 1e-11 relative to the oracle at the same messages; FP32 contexts
-2e-5 relative (FP32 site tensors and contractions); TFIM energy assembled from the
-expectation values equals the engine's own TFIM energy within 1e-9 (FP64) / 1e-6 (FP32).
+Tolerances: expectation values, FP64 contexts 1e-11 relative to the oracle at the same
+messages, FP32 2e-5 (FP32 site tensors and contractions); TFIM energy assembled from the
+expectation values equals the engine's own within 1e-9 (FP64) / 1e-6 (FP32); local-energy
+gradients within 1e-7 (FP64) / 2e-3 (FP32) of the oracle's largest entry.
 """
 import numpy as np
 import pytest
@@ -111,3 +113,68 @@ def test_expect_host_api(pkg, orc):
                 a[k], b[k] = B, A
     want2 = orc.expect_edge(t, fl.sites, m, a, b)[k]
     assert got2[e0] == pytest.approx(want2, rel=1e-11)
+
+
+# ---------------------------------------------------------------- local energies + gradients
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("dtype,gtol", [("f64", 1e-7), ("f32", 2e-3)])
+def test_local_energy_gradient_vs_oracle(pkg, orc, dtype, gtol):
+    """Random non-symmetric one- and two-site operators: energy and fixed-point gradient on
+    the device against the oracle's (explicit partials + adjoint solve, FD-pinned)."""
+    _say(f"case local energy gradient {dtype}")
+    t = pkg.build_topology(pkg.named_grid((4, 4)), 3, 2)
+    s = pkg.peps.random_sites(t, 73, scale=pkg.peps.peps_scale(2, 3))
+    rng = np.random.default_rng(13)
+    ov = rng.standard_normal((t.nv, 2, 2))
+    oa = rng.standard_normal((t.ne, 2, 2))
+    ob = rng.standard_normal((t.ne, 2, 2))
+    with pkg.Engine(t, s, dtype=dtype) as e:
+        e.init_messages("identity")
+        if dtype == "f64":
+            r = e.run(maxiter=2000, tol=1e-14, criterion="maxabs")
+        else:
+            r = e.run(maxiter=500, tol=1e-12, criterion="fidelity")
+        assert r.converged[0]
+        m = e.messages()
+        E, g, it, res = e.local_energy(ov, oa, ob, grad=True, adj_tol=1e-10, adj_maxiter=2000)
+        Ed, gd, _, _ = e.local_energy(ov, oa, ob, grad=True, adj_tol=1e-10, adj_maxiter=2000,
+                                      device=True)
+        gdev = gd.numpy().astype(np.float64)
+        gd.free()
+    s_used = s.astype(np.float32).astype(np.float64) if dtype == "f32" else s
+    m64, _, _, at = orc.bp(t, s_used, m, maxiter=3000, tol=1e-15, criterion="maxabs")
+    Eo, dm, da = orc.local_energy(t, s_used, m64, ov, oa, ob, partials=True)
+    want, _, _ = orc.fixed_point_gradient(t, s_used, m64, dm, da, maxiter=5000, tol=1e-13)
+    Em = orc.local_energy(t, s_used, m, ov, oa, ob)  # the oracle at the engine's messages
+    _say(f"E={E:.12f} oracle={Em:.12f} (at its own fixed point {Eo:.12f}) sweeps={it} "
+         f"res={res:.1e} grad err={_rel(g, want):.2e}")
+    assert E == pytest.approx(Em, rel=1e-12 if dtype == "f64" else 1e-5)
+    assert Ed == E
+    assert _rel(g, want) < gtol
+    assert _rel(gdev, g) < (1e-12 if dtype == "f64" else 1e-6)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_local_energy_reduces_to_tfim(pkg, dtype):
+    """O_v = -h X, A_e = -J Z, B_e = Z: energy and gradient equal the TFIM entry point's
+    (on FP32 contexts the TFIM runs on the MFMA kernels, the local energy on the generic
+    ones)."""
+    _say(f"case local energy = tfim {dtype}")
+    t = pkg.build_topology(pkg.named_grid((5, 5)), 4, 2)
+    s = pkg.peps.random_sites(t, 74, scale=pkg.peps.peps_scale(2, 4))
+    J, h = 1.0, 3.0
+    with pkg.Engine(t, s, dtype=dtype) as e:
+        e.init_messages("identity")
+        e.run(maxiter=500, tol=1e-12, criterion="fidelity")
+        E0, g0, _, _ = e.tfim_energy(J, h, grad=True, adj_tol=1e-9, adj_maxiter=2000)
+        E1, g1, _, _ = e.local_energy(np.broadcast_to(-h * X, (t.nv, 2, 2)),
+                                      np.broadcast_to(-J * Z, (t.ne, 2, 2)),
+                                      np.broadcast_to(Z, (t.ne, 2, 2)), grad=True,
+                                      adj_tol=1e-9, adj_maxiter=2000)
+    _say(f"E tfim={E0:.12f} local={E1:.12f} grad rel diff={_rel(g1, g0):.2e}")
+    assert E1 == pytest.approx(E0, rel=1e-10 if dtype == "f64" else 1e-6)
+    assert _rel(g1, g0) < (1e-8 if dtype == "f64" else 2e-4)

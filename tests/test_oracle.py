@@ -363,3 +363,42 @@ def test_expect_reproduces_tfim_energy(pkg, orc):
                          np.broadcast_to(Z, (t.ne, 2, 2)))
     assert -h * ev.sum() - J * ee.sum() == pytest.approx(orc.tfim_energy(t, s, m, J, h),
                                                           rel=1e-13)
+
+
+def _tfim_ops(t, J, h):
+    X = np.array([[0.0, 1.0], [1.0, 0.0]])
+    Z = np.array([[1.0, 0.0], [0.0, -1.0]])
+    return (np.broadcast_to(-h * X, (t.nv, 2, 2)), np.broadcast_to(-J * Z, (t.ne, 2, 2)),
+            np.broadcast_to(Z, (t.ne, 2, 2)))
+
+
+def test_local_energy_reduces_to_tfim(pkg, orc):
+    """With O_v = -h X, A_e = -J Z, B_e = Z the local energy and both explicit partials are
+    the (FD-pinned) TFIM energy's."""
+    t, s, m = _converged(pkg, orc, 3, 3, 34)
+    E0, dm0, da0 = orc.tfim_energy(t, s, m, J=0.7, h=2.5, partials=True)
+    ov, oa, ob = _tfim_ops(t, 0.7, 2.5)
+    E, dm, da = orc.local_energy(t, s, m, ov, oa, ob, partials=True)
+    assert E == pytest.approx(E0, rel=1e-13)
+    np.testing.assert_allclose(dm, dm0, rtol=1e-11, atol=1e-13 * np.max(np.abs(dm0)))
+    np.testing.assert_allclose(da, da0, rtol=1e-11, atol=1e-13 * np.max(np.abs(da0)))
+
+
+def test_local_energy_gradient_vs_fd(pkg, orc):
+    """Random non-symmetric one- and two-site operators: the implicit gradient (explicit
+    partials + adjoint solve) matches central finite differences of re-converged BP."""
+    t, s, m = _converged(pkg, orc, 3, 2, 35, scale=False)
+    rng = np.random.default_rng(12)
+    ov = rng.standard_normal((t.nv, 2, 2))
+    oa = rng.standard_normal((t.ne, 2, 2))
+    ob = rng.standard_normal((t.ne, 2, 2))
+    E, dm, da = orc.local_energy(t, s, m, ov, oa, ob, partials=True)
+    grad, it, res = orc.fixed_point_gradient(t, s, m, dm, da, maxiter=5000, tol=1e-14)
+    assert res < 1e-12
+    idx = rng.choice(s.size, 8, replace=False)
+    fd = _fd(pkg, orc, t, s, m,
+             lambda t_, s_, m_: orc.local_energy(t_, s_, m_, ov, oa, ob), idx, 1e-5)
+    np.testing.assert_allclose(grad[idx], fd, rtol=1e-6, atol=1e-8)
+    # the value is the sum of the expectation values
+    want = (orc.expect_vertex(t, s, m, ov).sum() + orc.expect_edge(t, s, m, oa, ob).sum())
+    assert E == pytest.approx(want, rel=1e-13)
