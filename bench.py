@@ -129,6 +129,44 @@ def uneven_batch_timing(pkg, torch, seed, sweeps=20):
     return out
 
 
+def local_energy_timing(pkg, torch, seed, L=16, D=6):
+    """Expectation values and a general local energy with its gradient on the C4 lattice
+    (16x16, d=2, D=6, FP32), random one-site and nearest-neighbour operators; wall times
+    after one untimed call of each (generic kernels), beside the TFIM energy + gradient on
+    the MFMA kernels."""
+    import time
+
+    t = pkg.build_topology(pkg.named_grid((L, L)), D, 2)
+    s = pkg.peps.random_sites(t, seed, scale=pkg.peps.peps_scale(2, D))
+    rng = np.random.default_rng(seed)
+    ov = rng.standard_normal((t.nv, 2, 2))
+    oa = rng.standard_normal((t.ne, 2, 2))
+    ob = rng.standard_normal((t.ne, 2, 2))
+    e = pkg.Engine(t, s, dtype="f32")
+    e.init_messages("identity")
+    e.run(maxiter=500, tol=1e-8)
+
+    def timed(f):
+        f()
+        t0 = time.perf_counter()
+        out = f()
+        return out, time.perf_counter() - t0
+
+    _, t_v = timed(lambda: e.expect_vertex(ov))
+    _, t_e = timed(lambda: e.expect_edge(oa, ob))
+    buf = e.local_energy(ov, oa, ob, grad=True, adj_tol=1e-6, adj_maxiter=2000, device=True)[1]
+    (E, _, it, res), t_l = timed(lambda: e.local_energy(ov, oa, ob, grad=True, adj_tol=1e-6,
+                                                         adj_maxiter=2000, device=True, out=buf))
+    (_, _, it_t, _), t_t = timed(lambda: e.tfim_energy(1.0, 3.0, grad=True, adj_tol=1e-6,
+                                                        adj_maxiter=2000, device=True, out=buf))
+    buf.free()
+    e.close()
+    return {"workload": f"{L}x{L} D={D} f32, random 2x2 one-site and nearest-neighbour "
+                        f"operators", "expect_vertex_s": t_v, "expect_edge_s": t_e,
+            "local_energy_plus_gradient_s": t_l, "local_energy_adjoint_sweeps": it,
+            "local_energy": E, "tfim_plus_gradient_mfma_s": t_t, "tfim_adjoint_sweeps": it_t}
+
+
 def large_bond_timing(pkg, torch, seed, L=24, D=16):
     """The mixed-precision workload at large bond dimension: BF16 MFMA sweeps (FP32 sums,
     per-message max-norm scaling) to fidelity 1e-5, then 3 FP64 polishing sweeps; the
@@ -187,11 +225,13 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--only", choices=["adjoint", "large_bond", "uneven"], default=None,
+    ap.add_argument("--only", choices=["adjoint", "large_bond", "uneven", "local_energy"],
+                    default=None,
                     help="adjoint: run only the forward + adjoint workload (L x L, D; three "
                          "energy + gradient calls); large_bond: only the 24x24 D=16 BF16 + "
                          "FP64-polish workload; uneven: only the 64-network batched "
-                         "workload; print the timings, for profiling")
+                         "workload; local_energy: only the expectation-value / local-energy "
+                         "timings; print the timings, for profiling")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, used to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -212,6 +252,11 @@ def main(argv=None):
         runs = [adjoint_timing(pkg, torch, args.L, args.D, args.dtype, args.seed)
                 for _ in range(3)]
         print(json.dumps({"adjoint_only": runs}), flush=True)
+        return 0
+    if args.only == "local_energy":
+        torch.cuda.init()
+        print(json.dumps({"local_energy_only": local_energy_timing(pkg, torch, args.seed)}),
+              flush=True)
         return 0
     if args.only == "uneven":
         torch.cuda.init()
@@ -328,6 +373,7 @@ def main(argv=None):
     if not args.no_extras and args.dtype == "f32":
         extras["uneven_batch"] = uneven_batch_timing(pkg, torch, args.seed)
         extras["large_bond_bf16"] = large_bond_timing(pkg, torch, args.seed)
+        extras["local_energy"] = local_energy_timing(pkg, torch, args.seed)
 
     cpu = None
     if not args.no_cpu:

@@ -227,6 +227,53 @@ function device_tfim_energy_and_grad!(ctx::DeviceContext, grad_ptr::Ptr{Cvoid}; 
     return E[]
 end
 
+# operators as an ld x ld x n array (Julia column-major): block i, row a, column b is
+# op[a, b, i]; the C side reads row-major ld x ld blocks, so pass the transposes
+_rowmajor(ops::AbstractArray{Float64, 3}) = permutedims(ops, (2, 1, 3))
+
+"""
+    device_expect_vertex(ctx, ops) -> Vector{Float64}
+
+`<O_v>_BP` for every vertex (topology order), `ops[:, :, v]` a `d x d` operator on the
+physical index of the ket layer (`ld x ld` blocks, top-left corner used).
+"""
+function device_expect_vertex(ctx::DeviceContext, ops::AbstractArray{Float64, 3})
+    rm = _rowmajor(ops)
+    out = zeros(Float64, size(ops, 3))
+    GC.@preserve rm out check(ccall((:bp_expect_vertex, libbp[]), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Cint, Ptr{Float64}), ctx.handle, rm, size(ops, 1), out),
+        "bp_expect_vertex")
+    return out
+end
+
+"""`<A_e B_e>_BP` for every topology edge; `A[:, :, e]` acts on the vertex owning the edge's
+lower message slot, `B[:, :, e]` on the other end."""
+function device_expect_edge(ctx::DeviceContext, A::AbstractArray{Float64, 3},
+        B::AbstractArray{Float64, 3})
+    ra, rb = _rowmajor(A), _rowmajor(B)
+    out = zeros(Float64, size(A, 3))
+    GC.@preserve ra rb out check(ccall((:bp_expect_edge, libbp[]), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Float64}), ctx.handle, ra, rb,
+        size(A, 1), out), "bp_expect_edge")
+    return out
+end
+
+"""Local energy `sum_v <O_v> + sum_e <A_e B_e>` and its gradient (flat site layout) by the
+device's fixed-point adjoint."""
+function device_local_energy_and_grad(ctx::DeviceContext, nsite::Integer, O, A, B;
+        adj_tol = 1.0e-8, adj_maxiter = 1000)
+    ro, ra, rb = _rowmajor(O), _rowmajor(A), _rowmajor(B)
+    E = Ref{Float64}(0.0)
+    g = zeros(Float64, nsite)
+    it = Ref{Cint}(0)
+    res = Ref{Float64}(0.0)
+    GC.@preserve ro ra rb g check(ccall((:bp_local_energy, libbp[]), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint, Ref{Float64},
+            Ptr{Float64}, Cdouble, Cint, Ref{Cint}, Ref{Float64}), ctx.handle, ro, ra, rb,
+        size(O, 1), E, g, adj_tol, adj_maxiter, it, res), "bp_local_energy")
+    return E[], g
+end
+
 """Autotune the kernel choices for this context's shapes; `mode` 0 applies the cache at
 `cache` only, 1 tunes what it lacks (and rewrites it), 2 retunes everything."""
 function autotune!(ctx::DeviceContext; cache::AbstractString = "", mode::Integer = 1)
