@@ -687,7 +687,8 @@ static int sweep_once(bp_context *c, const NetCtl &done) {
     return BP_OK;
 }
 
-// Padded site layouts of both vertex buckets (bond dimension <= 8, and 9..16 in BF16).
+// Padded site layouts of the vertex buckets (bond dimension <= 8 with its degree <= 3 copies,
+// and the large-bond bucket: 9..12 in FP32, 9..16 in BF16).
 static int upload_vertex_sites(bp_context *c) {
     bump_epoch();
     if (vertex_upload_sites(&c->vplan, c->stream, c->d_sites64, &c->d_vsites, c->d_site_off,
