@@ -167,6 +167,71 @@ def local_energy_timing(pkg, torch, seed, L=16, D=6):
             "local_energy": E, "tfim_plus_gradient_mfma_s": t_t, "tfim_adjoint_sweeps": it_t}
 
 
+def norm_network_labels(pkg, t, s):
+    """The double-layer tensors of <psi|psi> (ket A_u and bra A_u per vertex, sharing the
+    site label, links labelled per layer), as the oracle's exact norm builds them."""
+    tensors, labels = [], []
+    for u in range(t.nv):
+        A = s[t.site_off[u]:t.site_off[u + 1]].reshape(t.site_shape(u))
+        ports = range(t.port_off[u], t.port_off[u + 1])
+        tensors += [A, A]
+        labels += [(("s", u),) + tuple(("k", int(t.port_edge[q])) for q in ports),
+                   (("s", u),) + tuple(("b", int(t.port_edge[q])) for q in ports)]
+    return tensors, labels
+
+
+def contract_timing(pkg, seed, cases=((6, 3), (5, 4)), cpu_case=(6, 3), reps=5):
+    """Exact <psi|psi> of L x L PEPS norm networks (d=2) on the device (ContractionPlan with
+    the cheapest order, inputs resident in device memory, FP64 products on MFMA): device time
+    of the plan's launches (HIP events, the best of `reps` executions after one untimed) and
+    the algorithmic FP64 rate; the oracle's frontier contraction of `cpu_case` timed beside
+    it on one core, with the relative difference of the two values."""
+    import time
+
+    out = []
+    for L, D in cases:
+        t = pkg.build_topology(pkg.named_grid((L, L)), D, 2)
+        s = pkg.peps.random_sites(t, seed, scale=pkg.peps.peps_scale(2, D))
+        tensors, labels = norm_network_labels(pkg, t, s)
+        t0 = time.perf_counter()
+        plan = pkg.ContractionPlan([x.shape for x in tensors], labels, order_alg="cheapest")
+        t_plan = time.perf_counter() - t0
+        flat = np.concatenate([x.ravel() for x in tensors])
+        din = pkg._lib.DeviceArray(flat.size, np.float64)
+        dout = pkg._lib.DeviceArray(1, np.float64)
+        din.upload(flat)
+        plan.execute_device(din, dout)
+        ms = []
+        for _ in range(reps):
+            plan.execute_device(din, dout)
+            ms.append(plan.last_ms)
+        z = float(dout.numpy()[0])
+        t1 = time.perf_counter()
+        zh = plan(flat=flat)
+        t_host = time.perf_counter() - t1
+        best = min(ms)
+        rec = {"workload": f"exact <psi|psi>, {L}x{L} PEPS norm network, d=2, D={D}, "
+                           f"{len(tensors)} tensors, cheapest order",
+               "steps": len(tensors) - 1, "GFLOP": plan.flops / 1e9,
+               "workspace_MB": plan.workspace_bytes / 1e6, "plan_s": t_plan,
+               "device_ms_best": best, "device_ms_median": float(np.median(ms)),
+               "fp64_TFLOPs": plan.flops / (best * 1e-3) / 1e12,
+               "frac_fp64_peak": plan.flops / (best * 1e-3) / 78.6e12,
+               "host_buffers_call_s": t_host, "value": z, "host_value_equal": zh == z}
+        if (L, D) == cpu_case:
+            orc = __import__("__graft_entry__").import_oracle()
+            t2 = time.perf_counter()
+            zo = orc.norm_exact(t, s)
+            t_cpu = time.perf_counter() - t2
+            rec.update({"cpu_oracle_s": t_cpu, "cpu_oracle_order": "left-associative frontier",
+                        "cpu_cores": 1, "rel_diff_vs_oracle": abs(z - zo) / abs(zo)})
+        din.free()
+        dout.free()
+        plan.close()
+        out.append(rec)
+    return out
+
+
 def large_bond_timing(pkg, torch, seed, L=24, D=16):
     """The mixed-precision workload at large bond dimension: BF16 MFMA sweeps (FP32 sums,
     per-message max-norm scaling) to fidelity 1e-5, then 3 FP64 polishing sweeps; the
@@ -225,13 +290,15 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--only", choices=["adjoint", "large_bond", "uneven", "local_energy"],
+    ap.add_argument("--only", choices=["adjoint", "large_bond", "uneven", "local_energy",
+                                       "contract"],
                     default=None,
                     help="adjoint: run only the forward + adjoint workload (L x L, D; three "
                          "energy + gradient calls); large_bond: only the 24x24 D=16 BF16 + "
                          "FP64-polish workload; uneven: only the 64-network batched "
                          "workload; local_energy: only the expectation-value / local-energy "
-                         "timings; print the timings, for profiling")
+                         "timings; contract: only the exact-contraction timings; print the "
+                         "timings, for profiling")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, used to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -252,6 +319,10 @@ def main(argv=None):
         runs = [adjoint_timing(pkg, torch, args.L, args.D, args.dtype, args.seed)
                 for _ in range(3)]
         print(json.dumps({"adjoint_only": runs}), flush=True)
+        return 0
+    if args.only == "contract":
+        torch.cuda.init()
+        print(json.dumps({"contract_only": contract_timing(pkg, args.seed)}), flush=True)
         return 0
     if args.only == "local_energy":
         torch.cuda.init()
@@ -374,6 +445,7 @@ def main(argv=None):
         extras["uneven_batch"] = uneven_batch_timing(pkg, torch, args.seed)
         extras["large_bond_bf16"] = large_bond_timing(pkg, torch, args.seed)
         extras["local_energy"] = local_energy_timing(pkg, torch, args.seed)
+        extras["contract_network"] = contract_timing(pkg, args.seed)
 
     cpu = None
     if not args.no_cpu:

@@ -21,6 +21,8 @@
  *   bp_set_topology          MessageCache / NormNetwork graph structure
  *                            src/beliefpropagation/messagecache.jl:15-49, src/normnetwork.jl:13-28
  *   bp_upload_sites          NormNetwork vertex data lazy(A)*lazy(conj(A')) src/normnetwork.jl:49-54
+ *   bp_contract_network,     contract_network(tn; alg = Exact(; order, order_alg)) and
+ *   bp_cn_*                  contraction_order src/contract_network.jl:6-60
  * The adjoint entry points (bp_grad_*) and the energy estimator have no reference
  * counterpart: the reference has no AD and no expectation-value API.
  *
@@ -237,6 +239,53 @@ int bp_time_sweeps(bp_context *ctx, int n, double *avg_ms);
 /* Elapsed milliseconds of the last bp_run, measured with HIP events on the context's
  * stream around the sweep launches. */
 int bp_last_run_ms(bp_context *ctx, double *ms);
+
+/* ---------------------------------------------------------------- exact contraction
+ * Exact contraction of a labelled tensor network (contract_network with the Exact
+ * algorithm, contract_network.jl:12-37; the order choices of contraction_order,
+ * :40-60).  Tensors t = 0..nt-1 carry ranks[t] labels, concatenated in `labels`; label l
+ * has dimension label_dim[l] (0 <= l < n_labels).  A label on two tensors is summed over,
+ * a label on one stays open; no label may appear twice on one tensor or on more than two
+ * tensors.  The data of every tensor is row-major over its labels, the tensors
+ * concatenated in order.
+ *   order: NULL, or nt-1 pairwise steps (order[2s], order[2s+1]) over ids 0..nt-1 (the
+ *     tensors) and nt+s' (the result of step s'), each id used once (Exact's `order`);
+ *   order_alg (when order is NULL): BP_CN_ORDER_LEFT, ((T0 T1) T2)... (contraction_order
+ *     with LeftAssociative); BP_CN_ORDER_GREEDY: repeatedly the pair of live operands
+ *     sharing a label whose result grows the live size least (size(result) - size(a) -
+ *     size(b); ties to the earlier pair, results appended at the end; the role of the
+ *     reference's default Greedy() order_alg, whose algorithm lives in ITensorBase);
+ *     BP_CN_ORDER_CHEAPEST: the tree with the fewest FLOPs (then the smallest largest
+ *     intermediate) among the left-associative one and three greedy ones (costs: growth,
+ *     result size, result size over the larger operand's size).  The contracted values do
+ *     not depend on the order beyond roundoff; the result's label order does.
+ * Every step is one FP64 product C[a's open labels, b's open labels] = sum over the shared
+ * labels, so the result's labels are in that order (for the left-associative order: the
+ * open labels in first-appearance order of the tensor list).  Intermediates (and the
+ * gathered operands of transposes) may hold at most max_elems entries (<= 0: 2^31);
+ * BP_ERR_UNSUPPORTED otherwise.  An empty network contracts to the scalar 1.
+ *
+ * A plan (order, layouts, offset tables, device workspace) is built once and executed on
+ * any number of inputs of the same shapes: on host buffers (bp_cn_execute) or device
+ * buffers of this library's HIP runtime (bp_cn_execute_device, result left in device
+ * memory).  Both return after the result is written; bp_last_run_ms then gives the device
+ * time of the contraction's launches.  bp_contract_network plans, executes and frees in
+ * one call (out_cap: capacity of `out` in entries, out_labels: room for 64 labels). */
+typedef struct bp_cn_plan bp_cn_plan;
+enum { BP_CN_ORDER_LEFT = 0, BP_CN_ORDER_GREEDY = 1, BP_CN_ORDER_CHEAPEST = 2 };
+int bp_cn_plan_create(bp_context *ctx, int nt, const int *ranks, const int *labels,
+                      int n_labels, const int *label_dim, const int *order, int order_alg,
+                      long long max_elems, bp_cn_plan **out);
+int bp_cn_plan_info(const bp_cn_plan *plan, int *out_rank, int *out_labels,
+                    long long *out_size, long long *in_size, double *flops,
+                    long long *workspace_bytes);
+int bp_cn_execute(bp_context *ctx, bp_cn_plan *plan, const double *data, double *out);
+int bp_cn_execute_device(bp_context *ctx, bp_cn_plan *plan, const void *data, void *out);
+int bp_cn_plan_destroy(bp_cn_plan *plan);
+int bp_contract_network(bp_context *ctx, int nt, const int *ranks, const int *labels,
+                        int n_labels, const int *label_dim, const double *data,
+                        const int *order, int order_alg, long long max_elems, int *out_rank,
+                        int *out_labels, double *out, long long out_cap);
 
 #ifdef __cplusplus
 }

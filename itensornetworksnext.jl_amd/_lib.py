@@ -64,7 +64,17 @@ SIGNATURES = {
     "bp_set_option": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "bp_last_run_ms": (C.c_int, [_vp, _dp]),
     "bp_time_sweeps": (C.c_int, [_vp, C.c_int, _dp]),
+    "bp_cn_plan_create": (C.c_int, [_vp, C.c_int, _ip, _ip, C.c_int, _ip, _ip, C.c_int,
+                                    C.c_longlong, C.POINTER(C.c_void_p)]),
+    "bp_cn_plan_info": (C.c_int, [_vp, _ip, _ip, C.POINTER(C.c_longlong),
+                                  C.POINTER(C.c_longlong), _dp, C.POINTER(C.c_longlong)]),
+    "bp_cn_execute": (C.c_int, [_vp, _vp, _dp, _dp]),
+    "bp_cn_execute_device": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "bp_cn_plan_destroy": (C.c_int, [_vp]),
+    "bp_contract_network": (C.c_int, [_vp, C.c_int, _ip, _ip, C.c_int, _ip, _dp, _ip, C.c_int,
+                                      C.c_longlong, _ip, _ip, _dp, C.c_longlong]),
 }
+BP_CN_ORDER_LEFT, BP_CN_ORDER_GREEDY, BP_CN_ORDER_CHEAPEST = 0, 1, 2
 
 _lib = None
 
@@ -148,6 +158,16 @@ class DeviceArray:
         if rc != 0:
             raise BPError(BP_ERR_HIP, "hipMemcpy", f"hip error {rc}")
         return out
+
+    def upload(self, arr):
+        import numpy as np
+
+        a = np.ascontiguousarray(arr, dtype=self.dtype).reshape(-1)
+        if a.size != self.n:
+            raise ValueError(f"upload of {a.size} entries into a buffer of {self.n}")
+        rc = hip_runtime().hipMemcpy(self.ptr, a.ctypes.data, self.nbytes, 1)  # to device
+        if rc != 0:
+            raise BPError(BP_ERR_HIP, "hipMemcpy", f"hip error {rc}")
 
     def free(self):
         if getattr(self, "ptr", None):

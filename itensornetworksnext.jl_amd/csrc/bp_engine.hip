@@ -26,6 +26,7 @@
 #include "vertex16.h"
 #include "adjoint.h"
 #include "vertex_adjoint.h"
+#include "contract.h"
 
 // Process-wide plan epoch: moves on every device (re)allocation and every change of a
 // kernel choice, so that a cached sweep graph never replays stale pointers.
@@ -1863,6 +1864,109 @@ int bp_tfim_energy_device(bp_context *c, double J, double h, double *energy, voi
     if (!c || !grad) return set_err(BP_ERR_ARG, "NULL argument");
     return tfim_impl(c, J, h, energy, device_grad(c, grad), adj_tol, adj_maxiter, out_iters,
                      out_res);
+}
+
+// ------------------------------------------------------------- exact contraction
+
+int bp_cn_plan_create(bp_context *c, int nt, const int *ranks, const int *labels, int n_labels,
+                      const int *label_dim, const int *order, int order_alg,
+                      long long max_elems, bp_cn_plan **out) {
+    if (!c || !out) return set_err(BP_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    HIPCHK(hipSetDevice(c->device));
+    bp_cn_plan *p = new bp_cn_plan();
+    p->device = c->device;
+    std::string msg;
+    int rc = cn_build_plan(nt, ranks, labels, n_labels, label_dim, order, order_alg,
+                           max_elems, *p, msg);
+    if (rc == 0) rc = cn_alloc_plan(*p, msg);
+    if (rc != 0) {
+        cn_free_plan(*p);
+        delete p;
+        return set_err(rc == 1 ? BP_ERR_ARG : rc == 2 ? BP_ERR_HIP : rc == 4 ? BP_ERR_UNSUPPORTED
+                                                                            : BP_ERR_NOMEM,
+                       "contraction plan: %s", msg.c_str());
+    }
+    *out = p;
+    return BP_OK;
+}
+
+int bp_cn_plan_info(const bp_cn_plan *p, int *out_rank, int *out_labels, long long *out_size,
+                    long long *in_size, double *flops, long long *workspace_bytes) {
+    if (!p) return set_err(BP_ERR_ARG, "plan is NULL");
+    if (out_rank) *out_rank = (int)p->out_labels.size();
+    if (out_labels)
+        for (size_t i = 0; i < p->out_labels.size(); ++i) out_labels[i] = p->out_labels[i];
+    if (out_size) *out_size = p->out_size;
+    if (in_size) *in_size = p->in_total;
+    if (flops) *flops = p->flops;
+    if (workspace_bytes) *workspace_bytes = (long long)sizeof(double) * p->arena_elems;
+    return BP_OK;
+}
+
+int bp_cn_plan_destroy(bp_cn_plan *p) {
+    if (!p) return BP_OK;
+    (void)hipSetDevice(p->device);
+    cn_free_plan(*p);
+    delete p;
+    return BP_OK;
+}
+
+// enqueue the plan over device inputs `in`, copy the result to `out` (kind: a hipMemcpyKind),
+// synchronise; the launches' device time goes to last_ms
+static int cn_run(bp_context *c, bp_cn_plan *p, const double *in, void *out,
+                  hipMemcpyKind kind) {
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    const double *res = nullptr;
+    HIPCHK(cn_enqueue(*p, in, c->stream, &res));
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipMemcpyAsync(out, res, sizeof(double) * p->out_size, kind, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    return BP_OK;
+}
+
+int bp_cn_execute(bp_context *c, bp_cn_plan *p, const double *data, double *out) {
+    if (!c || !p || !out || (p->in_total > 0 && !data)) return set_err(BP_ERR_ARG, "NULL argument");
+    if (c->device != p->device)
+        return set_err(BP_ERR_ARG, "plan was made on device %d, context is on %d", p->device,
+                       c->device);
+    HIPCHK(hipSetDevice(c->device));
+    if (!p->staging && p->in_total > 0)
+        HIPCHK(hipMalloc((void **)&p->staging, sizeof(double) * p->in_total));
+    if (p->in_total > 0)
+        HIPCHK(hipMemcpyAsync(p->staging, data, sizeof(double) * p->in_total,
+                              hipMemcpyHostToDevice, c->stream));
+    return cn_run(c, p, p->staging, out, hipMemcpyDeviceToHost);
+}
+
+int bp_cn_execute_device(bp_context *c, bp_cn_plan *p, const void *data, void *out) {
+    if (!c || !p || !out || (p->in_total > 0 && !data)) return set_err(BP_ERR_ARG, "NULL argument");
+    if (c->device != p->device)
+        return set_err(BP_ERR_ARG, "plan was made on device %d, context is on %d", p->device,
+                       c->device);
+    HIPCHK(hipSetDevice(c->device));
+    return cn_run(c, p, (const double *)data, out, hipMemcpyDeviceToDevice);
+}
+
+int bp_contract_network(bp_context *c, int nt, const int *ranks, const int *labels,
+                        int n_labels, const int *label_dim, const double *data,
+                        const int *order, int order_alg, long long max_elems, int *out_rank,
+                        int *out_labels, double *out, long long out_cap) {
+    if (!out_rank || !out_labels || !out) return set_err(BP_ERR_ARG, "NULL argument");
+    bp_cn_plan *p = nullptr;
+    RETCHK(bp_cn_plan_create(c, nt, ranks, labels, n_labels, label_dim, order, order_alg,
+                             max_elems, &p));
+    int rc = BP_OK;
+    if (p->out_size > out_cap)
+        rc = set_err(BP_ERR_ARG, "result has %lld entries, out_cap is %lld", p->out_size,
+                     out_cap);
+    if (rc == BP_OK) rc = bp_cn_execute(c, p, data, out);
+    if (rc == BP_OK) bp_cn_plan_info(p, out_rank, out_labels, nullptr, nullptr, nullptr, nullptr);
+    bp_cn_plan_destroy(p);
+    return rc;
 }
 
 }  // extern "C"

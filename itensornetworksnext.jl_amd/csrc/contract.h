@@ -1,0 +1,685 @@
+// contract.h -- exact contraction of a small labelled tensor network on the device
+// (contract_network with the Exact algorithm, src/contract_network.jl:32-37).
+//
+// A contraction is planned once on the host and then executed as one stream of kernel
+// launches with no host synchronisation in between:
+//
+//  * the order is a binary tree of pairwise contractions: explicit (Exact's `order`), left
+//    associative over the tensor list (((T0 T1) T2) ...), greedy (repeatedly the pair of
+//    live operands sharing a label whose result grows the live size least, size(result) -
+//    size(a) - size(b), ties to the earlier pair), or the cheapest in FLOPs of the
+//    left-associative tree and three greedy trees (growth, result size, and result size
+//    over the larger operand's as the cost), all planned on the host from the labels;
+//  * every pairwise contraction is one FP64 matrix product C[FO, TO] = A[FO, SH] B[SH, TO]
+//    (FO / TO: the labels of a / b that stay open, SH: the shared ones, in a's order unless
+//    b's order saves a gather).  An operand whose labels already split into two contiguous
+//    groups is read in place, row-major or transposed; otherwise it is first gathered into
+//    matrix layout through two offset tables (a tensor transpose is a gather);
+//  * the result's labels are a's open labels followed by b's, so the left-associative order
+//    reproduces the labelled frontier contraction label for label;
+//  * intermediates, gathered operands and split-K partial products live in one device
+//    arena whose offsets the plan assigns first-fit, each block freed (for reuse by later
+//    launches on the same stream) as soon as the launch that reads it has been enqueued.
+//
+// The products run on v_mfma_f64_16x16x4f64 (FP64 keeps the result at the precision of a
+// CPU FP64 contraction): 64 x 64 output tile per 256-thread workgroup, one 32 x 32 quarter
+// (2 x 2 MFMA blocks) per wave, K in steps of 16 through LDS.  Lane l supplies A[l%16][l/16]
+// and B[l/16][l%16] and receives D[l/16 + 4r][l%16] in item r (measured on gfx950; the
+// FP32 16x16x4 instruction returns rows 4(l/16) + r instead).  Products with few output
+// tiles and a long K split K over the grid's y dimension into partial tiles that a second
+// kernel sums in a fixed order (deterministic results).
+#pragma once
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+typedef double cn_d4 __attribute__((ext_vector_type(4)));
+
+// dst[a * nc + c] = src[offa[a] + offc[c]]
+__global__ void k_cn_gather(const double *__restrict__ src, const long long *__restrict__ offa,
+                            const long long *__restrict__ offc, long long na, long long nc,
+                            double *__restrict__ dst) {
+    const long long n = na * nc;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long a = i / nc, c = i - a * nc;
+        dst[i] = src[offa[a] + offc[c]];
+    }
+}
+
+// C[m, n] (+ z * M * N for split K) = sum over k in [z * kchunk, (z + 1) * kchunk) of
+// A[m * sam + k * sak] * B[k * sbk + n * sbn]; C row-major.  blockIdx.x = output tile
+// (row-major over the tile grid), blockIdx.y = z.
+#define CN_TM 64
+#define CN_TK 16
+#define CN_LD (CN_TM + 4)
+__global__ __launch_bounds__(256) void k_cn_gemm(const double *__restrict__ A, long long sam,
+                                                 long long sak, const double *__restrict__ B,
+                                                 long long sbk, long long sbn,
+                                                 double *__restrict__ C, long long M,
+                                                 long long N, long long K, long long kchunk) {
+    __shared__ double As[CN_TK][CN_LD];  // As[k][m]
+    __shared__ double Bs[CN_TK][CN_LD];  // Bs[k][n]
+    const long long ntn = (N + CN_TM - 1) / CN_TM;
+    const long long m0 = ((long long)blockIdx.x / ntn) * CN_TM,
+                    n0 = ((long long)blockIdx.x % ntn) * CN_TM;
+    const long long kb = (long long)blockIdx.y * kchunk;
+    const long long ke = std::min(K, kb + kchunk);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    cn_d4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = cn_d4{0.0, 0.0, 0.0, 0.0};
+    // element order of the tile loads: along k when that operand is contiguous in k
+    const bool a_kfast = sak == 1, b_kfast = sbn != 1;
+    for (long long k0 = kb; k0 < ke; k0 += CN_TK) {
+        for (int i = threadIdx.x; i < CN_TK * CN_TM; i += 256) {
+            int kk, mm;
+            if (a_kfast) { kk = i % CN_TK; mm = i / CN_TK; }
+            else { mm = i % CN_TM; kk = i / CN_TM; }
+            const long long gm = m0 + mm, gk = k0 + kk;
+            As[kk][mm] = (gm < M && gk < ke) ? A[gm * sam + gk * sak] : 0.0;
+            int kb2, nn;
+            if (b_kfast) { kb2 = i % CN_TK; nn = i / CN_TK; }
+            else { nn = i % CN_TM; kb2 = i / CN_TM; }
+            const long long gn = n0 + nn, gk2 = k0 + kb2;
+            Bs[kb2][nn] = (gn < N && gk2 < ke) ? B[gk2 * sbk + gn * sbn] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < CN_TK; ks += 4) {
+            double a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = As[ks + lk][wr + 16 * i + li];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = Bs[ks + lk][wc + 16 * j + li];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0,
+                                                                      0, 0);
+        }
+        __syncthreads();
+    }
+    double *Cz = C + (long long)blockIdx.y * M * N;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long long gm = m0 + wr + 16 * i + lk + 4 * r;
+                const long long gn = n0 + wc + 16 * j + li;
+                if (gm < M && gn < N) Cz[gm * N + gn] = acc[i][j][r];
+            }
+}
+
+// C[i] = sum_z P[z * n + i], z in increasing order
+__global__ void k_cn_sum_parts(const double *__restrict__ P, int nparts, long long n,
+                               double *__restrict__ C) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int z = 0; z < nparts; ++z) s += P[(long long)z * n + i];
+        C[i] = s;
+    }
+}
+
+// ------------------------------------------------------------------ host plan
+
+enum { CN_ORDER_LEFT = 0, CN_ORDER_GREEDY = 1, CN_ORDER_CHEAPEST = 2 };
+
+struct CnStep {
+    int a = 0, b = 0;             // operand ids (< nt: input tensors, else nt + step)
+    long long M = 1, N = 1, K = 1;
+    // operand A: gathered (ga >= 0, arena offset; tables at offsets oar / oac) or read in
+    // place with strides (sam, sak); B likewise
+    long long ga = -1, oar = 0, oac = 0, sam = 0, sak = 0;
+    long long gb = -1, obr = 0, obc = 0, sbk = 0, sbn = 0;
+    long long c = 0;              // arena offset of the result
+    int splits = 1;
+    long long kchunk = 0, part = -1;  // split K: chunk and arena offset of the partials
+};
+
+struct bp_cn_plan {
+    int device = 0;
+    int nt = 0;
+    std::vector<long long> in_off, in_size;  // input tensor offsets / sizes (elements)
+    long long in_total = 0;
+    std::vector<CnStep> steps;
+    std::vector<int> out_labels;
+    long long out_size = 1;
+    double flops = 0.0;
+    long long arena_elems = 0, max_interm = 0;
+    double *arena = nullptr;
+    long long *offs = nullptr;
+    std::vector<long long> h_offs;  // offset tables (host copy)
+    double *staging = nullptr;  // device copy of host inputs for the host-buffer execution
+    double *one = nullptr;      // the scalar 1 (an empty network)
+};
+
+// first-fit allocator over element offsets (host bookkeeping only)
+struct CnArena {
+    std::vector<std::pair<long long, long long>> used;  // (offset, size), sorted by offset
+    long long high = 0;
+    long long alloc(long long n) {
+        n = (n + 31) / 32 * 32;
+        long long at = 0;
+        size_t pos = 0;
+        for (; pos < used.size(); ++pos) {
+            if (used[pos].first - at >= n) break;
+            at = used[pos].first + used[pos].second;
+        }
+        used.insert(used.begin() + pos, {at, n});
+        high = std::max(high, at + n);
+        return at;
+    }
+    void release(long long off) {
+        for (size_t i = 0; i < used.size(); ++i)
+            if (used[i].first == off) {
+                used.erase(used.begin() + i);
+                return;
+            }
+    }
+};
+
+struct CnOperand {
+    std::vector<int> lab;
+    long long size = 1;
+    long long arena = -1;  // arena offset (intermediates) or -1 (an input tensor)
+    bool live = false;
+};
+
+// offsets of every multi-index over the positions `pos` of a tensor with row-major strides
+// `str` and dimensions `dim` (per position), enumerated row-major over `pos`
+static void cn_offsets(const std::vector<int> &pos, const std::vector<long long> &str,
+                       const std::vector<int> &dim, std::vector<long long> &out) {
+    long long n = 1;
+    for (int p : pos) n *= dim[p];
+    const size_t base = out.size();
+    out.resize(base + (size_t)n);
+    for (long long x = 0; x < n; ++x) {
+        long long rem = x, o = 0;
+        for (int i = (int)pos.size() - 1; i >= 0; --i) {
+            const int d = dim[pos[i]];
+            o += (rem % d) * str[pos[i]];
+            rem /= d;
+        }
+        out[base + (size_t)x] = o;
+    }
+}
+
+// Plans one pairwise contraction; fills the step (gathers, strides, arena blocks, split K)
+// and returns the result operand.  Codes: 0, 3 (a buffer beyond max_elems), 5 (> 64 open
+// labels).
+static int cn_plan_step(const CnOperand &A, const CnOperand &B, const int *label_dim,
+                        long long max_elems, CnArena &ar, std::vector<long long> &offs,
+                        CnStep &st, CnOperand &R, std::string &msg) {
+    const int ra = (int)A.lab.size(), rb = (int)B.lab.size();
+    std::vector<int> fo, sha, shb, to;  // positions
+    for (int i = 0; i < ra; ++i) {
+        int f = -1;
+        for (int j = 0; j < rb; ++j)
+            if (B.lab[j] == A.lab[i]) f = j;
+        if (f >= 0) {
+            sha.push_back(i);
+            shb.push_back(f);
+        } else {
+            fo.push_back(i);
+        }
+    }
+    for (int j = 0; j < rb; ++j)
+        if (std::find(shb.begin(), shb.end(), j) == shb.end()) to.push_back(j);
+    if (fo.size() + to.size() > 64) {
+        msg = "more than 64 open labels on an intermediate";
+        return 5;
+    }
+    std::vector<int> da(ra), db(rb);
+    for (int i = 0; i < ra; ++i) da[i] = label_dim[A.lab[i]];
+    for (int j = 0; j < rb; ++j) db[j] = label_dim[B.lab[j]];
+    long long M = 1, N = 1, K = 1;
+    for (int i : fo) M *= da[i];
+    for (int j : to) N *= db[j];
+    for (int i : sha) K *= da[i];
+    // the shared labels in b's order: (positions in a, positions in b)
+    std::vector<std::pair<int, int>> bsorted;
+    for (size_t s = 0; s < sha.size(); ++s) bsorted.push_back({shb[s], sha[s]});
+    std::sort(bsorted.begin(), bsorted.end());
+    std::vector<int> sha_b, shb_b;
+    for (auto &p : bsorted) {
+        shb_b.push_back(p.first);
+        sha_b.push_back(p.second);
+    }
+    // in-place layouts: 1 row-major [open, shared], 2 transposed [shared, open], 0 neither
+    auto layout = [](const std::vector<int> &open, const std::vector<int> &sh, int r) {
+        std::vector<int> x = open;
+        x.insert(x.end(), sh.begin(), sh.end());
+        bool ok = true;
+        for (int i = 0; i < r; ++i) ok = ok && x[i] == i;
+        if (ok) return 1;
+        x = sh;
+        x.insert(x.end(), open.begin(), open.end());
+        ok = true;
+        for (int i = 0; i < r; ++i) ok = ok && x[i] == i;
+        return ok ? 2 : 0;
+    };
+    int la = layout(fo, sha, ra), lb = layout(to, shb, rb);
+    if ((la == 0) + (lb == 0) > 0) {
+        const int la2 = layout(fo, sha_b, ra), lb2 = layout(to, shb_b, rb);
+        if ((la2 == 0) + (lb2 == 0) < (la == 0) + (lb == 0)) {
+            sha = sha_b;
+            shb = shb_b;
+            la = la2;
+            lb = lb2;
+        }
+    }
+    const double fM = (double)M, fN = (double)N, fK = (double)K, cap = (double)max_elems;
+    if (fM * fN > cap || (la == 0 && fM * fK > cap) || (lb == 0 && fK * fN > cap)) {
+        msg = "an intermediate exceeds max_elems (" + std::to_string(max_elems) + ")";
+        return 3;
+    }
+    std::vector<long long> sa(ra), sb(rb);
+    {
+        long long s = 1;
+        for (int i = ra - 1; i >= 0; --i) { sa[i] = s; s *= da[i]; }
+        s = 1;
+        for (int j = rb - 1; j >= 0; --j) { sb[j] = s; s *= db[j]; }
+    }
+    st.M = M;
+    st.N = N;
+    st.K = K;
+    // lay out the gathered operands, then the partials and the result
+    if (la == 0) {
+        st.ga = ar.alloc(M * K);
+        st.oar = (long long)offs.size();
+        cn_offsets(fo, sa, da, offs);
+        st.oac = (long long)offs.size();
+        cn_offsets(sha, sa, da, offs);
+        st.sam = K;
+        st.sak = 1;
+    } else if (la == 1) {
+        st.sam = K;
+        st.sak = 1;
+    } else {
+        st.sam = 1;
+        st.sak = M;
+    }
+    // B is the matrix [SH, TO]; layout() numbers b's layouts from the open group's side
+    if (lb == 0) {
+        st.gb = ar.alloc(K * N);
+        st.obr = (long long)offs.size();
+        cn_offsets(shb, sb, db, offs);
+        st.obc = (long long)offs.size();
+        cn_offsets(to, sb, db, offs);
+        st.sbk = N;
+        st.sbn = 1;
+    } else if (lb == 1) {  // stored [open, shared]
+        st.sbk = 1;
+        st.sbn = K;
+    } else {  // stored [shared, open]
+        st.sbk = N;
+        st.sbn = 1;
+    }
+    const long long tiles = ((M + CN_TM - 1) / CN_TM) * ((N + CN_TM - 1) / CN_TM);
+    if (tiles < 256 && K >= 1024) {
+        long long want = std::min<long long>((512 + tiles - 1) / tiles, K / 256);
+        want = std::max<long long>(want, 1);
+        long long kc = (K + want - 1) / want;
+        kc = (kc + CN_TK - 1) / CN_TK * CN_TK;
+        const int splits = (int)((K + kc - 1) / kc);
+        if (splits > 1 && (long long)splits * M * N <= max_elems) {
+            st.splits = splits;
+            st.kchunk = kc;
+            st.part = ar.alloc((long long)splits * M * N);
+        }
+    }
+    if (st.splits == 1) st.kchunk = K;
+    st.c = ar.alloc(M * N);
+    if (st.ga >= 0) ar.release(st.ga);
+    if (st.gb >= 0) ar.release(st.gb);
+    if (st.part >= 0) ar.release(st.part);
+    R.lab.clear();
+    for (int i : fo) R.lab.push_back(A.lab[i]);
+    for (int j : to) R.lab.push_back(B.lab[j]);
+    R.size = M * N;
+    R.arena = st.c;
+    R.live = true;
+    return 0;
+}
+
+// ---- contraction orders (host only: labels and dimensions, no data)
+
+// the pairwise result's labels (a's open ones, then b's), its size and the shared size
+static void cn_pair(const std::vector<int> &A, const std::vector<int> &B, const int *label_dim,
+                    std::vector<int> *R, double &rsize, double &ksize, bool &shared) {
+    rsize = ksize = 1.0;
+    shared = false;
+    if (R) R->clear();
+    for (int l : A) {
+        const bool in = std::find(B.begin(), B.end(), l) != B.end();
+        if (in) {
+            shared = true;
+            ksize *= label_dim[l];
+        } else {
+            rsize *= label_dim[l];
+            if (R) R->push_back(l);
+        }
+    }
+    for (int l : B)
+        if (std::find(A.begin(), A.end(), l) == A.end()) {
+            rsize *= label_dim[l];
+            if (R) R->push_back(l);
+        }
+}
+
+static double cn_size(const std::vector<int> &A, const int *label_dim) {
+    double n = 1.0;
+    for (int l : A) n *= label_dim[l];
+    return n;
+}
+
+enum { CN_COST_GROWTH = 0, CN_COST_SIZE = 1, CN_COST_RATIO = 2 };
+
+// Greedy tree: at each step the pair of live operands (ids in creation order, results
+// appended) sharing a label with the least cost -- growth size(r) - size(a) - size(b),
+// result size, or size(r) / max(size(a), size(b)) -- ties to the earlier pair; pairs that
+// share nothing only when no pair shares a label.
+static std::vector<std::pair<int, int>> cn_greedy(std::vector<std::vector<int>> labs,
+                                                  const int *label_dim, int kind) {
+    const int nt = (int)labs.size();
+    std::vector<std::pair<int, int>> pairs;
+    std::vector<char> live(nt, 1);
+    std::vector<double> size(nt);
+    for (int t = 0; t < nt; ++t) size[t] = cn_size(labs[t], label_dim);
+    for (int s = 0; s < nt - 1; ++s) {
+        int ba = -1, bb = -1;
+        double best = 0.0;
+        bool best_shared = false;
+        const int n = (int)labs.size();
+        for (int x = 0; x < n; ++x) {
+            if (!live[x]) continue;
+            for (int y = x + 1; y < n; ++y) {
+                if (!live[y]) continue;
+                double rs, ks;
+                bool sh;
+                cn_pair(labs[x], labs[y], label_dim, nullptr, rs, ks, sh);
+                const double cost = kind == CN_COST_GROWTH ? rs - size[x] - size[y]
+                                    : kind == CN_COST_SIZE ? rs
+                                                           : rs / std::max(size[x], size[y]);
+                if (ba < 0 || (sh && !best_shared) || (sh == best_shared && cost < best)) {
+                    best = cost;
+                    ba = x;
+                    bb = y;
+                    best_shared = sh;
+                }
+            }
+        }
+        std::vector<int> R;
+        double rs, ks;
+        bool sh;
+        cn_pair(labs[ba], labs[bb], label_dim, &R, rs, ks, sh);
+        live[ba] = live[bb] = 0;
+        labs.push_back(R);
+        live.push_back(1);
+        size.push_back(rs);
+        pairs.push_back({ba, bb});
+    }
+    return pairs;
+}
+
+// FLOPs (2 per multiply-add) and largest intermediate of a tree
+static void cn_tree_cost(std::vector<std::vector<int>> labs, const int *label_dim,
+                         const std::vector<std::pair<int, int>> &pairs, double &flops,
+                         double &peak) {
+    flops = peak = 0.0;
+    for (const auto &pr : pairs) {
+        std::vector<int> R;
+        double rs, ks;
+        bool sh;
+        cn_pair(labs[pr.first], labs[pr.second], label_dim, &R, rs, ks, sh);
+        flops += 2.0 * rs * ks;
+        peak = std::max(peak, rs);
+        labs.push_back(R);
+    }
+}
+
+// The tree of an order algorithm: left associative, greedy (growth cost), or the cheapest
+// in FLOPs (then in largest intermediate, then in this order) of the left-associative tree
+// and the three greedy trees.
+static std::vector<std::pair<int, int>> cn_order(const std::vector<std::vector<int>> &labs,
+                                                 const int *label_dim, int order_alg) {
+    const int nt = (int)labs.size();
+    std::vector<std::pair<int, int>> left;
+    if (nt >= 2) {
+        left.push_back({0, 1});
+        for (int t = 2; t < nt; ++t) left.push_back({nt + t - 2, t});
+    }
+    if (order_alg == CN_ORDER_LEFT || nt < 2) return left;
+    if (order_alg == CN_ORDER_GREEDY) return cn_greedy(labs, label_dim, CN_COST_GROWTH);
+    std::vector<std::vector<std::pair<int, int>>> cands = {
+        left, cn_greedy(labs, label_dim, CN_COST_GROWTH), cn_greedy(labs, label_dim, CN_COST_SIZE),
+        cn_greedy(labs, label_dim, CN_COST_RATIO)};
+    size_t best = 0;
+    double bf = 0.0, bp = 0.0;
+    for (size_t i = 0; i < cands.size(); ++i) {
+        double f, pk;
+        cn_tree_cost(labs, label_dim, cands[i], f, pk);
+        if (i == 0 || f < bf || (f == bf && pk < bp)) {
+            best = i;
+            bf = f;
+            bp = pk;
+        }
+    }
+    return cands[best];
+}
+
+// Builds the plan (host side).  Codes as the C ABI's: 0, 1 (argument), 4 (unsupported).
+static int cn_build_plan(int nt, const int *ranks, const int *labels, int n_labels,
+                         const int *label_dim, const int *order, int order_alg,
+                         long long max_elems, bp_cn_plan &p, std::string &msg) {
+    if (nt < 0 || (nt > 0 && (!ranks || !labels)) || n_labels < 0 ||
+        (n_labels > 0 && !label_dim)) {
+        msg = "invalid tensor or label arrays";
+        return 1;
+    }
+    if (max_elems <= 0) max_elems = 1LL << 31;
+    max_elems = std::min(max_elems, 1LL << 34);  // 128 GiB of FP64 per buffer
+    for (int l = 0; l < n_labels; ++l)
+        if (label_dim[l] <= 0) {
+            msg = "label " + std::to_string(l) + " has dimension " + std::to_string(label_dim[l]);
+            return 1;
+        }
+    std::vector<int> count(n_labels, 0);
+    std::vector<CnOperand> ops(std::max(nt, 0));
+    p.nt = nt;
+    p.in_off.assign(nt, 0);
+    p.in_size.assign(nt, 1);
+    long long lo = 0, off = 0;
+    for (int t = 0; t < nt; ++t) {
+        if (ranks[t] < 0 || ranks[t] > 64) {
+            msg = "tensor " + std::to_string(t) + " has rank " + std::to_string(ranks[t]);
+            return 1;
+        }
+        long long n = 1;
+        for (int j = 0; j < ranks[t]; ++j) {
+            const int l = labels[lo + j];
+            if (l < 0 || l >= n_labels) {
+                msg = "tensor " + std::to_string(t) + ": label " + std::to_string(l) +
+                      " out of range";
+                return 1;
+            }
+            for (int i = 0; i < j; ++i)
+                if (labels[lo + i] == l) {
+                    msg = "tensor " + std::to_string(t) + " repeats label " + std::to_string(l);
+                    return 1;
+                }
+            if (++count[l] > 2) {
+                msg = "label " + std::to_string(l) + " appears on more than two tensors";
+                return 1;
+            }
+            ops[t].lab.push_back(l);
+            n *= label_dim[l];
+            if (n > max_elems) {
+                msg = "tensor " + std::to_string(t) + " exceeds max_elems";
+                return 1;
+            }
+        }
+        ops[t].size = n;
+        ops[t].live = true;
+        p.in_off[t] = off;
+        p.in_size[t] = n;
+        off += n;
+        lo += ranks[t];
+    }
+    p.in_total = off;
+    // the order as (a, b) pairs
+    std::vector<std::pair<int, int>> pairs;
+    if (nt >= 2) {
+        if (order) {
+            std::vector<int> used(2 * nt - 1, 0);
+            for (int s = 0; s < nt - 1; ++s) {
+                const int a = order[2 * s], b = order[2 * s + 1];
+                if (a < 0 || b < 0 || a >= nt + s || b >= nt + s || a == b || used[a] ||
+                    used[b]) {
+                    msg = "order step " + std::to_string(s) + " (" + std::to_string(a) + ", " +
+                          std::to_string(b) + ") is not a valid contraction tree step";
+                    return 1;
+                }
+                used[a] = used[b] = 1;
+                pairs.push_back({a, b});
+            }
+        } else if (order_alg == CN_ORDER_LEFT || order_alg == CN_ORDER_GREEDY ||
+                   order_alg == CN_ORDER_CHEAPEST) {
+            std::vector<std::vector<int>> labs;
+            for (int t = 0; t < nt; ++t) labs.push_back(ops[t].lab);
+            pairs = cn_order(labs, label_dim, order_alg);
+        } else {
+            msg = "unknown order_alg " + std::to_string(order_alg);
+            return 1;
+        }
+    }
+    CnArena ar;
+    std::vector<long long> offs;
+    ops.reserve(2 * nt);
+    for (int s = 0; s < nt - 1; ++s) {
+        const int a = pairs[s].first, b = pairs[s].second;
+        CnStep st;
+        st.a = a;
+        st.b = b;
+        CnOperand R;
+        const int rc = cn_plan_step(ops[a], ops[b], label_dim, max_elems, ar, offs, st, R, msg);
+        if (rc == 3 || rc == 5) return 4;
+        if (ops[a].arena >= 0) ar.release(ops[a].arena);
+        if (ops[b].arena >= 0) ar.release(ops[b].arena);
+        ops[a].live = ops[b].live = false;
+        p.flops += 2.0 * (double)st.M * (double)st.N * (double)st.K;
+        p.max_interm = std::max(p.max_interm, st.M * st.N);
+        ops.push_back(R);
+        p.steps.push_back(st);
+    }
+    p.arena_elems = ar.high;
+    if (nt == 0) {
+        p.out_size = 1;
+    } else {
+        const CnOperand &res = nt == 1 ? ops[0] : ops.back();
+        p.out_labels = res.lab;
+        p.out_size = res.size;
+    }
+    p.h_offs.swap(offs);
+    return 0;
+}
+
+// Allocates the plan's device workspace and uploads its offset tables (current device).
+// Codes: 0, 2 (HIP), 5 (allocation).
+static int cn_alloc_plan(bp_cn_plan &p, std::string &msg) {
+    const std::vector<long long> &offs = p.h_offs;
+    if (p.arena_elems > 0 && hipMalloc((void **)&p.arena, sizeof(double) * p.arena_elems) !=
+                                 hipSuccess) {
+        msg = "cannot allocate the contraction workspace (" +
+              std::to_string(sizeof(double) * p.arena_elems) + " bytes)";
+        return 5;
+    }
+    if (!offs.empty()) {
+        if (hipMalloc((void **)&p.offs, sizeof(long long) * offs.size()) != hipSuccess) {
+            msg = "cannot allocate the offset tables";
+            return 5;
+        }
+        if (hipMemcpy(p.offs, offs.data(), sizeof(long long) * offs.size(),
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            msg = "offset table upload failed";
+            return 2;
+        }
+    }
+    const double one = 1.0;
+    if (hipMalloc((void **)&p.one, sizeof(double)) != hipSuccess ||
+        hipMemcpy(p.one, &one, sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        msg = "cannot allocate the unit scalar";
+        return 5;
+    }
+    return 0;
+}
+
+static void cn_free_plan(bp_cn_plan &p) {
+    if (p.arena) (void)hipFree(p.arena);
+    if (p.offs) (void)hipFree(p.offs);
+    if (p.staging) (void)hipFree(p.staging);
+    if (p.one) (void)hipFree(p.one);
+    p.arena = p.staging = p.one = nullptr;
+    p.offs = nullptr;
+}
+
+// device address of operand `id` given the inputs at `in`
+static inline const double *cn_operand(const bp_cn_plan &p, const double *in, int id) {
+    if (id < p.nt) return in + p.in_off[id];
+    return p.arena + p.steps[id - p.nt].c;
+}
+
+static inline unsigned cn_blocks(long long n) {
+    return (unsigned)std::max<long long>(1, std::min<long long>((n + 255) / 256, 16384));
+}
+
+// Enqueues the whole contraction on `st`; returns the device address of the result.
+static hipError_t cn_enqueue(const bp_cn_plan &p, const double *in, hipStream_t st,
+                             const double **res) {
+    for (const CnStep &s : p.steps) {
+        const double *A = cn_operand(p, in, s.a), *B = cn_operand(p, in, s.b);
+        if (s.ga >= 0) {
+            double *g = p.arena + s.ga;
+            hipLaunchKernelGGL(k_cn_gather, dim3(cn_blocks(s.M * s.K)), dim3(256), 0, st, A,
+                               (const long long *)(p.offs + s.oar),
+                               (const long long *)(p.offs + s.oac), s.M, s.K, g);
+            A = g;
+        }
+        if (s.gb >= 0) {
+            double *g = p.arena + s.gb;
+            hipLaunchKernelGGL(k_cn_gather, dim3(cn_blocks(s.K * s.N)), dim3(256), 0, st, B,
+                               (const long long *)(p.offs + s.obr),
+                               (const long long *)(p.offs + s.obc), s.K, s.N, g);
+            B = g;
+        }
+        double *C = p.arena + s.c;
+        double *dst = s.splits > 1 ? p.arena + s.part : C;
+        const long long tiles = ((s.M + CN_TM - 1) / CN_TM) * ((s.N + CN_TM - 1) / CN_TM);
+        const dim3 grid((unsigned)tiles, (unsigned)s.splits);
+        hipLaunchKernelGGL(k_cn_gemm, grid, dim3(256), 0, st, A, s.sam, s.sak, B, s.sbk, s.sbn,
+                           dst, s.M, s.N, s.K, s.kchunk);
+        if (s.splits > 1)
+            hipLaunchKernelGGL(k_cn_sum_parts, dim3(cn_blocks(s.M * s.N)), dim3(256), 0, st,
+                               (const double *)dst, s.splits, s.M * s.N, C);
+    }
+    if (p.nt == 0)
+        *res = p.one;
+    else if (p.nt == 1)
+        *res = in;
+    else
+        *res = p.arena + p.steps.back().c;
+    return hipGetLastError();
+}

@@ -59,6 +59,10 @@ class ITensor:
     def dimnames(self):
         return list(self.inds)
 
+    def __getitem__(self, key):
+        """Entries by position; `t[()]` is the value of a 0-dimensional tensor."""
+        return self.data[key]
+
     def permute(self, inds: Sequence[Index]) -> np.ndarray:
         perm = [self.inds.index(i) for i in inds]
         return np.transpose(self.data, perm)
