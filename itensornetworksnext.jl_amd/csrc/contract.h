@@ -133,13 +133,129 @@ __global__ void k_cn_sum_parts(const double *__restrict__ P, int nparts, long lo
     }
 }
 
+// ---- streaming products for skinny shapes (VALU FP64): one operand streamed through the
+// threads, the other (at most CN_LDS_ELEMS entries with its table) staged in LDS.  Operands
+// are read in place through offset tables; the streamed side's row offset is split into a
+// high and a low table, off(m) = hi[m / lo_n] + lo[m % lo_n].
+#define CN_LDS_ELEMS 8000
+
+// C[m, n] = sum_k A[ahi[m / mlo] + alo[m % mlo] + acol[k]] * B[brow[k] + bcol[n]], one
+// thread per row m, 16 outputs of the row per pass.  Output staging (`stage`): 1, the whole
+// 256-row tile of C (a contiguous block of 256 N entries) is assembled in LDS and written by
+// consecutive threads; 2, each pass's 256 x 16 slab is (16 consecutive entries per row);
+// 0, each thread writes its own row.
+__global__ __launch_bounds__(256) void k_cn_rows(
+    const double *__restrict__ A, const long long *__restrict__ ahi,
+    const long long *__restrict__ alo, unsigned mlo, const long long *__restrict__ acol,
+    const double *__restrict__ B, const long long *__restrict__ brow,
+    const long long *__restrict__ bcol, double *__restrict__ C, unsigned M, int N, int K,
+    int stage) {
+    extern __shared__ double cn_lds[];
+    const int N16 = (N + 15) & ~15;
+    const int W = stage == 1 ? N : 16;                    // staged entries per row
+    double *Bs = cn_lds;                                  // [K][N16], zero columns >= N
+    long long *ac = (long long *)(cn_lds + K * N16);      // [K]
+    double *Cs = cn_lds + K * (N16 + 1);                  // [256][W] when staging
+    for (int i = threadIdx.x; i < K * N16; i += 256) {
+        const int k = i / N16, n = i - k * N16;
+        Bs[i] = n < N ? B[brow[k] + bcol[n]] : 0.0;
+    }
+    for (int k = threadIdx.x; k < K; k += 256) ac[k] = acol[k];
+    __syncthreads();
+    for (unsigned m0 = blockIdx.x * 256u; m0 < M; m0 += gridDim.x * 256u) {
+        const unsigned m = m0 + threadIdx.x;
+        const bool live = m < M;
+        const unsigned hi = live ? m / mlo : 0u, lo = live ? m - hi * mlo : 0u;
+        const double *a = A + ahi[hi] + alo[lo];
+        const unsigned rows = min(256u, M - m0);
+        for (int n0 = 0; n0 < N; n0 += 16) {
+            double acc[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+            if (live) {
+                for (int k = 0; k < K; ++k) {
+                    const double x = a[ac[k]];
+                    const double *b = Bs + k * N16 + n0;
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) acc[j] = fma(x, b[j], acc[j]);
+                }
+                double *c = stage == 0   ? C + (long long)m * N + n0
+                            : stage == 1 ? Cs + threadIdx.x * W + n0
+                                         : Cs + threadIdx.x * W;
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    if (n0 + j < N) c[j] = acc[j];
+            }
+            if (stage == 2) {
+                __syncthreads();
+                for (unsigned i = threadIdx.x; i < rows * 16u; i += 256) {
+                    const unsigned r = i >> 4, j = i & 15u;
+                    if (n0 + (int)j < N) C[(long long)(m0 + r) * N + n0 + j] = Cs[i];
+                }
+                __syncthreads();
+            }
+        }
+        if (stage == 1) {
+            __syncthreads();
+            double *cg = C + (long long)m0 * N;
+            for (unsigned i = threadIdx.x; i < rows * (unsigned)N; i += 256) cg[i] = Cs[i];
+            __syncthreads();
+        }
+    }
+}
+
+// C[m, n] = sum_k A[arow[m] + acol[k]] * B[bhi[n / nlo] + blo[n % nlo] + brow[k]], one
+// thread per column n (consecutive threads write consecutive entries of a row of C), 16 rows
+// at a time
+__global__ __launch_bounds__(256) void k_cn_cols(
+    const double *__restrict__ A, const long long *__restrict__ arow,
+    const long long *__restrict__ acol, const double *__restrict__ B,
+    const long long *__restrict__ bhi, const long long *__restrict__ blo, unsigned nlo,
+    const long long *__restrict__ brow, double *__restrict__ C, int M, unsigned N, int K) {
+    extern __shared__ double cn_lds[];
+    const int M16 = (M + 15) & ~15;
+    double *As = cn_lds;                                  // [K][M16], zero rows >= M
+    long long *br = (long long *)(cn_lds + K * M16);      // [K]
+    for (int i = threadIdx.x; i < K * M16; i += 256) {
+        const int k = i / M16, m = i - k * M16;
+        As[i] = m < M ? A[arow[m] + acol[k]] : 0.0;
+    }
+    for (int k = threadIdx.x; k < K; k += 256) br[k] = brow[k];
+    __syncthreads();
+    for (unsigned n = blockIdx.x * 256u + threadIdx.x; n < N; n += gridDim.x * 256u) {
+        const unsigned hi = n / nlo, lo = n - hi * nlo;
+        const double *b = B + bhi[hi] + blo[lo];
+        for (int m0 = 0; m0 < M; m0 += 16) {
+            double acc[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+            for (int k = 0; k < K; ++k) {
+                const double x = b[br[k]];
+                const double *a = As + k * M16 + m0;
+#pragma unroll
+                for (int j = 0; j < 16; ++j) acc[j] = fma(a[j], x, acc[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (m0 + j < M) C[(long long)(m0 + j) * N + n] = acc[j];
+        }
+    }
+}
+
 // ------------------------------------------------------------------ host plan
 
 enum { CN_ORDER_LEFT = 0, CN_ORDER_GREEDY = 1, CN_ORDER_CHEAPEST = 2 };
 
+enum { CN_KIND_MFMA = 0, CN_KIND_ROWS = 1, CN_KIND_COLS = 2 };
+
 struct CnStep {
     int a = 0, b = 0;             // operand ids (< nt: input tensors, else nt + step)
     long long M = 1, N = 1, K = 1;
+    int kind = CN_KIND_MFMA;
+    // streaming kinds: offsets of the tables (rows: ahi, alo, acol, brow, bcol; cols: arow,
+    // acol, bhi, blo, brow) and the low table's length
+    long long t[5] = {0, 0, 0, 0, 0};
+    long long lo_n = 1;
     // operand A: gathered (ga >= 0, arena offset; tables at offsets oar / oac) or read in
     // place with strides (sam, sak); B likewise
     long long ga = -1, oar = 0, oac = 0, sam = 0, sak = 0;
@@ -296,6 +412,54 @@ static int cn_plan_step(const CnOperand &A, const CnOperand &B, const int *label
     st.M = M;
     st.N = N;
     st.K = K;
+    // streaming kernels when the smaller side, with the shared labels, fits in LDS
+    const long long N16 = (N + 15) / 16 * 16, M16 = (M + 15) / 16 * 16;
+    if (M >= N && K * (N16 + 1) <= CN_LDS_ELEMS && M < (1LL << 31)) st.kind = CN_KIND_ROWS;
+    else if (N > M && K * (M16 + 1) <= CN_LDS_ELEMS && N < (1LL << 31)) st.kind = CN_KIND_COLS;
+    if (st.kind != CN_KIND_MFMA) {
+        // split the streamed side's labels: the trailing ones (product >= 256, or all) index
+        // the low table
+        const std::vector<int> &sp = st.kind == CN_KIND_ROWS ? fo : to;
+        const std::vector<int> &sd = st.kind == CN_KIND_ROWS ? da : db;
+        const std::vector<long long> &ss = st.kind == CN_KIND_ROWS ? sa : sb;
+        size_t cut = sp.size();
+        long long lo_n = 1;
+        while (cut > 0 && lo_n < 256) lo_n *= sd[sp[--cut]];
+        const std::vector<int> hi(sp.begin(), sp.begin() + cut), lo(sp.begin() + cut, sp.end());
+        st.lo_n = lo_n;
+        if (st.kind == CN_KIND_ROWS) {
+            st.t[0] = (long long)offs.size();
+            cn_offsets(hi, ss, sd, offs);
+            st.t[1] = (long long)offs.size();
+            cn_offsets(lo, ss, sd, offs);
+            st.t[2] = (long long)offs.size();
+            cn_offsets(sha, sa, da, offs);
+            st.t[3] = (long long)offs.size();
+            cn_offsets(shb, sb, db, offs);
+            st.t[4] = (long long)offs.size();
+            cn_offsets(to, sb, db, offs);
+        } else {
+            st.t[0] = (long long)offs.size();
+            cn_offsets(fo, sa, da, offs);
+            st.t[1] = (long long)offs.size();
+            cn_offsets(sha, sa, da, offs);
+            st.t[2] = (long long)offs.size();
+            cn_offsets(hi, ss, sd, offs);
+            st.t[3] = (long long)offs.size();
+            cn_offsets(lo, ss, sd, offs);
+            st.t[4] = (long long)offs.size();
+            cn_offsets(shb, sb, db, offs);
+        }
+        st.kchunk = K;
+        st.c = ar.alloc(M * N);
+        R.lab.clear();
+        for (int i : fo) R.lab.push_back(A.lab[i]);
+        for (int j : to) R.lab.push_back(B.lab[j]);
+        R.size = M * N;
+        R.arena = st.c;
+        R.live = true;
+        return 0;
+    }
     // lay out the gathered operands, then the partials and the result
     if (la == 0) {
         st.ga = ar.alloc(M * K);
@@ -651,6 +815,37 @@ static hipError_t cn_enqueue(const bp_cn_plan &p, const double *in, hipStream_t 
                              const double **res) {
     for (const CnStep &s : p.steps) {
         const double *A = cn_operand(p, in, s.a), *B = cn_operand(p, in, s.b);
+        if (s.kind == CN_KIND_ROWS) {
+            const long long N16 = (s.N + 15) / 16 * 16;
+            const long long base = s.K * (N16 + 1);
+            const int stage = base + 256 * s.N <= CN_LDS_ELEMS ? 1
+                              : base + 256 * 16 <= CN_LDS_ELEMS ? 2
+                                                                 : 0;
+            const size_t lds = sizeof(double) *
+                               (size_t)(base + (stage == 1 ? 256 * s.N : stage == 2 ? 4096 : 0));
+            const unsigned grid = (unsigned)std::min<long long>((s.M + 255) / 256, 8192);
+            hipLaunchKernelGGL(k_cn_rows, dim3(grid), dim3(256), lds, st, A,
+                               (const long long *)(p.offs + s.t[0]),
+                               (const long long *)(p.offs + s.t[1]), (unsigned)s.lo_n,
+                               (const long long *)(p.offs + s.t[2]), B,
+                               (const long long *)(p.offs + s.t[3]),
+                               (const long long *)(p.offs + s.t[4]), p.arena + s.c,
+                               (unsigned)s.M, (int)s.N, (int)s.K, stage);
+            continue;
+        }
+        if (s.kind == CN_KIND_COLS) {
+            const long long M16 = (s.M + 15) / 16 * 16;
+            const size_t lds = sizeof(double) * (size_t)(s.K * (M16 + 1));
+            const unsigned grid = (unsigned)std::min<long long>((s.N + 255) / 256, 8192);
+            hipLaunchKernelGGL(k_cn_cols, dim3(grid), dim3(256), lds, st, A,
+                               (const long long *)(p.offs + s.t[0]),
+                               (const long long *)(p.offs + s.t[1]), B,
+                               (const long long *)(p.offs + s.t[2]),
+                               (const long long *)(p.offs + s.t[3]), (unsigned)s.lo_n,
+                               (const long long *)(p.offs + s.t[4]), p.arena + s.c, (int)s.M,
+                               (unsigned)s.N, (int)s.K);
+            continue;
+        }
         if (s.ga >= 0) {
             double *g = p.arena + s.ga;
             hipLaunchKernelGGL(k_cn_gather, dim3(cn_blocks(s.M * s.K)), dim3(256), 0, st, A,
