@@ -182,10 +182,11 @@ def norm_network_labels(pkg, t, s):
 
 def contract_timing(pkg, seed, cases=((6, 3), (5, 4)), cpu_case=(6, 3), reps=5):
     """Exact <psi|psi> of L x L PEPS norm networks (d=2) on the device (ContractionPlan with
-    the cheapest order, inputs resident in device memory, FP64 products on MFMA): device time
-    of the plan's launches (HIP events, the best of `reps` executions after one untimed) and
-    the algorithmic FP64 rate; the oracle's frontier contraction of `cpu_case` timed beside
-    it on one core, with the relative difference of the two values."""
+    the cheapest order, inputs resident in device memory): device time of the plan's launches
+    (HIP events, the best of `reps` executions after one untimed), the algorithmic bytes
+    (operands read once, results written once) per second against the HBM peak (the products
+    are skinny: HBM-bound) and the FP64 rate; the oracle's frontier contraction of `cpu_case`
+    timed beside it on one core, with the relative difference of the two values."""
     import time
 
     out = []
@@ -215,6 +216,9 @@ def contract_timing(pkg, seed, cases=((6, 3), (5, 4)), cpu_case=(6, 3), reps=5):
                "steps": len(tensors) - 1, "GFLOP": plan.flops / 1e9,
                "workspace_MB": plan.workspace_bytes / 1e6, "plan_s": t_plan,
                "device_ms_best": best, "device_ms_median": float(np.median(ms)),
+               "algorithmic_GB": plan.bytes / 1e9,
+               "achieved_TBps": plan.bytes / (best * 1e-3) / 1e12,
+               "frac_hbm_peak": plan.bytes / (best * 1e-3) / 8e12,
                "fp64_TFLOPs": plan.flops / (best * 1e-3) / 1e12,
                "frac_fp64_peak": plan.flops / (best * 1e-3) / 78.6e12,
                "host_buffers_call_s": t_host, "value": z, "host_value_equal": zh == z}

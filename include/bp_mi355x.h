@@ -269,7 +269,10 @@ int bp_last_run_ms(bp_context *ctx, double *ms);
  * any number of inputs of the same shapes: on host buffers (bp_cn_execute) or device
  * buffers of this library's HIP runtime (bp_cn_execute_device, result left in device
  * memory).  Both return after the result is written; bp_last_run_ms then gives the device
- * time of the contraction's launches.  bp_contract_network plans, executes and frees in
+ * time of the contraction's launches.  bp_cn_plan_info (any output may be NULL): the
+ * result's rank, labels and entries, the input entries, the products' FP64 FLOPs, their
+ * algorithmic bytes (every operand read once, every result written once) and the device
+ * workspace.  bp_contract_network plans, executes and frees in
  * one call (out_cap: capacity of `out` in entries, out_labels: room for 64 labels). */
 typedef struct bp_cn_plan bp_cn_plan;
 enum { BP_CN_ORDER_LEFT = 0, BP_CN_ORDER_GREEDY = 1, BP_CN_ORDER_CHEAPEST = 2 };
@@ -277,7 +280,7 @@ int bp_cn_plan_create(bp_context *ctx, int nt, const int *ranks, const int *labe
                       int n_labels, const int *label_dim, const int *order, int order_alg,
                       long long max_elems, bp_cn_plan **out);
 int bp_cn_plan_info(const bp_cn_plan *plan, int *out_rank, int *out_labels,
-                    long long *out_size, long long *in_size, double *flops,
+                    long long *out_size, long long *in_size, double *flops, double *bytes,
                     long long *workspace_bytes);
 int bp_cn_execute(bp_context *ctx, bp_cn_plan *plan, const double *data, double *out);
 int bp_cn_execute_device(bp_context *ctx, bp_cn_plan *plan, const void *data, void *out);

@@ -67,7 +67,7 @@ SIGNATURES = {
     "bp_cn_plan_create": (C.c_int, [_vp, C.c_int, _ip, _ip, C.c_int, _ip, _ip, C.c_int,
                                     C.c_longlong, C.POINTER(C.c_void_p)]),
     "bp_cn_plan_info": (C.c_int, [_vp, _ip, _ip, C.POINTER(C.c_longlong),
-                                  C.POINTER(C.c_longlong), _dp, C.POINTER(C.c_longlong)]),
+                                  C.POINTER(C.c_longlong), _dp, _dp, C.POINTER(C.c_longlong)]),
     "bp_cn_execute": (C.c_int, [_vp, _vp, _dp, _dp]),
     "bp_cn_execute_device": (C.c_int, [_vp, _vp, _vp, _vp]),
     "bp_cn_plan_destroy": (C.c_int, [_vp]),

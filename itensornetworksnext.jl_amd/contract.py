@@ -165,16 +165,17 @@ class ContractionPlan:
         orank = C.c_int()
         olab = np.zeros(64, dtype=np.int32)
         osize, isize, ws = C.c_longlong(), C.c_longlong(), C.c_longlong()
-        fl = C.c_double()
+        fl, by = C.c_double(), C.c_double()
         L.check(self.lib.bp_cn_plan_info(self.h, C.byref(orank), _ip(olab), C.byref(osize),
-                                         C.byref(isize), C.byref(fl), C.byref(ws)),
-                "bp_cn_plan_info")
+                                         C.byref(isize), C.byref(fl), C.byref(by),
+                                         C.byref(ws)), "bp_cn_plan_info")
         inv = {v: k for k, v in self.names.items()}
         self.out_labels = [inv[int(i)] for i in olab[:orank.value]]
         self.out_shape = tuple(int(dims[self.names[l]]) for l in self.out_labels)
         self.out_size = int(osize.value)
         self.in_size = int(isize.value)
         self.flops = float(fl.value)
+        self.bytes = float(by.value)  # algorithmic: operands read once, results written once
         self.workspace_bytes = int(ws.value)
 
     def _flat_input(self, tensors, flat) -> np.ndarray:

@@ -1892,7 +1892,8 @@ int bp_cn_plan_create(bp_context *c, int nt, const int *ranks, const int *labels
 }
 
 int bp_cn_plan_info(const bp_cn_plan *p, int *out_rank, int *out_labels, long long *out_size,
-                    long long *in_size, double *flops, long long *workspace_bytes) {
+                    long long *in_size, double *flops, double *bytes,
+                    long long *workspace_bytes) {
     if (!p) return set_err(BP_ERR_ARG, "plan is NULL");
     if (out_rank) *out_rank = (int)p->out_labels.size();
     if (out_labels)
@@ -1900,6 +1901,7 @@ int bp_cn_plan_info(const bp_cn_plan *p, int *out_rank, int *out_labels, long lo
     if (out_size) *out_size = p->out_size;
     if (in_size) *in_size = p->in_total;
     if (flops) *flops = p->flops;
+    if (bytes) *bytes = p->bytes;
     if (workspace_bytes) *workspace_bytes = (long long)sizeof(double) * p->arena_elems;
     return BP_OK;
 }
@@ -1918,7 +1920,7 @@ static int cn_run(bp_context *c, bp_cn_plan *p, const double *in, void *out,
                   hipMemcpyKind kind) {
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     const double *res = nullptr;
-    HIPCHK(cn_enqueue(*p, in, c->stream, &res));
+    HIPCHK(cn_launch(*p, in, c->stream, &res));
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     HIPCHK(hipMemcpyAsync(out, res, sizeof(double) * p->out_size, kind, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1964,7 +1966,8 @@ int bp_contract_network(bp_context *c, int nt, const int *ranks, const int *labe
         rc = set_err(BP_ERR_ARG, "result has %lld entries, out_cap is %lld", p->out_size,
                      out_cap);
     if (rc == BP_OK) rc = bp_cn_execute(c, p, data, out);
-    if (rc == BP_OK) bp_cn_plan_info(p, out_rank, out_labels, nullptr, nullptr, nullptr, nullptr);
+    if (rc == BP_OK)
+        bp_cn_plan_info(p, out_rank, out_labels, nullptr, nullptr, nullptr, nullptr, nullptr);
     bp_cn_plan_destroy(p);
     return rc;
 }

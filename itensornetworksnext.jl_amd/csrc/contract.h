@@ -21,7 +21,8 @@
 //    arena whose offsets the plan assigns first-fit, each block freed (for reuse by later
 //    launches on the same stream) as soon as the launch that reads it has been enqueued.
 //
-// The products run on v_mfma_f64_16x16x4f64 (FP64 keeps the result at the precision of a
+// The launch sequence of a plan is captured once as a HIP graph per input address and
+// replayed.  The products run on v_mfma_f64_16x16x4f64 (FP64 keeps the result at the precision of a
 // CPU FP64 contraction): 64 x 64 output tile per 256-thread workgroup, one 32 x 32 quarter
 // (2 x 2 MFMA blocks) per wave, K in steps of 16 through LDS.  Lane l supplies A[l%16][l/16]
 // and B[l/16][l%16] and receives D[l/16 + 4r][l%16] in item r (measured on gfx950; the
@@ -173,7 +174,20 @@ __global__ __launch_bounds__(256) void k_cn_rows(
 #pragma unroll
             for (int j = 0; j < 16; ++j) acc[j] = 0.0;
             if (live) {
-                for (int k = 0; k < K; ++k) {
+                // four loads in flight per thread before their products
+                int k = 0;
+                for (; k + 4 <= K; k += 4) {
+                    double x[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) x[u] = a[ac[k + u]];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const double *b = Bs + (k + u) * N16 + n0;
+#pragma unroll
+                        for (int j = 0; j < 16; ++j) acc[j] = fma(x[u], b[j], acc[j]);
+                    }
+                }
+                for (; k < K; ++k) {
                     const double x = a[ac[k]];
                     const double *b = Bs + k * N16 + n0;
 #pragma unroll
@@ -229,7 +243,19 @@ __global__ __launch_bounds__(256) void k_cn_cols(
             double acc[16];
 #pragma unroll
             for (int j = 0; j < 16; ++j) acc[j] = 0.0;
-            for (int k = 0; k < K; ++k) {
+            int k = 0;
+            for (; k + 4 <= K; k += 4) {
+                double x[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) x[u] = b[br[k + u]];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double *a = As + (k + u) * M16 + m0;
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) acc[j] = fma(a[j], x[u], acc[j]);
+                }
+            }
+            for (; k < K; ++k) {
                 const double x = b[br[k]];
                 const double *a = As + k * M16 + m0;
 #pragma unroll
@@ -274,12 +300,17 @@ struct bp_cn_plan {
     std::vector<int> out_labels;
     long long out_size = 1;
     double flops = 0.0;
+    double bytes = 0.0;  // every operand read once and every result written once (FP64)
     long long arena_elems = 0, max_interm = 0;
     double *arena = nullptr;
     long long *offs = nullptr;
     std::vector<long long> h_offs;  // offset tables (host copy)
     double *staging = nullptr;  // device copy of host inputs for the host-buffer execution
     double *one = nullptr;      // the scalar 1 (an empty network)
+    // the launch sequence captured as a graph, for one input address and stream
+    hipGraphExec_t gexec = nullptr;
+    const double *g_in = nullptr;
+    hipStream_t g_stream = nullptr;
 };
 
 // first-fit allocator over element offsets (host bookkeeping only)
@@ -338,7 +369,8 @@ static void cn_offsets(const std::vector<int> &pos, const std::vector<long long>
 // labels).
 static int cn_plan_step(const CnOperand &A, const CnOperand &B, const int *label_dim,
                         long long max_elems, CnArena &ar, std::vector<long long> &offs,
-                        CnStep &st, CnOperand &R, std::string &msg) {
+                        CnStep &st, CnOperand &R, std::string &msg,
+                        const std::vector<int> &next_shared) {
     const int ra = (int)A.lab.size(), rb = (int)B.lab.size();
     std::vector<int> fo, sha, shb, to;  // positions
     for (int i = 0; i < ra; ++i) {
@@ -417,6 +449,12 @@ static int cn_plan_step(const CnOperand &A, const CnOperand &B, const int *label
     if (M >= N && K * (N16 + 1) <= CN_LDS_ELEMS && M < (1LL << 31)) st.kind = CN_KIND_ROWS;
     else if (N > M && K * (M16 + 1) <= CN_LDS_ELEMS && N < (1LL << 31)) st.kind = CN_KIND_COLS;
     if (st.kind != CN_KIND_MFMA) {
+        // b's open labels that the result's consumer contracts go first (outermost): the
+        // consumer then streams this result with fewer shared labels among its innermost
+        std::stable_partition(to.begin(), to.end(), [&](int j) {
+            return std::find(next_shared.begin(), next_shared.end(), B.lab[j]) !=
+                   next_shared.end();
+        });
         // split the streamed side's labels: the trailing ones (product >= 256, or all) index
         // the low table
         const std::vector<int> &sp = st.kind == CN_KIND_ROWS ? fo : to;
@@ -730,6 +768,29 @@ static int cn_build_plan(int nt, const int *ranks, const int *labels, int n_labe
             return 1;
         }
     }
+    // the labels each intermediate shares with its consumer's other operand (label sets do
+    // not depend on the orders within operands).  A streaming step lists these first among
+    // b's open labels; since the consumer contracts exactly them, every label that stays open
+    // keeps its relative order, and the final result's labels are in the plain rule's order.
+    std::vector<std::vector<int>> canon;
+    for (int t = 0; t < nt; ++t) canon.push_back(ops[t].lab);
+    for (const auto &pr : pairs) {
+        std::vector<int> R;
+        double rs, ks;
+        bool sh;
+        cn_pair(canon[pr.first], canon[pr.second], label_dim, &R, rs, ks, sh);
+        canon.push_back(R);
+    }
+    std::vector<std::vector<int>> next_shared(pairs.size());
+    for (size_t s2 = 0; s2 < pairs.size(); ++s2)
+        for (int side = 0; side < 2; ++side) {
+            const int id = side ? pairs[s2].second : pairs[s2].first;
+            const int other = side ? pairs[s2].first : pairs[s2].second;
+            if (id < nt) continue;
+            for (int l : canon[id])
+                if (std::find(canon[other].begin(), canon[other].end(), l) != canon[other].end())
+                    next_shared[id - nt].push_back(l);
+        }
     CnArena ar;
     std::vector<long long> offs;
     ops.reserve(2 * nt);
@@ -739,12 +800,14 @@ static int cn_build_plan(int nt, const int *ranks, const int *labels, int n_labe
         st.a = a;
         st.b = b;
         CnOperand R;
-        const int rc = cn_plan_step(ops[a], ops[b], label_dim, max_elems, ar, offs, st, R, msg);
+        const int rc = cn_plan_step(ops[a], ops[b], label_dim, max_elems, ar, offs, st, R, msg,
+                                    next_shared[s]);
         if (rc == 3 || rc == 5) return 4;
         if (ops[a].arena >= 0) ar.release(ops[a].arena);
         if (ops[b].arena >= 0) ar.release(ops[b].arena);
         ops[a].live = ops[b].live = false;
         p.flops += 2.0 * (double)st.M * (double)st.N * (double)st.K;
+        p.bytes += 8.0 * ((double)ops[a].size + (double)ops[b].size + (double)st.M * st.N);
         p.max_interm = std::max(p.max_interm, st.M * st.N);
         ops.push_back(R);
         p.steps.push_back(st);
@@ -792,6 +855,8 @@ static int cn_alloc_plan(bp_cn_plan &p, std::string &msg) {
 }
 
 static void cn_free_plan(bp_cn_plan &p) {
+    if (p.gexec) (void)hipGraphExecDestroy(p.gexec);
+    p.gexec = nullptr;
     if (p.arena) (void)hipFree(p.arena);
     if (p.offs) (void)hipFree(p.offs);
     if (p.staging) (void)hipFree(p.staging);
@@ -808,6 +873,13 @@ static inline const double *cn_operand(const bp_cn_plan &p, const double *in, in
 
 static inline unsigned cn_blocks(long long n) {
     return (unsigned)std::max<long long>(1, std::min<long long>((n + 255) / 256, 16384));
+}
+
+// The device address of the result for inputs at `in`.
+static const double *cn_result(const bp_cn_plan &p, const double *in) {
+    if (p.nt == 0) return p.one;
+    if (p.nt == 1) return in;
+    return p.arena + p.steps.back().c;
 }
 
 // Enqueues the whole contraction on `st`; returns the device address of the result.
@@ -870,11 +942,34 @@ static hipError_t cn_enqueue(const bp_cn_plan &p, const double *in, hipStream_t 
             hipLaunchKernelGGL(k_cn_sum_parts, dim3(cn_blocks(s.M * s.N)), dim3(256), 0, st,
                                (const double *)dst, s.splits, s.M * s.N, C);
     }
-    if (p.nt == 0)
-        *res = p.one;
-    else if (p.nt == 1)
-        *res = in;
-    else
-        *res = p.arena + p.steps.back().c;
+    *res = cn_result(p, in);
     return hipGetLastError();
+}
+
+// Launches the contraction on `st`: the captured graph of the launch sequence, captured on
+// the first call for this input address and stream (plans of fewer than 4 launches are
+// enqueued directly).
+static hipError_t cn_launch(bp_cn_plan &p, const double *in, hipStream_t st,
+                            const double **res) {
+    if (p.steps.size() < 4) return cn_enqueue(p, in, st, res);
+    if (!p.gexec || p.g_in != in || p.g_stream != st) {
+        if (p.gexec) (void)hipGraphExecDestroy(p.gexec);
+        p.gexec = nullptr;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        e = cn_enqueue(p, in, st, res);
+        const hipError_t e2 = hipStreamEndCapture(st, &g);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&p.gexec, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            p.gexec = nullptr;
+            return e;
+        }
+        p.g_in = in;
+        p.g_stream = st;
+    }
+    *res = cn_result(p, in);
+    return hipGraphLaunch(p.gexec, st);
 }

@@ -15,6 +15,10 @@ Adds two strategies to the package's BP plugin surface
   * `device_beliefpropagation(nn, messages; stopping_criterion, alg)` — the same solve in one
     call with convergence tracked on the device (bp_run), returning a `MessageCache`.
 
+Also expectation values and local energies with their adjoint gradients, and the exact
+contraction `device_contract_network` (bp_contract_network, the counterpart of
+`contract_network(tn; alg = Exact())`).
+
 This file is not exercised in the build container (no Julia there); the same C ABI is
 exercised by the Python ctypes binding and its tests.
 """
@@ -272,6 +276,39 @@ function device_local_energy_and_grad(ctx::DeviceContext, nsite::Integer, O, A, 
             Ptr{Float64}, Cdouble, Cint, Ref{Cint}, Ref{Float64}), ctx.handle, ro, ra, rb,
         size(O, 1), E, g, adj_tol, adj_maxiter, it, res), "bp_local_energy")
     return E[], g
+end
+
+const BP_CN_ORDER = Dict(:left_associative => Cint(0), :greedy => Cint(1), :cheapest => Cint(2))
+
+"""Exact contraction on the device (`contract_network(tn; alg = Exact())`,
+src/contract_network.jl:32-37): `tensors[t]` carries the 0-based integer labels
+`labels[t]` (in Julia index order; a label on two tensors is summed over). Returns the
+result array and its open labels in result order. Julia arrays are column-major, so the
+labels go to the row-major C side reversed, and the result comes back reversed."""
+function device_contract_network(ctx::DeviceContext, tensors::AbstractVector,
+        labels::AbstractVector{<:AbstractVector{<:Integer}}; order_alg::Symbol = :cheapest)
+    nl = maximum(Iterators.flatten(labels); init = -1) + 1
+    dims = zeros(Cint, max(nl, 1))
+    for (t, l) in zip(tensors, labels), (d, x) in zip(size(t), l)
+        dims[x + 1] = d
+    end
+    ranks = Cint[length(l) for l in labels]
+    labs = Cint[x for l in labels for x in reverse(l)]
+    data = isempty(tensors) ? zeros(Float64, 1) :
+        reduce(vcat, [vec(Array{Float64}(t)) for t in tensors])
+    counts = zeros(Int, max(nl, 1))
+    foreach(x -> counts[x + 1] += 1, Iterators.flatten(labels))
+    cap = prod(Int[dims[i] for i in 1:nl if counts[i] == 1]; init = 1)
+    orank = Ref{Cint}(0)
+    olabs = zeros(Cint, 64)
+    out = zeros(Float64, cap)
+    GC.@preserve ranks labs dims data olabs out check(ccall((:bp_contract_network, libbp[]),
+        Cint, (Ptr{Cvoid}, Cint, Ptr{Cint}, Ptr{Cint}, Cint, Ptr{Cint}, Ptr{Float64},
+            Ptr{Cint}, Cint, Clonglong, Ref{Cint}, Ptr{Cint}, Ptr{Float64}, Clonglong),
+        ctx.handle, length(tensors), ranks, labs, nl, dims, data, C_NULL,
+        BP_CN_ORDER[order_alg], 0, orank, olabs, out, cap), "bp_contract_network")
+    ol = reverse(Int.(olabs[1:orank[]]))
+    return reshape(out, Tuple(Int(dims[x + 1]) for x in ol)), ol
 end
 
 """Autotune the kernel choices for this context's shapes; `mode` 0 applies the cache at
