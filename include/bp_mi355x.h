@@ -23,6 +23,8 @@
  *   bp_upload_sites          NormNetwork vertex data lazy(A)*lazy(conj(A')) src/normnetwork.jl:49-54
  *   bp_contract_network,     contract_network(tn; alg = Exact(; order, order_alg)) and
  *   bp_cn_*                  contraction_order src/contract_network.jl:6-60
+ *   bp_apply_gates           apply_operator / BPApplyGate / apply_gate_bp_nsite!
+ *                            src/apply/apply_operators.jl:181-283
  * The adjoint entry points (bp_grad_*) and the energy estimator have no reference
  * counterpart: the reference has no AD and no expectation-value API.
  *
@@ -289,6 +291,35 @@ int bp_contract_network(bp_context *ctx, int nt, const int *ranks, const int *la
                         int n_labels, const int *label_dim, const double *data,
                         const int *order, int order_alg, long long max_elems, int *out_rank,
                         int *out_labels, double *out, long long out_cap);
+
+/* ---------------------------------------------------------------- gate application
+ * BP-gauged gate application (BPApplyGate, apply_gate_bp_nsite!, src/apply/
+ * apply_operators.jl:181-283) for a batch of gates, one workgroup per gate, FP64.  Gates of
+ * one call act independently (no gate may touch a site tensor or message another gate of
+ * the same call changes); a sequence of overlapping gates is a sequence of calls.
+ *   desc: BP_GATE_DESC ints per gate: kind (1: one site, 2: two sites), d1, d2 (physical
+ *     dimensions), k1, k2 (port counts, <= 8), p1, p2 (the port of each site on the gate's
+ *     bond; -1 for one site), maxdim (<= 0: no rank cap), normalize (0/1), then the 8 port
+ *     dimensions of site 1 and the 8 of site 2 (unused entries 0).
+ *   in: per gate, packed in this order: site 1's tensor [d1][port 0]...[port k1-1]
+ *     (row-major), site 2's tensor (two sites), the incoming messages (D x D, [ket][bra]) on
+ *     every port of site 1 except p1 in port order (one site: on every port when
+ *     normalize, none otherwise), then site 2's, then the gate: two sites
+ *     [t1][t2][s1][s2] (out indices first), one site [t][s].
+ *   out: per gate, at the capacity bp_gate_sizes reports: site 1's new tensor (the same
+ *     layout with port p1's dimension replaced by chi, the first d1*P1*chi entries), site
+ *     2's new tensor at its capacity, then the chi singular values (sorted, descending;
+ *     normalised to unit norm when normalize).  The messages on the gate's bond become
+ *     diag(S) both ways.  One-site gates write the new tensor only (chi = 0).
+ *   chi: per gate, the kept rank: min(d1 min(P1, d1 Db), d2 min(P2, d2 Db)) capped by
+ *     maxdim, then singular values <= rtol * S_max dropped (rtol < 0: none), at least 1.
+ * The gauge of a message M is X = V sqrt(lam) from M = V diag(lam) V^T, its inverse the
+ * pseudo-inverse (lam <= 1e-13 max(lam) dropped); the site factorisations and the gate's
+ * SVD are one-sided Jacobi SVDs.  bp_last_run_ms gives the kernel's device time. */
+enum { BP_GATE_DESC = 25 };
+int bp_gate_sizes(int ngates, const int *desc, long long *in_size, long long *out_size);
+int bp_apply_gates(bp_context *ctx, int ngates, const int *desc, const double *in,
+                   double *out, int *chi, double rtol);
 
 #ifdef __cplusplus
 }

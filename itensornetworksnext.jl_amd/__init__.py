@@ -7,13 +7,15 @@ vertex contraction and adjoint in hand-written HIP kernels for gfx950 behind the
 library `csrc/libbpmi355x.so` (declared in include/bp_mi355x.h).  There is no CPU
 fallback: the compute entry points raise when the library or a GPU is missing.
 """
-from . import _lib, algorithms, build, contract, engine, graphs, itensors, peps  # noqa: F401
+from . import _lib, algorithms, apply, build, contract, engine, graphs, itensors, peps  # noqa: F401
 from .algorithms import (AbstractAlgorithm, ArgumentError, MethodError,  # noqa: F401
                          StopAfterIteration, StopWhenAny, StopWhenConverged, default_algorithm,
                          select_algorithm, select_beliefpropagation_stopping_criterion)
 from .beliefpropagation import (JacobiGPUUpdate, MessageUpdateAlgorithm,  # noqa: F401
                                 SimpleMessageUpdate, beliefpropagation,
                                 default_beliefpropagation_edges, message_update_)
+from .apply import (BPApplyGate, Operator, apply_operator, apply_operators,  # noqa: F401
+                    operator, truncrank, trunctol)
 from .contract import (Cheapest, ContractionPlan, Exact, Flat, Greedy,  # noqa: F401
                        LeftAssociative, contract_labeled, contract_network)
 from .engine import Engine, RunResult  # noqa: F401
@@ -30,7 +32,8 @@ from .messagecache import (MessageCache, bethe_free_energy, edge_scalar,  # noqa
                            similar_message_environment, vertex_scalar, vertex_scalars)
 
 __all__ = [
-    "AbstractAlgorithm", "ArgumentError", "Cheapest", "ContractionPlan", "Engine", "Exact", "Flat",
+    "AbstractAlgorithm", "ArgumentError", "BPApplyGate", "Cheapest", "Operator",
+    "apply_operator", "apply_operators", "operator", "truncrank", "trunctol", "ContractionPlan", "Engine", "Exact", "Flat",
     "Greedy", "LeftAssociative", "contract_labeled", "contract_network", "ITensor", "ITensorNetwork", "Index",
     "JacobiGPUUpdate", "MessageCache", "MessageUpdateAlgorithm", "MethodError", "NamedGraph",
     "NormNetwork", "RunResult", "SimpleMessageUpdate", "StopAfterIteration", "StopWhenAny",

@@ -73,8 +73,12 @@ SIGNATURES = {
     "bp_cn_plan_destroy": (C.c_int, [_vp]),
     "bp_contract_network": (C.c_int, [_vp, C.c_int, _ip, _ip, C.c_int, _ip, _dp, _ip, C.c_int,
                                       C.c_longlong, _ip, _ip, _dp, C.c_longlong]),
+    "bp_gate_sizes": (C.c_int, [C.c_int, _ip, C.POINTER(C.c_longlong),
+                                C.POINTER(C.c_longlong)]),
+    "bp_apply_gates": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _ip, C.c_double]),
 }
 BP_CN_ORDER_LEFT, BP_CN_ORDER_GREEDY, BP_CN_ORDER_CHEAPEST = 0, 1, 2
+BP_GATE_DESC = 25
 
 _lib = None
 

@@ -27,6 +27,7 @@
 #include "adjoint.h"
 #include "vertex_adjoint.h"
 #include "contract.h"
+#include "gate.h"
 
 // Process-wide plan epoch: moves on every device (re)allocation and every change of a
 // kernel choice, so that a cached sweep graph never replays stale pointers.
@@ -1970,6 +1971,90 @@ int bp_contract_network(bp_context *c, int nt, const int *ranks, const int *labe
         bp_cn_plan_info(p, out_rank, out_labels, nullptr, nullptr, nullptr, nullptr, nullptr);
     bp_cn_plan_destroy(p);
     return rc;
+}
+
+// ------------------------------------------------------------- gate application
+
+int bp_gate_sizes(int ngates, const int *desc, long long *in_size, long long *out_size) {
+    if (ngates < 0 || (ngates > 0 && !desc)) return set_err(BP_ERR_ARG, "invalid gate list");
+    for (int g = 0; g < ngates; ++g) {
+        GtLayout L;
+        if (gt_layout(desc + (long long)g * GD_N, L))
+            return set_err(BP_ERR_ARG, "gate %d: invalid descriptor", g);
+        if (in_size) in_size[g] = L.in_total;
+        if (out_size) out_size[g] = L.out_total;
+    }
+    return BP_OK;
+}
+
+int bp_apply_gates(bp_context *c, int ngates, const int *desc, const double *in, double *out,
+                   int *chi, double rtol) {
+    if (!c) return set_err(BP_ERR_ARG, "ctx is NULL");
+    if (ngates < 0 || (ngates > 0 && (!desc || !in || !out || !chi)))
+        return set_err(BP_ERR_ARG, "NULL argument");
+    if (ngates == 0) return BP_OK;
+    std::vector<long long> offs((size_t)ngates * GO_N);
+    long long tin = 0, tout = 0, tws = 0;
+    for (int g = 0; g < ngates; ++g) {
+        GtLayout L;
+        if (gt_layout(desc + (long long)g * GD_N, L))
+            return set_err(BP_ERR_ARG, "gate %d: invalid descriptor (ports <= %d, matching "
+                                       "shared bond, min(m1, m2) <= %d)", g, GT_KMAX, GT_QMAX);
+        offs[(size_t)g * GO_N + GO_IN] = tin;
+        offs[(size_t)g * GO_N + GO_OUT] = tout;
+        offs[(size_t)g * GO_N + GO_WS] = tws;
+        tin += L.in_total;
+        tout += L.out_total;
+        tws += (L.ws_total + 31) / 32 * 32;
+    }
+    HIPCHK(hipSetDevice(c->device));
+    int *d_desc = nullptr, *d_chi = nullptr;
+    long long *d_offs = nullptr;
+    double *d_in = nullptr, *d_out = nullptr, *d_ws = nullptr;
+    auto release = [&]() {
+        if (d_desc) (void)hipFree(d_desc);
+        if (d_chi) (void)hipFree(d_chi);
+        if (d_offs) (void)hipFree(d_offs);
+        if (d_in) (void)hipFree(d_in);
+        if (d_out) (void)hipFree(d_out);
+        if (d_ws) (void)hipFree(d_ws);
+    };
+    if (hipMalloc((void **)&d_desc, sizeof(int) * GD_N * (size_t)ngates) != hipSuccess ||
+        hipMalloc((void **)&d_chi, sizeof(int) * (size_t)ngates) != hipSuccess ||
+        hipMalloc((void **)&d_offs, sizeof(long long) * offs.size()) != hipSuccess ||
+        hipMalloc((void **)&d_in, sizeof(double) * std::max(tin, 1LL)) != hipSuccess ||
+        hipMalloc((void **)&d_out, sizeof(double) * std::max(tout, 1LL)) != hipSuccess ||
+        hipMalloc((void **)&d_ws, sizeof(double) * std::max(tws, 1LL)) != hipSuccess) {
+        release();
+        return set_err(BP_ERR_NOMEM, "gate workspace (%lld doubles) does not fit", tws);
+    }
+    hipError_t e = hipMemcpyAsync(d_desc, desc, sizeof(int) * GD_N * (size_t)ngates,
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_offs, offs.data(), sizeof(long long) * offs.size(),
+                           hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_in, in, sizeof(double) * tin, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_gate, dim3((unsigned)ngates), dim3(GT_NT), 0, c->stream,
+                           (const int *)d_desc, (const long long *)d_offs, (const double *)d_in,
+                           d_out, d_ws, d_chi, rtol);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, d_out, sizeof(double) * tout, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(chi, d_chi, sizeof(int) * (size_t)ngates, hipMemcpyDeviceToHost,
+                           c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    release();
+    if (e != hipSuccess) return set_err(BP_ERR_HIP, "gate application: %s", hipGetErrorString(e));
+    c->last_ms = ms;
+    return BP_OK;
 }
 
 }  // extern "C"
