@@ -236,6 +236,100 @@ def contract_timing(pkg, seed, cases=((6, 3), (5, 4)), cpu_case=(6, 3), reps=5):
     return out
 
 
+class _PackedGate:
+    """A gate already in the packed C-ABI form (descriptor and inputs)."""
+
+    def __init__(self, desc, data):
+        self.desc = desc
+        self.data = data
+
+
+def gate_layer(pkg, t, s, m, gate_mats, maxdim, pairs):
+    """Packed two-site gates on the bonds `pairs` (vertex-index pairs) of topology t with
+    site tensors s and messages m (flat, engine layout); the same inputs in the oracle's
+    form."""
+    packed, plain = [], []
+    for (u1, u2), G in zip(pairs, gate_mats):
+        desc = np.zeros(pkg._lib.BP_GATE_DESC, dtype=np.int32)
+        desc[0], desc[1], desc[2] = 2, t.phys_dim[u1], t.phys_dim[u2]
+        parts, tens, msgs, ports = [], [], [], []
+        for side, (u, w) in enumerate(((u1, u2), (u2, u1))):
+            qs = list(range(t.port_off[u], t.port_off[u + 1]))
+            dims = [int(t.edge_dim[t.port_edge[q]]) for q in qs]
+            p = [int(t.port_nbr[q]) for q in qs].index(w)
+            desc[3 + side], desc[5 + side] = len(qs), p
+            desc[9 + 8 * side: 9 + 8 * side + len(qs)] = dims
+            A = s[t.site_off[u]:t.site_off[u + 1]].reshape(t.site_shape(u))
+            tens.append(A)
+            mm = {}
+            for j, q in enumerate(qs):
+                if j != p:
+                    sl = int(t.port_rev[q])
+                    mm[j] = m[t.msg_off[sl]:t.msg_off[sl + 1]].reshape(dims[j], dims[j])
+            msgs.append(mm)
+            ports.append(p)
+        desc[7] = maxdim
+        parts = [tens[0].ravel(), tens[1].ravel()]
+        parts += [msgs[0][j].ravel() for j in sorted(msgs[0])]
+        parts += [msgs[1][j].ravel() for j in sorted(msgs[1])]
+        parts.append(G.ravel())
+        packed.append(_PackedGate(desc, np.concatenate(parts)))
+        plain.append((tens[0], tens[1], ports[0], ports[1], msgs[0], msgs[1], G))
+    return packed, plain
+
+
+def gate_timing(pkg, seed, L=32, D=8, reps=3, cpu_sample=16):
+    """A simple-update layer on the L x L PEPS (d=2, bond dimension D): random two-site gates
+    on the L * L/2 vertex-disjoint horizontal bonds (i, 2j-1)-(i, 2j), truncated to D, with
+    the converged BP messages of the state (FP32 vertex path, downloaded as FP64) as the
+    environments; one bp_apply_gates call per layer.  Device time of the gate kernel (HIP
+    events, best of `reps` after one untimed call) and the call's wall time including the
+    PCIe copies; the numpy oracle timed on `cpu_sample` gates of the same layer; singular
+    values of those gates against the oracle's."""
+    import time
+
+    t = pkg.build_topology(pkg.named_grid((L, L)), D, 2)
+    s = pkg.peps.random_sites(t, seed, scale=pkg.peps.peps_scale(2, D))
+    e = pkg.Engine(t, s, dtype="f32")
+    e.init_messages("identity")
+    r = e.run(maxiter=200, tol=1e-10)
+    m = e.messages()
+    e.close()
+    idx = {v: i for i, v in enumerate(t.vertices)}
+    pairs = [(idx[(i, j)], idx[(i, j + 1)]) for i in range(1, L + 1) for j in range(1, L, 2)]
+    rng = np.random.default_rng(seed)
+    mats = [rng.standard_normal((4, 4)) for _ in pairs]
+    packed, plain = gate_layer(pkg, t, s, m, mats, D, pairs)
+    res, _ = pkg.apply.apply_gates_device(packed, -1.0)
+    ms, walls = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        res, kms = pkg.apply.apply_gates_device(packed, -1.0)
+        walls.append(time.perf_counter() - t0)
+        ms.append(kms)
+    orc = __import__("__graft_entry__").import_oracle()
+    t1 = time.perf_counter()
+    errs = []
+    for pg, (A1, A2, p1, p2, m1, m2, G), (out, chi) in zip(packed[:cpu_sample],
+                                                          plain[:cpu_sample],
+                                                          res[:cpu_sample]):
+        _, _, So = orc.apply_gate2(A1, A2, p1, p2, m1, m2, G, maxdim=D)
+        caps = pkg.apply._out_caps(pg.desc)
+        Sd = out[caps[0] + caps[1]:caps[0] + caps[1] + chi]
+        errs.append(float(np.max(np.abs(Sd - So)) / So[0]))
+    t_cpu = (time.perf_counter() - t1) / cpu_sample
+    best = min(ms)
+    n = len(pairs)
+    return {"workload": f"simple-update layer, {L}x{L} PEPS d=2 D={D}: {n} two-site gates on "
+                        f"disjoint bonds, truncated to {D}, converged BP messages "
+                        f"({int(r.sweeps[0])} FP32 sweeps)",
+            "gates": n, "kernel_ms_best": best, "kernel_ms_median": float(np.median(ms)),
+            "gates_per_s": n / (best * 1e-3), "call_wall_s": float(min(walls)),
+            "cpu_oracle_s_per_gate": t_cpu, "cpu_oracle_gates_per_s": 1.0 / t_cpu,
+            "cpu_cores": 1, "cpu_sample_gates": cpu_sample,
+            "max_rel_diff_S_vs_oracle": max(errs)}
+
+
 def large_bond_timing(pkg, torch, seed, L=24, D=16):
     """The mixed-precision workload at large bond dimension: BF16 MFMA sweeps (FP32 sums,
     per-message max-norm scaling) to fidelity 1e-5, then 3 FP64 polishing sweeps; the
@@ -295,14 +389,14 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--only", choices=["adjoint", "large_bond", "uneven", "local_energy",
-                                       "contract"],
+                                       "contract", "gates"],
                     default=None,
                     help="adjoint: run only the forward + adjoint workload (L x L, D; three "
                          "energy + gradient calls); large_bond: only the 24x24 D=16 BF16 + "
                          "FP64-polish workload; uneven: only the 64-network batched "
                          "workload; local_energy: only the expectation-value / local-energy "
-                         "timings; contract: only the exact-contraction timings; print the "
-                         "timings, for profiling")
+                         "timings; contract: only the exact-contraction timings; gates: only "
+                         "the gate-layer timing; print the timings, for profiling")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, used to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -323,6 +417,10 @@ def main(argv=None):
         runs = [adjoint_timing(pkg, torch, args.L, args.D, args.dtype, args.seed)
                 for _ in range(3)]
         print(json.dumps({"adjoint_only": runs}), flush=True)
+        return 0
+    if args.only == "gates":
+        torch.cuda.init()
+        print(json.dumps({"gates_only": gate_timing(pkg, args.seed)}), flush=True)
         return 0
     if args.only == "contract":
         torch.cuda.init()
@@ -450,6 +548,7 @@ def main(argv=None):
         extras["large_bond_bf16"] = large_bond_timing(pkg, torch, args.seed)
         extras["local_energy"] = local_energy_timing(pkg, torch, args.seed)
         extras["contract_network"] = contract_timing(pkg, args.seed)
+        extras["gate_layer"] = gate_timing(pkg, args.seed)
 
     cpu = None
     if not args.no_cpu:

@@ -1994,7 +1994,7 @@ int bp_apply_gates(bp_context *c, int ngates, const int *desc, const double *in,
         return set_err(BP_ERR_ARG, "NULL argument");
     if (ngates == 0) return BP_OK;
     std::vector<long long> offs((size_t)ngates * GO_N);
-    long long tin = 0, tout = 0, tws = 0;
+    long long tin = 0, tout = 0, tws = 0, lneed = 0;
     for (int g = 0; g < ngates; ++g) {
         GtLayout L;
         if (gt_layout(desc + (long long)g * GD_N, L))
@@ -2006,7 +2006,11 @@ int bp_apply_gates(bp_context *c, int ngates, const int *desc, const double *in,
         tin += L.in_total;
         tout += L.out_total;
         tws += (L.ws_total + 31) / 32 * 32;
+        lneed = std::max(lneed, gt_lds_need(L));
     }
+    // Jacobi sweeps staged in LDS up to 120 KiB per workgroup (matrices beyond stay in HBM)
+    const long long lds_cap = std::min<long long>(lneed, 120 * 1024 / 8);
+    const size_t lds_bytes = sizeof(double) * (size_t)lds_cap;
     HIPCHK(hipSetDevice(c->device));
     int *d_desc = nullptr, *d_chi = nullptr;
     long long *d_offs = nullptr;
@@ -2035,11 +2039,14 @@ int bp_apply_gates(bp_context *c, int ngates, const int *desc, const double *in,
                            hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(d_in, in, sizeof(double) * tin, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_gate,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_gate, dim3((unsigned)ngates), dim3(GT_NT), 0, c->stream,
+        hipLaunchKernelGGL(k_gate, dim3((unsigned)ngates), dim3(GT_NT), lds_bytes, c->stream,
                            (const int *)d_desc, (const long long *)d_offs, (const double *)d_in,
-                           d_out, d_ws, d_chi, rtol);
+                           d_out, d_ws, d_chi, rtol, lds_cap);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);

@@ -3,7 +3,7 @@
 //
 // Two-site gate on the bond between sites 1 and 2 (Val(2), :229-283):
 //  1. every incoming message M on a port other than the shared one is factored
-//     M = V diag(lam) V^T (cyclic Jacobi, one thread per message); the gauge is
+//     M = V diag(lam) V^T (parallel Jacobi over all the gate's messages); the gauge is
 //     X = V diag(sqrt(lam)) (X X^T = M) and its pseudo-inverse Xi = diag(1/sqrt(lam)) V^T
 //     (lam <= 1e-13 max(lam) treated as zero) -- the roles of gram_eigh_full_with_pinv;
 //  2. each site tensor absorbs X on those ports and is laid out as the matrix
@@ -60,6 +60,7 @@ struct GtLayout {
     long long out1, out2;        // new site tensor capacities (chimax on the shared port)
     // workspace offsets
     long long msg[2][GT_KMAX];   // eigen blocks per other port: W, V, X, Xi (D^2 each), lam
+    long long rot;               // rotation table of the message eigensolves (2 per pair)
     long long bufA[2], bufB[2], jac[2], jv[2], Q[2], R[2], ssig[2];
     long long th, thp, tv, sig, perm, raw, ws_total;
     int nmsg1, nmsg2;            // messages per site (other ports; one-site: all ports)
@@ -97,10 +98,13 @@ GT_FN int gt_layout(const int *dsc, GtLayout &L) {
         for (int j = 0; j < L.k1; ++j) L.size1 *= L.dims1[j];
         L.nmsg1 = L.norm ? L.k1 : 0;
         L.nmsg2 = 0;
+        long long npair = 0;
         for (int j = 0; j < L.nmsg1; ++j) {
             const long long D = L.dims1[j];
             L.msg[0][j] = take(4 * D * D + D);
+            npair += (D + 1) / 2;
         }
+        L.rot = take(2 * npair + 2);
         L.bufA[0] = take(L.size1);
         L.bufB[0] = take(L.size1);
         L.out1 = L.size1;
@@ -139,14 +143,17 @@ GT_FN int gt_layout(const int *dsc, GtLayout &L) {
     L.out2 = L.P2 * L.d2 * L.chimax;
     L.nmsg1 = L.k1 - 1;
     L.nmsg2 = L.k2 - 1;
+    long long npair = 0;
     for (int s = 0; s < 2; ++s) {
         const int k = s ? L.k2 : L.k1, p = s ? L.p2 : L.p1;
         const int *dims = s ? L.dims2 : L.dims1;
         for (int j = 0; j < k; ++j) {
             const long long D = dims[j];
             L.msg[s][j] = j == p ? -1 : take(4 * D * D + D);
+            if (j != p) npair += (D + 1) / 2;
         }
     }
+    L.rot = take(2 * npair + 2);
     for (int s = 0; s < 2; ++s) {
         const long long P = s ? L.P2 : L.P1;
         const long long n = s ? L.n2 : L.n1;
@@ -178,13 +185,33 @@ GT_FN int gt_layout(const int *dsc, GtLayout &L) {
     return 0;
 }
 
+// doubles of LDS that stage every phase of the gate: the message eigensolves, the Jacobi
+// sweeps, the rebuild's factors
+GT_FN long long gt_lds_need(const GtLayout &L) {
+    long long g = 0;  // the message eigensolves: W and V of every message
+    for (int j = 0; j < L.k1; ++j)
+        if (L.kind == 1 ? L.norm : j != L.p1) g += 2LL * L.dims1[j] * L.dims1[j];
+    if (L.kind != 2) return g;
+    for (int j = 0; j < L.k2; ++j)
+        if (j != L.p2) g += 2LL * L.dims2[j] * L.dims2[j];
+    const long long s1 = L.P1 * L.n1 + (long long)L.r1 * L.r1;
+    const long long s2 = L.P2 * L.n2 + (long long)L.r2 * L.r2;
+    const long long t = (long long)L.m1 * L.m2 + (long long)L.q * L.q;
+    const long long b1 = L.P1 * L.r1 + (long long)L.r1 * L.d1 * L.chimax;
+    const long long b2 = L.P2 * L.r2 + (long long)L.r2 * L.d2 * L.chimax;
+    return gt_max(gt_max(gt_max(s1, s2), gt_max(t, g)), gt_max(b1, b2));
+}
+
 // ------------------------------------------------------------------ block helpers
 
 struct GtDev {
     int tid, nt;
     double *red;   // [nt] reduction scratch
     int *flag;     // one int
+    double *lds;   // dynamic LDS for staged Jacobi sweeps
+    long long lds_cap;
     __device__ void sync() const { __syncthreads(); }
+    __device__ void mark(int) const {}  // phase boundary (a hook for instrumented contexts)
     __device__ double group_sum(double v, int gs) const {
         for (int o = gs >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, gs);
         return v;
@@ -202,62 +229,154 @@ GT_FN double gt_block_sum(const Ctx &c, double v) {
     return s;
 }
 
-// serial cyclic Jacobi: W (D x D, symmetric, row-major) -> V diag(lam) V^T
-GT_FN void gt_eigh_serial(double *W, double *V, double *lam, int D) {
-    for (int i = 0; i < D; ++i)
-        for (int j = 0; j < D; ++j) V[i * D + j] = i == j ? 1.0 : 0.0;
-    for (int sweep = 0; sweep < 60; ++sweep) {
-        double off = 0.0, dia = 0.0;
-        for (int p = 0; p < D; ++p) {
-            dia += W[p * D + p] * W[p * D + p];
-            for (int q = p + 1; q < D; ++q) off += W[p * D + q] * W[p * D + q];
+// Gauges of a gate's messages, all at once: for message i (dimension D[i], input M[i],
+// block E[i] = W, V, X, Xi (D^2 each), lam (D)) the symmetric part W = (M + M^T) / 2 is
+// diagonalised W = V diag(lam) V^T by parallel two-sided Jacobi: every round applies the
+// disjoint plane rotations of a round-robin pairing of every message together (angles,
+// then the column pass W J and V J, then the row pass J^T W), until no off-diagonal
+// element is above 1e-15 of its diagonal pair.  Then X = V sqrt(lam) (X X^T = W) and
+// Xi = pinv(X), lam <= 1e-13 max(lam) dropped.  `rot` holds (c, s) per (message, pair).
+template <class Ctx>
+GT_FN void gt_gauges(const Ctx &c, int nm, const double *const *M, double *const *E,
+                     const int *Ds, double *rot) {
+    int dmax = 0;
+    for (int i = 0; i < nm; ++i) dmax = Ds[i] > dmax ? Ds[i] : dmax;
+    long long tot = 0;
+    for (int i = 0; i < nm; ++i) tot += (long long)Ds[i] * Ds[i];
+    // W and V of every message: in LDS when they all fit, else in the E blocks
+    double *Wp[2 * GT_KMAX], *Vp[2 * GT_KMAX];
+    {
+        const bool staged = 2 * tot <= c.lds_cap;
+        long long off = 0;
+        for (int i = 0; i < nm; ++i) {
+            const long long DD = (long long)Ds[i] * Ds[i];
+            Wp[i] = staged ? c.lds + off : E[i];
+            Vp[i] = staged ? c.lds + off + DD : E[i] + DD;
+            off += 2 * DD;
         }
-        if (off <= 1e-30 * (dia + off) || off == 0.0) break;
-        for (int p = 0; p < D; ++p)
-            for (int q = p + 1; q < D; ++q) {
-                const double apq = W[p * D + q];
-                if (apq == 0.0) continue;
-                const double th = (W[q * D + q] - W[p * D + p]) / (2.0 * apq);
-                const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-                const double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
-                for (int k = 0; k < D; ++k) {
-                    const double wp = W[k * D + p], wq = W[k * D + q];
-                    W[k * D + p] = cs * wp - sn * wq;
-                    W[k * D + q] = sn * wp + cs * wq;
+    }
+    // W = sym(M), V = I
+    for (long long t = c.tid; t < tot; t += c.nt) {
+        long long rem = t;
+        int i = 0;
+        while (rem >= (long long)Ds[i] * Ds[i]) rem -= (long long)Ds[i] * Ds[i], ++i;
+        const int D = Ds[i], a = (int)(rem / D), b = (int)(rem % D);
+        Wp[i][a * D + b] = 0.5 * (M[i][a * D + b] + M[i][b * D + a]);
+        Vp[i][a * D + b] = a == b ? 1.0 : 0.0;
+    }
+    c.sync();
+    const int ne = dmax + (dmax & 1);
+    int npair = 0;
+    for (int i = 0; i < nm; ++i) npair += (Ds[i] + 1) / 2;
+    for (int sweep = 0; sweep < 60 && ne > 1; ++sweep) {
+        if (c.tid == 0) *c.flag = 0;
+        c.sync();
+        for (int r = 0; r < ne - 1; ++r) {
+            // angles, one task per (message, pair)
+            for (int t = c.tid; t < npair; t += c.nt) {
+                int i = 0, k = t;
+                while (k >= (Ds[i] + 1) / 2) k -= (Ds[i] + 1) / 2, ++i;
+                const int D = Ds[i], nei = D + (D & 1);
+                double cs = 1.0, sn = 0.0;
+                if (r < nei - 1) {
+                    int p, q;
+                    if (k == 0) {
+                        p = r;
+                        q = nei - 1;
+                    } else {
+                        p = (r + k) % (nei - 1);
+                        q = (r - k + nei - 1) % (nei - 1);
+                    }
+                    if (p > q) {
+                        const int x = p;
+                        p = q;
+                        q = x;
+                    }
+                    const double *W = Wp[i];
+                    if (q < D) {
+                        const double apq = W[p * D + q], app = W[p * D + p], aqq = W[q * D + q];
+                        if (apq != 0.0 && fabs(apq) > 1e-15 * sqrt(fabs(app * aqq))) {
+                            const double th = (aqq - app) / (2.0 * apq);
+                            const double tt =
+                                (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                            cs = 1.0 / sqrt(tt * tt + 1.0);
+                            sn = tt * cs;
+                            if (fabs(apq) > 1e-13 * sqrt(fabs(app * aqq))) *c.flag = 1;
+                        }
+                    }
                 }
-                for (int k = 0; k < D; ++k) {
-                    const double wp = W[p * D + k], wq = W[q * D + k];
-                    W[p * D + k] = cs * wp - sn * wq;
-                    W[q * D + k] = sn * wp + cs * wq;
-                }
-                for (int k = 0; k < D; ++k) {
-                    const double vp = V[k * D + p], vq = V[k * D + q];
-                    V[k * D + p] = cs * vp - sn * vq;
-                    V[k * D + q] = sn * vp + cs * vq;
-                }
+                rot[2 * t] = cs;
+                rot[2 * t + 1] = sn;
             }
-    }
-    for (int i = 0; i < D; ++i) lam[i] = W[i * D + i];
-}
-
-// gauge of one message block at `e` (W, V, X, Xi, lam): M (D x D, [ket][bra]) ->
-// X = V sqrt(lam), Xi = pinv(X)
-GT_FN void gt_gauge_serial(const double *M, double *e, int D) {
-    double *W = e, *V = e + D * D, *X = e + 2 * D * D, *Xi = e + 3 * D * D,
-           *lam = e + 4 * D * D;
-    for (int i = 0; i < D; ++i)
-        for (int j = 0; j < D; ++j) W[i * D + j] = 0.5 * (M[i * D + j] + M[j * D + i]);
-    gt_eigh_serial(W, V, lam, D);
-    double lmax = 0.0;
-    for (int i = 0; i < D; ++i) lmax = lam[i] > lmax ? lam[i] : lmax;
-    for (int x = 0; x < D; ++x) {
-        const double l = lam[x] > 1e-13 * lmax ? lam[x] : 0.0;
-        const double sq = sqrt(l), inv = l > 0.0 ? 1.0 / sq : 0.0;
-        for (int a = 0; a < D; ++a) {
-            X[a * D + x] = V[a * D + x] * sq;
-            Xi[x * D + a] = V[a * D + x] * inv;
+            c.sync();
+            // column pass: W <- W J, V <- V J (one task per message, pair, row)
+            for (int pass = 0; pass < 2; ++pass) {
+                long long ntask = 0;
+                for (int i = 0; i < nm; ++i) ntask += (long long)((Ds[i] + 1) / 2) * Ds[i];
+                for (long long t = c.tid; t < ntask; t += c.nt) {
+                    long long rem = t;
+                    int i = 0, base = 0;
+                    while (rem >= (long long)((Ds[i] + 1) / 2) * Ds[i]) {
+                        rem -= (long long)((Ds[i] + 1) / 2) * Ds[i];
+                        base += (Ds[i] + 1) / 2;
+                        ++i;
+                    }
+                    const int D = Ds[i], nei = D + (D & 1);
+                    const int k = (int)(rem / D), x = (int)(rem % D);
+                    if (r >= nei - 1) continue;
+                    int p, q;
+                    if (k == 0) {
+                        p = r;
+                        q = nei - 1;
+                    } else {
+                        p = (r + k) % (nei - 1);
+                        q = (r - k + nei - 1) % (nei - 1);
+                    }
+                    if (p > q) {
+                        const int y = p;
+                        p = q;
+                        q = y;
+                    }
+                    if (q >= D) continue;
+                    const double cs = rot[2 * (base + k)], sn = rot[2 * (base + k) + 1];
+                    if (sn == 0.0) continue;
+                    double *W = Wp[i], *V = Vp[i];
+                    if (pass == 0) {  // columns p, q of W and V, row x
+                        const double wp = W[x * D + p], wq = W[x * D + q];
+                        W[x * D + p] = cs * wp - sn * wq;
+                        W[x * D + q] = sn * wp + cs * wq;
+                        const double vp = V[x * D + p], vq = V[x * D + q];
+                        V[x * D + p] = cs * vp - sn * vq;
+                        V[x * D + q] = sn * vp + cs * vq;
+                    } else {  // rows p, q of W, column x
+                        const double wp = W[p * D + x], wq = W[q * D + x];
+                        W[p * D + x] = cs * wp - sn * wq;
+                        W[q * D + x] = sn * wp + cs * wq;
+                    }
+                }
+                c.sync();
+            }
         }
+        const int more = *c.flag;
+        c.sync();
+        if (!more) break;
     }
+    // lam, X, Xi
+    for (long long t = c.tid; t < tot; t += c.nt) {
+        long long rem = t;
+        int i = 0;
+        while (rem >= (long long)Ds[i] * Ds[i]) rem -= (long long)Ds[i] * Ds[i], ++i;
+        const int D = Ds[i], a = (int)(rem / D), x = (int)(rem % D);
+        const double *W = Wp[i], *V = Vp[i];
+        double lmax = 0.0;
+        for (int y = 0; y < D; ++y) lmax = W[y * D + y] > lmax ? W[y * D + y] : lmax;
+        const double l = W[x * D + x] > 1e-13 * lmax ? W[x * D + x] : 0.0;
+        const double sq = sqrt(l), inv = l > 0.0 ? 1.0 / sq : 0.0;
+        E[i][2 * D * D + a * D + x] = V[a * D + x] * sq;
+        E[i][3 * D * D + x * D + a] = V[a * D + x] * inv;
+        if (a == 0) E[i][4 * D * D + x] = W[x * D + x];
+    }
+    c.sync();
 }
 
 // out = in x_j Mx (mode product on port j of a [d0][dims...] tensor):
@@ -273,6 +392,7 @@ GT_FN void gt_mode(const Ctx &c, const double *in, double *out, long long size, 
         const long long i = idx % inner, b = (idx / inner) % D, o = idx / (inner * D);
         const double *src = in + o * inner * D + i;
         double s = 0.0;
+#pragma unroll 4
         for (int a = 0; a < D; ++a) s += src[a * inner] * Mx[a * D + b];
         out[idx] = s;
     }
@@ -315,6 +435,7 @@ GT_FN void gt_jacobi(const Ctx &c, double *A, int rows, int cols, double *V) {
                 double al = 0.0, be = 0.0, ga = 0.0;
                 double *ai = A + (long long)i * rows, *aj = A + (long long)jj * rows;
                 if (real)
+#pragma unroll 4
                     for (int x = l; x < rows; x += gs) {
                         const double u = ai[x], v = aj[x];
                         al += u * u;
@@ -328,6 +449,7 @@ GT_FN void gt_jacobi(const Ctx &c, double *A, int rows, int cols, double *V) {
                     const double ze = (be - al) / (2.0 * ga);
                     const double t = (ze >= 0.0 ? 1.0 : -1.0) / (fabs(ze) + sqrt(1.0 + ze * ze));
                     const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+#pragma unroll 4
                     for (int x = l; x < rows; x += gs) {
                         const double u = ai[x], v = aj[x];
                         ai[x] = cs * u - sn * v;
@@ -348,6 +470,25 @@ GT_FN void gt_jacobi(const Ctx &c, double *A, int rows, int cols, double *V) {
         c.sync();
         if (!more) break;
     }
+}
+
+// gt_jacobi in LDS when the matrix and V fit in the context's `lds` (lds_cap doubles):
+// copied in, rotated there (LDS latency instead of a global round trip per step), copied
+// back; in global memory otherwise.
+template <class Ctx>
+GT_FN void gt_jacobi_staged(const Ctx &c, double *A, int rows, int cols, double *V) {
+    const long long na = (long long)rows * cols, nv = (long long)cols * cols;
+    if (na + nv > c.lds_cap) {
+        gt_jacobi(c, A, rows, cols, V);
+        return;
+    }
+    double *la = c.lds, *lv = c.lds + na;
+    for (long long i = c.tid; i < na; i += c.nt) la[i] = A[i];
+    c.sync();
+    gt_jacobi(c, la, rows, cols, lv);
+    for (long long i = c.tid; i < na; i += c.nt) A[i] = la[i];
+    for (long long i = c.tid; i < nv; i += c.nt) V[i] = lv[i];
+    c.sync();
 }
 
 // ------------------------------------------------------------------ the pipeline
@@ -398,15 +539,16 @@ GT_FN void gt_site_qr(const Ctx &c, const GtLayout &L, int s, const double *A, d
     c.sync();
     double *JV = ws + L.jv[s];
     if (tall)
-        gt_jacobi(c, J, (int)P, n, JV);
+        gt_jacobi_staged(c, J, (int)P, n, JV);
     else
-        gt_jacobi(c, J, n, (int)P, JV);
+        gt_jacobi_staged(c, J, n, (int)P, JV);
     // Q and R
     double *Q = ws + L.Q[s], *R = ws + L.R[s];
     if (tall) {
         // columns of J: U S (P x n); V = JV (n x n)
         for (int col = c.tid; col < n; col += c.nt) {
             double s2 = 0.0;
+#pragma unroll 8
             for (long long x = 0; x < P; ++x) s2 += J[col * P + x] * J[col * P + x];
             ws[L.ssig[s] + col] = sqrt(s2);
         }
@@ -455,8 +597,32 @@ GT_FN void gt_site_rebuild(const Ctx &c, const GtLayout &L, int s, int chi, doub
     const bool tall = m1 >= m2;
     const double *J = ws + L.thp, *TV = ws + L.tv, *sig = ws + L.sig, *perm = ws + L.perm;
     double *cur = ws + L.bufA[s], *nxt = ws + L.bufB[s];
-    // left factor (site 1) rows (i1, t1): U[:, c] = tall ? J col / sigma : TV col;
-    // right factor (site 2) rows (i2, t2): V[:, c] = tall ? TV col : J col / sigma
+    // F[(i, t), c] = the kept singular vector entry times sqrt(S_c): site 1's left vectors
+    // U[:, c] (tall: Jacobi column / sigma, wide: V column), site 2's right vectors;
+    // F and Q in LDS when they fit
+    const long long nf = (long long)r * d * chi;
+    const bool staged = nf + P * r <= c.lds_cap;
+    double *F = staged ? c.lds : nxt;
+    for (long long idx = c.tid; idx < nf; idx += c.nt) {
+        const int cc = (int)(idx % chi);
+        const long long row = idx / chi;  // i * d + t
+        const int pc = (int)perm[cc];
+        const double rs = ws[L.raw + pc];
+        const double inv = rs > 0.0 ? 1.0 / rs : 0.0;  // raw sigma of the Jacobi column
+        double f;
+        if (s == 0)
+            f = tall ? J[(long long)pc * m1 + row] * inv : TV[(long long)pc * m1 + row];
+        else
+            f = tall ? TV[(long long)pc * m2 + row] : J[(long long)pc * m2 + row] * inv;
+        F[idx] = f * sqrt(sig[cc]);  // sig holds the kept, sorted, scaled values
+    }
+    const double *Qs = Q;
+    if (staged) {
+        double *lq = c.lds + nf;
+        for (long long idx = c.tid; idx < P * r; idx += c.nt) lq[idx] = Q[idx];
+        Qs = lq;
+    }
+    c.sync();
     for (long long idx = c.tid; idx < size; idx += c.nt) {
         long long rem = idx, x = 0, xs = 1;
         int cc = 0;
@@ -471,21 +637,10 @@ GT_FN void gt_site_rebuild(const Ctx &c, const GtLayout &L, int s, int chi, doub
             }
         }
         const int t = (int)rem;
-        const int pc = (int)perm[cc];
-        const double sq = sqrt(sig[cc]);  // sig holds the kept, sorted, scaled values
         double acc = 0.0;
-        const double rs = ws[L.raw + pc];
-        const double inv = rs > 0.0 ? 1.0 / rs : 0.0;  // raw sigma of the Jacobi column
-        for (int i = 0; i < r; ++i) {
-            const long long row = (long long)i * d + t;
-            double f;
-            if (s == 0)
-                f = tall ? J[(long long)pc * m1 + row] * inv : TV[(long long)pc * m1 + row];
-            else
-                f = tall ? TV[(long long)pc * m2 + row] : J[(long long)pc * m2 + row] * inv;
-            acc += Q[x * r + i] * f;
-        }
-        cur[idx] = acc * sq;
+#pragma unroll 4
+        for (int i = 0; i < r; ++i) acc += Qs[x * r + i] * F[((long long)i * d + t) * chi + cc];
+        cur[idx] = acc;
     }
     c.sync();
     // inverse gauges on the other ports
@@ -531,13 +686,18 @@ GT_FN void gt_run(const Ctx &c, const GtLayout &L, const double *in, double *out
         }
         c.sync();
         if (L.norm) {
-            // one thread per port; port j's message follows those of ports 0..j-1
+            // port j's message follows those of ports 0..j-1
+            const double *Ms[GT_KMAX];
+            double *Es[GT_KMAX];
+            int Ds[GT_KMAX];
             long long mo = 0;
             for (int j = 0; j < L.nmsg1; ++j) {
-                if (j % c.nt == c.tid) gt_gauge_serial(M + mo, ws + L.msg[0][j], L.dims1[j]);
+                Ms[j] = M + mo;
+                Es[j] = ws + L.msg[0][j];
+                Ds[j] = L.dims1[j];
                 mo += (long long)L.dims1[j] * L.dims1[j];
             }
-            c.sync();
+            gt_gauges(c, L.nmsg1, Ms, Es, Ds, ws + L.rot);
         }
         if (L.norm) {
             // gauge every port of res, then the norm
@@ -562,6 +722,7 @@ GT_FN void gt_run(const Ctx &c, const GtLayout &L, const double *in, double *out
         if (c.tid == 0) *chi_out = 0;
         return;
     }
+    c.mark(0);
     // inputs
     const double *A1 = in, *A2 = in + L.size1;
     const double *mp = A2 + L.size2;
@@ -577,23 +738,30 @@ GT_FN void gt_run(const Ctx &c, const GtLayout &L, const double *in, double *out
         }
     }
     const double *G = mp;
-    // 1. gauges, one thread per message
+    // 1. gauges of every message of the gate
     {
-        int idx = 0;
+        const double *Ms[2 * GT_KMAX];
+        double *Es[2 * GT_KMAX];
+        int Ds[2 * GT_KMAX], nm = 0;
         for (int s = 0; s < 2; ++s) {
             const int k = s ? L.k2 : L.k1, p = s ? L.p2 : L.p1;
             const int *dims = s ? L.dims2 : L.dims1;
             for (int j = 0; j < k; ++j) {
                 if (j == p) continue;
-                if (idx % c.nt == c.tid) gt_gauge_serial(mptr[s][j], ws + L.msg[s][j], dims[j]);
-                ++idx;
+                Ms[nm] = mptr[s][j];
+                Es[nm] = ws + L.msg[s][j];
+                Ds[nm] = dims[j];
+                ++nm;
             }
         }
+        gt_gauges(c, nm, Ms, Es, Ds, ws + L.rot);
     }
-    c.sync();
+    c.mark(1);
     // 2-3. gauged matrices and their factorisations
     gt_site_qr(c, L, 0, A1, ws);
+    c.mark(2);
     gt_site_qr(c, L, 1, A2, ws);
+    c.mark(3);
     // 4. Theta' in the Jacobi layout
     const int d1 = L.d1, d2 = L.d2, Db = L.Db, r1 = L.r1, r2 = L.r2, m1 = L.m1, m2 = L.m2;
     const double *R1 = ws + L.R[0], *R2 = ws + L.R[1];
@@ -622,11 +790,13 @@ GT_FN void gt_run(const Ctx &c, const GtLayout &L, const double *in, double *out
             J[(long long)row * m2 + col] = acc;
     }
     c.sync();
+    c.mark(4);
     // 5. SVD, sort, truncate, normalise
     if (tall)
-        gt_jacobi(c, J, m1, m2, ws + L.tv);
+        gt_jacobi_staged(c, J, m1, m2, ws + L.tv);
     else
-        gt_jacobi(c, J, m2, m1, ws + L.tv);
+        gt_jacobi_staged(c, J, m2, m1, ws + L.tv);
+    c.mark(5);
     const int q = L.q, rows = tall ? m1 : m2;
     double *sig = ws + L.sig, *perm = ws + L.perm, *raw = ws + L.raw;
     for (int i = c.tid; i < q; i += c.nt) {
@@ -635,41 +805,44 @@ GT_FN void gt_run(const Ctx &c, const GtLayout &L, const double *in, double *out
         raw[i] = sqrt(s2);  // sigma of Jacobi column i
     }
     c.sync();
+    // rank of every Jacobi column by sigma, descending (ties by index): perm[rank] = column
+    for (int i = c.tid; i < q; i += c.nt) {
+        int rank = 0;
+        for (int j = 0; j < q; ++j) rank += raw[j] > raw[i] || (raw[j] == raw[i] && j < i);
+        perm[rank] = (double)i;
+    }
+    c.sync();
     if (c.tid == 0) {
-        // sort column indices by sigma, descending (insertion sort; q <= GT_QMAX)
-        int idx[GT_QMAX];
-        for (int i = 0; i < q; ++i) idx[i] = i;
-        for (int i = 1; i < q; ++i) {
-            const int v = idx[i];
-            int j = i - 1;
-            while (j >= 0 && raw[idx[j]] < raw[v]) {
-                idx[j + 1] = idx[j];
-                --j;
-            }
-            idx[j + 1] = v;
-        }
+        const double top = raw[(int)perm[0]];
         int chi = L.chimax;
         if (rtol >= 0.0)
-            while (chi > 1 && raw[idx[chi - 1]] <= rtol * raw[idx[0]]) --chi;
+            while (chi > 1 && raw[(int)perm[chi - 1]] <= rtol * top) --chi;
         double nrm = 1.0;
         if (L.norm) {
             double s2 = 0.0;
-            for (int i = 0; i < chi; ++i) s2 += raw[idx[i]] * raw[idx[i]];
+            for (int i = 0; i < chi; ++i) s2 += raw[(int)perm[i]] * raw[(int)perm[i]];
             nrm = s2 > 0.0 ? sqrt(s2) : 1.0;
-        }
-        for (int i = 0; i < q; ++i) {
-            sig[i] = i < chi ? raw[idx[i]] / nrm : 0.0;  // kept, sorted, scaled
-            perm[i] = (double)idx[i];
         }
         *chi_out = chi;
         c.red[0] = (double)chi;
+        c.red[1] = nrm;
+    }
+    c.sync();
+    {
+        const int chi = (int)c.red[0];
+        const double nrm = c.red[1];
+        for (int i = c.tid; i < q; i += c.nt)  // kept, sorted, scaled
+            sig[i] = i < chi ? raw[(int)perm[i]] / nrm : 0.0;
     }
     c.sync();
     const int chi = (int)c.red[0];
     c.sync();
+    c.mark(6);
     // 6. new site tensors and the singular values
     gt_site_rebuild(c, L, 0, chi, ws, out);
+    c.mark(7);
     gt_site_rebuild(c, L, 1, chi, ws, out + L.out1);
+    c.mark(8);
     for (int i = c.tid; i < chi; i += c.nt) out[L.out1 + L.out2 + i] = sig[i];
     c.sync();
     (void)r1;
@@ -677,19 +850,21 @@ GT_FN void gt_run(const Ctx &c, const GtLayout &L, const double *in, double *out
 }
 
 // one workgroup per gate: desc (GD_N ints per gate), offs (GO_N per gate), chi (per gate)
+// LDS per workgroup: lds_cap doubles of dynamic LDS for the Jacobi sweeps
 __global__ __launch_bounds__(GT_NT) void k_gate(const int *__restrict__ desc,
                                                 const long long *__restrict__ offs,
                                                 const double *__restrict__ in,
                                                 double *__restrict__ out,
                                                 double *__restrict__ ws, int *__restrict__ chi,
-                                                double rtol) {
+                                                double rtol, long long lds_cap) {
     __shared__ GtLayout L;
     __shared__ double red[GT_NT];
     __shared__ int flag;
+    extern __shared__ double gt_lds[];
     const int g = blockIdx.x;
     if (threadIdx.x == 0) gt_layout(desc + (long long)g * GD_N, L);
     __syncthreads();
-    GtDev c{(int)threadIdx.x, GT_NT, red, &flag};
+    GtDev c{(int)threadIdx.x, GT_NT, red, &flag, gt_lds, lds_cap};
     const long long *o = offs + (long long)g * GO_N;
     gt_run(c, L, in + o[GO_IN], out + o[GO_OUT], ws + o[GO_WS], chi + g, rtol);
 }

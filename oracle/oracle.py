@@ -348,3 +348,91 @@ def fixed_point_gradient(topo, sites, msgs, dL_dm, dL_dA, maxiter=1000, tol=1e-1
                                           int(maxiter), float(tol), _p(grad), C.byref(it),
                                           C.byref(res)), "fixed_point_gradient")
     return grad, it.value, res.value
+
+
+# ------------------------------------------------------------ gate application (numpy)
+# BPApplyGate / apply_gate_bp_nsite! (src/apply/apply_operators.jl:214-283) restated with
+# numpy's LAPACK routines in the roles of the external factorisations: the message square
+# root X (X X^T = M) and its pseudo-inverse from numpy.linalg.eigh (gram_eigh_full_with_pinv,
+# TensorAlgebra), the site factorisation from numpy.linalg.qr (qr_compact,
+# MatrixAlgebraKit), the split from numpy.linalg.svd (svd_trunc).  Tensors are
+# [phys][port 0]...[port k-1]; `msgs` maps a port to its incoming D x D message.  Pinned in
+# test_oracle.py by the reference's gauge-invariant properties (test_apply_operator.jl):
+# an untruncated gate reproduces the gated two-site tensor exactly, and on a tree with
+# exact BP environments a truncated gate is the global optimal truncation.
+
+def _gate_gauge(M):
+    lam, V = np.linalg.eigh(0.5 * (M + M.T))
+    lmax = lam.max() if lam.size else 0.0
+    lam = np.where(lam > 1e-13 * lmax, lam, 0.0)
+    X = V * np.sqrt(lam)
+    inv = np.zeros_like(lam)
+    nz = lam > 0
+    inv[nz] = 1.0 / np.sqrt(lam[nz])
+    return X, (V * inv).T
+
+
+def _gate_mode(T, j, M):
+    return np.moveaxis(np.tensordot(T, M, axes=([j + 1], [0])), -1, j + 1)
+
+
+def apply_gate1(A, O, msgs=None, normalize=False):
+    """One-site gate (:214-227): O [t][s] on the physical index; with normalize, divided by
+    the norm of the result with every port's square root X absorbed."""
+    B = np.tensordot(O, A, axes=([1], [0]))
+    if normalize:
+        C_ = B
+        for j, M in (msgs or {}).items():
+            C_ = _gate_mode(C_, j, _gate_gauge(M)[0])
+        B = B / np.linalg.norm(C_)
+    return B
+
+
+def apply_gate2(A1, A2, p1, p2, msgs1, msgs2, G, maxdim=0, rtol=-1.0, normalize=False):
+    """Two-site gate (:229-283).  G is [t1][t2][s1][s2] (any shape of d1*d2 x d1*d2 entries).
+    Returns (new A1, new A2, S) with the shared port's dimension replaced by len(S)."""
+    k1, k2 = A1.ndim - 1, A2.ndim - 1
+    d1, d2 = A1.shape[0], A2.shape[0]
+    Db = A1.shape[1 + p1]
+    g1 = {j: _gate_gauge(M) for j, M in msgs1.items()}
+    g2 = {j: _gate_gauge(M) for j, M in msgs2.items()}
+    B1, B2 = A1, A2
+    for j, (X, _) in g1.items():
+        B1 = _gate_mode(B1, j, X)
+    for j, (X, _) in g2.items():
+        B2 = _gate_mode(B2, j, X)
+    o1 = [j for j in range(k1) if j != p1]
+    o2 = [j for j in range(k2) if j != p2]
+    X1 = np.transpose(B1, [1 + j for j in o1] + [0, 1 + p1]).reshape(-1, d1 * Db)
+    X2 = np.transpose(B2, [1 + j for j in o2] + [0, 1 + p2]).reshape(-1, d2 * Db)
+    Q1, R1 = np.linalg.qr(X1)
+    Q2, R2 = np.linalg.qr(X2)
+    r1, r2 = R1.shape[0], R2.shape[0]
+    T = np.einsum("iab,jcb->iajc", R1.reshape(r1, d1, Db), R2.reshape(r2, d2, Db))
+    Tp = np.einsum("xyac,iajc->ixjy", np.asarray(G).reshape(d1, d2, d1, d2), T)
+    U, S, Vt = np.linalg.svd(Tp.reshape(r1 * d1, r2 * d2), full_matrices=False)
+    chi = len(S) if maxdim <= 0 else min(len(S), maxdim)
+    if rtol >= 0:
+        while chi > 1 and S[chi - 1] <= rtol * S[0]:
+            chi -= 1
+    U, S, Vt = U[:, :chi], S[:chi], Vt[:chi]
+    if normalize:
+        S = S / np.linalg.norm(S)
+    N1 = (Q1 @ (U * np.sqrt(S)).reshape(r1, d1 * chi)).reshape(
+        [A1.shape[1 + j] for j in o1] + [d1, chi])
+    N2 = (Q2 @ (Vt.T * np.sqrt(S)).reshape(r2, d2 * chi)).reshape(
+        [A2.shape[1 + j] for j in o2] + [d2, chi])
+
+    def back(N, o, p, k):
+        perm = [len(o)] + [0] * k
+        for a, j in enumerate(o):
+            perm[1 + j] = a
+        perm[1 + p] = len(o) + 1
+        return np.transpose(N, perm)
+
+    A1n, A2n = back(N1, o1, p1, k1), back(N2, o2, p2, k2)
+    for j, (_, Xi) in g1.items():
+        A1n = _gate_mode(A1n, j, Xi)
+    for j, (_, Xi) in g2.items():
+        A2n = _gate_mode(A2n, j, Xi)
+    return A1n, A2n, S

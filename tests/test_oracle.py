@@ -402,3 +402,82 @@ def test_local_energy_gradient_vs_fd(pkg, orc):
     # the value is the sum of the expectation values
     want = (orc.expect_vertex(t, s, m, ov).sum() + orc.expect_edge(t, s, m, oa, ob).sum())
     assert E == pytest.approx(want, rel=1e-13)
+
+
+# ------------------------------------------------------------ gate application oracle
+
+def _two_site(A1, A2, p1, p2):
+    return np.tensordot(A1, A2, axes=([1 + p1], [1 + p2]))
+
+
+def test_gate_oracle_untruncated_exact(orc):
+    """test_apply_operator.jl:52-74: with invertible message square roots an untruncated
+    two-site gate reproduces G applied to the two-site tensor (any gauge)."""
+    rng = np.random.default_rng(21)
+    for trial in range(10):
+        d1, d2, D = 3, 2, 3
+        dims1, dims2 = [2, D, 3], [D, 4]
+        A1 = rng.standard_normal([d1] + dims1)
+        A2 = rng.standard_normal([d2] + dims2)
+        psd = lambda n: (lambda B: B @ B.T + 0.1 * np.eye(n))(rng.standard_normal((n, n)))
+        m1 = {0: psd(2), 2: psd(3)}
+        m2 = {1: psd(4)}
+        G = rng.standard_normal((d1 * d2, d1 * d2))
+        B1, B2, S = orc.apply_gate2(A1, A2, 1, 0, m1, m2, G)
+        got = _two_site(B1, B2, 1, 0)
+        want = np.tensordot(G.reshape(d1, d2, d1, d2), _two_site(A1, A2, 1, 0),
+                            axes=([2, 3], [0, 3]))
+        want = np.moveaxis(want, 1, 3)  # [t1, a, c, t2, e]
+        assert np.max(np.abs(got - want)) < 1e-12 * np.max(np.abs(want))
+        assert np.all(np.diff(S) <= 0)
+
+
+def test_gate_oracle_truncation_is_optimal_on_a_tree(pkg, orc):
+    """test_apply_operator.jl:76-98: on a path with the oracle's converged (exact) BP
+    environments, a rank-k two-site gate equals the global optimal truncation, and its
+    singular values are proportional to the Schmidt values."""
+    g = pkg.named_path_graph(4)
+    t = pkg.build_topology(g, 3, 2)
+    s = pkg.peps.random_sites(t, 5)
+    m, _, _, at = orc.bp(t, s, pkg.peps.identity_messages(t), maxiter=400, tol=1e-15,
+                         criterion="maxabs")
+    assert at >= 0
+    A = [s[t.site_off[u]:t.site_off[u + 1]].reshape(t.site_shape(u)) for u in range(t.nv)]
+
+    def incoming(u, skip):
+        out = {}
+        for j, q in enumerate(range(t.port_off[u], t.port_off[u + 1])):
+            w = int(t.port_nbr[q])
+            if w == skip:
+                continue
+            sl = int(t.port_rev[q])
+            D = int(t.edge_dim[t.port_edge[q]])
+            out[j] = m[t.msg_off[sl]:t.msg_off[sl + 1]].reshape(D, D)
+        return out
+
+    u1, u2 = 1, 2
+    p1 = [int(t.port_nbr[q]) for q in range(t.port_off[u1], t.port_off[u1 + 1])].index(u2)
+    p2 = [int(t.port_nbr[q]) for q in range(t.port_off[u2], t.port_off[u2 + 1])].index(u1)
+    rng = np.random.default_rng(3)
+    G = rng.standard_normal((4, 4))
+
+    def full(tensors):
+        labels = []
+        for u in range(t.nv):
+            ports = range(t.port_off[u], t.port_off[u + 1])
+            labels.append((("s", u),) + tuple(("e", int(t.port_edge[q])) for q in ports))
+        val, labs = orc.contract_labeled(tensors, labels)
+        return np.transpose(val, [labs.index(("s", u)) for u in range(t.nv)])
+
+    psi = full(A)
+    gated = np.moveaxis(np.tensordot(G.reshape(2, 2, 2, 2), psi, axes=([2, 3], [1, 2])),
+                        [0, 1], [1, 2])
+    U, S, Vt = np.linalg.svd(gated.reshape(4, 4), full_matrices=False)
+    for k in (1, 2, 3):
+        B1, B2, Sk = orc.apply_gate2(A[u1], A[u2], p1, p2, incoming(u1, u2), incoming(u2, u1),
+                                     G, maxdim=k)
+        T = list(A)
+        T[u1], T[u2] = B1, B2
+        best = (U[:, :k] * S[:k]) @ Vt[:k]
+        assert np.max(np.abs(full(T).reshape(4, 4) - best)) < 1e-10 * np.max(np.abs(best))
+        assert np.max(np.abs(Sk / Sk[0] - S[:k] / S[0])) < 1e-10
