@@ -25,6 +25,8 @@
  *   bp_cn_*                  contraction_order src/contract_network.jl:6-60
  *   bp_apply_gates           apply_operator / BPApplyGate / apply_gate_bp_nsite!
  *                            src/apply/apply_operators.jl:181-283
+ *   bp_apply_gates_resident  the same on the resident network (no marshalling between gate
+ *                            layers and BP runs)
  * The adjoint entry points (bp_grad_*) and the energy estimator have no reference
  * counterpart: the reference has no AD and no expectation-value API.
  *
@@ -320,6 +322,27 @@ enum { BP_GATE_DESC = 25 };
 int bp_gate_sizes(int ngates, const int *desc, long long *in_size, long long *out_size);
 int bp_apply_gates(bp_context *ctx, int ngates, const int *desc, const double *in,
                    double *out, int *chi, double rtol);
+
+/* Gates on the context's resident network: the same algorithm applied in place to the
+ * context's FP64 site tensors and current messages, with no host copy of either (the
+ * network stays on the device between gate layers and BP runs; ITensorNetwork /
+ * MessageCache are needed only to import and export, messagecache.jl:205-225,
+ * tensornetwork.jl:138-173).  kind[g] = 2: a two-site gate on the bond of message slot
+ * where[g] (site 1 = the slot's owner, site 2 = its neighbour); kind[g] = 1: a one-site gate
+ * on vertex where[g].  gates: the gate matrices packed in order ([t1][t2][s1][s2], or [t][s]).
+ * No two gates of one call may share a vertex.  Each bond keeps its dimension De: at most
+ * min(De, maxdim) singular values are kept (maxdim <= 0: De; then rtol as in
+ * bp_apply_gates) and the tensors and diag(S) are zero-padded to De; both messages of the bond become diag(S), in the working
+ * messages and, after bp_polish_fp64, in the FP64 ones.  out_chi (per gate) and out_s
+ * (ngates x s_stride, zero-padded; s_stride >= every gated bond's De) may be NULL.  Norm
+ * networks only.  bp_last_run_ms gives the device time of the gather, gate and scatter
+ * kernels; the working site copies are refreshed before return. */
+int bp_apply_gates_resident(bp_context *ctx, int ngates, const int *kind, const int *where,
+                            const double *gates, int maxdim, int normalize, double rtol,
+                            int *out_chi, double *out_s, int s_stride);
+
+/* The context's FP64 site tensors (site layout), e.g. after resident gates. */
+int bp_download_sites(bp_context *ctx, double *sites);
 
 #ifdef __cplusplus
 }

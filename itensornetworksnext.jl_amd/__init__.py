@@ -7,7 +7,8 @@ vertex contraction and adjoint in hand-written HIP kernels for gfx950 behind the
 library `csrc/libbpmi355x.so` (declared in include/bp_mi355x.h).  There is no CPU
 fallback: the compute entry points raise when the library or a GPU is missing.
 """
-from . import _lib, algorithms, apply, build, contract, engine, graphs, itensors, peps  # noqa: F401
+from . import (_lib, algorithms, apply, build, contract, engine, graphs, itensors,  # noqa: F401
+               peps, resident)
 from .algorithms import (AbstractAlgorithm, ArgumentError, MethodError,  # noqa: F401
                          StopAfterIteration, StopWhenAny, StopWhenConverged, default_algorithm,
                          select_algorithm, select_beliefpropagation_stopping_criterion)
@@ -19,6 +20,7 @@ from .apply import (BPApplyGate, Operator, apply_operator, apply_operators,  # n
 from .contract import (Cheapest, ContractionPlan, Exact, Flat, Greedy,  # noqa: F401
                        LeftAssociative, contract_labeled, contract_network)
 from .engine import Engine, RunResult  # noqa: F401
+from .resident import DeviceNetwork  # noqa: F401
 from .graphs import (NamedGraph, Topology, build_topology, concat_topologies,  # noqa: F401
                      forest_cover_edge_sequence, heavy_hex_lattice, named_comb_tree,
                      named_grid, named_path_graph)
@@ -32,7 +34,8 @@ from .messagecache import (MessageCache, bethe_free_energy, edge_scalar,  # noqa
                            similar_message_environment, vertex_scalar, vertex_scalars)
 
 __all__ = [
-    "AbstractAlgorithm", "ArgumentError", "BPApplyGate", "Cheapest", "Operator",
+    "AbstractAlgorithm", "ArgumentError", "BPApplyGate", "Cheapest", "DeviceNetwork",
+    "Operator",
     "apply_operator", "apply_operators", "operator", "truncrank", "trunctol", "ContractionPlan", "Engine", "Exact", "Flat",
     "Greedy", "LeftAssociative", "contract_labeled", "contract_network", "ITensor", "ITensorNetwork", "Index",
     "JacobiGPUUpdate", "MessageCache", "MessageUpdateAlgorithm", "MethodError", "NamedGraph",

@@ -76,6 +76,9 @@ SIGNATURES = {
     "bp_gate_sizes": (C.c_int, [C.c_int, _ip, C.POINTER(C.c_longlong),
                                 C.POINTER(C.c_longlong)]),
     "bp_apply_gates": (C.c_int, [_vp, C.c_int, _ip, _dp, _dp, _ip, C.c_double]),
+    "bp_apply_gates_resident": (C.c_int, [_vp, C.c_int, _ip, _ip, _dp, C.c_int, C.c_int,
+                                          C.c_double, _ip, _dp, C.c_int]),
+    "bp_download_sites": (C.c_int, [_vp, _dp]),
 }
 BP_CN_ORDER_LEFT, BP_CN_ORDER_GREEDY, BP_CN_ORDER_CHEAPEST = 0, 1, 2
 BP_GATE_DESC = 25

@@ -727,6 +727,8 @@ static int rebuild_vertex_plan(bp_context *c) {
 
 // ------------------------------------------------------------------ C ABI
 
+static int refresh_sites(bp_context *c);
+
 extern "C" {
 
 const char *bp_last_error(void) { return g_err.c_str(); }
@@ -919,6 +921,25 @@ int bp_upload_sites(bp_context *c, const double *sites) {
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipMemcpyAsync(c->d_sites64, sites, sizeof(double) * c->nsite,
                           hipMemcpyHostToDevice, c->stream));
+    RETCHK(refresh_sites(c));
+    c->have_sites = true;
+    return BP_OK;
+}
+
+int bp_download_sites(bp_context *c, double *sites) {
+    if (!c || !sites) return set_err(BP_ERR_ARG, "NULL argument");
+    if (!c->have_sites) return set_err(BP_ERR_STATE, "no site tensors");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(sites, c->d_sites64, sizeof(double) * c->nsite,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BP_OK;
+}
+}  // extern "C"
+
+// the working copies of the FP64 master sites: the context's arithmetic type and the
+// vertex kernels' padded layouts
+static int refresh_sites(bp_context *c) {
     const int blocks = (int)std::min<long long>((c->nsite + 255) / 256, 65535);
     if (c->dtype == BP_F64) {
         HIPCHK(hipMemcpyAsync(c->d_sites, c->d_sites64, sizeof(double) * c->nsite,
@@ -935,9 +956,9 @@ int bp_upload_sites(bp_context *c, const double *sites) {
     }
     if (c->vplan_ok) RETCHK(upload_vertex_sites(c));
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->have_sites = true;
     return BP_OK;
 }
+extern "C" {
 
 int bp_init_messages(bp_context *c, int kind, const double *host) {
     if (!c) return set_err(BP_ERR_ARG, "ctx is NULL");
@@ -2061,6 +2082,220 @@ int bp_apply_gates(bp_context *c, int ngates, const int *desc, const double *in,
     release();
     if (e != hipSuccess) return set_err(BP_ERR_HIP, "gate application: %s", hipGetErrorString(e));
     c->last_ms = ms;
+    return BP_OK;
+}
+
+int bp_apply_gates_resident(bp_context *c, int ngates, const int *kind, const int *where,
+                            const double *gates, int maxdim, int normalize, double rtol,
+                            int *out_chi, double *out_s, int s_stride) {
+    RETCHK(ready(c));
+    if (ngates < 0 || (ngates > 0 && (!kind || !where || !gates)))
+        return set_err(BP_ERR_ARG, "NULL argument");
+    if (c->plain) return set_err(BP_ERR_UNSUPPORTED, "gates act on norm networks");
+    if (out_s && s_stride < 1) return set_err(BP_ERR_ARG, "s_stride must be positive");
+    if (ngates == 0) return BP_OK;
+    std::vector<int> desc((size_t)ngates * GD_N, 0);
+    std::vector<long long> offs((size_t)ngates * GO_N), rec((size_t)ngates * GS_N, 0), seg;
+    std::vector<char> used(c->nv, 0);
+    long long tin = 0, tout = 0, tws = 0, lneed = 0, gsum = 0;
+    for (int g = 0; g < ngates; ++g) {
+        int *d = desc.data() + (size_t)g * GD_N;
+        long long *rr = rec.data() + (size_t)g * GS_N;
+        int us[2], ps[2], ks[2];
+        if (kind[g] == 2) {
+            const int q1 = where[g];
+            if (q1 < 0 || q1 >= c->nports) return set_err(BP_ERR_ARG, "gate %d: slot %d", g, q1);
+            us[0] = c->owner[q1];
+            us[1] = c->port_nbr[q1];
+            const int q2 = c->port_rev[q1];
+            ps[0] = q1 - c->port_off[us[0]];
+            ps[1] = q2 - c->port_off[us[1]];
+        } else if (kind[g] == 1) {
+            if (where[g] < 0 || where[g] >= c->nv)
+                return set_err(BP_ERR_ARG, "gate %d: vertex %d", g, where[g]);
+            us[0] = where[g];
+            ps[0] = -1;
+        } else {
+            return set_err(BP_ERR_ARG, "gate %d: kind %d", g, kind[g]);
+        }
+        const int ns = kind[g];
+        for (int s2 = 0; s2 < ns; ++s2) {
+            if (used[us[s2]])
+                return set_err(BP_ERR_ARG, "gate %d touches vertex %d of an earlier gate of the "
+                                           "same call", g, us[s2]);
+            used[us[s2]] = 1;
+            ks[s2] = c->port_off[us[s2] + 1] - c->port_off[us[s2]];
+            if (ks[s2] > GT_KMAX)
+                return set_err(BP_ERR_UNSUPPORTED, "vertex %d has %d ports (at most %d)",
+                               us[s2], ks[s2], GT_KMAX);
+        }
+        d[GD_KIND] = ns;
+        d[GD_D1] = c->phys_dim[us[0]];
+        d[GD_D2] = ns == 2 ? c->phys_dim[us[1]] : 0;
+        d[GD_K1] = ks[0];
+        d[GD_K2] = ns == 2 ? ks[1] : 0;
+        d[GD_P1] = ps[0];
+        d[GD_P2] = ns == 2 ? ps[1] : -1;
+        d[GD_NORM] = normalize ? 1 : 0;
+        for (int s2 = 0; s2 < ns; ++s2)
+            for (int j = 0; j < ks[s2]; ++j)
+                d[(s2 ? GD_DIMS2 : GD_DIMS1) + j] =
+                    c->edge_dim[c->port_edge[c->port_off[us[s2]] + j]];
+        const int De = ns == 2 ? d[GD_DIMS1 + ps[0]] : 0;
+        d[GD_MAXDIM] = maxdim > 0 && maxdim < De ? maxdim : De;  // the bond keeps De
+        GtLayout L;
+        if (gt_layout(d, L)) return set_err(BP_ERR_ARG, "gate %d: unsupported shape", g);
+        offs[(size_t)g * GO_N + GO_IN] = tin;
+        offs[(size_t)g * GO_N + GO_OUT] = tout;
+        offs[(size_t)g * GO_N + GO_WS] = tws;
+        // input segments: sites, the messages into every port (but the bond's), the gate
+        long long at = tin;
+        for (int s2 = 0; s2 < ns; ++s2) {
+            const long long sz = c->site_off[us[s2] + 1] - c->site_off[us[s2]];
+            seg.insert(seg.end(), {0, c->site_off[us[s2]], at, sz});
+            at += sz;
+        }
+        for (int s2 = 0; s2 < ns; ++s2) {
+            if (ns == 1 && !normalize) break;
+            for (int j = 0; j < ks[s2]; ++j) {
+                if (ns == 2 && j == ps[s2]) continue;
+                const int q = c->port_off[us[s2]] + j, sl = c->port_rev[q];
+                const long long len = c->msg_off[sl + 1] - c->msg_off[sl];
+                seg.insert(seg.end(), {1, c->msg_off[sl], at, len});
+                at += len;
+            }
+        }
+        const long long glen = ns == 2 ? (long long)d[GD_D1] * d[GD_D2] * d[GD_D1] * d[GD_D2]
+                                       : (long long)d[GD_D1] * d[GD_D1];
+        seg.insert(seg.end(), {2, gsum, at, glen});
+        at += glen;
+        gsum += glen;
+        if (at - tin != L.in_total) return set_err(BP_ERR_ARG, "gate %d: packing mismatch", g);
+        rr[GS_KIND] = ns;
+        rr[GS_OUT] = tout;
+        rr[GS_CAP1] = L.out1;
+        rr[GS_CAP2] = L.out2;
+        rr[GS_SITE1] = c->site_off[us[0]];
+        rr[GS_SITE2] = ns == 2 ? c->site_off[us[1]] : 0;
+        rr[GS_D1] = d[GD_D1];
+        rr[GS_D2] = d[GD_D2];
+        rr[GS_K1] = ks[0];
+        rr[GS_K2] = ns == 2 ? ks[1] : 0;
+        rr[GS_P1] = ps[0];
+        rr[GS_P2] = ns == 2 ? ps[1] : -1;
+        rr[GS_DE] = De;
+        if (ns == 2) {
+            const int q1 = where[g];
+            rr[GS_MSG1] = c->msg_off[q1];
+            rr[GS_MSG2] = c->msg_off[c->port_rev[q1]];
+        }
+        for (int j = 0; j < GT_KMAX; ++j) {
+            rr[GS_DIMS1 + j] = d[GD_DIMS1 + j];
+            rr[GS_DIMS2 + j] = d[GD_DIMS2 + j];
+        }
+        if (out_s && ns == 2 && De > s_stride)
+            return set_err(BP_ERR_ARG, "s_stride %d below the bond dimension %d", s_stride, De);
+        tin = at;
+        tout += L.out_total;
+        tws += (L.ws_total + 31) / 32 * 32;
+        lneed = std::max(lneed, gt_lds_need(L));
+    }
+    const long long lds_cap = std::min<long long>(lneed, 120 * 1024 / 8);
+    const size_t lds_bytes = sizeof(double) * (size_t)lds_cap;
+    const int nseg = (int)(seg.size() / 4);
+    HIPCHK(hipSetDevice(c->device));
+    bool is64 = false;
+    const void *msgs = current_msgs(c, &is64);
+    int *d_desc = nullptr, *d_chi = nullptr;
+    long long *d_offs = nullptr, *d_seg = nullptr, *d_rec = nullptr;
+    double *d_gates = nullptr, *d_in = nullptr, *d_out = nullptr, *d_ws = nullptr, *d_s = nullptr;
+    auto release = [&]() {
+        for (void *ptr : {(void *)d_desc, (void *)d_chi, (void *)d_offs, (void *)d_seg,
+                          (void *)d_rec, (void *)d_gates, (void *)d_in, (void *)d_out,
+                          (void *)d_ws, (void *)d_s})
+            if (ptr) (void)hipFree(ptr);
+    };
+    const size_t sbytes = out_s ? sizeof(double) * (size_t)ngates * s_stride : 0;
+    if (hipMalloc((void **)&d_desc, sizeof(int) * desc.size()) != hipSuccess ||
+        hipMalloc((void **)&d_chi, sizeof(int) * (size_t)ngates) != hipSuccess ||
+        hipMalloc((void **)&d_offs, sizeof(long long) * offs.size()) != hipSuccess ||
+        hipMalloc((void **)&d_seg, sizeof(long long) * std::max<size_t>(seg.size(), 1)) != hipSuccess ||
+        hipMalloc((void **)&d_rec, sizeof(long long) * rec.size()) != hipSuccess ||
+        hipMalloc((void **)&d_gates, sizeof(double) * std::max(gsum, 1LL)) != hipSuccess ||
+        hipMalloc((void **)&d_in, sizeof(double) * std::max(tin, 1LL)) != hipSuccess ||
+        hipMalloc((void **)&d_out, sizeof(double) * std::max(tout, 1LL)) != hipSuccess ||
+        hipMalloc((void **)&d_ws, sizeof(double) * std::max(tws, 1LL)) != hipSuccess ||
+        (out_s && hipMalloc((void **)&d_s, sbytes) != hipSuccess)) {
+        release();
+        return set_err(BP_ERR_NOMEM, "gate workspace (%lld doubles) does not fit", tws);
+    }
+    hipError_t e = hipSuccess;
+    auto up = [&](void *dst, const void *src, size_t n) {
+        if (e == hipSuccess) e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream);
+    };
+    up(d_desc, desc.data(), sizeof(int) * desc.size());
+    up(d_offs, offs.data(), sizeof(long long) * offs.size());
+    up(d_seg, seg.data(), sizeof(long long) * seg.size());
+    up(d_rec, rec.data(), sizeof(long long) * rec.size());
+    up(d_gates, gates, sizeof(double) * gsum);
+    if (e == hipSuccess && d_s) e = hipMemsetAsync(d_s, 0, sbytes, c->stream);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_gate,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    const unsigned gblocks = (unsigned)std::min(nseg, 65535);
+    double *m64 = nullptr;
+    if (e == hipSuccess) {
+        if (is64)
+            hipLaunchKernelGGL(k_gate_gather<double>, dim3(gblocks), dim3(256), 0, c->stream,
+                               (const long long *)d_seg, nseg, (const double *)c->d_sites64,
+                               (const double *)msgs, (const double *)d_gates, d_in);
+        else
+            hipLaunchKernelGGL(k_gate_gather<float>, dim3(gblocks), dim3(256), 0, c->stream,
+                               (const long long *)d_seg, nseg, (const double *)c->d_sites64,
+                               (const float *)msgs, (const double *)d_gates, d_in);
+        hipLaunchKernelGGL(k_gate, dim3((unsigned)ngates), dim3(GT_NT), lds_bytes, c->stream,
+                           (const int *)d_desc, (const long long *)d_offs, (const double *)d_in,
+                           d_out, d_ws, d_chi, rtol, lds_cap);
+        // the working messages, and the FP64 polish messages too when they are current
+        if (c->fp64_active) {
+            m64 = c->d_msg64[c->cur64];
+            if (c->dtype == BP_F64)
+                hipLaunchKernelGGL(k_gate_scatter<double>, dim3((unsigned)ngates), dim3(256), 0,
+                                   c->stream, (const long long *)d_rec, (const int *)d_chi,
+                                   (const double *)d_out, c->d_sites64, (double *)c->d_msg[c->cur],
+                                   m64, d_s, s_stride);
+            else
+                hipLaunchKernelGGL(k_gate_scatter<float>, dim3((unsigned)ngates), dim3(256), 0,
+                                   c->stream, (const long long *)d_rec, (const int *)d_chi,
+                                   (const double *)d_out, c->d_sites64, (float *)c->d_msg[c->cur],
+                                   m64, d_s, s_stride);
+        } else if (is64) {
+            hipLaunchKernelGGL(k_gate_scatter<double>, dim3((unsigned)ngates), dim3(256), 0,
+                               c->stream, (const long long *)d_rec, (const int *)d_chi,
+                               (const double *)d_out, c->d_sites64, (double *)c->d_msg[c->cur],
+                               (double *)nullptr, d_s, s_stride);
+        } else {
+            hipLaunchKernelGGL(k_gate_scatter<float>, dim3((unsigned)ngates), dim3(256), 0,
+                               c->stream, (const long long *)d_rec, (const int *)d_chi,
+                               (const double *)d_out, c->d_sites64, (float *)c->d_msg[c->cur],
+                               (double *)nullptr, d_s, s_stride);
+        }
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess && out_chi)
+        e = hipMemcpyAsync(out_chi, d_chi, sizeof(int) * (size_t)ngates, hipMemcpyDeviceToHost,
+                           c->stream);
+    if (e == hipSuccess && out_s)
+        e = hipMemcpyAsync(out_s, d_s, sbytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    release();
+    if (e != hipSuccess) return set_err(BP_ERR_HIP, "resident gates: %s", hipGetErrorString(e));
+    c->last_ms = ms;
+    RETCHK(refresh_sites(c));
     return BP_OK;
 }
 

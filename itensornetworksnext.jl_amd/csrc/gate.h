@@ -868,3 +868,78 @@ __global__ __launch_bounds__(GT_NT) void k_gate(const int *__restrict__ desc,
     const long long *o = offs + (long long)g * GO_N;
     gt_run(c, L, in + o[GO_IN], out + o[GO_OUT], ws + o[GO_WS], chi + g, rtol);
 }
+
+// ---------------------------------------------------------- gates on a resident network
+
+// Packs gate inputs from the engine's resident arrays: segment e = (source, source offset,
+// destination offset, length), source 0 the FP64 sites, 1 the messages, 2 the gate matrices.
+template <typename TM>
+__global__ void k_gate_gather(const long long *__restrict__ seg, int nseg,
+                              const double *__restrict__ sites, const TM *__restrict__ msgs,
+                              const double *__restrict__ gates, double *__restrict__ dst) {
+    for (int e = blockIdx.x; e < nseg; e += gridDim.x) {
+        const long long *t = seg + 4LL * e;
+        const int src = (int)t[0];
+        const long long so = t[1], dof = t[2], len = t[3];
+        for (long long i = threadIdx.x; i < len; i += blockDim.x)
+            dst[dof + i] = src == 0 ? sites[so + i] : src == 1 ? (double)msgs[so + i] : gates[so + i];
+    }
+}
+
+// per-gate scatter record (long long)
+enum {
+    GS_KIND = 0, GS_OUT, GS_CAP1, GS_CAP2, // packed output offset, the sites' output capacities
+    GS_SITE1, GS_SITE2,                    // resident site offsets
+    GS_D1, GS_D2, GS_K1, GS_K2, GS_P1, GS_P2, GS_DE,
+    GS_MSG1, GS_MSG2,                      // resident offsets of the bond's two messages
+    GS_DIMS1, GS_DIMS2 = GS_DIMS1 + GT_KMAX, GS_N = GS_DIMS2 + GT_KMAX
+};
+
+// Writes a gate's results back into the resident arrays, one workgroup per gate: the new
+// site tensors with the bond zero-padded from chi to its fixed dimension De, diag(S) padded
+// into both messages of the bond (the working messages mw, and m64 when not null), and S
+// into sout (stride s_stride, zero-padded).
+template <typename TM>
+__global__ void k_gate_scatter(const long long *__restrict__ rec, const int *__restrict__ chis,
+                               const double *__restrict__ out, double *__restrict__ sites,
+                               TM *__restrict__ mw, double *__restrict__ m64,
+                               double *__restrict__ sout, int s_stride) {
+    const long long *r = rec + (long long)blockIdx.x * GS_N;
+    const int kind = (int)r[GS_KIND];
+    const double *o = out + r[GS_OUT];
+    if (kind == 1) {
+        long long size = r[GS_D1];
+        for (int j = 0; j < (int)r[GS_K1]; ++j) size *= r[GS_DIMS1 + j];
+        for (long long i = threadIdx.x; i < size; i += blockDim.x) sites[r[GS_SITE1] + i] = o[i];
+        return;
+    }
+    const int chi = chis[blockIdx.x], De = (int)r[GS_DE];
+    for (int s = 0; s < 2; ++s) {
+        const int k = (int)r[s ? GS_K2 : GS_K1], p = (int)r[s ? GS_P2 : GS_P1];
+        const long long d = r[s ? GS_D2 : GS_D1];
+        const long long *dims = r + (s ? GS_DIMS2 : GS_DIMS1);
+        long long inner = 1, size = d;  // resident: port p has De; compact: chi
+        for (int j = 0; j < k; ++j) size *= dims[j];
+        for (int j = p + 1; j < k; ++j) inner *= dims[j];
+        const double *src = o + (s ? r[GS_CAP1] : 0);
+        double *dst = sites + r[s ? GS_SITE2 : GS_SITE1];
+        for (long long i = threadIdx.x; i < size; i += blockDim.x) {
+            const long long in_ = i % inner, b = (i / inner) % De, outer = i / (inner * De);
+            dst[i] = b < chi ? src[(outer * chi + b) * inner + in_] : 0.0;
+        }
+    }
+    const double *S = o + r[GS_CAP1] + r[GS_CAP2];
+    for (long long i = threadIdx.x; i < (long long)De * De; i += blockDim.x) {
+        const int a = (int)(i / De), b = (int)(i % De);
+        const double v = a == b && a < chi ? S[a] : 0.0;
+        mw[r[GS_MSG1] + i] = (TM)v;
+        mw[r[GS_MSG2] + i] = (TM)v;
+        if (m64) {
+            m64[r[GS_MSG1] + i] = v;
+            m64[r[GS_MSG2] + i] = v;
+        }
+    }
+    if (sout)
+        for (int i = threadIdx.x; i < s_stride; i += blockDim.x)
+            sout[(long long)blockIdx.x * s_stride + i] = i < chi ? S[i] : 0.0;
+}

@@ -230,6 +230,36 @@ class Engine:
                 "bp_expect_edge")
         return out
 
+    def sites(self) -> np.ndarray:
+        """The FP64 site tensors (site layout), as held on the device."""
+        out = np.zeros(int(self.topo.site_off[-1]))
+        L.check(self.lib.bp_download_sites(self.h, _dp(out)), "bp_download_sites")
+        return out
+
+    def apply_gates(self, kinds, wheres, gates, normalize: bool = False, rtol: float = -1.0,
+                    maxdim: int = 0):
+        """Gates on the resident network (bp_apply_gates_resident): kinds[g] 2 with
+        wheres[g] a message slot (site 1 its owner), or 1 with a vertex index; gates[g] the
+        matrix [(t1 t2)][(s1 s2)] (or [t][s]).  Returns (chi per gate, S as rows padded to
+        the largest gated bond dimension).  Bonds keep their dimension; at most
+        min(bond dimension, maxdim) singular values are kept (maxdim <= 0: no cap)."""
+        n = len(kinds)
+        k = _i32(kinds)
+        w = _i32(wheres)
+        g = _f64(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in gates])
+                 if n else np.zeros(1))
+        stride = 1
+        for kk, ww in zip(kinds, wheres):
+            if kk == 2:
+                stride = max(stride, int(self.topo.edge_dim[self.topo.port_edge[ww]]))
+        chi = np.zeros(max(n, 1), dtype=np.int32)
+        S = np.zeros((max(n, 1), stride))
+        L.check(self.lib.bp_apply_gates_resident(self.h, n, _ip(k), _ip(w), _dp(g),
+                                                 int(maxdim), int(normalize), float(rtol),
+                                                 _ip(chi), _dp(S), stride),
+                "bp_apply_gates_resident")
+        return chi[:n], S[:n]
+
     def set_option(self, name: str, value: int) -> None:
         """Kernel options (bp_set_option): "vertex_variant" (0 or 1) and
         "vertex_bond_mask" (bit D lets networks of largest bond dimension D use the vertex
