@@ -230,6 +230,12 @@ class Engine:
                 "bp_expect_edge")
         return out
 
+    def last_run_ms(self) -> float:
+        """Device time of the last timed call (bp_last_run_ms)."""
+        ms = C.c_double()
+        L.check(self.lib.bp_last_run_ms(self.h, C.byref(ms)), "bp_last_run_ms")
+        return float(ms.value)
+
     def sites(self) -> np.ndarray:
         """The FP64 site tensors (site layout), as held on the device."""
         out = np.zeros(int(self.topo.site_off[-1]))
