@@ -104,13 +104,6 @@ __global__ void k_reset_nets(int nnet, int *done, int *iter, double *delta, int 
 // network): StopWhenConverged on `res` (fidelity or max-abs per slot) and
 // StopAfterIteration(maxiter) (AlgorithmsInterfaceExtensions.jl:84-109,
 // beliefpropagation.jl:27-55).
-// Stopping parameters of a run, read from device memory so that a captured sweep sequence
-// serves every (maxiter, tol, criterion).
-struct RunParams {
-    int maxiter;
-    int crit;  // BP_CRIT_FIDELITY or BP_CRIT_MAXABS
-    double tol;
-};
 
 __global__ __launch_bounds__(1024) void k_reduce_nets(const int *pbeg, const double *res_fid,
                                                       const double *res_abs,
@@ -292,6 +285,8 @@ struct bp_context {
     double *d_zv = nullptr, *d_ze = nullptr, *d_lre = nullptr, *d_lim = nullptr;
     int *d_done = nullptr, *d_iter = nullptr, *d_at = nullptr;
     double *d_delta = nullptr;
+    unsigned long long *d_fold_acc = nullptr;  // per network (folded residual reduction)
+    int fold_n = 0;                             // networks d_fold_acc holds
     void *d_ws = nullptr;
     size_t ws_bytes = 0;
 
@@ -377,7 +372,7 @@ static void free_all(bp_context *c) {
                     c->d_msg_off, c->d_sites64, c->d_sites, c->d_msg[0], c->d_msg[1],
                     c->d_msg64[0], c->d_msg64[1], c->d_res_fid, c->d_res_abs, c->d_zv, c->d_ze,
                     c->d_lre, c->d_lim, c->d_delta, c->d_ws,
-                    c->d_vsites, c->d_sites_bf, c->d_runp};
+                    c->d_vsites, c->d_sites_bf, c->d_runp, c->d_fold_acc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->adj.release();
@@ -632,6 +627,14 @@ static int prepare_sweep(bp_context *c) {
     if (path < 0) return BP_OK;  // reported by sweep_once
     RETCHK(gen_set_lds_limits());
     RETCHK(build_buckets(c, path));
+    if (c->vplan_ok && c->fold_n != c->nnet) {
+        // folded residual accumulators (outside any graph capture): zero, and every folded
+        // sweep's bookkeeping kernel leaves them at zero
+        const size_t nb = sizeof(unsigned long long) * FOLD_SLOTS * (size_t)c->nnet;
+        RETCHK(dev_alloc((void **)&c->d_fold_acc, nb));
+        HIPCHK(hipMemset(c->d_fold_acc, 0, nb));
+        c->fold_n = c->nnet;
+    }
     if (path != BP_PATH_VERTEX)
         RETCHK(ensure_bucket_ws(c, c->dtype == BP_F64 ? sizeof(double) : sizeof(float)));
     return BP_OK;
@@ -643,6 +646,40 @@ static NetCtl make_ctl(bp_context *c, bool enabled) {
     k.done = c->d_done;
     k.enabled = enabled ? 1 : 0;
     return k;
+}
+
+// The same bookkeeping after a folded sweep: the per-network maxima are already in acc
+// (FOLD_SLOTS bit patterns of values >= 0 or NaN per network, which order like the values);
+// one thread per network, the slots reset for the next sweep.
+__global__ void k_reduce_folded(unsigned long long *acc, const RunParams *rp, int nnet,
+                                int *done, int *iter, double *delta, int *at) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nnet) return;
+    unsigned long long b = 0ull;
+    for (int k = 0; k < FOLD_SLOTS; ++k) {
+        const unsigned long long v = acc[(long long)n * FOLD_SLOTS + k];
+        b = v > b ? v : b;
+        acc[(long long)n * FOLD_SLOTS + k] = 0ull;
+    }
+    if (done[n]) return;
+    const double m = __longlong_as_double((long long)b);
+    const int it = iter[n] + 1;
+    iter[n] = it;
+    delta[n] = m;
+    if (rp->tol >= 0.0 && m < rp->tol) {
+        done[n] = 1;
+        at[n] = it;
+    } else if (rp->maxiter > 0 && it >= rp->maxiter) {
+        done[n] = 1;
+    }
+}
+
+static int launch_reduce_folded(bp_context *c) {
+    hipLaunchKernelGGL(k_reduce_folded, dim3((c->nnet + 63) / 64), dim3(64), 0, c->stream,
+                       c->d_fold_acc, (const RunParams *)c->d_runp, c->nnet, c->d_done,
+                       c->d_iter, c->d_delta, c->d_at);
+    HIPCHK(hipGetLastError());
+    return BP_OK;
 }
 
 // Per-network bookkeeping after a sweep: one workgroup of 1024 threads per network.
@@ -661,16 +698,33 @@ static const float *generic_sites32(const bp_context *c) {
 }
 
 // One Jacobi sweep on the working dtype: reads d_msg[cur], writes d_msg[cur^1].
-static int sweep_once(bp_context *c, const NetCtl &done) {
+// Whether the residual reduction can ride on the sweep kernel: the D <= 8 four-port kernel
+// alone updates every message of every network.
+static bool fold_possible(const bp_context *c) {
+    return c->vplan_ok && c->vplan.all && c->vplan.nv_k3 == 0 && c->vplan.nv16 == 0 &&
+           c->path != BP_PATH_GENERIC && c->d_fold_acc && c->fold_n == c->nnet &&
+           c->vplan.nv > 0;
+}
+
+// One Jacobi sweep; with `folded` non-null and the bookkeeping foldable, the sweep kernel
+// also performs it (*folded = true) and no separate bookkeeping launch is needed.
+static int sweep_once(bp_context *c, const NetCtl &done, bool *folded = nullptr) {
     const int path = resolve_path(c);
     if (path < 0) return set_err(BP_ERR_UNSUPPORTED, "vertex path not available: %s",
                                  c->vplan.why.c_str());
     if (c->gb_path != path)  // normally done by prepare_sweep, outside any graph capture
         RETCHK(prepare_sweep(c));
     void *src = c->d_msg[c->cur], *dst = c->d_msg[c->cur ^ 1];
+    SweepFold fold{};
+    if (folded) *folded = false;
+    if (folded && done.enabled && path == BP_PATH_VERTEX && fold_possible(c)) {
+        fold.acc = c->d_fold_acc;
+        fold.rp = c->d_runp;
+        *folded = true;
+    }
     if (path == BP_PATH_VERTEX || path == BP_PATH_MIXED) {
         RETCHK(vertex_sweep(&c->vplan, c->stream, c->topo(), c->d_vsites, src, dst, done,
-                            c->d_res_fid, c->d_res_abs));
+                            c->d_res_fid, c->d_res_abs, fold));
         RETCHK(vertex_k3_sweep(&c->vplan, c->stream, c->topo(), src, dst, done,
                                c->d_res_fid, c->d_res_abs));
         RETCHK(vertex16_sweep(&c->vplan, c->stream, c->topo(), src, dst, done, c->d_res_fid,
@@ -1124,8 +1178,9 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
                 HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                 int rc = BP_OK;
                 for (int i = 0; i < n && rc == BP_OK; ++i) {
-                    rc = sweep_once(c, ctl);
-                    if (rc == BP_OK) rc = launch_reduce(c);
+                    bool folded = false;
+                    rc = sweep_once(c, ctl, &folded);
+                    if (rc == BP_OK) rc = folded ? launch_reduce_folded(c) : launch_reduce(c);
                 }
                 hipError_t e = hipStreamEndCapture(c->stream, &graph);
                 if (rc != BP_OK) return rc;
@@ -1145,8 +1200,9 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
             if (n % 2) c->cur ^= 1;
         } else {
             for (int i = 0; i < n; ++i) {
-                RETCHK(sweep_once(c, ctl));
-                RETCHK(launch_reduce(c));
+                bool folded = false;
+                RETCHK(sweep_once(c, ctl, &folded));
+                RETCHK(folded ? launch_reduce_folded(c) : launch_reduce(c));
             }
         }
         launched += n;

@@ -35,6 +35,33 @@ struct NetCtl {
     int enabled;   // 0: plain sweep, every network updated
 };
 
+// Stopping parameters of a run, read from device memory so that a captured sweep sequence
+// serves every (maxiter, tol, criterion).
+struct RunParams {
+    int maxiter;
+    int crit;  // 0: fidelity, 1: max-abs (BP_CRIT_*)
+    double tol;
+};
+
+// The residual reduction of the per-network convergence bookkeeping, folded into a sweep
+// kernel that covers every message of every network (acc == nullptr: off): each work item
+// folds its largest residual under the run's criterion into one of its network's
+// FOLD_SLOTS accumulators with one atomic max (no fence: the kernel's end orders it before
+// the next launch; the slots keep the atomics of one network from serialising on one
+// address), and a one-thread-per-network kernel then applies the stopping rule and resets
+// them, instead of reducing every message slot's residual.
+#define FOLD_SLOTS 32  // accumulators per network (work item u folds into slot u % 32)
+struct SweepFold {
+    unsigned long long *acc;  // per network, FOLD_SLOTS bit patterns of largest residuals
+    const RunParams *rp;
+};
+
+// max that keeps a NaN and ignores values below zero (a fidelity residual can round to
+// -1e-16): the rule of the per-slot reduction.  Values >= 0 or NaN order like their bits.
+__device__ __forceinline__ double fold_max(double m, double r) {
+    return (r > m || r != r) ? r : m;
+}
+
 // Ordering of LDS traffic within one wave: lane A's store is seen by lane B's later load
 // once the wave's LDS operations have completed (s_waitcnt lgkmcnt(0)); the empty asm keeps
 // the compiler from moving LDS accesses across.  Used instead of __syncthreads where the

@@ -521,8 +521,9 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
                  float *__restrict__ mnew, const int4 *__restrict__ vin,
                  const int4 *__restrict__ vout, const int4 *__restrict__ vdim,
                  const int *__restrict__ vnet, const long long *__restrict__ msg_off, NetCtl ctl,
-                 double *__restrict__ res_fid, double *__restrict__ res_abs) {
+                 double *__restrict__ res_fid, double *__restrict__ res_abs, SweepFold fold) {
     extern __shared__ __align__(16) float vsm[];
+    __shared__ double fold_m[4];
     const int u = blockIdx.x;  // work item: the u-th vertex of the bucket
     const int w = threadIdx.x >> 6;     // wave = physical index s
     const int lane = threadIdx.x & 63;
@@ -620,6 +621,8 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
     out[1] = vx_final<false, true, BF>(G, bufA);
 
     // ---- sum over s, normalise, residual, write
+    double fm = 0.0;  // this wave's largest residual under the run's criterion (lane 0)
+    const bool fcrit = fold.acc && fold.rp->crit != 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) part[(w * 4 + j) * 64 + lane] = out[j];
     __syncthreads();
@@ -645,6 +648,17 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
             const double c = dn / sqrt(nn * oo);
             res_fid[q] = 1.0 - c * c;
             res_abs[q] = mx;
+            fm = fold_max(fm, fcrit ? mx : 1.0 - c * c);
+        }
+    }
+    if (fold.acc) {
+        if (lane == 0) fold_m[w] = fm;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = 0.0;
+            for (int s = 0; s < NS; ++s) m = fold_max(m, fold_m[s]);
+            atomicMax(fold.acc + (long long)net * FOLD_SLOTS + (u % FOLD_SLOTS),
+                      (unsigned long long)__double_as_longlong(m));
         }
     }
 }
@@ -662,8 +676,10 @@ void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__
                     float *__restrict__ mnew, const int4 *__restrict__ vin,
                     const int4 *__restrict__ vout, const int4 *__restrict__ vdim,
                     const int *__restrict__ vnet, const long long *__restrict__ msg_off,
-                    NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs) {
+                    NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs,
+                    SweepFold fold) {
     extern __shared__ __align__(16) float vsm[];
+    __shared__ double fold_m[4];
     const int u = blockIdx.x;
     const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -742,6 +758,8 @@ void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__
 
     // ---- sum over s, normalise, residual, write (as k_vertex_d8); the slot data are read
     // again here rather than kept live through the contractions
+    double fm = 0.0;
+    const bool fcrit = fold.acc && fold.rp->crit != 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) part[(w * 4 + j) * 64 + lane] = out[j];
     __syncthreads();
@@ -770,6 +788,17 @@ void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__
             const double c = dn / sqrt(nn * oo);
             res_fid[q] = 1.0 - c * c;
             res_abs[q] = mx;
+            fm = fold_max(fm, fcrit ? mx : 1.0 - c * c);
+        }
+    }
+    if (fold.acc) {
+        if (lane == 0) fold_m[w] = fm;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = 0.0;
+            for (int s = 0; s < NS; ++s) m = fold_max(m, fold_m[s]);
+            atomicMax(fold.acc + (long long)net * FOLD_SLOTS + (u % FOLD_SLOTS),
+                      (unsigned long long)__double_as_longlong(m));
         }
     }
 }
@@ -780,7 +809,7 @@ static size_t vertex_lo_lds(int ns) {
 
 static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *vsites,
                         const void *src, void *dst, const NetCtl &ctl, double *res_fid,
-                        double *res_abs) {
+                        double *res_abs, const SweepFold &fold) {
     (void)tp;
     const int n4 = p->nv - p->nv_k3;  // items on the four-port kernels
     if (n4 == 0) return 0;
@@ -791,7 +820,7 @@ static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *v
     case NS:                                                                                   \
         hipLaunchKernelGGL((k_vertex_d8_lo<NS>), dim3(n4), dim3(64 * NS), lds, st, vsites,     \
                            (const float *)src, (float *)dst, p->d_vin, p->d_vout, p->d_vdim,   \
-                           p->d_vnet, tp.msg_off, ctl, res_fid, res_abs);                      \
+                           p->d_vnet, tp.msg_off, ctl, res_fid, res_abs, fold);                \
         break;
             VX_LO(1)
             VX_LO(2)
@@ -809,7 +838,7 @@ static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *v
     case NS * 2 + BF:                                                                        \
         hipLaunchKernelGGL((k_vertex_d8<NS, BF>), dim3(n4), dim3(64 * NS), p->lds_bytes,     \
                            st, vsites, (const float *)src, (float *)dst, p->d_vin, p->d_vout, \
-                           p->d_vdim, p->d_vnet, tp.msg_off, ctl, res_fid, res_abs);         \
+                           p->d_vdim, p->d_vnet, tp.msg_off, ctl, res_fid, res_abs, fold);   \
         break;
         VX_CASE(1, 0)
         VX_CASE(2, 0)
