@@ -286,6 +286,8 @@ struct bp_context {
     int *d_done = nullptr, *d_iter = nullptr, *d_at = nullptr;
     double *d_delta = nullptr;
     unsigned long long *d_fold_acc = nullptr;  // per network (folded residual reduction)
+    char *h_state = nullptr;                    // pinned copy of the run state (bp_run)
+    size_t h_state_bytes = 0;
     int fold_n = 0;                             // networks d_fold_acc holds
     void *d_ws = nullptr;
     size_t ws_bytes = 0;
@@ -821,6 +823,7 @@ int bp_destroy(bp_context *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     free_all(c);
+    if (c->h_state) (void)hipHostFree(c->h_state);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -1161,10 +1164,20 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
     const NetCtl ctl = make_ctl(c, true);
     HIPCHK(hipGetLastError());
     const int chunk = 16;  // sweeps between host checks of the done flags (even)
-    // run state copied back after every chunk: delta, done, iter, at (one block)
+    // The run state (delta, done, iter, at: one block) comes back through a pinned buffer.
+    // With a tolerance the host reads it after every chunk and stops once every network has
+    // stopped; without one no network stops before maxiter, so every chunk is queued at once
+    // and the state is read once at the end (no idle gaps between chunks).
     const int nn = c->nnet;
-    std::vector<char> st(run_state_bytes(nn));
-    const int *done = (const int *)(st.data() + sizeof(double) * nn);
+    const size_t sbytes = run_state_bytes(nn);
+    if (c->h_state_bytes < sbytes) {
+        if (c->h_state) (void)hipHostFree(c->h_state);
+        c->h_state = nullptr;
+        c->h_state_bytes = 0;
+        HIPCHK(hipHostMalloc((void **)&c->h_state, sbytes, hipHostMallocDefault));
+        c->h_state_bytes = sbytes;
+    }
+    const bool check_chunks = tol >= 0.0;
     if (c->graph_epoch != g_epoch.load()) graphs_release(c);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     int launched = 0;
@@ -1206,21 +1219,24 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
             }
         }
         launched += n;
-        HIPCHK(hipEventRecord(c->ev1, c->stream));  // the last record ends the timed span
-        HIPCHK(hipMemcpyAsync(st.data(), c->d_delta, st.size(), hipMemcpyDeviceToHost,
-                              c->stream));
+        if (!check_chunks && launched < cap) continue;
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_state, c->d_delta, sbytes, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        const int *done = (const int *)(c->h_state + sizeof(double) * nn);
         bool all = true;
         for (int i = 0; i < nn; ++i) all = all && done[i];
         if (all) break;
     }
-    if (launched == 0) HIPCHK(hipEventRecord(c->ev1, c->stream));
-    HIPCHK(hipEventSynchronize(c->ev1));
+    if (launched == 0) {
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        HIPCHK(hipMemcpyAsync(c->h_state, c->d_delta, sbytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
-    if (launched == 0)
-        HIPCHK(hipMemcpy(st.data(), c->d_delta, st.size(), hipMemcpyDeviceToHost));
+    std::vector<char> st(c->h_state, c->h_state + sbytes);
     const char *p = st.data();
     if (out_delta) std::memcpy(out_delta, p, sizeof(double) * nn);
     p += sizeof(double) * nn + sizeof(int) * nn;  // past delta and done
