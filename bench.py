@@ -31,8 +31,13 @@ sys.path.insert(0, ROOT)
 # HBM bytes per launch of the dominant kernel, measured with two rocprofv3 --pmc passes
 # (FETCH_SIZE, WRITE_SIZE) over this bench and scaled by a 256 MiB device copy of known
 # traffic (fetch x2.00, write x1.00); the summary and its commands are in
-# profiles/k_vertex_d8_32x32_D8_fp32_hbm_traffic.md.  Keyed by (path, dtype, L, D).
-MEASURED_TRAFFIC = {("vertex", "f32", 32, 8): 37.32e6}
+# profiles/k_vertex_d8*_32x32_D8_fp32_hbm_traffic.md.  Keyed by (path, dtype, L, D, FP32
+# vertex kernel variant).
+MEASURED_TRAFFIC = {("vertex", "f32", 32, 8, 0): 37.54e6, ("vertex", "f32", 32, 8, 1): 41.22e6}
+# the sweep kernel that dominates each configuration (NS: the physical dimension)
+SWEEP_KERNEL = {("vertex", "f32", 0): "k_vertex_d8<NS, false>",
+                ("vertex", "f32", 1): "k_vertex_d8_lo<NS>",
+                ("vertex", "bf16", 0): "k_vertex_d8<NS, true>"}
 
 PEAK = {  # MI355X published dense peaks
     "hbm_GBps": 8000.0,
@@ -412,6 +417,9 @@ def main(argv=None):
                          "workload; local_energy: only the expectation-value / local-energy "
                          "timings; contract: only the exact-contraction timings; gates: only "
                          "the gate-layer timing; print the timings, for profiling")
+    ap.add_argument("--vertex-variant", type=int, choices=[0, 1], default=None,
+                    help="FP32 vertex sweep kernel to run instead of the tuned one (0: two LDS "
+                         "buffers per wave, 1: one buffer and two waves per SIMD)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, used to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -480,6 +488,10 @@ def main(argv=None):
     ntuned = eng.autotune(mode="all")  # kernel choices for this shape, outside the timed region
     t_tune = time.perf_counter() - t_tune
     path = eng.path()
+    fp32_vertex = path == "vertex" and args.dtype == "f32"
+    if fp32_vertex and args.vertex_variant is not None:
+        eng.set_option("vertex_variant", args.vertex_variant)
+    variant = eng.get_option("vertex_variant") if fp32_vertex else 0
     # warmup
     if args.warmup > 0:
         eng.run(maxiter=args.warmup, use_graph=True)
@@ -510,11 +522,12 @@ def main(argv=None):
         achieved = flops / (kms * 1e-3) / 1e12
     roofline = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
                 "frac": round(achieved / peak, 4),
-                "traffic": MEASURED_TRAFFIC.get((path, args.dtype, L, D)),
+                "traffic": MEASURED_TRAFFIC.get((path, args.dtype, L, D, variant)),
                 "kernel_ms": round(kms, 5), "algorithmic_flops_per_launch": flops,
                 "algorithmic_bytes_per_launch": byts}
 
-    extras = {"path": path, "device_ms_per_step": round(res.ms / args.steps, 5),
+    extras = {"path": path, "sweep_kernel": SWEEP_KERNEL.get((path, args.dtype, variant)),
+              "device_ms_per_step": round(res.ms / args.steps, 5),
               "autotune": {"decisions": ntuned, "seconds": round(t_tune, 3)}}
     if path == "vertex" and args.dtype == "f32":
         # both FP32 sweep kernels, timed the same way (the run above used the tuned one)

@@ -218,6 +218,9 @@ int bp_local_energy_device(bp_context *ctx, const double *ops_v, const double *o
  *                      vertex kernels (D 1..12 in FP32, 1..16 in BF16; default 0x1FFFE).
  * No reference counterpart. */
 int bp_set_option(bp_context *ctx, const char *name, int value);
+/* The current value of an option above (after bp_autotune: the tuned choice).  No reference
+ * counterpart. */
+int bp_get_option(bp_context *ctx, const char *name, int *value);
 
 /* Autotuner.  Times, on the context's own sites and messages, (a) for every largest bond
  * dimension D present that the vertex kernels take (<= 12 FP32, <= 16 BF16; norm networks,

@@ -62,6 +62,7 @@ SIGNATURES = {
                                          C.c_int, _ip, _dp]),
     "bp_autotune": (C.c_int, [_vp, C.c_char_p, C.c_int, _ip]),
     "bp_set_option": (C.c_int, [_vp, C.c_char_p, C.c_int]),
+    "bp_get_option": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_int)]),
     "bp_last_run_ms": (C.c_int, [_vp, _dp]),
     "bp_time_sweeps": (C.c_int, [_vp, C.c_int, _dp]),
     "bp_cn_plan_create": (C.c_int, [_vp, C.c_int, _ip, _ip, C.c_int, _ip, _ip, C.c_int,

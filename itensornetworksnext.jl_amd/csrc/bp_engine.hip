@@ -1114,6 +1114,14 @@ int bp_set_option(bp_context *c, const char *name, int value) {
     return set_err(BP_ERR_ARG, "unknown option %s", name);
 }
 
+int bp_get_option(bp_context *c, const char *name, int *value) {
+    if (!c || !name || !value) return set_err(BP_ERR_ARG, "NULL argument");
+    if (strcmp(name, "vertex_variant") == 0) { *value = c->vx_variant; return BP_OK; }
+    if (strcmp(name, "vertex16_waves") == 0) { *value = c->v16_waves; return BP_OK; }
+    if (strcmp(name, "vertex_bond_mask") == 0) { *value = (int)c->vx_mask; return BP_OK; }
+    return set_err(BP_ERR_ARG, "unknown option %s", name);
+}
+
 int bp_get_path(bp_context *c, int *path) {
     if (!c || !path) return set_err(BP_ERR_ARG, "NULL argument");
     *path = resolve_path(c);

@@ -497,6 +497,59 @@ __device__ __forceinline__ float vx_final_acc(const float (&G)[64], const float 
     return vx_reduce64(acc);
 }
 
+// The open contraction in two halves of 32 accumulators (a < 4, then a >= 4), each reduced
+// over the wave: entry (a, a') of half h ends on lanes 32h + (a - 4h) * 8 + a'.  Half the
+// registers of vx_final_acc for twice the reads of the bra tensor.
+template <bool HI, bool XL, int PITCH>
+__device__ __forceinline__ float vx_final_half(const float (&G)[64], const float *bufA) {
+    const int lane = threadIdx.x & 63;
+    float res = 0.f;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        float acc[32];
+#pragma unroll
+        for (int i = 0; i < 32; ++i) acc[i] = 0.f;
+#pragma unroll
+        for (int r = 0; r < 64; ++r) {
+            const float av = XL ? bufA[r * PITCH + lane] : bufA[lane * PITCH + r];
+            const int hi = r >> 3, lo = r & 7;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int ag = 4 * h + a;
+                if (HI) acc[a * 8 + hi] += G[ag * 8 + lo] * av;
+                else acc[a * 8 + lo] += G[hi * 8 + ag] * av;
+            }
+        }
+        // sum over lane bit 5 (both halves keep the total), then halve over bits 4..0
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc[i]),
+                                                            __float_as_uint(acc[i]), false, false);
+            acc[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(acc[i]),
+                                                            __float_as_uint(acc[i + 16]), false,
+                                                            false);
+            acc[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+        }
+#pragma unroll
+        for (int s = 8; s >= 1; s >>= 1) {
+            const bool up = (lane & s) != 0;
+#pragma unroll
+            for (int i = 0; i < s; ++i) {
+                const float lo = acc[i], hi = acc[i + s];
+                const float send = up ? lo : hi;
+                const float keep = up ? hi : lo;
+                acc[i] = keep + __shfl_xor(send, s, 64);
+            }
+        }
+        if ((lane >> 5) == h) res = acc[0];
+    }
+    return res;
+}
+
 template <bool HI, bool XL, bool BF, int PITCH = VX_PITCH>
 __device__ __forceinline__ float vx_final(const float (&Gin)[64], const float *bufA) {
     if constexpr (BF) {
@@ -668,8 +721,10 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
 // The same sweep with one LDS buffer per wave instead of two (36 KB per two-wave workgroup,
 // so four workgroups fit a CU) and a 256-register budget, i.e. two waves per SIMD: the
 // buffer first carries the E01 transpose, then a copy of A for phase A's bra (layout Y) and
-// phase B's ket (layout Y), then the E23 transpose; phase B's bra (layout X) is read from
-// the site array itself, one coalesced 256-byte row per register, mostly from L2.
+// phase B's ket (layout Y), then the E23 transpose, then A again (kept in registers across
+// phase B's mode products) for phase B's bra (layout X).  The open contractions accumulate
+// in two halves of 32 registers (vx_final_half) and each result goes to LDS as soon as it
+// is made, which keeps the kernel within the budget (a few values spill, once per wave).
 template <int NS>
 __global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__ mold,
@@ -721,47 +776,60 @@ void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__
 #pragma unroll
     for (int r = 0; r < 64; ++r) T[r] = src[r * 64 + lane];
     __syncthreads();
+    // results go to LDS as they are made (fewer live registers; with one wave per
+    // workgroup the compiler allocates best when they stay in registers to the end)
+    float *mypart = part + w * 4 * 64 + lane;
     float out[4];
+    auto put = [&](int j, float v) {
+        if constexpr (NS == 1) out[j] = v; else mypart[j * 64] = v;
+    };
 
-    // ---- phase A: E01 (X) -> Y through the buffer; then A into the buffer for the bra
+    // ---- phase A: E01 (X) -> Y through the buffer; then A into the buffer for the bra.
+    // The buffer belongs to this wave: wave-level LDS ordering suffices between its uses.
     vx_mode<true, false>(T, E, msm + 0 * 64);
     vx_mode<false, false>(E, G, msm + 1 * 64);
 #pragma unroll
     for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = G[r];
-    __syncthreads();
+    wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < 64; ++r) E[r] = buf[lane * VX_PITCH + r];  // E01 in Y
-    __syncthreads();
+    wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = T[r];  // A, readable as X or Y
-    __syncthreads();
+    wave_lds_sync();
     vx_mode<false, false>(E, G, msm + 3 * 64);
-    out[2] = vx_final<true, false, false>(G, buf);
+    put(2, vx_final_half<true, false, VX_PITCH>(G, buf));
     vx_mode<true, false>(E, G, msm + 2 * 64);
-    out[3] = vx_final<false, false, false>(G, buf);
+    put(3, vx_final_half<false, false, VX_PITCH>(G, buf));
 
-    // ---- phase B: E23 (Y) -> X through the buffer; bra in X from the site array
+    // ---- phase B: E23 (Y) -> X through the buffer, then A written back for the bra (X)
 #pragma unroll
     for (int r = 0; r < 64; ++r) T[r] = buf[lane * VX_PITCH + r];  // A in Y
     vx_mode<true, false>(T, E, msm + 2 * 64);
     vx_mode<false, false>(E, G, msm + 3 * 64);
-    __syncthreads();  // every lane has finished reading A from the buffer
+    wave_lds_sync();  // every lane has finished reading A from the buffer
 #pragma unroll
     for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = G[r];
-    __syncthreads();
+    wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < 64; ++r) E[r] = buf[r * VX_PITCH + lane];  // E23 in X
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = T[r];  // A again
+    wave_lds_sync();
     vx_mode<false, false>(E, G, msm + 1 * 64);
-    out[0] = vx_final<true, true, false, 64>(G, src);
+    put(0, vx_final_half<true, true, VX_PITCH>(G, buf));
     vx_mode<true, false>(E, G, msm + 0 * 64);
-    out[1] = vx_final<false, true, false, 64>(G, src);
+    put(1, vx_final_half<false, true, VX_PITCH>(G, buf));
 
     // ---- sum over s, normalise, residual, write (as k_vertex_d8); the slot data are read
     // again here rather than kept live through the contractions
     double fm = 0.0;
     const bool fcrit = fold.acc && fold.rp->crit != 0;
+    if constexpr (NS == 1) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) part[(w * 4 + j) * 64 + lane] = out[j];
+        for (int j = 0; j < 4; ++j) mypart[j * 64] = out[j];
+    }
     __syncthreads();
     const int4 vo2 = vout[u], vd2 = vdim[u];
     const int outs2[4] = {vo2.x, vo2.y, vo2.z, vo2.w};

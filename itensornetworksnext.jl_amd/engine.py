@@ -272,6 +272,13 @@ class Engine:
         kernels)."""
         L.check(self.lib.bp_set_option(self.h, name.encode(), int(value)), "bp_set_option")
 
+    def get_option(self, name: str) -> int:
+        """The current value of a kernel option (bp_get_option): after autotune, the tuned
+        choice."""
+        v = C.c_int(0)
+        L.check(self.lib.bp_get_option(self.h, name.encode(), C.byref(v)), "bp_get_option")
+        return v.value
+
     def autotune(self, cache: str | None = None, mode: str = "missing") -> int:
         """Time the kernel choices on this engine's data and apply the fastest (see
         bp_autotune in include/bp_mi355x.h).  cache: JSON file of earlier decisions (read,

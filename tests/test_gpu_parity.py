@@ -456,6 +456,20 @@ def test_vertex_low_lds_variant(pkg, orc, shape, D, d):
     assert _rel(res[1][1], res[0][1]) < 1e-5
 
 
+def test_kernel_options_round_trip(pkg):
+    """bp_get_option reads back what bp_set_option set, and rejects unknown names."""
+    _say("case kernel options round trip")
+    t = pkg.build_topology(pkg.named_grid((3, 3)), 8, 2)
+    s = pkg.peps.random_sites(t, 44, scale=pkg.peps.peps_scale(2, 8))
+    with pkg.Engine(t, s, dtype="f32") as e:
+        for name, value in (("vertex_variant", 1), ("vertex_variant", 0),
+                            ("vertex16_waves", 8), ("vertex_bond_mask", 0x1FE)):
+            e.set_option(name, value)
+            assert e.get_option(name) == value
+        with pytest.raises(pkg._lib.BPError, match="unknown option"):
+            e.get_option("no_such_option")
+
+
 # ---------------------------------------------------------------- bond dimensions 9..12 (FP32)
 
 @pytest.mark.parametrize("shape,D,d", [((4, 3), 12, 2), ((3, 3), 10, 2), ((3, 4), 9, 1),
