@@ -90,6 +90,72 @@ __device__ __forceinline__ T wave_max(T v) {
     return v;
 }
 
+// Exchanges inside 16-lane rows through DPP (a VALU operand modifier: no LDS round trip).
+// row_partner<s>(v) returns v of a lane whose index differs from this lane's in bit s and
+// possibly in lower bits only: l ^ 1 and l ^ 2 (quad permutations), l ^ 7 (row_half_mirror),
+// l ^ 15 (row_mirror).  A halving exchange that takes its steps in the order s = 8, 4, 2, 1
+// over such partners still sums every lane of the row into every entry (each step pairs
+// lanes that hold the same entries), and lane l ends with entry l % 16.
+template <int S>
+__device__ __forceinline__ float row_partner(float v) {
+    static_assert(S == 1 || S == 2 || S == 4 || S == 8, "row partner distance");
+    constexpr int ctrl = S == 1 ? 0xB1 : S == 2 ? 0x4E : S == 4 ? 0x141 : 0x140;
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, false));
+}
+
+template <int S>
+__device__ __forceinline__ double row_partner(double v) {
+    constexpr int ctrl = S == 1 ? 0xB1 : S == 2 ? 0x4E : S == 4 ? 0x141 : 0x140;
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), ctrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Both members of a pair of 16-lane rows (rows 0 and 1, rows 2 and 3) or of the two 32-lane
+// halves, through v_permlane16_swap / v_permlane32_swap (VALU): with both operands v, the
+// swap returns v of the pair's first member in [0] and of its second in [1], on every lane.
+struct DPair { double first, second; };
+
+__device__ __forceinline__ DPair rows16_pair(double v) {
+    const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+    const auto a = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    return {__hiloint2double((int)b[0], (int)a[0]), __hiloint2double((int)b[1], (int)a[1])};
+}
+
+__device__ __forceinline__ DPair halves32_pair(double v) {
+    const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+    const auto a = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    return {__hiloint2double((int)b[0], (int)a[0]), __hiloint2double((int)b[1], (int)a[1])};
+}
+
+// wave64 sum and NaN-propagating max of a double on the VALU alone (DPP inside rows, then
+// the row and half swaps); every lane gets the same result.
+__device__ __forceinline__ double wave_sum_v(double v) {
+    v += row_partner<1>(v);
+    v += row_partner<2>(v);
+    v += row_partner<4>(v);
+    v += row_partner<8>(v);
+    const DPair r = rows16_pair(v);
+    v = r.first + r.second;
+    const DPair h = halves32_pair(v);
+    return h.first + h.second;
+}
+
+__device__ __forceinline__ double vmax_nan(double v, double w) { return (w > v || w != w) ? w : v; }
+
+__device__ __forceinline__ double wave_max_v(double v) {
+    v = vmax_nan(v, row_partner<1>(v));
+    v = vmax_nan(v, row_partner<2>(v));
+    v = vmax_nan(v, row_partner<4>(v));
+    v = vmax_nan(v, row_partner<8>(v));
+    const DPair r = rows16_pair(v);
+    v = vmax_nan(r.first, r.second);
+    const DPair h = halves32_pair(v);
+    return vmax_nan(h.first, h.second);
+}
+
 // block-wide sum of a double over BP_BLOCK threads; `red` is LDS of >= BP_BLOCK/64 doubles.
 __device__ __forceinline__ double block_sum(double v, double *red) {
     v = wave_sum(v);

@@ -439,11 +439,25 @@ __device__ __forceinline__ void vx_mode(const float (&T)[64], float (&O)[64], co
     }
 }
 
+// One step of a halving exchange inside 16-lane rows: lane l keeps entries i + S * (bit S
+// of l) for i < S and adds the same entries from its DPP partner (row_partner<S>).
+template <int S, int N>
+__device__ __forceinline__ void vx_halve(float (&v)[N], int lane) {
+    const bool up = (lane & S) != 0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        const float lo = v[i], hi = v[i + S];
+        const float send = up ? lo : hi;
+        const float keep = up ? hi : lo;
+        v[i] = keep + row_partner<S>(send);
+    }
+}
+
 // Sum v[i] over the 64 lanes for every i; lane l returns entry l (halving exchange).
 // Step 1 pairs lane l with l^32 through v_permlane32_swap (lanes 32-63 of the first
 // operand trade places with lanes 0-31 of the second, so x' + y' holds entry i on the
 // low half and entry i+32 on the high half); step 2 does the same for 16-lane rows with
-// v_permlane16_swap; the last four steps exchange through ds_bpermute.  Values are
+// v_permlane16_swap; the last four steps exchange inside 16-lane rows through DPP.  Values are
 // selected after they are loaded so the array stays in registers.
 __device__ __forceinline__ float vx_reduce64(float (&v)[64]) {
     const int lane = threadIdx.x & 63;
@@ -459,17 +473,10 @@ __device__ __forceinline__ float vx_reduce64(float (&v)[64]) {
                                                         __float_as_uint(v[i + 16]), false, false);
         v[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
     }
-#pragma unroll
-    for (int s = 8; s >= 1; s >>= 1) {
-        const bool up = (lane & s) != 0;
-#pragma unroll
-        for (int i = 0; i < s; ++i) {
-            const float lo = v[i], hi = v[i + s];
-            const float send = up ? lo : hi;
-            const float keep = up ? hi : lo;
-            v[i] = keep + __shfl_xor(send, s, 64);
-        }
-    }
+    vx_halve<8>(v, lane);
+    vx_halve<4>(v, lane);
+    vx_halve<2>(v, lane);
+    vx_halve<1>(v, lane);
     return v[0];
 }
 
@@ -534,17 +541,10 @@ __device__ __forceinline__ float vx_final_half(const float (&G)[64], const float
                                                             false);
             acc[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
         }
-#pragma unroll
-        for (int s = 8; s >= 1; s >>= 1) {
-            const bool up = (lane & s) != 0;
-#pragma unroll
-            for (int i = 0; i < s; ++i) {
-                const float lo = acc[i], hi = acc[i + s];
-                const float send = up ? lo : hi;
-                const float keep = up ? hi : lo;
-                acc[i] = keep + __shfl_xor(send, s, 64);
-            }
-        }
+        vx_halve<8>(acc, lane);
+        vx_halve<4>(acc, lane);
+        vx_halve<2>(acc, lane);
+        vx_halve<1>(acc, lane);
         if ((lane >> 5) == h) res = acc[0];
     }
     return res;
@@ -686,7 +686,7 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
         float x = 0.f;
 #pragma unroll
         for (int s = 0; s < NS; ++s) x += part[(s * 4 + j) * 64 + lane];
-        const double S = wave_sum((double)x);
+        const double S = wave_sum_v((double)x);
         const double xn = S != 0.0 ? (double)x / S : (double)x;
         const float xf = (float)xn;  // exactly 0 on padded entries
         const int D = dims[j];
@@ -694,9 +694,9 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
         const long long o = outo[j] + la * D + lb;
         const double xo = real ? (double)mold[o] : 0.0;
         if (real) mnew[o] = xf;
-        const double dn = wave_sum((double)xf * xo), nn = wave_sum((double)xf * (double)xf),
-                     oo = wave_sum(xo * xo);
-        const double mx = wave_max(fabs((double)xf - xo));
+        const double dn = wave_sum_v((double)xf * xo),
+                     nn = wave_sum_v((double)xf * (double)xf), oo = wave_sum_v(xo * xo);
+        const double mx = wave_max_v(fabs((double)xf - xo));
         if (lane == 0) {
             const double c = dn / sqrt(nn * oo);
             res_fid[q] = 1.0 - c * c;
@@ -841,7 +841,7 @@ void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__
         float x = 0.f;
 #pragma unroll
         for (int s = 0; s < NS; ++s) x += part[(s * 4 + j) * 64 + lane];
-        const double S = wave_sum((double)x);
+        const double S = wave_sum_v((double)x);
         const double xn = S != 0.0 ? (double)x / S : (double)x;
         const float xf = (float)xn;
         const int D = dims2[j];
@@ -849,9 +849,9 @@ void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__
         const long long o = msg_off[q] + la * D + lb;
         const double xo = real ? (double)mold[o] : 0.0;
         if (real) mnew[o] = xf;
-        const double dn = wave_sum((double)xf * xo), nn = wave_sum((double)xf * (double)xf),
-                     oo = wave_sum(xo * xo);
-        const double mx = wave_max(fabs((double)xf - xo));
+        const double dn = wave_sum_v((double)xf * xo),
+                     nn = wave_sum_v((double)xf * (double)xf), oo = wave_sum_v(xo * xo);
+        const double mx = wave_max_v(fabs((double)xf - xo));
         if (lane == 0) {
             const double c = dn / sqrt(nn * oo);
             res_fid[q] = 1.0 - c * c;

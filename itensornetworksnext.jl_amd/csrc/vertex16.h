@@ -390,7 +390,7 @@ __global__ __launch_bounds__(256) void k_v16_finish(
         x[k] = v;
         ls += (double)v;
     }
-    const double S = wave_sum(ls);
+    const double S = wave_sum_v(ls);
     double dn = 0.0, nn = 0.0, oo = 0.0, mx = 0.0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -406,10 +406,10 @@ __global__ __launch_bounds__(256) void k_v16_finish(
         const double dd = fabs((double)xf - xo);
         mx = (dd > mx || dd != dd) ? dd : mx;
     }
-    dn = wave_sum(dn);
-    nn = wave_sum(nn);
-    oo = wave_sum(oo);
-    mx = wave_max(mx);
+    dn = wave_sum_v(dn);
+    nn = wave_sum_v(nn);
+    oo = wave_sum_v(oo);
+    mx = wave_max_v(mx);
     if (lane == 0) {
         const double c = dn / sqrt(nn * oo);
         res_fid[q] = 1.0 - c * c;

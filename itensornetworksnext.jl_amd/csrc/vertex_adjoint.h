@@ -164,7 +164,7 @@ __device__ __forceinline__ void vd_load_mats(const float *msgs, const float *W, 
             float x = have ? W[sl.outo[j] + a * D + b] : 0.f;
             if (Gn) {
                 const float g = have ? Gn[sl.outo[j] + a * D + b] : 0.f;
-                const double dot = wave_sum((double)x * (double)g);
+                const double dot = wave_sum_v((double)x * (double)g);
                 const double S = sl.outs[j] >= 0 ? Sn[sl.outs[j]] : 0.0;
                 if (S != 0.0) x = (float)(((double)x - dot / S) / S);
             }
@@ -190,7 +190,7 @@ __device__ __forceinline__ double vd_adj_r(const unsigned long long *acc, int j,
     const double x = __longlong_as_double((long long)a[lane]);
     double y = __longlong_as_double((long long)a[64 + lane]);
     if (y != y) y = 0.0;
-    const double ra = wave_max(x), rr = wave_max(y);
+    const double ra = wave_max_v(x), rr = wave_max_v(y);
     return rr > 0.0 ? ra / rr : ra;
 }
 
@@ -357,7 +357,7 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
 #pragma unroll
         for (int s = 0; s < NS; ++s) x += part[(s * 4 + j) * 64 + lane];
         if (j == 0 && z_out) {
-            const double z = wave_sum((double)x * (double)msm[lane]);
+            const double z = wave_sum_v((double)x * (double)msm[lane]);
             if (lane == 0) z_out[vid] = z;
         }
         // lane = entry (a, a') of the padded 8 x 8 block; real below the bond dimension
@@ -368,7 +368,7 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
             if (sl.outs[j] < 0) continue;
             if (out && real) out[sl.outo[j] + ent] = x;
             if (S_out) {
-                const double S = wave_sum((double)x);
+                const double S = wave_sum_v((double)x);
                 if (lane == 0) S_out[sl.outs[j]] = S;
             }
         } else {
@@ -384,8 +384,8 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
         }
     }
     if (resacc) {  // uniform over the workgroup
-        rd = wave_max(rd);
-        rv = wave_max(rv);
+        rd = wave_max_v(rd);
+        rv = wave_max_v(rv);
         __syncthreads();  // part[] is free again
         double *red = reinterpret_cast<double *>(part);
         if (lane == 0) { red[2 * w] = rd; red[2 * w + 1] = rv; }
