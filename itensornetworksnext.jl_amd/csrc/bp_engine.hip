@@ -91,13 +91,18 @@ __global__ void k_round_bf16(const double *in, float *out, long long n) {
         out[i] = bf16r((float)in[i]);
 }
 
-__global__ void k_reset_nets(int nnet, int *done, int *iter, double *delta, int *at) {
+__global__ void k_reset_nets(int nnet, int *done, int *iter, double *delta, int *at,
+                             unsigned long long *fold_acc, int fold_n) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= nnet) return;
     done[n] = 0;
     iter[n] = 0;
     delta[n] = INFINITY;
     at[n] = -1;
+    if (fold_acc && n < fold_n)  // every rotating buffer of the folded bookkeeping
+        for (int k = 0; k < FOLD_BUFS * FOLD_SLOTS; ++k)
+            fold_acc[(long long)(k / FOLD_SLOTS) * fold_n * FOLD_SLOTS +
+                     (long long)n * FOLD_SLOTS + k % FOLD_SLOTS] = 0ull;
 }
 
 // Per-network convergence bookkeeping after one Jacobi sweep (one workgroup per
@@ -630,9 +635,9 @@ static int prepare_sweep(bp_context *c) {
     RETCHK(gen_set_lds_limits());
     RETCHK(build_buckets(c, path));
     if (c->vplan_ok && c->fold_n != c->nnet) {
-        // folded residual accumulators (outside any graph capture): zero, and every folded
-        // sweep's bookkeeping kernel leaves them at zero
-        const size_t nb = sizeof(unsigned long long) * FOLD_SLOTS * (size_t)c->nnet;
+        // folded residual accumulators, FOLD_BUFS rotating buffers (outside any graph
+        // capture); every run starts by clearing them (k_reset_nets)
+        const size_t nb = sizeof(unsigned long long) * FOLD_BUFS * FOLD_SLOTS * (size_t)c->nnet;
         RETCHK(dev_alloc((void **)&c->d_fold_acc, nb));
         HIPCHK(hipMemset(c->d_fold_acc, 0, nb));
         c->fold_n = c->nnet;
@@ -676,9 +681,13 @@ __global__ void k_reduce_folded(unsigned long long *acc, const RunParams *rp, in
     }
 }
 
-static int launch_reduce_folded(bp_context *c) {
+// After the last folded sweep (run index k) of a sequence: its stopping rule, and the rest
+// of the sequence's StopAfterIteration.
+static int launch_reduce_folded(bp_context *c, int k) {
+    unsigned long long *acc =
+        c->d_fold_acc + (size_t)(k % FOLD_BUFS) * FOLD_SLOTS * (size_t)c->fold_n;
     hipLaunchKernelGGL(k_reduce_folded, dim3((c->nnet + 63) / 64), dim3(64), 0, c->stream,
-                       c->d_fold_acc, (const RunParams *)c->d_runp, c->nnet, c->d_done,
+                       acc, (const RunParams *)c->d_runp, c->nnet, c->d_done,
                        c->d_iter, c->d_delta, c->d_at);
     HIPCHK(hipGetLastError());
     return BP_OK;
@@ -705,12 +714,15 @@ static const float *generic_sites32(const bp_context *c) {
 static bool fold_possible(const bp_context *c) {
     return c->vplan_ok && c->vplan.all && c->vplan.nv_k3 == 0 && c->vplan.nv16 == 0 &&
            c->path != BP_PATH_GENERIC && c->d_fold_acc && c->fold_n == c->nnet &&
-           c->vplan.nv > 0;
+           c->vplan.nv > 0 && c->vplan.d_lead;
 }
 
 // One Jacobi sweep; with `folded` non-null and the bookkeeping foldable, the sweep kernel
-// also performs it (*folded = true) and no separate bookkeeping launch is needed.
-static int sweep_once(bp_context *c, const NetCtl &done, bool *folded = nullptr) {
+// also performs it (*folded = true): sweep `k` of the run folds its residuals into buffer
+// k % FOLD_BUFS and, with `apply_prev`, first applies the stopping rule of sweep k - 1; the
+// sequence's last sweep is followed by launch_reduce_folded(c, k).
+static int sweep_once(bp_context *c, const NetCtl &done, bool *folded = nullptr, int k = 0,
+                      bool apply_prev = false) {
     const int path = resolve_path(c);
     if (path < 0) return set_err(BP_ERR_UNSUPPORTED, "vertex path not available: %s",
                                  c->vplan.why.c_str());
@@ -720,8 +732,16 @@ static int sweep_once(bp_context *c, const NetCtl &done, bool *folded = nullptr)
     SweepFold fold{};
     if (folded) *folded = false;
     if (folded && done.enabled && path == BP_PATH_VERTEX && fold_possible(c)) {
-        fold.acc = c->d_fold_acc;
+        const size_t stride = FOLD_SLOTS * (size_t)c->fold_n;
+        fold.acc = c->d_fold_acc + (size_t)(k % FOLD_BUFS) * stride;
         fold.rp = c->d_runp;
+        fold.prev = apply_prev ? c->d_fold_acc + (size_t)((k + FOLD_BUFS - 1) % FOLD_BUFS) * stride
+                               : nullptr;
+        fold.clear = c->d_fold_acc + (size_t)((k + 1) % FOLD_BUFS) * stride;
+        fold.lead = c->vplan.d_lead;
+        fold.iter = c->d_iter;
+        fold.delta = c->d_delta;
+        fold.at = c->d_at;
         *folded = true;
     }
     if (path == BP_PATH_VERTEX || path == BP_PATH_MIXED) {
@@ -1168,7 +1188,8 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
                               hipMemcpyHostToDevice, c->stream));
     }
     hipLaunchKernelGGL(k_reset_nets, dim3((c->nnet + 255) / 256), dim3(256), 0, c->stream,
-                       c->nnet, c->d_done, c->d_iter, c->d_delta, c->d_at);
+                       c->nnet, c->d_done, c->d_iter, c->d_delta, c->d_at, c->d_fold_acc,
+                       c->fold_n);
     const NetCtl ctl = make_ctl(c, true);
     HIPCHK(hipGetLastError());
     const int chunk = 16;  // sweeps between host checks of the done flags (even)
@@ -1198,11 +1219,12 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
                 const int cur0 = c->cur;
                 HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
                 int rc = BP_OK;
+                bool folded = false;
                 for (int i = 0; i < n && rc == BP_OK; ++i) {
-                    bool folded = false;
-                    rc = sweep_once(c, ctl, &folded);
-                    if (rc == BP_OK) rc = folded ? launch_reduce_folded(c) : launch_reduce(c);
+                    rc = sweep_once(c, ctl, &folded, launched + i, i > 0);
+                    if (rc == BP_OK && !folded) rc = launch_reduce(c);
                 }
+                if (rc == BP_OK && folded) rc = launch_reduce_folded(c, launched + n - 1);
                 hipError_t e = hipStreamEndCapture(c->stream, &graph);
                 if (rc != BP_OK) return rc;
                 if (e != hipSuccess) return set_err(BP_ERR_HIP, "graph capture failed: %s",
@@ -1220,11 +1242,12 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
             HIPCHK(hipGraphLaunch(gexec, c->stream));
             if (n % 2) c->cur ^= 1;
         } else {
+            bool folded = false;
             for (int i = 0; i < n; ++i) {
-                bool folded = false;
-                RETCHK(sweep_once(c, ctl, &folded));
-                RETCHK(folded ? launch_reduce_folded(c) : launch_reduce(c));
+                RETCHK(sweep_once(c, ctl, &folded, launched + i, i > 0));
+                if (!folded) RETCHK(launch_reduce(c));
             }
+            if (folded) RETCHK(launch_reduce_folded(c, launched + n - 1));
         }
         launched += n;
         if (!check_chunks && launched < cap) continue;

@@ -43,17 +43,28 @@ struct RunParams {
     double tol;
 };
 
-// The residual reduction of the per-network convergence bookkeeping, folded into a sweep
-// kernel that covers every message of every network (acc == nullptr: off): each work item
-// folds its largest residual under the run's criterion into one of its network's
-// FOLD_SLOTS accumulators with one atomic max (no fence: the kernel's end orders it before
-// the next launch; the slots keep the atomics of one network from serialising on one
-// address), and a one-thread-per-network kernel then applies the stopping rule and resets
-// them, instead of reducing every message slot's residual.
+// The per-network convergence bookkeeping, folded into a sweep kernel that covers every
+// message of every network (acc == nullptr: off).  Each work item folds its largest residual
+// under the run's criterion into one of its network's FOLD_SLOTS accumulators with one
+// atomic max (no fence: the kernel's end orders it before the next launch; the slots keep
+// the atomics of one network from serialising on one address).  The stopping rule on those
+// maxima is applied by the next sweep kernel as it starts (prev set: every work item of a
+// network reads the same maxima and reaches the same decision, and the network's lead item
+// records it), or by a one-thread-per-network kernel after the last sweep of a sequence.
+// Accumulators rotate through FOLD_BUFS buffers: sweep k folds into buffer k % FOLD_BUFS,
+// reads buffer (k - 1) % FOLD_BUFS and clears buffer (k + 1) % FOLD_BUFS, which sweep k - 2
+// read last and sweep k + 1 folds into.
 #define FOLD_SLOTS 32  // accumulators per network (work item u folds into slot u % 32)
+#define FOLD_BUFS 4    // rotating accumulator buffers (a divisor of the captured sweep count)
 struct SweepFold {
-    unsigned long long *acc;  // per network, FOLD_SLOTS bit patterns of largest residuals
+    unsigned long long *acc;         // this sweep's: per network, FOLD_SLOTS residual bits
     const RunParams *rp;
+    const unsigned long long *prev;  // the previous sweep's, its rule not yet applied (or null)
+    unsigned long long *clear;       // the next sweep's, cleared by each network's lead item
+    const int *lead;                 // per network: its lead work item
+    int *iter;                       // per network run state, written by the lead item
+    double *delta;
+    int *at;
 };
 
 // max that keeps a NaN and ignores values below zero (a fidelity residual can round to

@@ -58,6 +58,7 @@ struct VertexPlan {
     int4 *d_vdim = nullptr;     // per item: bond dimension per padded port (0: dummy)
     unsigned mask = 0x1FEu;     // bit D: networks of largest bond dimension D may join
     int *d_vnet = nullptr;      // network of each item
+    int *d_lead = nullptr;      // per network: its first item in the D <= 8 bucket
     int *d_vlist = nullptr;     // vertex of each item
     int *d_gen_slots = nullptr; // message slots outside the bucket
     std::vector<int> gen_slots; // the same on the host
@@ -107,6 +108,8 @@ static void vertex_plan_release(VertexPlan *p) {
     p->d_v16part = nullptr;
     p->nv16 = 0;
     if (p->d_vnet) (void)hipFree(p->d_vnet);
+    if (p->d_lead) (void)hipFree(p->d_lead);
+    p->d_lead = nullptr;
     if (p->d_vlist) (void)hipFree(p->d_vlist);
     if (p->d_gen_slots) (void)hipFree(p->d_gen_slots);
     p->d_vin = p->d_vout = p->d_vdim = nullptr;
@@ -235,6 +238,14 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
         hipMalloc((void **)&p->d_vk3part, sizeof(float) * (size_t)nk3 * d * 4 * 256) != hipSuccess)
         rc = 5;
     if (!rc) rc = vx_upload(&p->d_vnet, vn);
+    if (!rc && nx > 0) {
+        int nn = 0;
+        for (int u = 0; u < nv; ++u) nn = std::max(nn, vnet[u] + 1);
+        std::vector<int> lead(nn, -1);
+        for (int b = 0; b < nx; ++b)
+            if (lead[vn[b]] < 0) lead[vn[b]] = b;
+        rc = vx_upload(&p->d_lead, lead);
+    }
     if (!rc) rc = vx_upload(&p->d_vlist, vlist);
     if (!rc) rc = vx_upload(&p->d_gen_slots, gen_slots);
     if (rc) {
@@ -566,6 +577,37 @@ __device__ __forceinline__ float vx_final(const float (&Gin)[64], const float *b
     }
 }
 
+// Whether network `net` is stopped for this sweep.  With fold.prev set, the previous sweep's
+// stopping rule (StopWhenConverged on the maximum of its FOLD_SLOTS accumulators) is applied
+// first: every wave of every work item of the network computes the same decision (a racing
+// read of the lead's done flag sees 0 or 1, and either way ends stopped), and the lead item
+// records the iteration, delta, done and at.  StopAfterIteration is left to the kernel that
+// ends the sequence: no sequence runs past maxiter.  The lead item also clears fold.clear.
+__device__ __forceinline__ bool vx_fold_begin(const NetCtl &ctl, const SweepFold &fold, int net,
+                                              int u) {
+    const int lane = threadIdx.x & 63;
+    const bool lead = fold.lead && fold.lead[net] == u;
+    if (lead && fold.clear && threadIdx.x < FOLD_SLOTS)
+        fold.clear[(long long)net * FOLD_SLOTS + threadIdx.x] = 0ull;
+    const bool was = ctl.enabled && ctl.done[net];
+    if (!fold.prev) return was;
+    const unsigned long long b =
+        lane < FOLD_SLOTS ? fold.prev[(long long)net * FOLD_SLOTS + lane] : 0ull;
+    const double m = wave_max_v(__longlong_as_double((long long)b));
+    const double tol = fold.rp->tol;
+    const bool newly = !was && tol >= 0.0 && m < tol;
+    if (lead && threadIdx.x == 0 && !was) {
+        const int it = fold.iter[net] + 1;
+        fold.iter[net] = it;
+        fold.delta[net] = m;
+        if (newly) {
+            ctl.done[net] = 1;
+            fold.at[net] = it;
+        }
+    }
+    return was || newly;
+}
+
 // ---------------------------------------------------------------- the sweep kernel
 
 template <int NS, bool BF>
@@ -599,7 +641,7 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
     const int la = lane >> 3, lb = lane & 7;
 
     const int net = vnet[u];
-    if (ctl.enabled && ctl.done[net]) {  // uniform over the workgroup
+    if (vx_fold_begin(ctl, fold, net, u)) {  // uniform over the workgroup
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             if (j % NS != w || outs[j] < 0) continue;
@@ -753,7 +795,7 @@ void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__
     }
     const int la = lane >> 3, lb = lane & 7;
     const int net = vnet[u];
-    if (ctl.enabled && ctl.done[net]) {
+    if (vx_fold_begin(ctl, fold, net, u)) {  // uniform over the workgroup
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             if (j % NS != w || outs[j] < 0) continue;
