@@ -215,7 +215,10 @@ int bp_local_energy_device(bp_context *ctx, const double *ops_v, const double *o
  *   "vertex16_waves"   waves per workgroup of the BF16 kernel for bond dimensions 9..16
  *                      (4, 8 or 16; default 16);
  *   "vertex_bond_mask" bit D set: networks whose largest bond dimension is D may run on the
- *                      vertex kernels (D 1..12 in FP32, 1..16 in BF16; default 0x1FFFE).
+ *                      vertex kernels (D 1..12 in FP32, 1..16 in BF16; default 0x1FFFE);
+ *   "fold_bookkeeping" 1 (default): when one D <= 8 vertex launch covers every network, the
+ *                      sweep kernels also do the per-network convergence bookkeeping;
+ *                      0: a separate bookkeeping kernel after every sweep (same results).
  * No reference counterpart. */
 int bp_set_option(bp_context *ctx, const char *name, int value);
 /* The current value of an option above (after bp_autotune: the tuned choice).  No reference

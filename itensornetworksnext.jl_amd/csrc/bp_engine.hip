@@ -294,6 +294,7 @@ struct bp_context {
     char *h_state = nullptr;                    // pinned copy of the run state (bp_run)
     size_t h_state_bytes = 0;
     int fold_n = 0;                             // networks d_fold_acc holds
+    int fold_on = 1;                            // option "fold_bookkeeping"
     void *d_ws = nullptr;
     size_t ws_bytes = 0;
 
@@ -714,7 +715,7 @@ static const float *generic_sites32(const bp_context *c) {
 static bool fold_possible(const bp_context *c) {
     return c->vplan_ok && c->vplan.all && c->vplan.nv_k3 == 0 && c->vplan.nv16 == 0 &&
            c->path != BP_PATH_GENERIC && c->d_fold_acc && c->fold_n == c->nnet &&
-           c->vplan.nv > 0 && c->vplan.d_lead;
+           c->vplan.nv > 0 && c->vplan.d_lead && c->fold_on;
 }
 
 // One Jacobi sweep; with `folded` non-null and the bookkeeping foldable, the sweep kernel
@@ -1126,6 +1127,11 @@ int bp_set_option(bp_context *c, const char *name, int value) {
         c->vplan.v16_waves = value;
         return BP_OK;
     }
+    if (strcmp(name, "fold_bookkeeping") == 0) {
+        if (value != 0 && value != 1) return set_err(BP_ERR_ARG, "fold_bookkeeping is 0 or 1");
+        c->fold_on = value;
+        return BP_OK;
+    }
     if (strcmp(name, "vertex_bond_mask") == 0) {
         c->vx_mask = (unsigned)value & 0x1FFFEu;
         if (c->have_topo) RETCHK(rebuild_vertex_plan(c));
@@ -1139,6 +1145,7 @@ int bp_get_option(bp_context *c, const char *name, int *value) {
     if (strcmp(name, "vertex_variant") == 0) { *value = c->vx_variant; return BP_OK; }
     if (strcmp(name, "vertex16_waves") == 0) { *value = c->v16_waves; return BP_OK; }
     if (strcmp(name, "vertex_bond_mask") == 0) { *value = (int)c->vx_mask; return BP_OK; }
+    if (strcmp(name, "fold_bookkeeping") == 0) { *value = c->fold_on; return BP_OK; }
     return set_err(BP_ERR_ARG, "unknown option %s", name);
 }
 

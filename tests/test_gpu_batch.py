@@ -87,3 +87,38 @@ def test_uneven_workload_full_size(pkg, orc):
         want, _, _, _ = orc.bp(t, s, pkg.peps.identity_messages(t), maxiter=200, tol=1e-6,
                                criterion="fidelity", nthreads=8)
         assert _rel(mb[offs[i]:offs[i + 1]], want) < 1e-3
+
+
+def test_folded_bookkeeping_matches_separate_kernel(pkg):
+    """Networks of one batch that stop at different sweeps, in the middle of the 16-sweep
+    graphs: the bookkeeping folded into the D <= 8 sweep kernels (each sweep applies the
+    previous sweep's stopping rule) gives bit for bit the messages, sweep counts, deltas and
+    convergence sweeps of a separate bookkeeping kernel after every sweep."""
+    _say("case folded bookkeeping vs separate kernel")
+    tops, sites = [], []
+    for k, (L, D) in enumerate([(6, 8), (4, 8), (5, 6), (3, 4), (7, 8)]):
+        t = pkg.build_topology(pkg.named_grid((L, L)), D, 2)
+        tops.append(t)
+        sites.append(pkg.peps.random_sites(t, 90 + k, scale=pkg.peps.peps_scale(2, D)))
+    b = pkg.concat_topologies(tops)
+    runs = [dict(maxiter=300, tol=1e-9, criterion="fidelity"),
+            dict(maxiter=300, tol=1e-5, criterion="maxabs"),
+            dict(maxiter=21, tol=1e-12, criterion="fidelity")]
+    for kw in runs:
+        out = {}
+        for fold in (1, 0):
+            for graph in (True, False):
+                with pkg.Engine(b, np.concatenate(sites), dtype="f32") as e:
+                    assert e.path() == "vertex"
+                    e.set_option("fold_bookkeeping", fold)
+                    e.init_messages("identity")
+                    r = e.run(use_graph=graph, **kw)
+                    out[(fold, graph)] = (r, e.messages())
+        r1, m1 = out[(1, True)]
+        _say(f"{kw}: sweeps {r1.sweeps.tolist()} at {r1.at_iteration.tolist()}")
+        assert len(set(r1.sweeps.tolist())) > 1  # the networks stop at different sweeps
+        for key, (r, m) in out.items():
+            assert np.array_equal(r.sweeps, r1.sweeps), key
+            assert np.array_equal(r.at_iteration, r1.at_iteration), key
+            assert np.array_equal(r.delta, r1.delta), key
+            assert np.array_equal(m, m1), key
