@@ -301,12 +301,12 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
     if constexpr (HASW) {
         vx_load_m(wsm + 2 * 64, Wm);
         vd_mp2<true>(R3, M, R1, Wm, R3);
-        o[2] = vx_final<true, false, false>(R2, bufA);
-        o[3] = vx_final<false, false, false>(R3, bufA);
+        o[2] = vx_final_half<true, false, VX_PITCH>(R2, bufA);
+        o[3] = vx_final_half<false, false, VX_PITCH>(R3, bufA);
     } else {
         vd_mp1<true>(R1, M, R1);
-        o[2] = vx_final<true, false, false>(R2, bufA);
-        o[3] = vx_final<false, false, false>(R1, bufA);
+        o[2] = vx_final_half<true, false, VX_PITCH>(R2, bufA);
+        o[3] = vx_final_half<false, false, VX_PITCH>(R1, bufA);
     }
 
     // ---- phase B: E23 (and its tangent) from the ket slice in Y -> X; out0, out1
@@ -336,12 +336,12 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
     if constexpr (HASW) {
         vx_load_m(wsm + 0 * 64, Wm);
         vd_mp2<true>(R3, M, R1, Wm, R3);
-        o[0] = vx_final<true, true, false>(R2, bufA);
-        o[1] = vx_final<false, true, false>(R3, bufA);
+        o[0] = vx_final_half<true, true, VX_PITCH>(R2, bufA);
+        o[1] = vx_final_half<false, true, VX_PITCH>(R3, bufA);
     } else {
         vd_mp1<true>(R1, M, R1);
-        o[0] = vx_final<true, true, false>(R2, bufA);
-        o[1] = vx_final<false, true, false>(R1, bufA);
+        o[0] = vx_final_half<true, true, VX_PITCH>(R2, bufA);
+        o[1] = vx_final_half<false, true, VX_PITCH>(R1, bufA);
     }
 
     // ---- sum over slices, write
