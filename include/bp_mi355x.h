@@ -210,7 +210,8 @@ int bp_local_energy_device(bp_context *ctx, const double *ops_v, const double *o
                            double *out_adj_res);
 
 /* Kernel options (what the autotuner decides, settable by hand):
- *   "vertex_variant"   0: FP32 vertex sweep kernel with two LDS buffers per wave (default),
+ *   "vertex_variant"   0: D <= 8 vertex sweep kernel (FP32 or BF16) with two LDS buffers
+ *                      per wave (default),
  *                      1: one buffer per wave and two waves per SIMD;
  *   "vertex16_waves"   waves per workgroup of the BF16 kernel for bond dimensions 9..16
  *                      (4, 8 or 16; default 16);
@@ -229,7 +230,7 @@ int bp_get_option(bp_context *ctx, const char *name, int *value);
  * dimension D present that the vertex kernels take (<= 12 FP32, <= 16 BF16; norm networks,
  * automatic path) whether its networks sweep faster on the vertex kernels (bonds padded to
  * 8, 12 or 16) or on the generic kernel, and
- * (b) the FP32 vertex sweep kernel variant ("vertex_variant" of bp_set_option) and the
+ * (b) the D <= 8 vertex sweep kernel variant ("vertex_variant" of bp_set_option) and the
  * waves per workgroup of the bond-dimension-16 kernel ("vertex16_waves"), and
  * (c) for every bucket of generic-kernel slots (owner's physical dimension and largest
  * bond dimension) the workgroup size (128/256/512) and whether intermediates live in LDS

@@ -1291,8 +1291,8 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
 //   "vx|<dtype>|d<d>|D<D>": {"vertex": 0|1, "us": t_chosen, "us_alt": t_other}
 //       networks whose largest bond dimension is D <= 8 run on the vertex kernels (padded
 //       to 8) or on the generic kernel, whichever sweeps faster;
-//   "vxk|f32|d<d>": {"variant": 0|1, "us": t_chosen, "us_alt": t_other}
-//       the FP32 vertex sweep kernel: two LDS buffers per wave, or one buffer per wave
+//   "vxk|<f32|bf16>|d<d>": {"variant": 0|1, "us": t_chosen, "us_alt": t_other}
+//       the D <= 8 vertex sweep kernel: two LDS buffers per wave, or one buffer per wave
 //       with two waves per SIMD;
 //   "v16w|bf16|d<d>": {"waves": 4|8|16, "us": t_best, "us_alt": t_worst}
 //       waves per slice workgroup of the bond-dimension-16 kernel;
@@ -1464,10 +1464,11 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
         }
         if (rc == BP_OK) rc = rebuild_vertex_plan(c);
     }
-    // 1b. FP32 vertex sweep kernel variant, timed on full sweeps
-    if (rc == BP_OK && c->dtype == BP_F32 && c->vplan_ok && c->path != BP_PATH_GENERIC) {
+    // 1b. D <= 8 vertex sweep kernel variant (FP32 or BF16), timed on full sweeps
+    if (rc == BP_OK && (c->dtype == BP_F32 || c->dtype == BP_BF16) && c->vplan_ok &&
+        c->vplan.nv > c->vplan.nv_k3 && c->path != BP_PATH_GENERIC) {
         std::ostringstream o;
-        o << "vxk|f32|d" << c->vplan.d;
+        o << "vxk|" << (c->dtype == BP_F32 ? "f32" : "bf16") << "|d" << c->vplan.d;
         const std::string key = o.str();
         auto it = c->tune.find(key);
         if (it == c->tune.end() && mode >= 1) {

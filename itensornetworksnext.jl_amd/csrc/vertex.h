@@ -64,7 +64,7 @@ struct VertexPlan {
     std::vector<int> gen_slots; // the same on the host
     int ngen = 0;
     size_t lds_bytes = 0;
-    int variant = 0;            // FP32 sweep kernel: 0 k_vertex_d8, 1 k_vertex_d8_lo
+    int variant = 0;            // D <= 8 sweep kernel: 0 k_vertex_d8, 1 k_vertex_d8_lo
     // networks whose largest bond dimension is 9..16 in BF16 (k_vertex_d16, vertex16.h) or
     // 9..12 in FP32 (k_vertex_d12, vertex12.h): the large-bond bucket
     int nv16 = 0;
@@ -758,7 +758,13 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
     }
 }
 
-// ---------------------------------------------------------------- low-LDS variant (FP32)
+// BF16 rounding (nearest even) of a partial product in place, before an open contraction.
+__device__ __forceinline__ void vx_round_bf16(float (&G)[64]) {
+#pragma unroll
+    for (int i = 0; i < 64; i += 2) bf16_round2(G[i], G[i + 1]);
+}
+
+// ---------------------------------------------------------------- low-LDS variant (FP32 and BF16)
 //
 // The same sweep with one LDS buffer per wave instead of two (36 KB per two-wave workgroup,
 // so four workgroups fit a CU) and a 256-register budget, i.e. two waves per SIMD: the
@@ -767,7 +773,7 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
 // phase B's mode products) for phase B's bra (layout X).  The open contractions accumulate
 // in two halves of 32 registers (vx_final_half) and each result goes to LDS as soon as it
 // is made, which keeps the kernel within the budget (a few values spill, once per wave).
-template <int NS>
+template <int NS, bool BF>
 __global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__ mold,
                     float *__restrict__ mnew, const int4 *__restrict__ vin,
@@ -811,12 +817,26 @@ void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__
         float x;
         if (ins[j] < 0) x = la == lb ? 1.f : 0.f;
         else x = (la < D && lb < D) ? mold[ino[j] + la * D + lb] : 0.f;
+        if constexpr (BF) {
+            const float mx = wave_max(fabsf(x));
+            x = mx > 0.f ? x / mx : x;
+        }
         msm[j * 64 + lane] = x;
     }
-    const float *src = (const float *)vsites_v + ((long long)u * NS + w) * 4096;
     float T[64], E[64], G[64];
+    if constexpr (BF) {
+        const unsigned *src = (const unsigned *)vsites_v + ((long long)u * NS + w) * 2048;
 #pragma unroll
-    for (int r = 0; r < 64; ++r) T[r] = src[r * 64 + lane];
+        for (int r2 = 0; r2 < 32; ++r2) {
+            const unsigned pr = src[r2 * 64 + lane];
+            T[2 * r2] = bf16_to_f((unsigned short)(pr & 0xffffu));
+            T[2 * r2 + 1] = bf16_to_f((unsigned short)(pr >> 16));
+        }
+    } else {
+        const float *src = (const float *)vsites_v + ((long long)u * NS + w) * 4096;
+#pragma unroll
+        for (int r = 0; r < 64; ++r) T[r] = src[r * 64 + lane];
+    }
     __syncthreads();
     // results go to LDS as they are made (fewer live registers; with one wave per
     // workgroup the compiler allocates best when they stay in registers to the end)
@@ -826,10 +846,11 @@ void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__
         if constexpr (NS == 1) out[j] = v; else mypart[j * 64] = v;
     };
 
+
     // ---- phase A: E01 (X) -> Y through the buffer; then A into the buffer for the bra.
     // The buffer belongs to this wave: wave-level LDS ordering suffices between its uses.
-    vx_mode<true, false>(T, E, msm + 0 * 64);
-    vx_mode<false, false>(E, G, msm + 1 * 64);
+    vx_mode<true, BF>(T, E, msm + 0 * 64);
+    vx_mode<false, BF>(E, G, msm + 1 * 64);
 #pragma unroll
     for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = G[r];
     wave_lds_sync();
@@ -839,16 +860,18 @@ void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__
 #pragma unroll
     for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = T[r];  // A, readable as X or Y
     wave_lds_sync();
-    vx_mode<false, false>(E, G, msm + 3 * 64);
+    vx_mode<false, BF>(E, G, msm + 3 * 64);
+    if constexpr (BF) vx_round_bf16(G);  // the partial product dies here
     put(2, vx_final_half<true, false, VX_PITCH>(G, buf));
-    vx_mode<true, false>(E, G, msm + 2 * 64);
+    vx_mode<true, BF>(E, G, msm + 2 * 64);
+    if constexpr (BF) vx_round_bf16(G);  // the partial product dies here
     put(3, vx_final_half<false, false, VX_PITCH>(G, buf));
 
     // ---- phase B: E23 (Y) -> X through the buffer, then A written back for the bra (X)
 #pragma unroll
     for (int r = 0; r < 64; ++r) T[r] = buf[lane * VX_PITCH + r];  // A in Y
-    vx_mode<true, false>(T, E, msm + 2 * 64);
-    vx_mode<false, false>(E, G, msm + 3 * 64);
+    vx_mode<true, BF>(T, E, msm + 2 * 64);
+    vx_mode<false, BF>(E, G, msm + 3 * 64);
     wave_lds_sync();  // every lane has finished reading A from the buffer
 #pragma unroll
     for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = G[r];
@@ -859,9 +882,11 @@ void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__
 #pragma unroll
     for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = T[r];  // A again
     wave_lds_sync();
-    vx_mode<false, false>(E, G, msm + 1 * 64);
+    vx_mode<false, BF>(E, G, msm + 1 * 64);
+    if constexpr (BF) vx_round_bf16(G);  // the partial product dies here
     put(0, vx_final_half<true, true, VX_PITCH>(G, buf));
-    vx_mode<true, false>(E, G, msm + 0 * 64);
+    vx_mode<true, BF>(E, G, msm + 0 * 64);
+    if constexpr (BF) vx_round_bf16(G);  // the partial product dies here
     put(1, vx_final_half<false, true, VX_PITCH>(G, buf));
 
     // ---- sum over s, normalise, residual, write (as k_vertex_d8); the slot data are read
@@ -923,19 +948,23 @@ static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *v
     (void)tp;
     const int n4 = p->nv - p->nv_k3;  // items on the four-port kernels
     if (n4 == 0) return 0;
-    if (p->variant == 1 && !p->bf16) {
+    if (p->variant == 1) {
         const size_t lds = vertex_lo_lds(p->d);
-        switch (p->d) {
-#define VX_LO(NS)                                                                              \
-    case NS:                                                                                   \
-        hipLaunchKernelGGL((k_vertex_d8_lo<NS>), dim3(n4), dim3(64 * NS), lds, st, vsites,     \
+        switch (p->d * 2 + (p->bf16 ? 1 : 0)) {
+#define VX_LO(NS, BF)                                                                          \
+    case NS * 2 + BF:                                                                          \
+        hipLaunchKernelGGL((k_vertex_d8_lo<NS, BF>), dim3(n4), dim3(64 * NS), lds, st, vsites, \
                            (const float *)src, (float *)dst, p->d_vin, p->d_vout, p->d_vdim,   \
                            p->d_vnet, tp.msg_off, ctl, res_fid, res_abs, fold);                \
         break;
-            VX_LO(1)
-            VX_LO(2)
-            VX_LO(3)
-            VX_LO(4)
+            VX_LO(1, 0)
+            VX_LO(2, 0)
+            VX_LO(3, 0)
+            VX_LO(4, 0)
+            VX_LO(1, 1)
+            VX_LO(2, 1)
+            VX_LO(3, 1)
+            VX_LO(4, 1)
 #undef VX_LO
             default:
                 return 4;

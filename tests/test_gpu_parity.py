@@ -264,28 +264,32 @@ def test_vertex_batch_with_masks(pkg, orc):
 
 # ---------------------------------------------------------------- BF16 + FP64 polish
 
-def test_bf16_single_sweep_within_bound(pkg, orc):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_bf16_single_sweep_within_bound(pkg, orc, variant):
     """BF16 operands / FP32 accumulation: one sweep within the error bound with the BF16
     unit roundoff (gamma = 24 u, u = 2^-8, covering the ~10 roundings along a product
-    chain), from well-conditioned (identity) messages."""
-    _say("case bf16 single sweep")
+    chain), from well-conditioned (identity) messages; both D <= 8 sweep kernels (two LDS
+    buffers per wave, or one buffer and two waves per SIMD)."""
+    _say(f"case bf16 single sweep, kernel variant {variant}")
     t = pkg.build_topology(pkg.named_grid((6, 5)), 8, 2)
     s = pkg.peps.random_sites(t, 19, scale=pkg.peps.peps_scale(2, 8))
     m = pkg.peps.identity_messages(t)
     with pkg.Engine(t, s, dtype="bf16") as e:
         assert e.path() == "vertex"
+        e.set_option("vertex_variant", variant)
         e.set_messages(m)
         e.sweep()
         got = e.messages()
     assert _fp32_sweep_check(pkg, orc, t, s, m, got, gamma=24 * 2.0 ** -8) < 1.0
 
 
-def test_bf16_converged_then_fp64_polish(pkg, orc):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_bf16_converged_then_fp64_polish(pkg, orc, variant):
     """Mixed-precision workload at D = 8: converge in BF16, report the error against the
     FP64 oracle, then 3 FP64 polishing sweeps shrink it (stated bounds: BF16 messages
     within 5e-2 relative, log-norm within 1e-2 relative; after polishing, messages at
-    least 4x closer and log-norm within 1e-4 relative)."""
-    _say("case bf16 converged + polish")
+    least 4x closer and log-norm within 1e-4 relative); both D <= 8 sweep kernels."""
+    _say(f"case bf16 converged + polish, kernel variant {variant}")
     t = _grid(pkg, 12, 8)
     s = pkg.peps.random_sites(t, 23, scale=pkg.peps.peps_scale(2, 8))
     m0 = pkg.peps.identity_messages(t)
@@ -293,6 +297,7 @@ def test_bf16_converged_then_fp64_polish(pkg, orc):
     assert at > 0
     lw = orc.bethe_free_energy(t, s, want)
     with pkg.Engine(t, s, dtype="bf16") as e:
+        e.set_option("vertex_variant", variant)
         e.set_messages(m0)
         r = e.run(maxiter=200, tol=1e-5)
         m_bf = e.messages()
