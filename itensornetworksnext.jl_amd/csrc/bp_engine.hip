@@ -302,8 +302,9 @@ struct bp_context {
     VertexPlan vplan;
     bool vplan_ok = false;
     unsigned vx_mask = 0x1FFFEu;  // bit D: networks of largest bond dimension D may use it
-    int vx_variant = 0;         // FP32 vertex sweep kernel (0: two LDS buffers per wave,
-                                // 1: one buffer per wave, two waves per SIMD)
+    int vx_variant = -1;        // D <= 8 vertex sweep kernel (0: two LDS buffers per wave,
+                                // 1: one buffer per wave, two waves per SIMD; -1: by dtype,
+                                // the faster one at 32x32 D=8: 1 in FP32, 0 in BF16)
     int v16_waves = 16;         // waves per slice workgroup of the bond-dimension-16 kernel
 
     // cached sweep graphs (16 sweeps + bookkeeping), one per starting buffer parity; valid
@@ -361,6 +362,10 @@ static int dev_alloc(void **p, size_t bytes) {
     if (bytes == 0) bytes = 16;
     HIPCHK(hipMalloc(p, bytes));
     return BP_OK;
+}
+
+static int vx_variant_of(const bp_context *c) {
+    return c->vx_variant >= 0 ? c->vx_variant : (c->dtype == BP_F32 ? 1 : 0);
 }
 
 static size_t run_state_bytes(int nnet) { return (sizeof(double) + 3 * sizeof(int)) * nnet; }
@@ -796,7 +801,7 @@ static int rebuild_vertex_plan(bp_context *c) {
         c->vplan_ok = false;
         c->vplan.why = "could not raise the LDS limit of the vertex kernel";
     }
-    c->vplan.variant = c->vx_variant;
+    c->vplan.variant = vx_variant_of(c);
     c->vplan.v16_waves = c->v16_waves;
     if (c->vplan_ok && c->have_sites) RETCHK(upload_vertex_sites(c));
     return BP_OK;
@@ -1142,7 +1147,7 @@ int bp_set_option(bp_context *c, const char *name, int value) {
 
 int bp_get_option(bp_context *c, const char *name, int *value) {
     if (!c || !name || !value) return set_err(BP_ERR_ARG, "NULL argument");
-    if (strcmp(name, "vertex_variant") == 0) { *value = c->vx_variant; return BP_OK; }
+    if (strcmp(name, "vertex_variant") == 0) { *value = vx_variant_of(c); return BP_OK; }
     if (strcmp(name, "vertex16_waves") == 0) { *value = c->v16_waves; return BP_OK; }
     if (strcmp(name, "vertex_bond_mask") == 0) { *value = (int)c->vx_mask; return BP_OK; }
     if (strcmp(name, "fold_bookkeeping") == 0) { *value = c->fold_on; return BP_OK; }
@@ -1488,7 +1493,7 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
             }
         }
         if (it != c->tune.end()) c->vx_variant = it->second.vertex;
-        c->vplan.variant = c->vx_variant;
+        c->vplan.variant = vx_variant_of(c);
     }
     // 1c. waves per slice workgroup of the bond-dimension-16 kernel (BF16)
     if (rc == BP_OK && c->vplan_ok && c->vplan.nv16 > 0 && c->vplan.bf16 &&
