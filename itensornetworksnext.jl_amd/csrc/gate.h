@@ -491,6 +491,88 @@ GT_FN void gt_jacobi_staged(const Ctx &c, double *A, int rows, int cols, double 
     c.sync();
 }
 
+// Householder QR of a tall matrix A (rows >= cols, column-major, in place), LAPACK's geqr2
+// followed by org2r: R (cols x cols, row-major, upper triangular) is written to `R`, and A
+// is overwritten by Q (rows x cols, orthonormal columns, Q R = the input).  Column c belongs
+// to thread group c % ng (groups of gs threads, gs a power of two <= 64, gs * cols <= nt
+// when possible) and row i to thread i % gs of a group, in every step: a thread reads
+// elements of its group's columns only in rows it owns, and values from other threads
+// arrive through group_sum (dot products, the diagonal element) or after a barrier.  A
+// factorisation step needs one barrier (after the owner of column j has formed its
+// reflector), a Q step two.  tau: cols doubles of scratch.
+template <class Ctx>
+GT_FN void gt_householder(const Ctx &c, double *A, long long rows, int cols, double *tau,
+                          double *R) {
+    int gs = 1;
+    while (gs * 2 <= 64 && gs * 2 * cols <= c.nt) gs *= 2;
+    const int ng = c.nt / gs > 0 ? c.nt / gs : 1;
+    const int g = c.tid / gs, l = c.tid % gs;
+    const bool in_group = g < ng;
+    // first row >= i0 that this thread owns
+    auto first = [&](long long i0) { return i0 + ((l - i0 % gs) % gs + gs) % gs; };
+    // H_j (v_j = 1, v below the diagonal of column j, factor tj) applied to this group's
+    // columns > j, rows >= j
+    auto reflect = [&](int j, double tj) {
+        const double *aj = A + (long long)j * rows;
+        const long long i1 = first(j + 1);
+        for (int col = j + 1 + ((g - (j + 1) % ng) % ng + ng) % ng; col < cols; col += ng) {
+            double *ac = A + (long long)col * rows;
+            double w = l == j % gs ? ac[j] : 0.0;  // v_j = 1; row j is this thread's
+            for (long long i = i1; i < rows; i += gs) w += aj[i] * ac[i];
+            w = c.group_sum(w, gs) * tj;
+            if (l == j % gs) ac[j] -= w;
+            for (long long i = i1; i < rows; i += gs) ac[i] -= w * aj[i];
+        }
+    };
+    c.sync();
+    // factorisation: reflector j maps A[j:, j] to beta e_j
+    for (int j = 0; j < cols; ++j) {
+        double *aj = A + (long long)j * rows;
+        if (in_group && j % ng == g) {
+            const long long i1 = first(j + 1);
+            // alpha = A[j, j] from the thread that owns row j
+            const double alpha = c.group_sum(l == j % gs ? aj[j] : 0.0, gs);
+            double s2 = 0.0;
+            for (long long i = i1; i < rows; i += gs) s2 += aj[i] * aj[i];
+            s2 = c.group_sum(s2, gs);
+            double t = 0.0;
+            if (s2 > 0.0) {
+                const double nrm = sqrt(alpha * alpha + s2);
+                const double beta = alpha >= 0.0 ? -nrm : nrm;
+                t = (beta - alpha) / beta;
+                const double sc = 1.0 / (alpha - beta);
+                for (long long i = i1; i < rows; i += gs) aj[i] *= sc;
+                if (l == j % gs) aj[j] = beta;
+            }
+            if (l == 0) tau[j] = t;
+        }
+        c.sync();
+        const double tj = tau[j];
+        if (in_group && tj != 0.0) reflect(j, tj);
+    }
+    c.sync();
+    for (long long idx = c.tid; idx < (long long)cols * cols; idx += c.nt) {
+        const int i = (int)(idx / cols), col = (int)(idx % cols);
+        R[idx] = col >= i ? A[(long long)col * rows + i] : 0.0;
+    }
+    c.sync();
+    // Q = H_0 ... H_{cols-1} [I; 0], accumulated backwards in place
+    for (int j = cols - 1; j >= 0; --j) {
+        const double tj = tau[j];
+        if (in_group && tj != 0.0) reflect(j, tj);
+        c.sync();
+        if (in_group && j % ng == g) {
+            double *aj = A + (long long)j * rows;
+            for (long long i = first(0); i < rows; i += gs) {
+                if (i < j) aj[i] = 0.0;
+                else if (i == j) aj[i] = 1.0 - tj;
+                else aj[i] *= -tj;
+            }
+        }
+        c.sync();
+    }
+}
+
 // ------------------------------------------------------------------ the pipeline
 
 // Site s's gauged matrix, factored: Q (P x r, row-major) and R (r x n, row-major).
@@ -538,12 +620,26 @@ GT_FN void gt_site_qr(const Ctx &c, const GtLayout &L, int s, const double *A, d
     }
     c.sync();
     double *JV = ws + L.jv[s];
-    if (tall)
-        gt_jacobi_staged(c, J, (int)P, n, JV);
-    else
-        gt_jacobi_staged(c, J, n, (int)P, JV);
-    // Q and R
     double *Q = ws + L.Q[s], *R = ws + L.R[s];
+    if (tall) {
+        // Householder QR (in LDS when the matrix fits), Q written row-major
+        const bool staged = P * n + n <= c.lds_cap;
+        double *A = staged ? c.lds : J;
+        double *tau = staged ? c.lds + P * n : ws + L.ssig[s];
+        if (staged)
+            for (long long i = c.tid; i < P * n; i += c.nt) A[i] = J[i];
+        gt_householder(c, A, P, n, tau, R);
+        for (long long idx = c.tid; idx < P * n; idx += c.nt) {
+            const long long x = idx / n;
+            const int i = (int)(idx % n);
+            Q[idx] = A[(long long)i * P + x];
+        }
+        c.sync();
+        (void)d;
+        return;
+    }
+    gt_jacobi_staged(c, J, n, (int)P, JV);
+    // Q and R
     if (tall) {
         // columns of J: U S (P x n); V = JV (n x n)
         for (int col = c.tid; col < n; col += c.nt) {
