@@ -212,8 +212,23 @@ struct GtDev {
     long long lds_cap;
     __device__ void sync() const { __syncthreads(); }
     __device__ void mark(int) const {}  // phase boundary (a hook for instrumented contexts)
+    // sum over an aligned group of gs lanes (a power of two <= 64, uniform over the wave)
+    // on the VALU: DPP partners inside 16-lane rows, then the row and half swaps; every
+    // partner lies in the same group, so lanes of other (possibly inactive) groups are not
+    // read
     __device__ double group_sum(double v, int gs) const {
-        for (int o = gs >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, gs);
+        if (gs >= 2) v += row_partner<1>(v);
+        if (gs >= 4) v += row_partner<2>(v);
+        if (gs >= 8) v += row_partner<4>(v);
+        if (gs >= 16) v += row_partner<8>(v);
+        if (gs >= 32) {
+            const DPair r = rows16_pair(v);
+            v = r.first + r.second;
+        }
+        if (gs >= 64) {
+            const DPair h = halves32_pair(v);
+            v = h.first + h.second;
+        }
         return v;
     }
 };
@@ -501,7 +516,7 @@ GT_FN void gt_jacobi_staged(const Ctx &c, double *A, int rows, int cols, double 
 // factorisation step needs one barrier (after the owner of column j has formed its
 // reflector), a Q step two.  tau: cols doubles of scratch.
 template <class Ctx>
-GT_FN void gt_householder(const Ctx &c, double *A, long long rows, int cols, double *tau,
+GT_FN void gt_householder(const Ctx &c, double *A, int rows, int cols, double *tau,
                           double *R) {
     int gs = 1;
     while (gs * 2 <= 64 && gs * 2 * cols <= c.nt) gs *= 2;
@@ -509,19 +524,21 @@ GT_FN void gt_householder(const Ctx &c, double *A, long long rows, int cols, dou
     const int g = c.tid / gs, l = c.tid % gs;
     const bool in_group = g < ng;
     // first row >= i0 that this thread owns
-    auto first = [&](long long i0) { return i0 + ((l - i0 % gs) % gs + gs) % gs; };
+    auto first = [&](int i0) { return i0 + ((l - i0 % gs) % gs + gs) % gs; };
     // H_j (v_j = 1, v below the diagonal of column j, factor tj) applied to this group's
     // columns > j, rows >= j
     auto reflect = [&](int j, double tj) {
         const double *aj = A + (long long)j * rows;
-        const long long i1 = first(j + 1);
+        const int i1 = first(j + 1);
         for (int col = j + 1 + ((g - (j + 1) % ng) % ng + ng) % ng; col < cols; col += ng) {
             double *ac = A + (long long)col * rows;
             double w = l == j % gs ? ac[j] : 0.0;  // v_j = 1; row j is this thread's
-            for (long long i = i1; i < rows; i += gs) w += aj[i] * ac[i];
+#pragma unroll 8
+            for (int i = i1; i < rows; i += gs) w += aj[i] * ac[i];
             w = c.group_sum(w, gs) * tj;
             if (l == j % gs) ac[j] -= w;
-            for (long long i = i1; i < rows; i += gs) ac[i] -= w * aj[i];
+#pragma unroll 8
+            for (int i = i1; i < rows; i += gs) ac[i] -= w * aj[i];
         }
     };
     c.sync();
@@ -529,11 +546,12 @@ GT_FN void gt_householder(const Ctx &c, double *A, long long rows, int cols, dou
     for (int j = 0; j < cols; ++j) {
         double *aj = A + (long long)j * rows;
         if (in_group && j % ng == g) {
-            const long long i1 = first(j + 1);
+            const int i1 = first(j + 1);
             // alpha = A[j, j] from the thread that owns row j
             const double alpha = c.group_sum(l == j % gs ? aj[j] : 0.0, gs);
             double s2 = 0.0;
-            for (long long i = i1; i < rows; i += gs) s2 += aj[i] * aj[i];
+#pragma unroll 8
+            for (int i = i1; i < rows; i += gs) s2 += aj[i] * aj[i];
             s2 = c.group_sum(s2, gs);
             double t = 0.0;
             if (s2 > 0.0) {
@@ -541,7 +559,8 @@ GT_FN void gt_householder(const Ctx &c, double *A, long long rows, int cols, dou
                 const double beta = alpha >= 0.0 ? -nrm : nrm;
                 t = (beta - alpha) / beta;
                 const double sc = 1.0 / (alpha - beta);
-                for (long long i = i1; i < rows; i += gs) aj[i] *= sc;
+#pragma unroll 8
+                for (int i = i1; i < rows; i += gs) aj[i] *= sc;
                 if (l == j % gs) aj[j] = beta;
             }
             if (l == 0) tau[j] = t;
@@ -563,7 +582,7 @@ GT_FN void gt_householder(const Ctx &c, double *A, long long rows, int cols, dou
         c.sync();
         if (in_group && j % ng == g) {
             double *aj = A + (long long)j * rows;
-            for (long long i = first(0); i < rows; i += gs) {
+            for (int i = first(0); i < rows; i += gs) {
                 if (i < j) aj[i] = 0.0;
                 else if (i == j) aj[i] = 1.0 - tj;
                 else aj[i] *= -tj;
@@ -595,6 +614,7 @@ GT_FN void gt_site_qr(const Ctx &c, const GtLayout &L, int s, const double *A, d
         cur = nxt;
         nxt = t;
     }
+    if (s == 0) c.mark(9);
     // to the Jacobi layout: tall (P >= n) columns (t, b), wide columns x
     double *J = ws + L.jac[s];
     const bool tall = P >= n;
@@ -628,7 +648,9 @@ GT_FN void gt_site_qr(const Ctx &c, const GtLayout &L, int s, const double *A, d
         double *tau = staged ? c.lds + P * n : ws + L.ssig[s];
         if (staged)
             for (long long i = c.tid; i < P * n; i += c.nt) A[i] = J[i];
-        gt_householder(c, A, P, n, tau, R);
+        if (s == 0) c.mark(10);
+        gt_householder(c, A, (int)P, n, tau, R);
+        if (s == 0) c.mark(11);
         for (long long idx = c.tid; idx < P * n; idx += c.nt) {
             const long long x = idx / n;
             const int i = (int)(idx % n);
