@@ -133,6 +133,7 @@ GT_FN int gt_layout(const int *dsc, GtLayout &L) {
     L.n2 = L.d2 * L.Db;
     L.size1 = L.P1 * L.n1;
     L.size2 = L.P2 * L.n2;
+    if (L.size1 > 0x7fffffffLL || L.size2 > 0x7fffffffLL) return 1;  // 32-bit site indices
     L.r1 = (int)gt_min(L.P1, L.n1);
     L.r2 = (int)gt_min(L.P2, L.n2);
     L.m1 = L.r1 * L.d1;
@@ -394,6 +395,21 @@ GT_FN void gt_gauges(const Ctx &c, int nm, const double *const *M, double *const
     c.sync();
 }
 
+template <typename I, class Ctx>
+GT_FN void gt_mode_idx(const Ctx &c, const double *in, double *out, I size, I inner, int D,
+                       const double *Mx) {
+    for (I idx = c.tid; idx < size; idx += c.nt) {
+        const I q = idx / inner;
+        const I i = idx - q * inner, o = q / D;
+        const int b = (int)(q - o * D);
+        const double *src = in + (long long)o * inner * D + i;
+        double s = 0.0;
+#pragma unroll 4
+        for (int a = 0; a < D; ++a) s += src[(long long)a * inner] * Mx[a * D + b];
+        out[idx] = s;
+    }
+}
+
 // out = in x_j Mx (mode product on port j of a [d0][dims...] tensor):
 // out[o, b, i] = sum_a in[o, a, i] Mx[a * D + b]; `dims` are the port dimensions (the
 // port j dimension D is unchanged)
@@ -403,14 +419,10 @@ GT_FN void gt_mode(const Ctx &c, const double *in, double *out, long long size, 
     long long inner = 1;
     for (int l = j + 1; l < k; ++l) inner *= dims[l];
     const int D = dims[j];
-    for (long long idx = c.tid; idx < size; idx += c.nt) {
-        const long long i = idx % inner, b = (idx / inner) % D, o = idx / (inner * D);
-        const double *src = in + o * inner * D + i;
-        double s = 0.0;
-#pragma unroll 4
-        for (int a = 0; a < D; ++a) s += src[a * inner] * Mx[a * D + b];
-        out[idx] = s;
-    }
+    if (size <= 0x7fffffffLL)  // 32-bit index arithmetic (64-bit division is a long routine)
+        gt_mode_idx<int>(c, in, out, (int)size, (int)inner, D, Mx);
+    else
+        gt_mode_idx<long long>(c, in, out, size, inner, D, Mx);
 }
 
 // One-sided Jacobi SVD of A (rows x cols, column-major: column c at A + c * rows): the
@@ -618,12 +630,14 @@ GT_FN void gt_site_qr(const Ctx &c, const GtLayout &L, int s, const double *A, d
     // to the Jacobi layout: tall (P >= n) columns (t, b), wide columns x
     double *J = ws + L.jac[s];
     const bool tall = P >= n;
-    for (long long idx = c.tid; idx < size; idx += c.nt) {
-        long long rem = idx, x = 0, xs = 1;
+    // (32-bit index arithmetic; a gate's site tensor is far below 2^31 entries)
+    for (int idx = c.tid; idx < (int)size; idx += c.nt) {
+        int rem = idx, x = 0, xs = 1;
         int bb = 0;
         for (int j = k - 1; j >= 0; --j) {
-            const int a = (int)(rem % dims[j]);
-            rem /= dims[j];
+            const int qt = rem / dims[j];
+            const int a = rem - qt * dims[j];
+            rem = qt;
             if (j == p) {
                 bb = a;
             } else {
@@ -631,12 +645,12 @@ GT_FN void gt_site_qr(const Ctx &c, const GtLayout &L, int s, const double *A, d
                 xs *= dims[j];
             }
         }
-        const int t = (int)rem;
+        const int t = rem;
         const long long col = (long long)t * Db + bb;
         if (tall)
             J[col * P + x] = src[idx];
         else
-            J[x * n + col] = src[idx];
+            J[(long long)x * n + col] = src[idx];
     }
     c.sync();
     double *JV = ws + L.jv[s];
