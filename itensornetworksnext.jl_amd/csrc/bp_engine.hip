@@ -303,8 +303,8 @@ struct bp_context {
     bool vplan_ok = false;
     unsigned vx_mask = 0x1FFFEu;  // bit D: networks of largest bond dimension D may use it
     int vx_variant = -1;        // D <= 8 vertex sweep kernel (0: two LDS buffers per wave,
-                                // 1: one buffer per wave, two waves per SIMD; -1: by dtype,
-                                // the faster one at 32x32 D=8: 1 in FP32, 0 in BF16)
+                                // 1: one buffer per wave, two waves per SIMD; -1: the faster
+                                // one at 32x32 D=8: 1 in FP32 with d >= 2, else 0)
     int v16_waves = 16;         // waves per slice workgroup of the bond-dimension-16 kernel
 
     // cached sweep graphs (16 sweeps + bookkeeping), one per starting buffer parity; valid
@@ -364,8 +364,11 @@ static int dev_alloc(void **p, size_t bytes) {
     return BP_OK;
 }
 
+// Measured at 32x32 D=8 (sweep kernel, us): FP32 d=1 23.3 / 24.4, d=2 40.6 / 36.4,
+// d=3 72.6 / 59.6, d=4 75.5 / 65.3 (two-buffer / low-LDS); BF16 d=2 31.3 / 32.4.
 static int vx_variant_of(const bp_context *c) {
-    return c->vx_variant >= 0 ? c->vx_variant : (c->dtype == BP_F32 ? 1 : 0);
+    if (c->vx_variant >= 0) return c->vx_variant;
+    return c->dtype == BP_F32 && c->vplan.d >= 2 ? 1 : 0;
 }
 
 static size_t run_state_bytes(int nnet) { return (sizeof(double) + 3 * sizeof(int)) * nnet; }
