@@ -104,6 +104,15 @@ int bp_upload_sites(bp_context *ctx, const double *sites);
 int bp_init_messages(bp_context *ctx, int kind, const double *host);
 int bp_download_messages(bp_context *ctx, double *out);
 
+/* Messages of selected slots, packed in slot-list order into an FP64 host buffer (each
+ * slot's D x D or length-D message after the previous one): the per-edge cache access of
+ * message_update!(alg, cache, factors, edge) (beliefpropagation.jl:242-257 reads the
+ * edge's incoming messages, messagecache.jl:124-131, and writes cache[edge]), so a
+ * per-edge strategy moves O(degree * D^2) values per update instead of the whole cache.
+ * Reads and writes the current messages (the FP64 ones after bp_polish_fp64). */
+int bp_set_slot_messages(bp_context *ctx, int n, const int *slots, const double *data);
+int bp_get_slot_messages(bp_context *ctx, int n, const int *slots, double *out);
+
 /* Choose the kernel path (default BP_PATH_AUTO). */
 int bp_set_path(bp_context *ctx, int path);
 /* The path the next sweep will use (after AUTO resolution). */
@@ -220,10 +229,16 @@ int bp_local_energy_device(bp_context *ctx, const double *ops_v, const double *o
  *                      vertex kernels (D 1..12 in FP32, 1..16 in BF16; default 0x1FFFE);
  *   "fold_bookkeeping" 1 (default): when one D <= 8 vertex launch covers every network, the
  *                      sweep kernels also do the per-network convergence bookkeeping;
- *                      0: a separate bookkeeping kernel after every sweep (same results).
- * No reference counterpart. */
+ *                      0: a separate bookkeeping kernel after every sweep (same results);
+ *   "normalize"        1 (default): every update divides the new message by its sum when
+ *                      the sum is nonzero; 0: messages are kept unnormalised, as
+ *                      SimpleMessageUpdate(; normalize = false) (beliefpropagation.jl:237-253);
+ *                      sweeps then run on the generic kernel and gradients are refused.
+ * Only "normalize" has a reference counterpart. */
 int bp_set_option(bp_context *ctx, const char *name, int value);
-/* The current value of an option above (after bp_autotune: the tuned choice).  No reference
+/* The current value of an option above (after bp_autotune: the tuned choice).
+ * "vertex_variant" reads -1 while no D <= 8 vertex plan exists (no sites or topology yet,
+ * or the vertex kernels declined the network) and no variant was set.  No reference
  * counterpart. */
 int bp_get_option(bp_context *ctx, const char *name, int *value);
 

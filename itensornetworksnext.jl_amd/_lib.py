@@ -35,6 +35,8 @@ SIGNATURES = {
     "bp_upload_sites": (C.c_int, [_vp, _dp]),
     "bp_init_messages": (C.c_int, [_vp, C.c_int, _dp]),
     "bp_download_messages": (C.c_int, [_vp, _dp]),
+    "bp_set_slot_messages": (C.c_int, [_vp, C.c_int, _ip, _dp]),
+    "bp_get_slot_messages": (C.c_int, [_vp, C.c_int, _ip, _dp]),
     "bp_set_path": (C.c_int, [_vp, C.c_int]),
     "bp_get_path": (C.c_int, [_vp, _ip]),
     "bp_sweep": (C.c_int, [_vp]),

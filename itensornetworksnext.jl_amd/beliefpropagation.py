@@ -28,7 +28,7 @@ import numpy as np
 from . import algorithms as A
 from .graphs import forest_cover_edge_sequence
 from .itensors import to_topology
-from .messagecache import MessageCache, _engine, _flat, incoming_messages, iterate_diff
+from .messagecache import MessageCache, _flat, incoming_messages, iterate_diff
 
 
 class MessageUpdateAlgorithm(A.AbstractAlgorithm):
@@ -39,27 +39,70 @@ class MessageUpdateAlgorithm(A.AbstractAlgorithm):
         raise A.MethodError(f"message_update_ not defined for {type(self).__name__}")
 
 
+class JacobiSweep:
+    """An `edges` entry that stands for one whole Jacobi sweep: `message_update_` with a
+    device strategy and this token updates every directed message at once (the sweep
+    problem only needs `edges` to be indexable with a length, beliefpropagation.jl:86,
+    205)."""
+
+    def __repr__(self):
+        return "JacobiSweep()"
+
+
+JACOBI_SWEEP = JacobiSweep()
+
+
+def _factor_engine(factors, dtype: str, normalize: bool = True, path: str = "generic"):
+    """One device context per (network, dtype, normalize), kept on the network object so
+    per-edge updates do not re-upload the site tensors.  Messages start at the identity;
+    callers upload what an update reads."""
+    from .engine import Engine
+
+    fl = _flat(factors)
+    key = (dtype, bool(normalize), path)
+    cache = getattr(factors, "_bp_engines", None)
+    if cache is None:
+        cache = {}
+        try:
+            factors._bp_engines = cache
+        except AttributeError:
+            pass
+    e = cache.get(key)
+    if e is None:
+        e = Engine(fl.topo, fl.sites, dtype=dtype, path=path)
+        if not normalize:
+            e.set_option("normalize", 0)
+        e.init_messages("identity")
+        cache[key] = e
+    return fl, e
+
+
 class SimpleMessageUpdate(MessageUpdateAlgorithm):
-    """Contract the source factor with its incoming messages, divide by the sum.
+    """Contract the source factor with its incoming messages, divide by the sum unless
+    `normalize` is false (beliefpropagation.jl:237-257).
 
     dtype selects the device precision ("f64" keeps the reference's Float64 semantics)."""
 
     def __init__(self, normalize: bool = True, contraction_alg=None, dtype: str = "f64"):
-        if not normalize:
-            raise NotImplementedError("the device update always sum-normalises (normalize=True)")
-        self.normalize = normalize
+        self.normalize = bool(normalize)
         self.contraction_alg = contraction_alg
         self.dtype = dtype
 
     def update(self, cache, factors, edge):
-        fl, e = _engine(factors, cache, self.dtype)
-        try:
-            q = fl.slot(edge)
-            e.update_slot(q)
-            flat = e.messages()
-        finally:
-            e.close()
-        cache[edge] = flat[fl.topo.msg_off[q]:fl.topo.msg_off[q + 1]].reshape(fl.topo.msg_shape(q))
+        fl, e = _factor_engine(factors, self.dtype, self.normalize)
+        t = fl.topo
+        q = fl.slot(edge)
+        u = int(t.port_owner()[q])
+        ins = [int(t.port_rev[j]) for j in range(t.port_off[u], t.port_off[u + 1]) if j != q]
+        own = t.port_owner()
+        if ins:
+            data = []
+            for r in ins:
+                key = (t.vertices[own[r]], t.vertices[t.port_nbr[r]])
+                data.append(np.asarray(cache[key], dtype=np.float64).reshape(-1))
+            e.set_slot_messages(ins, np.concatenate(data))
+        e.update_slot(q)
+        cache[edge] = e.slot_messages([q]).reshape(t.msg_shape(q))
         return cache
 
     def __repr__(self):
@@ -82,6 +125,15 @@ class JacobiGPUUpdate(MessageUpdateAlgorithm):
         self.polish = int(polish)
 
     def update(self, cache, factors, edge):
+        if isinstance(edge, JacobiSweep):
+            # one device sweep over every directed message, written back into the cache
+            fl, e = _factor_engine(factors, self.dtype, True, self.path)
+            e.set_messages(cache.to_flat(fl.topo))
+            e.sweep()
+            new = MessageCache.from_flat(fl.topo, e.messages())
+            for k, m in new.items():
+                cache[k] = m
+            return cache
         # one message in isolation is just the sequential update
         return SimpleMessageUpdate(dtype=self.dtype if self.dtype != "bf16" else "f32").update(
             cache, factors, edge)
@@ -105,6 +157,7 @@ def message_update_(*args, alg=None, **kwargs):
 
 
 A.register_default(message_update_, lambda argtypes, **kw: SimpleMessageUpdate(**kw))
+A.register_iterate_diff(MessageCache, iterate_diff)
 
 
 def default_beliefpropagation_edges(factors):
@@ -112,11 +165,85 @@ def default_beliefpropagation_edges(factors):
     return forest_cover_edge_sequence(g)
 
 
+# ------------------------------------------------------------------ the nested solver
+# beliefpropagation.jl:94-210: the outer loop is a NestedAlgorithm whose step solves one
+# sweep problem; the sweep's own loop calls message_update! once per entry of `edges`.
+
+
+class BeliefPropagationProblem(A.Problem):
+    def __init__(self, factors):
+        self.factors = factors
+
+
+class BeliefPropagationSweepProblem(A.Problem):
+    def __init__(self, factors, edges):
+        self.factors = factors
+        self.edges = edges
+
+
+class BeliefPropagationSweepState(A.State):
+    pass
+
+
+class BeliefPropagationSweepAlgorithm(A.Algorithm):
+    """One sweep: step i updates edges[i] (beliefpropagation.jl:155-210); stops after
+    len(edges) steps."""
+
+    def __init__(self, message_update_algorithm=None, stopping_criterion=None):
+        self.message_update_algorithm = (message_update_algorithm
+                                         if message_update_algorithm is not None
+                                         else SimpleMessageUpdate())
+        self.stopping_criterion = stopping_criterion
+
+    def initialize_state(self, problem, iterate=None, iteration: int = 0):
+        st = self.stopping_criterion.initialize_state(problem, self, iterate=iterate)
+        return BeliefPropagationSweepState(iterate, iteration, st)
+
+    def step_(self, problem, state):
+        edge = problem.edges[state.iteration - 1]  # Julia's 1-based edges[state.iteration]
+        message_update_(self.message_update_algorithm, state.iterate, problem.factors, edge)
+        return state
+
+
+class BeliefPropagationState(A.NestedState):
+    pass
+
+
+class BeliefPropagationAlgorithm(A.NestedAlgorithm):
+    """BP outer loop (beliefpropagation.jl:100-151): any `subalgorithm` AI.Algorithm runs
+    one sweep per outer step; `stopping_criterion` decides when BP stops."""
+
+    def __init__(self, edges, subalgorithm, stopping_criterion):
+        self.edges = edges
+        self.subalgorithm = subalgorithm
+        self.stopping_criterion = stopping_criterion
+
+    def initialize_state(self, problem, iterate=None, iteration: int = 0):
+        sub = BeliefPropagationSweepProblem(problem.factors, self.edges)
+        substate = A.initialize_state(sub, self.subalgorithm, iterate=iterate)
+        st = self.stopping_criterion.initialize_state(problem, self, iterate=iterate)
+        return BeliefPropagationState(substate, iteration, st)
+
+    def initialize_state_(self, problem, state, iteration: int = 0, **kwargs):
+        state.iteration = iteration
+        self.stopping_criterion.initialize_state_(problem, self, state.stopping_criterion_state)
+        return state
+
+    def initialize_subsolve(self, problem, state):
+        sub = BeliefPropagationSweepProblem(problem.factors, self.edges)
+        return sub, self.subalgorithm, state.substate
+
+
 def beliefpropagation(factors, messages, *, edges=None, stopping_criterion=None,
                       message_update_algorithm=None) -> MessageCache:
     """Run BP on `factors` (a NormNetwork: double-layer, matrix messages; or an
     ITensorNetwork: plain, vector messages) from `messages`; returns the converged
-    MessageCache.  `result.state` records iteration, delta and at_iteration."""
+    MessageCache.  `result.state` records iteration, delta and at_iteration.
+
+    SimpleMessageUpdate and JacobiGPUUpdate with the default schedule run as one device
+    call each (bp_run_sequential / bp_run, the stopping rule applied on the device); any
+    other strategy, or a JacobiSweep schedule, runs the reference's nested solver with one
+    message_update_ call per entry of `edges`."""
     cache = messages.copy() if isinstance(messages, MessageCache) else MessageCache(messages)
     algo = A.select_algorithm(message_update_, message_update_algorithm,
                               (type(cache), type(factors), tuple))
@@ -124,8 +251,9 @@ def beliefpropagation(factors, messages, *, edges=None, stopping_criterion=None,
     maxiter, tol = crit.limits()
     fl = _flat(factors)
     topo = fl.topo
+    token = edges is not None and any(isinstance(ed, JacobiSweep) for ed in edges)
 
-    if isinstance(algo, JacobiGPUUpdate):
+    if isinstance(algo, JacobiGPUUpdate) and not token:
         if edges is not None and len(edges) != topo.nports:
             raise A.ArgumentError("a Jacobi sweep updates every directed edge once; custom "
                                   "`edges` need a sequential strategy")
@@ -146,12 +274,14 @@ def beliefpropagation(factors, messages, *, edges=None, stopping_criterion=None,
     if edges is None:
         edges = default_beliefpropagation_edges(factors)
 
-    if isinstance(algo, SimpleMessageUpdate):
+    if isinstance(algo, SimpleMessageUpdate) and type(algo) is SimpleMessageUpdate and not token:
         from .engine import Engine
 
         order = [fl.slot(ed) for ed in edges]
         e = Engine(topo, fl.sites, dtype=algo.dtype, path="generic")
         try:
+            if not algo.normalize:
+                e.set_option("normalize", 0)
             e.set_messages(cache.to_flat(topo))
             r = e.run_sequential(order, maxiter=maxiter, tol=tol)
             out = MessageCache.from_flat(topo, e.messages())
@@ -160,25 +290,27 @@ def beliefpropagation(factors, messages, *, edges=None, stopping_criterion=None,
         out.state = _state(int(r.sweeps[0]), float(r.delta[0]), int(r.at_iteration[0]), tol)
         return out
 
-    # generic strategy: the reference's nested loop, one message_update! per edge
-    prev = cache.copy()
-    it, at, delta = 0, -1, float("inf")
-    while True:
-        if maxiter is not None and it >= maxiter:
-            break
-        if tol is not None:
-            d = iterate_diff(cache, prev)
-            prev = cache.copy()
-            if it > 0:
-                delta = d
-                if d < tol:
-                    at = it
-                    break
-        it += 1
-        for ed in edges:
-            algo.update(cache, factors, ed)
-    cache.state = _state(it, delta, at, tol)
-    return cache
+    # the reference's nested loop (beliefpropagation.jl:84-92)
+    sub = BeliefPropagationSweepAlgorithm(message_update_algorithm=algo,
+                                          stopping_criterion=A.StopAfterIteration(len(edges)))
+    alg = BeliefPropagationAlgorithm(edges, sub, crit)
+    problem = BeliefPropagationProblem(factors)
+    state = A.initialize_state(problem, alg, iterate=cache)
+    A.solve_(problem, alg, state)
+    out = state.iterate
+    st = _criterion_state(state.stopping_criterion_state)
+    out.state = _state(state.iteration, st.delta if st else float("inf"),
+                       st.at_iteration if st else -1, tol)
+    return out
+
+
+def _criterion_state(st):
+    if isinstance(st, A.StopWhenConvergedState):
+        return st
+    for s in getattr(st, "states", ()):
+        if isinstance(s, A.StopWhenConvergedState):
+            return s
+    return None
 
 
 def _state(iteration, delta, at, tol):
@@ -187,6 +319,9 @@ def _state(iteration, delta, at, tol):
     return st
 
 
-__all__ = ["MessageUpdateAlgorithm", "SimpleMessageUpdate", "JacobiGPUUpdate",
-           "message_update_", "beliefpropagation", "default_beliefpropagation_edges",
-           "incoming_messages", "iterate_diff"]
+__all__ = ["MessageUpdateAlgorithm", "SimpleMessageUpdate", "JacobiGPUUpdate", "JacobiSweep",
+           "JACOBI_SWEEP", "message_update_", "beliefpropagation",
+           "default_beliefpropagation_edges", "incoming_messages", "iterate_diff",
+           "BeliefPropagationProblem", "BeliefPropagationAlgorithm", "BeliefPropagationState",
+           "BeliefPropagationSweepProblem", "BeliefPropagationSweepAlgorithm",
+           "BeliefPropagationSweepState"]

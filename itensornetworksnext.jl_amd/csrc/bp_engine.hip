@@ -295,6 +295,7 @@ struct bp_context {
     size_t h_state_bytes = 0;
     int fold_n = 0;                             // networks d_fold_acc holds
     int fold_on = 1;                            // option "fold_bookkeeping"
+    int normalize = 1;                          // option "normalize" (0: SimpleMessageUpdate(normalize = false))
     void *d_ws = nullptr;
     size_t ws_bytes = 0;
 
@@ -368,6 +369,7 @@ static int dev_alloc(void **p, size_t bytes) {
 // d=3 72.6 / 59.6, d=4 75.5 / 65.3 (two-buffer / low-LDS); BF16 d=2 31.3 / 32.4.
 static int vx_variant_of(const bp_context *c) {
     if (c->vx_variant >= 0) return c->vx_variant;
+    if (!c->vplan_ok) return -1;  // no vertex plan: no D <= 8 sweep kernel is chosen yet
     return c->dtype == BP_F32 && c->vplan.d >= 2 ? 1 : 0;
 }
 
@@ -439,14 +441,14 @@ static int ensure_ws(bp_context *c, const GenLaunch &g, size_t esz) {
 template <typename T>
 static int launch_update_generic(bp_context *c, const T *sites, const T *mold, T *mnew,
                                  const NetCtl &ctl, int q0, int q1, const int *list = nullptr,
-                                 int bf16 = 0) {
+                                 int flags = 0) {
     if (q1 <= q0) return BP_OK;
     GenLaunch g = gen_launch(c, q1 - q0, sizeof(T));
     RETCHK(ensure_ws(c, g, sizeof(T)));
     hipLaunchKernelGGL(k_update_generic<T>, dim3(g.grid), dim3(BP_BLOCK), g.lds_bytes, c->stream,
                        c->topo(), sites, mold, mnew, ctl, c->d_res_fid, c->d_res_abs, q0, q1,
                        list, g.mats_elems, g.out_elems, g.red_elems, g.buf_elems, g.use_lds,
-                       (T *)c->d_ws, g.ws_stride, bf16);
+                       (T *)c->d_ws, g.ws_stride, flags);
     HIPCHK(hipGetLastError());
     return BP_OK;
 }
@@ -562,7 +564,7 @@ static int build_buckets(bp_context *c, int path) {
 
 template <typename T>
 static int launch_bucket(bp_context *c, const GenBucket &b, const T *sites, const T *mold,
-                         T *mnew, const NetCtl &ctl, int bf16) {
+                         T *mnew, const NetCtl &ctl, int flags) {
     if (b.slots.empty()) return BP_OK;
     const GenLaunch g = gen_launch_bucket(b, sizeof(T));
     const int n = (int)b.slots.size();
@@ -570,7 +572,7 @@ static int launch_bucket(bp_context *c, const GenBucket &b, const T *sites, cons
     hipLaunchKernelGGL((k_update_generic<T, BS>), dim3(g.grid), dim3(BS), g.lds_bytes,         \
                        c->stream, c->topo(), sites, mold, mnew, ctl, c->d_res_fid,              \
                        c->d_res_abs, 0, n, b.d_slots, g.mats_elems, g.out_elems, g.red_elems,   \
-                       g.buf_elems, g.use_lds, (T *)c->d_ws, g.ws_stride, bf16)
+                       g.buf_elems, g.use_lds, (T *)c->d_ws, g.ws_stride, flags)
     switch (b.bs) {
         case 128: GEN_B(128); break;
         case 512: GEN_B(512); break;
@@ -597,8 +599,8 @@ static int ensure_bucket_ws(bp_context *c, size_t esz) {
 
 template <typename T>
 static int launch_buckets(bp_context *c, const T *sites, const T *mold, T *mnew,
-                          const NetCtl &ctl, int bf16) {
-    for (const auto &b : c->gb) RETCHK(launch_bucket<T>(c, b, sites, mold, mnew, ctl, bf16));
+                          const NetCtl &ctl, int flags) {
+    for (const auto &b : c->gb) RETCHK(launch_bucket<T>(c, b, sites, mold, mnew, ctl, flags));
     return BP_OK;
 }
 
@@ -629,7 +631,14 @@ static int launch_edge_scalars(bp_context *c, const T *msgs) {
 // kernel, or the vertex bucket plus the generic kernel for the remaining networks.
 
 
+// Flags of the generic kernel for this context (BF16 operands, no normalisation).
+static int gen_flags(const bp_context *c, bool bf16) {
+    return (bf16 ? GEN_FLAG_BF16 : 0) | (c->normalize ? 0 : GEN_FLAG_NO_NORMALIZE);
+}
+
 static int resolve_path(bp_context *c) {
+    // the vertex kernels always normalise: unnormalised updates run on the generic kernel
+    if (!c->normalize) return c->path == BP_PATH_VERTEX ? -1 : BP_PATH_GENERIC;
     if (c->path == BP_PATH_GENERIC || !c->vplan_ok) return c->path == BP_PATH_VERTEX ? -1 : BP_PATH_GENERIC;
     if (c->vplan.all) return BP_PATH_VERTEX;
     // AUTO: vertex kernel for its bucket, generic kernel for the rest
@@ -765,10 +774,10 @@ static int sweep_once(bp_context *c, const NetCtl &done, bool *folded = nullptr,
     if (path != BP_PATH_VERTEX) {
         if (c->dtype == BP_F64)
             RETCHK(launch_buckets<double>(c, (const double *)c->d_sites, (const double *)src,
-                                          (double *)dst, done, 0));
+                                          (double *)dst, done, gen_flags(c, false)));
         else
             RETCHK(launch_buckets<float>(c, generic_sites32(c), (const float *)src, (float *)dst,
-                                         done, c->dtype == BP_BF16));
+                                         done, gen_flags(c, c->dtype == BP_BF16)));
     }
     c->cur ^= 1;
     return BP_OK;
@@ -1107,10 +1116,65 @@ int bp_download_messages(bp_context *c, double *out) {
     return BP_OK;
 }
 
+// Slot ranges of the message buffer: checks the slots and sizes a host buffer.
+static int slot_span(const bp_context *c, int n, const int *slots, long long *total) {
+    if (n < 0 || (n > 0 && !slots)) return set_err(BP_ERR_ARG, "bad slot list");
+    long long t = 0;
+    for (int i = 0; i < n; ++i) {
+        if (slots[i] < 0 || slots[i] >= c->nports)
+            return set_err(BP_ERR_ARG, "slot %d out of range", slots[i]);
+        t += c->msg_off[slots[i] + 1] - c->msg_off[slots[i]];
+    }
+    *total = t;
+    return BP_OK;
+}
+
+// Copies between the slots' messages and a packed FP64 host buffer (dir 0: host -> device).
+static int slot_copy(bp_context *c, int n, const int *slots, double *host, int dir) {
+    long long total = 0;
+    RETCHK(slot_span(c, n, slots, &total));
+    HIPCHK(hipSetDevice(c->device));
+    const bool f64 = c->fp64_active || c->dtype == BP_F64;
+    char *base = c->fp64_active ? (char *)c->d_msg64[c->cur64] : (char *)c->d_msg[c->cur];
+    const size_t esz = f64 ? sizeof(double) : sizeof(float);
+    std::vector<float> tmp(f64 ? 0 : (size_t)total);
+    if (!f64 && dir == 0)
+        for (long long i = 0; i < total; ++i) tmp[i] = (float)host[i];
+    long long off = 0;
+    for (int i = 0; i < n; ++i) {
+        const long long mo = c->msg_off[slots[i]], len = c->msg_off[slots[i] + 1] - mo;
+        char *dev = base + esz * mo;
+        char *h = f64 ? (char *)(host + off) : (char *)(tmp.data() + off);
+        if (dir == 0)
+            HIPCHK(hipMemcpyAsync(dev, h, esz * len, hipMemcpyHostToDevice, c->stream));
+        else
+            HIPCHK(hipMemcpyAsync(h, dev, esz * len, hipMemcpyDeviceToHost, c->stream));
+        off += len;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!f64 && dir == 1)
+        for (long long i = 0; i < total; ++i) host[i] = tmp[i];
+    return BP_OK;
+}
+
+int bp_set_slot_messages(bp_context *c, int n, const int *slots, const double *data) {
+    if (!c || (n > 0 && !data)) return set_err(BP_ERR_ARG, "NULL argument");
+    if (!c->have_msgs) return set_err(BP_ERR_STATE, "no messages (bp_init_messages first)");
+    return slot_copy(c, n, slots, const_cast<double *>(data), 0);
+}
+
+int bp_get_slot_messages(bp_context *c, int n, const int *slots, double *out) {
+    if (!c || (n > 0 && !out)) return set_err(BP_ERR_ARG, "NULL argument");
+    if (!c->have_msgs) return set_err(BP_ERR_STATE, "no messages");
+    return slot_copy(c, n, slots, out, 1);
+}
+
 int bp_set_path(bp_context *c, int path) {
     if (!c) return set_err(BP_ERR_ARG, "ctx is NULL");
     if (path != BP_PATH_AUTO && path != BP_PATH_GENERIC && path != BP_PATH_VERTEX)
         return set_err(BP_ERR_ARG, "unknown path %d", path);
+    if (path == BP_PATH_VERTEX && !c->normalize)
+        return set_err(BP_ERR_UNSUPPORTED, "the vertex kernels always normalise (option normalize is 0)");
     if (path == BP_PATH_VERTEX && !(c->vplan_ok && c->vplan.all))
         return set_err(BP_ERR_UNSUPPORTED, "vertex path not available for every network: %s",
                        c->vplan.why.c_str());
@@ -1140,6 +1204,13 @@ int bp_set_option(bp_context *c, const char *name, int value) {
         c->fold_on = value;
         return BP_OK;
     }
+    if (strcmp(name, "normalize") == 0) {
+        if (value != 0 && value != 1) return set_err(BP_ERR_ARG, "normalize is 0 or 1");
+        if (!value && c->path == BP_PATH_VERTEX)
+            return set_err(BP_ERR_UNSUPPORTED, "the vertex kernels always normalise");
+        c->normalize = value;
+        return BP_OK;
+    }
     if (strcmp(name, "vertex_bond_mask") == 0) {
         c->vx_mask = (unsigned)value & 0x1FFFEu;
         if (c->have_topo) RETCHK(rebuild_vertex_plan(c));
@@ -1154,6 +1225,7 @@ int bp_get_option(bp_context *c, const char *name, int *value) {
     if (strcmp(name, "vertex16_waves") == 0) { *value = c->v16_waves; return BP_OK; }
     if (strcmp(name, "vertex_bond_mask") == 0) { *value = (int)c->vx_mask; return BP_OK; }
     if (strcmp(name, "fold_bookkeeping") == 0) { *value = c->fold_on; return BP_OK; }
+    if (strcmp(name, "normalize") == 0) { *value = c->normalize; return BP_OK; }
     return set_err(BP_ERR_ARG, "unknown option %s", name);
 }
 
@@ -1207,7 +1279,10 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
                        c->fold_n);
     const NetCtl ctl = make_ctl(c, true);
     HIPCHK(hipGetLastError());
-    const int chunk = 16;  // sweeps between host checks of the done flags (even)
+    // sweeps between host checks of the done flags; a captured chunk bakes in the folded
+    // accumulator indices k % FOLD_BUFS, so every replay must start on residue 0
+    constexpr int chunk = 16;
+    static_assert(chunk % FOLD_BUFS == 0, "a captured sweep chunk must span whole accumulator cycles");
     // The run state (delta, done, iter, at: one block) comes back through a pinned buffer.
     // With a tolerance the host reads it after every chunk and stops once every network has
     // stopped; without one no network stops before maxiter, so every chunk is queued at once
@@ -1620,11 +1695,12 @@ int bp_update_slot(bp_context *c, int slot) {
     void *m = c->d_msg[c->cur];
     if (c->dtype == BP_F64)
         RETCHK(launch_update_generic<double>(c, (const double *)c->d_sites, (const double *)m,
-                                             (double *)m, make_ctl(c, false), slot, slot + 1));
+                                             (double *)m, make_ctl(c, false), slot, slot + 1,
+                                             nullptr, gen_flags(c, false)));
     else
         RETCHK(launch_update_generic<float>(c, generic_sites32(c), (const float *)m,
                                             (float *)m, make_ctl(c, false), slot, slot + 1,
-                                            nullptr, c->dtype == BP_BF16));
+                                            nullptr, gen_flags(c, c->dtype == BP_BF16)));
     HIPCHK(hipStreamSynchronize(c->stream));
     return BP_OK;
 }
@@ -1732,11 +1808,11 @@ int bp_run_sequential(bp_context *c, const int *order, int norder, int maxiter, 
             if (c->dtype == BP_F64)
                 rc = launch_update_generic<double>(c, (const double *)c->d_sites,
                                                    (const double *)m, (double *)m, off, order[i],
-                                                   order[i] + 1);
+                                                   order[i] + 1, nullptr, gen_flags(c, false));
             else
                 rc = launch_update_generic<float>(c, generic_sites32(c), (const float *)m,
                                                   (float *)m, off, order[i], order[i] + 1,
-                                                  nullptr, c->dtype == BP_BF16);
+                                                  nullptr, gen_flags(c, c->dtype == BP_BF16));
         }
         if (rc != BP_OK) break;
         ++it;
@@ -1842,7 +1918,8 @@ int bp_polish_fp64(bp_context *c, int nsweeps, double *out_delta) {
     }
     for (int i = 0; i < nsweeps; ++i) {
         RETCHK(launch_update_generic<double>(c, c->d_sites64, c->d_msg64[c->cur64],
-                                             c->d_msg64[c->cur64 ^ 1], make_ctl(c, false), 0, c->nports));
+                                             c->d_msg64[c->cur64 ^ 1], make_ctl(c, false), 0,
+                                             c->nports, nullptr, gen_flags(c, false)));
         c->cur64 ^= 1;
     }
     if (out_delta) {
@@ -1862,6 +1939,7 @@ int bp_polish_fp64(bp_context *c, int nsweeps, double *out_delta) {
 
 static int grad_lognorm_impl(bp_context *c, const GradOut &go) {
     RETCHK(ready(c));
+    if (!c->normalize) return set_err(BP_ERR_UNSUPPORTED, "gradients need normalised BP (option normalize)");
     if (c->plain) return set_err(BP_ERR_UNSUPPORTED, "gradient needs a norm network");
     HIPCHK(hipSetDevice(c->device));
     bool is64 = false;
@@ -1884,6 +1962,7 @@ static int tfim_impl(bp_context *c, double J, double h, double *energy, const Gr
     RETCHK(ready(c));
     if (!energy) return set_err(BP_ERR_ARG, "NULL argument");
     if (c->plain) return set_err(BP_ERR_UNSUPPORTED, "energy needs a norm network");
+    if (!c->normalize && go.any()) return set_err(BP_ERR_UNSUPPORTED, "gradients need normalised BP (option normalize)");
     for (int u = 0; u < c->nv; ++u)
         if (c->phys_dim[u] != 2) return set_err(BP_ERR_UNSUPPORTED, "TFIM needs phys dim 2");
     HIPCHK(hipSetDevice(c->device));
@@ -1942,6 +2021,7 @@ static int local_impl(bp_context *c, const double *ops_v, const double *ops_firs
     if ((ops_first == nullptr) != (ops_second == nullptr))
         return set_err(BP_ERR_ARG, "edge operators come in (first, second) pairs");
     if (c->plain) return set_err(BP_ERR_UNSUPPORTED, "energies need a norm network");
+    if (!c->normalize && go.any()) return set_err(BP_ERR_UNSUPPORTED, "gradients need normalised BP (option normalize)");
     for (int u = 0; u < c->nv; ++u)
         if (c->phys_dim[u] > ld)
             return set_err(BP_ERR_ARG, "operator block %d smaller than physical dimension %d", ld,

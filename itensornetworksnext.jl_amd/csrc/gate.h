@@ -133,7 +133,8 @@ GT_FN int gt_layout(const int *dsc, GtLayout &L) {
     L.n2 = L.d2 * L.Db;
     L.size1 = L.P1 * L.n1;
     L.size2 = L.P2 * L.n2;
-    if (L.size1 > 0x7fffffffLL || L.size2 > 0x7fffffffLL) return 1;  // 32-bit site indices
+    // 32-bit site indices: a grid-stride loop idx += GT_NT must not pass INT_MAX
+    if (L.size1 > 0x7fffffffLL - GT_NT || L.size2 > 0x7fffffffLL - GT_NT) return 1;
     L.r1 = (int)gt_min(L.P1, L.n1);
     L.r2 = (int)gt_min(L.P2, L.n2);
     L.m1 = L.r1 * L.d1;
@@ -419,7 +420,8 @@ GT_FN void gt_mode(const Ctx &c, const double *in, double *out, long long size, 
     long long inner = 1;
     for (int l = j + 1; l < k; ++l) inner *= dims[l];
     const int D = dims[j];
-    if (size <= 0x7fffffffLL)  // 32-bit index arithmetic (64-bit division is a long routine)
+    if (size <= 0x7fffffffLL - GT_NT)  // 32-bit index arithmetic (64-bit division is a long
+                                       // routine); idx += nt stays below INT_MAX
         gt_mode_idx<int>(c, in, out, (int)size, (int)inner, D, Mx);
     else
         gt_mode_idx<long long>(c, in, out, size, inner, D, Mx);
@@ -630,7 +632,8 @@ GT_FN void gt_site_qr(const Ctx &c, const GtLayout &L, int s, const double *A, d
     // to the Jacobi layout: tall (P >= n) columns (t, b), wide columns x
     double *J = ws + L.jac[s];
     const bool tall = P >= n;
-    // (32-bit index arithmetic; a gate's site tensor is far below 2^31 entries)
+    // (32-bit index arithmetic: the layout rejects site tensors of INT_MAX - GT_NT entries
+    // or more, so idx += nt cannot overflow)
     for (int idx = c.tid; idx < (int)size; idx += c.nt) {
         int rem = idx, x = 0, xs = 1;
         int bb = 0;

@@ -15,6 +15,9 @@
 
 // out[pre, b, post] = sum_a in[pre, a, post] * m[a, b]; m is na x nb (row-major).
 // trans != 0 uses m[b, a] instead (m square).
+#define GEN_FLAG_BF16 1          // BF16 product operands (FP32 kernels)
+#define GEN_FLAG_NO_NORMALIZE 2  // skip the sum normalisation
+
 template <typename T>
 __device__ void block_mode_product(const T *in, T *out, const int *dims, int r, int ax,
                                    const T *m, int nb, int trans = 0) {
@@ -171,7 +174,9 @@ __device__ void load_incoming(const DevTopo &tp, int u, const VertexShape &vs, c
 // One directed message per workgroup iteration: update, normalise, residual.  Work items
 // i in [item_begin, item_end) are message slots, or slot_list[i] when a list is given (the
 // slots of one dispatch bucket).
-// bf16 != 0 (FP32 only): every product operand is rounded to BF16 and sums stay in FP32, as
+// flags: GEN_FLAG_NO_NORMALIZE keeps the unnormalised message (SimpleMessageUpdate with
+// normalize = false, beliefpropagation.jl:237-253); GEN_FLAG_BF16 (FP32 only): every
+// product operand is rounded to BF16 and sums stay in FP32, as
 // on the vertex kernels: the sites arrive rounded, each incoming message is scaled by
 // 1 / max|m| (the scale cancels in the sum normalisation) and rounded, and every partial
 // product is rounded before it enters the next product.
@@ -181,10 +186,12 @@ __global__ __launch_bounds__(BS) void k_update_generic(
     NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs,
     int item_begin, int item_end, const int *__restrict__ slot_list, int mats_elems,
     int out_elems, int red_elems, long long buf_elems, int use_lds, T *ws, long long ws_stride,
-    int bf16) {
+    int flags) {
     extern __shared__ __align__(16) char smem[];
     GenSmem<T> sm = carve<T>(smem, mats_elems, out_elems, red_elems, buf_elems, use_lds, ws,
                              ws_stride);
+    const int bf16 = flags & GEN_FLAG_BF16;
+    const bool normalize = !(flags & GEN_FLAG_NO_NORMALIZE);
     for (int it = item_begin + blockIdx.x; it < item_end; it += gridDim.x) {
         const int q = slot_list ? slot_list[it] : it;
         const int u = tp.owner[q];
@@ -252,7 +259,7 @@ __global__ __launch_bounds__(BS) void k_update_generic(
         double dno = 0.0, dnn = 0.0, doo = 0.0, dmx = 0.0;
         for (int i = threadIdx.x; i < len; i += blockDim.x) {
             T x = sm.out[i];
-            if (S != 0.0) x = (T)((double)x / S);
+            if (normalize && S != 0.0) x = (T)((double)x / S);
             const double xo = (double)old[i], xn = (double)x;  // read before an in-place write
             mnew[mo + i] = x;
             dno += xn * xo;

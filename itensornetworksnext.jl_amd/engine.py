@@ -109,6 +109,25 @@ class Engine:
                 "ones": L.BP_INIT_ONES}[kind]
         L.check(self.lib.bp_init_messages(self.h, code, None), "bp_init_messages")
 
+    def set_slot_messages(self, slots, data) -> None:
+        """Write the messages of `slots` (packed FP64, slot-list order)."""
+        sl = _i32(slots)
+        d = _f64(data)
+        need = int(sum(int(self.topo.msg_off[q + 1] - self.topo.msg_off[q]) for q in sl))
+        if d.size != need:
+            raise ValueError(f"{d.size} values for slots holding {need}")
+        L.check(self.lib.bp_set_slot_messages(self.h, len(sl), _ip(sl), _dp(d)),
+                "bp_set_slot_messages")
+
+    def slot_messages(self, slots) -> np.ndarray:
+        """The messages of `slots`, packed FP64 in slot-list order."""
+        sl = _i32(slots)
+        need = int(sum(int(self.topo.msg_off[q + 1] - self.topo.msg_off[q]) for q in sl))
+        out = np.zeros(need)
+        L.check(self.lib.bp_get_slot_messages(self.h, len(sl), _ip(sl), _dp(out)),
+                "bp_get_slot_messages")
+        return out
+
     def messages(self) -> np.ndarray:
         out = np.zeros(int(self.topo.msg_off[-1]))
         L.check(self.lib.bp_download_messages(self.h, _dp(out)), "bp_download_messages")
@@ -174,9 +193,11 @@ class Engine:
         return fid, mab
 
     def iterate_diff(self, other) -> float:
+        o = _f64(other)
+        if o.size != int(self.topo.msg_off[-1]):
+            raise ValueError("message buffer size does not match the topology")
         out = C.c_double(0)
-        L.check(self.lib.bp_iterate_diff(self.h, _dp(_f64(other)), C.byref(out)),
-                "bp_iterate_diff")
+        L.check(self.lib.bp_iterate_diff(self.h, _dp(o), C.byref(out)), "bp_iterate_diff")
         return out.value
 
     # -- scalars

@@ -204,6 +204,102 @@ def normnetwork(tn: ITensorNetwork, braname=None) -> NormNetwork:
     return NormNetwork(tn, braname)
 
 
+def braname(nn: NormNetwork, name: Index) -> Index:
+    """The bra-layer name of `name`: its link partner, or itself for a site index
+    (normnetwork.jl:74-81)."""
+    return nn.bra_index(name)
+
+
+def kettensor(nn: NormNetwork, v) -> ITensor:
+    return nn.kettensor(v)
+
+
+def bratensor(nn: NormNetwork, v) -> ITensor:
+    return nn.bratensor(v)
+
+
+def conj_bratensor(nn: NormNetwork, v) -> ITensor:
+    return nn.conj_bratensor(v)
+
+
+class _LayerView:
+    """One layer of a NormNetwork seen as a network of its own (normnetworkview.jl:6-58):
+    the parent's graph, vertices and edges; vertex data is the layer's tensor; settable
+    and insertable only if the parent is (a NormNetwork is neither)."""
+
+    def __init__(self, parent: NormNetwork):
+        self.parent = parent
+
+    def vertices(self):
+        return self.parent.vertices()
+
+    def edges(self):
+        return self.parent.edges()
+
+    @property
+    def graph(self):
+        return self.parent.graph
+
+    def __getitem__(self, v) -> ITensor:
+        return self.get_vertex_data(v)
+
+    def get_vertex_data(self, v) -> ITensor:  # pragma: no cover - interface
+        raise NotImplementedError
+
+    def is_vertex_assigned(self, v) -> bool:
+        return v in self.parent.ket
+
+    def issettable(self) -> bool:
+        return False
+
+    def isinsertable(self) -> bool:
+        return False
+
+    def linkinds(self, edge) -> List[Index]:
+        """The layer's link indices shared by the edge's two tensors."""
+        u, w = edge
+        mine = self[u].inds
+        other = set(self[w].inds)
+        return [i for i in mine if i in other]
+
+    def linknames(self, edge) -> List[Index]:
+        return self.linkinds(edge)
+
+
+class KetView(_LayerView):
+    """The ket layer: the bare ket tensors (`kettensor`)."""
+
+    def get_vertex_data(self, v) -> ITensor:
+        return self.parent.kettensor(v)
+
+
+class BraView(_LayerView):
+    """The bra layer: conj of the ket tensors with the bra link names (`bratensor`)."""
+
+    def get_vertex_data(self, v) -> ITensor:
+        return self.parent.bratensor(v)
+
+
+def is_vertex_assigned(tn, v) -> bool:
+    if isinstance(tn, (KetView, BraView)):
+        return tn.is_vertex_assigned(v)
+    return v in (tn.ket if isinstance(tn, NormNetwork) else tn)
+
+
+def issettable(tn) -> bool:
+    return tn.issettable() if isinstance(tn, (KetView, BraView)) else not isinstance(tn, NormNetwork)
+
+
+def isinsertable(tn) -> bool:
+    return tn.isinsertable() if isinstance(tn, (KetView, BraView)) else not isinstance(tn, NormNetwork)
+
+
+def linknames(tn, edge) -> List[Index]:
+    if isinstance(tn, (KetView, BraView)):
+        return tn.linknames(edge)
+    return linkinds(tn, edge)
+
+
 # ------------------------------------------------------------------ flattening
 
 class Flattened:

@@ -29,28 +29,74 @@ def _edge(e) -> Edge:
 
 
 class MessageCache:
-    """Directed edge -> message."""
+    """Directed edge -> message, over a directed graph of its own (messagecache.jl:15-49).
 
-    def __init__(self, messages=None):
-        self.messages: Dict[Edge, np.ndarray] = {}
+    The graph starts from the vertices given and the keys of the messages (every key is an
+    edge); setting a message on a missing edge adds the edge (`set_edge_data!`, :92-96),
+    `rem_edge_` drops both (:54-58).  Numeric messages are stored as float64 arrays; any
+    other value (the reference's tests store strings) is kept as it is."""
+
+    def __init__(self, messages=None, vertices=None):
+        self.messages: Dict[Edge, object] = {}
+        self._vertices: Dict[Hashable, None] = {}
+        self._edges: Dict[Edge, None] = {}
         self.state = None  # stopping-criterion state of the run that produced this cache
+        for v in (vertices or ()):
+            self._vertices.setdefault(v, None)
         if messages is not None:
             items = messages.items() if hasattr(messages, "items") else messages
             for e, m in items:
                 self[e] = m
 
+    # -- graph interface
+    def _add_edge(self, e: Edge) -> None:
+        self._vertices.setdefault(e[0], None)
+        self._vertices.setdefault(e[1], None)
+        self._edges.setdefault(e, None)
+
+    def add_vertex(self, v) -> "MessageCache":
+        self._vertices.setdefault(v, None)
+        return self
+
+    def add_edge(self, e) -> "MessageCache":
+        self._add_edge(_edge(e))
+        return self
+
+    def rem_edge(self, e) -> "MessageCache":
+        e = _edge(e)
+        self.messages.pop(e, None)
+        self._edges.pop(e, None)
+        return self
+
+    def edges(self) -> List[Edge]:
+        return list(self._edges)
+
+    def vertices(self) -> List[Hashable]:
+        return list(self._vertices)
+
+    def has_edge(self, e) -> bool:
+        return _edge(e) in self._edges
+
+    def has_vertex(self, v) -> bool:
+        return v in self._vertices
+
     # -- dictionary interface
-    def __getitem__(self, e) -> np.ndarray:
+    def __getitem__(self, e):
         return self.messages[_edge(e)]
 
     def __setitem__(self, e, m) -> None:
-        self.messages[_edge(e)] = np.asarray(m, dtype=np.float64)
+        e = _edge(e)
+        self._add_edge(e)
+        self.messages[e] = _value(m)
 
     def __contains__(self, e) -> bool:
         return _edge(e) in self.messages
 
     def __len__(self) -> int:
         return len(self.messages)
+
+    def is_edge_assigned(self, e) -> bool:
+        return _edge(e) in self.messages
 
     def keys(self):
         return self.messages.keys()
@@ -61,26 +107,23 @@ class MessageCache:
     def items(self):
         return self.messages.items()
 
-    def edges(self) -> List[Edge]:
-        return list(self.messages.keys())
-
-    def vertices(self) -> List[Hashable]:
-        seen = {}
-        for (u, v) in self.messages:
-            seen.setdefault(u, None)
-            seen.setdefault(v, None)
-        return list(seen)
-
-    def has_edge(self, e) -> bool:
-        return _edge(e) in self.messages
-
     def copy(self) -> "MessageCache":
-        c = MessageCache({e: m.copy() for e, m in self.messages.items()})
+        c = MessageCache(vertices=self.vertices())
+        for e in self._edges:
+            c._add_edge(e)
+        for e, m in self.messages.items():
+            c.messages[e] = m.copy() if isinstance(m, np.ndarray) else m
         c.state = self.state
         return c
 
     def map(self, f) -> "MessageCache":
-        return MessageCache({e: f(m) for e, m in self.messages.items()})
+        """map(f, cache): same graph, f applied to every message (messagecache.jl:109-114)."""
+        c = MessageCache(vertices=self.vertices())
+        for e in self._edges:
+            c._add_edge(e)
+        for e, m in self.messages.items():
+            c[e] = f(m)
+        return c
 
     def __repr__(self):
         return f"MessageCache({len(self)} directed edges)"
@@ -103,17 +146,103 @@ class MessageCache:
     @classmethod
     def from_flat(cls, topo, flat) -> "MessageCache":
         own = topo.port_owner()
-        c = cls()
+        c = cls(vertices=list(topo.vertices))
         for q in range(topo.nports):
             e = (topo.vertices[own[q]], topo.vertices[topo.port_nbr[q]])
-            c.messages[e] = np.array(flat[topo.msg_off[q]:topo.msg_off[q + 1]]).reshape(
-                topo.msg_shape(q))
+            c[e] = np.array(flat[topo.msg_off[q]:topo.msg_off[q + 1]]).reshape(topo.msg_shape(q))
         return c
 
 
-def messagecache(pairs) -> MessageCache:
-    """messagecache(pairs) / messagecache(f, edges) (messagecache.jl:51-52)."""
-    return MessageCache(dict(pairs))
+def _value(m):
+    if isinstance(m, (np.ndarray, list, tuple, float, int, np.number)):
+        return np.asarray(m, dtype=np.float64)
+    return m
+
+
+def messagecache(f_or_pairs, edges=None) -> MessageCache:
+    """messagecache(pairs) and messagecache(f, edges) = messagecache(e => f(e) for e in
+    edges) (messagecache.jl:51-52)."""
+    if edges is None:
+        pairs = f_or_pairs
+        return MessageCache(dict(pairs.items() if hasattr(pairs, "items") else pairs))
+    return MessageCache({_edge(e): f_or_pairs(_edge(e)) for e in edges})
+
+
+def has_edge(cache: MessageCache, e) -> bool:
+    return cache.has_edge(e)
+
+
+def rem_edge_(cache: MessageCache, e) -> MessageCache:
+    """Graphs.rem_edge!(cache, edge): drops the message and the edge (messagecache.jl:54-58)."""
+    return cache.rem_edge(e)
+
+
+def add_vertex_(cache: MessageCache, v) -> MessageCache:
+    return cache.add_vertex(v)
+
+
+def set_edge_data_(cache: MessageCache, val, e) -> MessageCache:
+    """set_edge_data!: adds the edge when missing (messagecache.jl:92-96)."""
+    cache[e] = val
+    return cache
+
+
+def insert_edge_data_(cache: MessageCache, e, val) -> MessageCache:
+    """insert_edge_data!: a new key only (Dictionaries' insert! fails on an existing one)."""
+    if cache.is_edge_assigned(e):
+        raise KeyError(f"edge {e!r} already holds a message")
+    cache[e] = val
+    return cache
+
+
+def edge_data(cache: MessageCache) -> Dict[Edge, object]:
+    """The edge -> message dictionary."""
+    return cache.messages
+
+
+def edge_data_type(cache: MessageCache) -> type:
+    """The common type of the stored messages (valtype / edge_data_type)."""
+    ts = {type(m) for m in cache.values()}
+    return ts.pop() if len(ts) == 1 else object
+
+
+valtype = edge_data_type
+
+
+def copyto_(dst: MessageCache, src, edges=None) -> MessageCache:
+    """copyto!(dst, src) and copyto!(dst, src, edges): the messages of `src` (any edge ->
+    message mapping), or only those on `edges`, written into `dst` (messagecache.jl:42-49;
+    test_beliefpropagation.jl:61-79)."""
+    items = src.items() if hasattr(src, "items") else src
+    if edges is None:
+        for e, m in items:
+            dst[e] = m
+        return dst
+    for e in edges:
+        dst[e] = src[_edge(e)]
+    return dst
+
+
+def map_(f, dst: MessageCache, src) -> MessageCache:
+    """map!(f, dst, src): dst[e] = f(src[e]) for every key of src (messagecache.jl:116-120)."""
+    for e in list(src.keys()):
+        dst[e] = f(src[e])
+    return dst
+
+
+def subgraph(cache: MessageCache, vertices) -> MessageCache:
+    """The cache induced on `vertices`: those vertices, the edges between them and their
+    messages (DataGraphs' subgraph of an edge-data graph; test_beliefpropagation.jl:117-132)."""
+    keep = list(dict.fromkeys(vertices))
+    ks = set(keep)
+    sub = MessageCache(vertices=keep)
+    for e in cache.edges():
+        if e[0] in ks and e[1] in ks:
+            sub._add_edge(e)
+            if cache.is_edge_assigned(e):
+                m = cache[e]
+                sub.messages[e] = m.copy() if isinstance(m, np.ndarray) else m
+    return sub
 
 
 def incoming_messages(cache: MessageCache, edge) -> List[np.ndarray]:
@@ -322,10 +451,10 @@ def similar_message_environment(nn) -> MessageCache:
     fl = _flat(nn)
     topo = fl.topo
     own = topo.port_owner()
-    c = MessageCache()
+    c = MessageCache(vertices=list(topo.vertices))
     for q in range(topo.nports):
         e = (topo.vertices[own[q]], topo.vertices[topo.port_nbr[q]])
-        c.messages[e] = np.zeros(topo.msg_shape(q))
+        c[e] = np.zeros(topo.msg_shape(q))
     return c
 
 

@@ -619,62 +619,86 @@ static long long max_mat_store(const orc_graph *g) {
     return m > 0 ? m : 1;
 }
 
+/* Threads used by orc_norm_vjp (and so by orc_fixed_point_gradient); 1 by default. */
+static int g_vjp_threads = 1;
+
+void orc_set_num_threads(int n) { g_vjp_threads = n > 0 ? n : 1; }
+
+static long long max_msg_len(const orc_graph *g) {
+    long long m = 1;
+    int nports = g->port_off[g->nv];
+    for (int q = 0; q < nports; ++q)
+        if (msg_len(g, q) > m) m = msg_len(g, q);
+    return m;
+}
+
+/* The vertex-u share of orc_norm_vjp.  Every slot written into mbar is an in-slot of u
+ * (port_rev of a port of u), consumed by u alone, and abar is written at u's site only,
+ * so different vertices never write the same entry. */
+static void norm_vjp_vertex(const orc_graph *g, const double *sites, const double *msgs,
+                            const double *gbar, double *mbar, double *abar, int u, double *w0,
+                            double *w1, double *w2, double *mt, double *gg, double *tmp) {
+    int base = g->port_off[u], k = vdeg(g, u);
+    const double *A = sites + g->site_off[u];
+    for (int p = 0; p < k; ++p) {
+        int q = base + p;
+        long long n = msg_len(g, q);
+        /* recompute unnormalised g_q and its sum */
+        const double *mats[MAXR];
+        for (int j = 0; j < k; ++j) mats[j] = msgs + g->msg_off[g->port_rev[base + j]];
+        vertex_open_w(g, u, A, A, mats, p, tmp, w0, w1);
+        double S = 0.0;
+        for (long long i = 0; i < n; ++i) S += tmp[i];
+        const double *fb = gbar + g->msg_off[q];
+        /* through normalisation f = g / S:  gbar_g = (fbar - <fbar, f>) / S */
+        if (S != 0.0) {
+            double fd = 0.0;
+            for (long long i = 0; i < n; ++i) fd += fb[i] * tmp[i] / S;
+            for (long long i = 0; i < n; ++i) gg[i] = (fb[i] - fd) / S;
+        } else {
+            memcpy(gg, fb, sizeof(double) * n);
+        }
+        /* port p carries gbar_g, the others carry incoming messages */
+        mats[p] = gg;
+        if (mbar) {
+            for (int j = 0; j < k; ++j) {
+                if (j == p) continue;
+                vertex_open_w(g, u, A, A, mats, j, tmp, w0, w1);
+                double *mb = mbar + g->msg_off[g->port_rev[base + j]];
+                long long nj = msg_len(g, g->port_rev[base + j]);
+                for (long long i = 0; i < nj; ++i) mb[i] += tmp[i];
+            }
+        }
+        if (abar) {
+            accum_site_grad(g, u, A, A, mats, 1.0, 1.0, NULL, abar + g->site_off[u], w0, w2, mt);
+        }
+    }
+}
+
 int orc_norm_vjp(const orc_graph *g, const double *sites, const double *msgs,
                  const double *gbar, double *mbar, double *abar) {
     long long wn = max_site_size(g);
     long long ms = max_mat_store(g);
-    double *w0 = (double *)malloc(sizeof(double) * wn);
-    double *w1 = (double *)malloc(sizeof(double) * wn);
-    double *w2 = (double *)malloc(sizeof(double) * wn);
-    double *mt = (double *)malloc(sizeof(double) * ms);
-    double *gg = (double *)malloc(sizeof(double) * 1024 * 1024);
-    double *tmp = (double *)malloc(sizeof(double) * 1024 * 1024);
-    if (!w0 || !w1 || !w2 || !mt || !gg || !tmp) {
-        free(w0); free(w1); free(w2); free(mt); free(gg); free(tmp);
-        return 1;
-    }
-    for (int u = 0; u < g->nv; ++u) {
-        int base = g->port_off[u], k = vdeg(g, u);
-        const double *A = sites + g->site_off[u];
-        for (int p = 0; p < k; ++p) {
-            int q = base + p;
-            long long n = msg_len(g, q);
-            int D = g->edge_dim[g->port_edge[q]];
-            /* recompute unnormalised g_q and its sum */
-            const double *mats[MAXR];
-            for (int j = 0; j < k; ++j) mats[j] = msgs + g->msg_off[g->port_rev[base + j]];
-            vertex_open_w(g, u, A, A, mats, p, tmp, w0, w1);
-            double S = 0.0;
-            for (long long i = 0; i < n; ++i) S += tmp[i];
-            const double *fb = gbar + g->msg_off[q];
-            /* through normalisation f = g / S:  gbar_g = (fbar - <fbar, f>) / S */
-            if (S != 0.0) {
-                double fd = 0.0;
-                for (long long i = 0; i < n; ++i) fd += fb[i] * tmp[i] / S;
-                for (long long i = 0; i < n; ++i) gg[i] = (fb[i] - fd) / S;
-            } else {
-                memcpy(gg, fb, sizeof(double) * n);
-            }
-            (void)D;
-            /* port p carries gbar_g, the others carry incoming messages */
-            mats[p] = gg;
-            if (mbar) {
-                for (int j = 0; j < k; ++j) {
-                    if (j == p) continue;
-                    vertex_open_w(g, u, A, A, mats, j, tmp, w0, w1);
-                    double *mb = mbar + g->msg_off[g->port_rev[base + j]];
-                    long long nj = msg_len(g, g->port_rev[base + j]);
-                    for (long long i = 0; i < nj; ++i) mb[i] += tmp[i];
-                }
-            }
-            if (abar) {
-                accum_site_grad(g, u, A, A, mats, 1.0, 1.0, NULL, abar + g->site_off[u], w0, w2,
-                                mt);
-            }
+    long long ml = max_msg_len(g);
+    int fail = 0;
+#pragma omp parallel num_threads(g_vjp_threads) reduction(| : fail)
+    {
+        double *w0 = (double *)malloc(sizeof(double) * wn);
+        double *w1 = (double *)malloc(sizeof(double) * wn);
+        double *w2 = (double *)malloc(sizeof(double) * wn);
+        double *mt = (double *)malloc(sizeof(double) * ms);
+        double *gg = (double *)malloc(sizeof(double) * ml);
+        double *tmp = (double *)malloc(sizeof(double) * ml);
+        if (!w0 || !w1 || !w2 || !mt || !gg || !tmp) {
+            fail = 1;
+        } else {
+#pragma omp for schedule(dynamic, 4)
+            for (int u = 0; u < g->nv; ++u)
+                norm_vjp_vertex(g, sites, msgs, gbar, mbar, abar, u, w0, w1, w2, mt, gg, tmp);
         }
+        free(w0); free(w1); free(w2); free(mt); free(gg); free(tmp);
     }
-    free(w0); free(w1); free(w2); free(mt); free(gg); free(tmp);
-    return 0;
+    return fail;
 }
 
 int orc_grad_lognorm_explicit(const orc_graph *g, const double *sites, const double *msgs,

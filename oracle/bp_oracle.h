@@ -120,6 +120,8 @@ int orc_vertex_env(const orc_graph *g, int u, const double *Abra, const double *
  *   mbar[q_in]  += (d F / d m)^T gbar   (into the incoming message slots)
  *   abar[site]  += (d F / d A)^T gbar   (real site tensors, ket and bra both)
  * where F is one Jacobi sweep followed by sum normalisation.  Either output may be NULL. */
+/* Threads for orc_norm_vjp and orc_fixed_point_gradient (OpenMP over vertices; default 1). */
+void orc_set_num_threads(int n);
 int orc_norm_vjp(const orc_graph *g, const double *sites, const double *msgs,
                  const double *gbar, double *mbar, double *abar);
 

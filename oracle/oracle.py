@@ -68,6 +68,8 @@ def _declare(L):
                                        C.POINTER(C.c_longlong), C.c_longlong, _ip, _ip, _dp,
                                        C.c_longlong]
     L.orc_norm_vjp.argtypes = [G, _dp, _dp, _dp, _dp, _dp]
+    L.orc_set_num_threads.argtypes = [C.c_int]
+    L.orc_set_num_threads.restype = None
     L.orc_grad_lognorm_explicit.argtypes = [G, _dp, _dp, _dp]
     L.orc_tfim_energy.argtypes = [G, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp]
     L.orc_expect_vertex.argtypes = [G, _dp, _dp, _dp, C.c_int, _dp]
@@ -271,7 +273,13 @@ def plain_exact(topo, sites):
     return float(val)
 
 
-def norm_vjp(topo, sites, msgs, gbar, want_m=True, want_a=True):
+def set_num_threads(n):
+    """Threads of norm_vjp / fixed_point_gradient (OpenMP over vertices)."""
+    lib().orc_set_num_threads(int(n))
+
+
+def norm_vjp(topo, sites, msgs, gbar, want_m=True, want_a=True, nthreads=1):
+    set_num_threads(nthreads)
     g = OracleGraph(topo)
     mbar = np.zeros(int(topo.msg_off[-1])) if want_m else None
     abar = np.zeros(int(topo.site_off[-1])) if want_a else None
@@ -337,7 +345,8 @@ def local_energy(topo, sites, msgs, ops_v=None, ops_first=None, ops_second=None,
     return (E.value, dm, da) if partials else E.value
 
 
-def fixed_point_gradient(topo, sites, msgs, dL_dm, dL_dA, maxiter=1000, tol=1e-12):
+def fixed_point_gradient(topo, sites, msgs, dL_dm, dL_dA, maxiter=1000, tol=1e-12, nthreads=1):
+    set_num_threads(nthreads)
     g = OracleGraph(topo)
     grad = np.zeros(int(topo.site_off[-1]))
     it = C.c_int(0)

@@ -481,3 +481,84 @@ def test_gate_oracle_truncation_is_optimal_on_a_tree(pkg, orc):
         best = (U[:, :k] * S[:k]) @ Vt[:k]
         assert np.max(np.abs(full(T).reshape(4, 4) - best)) < 1e-10 * np.max(np.abs(best))
         assert np.max(np.abs(Sk / Sk[0] - S[:k] / S[0])) < 1e-10
+
+
+# ------------------------------------------------------------ iterate_diff closed forms
+# beliefpropagation.jl:261-267: max over directed edges of 1 - |<m1^, m2^>|^2 with
+# m^ = m / ||m||_F.  For I_2 against diag(1, 0): <I/sqrt2, diag(1,0)> = 1/sqrt2, so the
+# distance is 1 - 1/2 = 0.5 (without the square it would be 1 - 0.707 = 0.293); m against
+# -m or 3m: 0 (phase and scale drop out); orthogonal messages: 1.
+
+def _two_vertex_topo(pkg):
+    t = pkg.build_topology(pkg.named_path_graph(2), 2, 2)
+    assert t.nports == 2
+    return t
+
+
+def _pack(t, mats):
+    out = np.zeros(int(t.msg_off[-1]))
+    for q, m in enumerate(mats):
+        out[t.msg_off[q]:t.msg_off[q + 1]] = np.asarray(m, dtype=float).ravel()
+    return out
+
+
+@pytest.mark.parametrize("a,b,want", [
+    (np.eye(2), np.diag([1.0, 0.0]), 0.5),
+    (np.array([[1.0, 2.0], [3.0, 4.0]]), -np.array([[1.0, 2.0], [3.0, 4.0]]), 0.0),
+    (np.array([[1.0, 2.0], [3.0, 4.0]]), 3.0 * np.array([[1.0, 2.0], [3.0, 4.0]]), 0.0),
+    (np.diag([1.0, 0.0]), np.diag([0.0, 1.0]), 1.0),
+    (np.array([[1.0, 1.0], [0.0, 0.0]]), np.array([[1.0, 0.0], [0.0, 0.0]]), 0.5),
+])
+def test_iterate_diff_closed_forms(pkg, orc, a, b, want):
+    t = _two_vertex_topo(pkg)
+    m1 = _pack(t, [a, a])
+    m2 = _pack(t, [b, b])
+    assert orc.iterate_diff(t, m1, m2) == pytest.approx(want, abs=1e-15)
+
+
+def test_iterate_diff_is_the_maximum_over_edges(pkg, orc):
+    t = _two_vertex_topo(pkg)
+    m1 = _pack(t, [np.eye(2), np.eye(2)])
+    m2 = _pack(t, [np.eye(2), np.diag([1.0, 0.0])])
+    assert orc.iterate_diff(t, m1, m2) == pytest.approx(0.5, abs=1e-15)
+    assert orc.iterate_diff(t, m2, m1) == pytest.approx(0.5, abs=1e-15)
+
+
+# ------------------------------------------------------------ loop-count semantics
+# StopAfterIteration(maxiter) | StopWhenConverged(tol) (AlgorithmsInterfaceExtensions.jl:
+# 84-109).  On an N-vertex path, Jacobi message m_{k -> k+1} depends on the k - 1 messages
+# behind it, so every message is exact after N - 1 sweeps and not before; sweep N repeats
+# sweep N - 1 bit for bit (same inputs), so its max-abs change is exactly 0 and the run
+# stops at iteration N.  These counts are fixed by the path length, not by the oracle.
+
+@pytest.mark.parametrize("N", [2, 4, 7])
+def test_path_jacobi_exact_after_n_minus_1_sweeps(pkg, orc, N):
+    t = pkg.build_topology(pkg.named_path_graph(N), 3, 2)
+    s = pkg.peps.random_sites(t, 5)
+    m0 = pkg.peps.ones_messages(t)
+    exact = math.log(orc.norm_exact(t, s))
+    fixed, _, _, _ = orc.bp(t, s, m0, maxiter=3 * N)
+    m, sw, dl, at = orc.bp(t, s, m0, maxiter=N - 1)
+    assert (sw, at) == (N - 1, -1)
+    np.testing.assert_allclose(m, fixed, rtol=0, atol=1e-14)
+    assert orc.bethe_free_energy(t, s, m) == pytest.approx(exact, abs=1e-12)
+    if N > 2:
+        # the messages m_{1->2} ... m_{N-1->N} chain: one of them is still off
+        m, _, _, _ = orc.bp(t, s, m0, maxiter=N - 2)
+        assert np.max(np.abs(m - fixed)) > 1e-6
+    m, sw, dl, at = orc.bp(t, s, m0, maxiter=100, tol=1e-300, criterion="maxabs")
+    assert (sw, at, dl) == (N, N, 0.0)
+
+
+def test_convergence_check_skips_iteration_zero(pkg, orc):
+    """With a tolerance every delta passes, the run still sweeps once: the check at
+    iteration 0 (where delta would be 0) is skipped (AIE.jl:98-99)."""
+    t = pkg.build_topology(pkg.named_grid((3, 3)), 2, 2)
+    s = pkg.peps.random_sites(t, 6)
+    m0 = pkg.peps.ones_messages(t)
+    for crit in ("fidelity", "maxabs"):
+        _, sw, dl, at = orc.bp(t, s, m0, tol=1e300, criterion=crit)
+        assert (sw, at) == (1, 1)
+        assert dl > 0.0
+    _, sw, dl, at = orc.bp(t, s, m0, maxiter=5)
+    assert (sw, at) == (5, -1) and dl == math.inf
