@@ -396,6 +396,74 @@ def large_bond_timing(pkg, torch, seed, L=24, D=16):
     return out
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def precision_vs_oracle(pkg, oracle, threads, seed):
+    """Relative errors (max-abs over the largest entry) of the GPU results against the FP64
+    oracle's converged solution: the 32x32 D=8 FP32 forward at fidelity 1e-7 and 1e-12,
+    its log-norm and TFIM gradients, FP64 polishing; the 24x24 D=16 BF16 solve and each
+    polishing stage."""
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    out = {}
+    t = pkg.build_topology(pkg.named_grid((32, 32)), 8, 2)
+    s = pkg.peps.random_sites(t, seed, scale=pkg.peps.peps_scale(2, 8))
+    m0 = pkg.peps.identity_messages(t)
+    m64, _, _, _ = oracle.bp(t, s, m0, maxiter=400, tol=1e-15, criterion="maxabs",
+                             nthreads=threads)
+    ln64 = oracle.bethe_free_energy(t, s, m64)
+    gl64 = oracle.grad_lognorm_explicit(t, s, m64)
+    E64, dm, da = oracle.tfim_energy(t, s, m64, 1.0, 3.0, partials=True)
+    g64, _, _ = oracle.fixed_point_gradient(t, s, m64, dm, da, maxiter=500, tol=1e-13,
+                                            nthreads=threads)
+    with pkg.Engine(t, s, dtype="f32") as e:
+        for tol in (1e-7, 1e-12):
+            e.set_messages(m0)
+            r = e.run(maxiter=200, tol=tol)
+            entry = {"sweeps": int(r.sweeps[0]), "messages": rel(e.messages(), m64),
+                     "lognorm": abs(e.lognorm() - ln64) / abs(ln64)}
+            if tol == 1e-12:
+                E, g, it, res = e.tfim_energy(1.0, 3.0, grad=True, adj_tol=1e-6,
+                                              adj_maxiter=200)
+                entry.update({"lognorm_gradient": rel(e.grad_lognorm(), gl64),
+                              "tfim_energy": abs(E - E64) / abs(E64), "tfim_gradient": rel(g, g64),
+                              "adjoint_sweeps": it})
+            out[f"c2_fp32_fidelity_{tol:g}"] = entry
+        e.polish_fp64(10)
+        out["c2_fp32_then_10_fp64_polish"] = {"messages": rel(e.messages(), m64),
+                                              "lognorm": abs(e.lognorm() - ln64) / abs(ln64)}
+    t = pkg.build_topology(pkg.named_grid((24, 24)), 16, 2)
+    s = pkg.peps.random_sites(t, seed, scale=pkg.peps.peps_scale(2, 16))
+    m0 = pkg.peps.identity_messages(t)
+    m64, _, _, _ = oracle.bp(t, s, m0, maxiter=60, tol=1e-13, criterion="maxabs",
+                             nthreads=threads)
+    ln64 = oracle.bethe_free_energy(t, s, m64)
+    with pkg.Engine(t, s, dtype="bf16") as e:
+        e.set_messages(m0)
+        r = e.run(maxiter=200, tol=1e-5)
+        stages = {"bf16": (int(r.sweeps[0]), 0)}
+        errs = {}
+        for name, n in (("bf16", 0), ("bf16_then_3_fp64_polish", 3),
+                        ("bf16_then_6_fp64_polish", 3)):
+            if n:
+                e.polish_fp64(n)
+            errs[name] = {"messages": rel(e.messages(), m64),
+                          "lognorm": abs(e.lognorm() - ln64) / abs(ln64)}
+        errs["bf16"]["sweeps"] = stages["bf16"][0]
+    out["c5_24x24_D16"] = errs
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -589,20 +657,30 @@ def main(argv=None):
     if not args.no_cpu:
         oracle = ge.import_oracle()
         threads = min(os.cpu_count() or 1, 16)
-        m0 = pkg.peps.identity_messages(topo)
-        n = 0
-        t2 = time.perf_counter()
-        cur = m0
-        while True:
-            cur, _, _ = oracle.sweep_jacobi(topo, sites, cur, nthreads=threads)
-            n += 1
-            if time.perf_counter() - t2 > args.cpu_seconds:
-                break
-        wc = time.perf_counter() - t2
+
+        def cpu_rate(nthreads, seconds):
+            cur = pkg.peps.identity_messages(topo)
+            n = 0
+            t2 = time.perf_counter()
+            while True:
+                cur, _, _ = oracle.sweep_jacobi(topo, sites, cur, nthreads=nthreads)
+                n += 1
+                if time.perf_counter() - t2 > seconds:
+                    break
+            return n, time.perf_counter() - t2
+
+        n1, w1 = cpu_rate(1, max(2.0, args.cpu_seconds / 3))
+        n, wc = cpu_rate(threads, args.cpu_seconds)
         cpu = {"value": nmsg * n / wc, "unit": "msg updates/s", "cores": threads,
                "kind": "port",
-               "sample": f"{n} FP64 Jacobi sweeps of the same 32x32 D=8 network "
-                         f"({n * nmsg} message updates) by the C oracle with OpenMP"}
+               "sample": f"{n} FP64 Jacobi sweeps of the same {L}x{L} D={D} network "
+                         f"({n * nmsg} message updates) by the C oracle with OpenMP over "
+                         f"{threads} threads; the 1-thread leg ran {n1} sweeps",
+               "value_1_thread": nmsg * n1 / w1, "cpu_model": cpu_model()}
+        if not args.no_extras and args.dtype == "f32" and (L, D) == (32, 8):
+            # the oracle as the checker: errors of the GPU results against FP64
+            extras["precision_vs_fp64_oracle"] = precision_vs_oracle(pkg, oracle, threads,
+                                                                     args.seed)
 
     line = {
         "metric": "BP msg updates/s, 32x32 D=8 PEPS norm net",

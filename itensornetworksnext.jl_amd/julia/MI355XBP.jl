@@ -25,10 +25,11 @@ exercised by the Python ctypes binding and its tests.
 module MI355XBP
 
 using ITensorNetworksNext: ITensorNetworksNext, MessageCache, MessageUpdateAlgorithm,
-    NormNetwork, beliefpropagation, kettensor, linkinds, siteinds
+    NormNetwork, beliefpropagation, braname, kettensor, linkinds, siteinds
+using ITensorNetworksNext.AlgorithmsInterfaceExtensions: AlgorithmsInterfaceExtensions as AIE
 using ITensorBase: dimnames, unnamed, name
 using Graphs: vertices, edges, src, dst, neighbors
-import ITensorNetworksNext: message_update!
+import ITensorNetworksNext: message_update!, bethe_free_energy
 
 const libbp = Ref{String}(joinpath(@__DIR__, "..", "csrc", "libbpmi355x.so"))
 
@@ -45,6 +46,7 @@ end
 """A device context holding one flattened NormNetwork."""
 mutable struct DeviceContext
     handle::Ptr{Cvoid}
+    nn::NormNetwork
     vertices::Vector{Any}
     slots::Dict{Any, UnitRange{Int}}   # directed edge => range in the flat message buffer
     dims::Dict{Any, Int}
@@ -108,9 +110,26 @@ function DeviceContext(nn::NormNetwork; dtype::Symbol = :f32, device::Integer = 
         dims[(v, w)] = D
         off += D * D
     end
-    ctx = DeviceContext(h[], vs, slots, dims, off)
+    ctx = DeviceContext(h[], nn, vs, slots, dims, off)
     finalizer(c -> ccall((:bp_destroy, libbp[]), Cint, (Ptr{Cvoid},), c.handle), ctx)
     return ctx
+end
+
+"""The names of the message of edge v => w in the engine's axis order [ket][bra]: the ket
+link name the edge's tensors share and its bra partner (`braname`,
+src/normnetwork.jl:74-81).  The reference builds each message with named bra and ket axes
+(src/beliefpropagation/messagecache.jl:205-225); permuting by these names, not by the
+message's storage order, keeps the orientation right whatever order `similar_operator`
+chose."""
+function message_names(nn::NormNetwork, v, w)
+    k = name(only(linkinds(nn.ket, (v, w))))
+    return (k, braname(nn, k))
+end
+
+# the message of edge v => w as a D x D Float64 matrix indexed [ket, bra]
+function ketbra_matrix(nn::NormNetwork, cache, v, w)
+    m = cache[v => w]
+    return Array{Float64}(unnamed(m, message_names(nn, v, w)...))
 end
 
 """Messages [ket][bra] as row-major D x D blocks of the flat buffer (the message of
@@ -118,8 +137,8 @@ edge v => w is produced at v and consumed at w)."""
 function upload!(ctx::DeviceContext, cache)
     flat = zeros(Float64, ctx.nmsg)
     for (e, r) in ctx.slots
-        m = Array{Float64}(unnamed(cache[e[1] => e[2]]))
-        flat[r] = vec(permutedims(m, (2, 1)))
+        m = ketbra_matrix(ctx.nn, cache, e[1], e[2])
+        flat[r] = vec(permutedims(m, (2, 1)))  # row-major [ket][bra]
     end
     GC.@preserve flat check(ccall((:bp_init_messages, libbp[]), Cint,
         (Ptr{Cvoid}, Cint, Ptr{Float64}), ctx.handle, BP_INIT_UPLOAD, flat), "bp_init_messages")
@@ -133,7 +152,9 @@ function download!(cache, ctx::DeviceContext)
     for (e, r) in ctx.slots
         D = ctx.dims[e]
         msg = cache[e[1] => e[2]]
-        copyto!(unnamed(msg), permutedims(reshape(flat[r], D, D), (2, 1)))
+        # [ket, bra] matrix, written through the message's own axis names
+        copyto!(unnamed(msg, message_names(ctx.nn, e[1], e[2])...),
+            permutedims(reshape(flat[r], D, D), (2, 1)))
     end
     return cache
 end
@@ -156,6 +177,56 @@ function message_update!(alg::DeviceJacobiUpdate, cache, factors::NormNetwork, :
     upload!(ctx, cache)
     check(ccall((:bp_sweep, libbp[]), Cint, (Ptr{Cvoid},), ctx.handle), "bp_sweep")
     download!(cache, ctx)
+    return cache
+end
+
+"""
+    DeviceMessageCache(ctx)
+
+Messages that stay in the device context: `message_update!(alg, cache::DeviceMessageCache,
+factors, ::JacobiSweep)` sweeps without moving a message over PCIe; `copy` takes the host
+snapshot that `StopWhenConverged` keeps as its previous iterate
+(AlgorithmsInterfaceExtensions.jl:74, 96); `AIE.iterate_diff` and `bethe_free_energy`
+evaluate on the device (the methods the reference's generic code calls,
+beliefpropagation.jl:261-267, messagecache.jl:185-201).
+"""
+struct DeviceMessageCache
+    ctx::DeviceContext
+end
+
+"""A host snapshot of a device cache's messages (flat, the engine's slot layout)."""
+struct DeviceSnapshot
+    flat::Vector{Float64}
+end
+
+function Base.copy(c::DeviceMessageCache)
+    flat = zeros(Float64, c.ctx.nmsg)
+    GC.@preserve flat check(ccall((:bp_download_messages, libbp[]), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}), c.ctx.handle, flat), "bp_download_messages")
+    return DeviceSnapshot(flat)
+end
+
+# iterate_diff of the device messages against the previous iterate's snapshot
+function AIE.iterate_diff(c::DeviceMessageCache, prev::DeviceSnapshot)
+    out = Ref{Float64}(0.0)
+    GC.@preserve prev check(ccall((:bp_iterate_diff, libbp[]), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ref{Float64}), c.ctx.handle, prev.flat, out),
+        "bp_iterate_diff")
+    return out[]
+end
+
+bethe_free_energy(::NormNetwork, c::DeviceMessageCache) = device_lognorm(c.ctx)
+
+"""A device cache for `nn` holding `messages` (uploaded once)."""
+function device_messagecache(alg::DeviceJacobiUpdate, nn::NormNetwork, messages)
+    ctx = context(alg, nn)
+    upload!(ctx, MessageCache(messages))
+    return DeviceMessageCache(ctx)
+end
+
+function message_update!(::DeviceJacobiUpdate, cache::DeviceMessageCache, ::NormNetwork,
+        ::JacobiSweep)
+    check(ccall((:bp_sweep, libbp[]), Cint, (Ptr{Cvoid},), cache.ctx.handle), "bp_sweep")
     return cache
 end
 
@@ -323,27 +394,78 @@ function autotune!(ctx::DeviceContext; cache::AbstractString = "", mode::Integer
     return Int(n[])
 end
 
-# ChainRulesCore rrule: the pullback of the converged-BP energy with respect to the flat
-# site buffer is the adjoint gradient (implicit differentiation of the fixed point).
-function tfim_energy(ctx::DeviceContext, sites::AbstractVector{Float64}; J = 1.0, h = 3.0)
+# ChainRulesCore rrules: the pullback of a converged-BP quantity with respect to the flat
+# site buffer is its adjoint gradient (implicit differentiation of the fixed point).  Each
+# rrule converges BP once and takes the value and the gradient from one library call.
+
+function converge!(ctx::DeviceContext, sites::AbstractVector{Float64}; maxiter = 2000,
+        tol = 1.0e-12)
     GC.@preserve sites check(ccall((:bp_upload_sites, libbp[]), Cint,
         (Ptr{Cvoid}, Ptr{Float64}), ctx.handle, sites), "bp_upload_sites")
     check(ccall((:bp_init_messages, libbp[]), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}),
         ctx.handle, Cint(0), C_NULL), "bp_init_messages")
     check(ccall((:bp_run, libbp[]), Cint,
             (Ptr{Cvoid}, Cint, Cdouble, Cint, Cint, Ptr{Cint}, Ptr{Float64}, Ptr{Cint}),
-            ctx.handle, 2000, 1.0e-12, BP_CRIT_FIDELITY, 1, C_NULL, C_NULL, C_NULL), "bp_run")
-    E, _ = device_tfim_energy_and_grad(ctx, length(sites); J, h)
-    return E
+            ctx.handle, maxiter, tol, BP_CRIT_FIDELITY, 1, C_NULL, C_NULL, C_NULL), "bp_run")
+    return ctx
+end
+
+function tfim_energy(ctx::DeviceContext, sites::AbstractVector{Float64}; J = 1.0, h = 3.0)
+    converge!(ctx, sites)
+    E = Ref{Float64}(0.0)
+    it = Ref{Cint}(0)
+    res = Ref{Float64}(0.0)
+    check(ccall((:bp_tfim_energy, libbp[]), Cint,
+        (Ptr{Cvoid}, Cdouble, Cdouble, Ref{Float64}, Ptr{Float64}, Cdouble, Cint, Ref{Cint},
+            Ref{Float64}), ctx.handle, J, h, E, C_NULL, 1.0e-8, 1000, it, res),
+        "bp_tfim_energy")
+    return E[]
+end
+
+"""BP log-norm (bethe_free_energy) of the network with site buffer `sites`."""
+function lognorm(ctx::DeviceContext, sites::AbstractVector{Float64})
+    converge!(ctx, sites)
+    return device_lognorm(ctx)
+end
+
+"""Local energy sum_v <O_v> + sum_e <A_e B_e> of the network with site buffer `sites`."""
+function local_energy(ctx::DeviceContext, sites::AbstractVector{Float64}, O, A, B)
+    converge!(ctx, sites)
+    ro, ra, rb = _rowmajor(O), _rowmajor(A), _rowmajor(B)
+    E = Ref{Float64}(0.0)
+    it = Ref{Cint}(0)
+    res = Ref{Float64}(0.0)
+    GC.@preserve ro ra rb check(ccall((:bp_local_energy, libbp[]), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cint, Ref{Float64},
+            Ptr{Float64}, Cdouble, Cint, Ref{Cint}, Ref{Float64}), ctx.handle, ro, ra, rb,
+        size(O, 1), E, C_NULL, 1.0e-8, 1000, it, res), "bp_local_energy")
+    return E[]
 end
 
 using ChainRulesCore: ChainRulesCore, NoTangent
 
 function ChainRulesCore.rrule(::typeof(tfim_energy), ctx::DeviceContext,
         sites::AbstractVector{Float64}; J = 1.0, h = 3.0)
-    E = tfim_energy(ctx, sites; J, h)
-    _, g = device_tfim_energy_and_grad(ctx, length(sites); J, h)
+    converge!(ctx, sites)
+    E, g = device_tfim_energy_and_grad(ctx, length(sites); J, h)  # value and gradient at once
     pullback(ΔE) = (NoTangent(), NoTangent(), ΔE .* g)
+    return E, pullback
+end
+
+function ChainRulesCore.rrule(::typeof(lognorm), ctx::DeviceContext,
+        sites::AbstractVector{Float64})
+    converge!(ctx, sites)
+    L = device_lognorm(ctx)
+    g = device_grad_lognorm(ctx, length(sites))
+    pullback(ΔL) = (NoTangent(), NoTangent(), ΔL .* g)
+    return L, pullback
+end
+
+function ChainRulesCore.rrule(::typeof(local_energy), ctx::DeviceContext,
+        sites::AbstractVector{Float64}, O, A, B)
+    converge!(ctx, sites)
+    E, g = device_local_energy_and_grad(ctx, length(sites), O, A, B)
+    pullback(ΔE) = (NoTangent(), NoTangent(), ΔE .* g, NoTangent(), NoTangent(), NoTangent())
     return E, pullback
 end
 

@@ -173,3 +173,66 @@ def test_resident_rejects_overlapping_gates(pkg):
         G = np.eye(4)
         with pytest.raises(pkg._lib.BPError, match="earlier gate"):
             dn.engine.apply_gates([2, 2], [q12, q23], [G, G])
+
+
+def test_resident_layer_against_oracle(pkg, orc):
+    """One resident layer checked directly against the oracle: every gate against
+    orc.apply_gate2 on the same site tensors and converged messages (the two-site product
+    and the singular values), then BP on the gated network against orc.bp on the oracle's
+    gated sites (the log-norm, which does not depend on the bond gauge of the factors)."""
+    _say("case resident layer vs oracle apply_gate2 + bp")
+    g = pkg.named_grid((4, 4))
+    D = 3
+    tn, site, rng = _net(pkg, g, D, 2, 33)
+    gates, pairs = _layer(pkg, rng, g, site)
+    with pkg.DeviceNetwork(tn, dtype="f64") as dn:
+        assert dn.beliefpropagation(maxiter=3000, tol=1e-14, criterion="maxabs").converged[0]
+        t = dn.topo
+        sites0, msgs = dn.engine.sites(), dn.engine.messages()
+
+        def tensor(sites, u):
+            dims = [int(t.phys_dim[u])] + [int(t.edge_dim[t.port_edge[q]])
+                                          for q in range(t.port_off[u], t.port_off[u + 1])]
+            return sites[t.site_off[u]:t.site_off[u + 1]].reshape(dims)
+
+        def incoming(u, w):
+            out = {}
+            for j, q in enumerate(range(t.port_off[u], t.port_off[u + 1])):
+                if int(t.port_nbr[q]) == w:
+                    continue
+                sl = int(t.port_rev[q])
+                Dq = int(t.edge_dim[t.port_edge[q]])
+                out[j] = msgs[t.msg_off[sl]:t.msg_off[sl + 1]].reshape(Dq, Dq)
+            return out
+
+        def port(u, w):
+            return [int(t.port_nbr[q]) for q in range(t.port_off[u], t.port_off[u + 1])].index(w)
+
+        want = {}
+        osites = sites0.copy()
+        for op in gates:
+            kind, slot, mat, vs = dn._gate(op)
+            u1, u2 = dn._vidx[vs[0]], dn._vidx[vs[1]]
+            p1, p2 = port(u1, u2), port(u2, u1)
+            B1, B2, S = orc.apply_gate2(tensor(sites0, u1), tensor(sites0, u2), p1, p2,
+                                        incoming(u1, u2), incoming(u2, u1), mat, maxdim=D)
+            assert len(S) == D
+            want[(u1, u2)] = (p1, p2, B1, B2, S)
+            osites[t.site_off[u1]:t.site_off[u1 + 1]] = B1.ravel()
+            osites[t.site_off[u2]:t.site_off[u2 + 1]] = B2.ravel()
+        S_dev = dn.apply_operators(gates, trunc=pkg.truncrank(D))
+        sites1 = dn.engine.sites()
+        for ((u1, u2), (p1, p2, B1, B2, S)), s_dev in zip(want.items(), S_dev):
+            np.testing.assert_allclose(s_dev, S, rtol=1e-10, atol=0)
+            got = np.tensordot(tensor(sites1, u1), tensor(sites1, u2), axes=([1 + p1], [1 + p2]))
+            exp = np.tensordot(B1, B2, axes=([1 + p1], [1 + p2]))
+            assert _rel(got, exp) < 1e-10
+        dn.engine.init_messages("identity")
+        assert dn.beliefpropagation(maxiter=3000, tol=1e-14, criterion="maxabs").converged[0]
+        ln_dev = dn.engine.lognorm()
+    m, sw, dl, at = orc.bp(t, osites, pkg.peps.identity_messages(t), maxiter=3000, tol=1e-14,
+                           criterion="maxabs")
+    assert at > 0
+    ln = orc.bethe_free_energy(t, osites, m)
+    _say(f"gated log-norm device={ln_dev:.12f} oracle={ln:.12f}")
+    assert abs(ln_dev - ln) < 1e-10 * max(1.0, abs(ln))
