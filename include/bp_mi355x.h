@@ -40,8 +40,10 @@
  *     vertices (hence ports) of one network must be contiguous.
  * Host buffers are FP64 and copied; the caller keeps ownership.
  *
- * Threading: a context is used by one host thread at a time.  The library keeps no
- * global mutable state beyond the HIP runtime's.
+ * Threading: a context is used by one host thread at a time.  Beyond the HIP runtime's,
+ * the library's only global mutable state is the per-device address of its static
+ * scratch array (set once per device under a lock, by the first bp_create there) and the
+ * array itself (bp_iterate_diff_noalloc).
  */
 #ifndef BP_MI355X_H
 #define BP_MI355X_H
@@ -146,6 +148,15 @@ int bp_residuals(bp_context *ctx, double *fid, double *maxabs);
 
 /* iterate_diff between the current messages and an FP64 host message buffer. */
 int bp_iterate_diff(bp_context *ctx, const double *other, double *out);
+
+/* bp_iterate_diff that allocates nothing: no counterpart in the reference.  For callers
+ * that cannot allocate device memory per call (tight convergence-check loops, serving
+ * loops, harnesses that forbid hipMalloc).  The same result as bp_iterate_diff; the input
+ * is staged and the per-slot distances kept in the library's static per-device scratch
+ * array (8 MiB, 1 Mi doubles), so 2 * nports + (message entries) + 1 must not exceed
+ * 1048576 (BP_ERR_ARG otherwise; 32x32 D=8 needs 261889).  Calls that overlap on one GPU
+ * share that array and race. */
+int bp_iterate_diff_noalloc(bp_context *ctx, const double *other, double *out);
 
 /* Vertex scalars Z_v (length nv) and edge scalars Z_e (length ne, edge order of the
  * topology), FP64 accumulation. */

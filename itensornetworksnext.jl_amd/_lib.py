@@ -46,6 +46,7 @@ SIGNATURES = {
                                     _ip]),
     "bp_residuals": (C.c_int, [_vp, _dp, _dp]),
     "bp_iterate_diff": (C.c_int, [_vp, _dp, _dp]),
+    "bp_iterate_diff_noalloc": (C.c_int, [_vp, _dp, _dp]),
     "bp_vertex_scalars": (C.c_int, [_vp, _dp]),
     "bp_edge_scalars": (C.c_int, [_vp, _dp]),
     "bp_lognorm": (C.c_int, [_vp, _dp, _dp]),

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <cstdarg>
 #include <cstdio>
@@ -61,6 +62,21 @@ static int set_err(int code, const char *fmt, ...) {
         int rc_ = (expr);       \
         if (rc_ != BP_OK) return rc_; \
     } while (0)
+
+// ------------------------------------------------------------------ no-allocation scratch
+//
+// bp_iterate_diff_noalloc stages its input and keeps its per-slot partial results in this
+// file-scope device array instead of memory it allocates (1 Mi doubles = 8 MiB, held by
+// every GPU the library runs on until the process ends).  In host code `scratch` names a
+// host-side stand-in, not a device address: the device address comes from
+// hipGetSymbolAddress, taken once per device when the first context is created there
+// (bp_create) and kept in g_scratch_ptr.  Calls on one GPU share the array, so calls that
+// overlap on one GPU race.
+#define BP_SCRATCH_DOUBLES (1 << 20)
+#define BP_SCRATCH_DEVICES 64
+static __device__ __align__(256) double scratch[BP_SCRATCH_DOUBLES];
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[BP_SCRATCH_DEVICES] = {};
 
 // ------------------------------------------------------------------ small kernels
 
@@ -823,6 +839,22 @@ static int rebuild_vertex_plan(bp_context *c) {
 
 static int refresh_sites(bp_context *c);
 
+// The device address of `scratch` on `device` (current), looked up once per device.
+static int scratch_setup(int device) {
+    if (device < 0 || device >= BP_SCRATCH_DEVICES)
+        return set_err(BP_ERR_UNSUPPORTED, "device %d beyond the %d the library tracks", device,
+                       BP_SCRATCH_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return BP_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_err(BP_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s",
+                       device, hipGetErrorString(e));
+    g_scratch_ptr[device] = (double *)p;
+    return BP_OK;
+}
+
 extern "C" {
 
 const char *bp_last_error(void) { return g_err.c_str(); }
@@ -838,6 +870,7 @@ int bp_create(bp_context **out, int device, int dtype) {
     if (device < 0 || device >= ndev)
         return set_err(BP_ERR_ARG, "device %d out of range (%d devices)", device, ndev);
     HIPCHK(hipSetDevice(device));
+    RETCHK(scratch_setup(device));
     bp_context *c = new bp_context();
     c->device = device;
     c->dtype = dtype;
@@ -1729,24 +1762,28 @@ static bool vertex_adjoint_ok(const bp_context *c, bool is64) {
            c->vplan.nv16 == 0 && c->path != BP_PATH_GENERIC && c->d_vsites != nullptr;
 }
 
-// per-slot distances of the current messages against `other` (device buffer)
-static int diff_launch(bp_context *c, const void *other, bool other64) {
+// per-slot distances of the current messages against `other` (device buffer) into
+// fid / mab (the context's residual buffers when null)
+static int diff_launch(bp_context *c, const void *other, bool other64, double *fid = nullptr,
+                       double *mab = nullptr) {
     bool is64 = false;
     const void *m = current_msgs(c, &is64);
+    if (!fid) fid = c->d_res_fid;
+    if (!mab) mab = c->d_res_abs;
     if (c->nports == 0) return BP_OK;
     const dim3 g(c->nports), b(64);
     if (is64 && other64)
         hipLaunchKernelGGL((k_diff_slots<double, double>), g, b, 0, c->stream, c->topo(),
-                           (const double *)m, (const double *)other, c->d_res_fid, c->d_res_abs);
+                           (const double *)m, (const double *)other, fid, mab);
     else if (is64)
         hipLaunchKernelGGL((k_diff_slots<double, float>), g, b, 0, c->stream, c->topo(),
-                           (const double *)m, (const float *)other, c->d_res_fid, c->d_res_abs);
+                           (const double *)m, (const float *)other, fid, mab);
     else if (other64)
         hipLaunchKernelGGL((k_diff_slots<float, double>), g, b, 0, c->stream, c->topo(),
-                           (const float *)m, (const double *)other, c->d_res_fid, c->d_res_abs);
+                           (const float *)m, (const double *)other, fid, mab);
     else
         hipLaunchKernelGGL((k_diff_slots<float, float>), g, b, 0, c->stream, c->topo(),
-                           (const float *)m, (const float *)other, c->d_res_fid, c->d_res_abs);
+                           (const float *)m, (const float *)other, fid, mab);
     HIPCHK(hipGetLastError());
     return BP_OK;
 }
@@ -1896,6 +1933,51 @@ int bp_lognorm(bp_context *c, double *out_re, double *out_im) {
     if (out_im)
         HIPCHK(hipMemcpyAsync(out_im, c->d_lim, sizeof(double) * c->nnet, hipMemcpyDeviceToHost,
                               c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return BP_OK;
+}
+
+// out[0] = max over v[0, n) with a NaN kept (-Inf when n = 0): one workgroup, no atomics.
+__global__ __launch_bounds__(256) void k_max_slots(const double *v, int n, double *out) {
+    __shared__ double red[256];
+    double mx = -INFINITY;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double x = v[i];
+        mx = (x > mx || x != x) ? x : mx;
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            const double o = red[threadIdx.x + h];
+            if (o > red[threadIdx.x] || o != o) red[threadIdx.x] = o;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+int bp_iterate_diff_noalloc(bp_context *c, const double *other, double *out) {
+    RETCHK(ready(c));
+    if (!other || !out) return set_err(BP_ERR_ARG, "NULL argument");
+    // scratch layout: fid [nports] | max-abs [nports] | staged `other` [nmsg] | result [1]
+    const long long need = 2LL * c->nports + c->nmsg + 1;
+    if (need > BP_SCRATCH_DOUBLES)
+        return set_err(BP_ERR_ARG,
+                       "bp_iterate_diff_noalloc: 2 * %d slots + %lld message entries + 1 exceed "
+                       "the %d-double scratch; use bp_iterate_diff",
+                       c->nports, c->nmsg, BP_SCRATCH_DOUBLES);
+    HIPCHK(hipSetDevice(c->device));
+    double *sc = g_scratch_ptr[c->device];
+    if (!sc) return set_err(BP_ERR_STATE, "no scratch address for device %d", c->device);
+    double *fid = sc, *mab = sc + c->nports, *stage = mab + c->nports, *res = stage + c->nmsg;
+    if (c->nmsg > 0)
+        HIPCHK(hipMemcpyAsync(stage, other, sizeof(double) * c->nmsg, hipMemcpyHostToDevice,
+                              c->stream));
+    RETCHK(diff_launch(c, stage, true, fid, mab));
+    hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(256), 0, c->stream, fid, c->nports, res);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, res, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return BP_OK;
 }
