@@ -25,6 +25,7 @@
 #include "generic.h"
 #include "vertex.h"
 #include "vertex16.h"
+#include "vertex64.h"
 #include "adjoint.h"
 #include "vertex_adjoint.h"
 #include "contract.h"
@@ -324,6 +325,14 @@ struct bp_context {
                                 // one at 32x32 D=8: 1 in FP32 with d >= 2, else 0)
     int v16_waves = 16;         // waves per slice workgroup of the bond-dimension-16 kernel
 
+    // FP64 D <= 8 vertex kernel (vertex64.h): the sweeps of FP64 contexts and the FP64
+    // polish of every context.  Built on first use for the current topology and mask.
+    Vertex64Plan v64;
+    bool v64_tried = false;     // plan attempted (v64_plan_ok tells the outcome)
+    bool v64_plan_ok = false;
+    bool v64_sites = false;     // padded FP64 slices match the current sites
+    bool v64_ok = false;        // plan and slices ready
+
     // cached sweep graphs (16 sweeps + bookkeeping), one per starting buffer parity; valid
     // while graph_epoch equals the process-wide plan epoch (which moves whenever a captured
     // pointer or kernel choice may have changed)
@@ -411,6 +420,8 @@ static void free_all(bp_context *c) {
         if (p) (void)hipFree(p);
     c->adj.release();
     vertex_plan_release(&c->vplan);
+    vertex64_release(&c->v64);
+    c->v64_tried = c->v64_plan_ok = c->v64_sites = c->v64_ok = false;
     for (auto &b : c->gb)
         if (b.d_slots) (void)hipFree(b.d_slots);
     c->gb.clear();
@@ -647,6 +658,40 @@ static int launch_edge_scalars(bp_context *c, const T *msgs) {
 // kernel, or the vertex bucket plus the generic kernel for the remaining networks.
 
 
+// The FP64 vertex plan for this context's networks (bond dimensions enabled: the context's
+// vertex mask in FP64 contexts, 6..8 for the polish of FP32/BF16 contexts), with its padded
+// FP64 slices when sites are present.  Allocates: never called inside a graph capture
+// (prepare_sweep runs first).
+static int ensure_v64(bp_context *c) {
+    if (!c->have_topo) return BP_OK;
+    if (!c->v64_tried) {
+        const unsigned mask = c->dtype == BP_F64 ? c->vx_mask : 0x1C0u;
+        c->v64_plan_ok = vertex64_plan_build(&c->v64, c->plain, c->nv, c->port_off.data(),
+                                             c->port_edge.data(), c->edge_dim.data(),
+                                             c->phys_dim.data(), c->port_rev.data(),
+                                             c->vnet.data(), mask) == 0 &&
+                         vertex64_set_lds_limit(&c->v64) == 0;
+        c->v64_tried = true;
+        c->v64_sites = false;
+        bump_epoch();
+    }
+    if (c->v64_plan_ok && c->have_sites && !c->v64_sites) {
+        if (vertex64_upload_sites(&c->v64, c->stream, c->d_sites64, c->d_site_off,
+                                  c->d_port_off, c->d_port_edge, c->d_edge_dim) != 0)
+            return set_err(BP_ERR_HIP, "FP64 vertex site layout failed");
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->v64_sites = true;
+        bump_epoch();
+    }
+    c->v64_ok = c->v64_plan_ok && c->v64_sites;
+    return BP_OK;
+}
+
+static void v64_invalidate(bp_context *c) {
+    vertex64_release(&c->v64);
+    c->v64_tried = c->v64_plan_ok = c->v64_sites = c->v64_ok = false;
+}
+
 // Flags of the generic kernel for this context (BF16 operands, no normalisation).
 static int gen_flags(const bp_context *c, bool bf16) {
     return (bf16 ? GEN_FLAG_BF16 : 0) | (c->normalize ? 0 : GEN_FLAG_NO_NORMALIZE);
@@ -655,6 +700,10 @@ static int gen_flags(const bp_context *c, bool bf16) {
 static int resolve_path(bp_context *c) {
     // the vertex kernels always normalise: unnormalised updates run on the generic kernel
     if (!c->normalize) return c->path == BP_PATH_VERTEX ? -1 : BP_PATH_GENERIC;
+    if (c->dtype == BP_F64) {  // the FP64 vertex kernel covers every network or none
+        if (c->path != BP_PATH_GENERIC && c->v64_ok) return BP_PATH_VERTEX;
+        return c->path == BP_PATH_VERTEX ? -1 : BP_PATH_GENERIC;
+    }
     if (c->path == BP_PATH_GENERIC || !c->vplan_ok) return c->path == BP_PATH_VERTEX ? -1 : BP_PATH_GENERIC;
     if (c->vplan.all) return BP_PATH_VERTEX;
     // AUTO: vertex kernel for its bucket, generic kernel for the rest
@@ -664,6 +713,7 @@ static int resolve_path(bp_context *c) {
 // Allocate whatever workspace the next sweeps need, before any graph capture (no
 // allocation may happen inside a captured region).
 static int prepare_sweep(bp_context *c) {
+    if (c->dtype == BP_F64) RETCHK(ensure_v64(c));
     const int path = resolve_path(c);
     if (path < 0) return BP_OK;  // reported by sweep_once
     RETCHK(gen_set_lds_limits());
@@ -778,7 +828,11 @@ static int sweep_once(bp_context *c, const NetCtl &done, bool *folded = nullptr,
         fold.at = c->d_at;
         *folded = true;
     }
-    if (path == BP_PATH_VERTEX || path == BP_PATH_MIXED) {
+    if (path == BP_PATH_VERTEX && c->dtype == BP_F64) {
+        if (vertex64_sweep(&c->v64, c->stream, c->d_msg_off, (const double *)src, (double *)dst,
+                           done, c->d_res_fid, c->d_res_abs) != 0)
+            return set_err(BP_ERR_HIP, "FP64 vertex sweep launch failed");
+    } else if (path == BP_PATH_VERTEX || path == BP_PATH_MIXED) {
         RETCHK(vertex_sweep(&c->vplan, c->stream, c->topo(), c->d_vsites, src, dst, done,
                             c->d_res_fid, c->d_res_abs, fold));
         RETCHK(vertex_k3_sweep(&c->vplan, c->stream, c->topo(), src, dst, done,
@@ -818,6 +872,7 @@ static int upload_vertex_sites(bp_context *c) {
 // errors), with the padded site layout when sites are present; generic buckets follow.
 static int rebuild_vertex_plan(bp_context *c) {
     bump_epoch();
+    v64_invalidate(c);
     vertex_plan_release(&c->vplan);
     free_buckets(c);
     c->vplan_ok = vertex_plan_build(&c->vplan, c->dtype, c->plain, c->nv, c->port_off.data(),
@@ -874,6 +929,9 @@ int bp_create(bp_context **out, int device, int dtype) {
     bp_context *c = new bp_context();
     c->device = device;
     c->dtype = dtype;
+    // FP64 contexts: the FP64 vertex kernel for largest bond dimensions 6..8 by default
+    // (below 6 the zero padding to 8 costs more than the generic kernel saves)
+    if (dtype == BP_F64) c->vx_mask = 0x1C0u;
     if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
@@ -1083,6 +1141,12 @@ static int refresh_sites(bp_context *c) {
         }
     }
     if (c->vplan_ok) RETCHK(upload_vertex_sites(c));
+    c->v64_sites = false;
+    c->v64_ok = false;
+    if (c->v64_tried) {
+        c->have_sites = true;  // the padded FP64 slices follow the new sites now
+        RETCHK(ensure_v64(c));
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     return BP_OK;
 }
@@ -1208,9 +1272,15 @@ int bp_set_path(bp_context *c, int path) {
         return set_err(BP_ERR_ARG, "unknown path %d", path);
     if (path == BP_PATH_VERTEX && !c->normalize)
         return set_err(BP_ERR_UNSUPPORTED, "the vertex kernels always normalise (option normalize is 0)");
-    if (path == BP_PATH_VERTEX && !(c->vplan_ok && c->vplan.all))
+    if (path == BP_PATH_VERTEX && c->dtype == BP_F64) {
+        RETCHK(ensure_v64(c));
+        if (!c->v64_plan_ok)
+            return set_err(BP_ERR_UNSUPPORTED, "FP64 vertex path not available: %s",
+                           c->v64.why.c_str());
+    } else if (path == BP_PATH_VERTEX && !(c->vplan_ok && c->vplan.all)) {
         return set_err(BP_ERR_UNSUPPORTED, "vertex path not available for every network: %s",
                        c->vplan.why.c_str());
+    }
     bump_epoch();
     c->path = path;
     return BP_OK;
@@ -1264,6 +1334,7 @@ int bp_get_option(bp_context *c, const char *name, int *value) {
 
 int bp_get_path(bp_context *c, int *path) {
     if (!c || !path) return set_err(BP_ERR_ARG, "NULL argument");
+    if (c->dtype == BP_F64 && c->have_sites) RETCHK(ensure_v64(c));
     *path = resolve_path(c);
     if (*path < 0)
         return set_err(BP_ERR_UNSUPPORTED, "no kernel path for this configuration: %s",
@@ -1537,7 +1608,7 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
     c->fp64_active = false;
     int rc = BP_OK;
     // 1. vertex kernels or generic kernel, per largest bond dimension (<= 12 FP32, <= 16 BF16)
-    if ((c->dtype == BP_F32 || c->dtype == BP_BF16) && !c->plain && c->path == BP_PATH_AUTO) {
+    if (!c->plain && c->path == BP_PATH_AUTO) {
         std::map<int, int> dmax_d;  // largest bond dimension -> physical dimension
         for (int n = 0; n < c->nnet && rc == BP_OK; ++n) {
             int dm = 0, deg = 0, d = c->phys_dim[c->vbeg[n]];
@@ -1546,7 +1617,7 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
                 for (int q = c->port_off[u]; q < c->port_off[u + 1]; ++q)
                     dm = std::max(dm, c->edge_dim[c->port_edge[q]]);
             }
-            const int dlim = c->dtype == BP_BF16 ? 16 : 12;
+            const int dlim = c->dtype == BP_BF16 ? 16 : c->dtype == BP_F32 ? 12 : VX_D;
             if (dm >= 1 && dm <= dlim && deg <= 4 && d <= 4) dmax_d[dm] = d;
         }
         for (const auto &kv : dmax_d) {
@@ -1998,10 +2069,21 @@ int bp_polish_fp64(bp_context *c, int nsweeps, double *out_delta) {
         c->cur64 = 0;
         c->fp64_active = true;
     }
+    // FP64 sweeps on the FP64 vertex kernel when it covers the networks (bond dimensions
+    // 6..8 in FP32/BF16 contexts), else on the generic kernel
+    if (c->normalize) RETCHK(ensure_v64(c));
+    const bool v64 = c->normalize && c->v64_ok && c->path != BP_PATH_GENERIC;
     for (int i = 0; i < nsweeps; ++i) {
-        RETCHK(launch_update_generic<double>(c, c->d_sites64, c->d_msg64[c->cur64],
-                                             c->d_msg64[c->cur64 ^ 1], make_ctl(c, false), 0,
-                                             c->nports, nullptr, gen_flags(c, false)));
+        if (v64) {
+            if (vertex64_sweep(&c->v64, c->stream, c->d_msg_off, c->d_msg64[c->cur64],
+                               c->d_msg64[c->cur64 ^ 1], make_ctl(c, false), c->d_res_fid,
+                               c->d_res_abs) != 0)
+                return set_err(BP_ERR_HIP, "FP64 vertex sweep launch failed");
+        } else {
+            RETCHK(launch_update_generic<double>(c, c->d_sites64, c->d_msg64[c->cur64],
+                                                 c->d_msg64[c->cur64 ^ 1], make_ctl(c, false), 0,
+                                                 c->nports, nullptr, gen_flags(c, false)));
+        }
         c->cur64 ^= 1;
     }
     if (out_delta) {
