@@ -33,12 +33,14 @@ sys.path.insert(0, ROOT)
 # traffic (fetch x2.00, write x1.00); the summary and its commands are in
 # profiles/k_vertex_d8*_32x32_D8_fp32_hbm_traffic.md.  Keyed by (path, dtype, L, D, FP32
 # vertex kernel variant).
-MEASURED_TRAFFIC = {("vertex", "f32", 32, 8, 0): 37.53e6, ("vertex", "f32", 32, 8, 1): 39.11e6}
+MEASURED_TRAFFIC = {("vertex", "f32", 32, 8, 0): 37.53e6, ("vertex", "f32", 32, 8, 1): 39.11e6,
+                    ("vertex", "f64", 32, 8, 0): 210.3e6}
 # the sweep kernel that dominates each configuration (NS: the physical dimension)
 SWEEP_KERNEL = {("vertex", "f32", 0): "k_vertex_d8<NS, false>",
                 ("vertex", "f32", 1): "k_vertex_d8_lo<NS, false>",
                 ("vertex", "bf16", 0): "k_vertex_d8<NS, true>",
-                ("vertex", "bf16", 1): "k_vertex_d8_lo<NS, true>"}
+                ("vertex", "bf16", 1): "k_vertex_d8_lo<NS, true>",
+                ("vertex", "f64", 0): "k_vertex_d8_f64<NS>"}
 
 PEAK = {  # MI355X published dense peaks
     "hbm_GBps": 8000.0,
@@ -607,20 +609,28 @@ def main(argv=None):
             extras[f"vertex_variant_{v}_kernel_ms"] = e2.time_sweeps(100)
             e2.close()
     if not args.no_extras and args.dtype != "f64":
-        e64 = pkg.Engine(topo, sites, dtype="f64", path="generic")
-        e64.init_messages("identity")
-        e64.autotune(mode="all")  # the FP64 reference path at its best launch configuration
-        e64.run(maxiter=3)
-        e64.init_messages("identity")
-        n64 = 20
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        r64 = e64.run(maxiter=n64, use_graph=True)
-        torch.cuda.synchronize()
-        w64 = time.perf_counter() - t1
-        extras["fp64_plain_gpu_msg_per_s"] = nmsg * int(r64.sweeps[0]) / w64
-        extras["speedup_vs_fp64_plain_gpu"] = value / extras["fp64_plain_gpu_msg_per_s"]
-        e64.close()
+        # the single-network FP64 GPU paths on the same workload: the FP64 vertex kernel
+        # (the engine's FP64 path at this shape) and the generic FP64 kernel
+        for name, p64 in (("fp64_vertex", "auto"), ("fp64_generic", "generic")):
+            e64 = pkg.Engine(topo, sites, dtype="f64", path=p64)
+            e64.init_messages("identity")
+            e64.autotune(mode="all")  # each FP64 path at its best launch configuration
+            e64.run(maxiter=3)
+            e64.init_messages("identity")
+            n64 = 100 if p64 == "auto" else 20
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            r64 = e64.run(maxiter=n64, use_graph=True)
+            torch.cuda.synchronize()
+            w64 = time.perf_counter() - t1
+            k64 = e64.time_sweeps(50)
+            extras[name] = {"path": e64.path(), "msg_per_s": nmsg * int(r64.sweeps[0]) / w64,
+                            "kernel_ms": k64,
+                            "algorithmic_TFLOPs": flops / (k64 * 1e-3) / 1e12,
+                            "frac_fp64_peak": flops / (k64 * 1e-3) / 1e12 / PEAK["fp64_TFLOPs"]}
+            e64.close()
+        extras["speedup_vs_fp64_vertex_gpu"] = value / extras["fp64_vertex"]["msg_per_s"]
+        extras["speedup_vs_fp64_generic_gpu"] = value / extras["fp64_generic"]["msg_per_s"]
     eng.close()
     if not args.no_extras and args.dtype == "f32" and D == 8:
         eb = pkg.Engine(topo, sites, dtype="bf16")
@@ -645,6 +655,12 @@ def main(argv=None):
     if not args.no_extras:
         extras["adjoint"] = [adjoint_timing(pkg, torch, L, D, args.dtype, args.seed),
                              adjoint_timing(pkg, torch, 16, 6, args.dtype, args.seed)]
+        if "fp64_vertex" in extras:
+            # the target ratio: FP32 forward + adjoint message updates/s over the
+            # single-network FP64 GPU path's (both of its kernels)
+            fa = extras["adjoint"][0]["forward_plus_adjoint_msg_per_s"]
+            extras["forward_plus_adjoint_over_fp64_vertex"] = fa / extras["fp64_vertex"]["msg_per_s"]
+            extras["forward_plus_adjoint_over_fp64_generic"] = fa / extras["fp64_generic"]["msg_per_s"]
 
     if not args.no_extras and args.dtype == "f32":
         extras["uneven_batch"] = uneven_batch_timing(pkg, torch, args.seed)
