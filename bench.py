@@ -40,6 +40,8 @@ SWEEP_KERNEL = {("vertex", "f32", 0): "k_vertex_d8<NS, false>",
                 ("vertex", "f32", 1): "k_vertex_d8_lo<NS, false>",
                 ("vertex", "bf16", 0): "k_vertex_d8<NS, true>",
                 ("vertex", "bf16", 1): "k_vertex_d8_lo<NS, true>",
+                ("vertex", "f32", 2): "k_vertex_d8_mo<NS, false>",
+                ("vertex", "f32", 3): "k_vertex_d8_mo<NS, true>",
                 ("vertex", "f64", 0): "k_vertex_d8_f64<NS>"}
 
 PEAK = {  # MI355X published dense peaks
@@ -488,9 +490,10 @@ def main(argv=None):
                          "workload; local_energy: only the expectation-value / local-energy "
                          "timings; contract: only the exact-contraction timings; gates: only "
                          "the gate-layer timing; print the timings, for profiling")
-    ap.add_argument("--vertex-variant", type=int, choices=[0, 1], default=None,
+    ap.add_argument("--vertex-variant", type=int, choices=[0, 1, 2, 3], default=None,
                     help="FP32 vertex sweep kernel to run instead of the tuned one (0: two LDS "
-                         "buffers per wave, 1: one buffer and two waves per SIMD)")
+                         "buffers per wave, 1: one buffer and two waves per SIMD, 2: MFMA open "
+                         "contractions, bra in LDS, 3: MFMA open contractions, bra from HBM)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, used to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE)")

@@ -1290,7 +1290,10 @@ int bp_set_option(bp_context *c, const char *name, int value) {
     if (!c || !name) return set_err(BP_ERR_ARG, "NULL argument");
     bump_epoch();
     if (strcmp(name, "vertex_variant") == 0) {
-        if (value != 0 && value != 1) return set_err(BP_ERR_ARG, "vertex_variant is 0 or 1");
+        if (value < 0 || value > 3) return set_err(BP_ERR_ARG, "vertex_variant is 0, 1, 2 or 3");
+        if (value >= 2 && c->dtype != BP_F32)
+            return set_err(BP_ERR_UNSUPPORTED, "vertex_variant %d (MFMA open contractions) is FP32 only",
+                           value);
         c->vx_variant = value;
         c->vplan.variant = value;
         return BP_OK;
@@ -1659,17 +1662,24 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
         const std::string key = o.str();
         auto it = c->tune.find(key);
         if (it == c->tune.end() && mode >= 1) {
-            double t[2] = {-1.0, -1.0};
-            for (int v = 0; v < 2 && rc == BP_OK; ++v) {
+            // FP32: four kernels (two with VALU open contractions, two with MFMA ones); BF16: two
+            const int nvar = c->dtype == BP_F32 ? 4 : 2;
+            double t[4] = {-1.0, -1.0, -1.0, -1.0};
+            int best = 0;
+            for (int v = 0; v < nvar && rc == BP_OK; ++v) {
                 c->vplan.variant = v;
                 t[v] = tune_time_sweep(c, 3);
                 if (t[v] < 0.0) rc = set_err(BP_ERR_HIP, "autotuner sweep failed");
+                else if (t[v] < t[best]) best = v;
             }
             if (rc == BP_OK) {
+                double alt = 1e30;
+                for (int v = 0; v < nvar; ++v)
+                    if (v != best) alt = std::min(alt, t[v]);
                 TuneEntry e;
-                e.vertex = t[1] < t[0] ? 1 : 0;  // the chosen variant
-                e.us = 1e3 * std::min(t[0], t[1]);
-                e.us_alt = 1e3 * std::max(t[0], t[1]);
+                e.vertex = best;  // the chosen variant
+                e.us = 1e3 * t[best];
+                e.us_alt = 1e3 * alt;  // the runner-up
                 it = c->tune.emplace(key, e).first;
                 ++tuned;
             }

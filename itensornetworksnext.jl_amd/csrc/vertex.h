@@ -64,7 +64,8 @@ struct VertexPlan {
     std::vector<int> gen_slots; // the same on the host
     int ngen = 0;
     size_t lds_bytes = 0;
-    int variant = 0;            // D <= 8 sweep kernel: 0 k_vertex_d8, 1 k_vertex_d8_lo
+    int variant = 0;            // D <= 8 sweep kernel: 0 k_vertex_d8, 1 k_vertex_d8_lo,
+                                // 2 k_vertex_d8_mo (FP32)
     // networks whose largest bond dimension is 9..16 in BF16 (k_vertex_d16, vertex16.h) or
     // 9..12 in FP32 (k_vertex_d12, vertex12.h): the large-bond bucket
     int nv16 = 0;
@@ -82,6 +83,8 @@ struct VertexPlan {
     float *d_vk3part = nullptr;            // per-slice unnormalised messages
     float *d_v16part = nullptr;            // per-slice unnormalised messages
     int v16_waves = 16;                    // waves per slice workgroup: 4, 8 or 16
+    float *d_vsites_y = nullptr;           // FP32: the padded slices in layout Y order (the
+                                           // bra rows of k_vertex_d8_mo<NS, true>)
 };
 
 static constexpr int VX_D = 8;
@@ -107,6 +110,8 @@ static void vertex_plan_release(VertexPlan *p) {
     p->nv_k3 = 0;
     p->d_v16part = nullptr;
     p->nv16 = 0;
+    if (p->d_vsites_y) (void)hipFree(p->d_vsites_y);
+    p->d_vsites_y = nullptr;
     if (p->d_vnet) (void)hipFree(p->d_vnet);
     if (p->d_lead) (void)hipFree(p->d_lead);
     p->d_lead = nullptr;
@@ -324,11 +329,12 @@ __device__ __forceinline__ float vx_pad_value(const double *sites64, const long 
     return (float)sites64[site_off[u] + s * n + off];
 }
 
-// FP32: vsites[u][s][r * 64 + l].  BF16: vsites[u][s][(r / 2) * 128 + 2 l + (r % 2)].
+// FP32: vsites[u][s][r * 64 + l] (layout X) and, when vsy is given, vsy[u][s][l * 64 + r]
+// (layout Y).  BF16: vsites[u][s][(r / 2) * 128 + 2 l + (r % 2)].
 __global__ void k_vertex_pad_sites(const double *sites64, const long long *site_off,
                                    const int *port_off, const int *port_edge,
                                    const int *edge_dim, const int *vlist, int d, int nv, int bf16,
-                                   void *vsites) {
+                                   void *vsites, float *vsy) {
     const long long total = (long long)nv * d * 4096;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
          i += (long long)gridDim.x * blockDim.x) {
@@ -341,6 +347,7 @@ __global__ void k_vertex_pad_sites(const double *sites64, const long long *site_
             ((unsigned short *)vsites)[(us << 12) + (r >> 1) * 128 + 2 * l + (r & 1)] = bf16_rne(v);
         } else {
             ((float *)vsites)[i] = v;
+            if (vsy) vsy[(us << 12) + (long long)(e & 63) * 64 + (e >> 6)] = v;
         }
     }
 }
@@ -354,9 +361,12 @@ static int vertex_upload_sites(VertexPlan *p, hipStream_t st, const double *d_si
     *d_vsites = nullptr;
     if (p->nv == 0) return 0;
     if (hipMalloc(d_vsites, bytes) != hipSuccess) return 5;
+    if (p->d_vsites_y) (void)hipFree(p->d_vsites_y);
+    p->d_vsites_y = nullptr;
+    if (!p->bf16 && hipMalloc((void **)&p->d_vsites_y, bytes) != hipSuccess) return 5;
     hipLaunchKernelGGL(k_vertex_pad_sites, dim3(2048), dim3(256), 0, st, d_sites64, d_site_off,
                        d_port_off, d_port_edge, d_edge_dim, p->d_vlist, p->d, p->nv, p->bf16,
-                       *d_vsites);
+                       *d_vsites, p->d_vsites_y);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
@@ -758,6 +768,264 @@ void k_vertex_d8(const void *__restrict__ vsites_v, const float *__restrict__ mo
     }
 }
 
+// ---------------------------------------------------------------- MFMA open contractions
+//
+// Variant 2 of the D <= 8 FP32 sweep (k_vertex_d8_mo): the open contractions run on the
+// MFMA pipe too.  out[a][a'] = sum_K G[a][K] A[a'][K] over the K = 512 values of the three
+// other ports is, per v_mfma_f32_4x4x1_16b_f32, 16 blocks of 4x4 outer products: block k
+// (lanes 4k..4k+3) takes one K value per instruction, lane 4k+q supplying G[4t+q][K] as the
+// A operand and A[4t'+q][K] as the B operand, so register i of lane 4k+j accumulates
+// out[4t+i][4t'+j] over the K values of block k (K = 16c + k, chunk c = 0..31); the four
+// 4x4 tiles (t, t') and a sum over the 16 blocks finish it.  The operands need K spread over
+// lane blocks, but the registers hold K along one register port and the lanes, so G is first
+// written to LDS as gbuf[K][a] (K = o * 64 + lane, o the register index that is not the open
+// port; a stored in the order 0,4,1,5,2,6,3,7 so that rows q and q + 4 are one 8-byte read)
+// and the bra values are gathered from the wave's copy of A in LDS.  Per slice that is 512
+// MFMAs and 96 reduction instructions per open instead of 2048 VALU FMAs and about 1500
+// reduction instructions.
+
+// G (registers; open index a = register hi (HI) or lo (LO)) to gbuf[(o*64 + lane)*8 + perm(a)].
+template <bool HI>
+__device__ __forceinline__ void vmo_stage(const float (&G)[64], float *gbuf) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+        float v[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) v[2 * (a & 3) + (a >> 2)] = HI ? G[a * 8 + o] : G[o * 8 + a];
+        float4 *dst = reinterpret_cast<float4 *>(gbuf + (o * 64 + lane) * 8);
+        dst[0] = make_float4(v[0], v[1], v[2], v[3]);
+        dst[1] = make_float4(v[4], v[5], v[6], v[7]);
+    }
+}
+
+template <int N>
+__device__ __forceinline__ float vmo_ror(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x120 + N, 0xF, 0xF, false));
+}
+
+// Open contraction from the staged G (gbuf) and the bra A (bufA, the slice stored as
+// bufA[(p0*8 + p1) * VX_PITCH + p2*8 + p3]).  OPEN names the open port and the layout of G:
+// 2 (layout Y, port 2 = register hi), 3 (Y, port 3 = register lo), 0 (X, port 0 = hi),
+// 1 (X, port 1 = lo).  Entry (a, a') of the slice's result goes to out[a*8 + a'] (LDS),
+// written by lanes 0..3.
+//
+// GB (bra from global): `bufA` is instead the slice's padded copy in HBM in the layout of G
+// (layout Y for OPEN 2, 3; layout X for OPEN 0, 1; row r of lane l at [r * 64 + l]), so the
+// bra element (a', K) sits at (a' * 8 + K / 64) * 64 + K % 64 for an open on the register hi
+// port and at ((K / 64) * 8 + a') * 64 + K % 64 on the lo port: four 64-byte runs per load.
+template <int OPEN, bool GB = false>
+__device__ __forceinline__ void vmo_open(const float *gbuf, const float *bufA, float *out) {
+    static_assert(VX_PITCH == 65, "bra address offsets assume a row pitch of 65");
+    const int lane = threadIdx.x & 63;
+    const int k = lane >> 2, q = lane & 3;
+    constexpr bool HIO = OPEN == 0 || OPEN == 2;
+    // bra element (a' = q, K = 16c + k): K % 64 = 16 (c % 4) + k, K / 64 = c / 4
+    const int base = GB ? (HIO ? q * 512 + k : q * 64 + k)
+                   : OPEN == 2 ? k * 65 + q * 8 : OPEN == 3 ? k * 65 + q
+                   : OPEN == 0 ? q * 520 + k : q * 65 + k;
+    constexpr int D4 = GB ? (HIO ? 2048 : 256)
+                     : OPEN == 2 ? 32 : OPEN == 3 ? 4 : OPEN == 0 ? 2080 : 260;  // a' + 4
+    const float *bp = bufA + base;
+    const float *gp = gbuf + k * 8 + 2 * q;
+    v4f acc00 = {0.f, 0.f, 0.f, 0.f}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+        const int coff = GB ? (HIO ? (c >> 2) * 64 + (c & 3) * 16 : (c >> 2) * 512 + (c & 3) * 16)
+                       : OPEN == 2 ? (c & 3) * 1040 + (c >> 2)
+                       : OPEN == 3 ? (c & 3) * 1040 + (c >> 2) * 8
+                       : OPEN == 0 ? (c >> 2) * 65 + (c & 3) * 16
+                                   : (c >> 2) * 520 + (c & 3) * 16;
+        const float2 g = *reinterpret_cast<const float2 *>(gp + c * 128);
+        const float b0 = bp[coff], b1 = bp[coff + D4];
+        acc00 = __builtin_amdgcn_mfma_f32_4x4x1f32(g.x, b0, acc00, 0, 0, 0);
+        acc01 = __builtin_amdgcn_mfma_f32_4x4x1f32(g.x, b1, acc01, 0, 0, 0);
+        acc10 = __builtin_amdgcn_mfma_f32_4x4x1f32(g.y, b0, acc10, 0, 0, 0);
+        acc11 = __builtin_amdgcn_mfma_f32_4x4x1f32(g.y, b1, acc11, 0, 0, 0);
+    }
+    float v[16];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        v[i] = acc00[i];
+        v[4 + i] = acc01[i];
+        v[8 + i] = acc10[i];
+        v[12 + i] = acc11[i];
+    }
+    // sum over the 16 blocks: lane bits 2 and 3 by rotations inside 16-lane rows, bits 4
+    // and 5 by the row and half swaps (each leaves the total on every lane)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        float x = v[i] + vmo_ror<4>(v[i]);
+        x += vmo_ror<8>(x);
+        const auto r16 = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x),
+                                                          false, false);
+        x = __uint_as_float(r16[0]) + __uint_as_float(r16[1]);
+        const auto r32 = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x),
+                                                          false, false);
+        v[i] = __uint_as_float(r32[0]) + __uint_as_float(r32[1]);
+    }
+    if (lane < 4) {  // lane j holds out[4t + i][4t' + j] in v[(2t + t') * 4 + i]
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                out[(4 * (tt >> 1) + i) * 8 + 4 * (tt & 1) + lane] = v[tt * 4 + i];
+    }
+}
+
+// GB = false (variant 2): the bra A lives in a second LDS buffer per wave (one wave per
+// SIMD).  GB = true (variant 3): the bra rows are gathered from the padded X and Y copies of
+// the sites in HBM (L2 hits after the first), so one LDS buffer per wave suffices and two
+// waves share a SIMD.
+template <int NS, bool GB>
+__global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(GB ? 2 : 1, GB ? 2 : 1)))
+void k_vertex_d8_mo(const void *__restrict__ vsites_v, const float *__restrict__ vsites_y,
+                    const float *__restrict__ mold,
+                    float *__restrict__ mnew, const int4 *__restrict__ vin,
+                    const int4 *__restrict__ vout, const int4 *__restrict__ vdim,
+                    const int *__restrict__ vnet, const long long *__restrict__ msg_off,
+                    NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs,
+                    SweepFold fold) {
+    extern __shared__ __align__(16) float vsm[];
+    __shared__ double fold_m[4];
+    const int u = blockIdx.x;
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    float *bufA = vsm + w * (GB ? 1 : 2) * VX_BUF;  // A (written once; unused with GB)
+    float *bufT = GB ? bufA : bufA + VX_BUF;        // transposes, then each open's staged G
+    float *msm = vsm + NS * (GB ? 1 : 2) * VX_BUF;
+    float *part = msm + 4 * 64;
+    const int4 vi = vin[u], vo = vout[u], vdm = vdim[u];
+    const int ins[4] = {vi.x, vi.y, vi.z, vi.w};
+    const int outs[4] = {vo.x, vo.y, vo.z, vo.w};
+    const int dims[4] = {vdm.x, vdm.y, vdm.z, vdm.w};
+    long long ino[4], outo[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        ino[j] = ins[j] >= 0 ? msg_off[ins[j]] : 0;
+        outo[j] = outs[j] >= 0 ? msg_off[outs[j]] : 0;
+    }
+    const int la = lane >> 3, lb = lane & 7;
+    const int net = vnet[u];
+    if (vx_fold_begin(ctl, fold, net, u)) {  // uniform over the workgroup
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j % NS != w || outs[j] < 0) continue;
+            if (lane < dims[j] * dims[j]) mnew[outo[j] + lane] = mold[outo[j] + lane];
+            if (lane == 0) { res_fid[outs[j]] = 0.0; res_abs[outs[j]] = 0.0; }
+        }
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j % NS != w) continue;
+        const int D = dims[j];
+        float x;
+        if (ins[j] < 0) x = la == lb ? 1.f : 0.f;
+        else x = (la < D && lb < D) ? mold[ino[j] + la * D + lb] : 0.f;
+        msm[j * 64 + lane] = x;
+    }
+    float T[64], E[64], G[64];
+    const float *sx = (const float *)vsites_v + ((long long)u * NS + w) * 4096;
+    const float *sy = GB ? vsites_y + ((long long)u * NS + w) * 4096 : nullptr;
+#pragma unroll
+    for (int r = 0; r < 64; ++r) T[r] = sx[r * 64 + lane];
+    if constexpr (!GB) {
+#pragma unroll
+        for (int r = 0; r < 64; ++r) bufA[r * VX_PITCH + lane] = T[r];
+    }
+    __syncthreads();  // messages of every wave, and this wave's A
+    float *po = part + w * 4 * 64;
+    const float *braY = GB ? sy : bufA;  // bra of the phase-A opens (layout Y)
+    const float *braX = GB ? sx : bufA;  // bra of the phase-B opens (layout X)
+
+    // ---- phase A: E01 = A x0 m0 x1 m1 (X) -> Y; out2 (open p2), out3 (open p3)
+    vx_mode<true, false>(T, E, msm + 0 * 64);
+    vx_mode<false, false>(E, G, msm + 1 * 64);
+#pragma unroll
+    for (int r = 0; r < 64; ++r) bufT[r * VX_PITCH + lane] = G[r];
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) E[r] = bufT[lane * VX_PITCH + r];  // E01 in Y
+    vx_mode<false, false>(E, G, msm + 3 * 64);
+    wave_lds_sync();  // every lane has read E01 from bufT
+    vmo_stage<true>(G, bufT);
+    wave_lds_sync();
+    vmo_open<2, GB>(bufT, braY, po + 2 * 64);
+    vx_mode<true, false>(E, G, msm + 2 * 64);
+    wave_lds_sync();  // the staged G of out2 is consumed
+    vmo_stage<false>(G, bufT);
+    wave_lds_sync();
+    vmo_open<3, GB>(bufT, braY, po + 3 * 64);
+
+    // ---- phase B: E23 = A x2 m2 x3 m3 (Y) -> X; out0 (open p0), out1 (open p1)
+    if constexpr (GB) {
+#pragma unroll
+        for (int r = 0; r < 64; ++r) T[r] = sy[r * 64 + lane];  // A in Y
+    } else {
+#pragma unroll
+        for (int r = 0; r < 64; ++r) T[r] = bufA[lane * VX_PITCH + r];  // A in Y
+    }
+    vx_mode<true, false>(T, E, msm + 2 * 64);
+    vx_mode<false, false>(E, G, msm + 3 * 64);
+    wave_lds_sync();  // the staged G of out3 is consumed
+#pragma unroll
+    for (int r = 0; r < 64; ++r) bufT[lane * VX_PITCH + r] = G[r];
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) E[r] = bufT[r * VX_PITCH + lane];  // E23 in X
+    vx_mode<false, false>(E, G, msm + 1 * 64);
+    wave_lds_sync();
+    vmo_stage<true>(G, bufT);
+    wave_lds_sync();
+    vmo_open<0, GB>(bufT, braX, po + 0 * 64);
+    vx_mode<true, false>(E, G, msm + 0 * 64);
+    wave_lds_sync();
+    vmo_stage<false>(G, bufT);
+    wave_lds_sync();
+    vmo_open<1, GB>(bufT, braX, po + 1 * 64);
+
+    // ---- sum over s, normalise, residual, write (as k_vertex_d8)
+    double fm = 0.0;
+    const bool fcrit = fold.acc && fold.rp->crit != 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int q = outs[j];
+        if (j % NS != w || q < 0) continue;
+        float x = 0.f;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) x += part[(s * 4 + j) * 64 + lane];
+        const double S = wave_sum_v((double)x);
+        const double xn = S != 0.0 ? (double)x / S : (double)x;
+        const float xf = (float)xn;
+        const int D = dims[j];
+        const bool real = la < D && lb < D;
+        const long long o = outo[j] + la * D + lb;
+        const double xo = real ? (double)mold[o] : 0.0;
+        if (real) mnew[o] = xf;
+        const double dn = wave_sum_v((double)xf * xo),
+                     nn = wave_sum_v((double)xf * (double)xf), oo = wave_sum_v(xo * xo);
+        const double mx = wave_max_v(fabs((double)xf - xo));
+        if (lane == 0) {
+            const double c = dn / sqrt(nn * oo);
+            res_fid[q] = 1.0 - c * c;
+            res_abs[q] = mx;
+            fm = fold_max(fm, fcrit ? mx : 1.0 - c * c);
+        }
+    }
+    if (fold.acc) {
+        if (lane == 0) fold_m[w] = fm;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = 0.0;
+            for (int s = 0; s < NS; ++s) m = fold_max(m, fold_m[s]);
+            atomicMax(fold.acc + (long long)net * FOLD_SLOTS + (u % FOLD_SLOTS),
+                      (unsigned long long)__double_as_longlong(m));
+        }
+    }
+}
+
 // BF16 rounding (nearest even) of a partial product in place, before an open contraction.
 __device__ __forceinline__ void vx_round_bf16(float (&G)[64]) {
 #pragma unroll
@@ -948,6 +1216,31 @@ static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *v
     (void)tp;
     const int n4 = p->nv - p->nv_k3;  // items on the four-port kernels
     if (n4 == 0) return 0;
+    if ((p->variant == 2 || p->variant == 3) && !p->bf16) {  // MFMA open contractions (FP32)
+        const bool gb = p->variant == 3;
+        const size_t lds = gb ? vertex_lo_lds(p->d) : p->lds_bytes;
+        switch (p->d * 2 + (gb ? 1 : 0)) {
+#define VX_MO(NS, GB)                                                                          \
+    case NS * 2 + GB:                                                                          \
+        hipLaunchKernelGGL((k_vertex_d8_mo<NS, GB>), dim3(n4), dim3(64 * NS), lds, st, vsites, \
+                           (const float *)p->d_vsites_y, (const float *)src, (float *)dst,     \
+                           p->d_vin, p->d_vout, p->d_vdim, p->d_vnet, tp.msg_off, ctl,         \
+                           res_fid, res_abs, fold);                                            \
+        break;
+            VX_MO(1, 0)
+            VX_MO(2, 0)
+            VX_MO(3, 0)
+            VX_MO(4, 0)
+            VX_MO(1, 1)
+            VX_MO(2, 1)
+            VX_MO(3, 1)
+            VX_MO(4, 1)
+#undef VX_MO
+            default:
+                return 4;
+        }
+        return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
     if (p->variant == 1) {
         const size_t lds = vertex_lo_lds(p->d);
         switch (p->d * 2 + (p->bf16 ? 1 : 0)) {
@@ -999,7 +1292,16 @@ static int vertex_set_lds_limits(const VertexPlan *p) {
                           (const void *)k_vertex_d8<3, false>, (const void *)k_vertex_d8<4, false>,
                           (const void *)k_vertex_d8<1, true>, (const void *)k_vertex_d8<2, true>,
                           (const void *)k_vertex_d8<3, true>, (const void *)k_vertex_d8<4, true>};
+    const void *mo[4] = {(const void *)k_vertex_d8_mo<1, false>, (const void *)k_vertex_d8_mo<2, false>,
+                         (const void *)k_vertex_d8_mo<3, false>, (const void *)k_vertex_d8_mo<4, false>};
+    const void *mg[4] = {(const void *)k_vertex_d8_mo<1, true>, (const void *)k_vertex_d8_mo<2, true>,
+                         (const void *)k_vertex_d8_mo<3, true>, (const void *)k_vertex_d8_mo<4, true>};
     if (p->d < 1 || p->d > 4) return 4;
+    if (!p->bf16 && (hipFuncSetAttribute(mo[p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)p->lds_bytes) != hipSuccess ||
+                     hipFuncSetAttribute(mg[p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)vertex_lo_lds(p->d)) != hipSuccess))
+        return 2;
     return hipFuncSetAttribute(fns[(p->bf16 ? 4 : 0) + p->d - 1],
                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)p->lds_bytes) == hipSuccess ? 0 : 2;
