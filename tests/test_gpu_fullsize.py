@@ -116,3 +116,50 @@ def test_c5_bf16_then_polish_fullsize(pkg, orc):
     assert stages[1][1] < 1e-4 and stages[1][2] < 1e-12
     assert stages[2][1] < 1e-6 and stages[2][2] < 1e-12
     assert stages[2][1] < stages[1][1] < stages[0][1]
+
+
+def test_c4_fp32_gradient_fullsize(pkg, orc):
+    """Differentiation workload at full size: 16x16 PEPS, d = 2, D = 6, FP32 forward (the
+    D <= 8 vertex kernels) and FP32 adjoint sweeps, against the FP64 oracle's fixed-point
+    gradient (stated tolerance 1e-5 relative to the largest entry; measured 2.3e-7) and
+    against central differences (step 1e-3, 20 seeded entries) of the FP64 engine's
+    converged energy (2e-3: the O(step^2) truncation of the difference quotient; measured
+    7.6e-6).  The energy is a sum of terms of both signs that nearly cancel here (about
+    -0.8 from terms of order 3 per site), so its stated tolerance is 5e-6 relative
+    (measured 1.3e-6)."""
+    _say("case c4 16x16 D=6 fp32 gradient vs oracle and central differences")
+    t = pkg.build_topology(pkg.named_grid((16, 16)), 6, 2)
+    s = pkg.peps.random_sites(t, 33, scale=pkg.peps.peps_scale(2, 6))
+    m0 = pkg.peps.identity_messages(t)
+    m64, _, _, at = orc.bp(t, s, m0, maxiter=400, tol=1e-15, criterion="maxabs", nthreads=NT)
+    assert at > 0
+    E64, dm, da = orc.tfim_energy(t, s, m64, 1.0, 3.0, partials=True)
+    g64, _, res64 = orc.fixed_point_gradient(t, s, m64, dm, da, maxiter=500, tol=1e-13,
+                                             nthreads=NT)
+    with pkg.Engine(t, s, dtype="f32") as e:
+        assert e.path() == "vertex"
+        e.set_messages(m0)
+        assert e.run(maxiter=400, tol=1e-12).converged[0]
+        E, g, it, res = e.tfim_energy(1.0, 3.0, grad=True, adj_tol=1e-6, adj_maxiter=400)
+    eg = _rel(g, g64)
+    idx = np.random.default_rng(7).choice(s.size, 20, replace=False)
+    step = 1e-3
+    fd = []
+    with pkg.Engine(t, s, dtype="f64") as e:
+        for i in idx:
+            vals = []
+            for sg in (1.0, -1.0):
+                s2 = s.copy()
+                s2[i] += sg * step
+                e.set_sites(s2)
+                e.set_messages(m64)
+                assert e.run(maxiter=2000, tol=1e-13, criterion="maxabs").converged[0]
+                vals.append(e.tfim_energy(1.0, 3.0))
+            fd.append((vals[0] - vals[1]) / (2 * step))
+    fd = np.array(fd)
+    efd = float(np.max(np.abs(g[idx] - fd)) / np.max(np.abs(fd)))
+    _say(f"E rel={abs(E - E64) / abs(E64):.2e} grad vs oracle={eg:.2e} vs FD={efd:.2e} "
+         f"adjoint sweeps={it}")
+    assert abs(E - E64) / abs(E64) < 5e-6
+    assert eg < 1e-5
+    assert efd < 2e-3
