@@ -1479,11 +1479,13 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
 // Choices, each timed with HIP events on the context's own data (min of 3 launches after
 // a warm-up) and kept in a JSON cache, one entry per line:
 //   "vx|<dtype>|d<d>|D<D>": {"vertex": 0|1, "us": t_chosen, "us_alt": t_other}
-//       networks whose largest bond dimension is D <= 8 run on the vertex kernels (padded
-//       to 8) or on the generic kernel, whichever sweeps faster;
-//   "vxk|<f32|bf16>|d<d>": {"variant": 0|1, "us": t_chosen, "us_alt": t_other}
-//       the D <= 8 vertex sweep kernel: two LDS buffers per wave, or one buffer per wave
-//       with two waves per SIMD;
+//       networks whose largest bond dimension is D run on the vertex kernels (padded to 8,
+//       12 or 16; FP64: the FP64 vertex kernel, D <= 8) or on the generic kernel,
+//       whichever sweeps faster;
+//   "vxk|<f32|bf16>|d<d>": {"variant": 0..3, "us": t_chosen, "us_alt": t_next}
+//       the D <= 8 vertex sweep kernel: 0 two LDS buffers per wave, 1 one buffer per wave
+//       with two waves per SIMD, 2 and 3 (FP32 only) open contractions on MFMA with the
+//       bra from LDS or from the padded site copies in HBM;
 //   "v16w|bf16|d<d>": {"waves": 4|8|16, "us": t_best, "us_alt": t_worst}
 //       waves per slice workgroup of the bond-dimension-16 kernel;
 //   "gen|<dtype>|d<d>|D<D>": {"block": 128|256|512, "lds": 0|1|2, "us": t, "us_alt": t0}
