@@ -65,7 +65,11 @@ enum {
 enum {
     BP_F64 = 0,  /* FP64 storage and arithmetic */
     BP_F32 = 1,  /* FP32 storage and arithmetic (MFMA FP32 on the vertex path) */
-    BP_BF16 = 2  /* BF16 MFMA operands, FP32 accumulation and FP32 messages */
+    BP_BF16 = 2, /* BF16 MFMA operands, FP32 accumulation and FP32 messages */
+    BP_C128 = 3  /* complex FP64 (ComplexF64, test_beliefpropagation.jl:154): generic kernel;
+                    every host buffer of sites, messages and scalars holds interleaved
+                    (re, im) doubles, twice the real length; BP sweeps (Jacobi and
+                    sequential), scalars, log-norm and iterate_diff only */
 };
 
 /* message initialisations (message_environment(f, nn)) */
@@ -86,7 +90,7 @@ typedef struct bp_context bp_context;
 const char *bp_last_error(void);
 int bp_version(void);
 
-/* Lifecycle.  device: HIP device ordinal.  dtype: BP_F64 / BP_F32 / BP_BF16. */
+/* Lifecycle.  device: HIP device ordinal.  dtype: BP_F64 / BP_F32 / BP_BF16 / BP_C128. */
 int bp_create(bp_context **out, int device, int dtype);
 int bp_destroy(bp_context *ctx);
 /* Run on a caller-provided hipStream_t (passed as void*); NULL restores the
@@ -159,13 +163,15 @@ int bp_iterate_diff(bp_context *ctx, const double *other, double *out);
 int bp_iterate_diff_noalloc(bp_context *ctx, const double *other, double *out);
 
 /* Vertex scalars Z_v (length nv) and edge scalars Z_e (length ne, edge order of the
- * topology), FP64 accumulation. */
+ * topology), FP64 accumulation; BP_C128 contexts write interleaved complex values
+ * (2 nv and 2 ne doubles). */
 int bp_vertex_scalars(bp_context *ctx, double *zv);
 int bp_edge_scalars(bp_context *ctx, double *ze);
 
 /* BP log-norm per network (length nnet): real and imaginary parts (imaginary part is
  * pi * (#negative vertex terms - #negative edge terms), as the reference's complex
- * promotion gives); -Inf when an edge term is zero.  Compensated FP64 sums. */
+ * promotion gives; in BP_C128 contexts the sum of the terms' principal arguments);
+ * -Inf when an edge term is zero.  Compensated FP64 sums. */
 int bp_lognorm(bp_context *ctx, double *out_re, double *out_im);
 
 /* FP64 polishing: run `nsweeps` FP64 Jacobi sweeps from the current messages on the FP64

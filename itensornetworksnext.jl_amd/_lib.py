@@ -11,13 +11,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "csrc", "libbpmi355x.so")
 
 BP_OK, BP_ERR_ARG, BP_ERR_HIP, BP_ERR_STATE, BP_ERR_UNSUPPORTED, BP_ERR_NOMEM = range(6)
-BP_F64, BP_F32, BP_BF16 = 0, 1, 2
+BP_F64, BP_F32, BP_BF16, BP_C128 = 0, 1, 2, 3
 BP_INIT_IDENTITY, BP_INIT_ONES, BP_INIT_UPLOAD = 0, 1, 2
 BP_CRIT_FIDELITY, BP_CRIT_MAXABS = 0, 1
 BP_PATH_AUTO, BP_PATH_GENERIC, BP_PATH_VERTEX, BP_PATH_MIXED = 0, 1, 2, 3
 
 DTYPES = {"f64": BP_F64, "float64": BP_F64, "f32": BP_F32, "float32": BP_F32,
-          "bf16": BP_BF16, "bfloat16": BP_BF16}
+          "bf16": BP_BF16, "bfloat16": BP_BF16, "c128": BP_C128, "complex128": BP_C128}
 PATHS = {"auto": BP_PATH_AUTO, "generic": BP_PATH_GENERIC, "vertex": BP_PATH_VERTEX}
 
 # Every exported symbol and its signature: (restype, argtypes).

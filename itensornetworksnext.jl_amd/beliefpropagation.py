@@ -28,7 +28,7 @@ import numpy as np
 from . import algorithms as A
 from .graphs import forest_cover_edge_sequence
 from .itensors import to_topology
-from .messagecache import MessageCache, _flat, incoming_messages, iterate_diff
+from .messagecache import MessageCache, _flat, context_dtype, incoming_messages, iterate_diff
 
 
 class MessageUpdateAlgorithm(A.AbstractAlgorithm):
@@ -89,7 +89,9 @@ class SimpleMessageUpdate(MessageUpdateAlgorithm):
         self.dtype = dtype
 
     def update(self, cache, factors, edge):
-        fl, e = _factor_engine(factors, self.dtype, self.normalize)
+        fl = _flat(factors)
+        fl, e = _factor_engine(factors, context_dtype(self.dtype, fl.sites, cache),
+                               self.normalize)
         t = fl.topo
         q = fl.slot(edge)
         u = int(t.port_owner()[q])
@@ -99,7 +101,7 @@ class SimpleMessageUpdate(MessageUpdateAlgorithm):
             data = []
             for r in ins:
                 key = (t.vertices[own[r]], t.vertices[t.port_nbr[r]])
-                data.append(np.asarray(cache[key], dtype=np.float64).reshape(-1))
+                data.append(np.asarray(cache[key]).reshape(-1))
             e.set_slot_messages(ins, np.concatenate(data))
         e.update_slot(q)
         cache[edge] = e.slot_messages([q]).reshape(t.msg_shape(q))
@@ -127,7 +129,9 @@ class JacobiGPUUpdate(MessageUpdateAlgorithm):
     def update(self, cache, factors, edge):
         if isinstance(edge, JacobiSweep):
             # one device sweep over every directed message, written back into the cache
-            fl, e = _factor_engine(factors, self.dtype, True, self.path)
+            fl = _flat(factors)
+            fl, e = _factor_engine(factors, context_dtype(self.dtype, fl.sites, cache), True,
+                                   self.path)
             e.set_messages(cache.to_flat(fl.topo))
             e.sweep()
             new = MessageCache.from_flat(fl.topo, e.messages())
@@ -259,11 +263,12 @@ def beliefpropagation(factors, messages, *, edges=None, stopping_criterion=None,
                                   "`edges` need a sequential strategy")
         from .engine import Engine
 
-        e = Engine(topo, fl.sites, dtype=algo.dtype, path=algo.path)
+        dtype = context_dtype(algo.dtype, fl.sites, cache)
+        e = Engine(topo, fl.sites, dtype=dtype, path=algo.path)
         try:
             e.set_messages(cache.to_flat(topo))
             r = e.run(maxiter=maxiter, tol=tol, criterion=algo.criterion, use_graph=algo.use_graph)
-            if algo.polish:
+            if algo.polish and dtype != "c128":  # complex runs are FP64 already
                 e.polish_fp64(algo.polish)
             out = MessageCache.from_flat(topo, e.messages())
         finally:
@@ -278,7 +283,8 @@ def beliefpropagation(factors, messages, *, edges=None, stopping_criterion=None,
         from .engine import Engine
 
         order = [fl.slot(ed) for ed in edges]
-        e = Engine(topo, fl.sites, dtype=algo.dtype, path="generic")
+        e = Engine(topo, fl.sites, dtype=context_dtype(algo.dtype, fl.sites, cache),
+                   path="generic")
         try:
             if not algo.normalize:
                 e.set_option("normalize", 0)

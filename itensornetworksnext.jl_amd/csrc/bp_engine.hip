@@ -23,6 +23,7 @@
 #include "../../include/bp_mi355x.h"
 #include "common.h"
 #include "generic.h"
+#include "complex.h"
 #include "vertex.h"
 #include "vertex16.h"
 #include "vertex64.h"
@@ -185,17 +186,9 @@ __global__ void k_diff_slots(DevTopo tp, const Ta *a, const Tb *b, double *res_f
     }
 }
 
-// BP log-norm per network (one workgroup per network): Neumaier-compensated
+// BP log-norm per network (one workgroup per network): Neumaier-compensated (NeuSum)
 // sum log|Z_v| - sum log|Z_e| with the reference's complex promotion and -Inf rule
 // (messagecache.jl:185-201).
-struct NeuSum {
-    double s = 0.0, c = 0.0;
-    __device__ void add(double x) {
-        const double t = s + x;
-        if (fabs(s) >= fabs(x)) c += (s - t) + x; else c += (x - t) + s;
-        s = t;
-    }
-};
 
 __global__ __launch_bounds__(BP_BLOCK) void k_lognorm_nets(DevTopo tp, const int *vbeg,
                                                             const int *pbeg, const double *zv,
@@ -350,7 +343,12 @@ struct bp_context {
     // adjoint scratch
     AdjointScratch adj;
 
-    size_t elt() const { return dtype == BP_F64 ? sizeof(double) : sizeof(float); }
+    size_t elt() const {
+        return dtype == BP_C128 ? sizeof(cplx) : dtype == BP_F64 ? sizeof(double) : sizeof(float);
+    }
+    // doubles per element in host buffers, FP64 site copies and scalars (2: complex)
+    int hw() const { return dtype == BP_C128 ? 2 : 1; }
+    bool is_cplx() const { return dtype == BP_C128; }
 
     DevTopo topo() const {
         DevTopo t;
@@ -443,7 +441,7 @@ static GenLaunch gen_launch(bp_context *c, int nitems, size_t esz) {
     GenLaunch g;
     g.mats_elems = std::max(c->max_mats, 1);
     g.out_elems = std::max(c->maxD * c->maxD, 1);
-    g.red_elems = std::max(c->maxD * c->maxD, BP_BLOCK);
+    g.red_elems = (esz == sizeof(cplx) ? 2 : 1) * std::max(c->maxD * c->maxD, BP_BLOCK);
     g.buf_elems = std::max(c->max_site, 1LL);
     size_t base = sizeof(double) * g.red_elems + esz * (g.mats_elems + g.out_elems) + 16;
     size_t with = base + 2 * esz * g.buf_elems;
@@ -472,17 +470,25 @@ static int launch_update_generic(bp_context *c, const T *sites, const T *mold, T
     if (q1 <= q0) return BP_OK;
     GenLaunch g = gen_launch(c, q1 - q0, sizeof(T));
     RETCHK(ensure_ws(c, g, sizeof(T)));
-    hipLaunchKernelGGL(k_update_generic<T>, dim3(g.grid), dim3(BP_BLOCK), g.lds_bytes, c->stream,
-                       c->topo(), sites, mold, mnew, ctl, c->d_res_fid, c->d_res_abs, q0, q1,
-                       list, g.mats_elems, g.out_elems, g.red_elems, g.buf_elems, g.use_lds,
-                       (T *)c->d_ws, g.ws_stride, flags);
+    if constexpr (std::is_same<T, cplx>::value)
+        hipLaunchKernelGGL(k_update_cplx<BP_BLOCK>, dim3(g.grid), dim3(BP_BLOCK), g.lds_bytes,
+                           c->stream, c->topo(), sites, mold, mnew, ctl, c->d_res_fid,
+                           c->d_res_abs, q0, q1, list, g.mats_elems, g.out_elems, g.red_elems,
+                           g.buf_elems, g.use_lds, (T *)c->d_ws, g.ws_stride, flags);
+    else
+        hipLaunchKernelGGL(k_update_generic<T>, dim3(g.grid), dim3(BP_BLOCK), g.lds_bytes,
+                           c->stream, c->topo(), sites, mold, mnew, ctl, c->d_res_fid,
+                           c->d_res_abs, q0, q1, list, g.mats_elems, g.out_elems, g.red_elems,
+                           g.buf_elems, g.use_lds, (T *)c->d_ws, g.ws_stride, flags);
     HIPCHK(hipGetLastError());
     return BP_OK;
 }
 
 // ---- generic buckets
 
-static const char *dtype_name(int dt) { return dt == BP_F64 ? "f64" : dt == BP_F32 ? "f32" : "bf16"; }
+static const char *dtype_name(int dt) {
+    return dt == BP_F64 ? "f64" : dt == BP_F32 ? "f32" : dt == BP_C128 ? "c128" : "bf16";
+}
 
 static constexpr size_t GEN_LDS_MAX = 160 * 1024;  // LDS per workgroup on gfx950
 static constexpr size_t GEN_LDS_DEFAULT = 64 * 1024;
@@ -491,7 +497,8 @@ static GenLaunch gen_launch_bucket(const GenBucket &b, size_t esz) {
     GenLaunch g;
     g.mats_elems = std::max(b.max_mats, 1);
     g.out_elems = std::max(b.maxD * b.maxD, 1);
-    g.red_elems = std::max(b.maxD * b.maxD, b.bs);
+    // complex kernels (16-byte elements) keep interleaved (re, im) partial sums
+    g.red_elems = (esz == sizeof(cplx) ? 2 : 1) * std::max(b.maxD * b.maxD, b.bs);
     g.buf_elems = std::max(b.max_site, 1LL);
     const size_t base = sizeof(double) * g.red_elems + esz * (g.mats_elems + g.out_elems) + 16;
     const size_t with = base + 2 * esz * g.buf_elems;
@@ -512,12 +519,15 @@ static bool bucket_lds_fits(const GenBucket &b, size_t esz, int mode) {
 static int gen_set_lds_limits() {
     static int done = 0;
     if (done) return BP_OK;
-    const void *fns[6] = {(const void *)k_update_generic<float, 128>,
+    const void *fns[9] = {(const void *)k_update_generic<float, 128>,
                           (const void *)k_update_generic<float, 256>,
                           (const void *)k_update_generic<float, 512>,
                           (const void *)k_update_generic<double, 128>,
                           (const void *)k_update_generic<double, 256>,
-                          (const void *)k_update_generic<double, 512>};
+                          (const void *)k_update_generic<double, 512>,
+                          (const void *)k_update_cplx<128>,
+                          (const void *)k_update_cplx<256>,
+                          (const void *)k_update_cplx<512>};
     for (const void *f : fns)
         HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)GEN_LDS_MAX));
@@ -596,10 +606,16 @@ static int launch_bucket(bp_context *c, const GenBucket &b, const T *sites, cons
     const GenLaunch g = gen_launch_bucket(b, sizeof(T));
     const int n = (int)b.slots.size();
 #define GEN_B(BS)                                                                               \
-    hipLaunchKernelGGL((k_update_generic<T, BS>), dim3(g.grid), dim3(BS), g.lds_bytes,         \
-                       c->stream, c->topo(), sites, mold, mnew, ctl, c->d_res_fid,              \
-                       c->d_res_abs, 0, n, b.d_slots, g.mats_elems, g.out_elems, g.red_elems,   \
-                       g.buf_elems, g.use_lds, (T *)c->d_ws, g.ws_stride, flags)
+    if constexpr (std::is_same<T, cplx>::value)                                                 \
+        hipLaunchKernelGGL((k_update_cplx<BS>), dim3(g.grid), dim3(BS), g.lds_bytes, c->stream,  \
+                           c->topo(), sites, mold, mnew, ctl, c->d_res_fid, c->d_res_abs, 0, n, \
+                           b.d_slots, g.mats_elems, g.out_elems, g.red_elems, g.buf_elems,      \
+                           g.use_lds, (T *)c->d_ws, g.ws_stride, flags);                        \
+    else                                                                                        \
+        hipLaunchKernelGGL((k_update_generic<T, BS>), dim3(g.grid), dim3(BS), g.lds_bytes,     \
+                           c->stream, c->topo(), sites, mold, mnew, ctl, c->d_res_fid,          \
+                           c->d_res_abs, 0, n, b.d_slots, g.mats_elems, g.out_elems,            \
+                           g.red_elems, g.buf_elems, g.use_lds, (T *)c->d_ws, g.ws_stride, flags)
     switch (b.bs) {
         case 128: GEN_B(128); break;
         case 512: GEN_B(512); break;
@@ -635,9 +651,14 @@ template <typename T>
 static int launch_vertex_scalars(bp_context *c, const T *sites, const T *msgs) {
     GenLaunch g = gen_launch(c, c->nv, sizeof(T));
     RETCHK(ensure_ws(c, g, sizeof(T)));
-    hipLaunchKernelGGL(k_vertex_scalar_generic<T>, dim3(g.grid), dim3(BP_BLOCK), g.lds_bytes,
-                       c->stream, c->topo(), sites, msgs, c->d_zv, g.mats_elems, g.out_elems,
-                       g.red_elems, g.buf_elems, g.use_lds, (T *)c->d_ws, g.ws_stride);
+    if constexpr (std::is_same<T, cplx>::value)
+        hipLaunchKernelGGL(k_vertex_scalar_cplx, dim3(g.grid), dim3(BP_BLOCK), g.lds_bytes,
+                           c->stream, c->topo(), sites, msgs, c->d_zv, g.mats_elems, g.out_elems,
+                           g.red_elems, g.buf_elems, g.use_lds, (T *)c->d_ws, g.ws_stride);
+    else
+        hipLaunchKernelGGL(k_vertex_scalar_generic<T>, dim3(g.grid), dim3(BP_BLOCK), g.lds_bytes,
+                           c->stream, c->topo(), sites, msgs, c->d_zv, g.mats_elems, g.out_elems,
+                           g.red_elems, g.buf_elems, g.use_lds, (T *)c->d_ws, g.ws_stride);
     HIPCHK(hipGetLastError());
     return BP_OK;
 }
@@ -645,9 +666,14 @@ static int launch_vertex_scalars(bp_context *c, const T *sites, const T *msgs) {
 template <typename T>
 static int launch_edge_scalars(bp_context *c, const T *msgs) {
     const int blocks = (c->nports + 255) / 256;
-    if (blocks > 0)
-        hipLaunchKernelGGL(k_edge_scalars<T>, dim3(blocks), dim3(256), 0, c->stream, c->topo(),
-                           msgs, c->d_ze);
+    if (blocks > 0) {
+        if constexpr (std::is_same<T, cplx>::value)
+            hipLaunchKernelGGL(k_edge_scalars_cplx, dim3(blocks), dim3(256), 0, c->stream,
+                               c->topo(), msgs, c->d_ze);
+        else
+            hipLaunchKernelGGL(k_edge_scalars<T>, dim3(blocks), dim3(256), 0, c->stream,
+                               c->topo(), msgs, c->d_ze);
+    }
     HIPCHK(hipGetLastError());
     return BP_OK;
 }
@@ -699,7 +725,7 @@ static int gen_flags(const bp_context *c, bool bf16) {
 
 static int resolve_path(bp_context *c) {
     // the vertex kernels always normalise: unnormalised updates run on the generic kernel
-    if (!c->normalize) return c->path == BP_PATH_VERTEX ? -1 : BP_PATH_GENERIC;
+    if (!c->normalize || c->is_cplx()) return c->path == BP_PATH_VERTEX ? -1 : BP_PATH_GENERIC;
     if (c->dtype == BP_F64) {  // the FP64 vertex kernel covers every network or none
         if (c->path != BP_PATH_GENERIC && c->v64_ok) return BP_PATH_VERTEX;
         return c->path == BP_PATH_VERTEX ? -1 : BP_PATH_GENERIC;
@@ -727,7 +753,7 @@ static int prepare_sweep(bp_context *c) {
         c->fold_n = c->nnet;
     }
     if (path != BP_PATH_VERTEX)
-        RETCHK(ensure_bucket_ws(c, c->dtype == BP_F64 ? sizeof(double) : sizeof(float)));
+        RETCHK(ensure_bucket_ws(c, c->elt()));
     return BP_OK;
 }
 
@@ -842,7 +868,10 @@ static int sweep_once(bp_context *c, const NetCtl &done, bool *folded = nullptr,
         HIPCHK(hipGetLastError());
     }
     if (path != BP_PATH_VERTEX) {
-        if (c->dtype == BP_F64)
+        if (c->is_cplx())
+            RETCHK(launch_buckets<cplx>(c, (const cplx *)c->d_sites, (const cplx *)src,
+                                        (cplx *)dst, done, gen_flags(c, false)));
+        else if (c->dtype == BP_F64)
             RETCHK(launch_buckets<double>(c, (const double *)c->d_sites, (const double *)src,
                                           (double *)dst, done, gen_flags(c, false)));
         else
@@ -875,6 +904,11 @@ static int rebuild_vertex_plan(bp_context *c) {
     v64_invalidate(c);
     vertex_plan_release(&c->vplan);
     free_buckets(c);
+    if (c->is_cplx()) {  // complex arithmetic runs on the generic kernel only
+        c->vplan_ok = false;
+        c->vplan.why = "complex contexts run on the generic kernel";
+        return BP_OK;
+    }
     c->vplan_ok = vertex_plan_build(&c->vplan, c->dtype, c->plain, c->nv, c->port_off.data(),
                                     c->port_edge.data(), c->edge_dim.data(), c->phys_dim.data(),
                                     c->site_off.data(), c->msg_off.data(), c->port_rev.data(),
@@ -918,7 +952,7 @@ int bp_version(void) { return 10000; }
 
 int bp_create(bp_context **out, int device, int dtype) {
     if (!out) return set_err(BP_ERR_ARG, "out is NULL");
-    if (dtype != BP_F64 && dtype != BP_F32 && dtype != BP_BF16)
+    if (dtype != BP_F64 && dtype != BP_F32 && dtype != BP_BF16 && dtype != BP_C128)
         return set_err(BP_ERR_ARG, "unknown dtype %d", dtype);
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
@@ -932,6 +966,7 @@ int bp_create(bp_context **out, int device, int dtype) {
     // FP64 contexts: the FP64 vertex kernel for largest bond dimensions 6..8 by default
     // (below 6 the zero padding to 8 costs more than the generic kernel saves)
     if (dtype == BP_F64) c->vx_mask = 0x1C0u;
+    if (dtype == BP_C128) c->vx_mask = 0u;
     if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         delete c;
@@ -1078,8 +1113,8 @@ int bp_set_topology(bp_context *c, int nv, int ne, const int *port_off, const in
     RETCHK(dev_alloc(&c->d_msg[1], es * c->nmsg));
     RETCHK(dev_alloc((void **)&c->d_res_fid, sizeof(double) * np));
     RETCHK(dev_alloc((void **)&c->d_res_abs, sizeof(double) * np));
-    RETCHK(dev_alloc((void **)&c->d_zv, sizeof(double) * nv));
-    RETCHK(dev_alloc((void **)&c->d_ze, sizeof(double) * std::max(ne, 1)));
+    RETCHK(dev_alloc((void **)&c->d_zv, sizeof(double) * c->hw() * nv));
+    RETCHK(dev_alloc((void **)&c->d_ze, sizeof(double) * c->hw() * std::max(ne, 1)));
     RETCHK(dev_alloc((void **)&c->d_lre, sizeof(double) * nnet));
     RETCHK(dev_alloc((void **)&c->d_lim, sizeof(double) * nnet));
     // per-network run state in one block (one copy back per run): delta, done, iter, at
@@ -1087,7 +1122,7 @@ int bp_set_topology(bp_context *c, int nv, int ne, const int *port_off, const in
     c->d_done = (int *)(c->d_delta + nnet);
     c->d_iter = c->d_done + nnet;
     c->d_at = c->d_iter + nnet;
-    RETCHK(dev_alloc((void **)&c->d_sites64, sizeof(double) * c->nsite));
+    RETCHK(dev_alloc((void **)&c->d_sites64, sizeof(double) * c->hw() * c->nsite));
     RETCHK(dev_alloc(&c->d_sites, es * c->nsite));
     if (c->dtype == BP_BF16) RETCHK(dev_alloc((void **)&c->d_sites_bf, sizeof(float) * c->nsite));
     if (c->d_msg64[0]) { (void)hipFree(c->d_msg64[0]); c->d_msg64[0] = nullptr; }
@@ -1105,7 +1140,7 @@ int bp_upload_sites(bp_context *c, const double *sites) {
     if (!c || !sites) return set_err(BP_ERR_ARG, "NULL argument");
     if (!c->have_topo) return set_err(BP_ERR_STATE, "bp_set_topology first");
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipMemcpyAsync(c->d_sites64, sites, sizeof(double) * c->nsite,
+    HIPCHK(hipMemcpyAsync(c->d_sites64, sites, sizeof(double) * c->hw() * c->nsite,
                           hipMemcpyHostToDevice, c->stream));
     RETCHK(refresh_sites(c));
     c->have_sites = true;
@@ -1116,7 +1151,7 @@ int bp_download_sites(bp_context *c, double *sites) {
     if (!c || !sites) return set_err(BP_ERR_ARG, "NULL argument");
     if (!c->have_sites) return set_err(BP_ERR_STATE, "no site tensors");
     HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipMemcpyAsync(sites, c->d_sites64, sizeof(double) * c->nsite,
+    HIPCHK(hipMemcpyAsync(sites, c->d_sites64, sizeof(double) * c->hw() * c->nsite,
                           hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return BP_OK;
@@ -1127,8 +1162,8 @@ int bp_download_sites(bp_context *c, double *sites) {
 // vertex kernels' padded layouts
 static int refresh_sites(bp_context *c) {
     const int blocks = (int)std::min<long long>((c->nsite + 255) / 256, 65535);
-    if (c->dtype == BP_F64) {
-        HIPCHK(hipMemcpyAsync(c->d_sites, c->d_sites64, sizeof(double) * c->nsite,
+    if (c->dtype == BP_F64 || c->is_cplx()) {
+        HIPCHK(hipMemcpyAsync(c->d_sites, c->d_sites64, sizeof(double) * c->hw() * c->nsite,
                               hipMemcpyDeviceToDevice, c->stream));
     } else {
         hipLaunchKernelGGL((k_convert<double, float>), dim3(std::max(blocks, 1)), dim3(256), 0,
@@ -1160,8 +1195,8 @@ int bp_init_messages(bp_context *c, int kind, const double *host) {
     c->fp64_active = false;
     if (kind == BP_INIT_UPLOAD) {
         if (!host) return set_err(BP_ERR_ARG, "host buffer required for BP_INIT_UPLOAD");
-        if (c->dtype == BP_F64) {
-            HIPCHK(hipMemcpyAsync(c->d_msg[0], host, sizeof(double) * c->nmsg,
+        if (c->dtype == BP_F64 || c->is_cplx()) {
+            HIPCHK(hipMemcpyAsync(c->d_msg[0], host, sizeof(double) * c->hw() * c->nmsg,
                                   hipMemcpyHostToDevice, c->stream));
         } else {
             double *tmp = nullptr;
@@ -1177,7 +1212,10 @@ int bp_init_messages(bp_context *c, int kind, const double *host) {
     } else if (kind == BP_INIT_IDENTITY || kind == BP_INIT_ONES) {
         if (c->nports == 0) {
             // no edges: nothing to initialise
-        } else if (c->dtype == BP_F64)
+        } else if (c->is_cplx())
+            hipLaunchKernelGGL(k_init_messages<cplx>, dim3(c->nports), dim3(64), 0, c->stream,
+                               c->topo(), (cplx *)c->d_msg[0], kind);
+        else if (c->dtype == BP_F64)
             hipLaunchKernelGGL(k_init_messages<double>, dim3(c->nports), dim3(64), 0, c->stream,
                                c->topo(), (double *)c->d_msg[0], kind);
         else
@@ -1199,8 +1237,8 @@ int bp_download_messages(bp_context *c, double *out) {
     if (c->fp64_active) {
         HIPCHK(hipMemcpyAsync(out, c->d_msg64[c->cur64], sizeof(double) * c->nmsg,
                               hipMemcpyDeviceToHost, c->stream));
-    } else if (c->dtype == BP_F64) {
-        HIPCHK(hipMemcpyAsync(out, c->d_msg[c->cur], sizeof(double) * c->nmsg,
+    } else if (c->dtype == BP_F64 || c->is_cplx()) {
+        HIPCHK(hipMemcpyAsync(out, c->d_msg[c->cur], sizeof(double) * c->hw() * c->nmsg,
                               hipMemcpyDeviceToHost, c->stream));
     } else {
         std::vector<float> tmp(c->nmsg);
@@ -1231,9 +1269,11 @@ static int slot_copy(bp_context *c, int n, const int *slots, double *host, int d
     long long total = 0;
     RETCHK(slot_span(c, n, slots, &total));
     HIPCHK(hipSetDevice(c->device));
-    const bool f64 = c->fp64_active || c->dtype == BP_F64;
+    // FP64 and complex messages copy as they are (hw doubles per element); FP32 converts
+    const bool f64 = c->fp64_active || c->dtype == BP_F64 || c->is_cplx();
+    const int hw = c->fp64_active ? 1 : c->hw();
     char *base = c->fp64_active ? (char *)c->d_msg64[c->cur64] : (char *)c->d_msg[c->cur];
-    const size_t esz = f64 ? sizeof(double) : sizeof(float);
+    const size_t esz = f64 ? sizeof(double) * hw : sizeof(float);
     std::vector<float> tmp(f64 ? 0 : (size_t)total);
     if (!f64 && dir == 0)
         for (long long i = 0; i < total; ++i) tmp[i] = (float)host[i];
@@ -1241,7 +1281,7 @@ static int slot_copy(bp_context *c, int n, const int *slots, double *host, int d
     for (int i = 0; i < n; ++i) {
         const long long mo = c->msg_off[slots[i]], len = c->msg_off[slots[i] + 1] - mo;
         char *dev = base + esz * mo;
-        char *h = f64 ? (char *)(host + off) : (char *)(tmp.data() + off);
+        char *h = f64 ? (char *)(host + hw * off) : (char *)(tmp.data() + off);
         if (dir == 0)
             HIPCHK(hipMemcpyAsync(dev, h, esz * len, hipMemcpyHostToDevice, c->stream));
         else
@@ -1272,6 +1312,8 @@ int bp_set_path(bp_context *c, int path) {
         return set_err(BP_ERR_ARG, "unknown path %d", path);
     if (path == BP_PATH_VERTEX && !c->normalize)
         return set_err(BP_ERR_UNSUPPORTED, "the vertex kernels always normalise (option normalize is 0)");
+    if (path == BP_PATH_VERTEX && c->is_cplx())
+        return set_err(BP_ERR_UNSUPPORTED, "complex contexts run on the generic kernel");
     if (path == BP_PATH_VERTEX && c->dtype == BP_F64) {
         RETCHK(ensure_v64(c));
         if (!c->v64_plan_ok)
@@ -1349,6 +1391,14 @@ static int ready(bp_context *c) {
     if (!c) return set_err(BP_ERR_ARG, "ctx is NULL");
     if (!c->have_topo || !c->have_sites || !c->have_msgs)
         return set_err(BP_ERR_STATE, "topology, sites and messages must be set first");
+    return BP_OK;
+}
+
+// Entry points whose kernels compute in real arithmetic only.
+static int real_only(const bp_context *c, const char *what) {
+    if (c->is_cplx())
+        return set_err(BP_ERR_UNSUPPORTED, "%s: complex contexts support BP, scalars, log-norm "
+                       "and iterate_diff only", what);
     return BP_OK;
 }
 
@@ -1575,6 +1625,9 @@ static double tune_time_bucket(bp_context *c, const GenBucket &b, int reps) {
     const NetCtl ctl = make_ctl(c, false);
     void *src = c->d_msg[c->cur], *dst = c->d_msg[c->cur ^ 1];
     auto launch = [&]() {
+        if (c->is_cplx())
+            return launch_bucket<cplx>(c, b, (const cplx *)c->d_sites, (const cplx *)src,
+                                       (cplx *)dst, ctl, 0);
         if (c->dtype == BP_F64)
             return launch_bucket<double>(c, b, (const double *)c->d_sites, (const double *)src,
                                          (double *)dst, ctl, 0);
@@ -1613,7 +1666,7 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
     c->fp64_active = false;
     int rc = BP_OK;
     // 1. vertex kernels or generic kernel, per largest bond dimension (<= 12 FP32, <= 16 BF16)
-    if (!c->plain && c->path == BP_PATH_AUTO) {
+    if (!c->plain && !c->is_cplx() && c->path == BP_PATH_AUTO) {
         std::map<int, int> dmax_d;  // largest bond dimension -> physical dimension
         for (int n = 0; n < c->nnet && rc == BP_OK; ++n) {
             int dm = 0, deg = 0, d = c->phys_dim[c->vbeg[n]];
@@ -1809,7 +1862,11 @@ int bp_update_slot(bp_context *c, int slot) {
     HIPCHK(hipSetDevice(c->device));
     c->fp64_active = false;
     void *m = c->d_msg[c->cur];
-    if (c->dtype == BP_F64)
+    if (c->is_cplx())
+        RETCHK(launch_update_generic<cplx>(c, (const cplx *)c->d_sites, (const cplx *)m,
+                                           (cplx *)m, make_ctl(c, false), slot, slot + 1,
+                                           nullptr, gen_flags(c, false)));
+    else if (c->dtype == BP_F64)
         RETCHK(launch_update_generic<double>(c, (const double *)c->d_sites, (const double *)m,
                                              (double *)m, make_ctl(c, false), slot, slot + 1,
                                              nullptr, gen_flags(c, false)));
@@ -1855,7 +1912,10 @@ static int diff_launch(bp_context *c, const void *other, bool other64, double *f
     if (!mab) mab = c->d_res_abs;
     if (c->nports == 0) return BP_OK;
     const dim3 g(c->nports), b(64);
-    if (is64 && other64)
+    if (c->is_cplx() && !c->fp64_active)  // complex messages, `other` complex too
+        hipLaunchKernelGGL(k_diff_slots_cplx, g, b, 0, c->stream, c->topo(), (const cplx *)m,
+                           (const cplx *)other, fid, mab);
+    else if (is64 && other64)
         hipLaunchKernelGGL((k_diff_slots<double, double>), g, b, 0, c->stream, c->topo(),
                            (const double *)m, (const double *)other, fid, mab);
     else if (is64)
@@ -1876,9 +1936,10 @@ int bp_iterate_diff(bp_context *c, const double *other, double *out) {
     if (!other || !out) return set_err(BP_ERR_ARG, "NULL argument");
     HIPCHK(hipSetDevice(c->device));
     double *d_other = nullptr;
-    HIPCHK(hipMalloc((void **)&d_other, sizeof(double) * std::max<long long>(c->nmsg, 1)));
+    const long long nd = c->hw() * c->nmsg;  // doubles in `other`
+    HIPCHK(hipMalloc((void **)&d_other, sizeof(double) * std::max<long long>(nd, 1)));
     int rc = BP_OK;
-    if (hipMemcpyAsync(d_other, other, sizeof(double) * c->nmsg, hipMemcpyHostToDevice,
+    if (hipMemcpyAsync(d_other, other, sizeof(double) * nd, hipMemcpyHostToDevice,
                        c->stream) != hipSuccess)
         rc = set_err(BP_ERR_HIP, "copy failed");
     if (rc == BP_OK) rc = diff_launch(c, d_other, true);
@@ -1925,7 +1986,11 @@ int bp_run_sequential(bp_context *c, const int *order, int norder, int maxiter, 
         }
         void *m = c->d_msg[c->cur];
         for (int i = 0; i < norder && rc == BP_OK; ++i) {
-            if (c->dtype == BP_F64)
+            if (c->is_cplx())
+                rc = launch_update_generic<cplx>(c, (const cplx *)c->d_sites, (const cplx *)m,
+                                                 (cplx *)m, off, order[i], order[i] + 1, nullptr,
+                                                 gen_flags(c, false));
+            else if (c->dtype == BP_F64)
                 rc = launch_update_generic<double>(c, (const double *)c->d_sites,
                                                    (const double *)m, (double *)m, off, order[i],
                                                    order[i] + 1, nullptr, gen_flags(c, false));
@@ -1971,7 +2036,10 @@ int bp_run_sequential(bp_context *c, const int *order, int norder, int maxiter, 
 static int scalars(bp_context *c) {
     bool is64 = false;
     const void *m = current_msgs(c, &is64);
-    if (is64) {
+    if (c->is_cplx()) {
+        RETCHK(launch_vertex_scalars<cplx>(c, (const cplx *)c->d_sites64, (const cplx *)m));
+        RETCHK(launch_edge_scalars<cplx>(c, (const cplx *)m));
+    } else if (is64) {
         RETCHK(launch_vertex_scalars<double>(c, c->d_sites64, (const double *)m));
         RETCHK(launch_edge_scalars<double>(c, (const double *)m));
     } else {
@@ -1986,7 +2054,8 @@ int bp_vertex_scalars(bp_context *c, double *zv) {
     if (!zv) return set_err(BP_ERR_ARG, "NULL argument");
     HIPCHK(hipSetDevice(c->device));
     RETCHK(scalars(c));
-    HIPCHK(hipMemcpyAsync(zv, c->d_zv, sizeof(double) * c->nv, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(zv, c->d_zv, sizeof(double) * c->hw() * c->nv, hipMemcpyDeviceToHost,
+                          c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return BP_OK;
 }
@@ -1997,8 +2066,8 @@ int bp_edge_scalars(bp_context *c, double *ze) {
     HIPCHK(hipSetDevice(c->device));
     RETCHK(scalars(c));
     if (c->ne > 0)
-        HIPCHK(hipMemcpyAsync(ze, c->d_ze, sizeof(double) * c->ne, hipMemcpyDeviceToHost,
-                              c->stream));
+        HIPCHK(hipMemcpyAsync(ze, c->d_ze, sizeof(double) * c->hw() * c->ne,
+                              hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return BP_OK;
 }
@@ -2008,8 +2077,13 @@ int bp_lognorm(bp_context *c, double *out_re, double *out_im) {
     if (!out_re) return set_err(BP_ERR_ARG, "NULL argument");
     HIPCHK(hipSetDevice(c->device));
     RETCHK(scalars(c));
-    hipLaunchKernelGGL(k_lognorm_nets, dim3(c->nnet), dim3(BP_BLOCK), 0, c->stream, c->topo(),
-                       c->d_vbeg, c->d_pbeg, c->d_zv, c->d_ze, c->d_lre, c->d_lim);
+    if (c->is_cplx())
+        hipLaunchKernelGGL(k_lognorm_nets_cplx, dim3(c->nnet), dim3(BP_BLOCK), 0, c->stream,
+                           c->topo(), c->d_vbeg, c->d_pbeg, c->d_zv, c->d_ze, c->d_lre, c->d_lim);
+    else
+        hipLaunchKernelGGL(k_lognorm_nets, dim3(c->nnet), dim3(BP_BLOCK), 0, c->stream,
+                           c->topo(), c->d_vbeg, c->d_pbeg, c->d_zv, c->d_ze, c->d_lre,
+                           c->d_lim);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out_re, c->d_lre, sizeof(double) * c->nnet, hipMemcpyDeviceToHost,
                           c->stream));
@@ -2043,19 +2117,22 @@ __global__ __launch_bounds__(256) void k_max_slots(const double *v, int n, doubl
 int bp_iterate_diff_noalloc(bp_context *c, const double *other, double *out) {
     RETCHK(ready(c));
     if (!other || !out) return set_err(BP_ERR_ARG, "NULL argument");
-    // scratch layout: fid [nports] | max-abs [nports] | staged `other` [nmsg] | result [1]
-    const long long need = 2LL * c->nports + c->nmsg + 1;
+    // scratch layout: fid [nports] | max-abs [nports] | staged `other` [hw * nmsg doubles;
+    // its offset, 16 * nports bytes, keeps complex entries 16-byte aligned] | result [1]
+    const long long nd = c->hw() * c->nmsg;
+    const long long need = 2LL * c->nports + nd + 1;
     if (need > BP_SCRATCH_DOUBLES)
         return set_err(BP_ERR_ARG,
-                       "bp_iterate_diff_noalloc: 2 * %d slots + %lld message entries + 1 exceed "
+                       "bp_iterate_diff_noalloc: 2 * %d slots + %lld message doubles + 1 exceed "
                        "the %d-double scratch; use bp_iterate_diff",
-                       c->nports, c->nmsg, BP_SCRATCH_DOUBLES);
+                       c->nports, nd, BP_SCRATCH_DOUBLES);
     HIPCHK(hipSetDevice(c->device));
     double *sc = g_scratch_ptr[c->device];
     if (!sc) return set_err(BP_ERR_STATE, "no scratch address for device %d", c->device);
-    double *fid = sc, *mab = sc + c->nports, *stage = mab + c->nports, *res = stage + c->nmsg;
-    if (c->nmsg > 0)
-        HIPCHK(hipMemcpyAsync(stage, other, sizeof(double) * c->nmsg, hipMemcpyHostToDevice,
+    double *fid = sc, *mab = sc + c->nports;
+    double *stage = mab + c->nports, *res = stage + nd;
+    if (nd > 0)
+        HIPCHK(hipMemcpyAsync(stage, other, sizeof(double) * nd, hipMemcpyHostToDevice,
                               c->stream));
     RETCHK(diff_launch(c, stage, true, fid, mab));
     hipLaunchKernelGGL(k_max_slots, dim3(1), dim3(256), 0, c->stream, fid, c->nports, res);
@@ -2067,6 +2144,7 @@ int bp_iterate_diff_noalloc(bp_context *c, const double *other, double *out) {
 
 int bp_polish_fp64(bp_context *c, int nsweeps, double *out_delta) {
     RETCHK(ready(c));
+    RETCHK(real_only(c, "bp_polish_fp64"));
     HIPCHK(hipSetDevice(c->device));
     if (!c->d_msg64[0]) RETCHK(dev_alloc((void **)&c->d_msg64[0], sizeof(double) * c->nmsg));
     if (!c->d_msg64[1]) RETCHK(dev_alloc((void **)&c->d_msg64[1], sizeof(double) * c->nmsg));
@@ -2115,6 +2193,7 @@ int bp_polish_fp64(bp_context *c, int nsweeps, double *out_delta) {
 
 static int grad_lognorm_impl(bp_context *c, const GradOut &go) {
     RETCHK(ready(c));
+    RETCHK(real_only(c, "log-norm gradient"));
     if (!c->normalize) return set_err(BP_ERR_UNSUPPORTED, "gradients need normalised BP (option normalize)");
     if (c->plain) return set_err(BP_ERR_UNSUPPORTED, "gradient needs a norm network");
     HIPCHK(hipSetDevice(c->device));
@@ -2136,6 +2215,7 @@ static int grad_lognorm_impl(bp_context *c, const GradOut &go) {
 static int tfim_impl(bp_context *c, double J, double h, double *energy, const GradOut &go,
                      double adj_tol, int adj_maxiter, int *out_iters, double *out_res) {
     RETCHK(ready(c));
+    RETCHK(real_only(c, "TFIM energy"));
     if (!energy) return set_err(BP_ERR_ARG, "NULL argument");
     if (c->plain) return set_err(BP_ERR_UNSUPPORTED, "energy needs a norm network");
     if (!c->normalize && go.any()) return set_err(BP_ERR_UNSUPPORTED, "gradients need normalised BP (option normalize)");
@@ -2162,6 +2242,7 @@ static int tfim_impl(bp_context *c, double J, double h, double *energy, const Gr
 static int expect_impl(bp_context *c, const double *ops_v, const double *ops_a,
                        const double *ops_b, int ld, double *out) {
     RETCHK(ready(c));
+    RETCHK(real_only(c, "expectation values"));
     if (!out || (!ops_v && (!ops_a || !ops_b))) return set_err(BP_ERR_ARG, "NULL argument");
     if (c->plain) return set_err(BP_ERR_UNSUPPORTED, "expectation values need a norm network");
     for (int u = 0; u < c->nv; ++u)
@@ -2192,6 +2273,7 @@ static int local_impl(bp_context *c, const double *ops_v, const double *ops_firs
                       const double *ops_second, int ld, double *energy, const GradOut &go,
                       double adj_tol, int adj_maxiter, int *out_iters, double *out_res) {
     RETCHK(ready(c));
+    RETCHK(real_only(c, "local energy"));
     if (!energy || (!ops_v && (!ops_first || !ops_second)))
         return set_err(BP_ERR_ARG, "NULL argument");
     if ((ops_first == nullptr) != (ops_second == nullptr))
@@ -2464,6 +2546,7 @@ int bp_apply_gates_resident(bp_context *c, int ngates, const int *kind, const in
                             const double *gates, int maxdim, int normalize, double rtol,
                             int *out_chi, double *out_s, int s_stride) {
     RETCHK(ready(c));
+    RETCHK(real_only(c, "bp_apply_gates_resident"));
     if (ngates < 0 || (ngates > 0 && (!kind || !where || !gates)))
         return set_err(BP_ERR_ARG, "NULL argument");
     if (c->plain) return set_err(BP_ERR_UNSUPPORTED, "gates act on norm networks");

@@ -167,6 +167,16 @@ __device__ __forceinline__ double wave_max_v(double v) {
     return vmax_nan(h.first, h.second);
 }
 
+// Neumaier-compensated running sum of doubles (the log-norm accumulations).
+struct NeuSum {
+    double s = 0.0, c = 0.0;
+    __device__ void add(double x) {
+        const double t = s + x;
+        if (fabs(s) >= fabs(x)) c += (s - t) + x; else c += (x - t) + s;
+        s = t;
+    }
+};
+
 // block-wide sum of a double over BP_BLOCK threads; `red` is LDS of >= BP_BLOCK/64 doubles.
 __device__ __forceinline__ double block_sum(double v, double *red) {
     v = wave_sum(v);

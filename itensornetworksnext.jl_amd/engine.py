@@ -48,7 +48,8 @@ class RunResult:
 class Engine:
     """Device-resident BP state for one topology.
 
-    dtype: "f64" | "f32" | "bf16" (contraction precision).  device: HIP ordinal.
+    dtype: "f64" | "f32" | "bf16" (contraction precision) | "c128" (complex FP64: sites,
+    messages and scalars are complex128 arrays).  device: HIP ordinal.
     """
 
     def __init__(self, topo: Topology, sites=None, dtype: str = "f64", device: int = 0,
@@ -56,6 +57,7 @@ class Engine:
         self.lib = L.load()
         self.topo = topo
         self.dtype = dtype
+        self.cplx = L.DTYPES[dtype] == L.BP_C128
         self.device = int(device)
         h = C.c_void_p()
         L.check(self.lib.bp_create(C.byref(h), int(device), L.DTYPES[dtype]), "bp_create")
@@ -92,17 +94,34 @@ class Engine:
         self.close()
 
     # -- data
+    def _host(self, a):
+        """A host buffer in the context's element type: float64, or complex128 passed to
+        the library as interleaved (re, im) doubles."""
+        if self.cplx:
+            return np.ascontiguousarray(a, dtype=np.complex128).reshape(-1)
+        if np.iscomplexobj(a):
+            raise TypeError(f"complex data for a {self.dtype} context (use dtype='c128')")
+        return _f64(a).reshape(-1)
+
+    def _zeros(self, n):
+        return np.zeros(n, dtype=np.complex128 if self.cplx else np.float64)
+
+    @staticmethod
+    def _ptr(a):
+        return _dp(a.view(np.float64)) if np.iscomplexobj(a) else _dp(a)
+
     def set_sites(self, sites) -> None:
-        s = _f64(sites)
+        s = self._host(sites)
         if s.size != int(self.topo.site_off[-1]):
             raise ValueError(f"sites has {s.size} entries, topology needs {self.topo.site_off[-1]}")
-        L.check(self.lib.bp_upload_sites(self.h, _dp(s)), "bp_upload_sites")
+        L.check(self.lib.bp_upload_sites(self.h, self._ptr(s)), "bp_upload_sites")
 
     def set_messages(self, msgs) -> None:
-        m = _f64(msgs)
+        m = self._host(msgs)
         if m.size != int(self.topo.msg_off[-1]):
             raise ValueError("message buffer size does not match the topology")
-        L.check(self.lib.bp_init_messages(self.h, L.BP_INIT_UPLOAD, _dp(m)), "bp_init_messages")
+        L.check(self.lib.bp_init_messages(self.h, L.BP_INIT_UPLOAD, self._ptr(m)),
+                "bp_init_messages")
 
     def init_messages(self, kind: str = "identity") -> None:
         code = {"identity": L.BP_INIT_IDENTITY, "one": L.BP_INIT_IDENTITY,
@@ -112,25 +131,25 @@ class Engine:
     def set_slot_messages(self, slots, data) -> None:
         """Write the messages of `slots` (packed FP64, slot-list order)."""
         sl = _i32(slots)
-        d = _f64(data)
+        d = self._host(data)
         need = int(sum(int(self.topo.msg_off[q + 1] - self.topo.msg_off[q]) for q in sl))
         if d.size != need:
             raise ValueError(f"{d.size} values for slots holding {need}")
-        L.check(self.lib.bp_set_slot_messages(self.h, len(sl), _ip(sl), _dp(d)),
+        L.check(self.lib.bp_set_slot_messages(self.h, len(sl), _ip(sl), self._ptr(d)),
                 "bp_set_slot_messages")
 
     def slot_messages(self, slots) -> np.ndarray:
         """The messages of `slots`, packed FP64 in slot-list order."""
         sl = _i32(slots)
         need = int(sum(int(self.topo.msg_off[q + 1] - self.topo.msg_off[q]) for q in sl))
-        out = np.zeros(need)
-        L.check(self.lib.bp_get_slot_messages(self.h, len(sl), _ip(sl), _dp(out)),
+        out = self._zeros(need)
+        L.check(self.lib.bp_get_slot_messages(self.h, len(sl), _ip(sl), self._ptr(out)),
                 "bp_get_slot_messages")
         return out
 
     def messages(self) -> np.ndarray:
-        out = np.zeros(int(self.topo.msg_off[-1]))
-        L.check(self.lib.bp_download_messages(self.h, _dp(out)), "bp_download_messages")
+        out = self._zeros(int(self.topo.msg_off[-1]))
+        L.check(self.lib.bp_download_messages(self.h, self._ptr(out)), "bp_download_messages")
         return out
 
     # -- path
@@ -193,22 +212,22 @@ class Engine:
         return fid, mab
 
     def iterate_diff(self, other) -> float:
-        o = _f64(other)
+        o = self._host(other)
         if o.size != int(self.topo.msg_off[-1]):
             raise ValueError("message buffer size does not match the topology")
         out = C.c_double(0)
-        L.check(self.lib.bp_iterate_diff(self.h, _dp(o), C.byref(out)), "bp_iterate_diff")
+        L.check(self.lib.bp_iterate_diff(self.h, self._ptr(o), C.byref(out)), "bp_iterate_diff")
         return out.value
 
     # -- scalars
     def vertex_scalars(self) -> np.ndarray:
-        out = np.zeros(self.topo.nv)
-        L.check(self.lib.bp_vertex_scalars(self.h, _dp(out)), "bp_vertex_scalars")
+        out = self._zeros(self.topo.nv)
+        L.check(self.lib.bp_vertex_scalars(self.h, self._ptr(out)), "bp_vertex_scalars")
         return out
 
     def edge_scalars(self) -> np.ndarray:
-        out = np.zeros(max(self.topo.ne, 1))
-        L.check(self.lib.bp_edge_scalars(self.h, _dp(out)), "bp_edge_scalars")
+        out = self._zeros(max(self.topo.ne, 1))
+        L.check(self.lib.bp_edge_scalars(self.h, self._ptr(out)), "bp_edge_scalars")
         return out[: self.topo.ne]
 
     def lognorms(self) -> np.ndarray:
@@ -216,6 +235,8 @@ class Engine:
         re = np.zeros(nn)
         im = np.zeros(nn)
         L.check(self.lib.bp_lognorm(self.h, _dp(re), _dp(im)), "bp_lognorm")
+        if self.cplx:  # complex element types give a complex log-norm
+            return re + 1j * im
         return np.where(im != 0.0, re + 1j * im, re) if np.any(im != 0.0) else re
 
     def lognorm(self):
@@ -259,8 +280,8 @@ class Engine:
 
     def sites(self) -> np.ndarray:
         """The FP64 site tensors (site layout), as held on the device."""
-        out = np.zeros(int(self.topo.site_off[-1]))
-        L.check(self.lib.bp_download_sites(self.h, _dp(out)), "bp_download_sites")
+        out = self._zeros(int(self.topo.site_off[-1]))
+        L.check(self.lib.bp_download_sites(self.h, self._ptr(out)), "bp_download_sites")
         return out
 
     def apply_gates(self, kinds, wheres, gates, normalize: bool = False, rtol: float = -1.0,

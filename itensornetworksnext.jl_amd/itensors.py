@@ -339,16 +339,17 @@ def to_topology(factors) -> Flattened:
             raise ValueError("plain BP needs every index shared by two tensors (vertex "
                              f"{v} has unshared indices {si})")
     topo = build_topology(g, dims, site_dims, plain=plain)
-    chunks = []
+    arrs = []
     for ui, u in enumerate(topo.vertices):
         t = tn[u]
         order = list(tn.siteinds(u))
         for q in range(topo.port_off[ui], topo.port_off[ui + 1]):
             w = topo.vertices[topo.port_nbr[q]]
             order += port_inds[(u, w)]
-        arr = t.permute(order)
-        chunks.append(np.ascontiguousarray(arr, dtype=np.float64).reshape(-1))
-    sites = np.concatenate(chunks) if chunks else np.zeros(0)
-    if np.iscomplexobj(sites):
-        raise TypeError("the engine computes with real tensors")
+        arrs.append(t.permute(order))
+    # complex tensors (ComplexF32 promotes, as Julia's scalars do) give a complex128 site
+    # buffer for a complex context; real ones float64
+    ct = np.complex128 if any(np.iscomplexobj(a) for a in arrs) else np.float64
+    chunks = [np.ascontiguousarray(a, dtype=ct).reshape(-1) for a in arrs]
+    sites = np.concatenate(chunks) if chunks else np.zeros(0, dtype=ct)
     return Flattened(topo, sites, port_inds, plain)

@@ -33,8 +33,9 @@ class MessageCache:
 
     The graph starts from the vertices given and the keys of the messages (every key is an
     edge); setting a message on a missing edge adds the edge (`set_edge_data!`, :92-96),
-    `rem_edge_` drops both (:54-58).  Numeric messages are stored as float64 arrays; any
-    other value (the reference's tests store strings) is kept as it is."""
+    `rem_edge_` drops both (:54-58).  Numeric messages are stored as float64 arrays
+    (complex128 when complex); any other value (the reference's tests store strings) is
+    kept as it is."""
 
     def __init__(self, messages=None, vertices=None):
         self.messages: Dict[Edge, object] = {}
@@ -128,9 +129,14 @@ class MessageCache:
     def __repr__(self):
         return f"MessageCache({len(self)} directed edges)"
 
+    def is_complex(self) -> bool:
+        """Whether any message is complex (the cache's element type is then complex)."""
+        return any(np.iscomplexobj(m) for m in self.messages.values())
+
     # -- flat buffers in a topology's slot order
     def to_flat(self, topo) -> np.ndarray:
-        out = np.zeros(int(topo.msg_off[-1]))
+        out = np.zeros(int(topo.msg_off[-1]),
+                       dtype=np.complex128 if self.is_complex() else np.float64)
         own = topo.port_owner()
         for q in range(topo.nports):
             e = (topo.vertices[own[q]], topo.vertices[topo.port_nbr[q]])
@@ -154,9 +160,23 @@ class MessageCache:
 
 
 def _value(m):
-    if isinstance(m, (np.ndarray, list, tuple, float, int, np.number)):
-        return np.asarray(m, dtype=np.float64)
+    if isinstance(m, (np.ndarray, list, tuple, float, int, complex, np.number)):
+        return np.asarray(m, dtype=np.complex128 if np.iscomplexobj(m) else np.float64)
     return m
+
+
+def _scalar(x):
+    """A Python scalar of the element type: complex for complex networks."""
+    return complex(x) if np.iscomplexobj(x) else float(x)
+
+
+def context_dtype(dtype: str, sites, *caches) -> str:
+    """The engine dtype for `dtype` requested on these sites and messages: complex data
+    (complex site tensors or messages) runs in complex FP64 ("c128"), as Julia promotes the
+    arithmetic to the complex element type."""
+    if np.iscomplexobj(sites) or any(c is not None and c.is_complex() for c in caches):
+        return "c128"
+    return dtype
 
 
 def messagecache(f_or_pairs, edges=None) -> MessageCache:
@@ -275,14 +295,16 @@ def _engine(factors, messages, dtype="f64"):
 
     fl = _flat(factors)
     mc = messages if isinstance(messages, MessageCache) else MessageCache(messages)
-    e = Engine(fl.topo, fl.sites, dtype=dtype, path="generic" if dtype == "f64" else "auto")
+    dtype = context_dtype(dtype, fl.sites, mc)
+    e = Engine(fl.topo, fl.sites, dtype=dtype, path="auto" if dtype in ("f32", "bf16") else "generic")
     e.set_messages(mc.to_flat(fl.topo))
     return fl, e
 
 
-def _cache_engine(cache: MessageCache):
+def _cache_engine(cache: MessageCache, *others: MessageCache):
     """An engine over the graph of a cache alone (no factors): zero site tensors, used for
-    the message-only quantities (iterate_diff, edge scalars)."""
+    the message-only quantities (iterate_diff, edge scalars); complex when any of the
+    caches is."""
     from .engine import Engine
 
     vs = cache.vertices()
@@ -298,7 +320,8 @@ def _cache_engine(cache: MessageCache):
             raise ValueError("mixed vector and matrix messages")
         dims[(u, v)] = m.shape[0]
     topo = build_topology(g, dims, 1, plain=bool(plain))
-    e = Engine(topo, np.zeros(int(topo.site_off[-1])), dtype="f64", path="generic")
+    dt = context_dtype("f64", None, cache, *others)
+    e = Engine(topo, np.zeros(int(topo.site_off[-1])), dtype=dt, path="generic")
     return topo, e
 
 
@@ -315,8 +338,9 @@ def vertex_scalars(factors, messages, vertices=None) -> np.ndarray:
     return np.array([z[idx[v]] for v in vertices])
 
 
-def vertex_scalar(factors, messages, vertex) -> float:
-    return float(vertex_scalars(factors, messages, [vertex])[0])
+def vertex_scalar(factors, messages, vertex):
+    """Z_v (messagecache.jl:139-143): a float, or a complex for complex networks."""
+    return _scalar(vertex_scalars(factors, messages, [vertex])[0])
 
 
 def edge_scalars(cache: MessageCache, edges=None) -> np.ndarray:
@@ -339,13 +363,13 @@ def edge_scalars(cache: MessageCache, edges=None) -> np.ndarray:
     return np.array(out)
 
 
-def edge_scalar(cache: MessageCache, edge) -> float:
-    return float(edge_scalars(cache, [edge])[0])
+def edge_scalar(cache: MessageCache, edge):
+    return _scalar(edge_scalars(cache, [edge])[0])
 
 
-def region_scalar(factors, messages, region) -> float:
+def region_scalar(factors, messages, region):
     """The product of the vertex scalars of the region (messagecache.jl:180-182)."""
-    return float(np.prod(vertex_scalars(factors, messages, list(region))))
+    return _scalar(np.prod(vertex_scalars(factors, messages, list(region))))
 
 
 def bethe_free_energy(factors, messages):
@@ -430,8 +454,9 @@ def expect2(factors, messages, ops) -> dict:
 
 
 def iterate_diff(cache1: MessageCache, cache2: MessageCache) -> float:
-    """max over directed edges of 1 - |<m1^, m2^>|^2 (beliefpropagation.jl:261-267)."""
-    topo, e = _cache_engine(cache1)
+    """max over directed edges of 1 - |<m1^, m2^>|^2 (beliefpropagation.jl:261-267), the
+    first argument conjugated for complex messages."""
+    topo, e = _cache_engine(cache1, cache2)
     try:
         e.set_messages(cache1.to_flat(topo))
         return e.iterate_diff(cache2.to_flat(topo))
@@ -447,14 +472,16 @@ def one(template: np.ndarray) -> np.ndarray:
 
 
 def similar_message_environment(nn) -> MessageCache:
-    """A zero message for every in-edge of every vertex (messagecache.jl:205-225)."""
+    """A zero message for every in-edge of every vertex, of the site tensors' element type
+    (messagecache.jl:205-225)."""
     fl = _flat(nn)
     topo = fl.topo
     own = topo.port_owner()
     c = MessageCache(vertices=list(topo.vertices))
+    dt = np.complex128 if np.iscomplexobj(fl.sites) else np.float64
     for q in range(topo.nports):
         e = (topo.vertices[own[q]], topo.vertices[topo.port_nbr[q]])
-        c[e] = np.zeros(topo.msg_shape(q))
+        c[e] = np.zeros(topo.msg_shape(q), dtype=dt)
     return c
 
 
