@@ -33,7 +33,7 @@ import ITensorNetworksNext: message_update!, bethe_free_energy
 
 const libbp = Ref{String}(joinpath(@__DIR__, "..", "csrc", "libbpmi355x.so"))
 
-const BP_F64, BP_F32, BP_BF16 = Cint(0), Cint(1), Cint(2)
+const BP_F64, BP_F32, BP_BF16, BP_C128 = Cint(0), Cint(1), Cint(2), Cint(3)
 const BP_INIT_UPLOAD = Cint(2)
 const BP_CRIT_FIDELITY = Cint(0)
 
@@ -51,11 +51,17 @@ mutable struct DeviceContext
     slots::Dict{Any, UnitRange{Int}}   # directed edge => range in the flat message buffer
     dims::Dict{Any, Int}
     nmsg::Int
+    T::DataType   # element type of the host buffers: Float64, or ComplexF64 (BP_C128)
 end
 
 function DeviceContext(nn::NormNetwork; dtype::Symbol = :f32, device::Integer = 0)
     h = Ref{Ptr{Cvoid}}(C_NULL)
-    dt = dtype === :f64 ? BP_F64 : dtype === :bf16 ? BP_BF16 : BP_F32
+    # complex site tensors (ComplexF32 promotes) run as BP_C128 whatever `dtype` asks:
+    # sites, messages and scalars then cross the ABI as interleaved (re, im) doubles, which
+    # is the memory layout of a Vector{ComplexF64}
+    iscplx = any(v -> eltype(unnamed(kettensor(nn, v))) <: Complex, vertices(nn))
+    T = iscplx ? ComplexF64 : Float64
+    dt = iscplx ? BP_C128 : dtype === :f64 ? BP_F64 : dtype === :bf16 ? BP_BF16 : BP_F32
     check(ccall((:bp_create, libbp[]), Cint, (Ref{Ptr{Cvoid}}, Cint, Cint), h, device, dt),
         "bp_create")
     vs = collect(vertices(nn))
@@ -90,17 +96,17 @@ function DeviceContext(nn::NormNetwork; dtype::Symbol = :f32, device::Integer = 
                 Ptr{Cint}, Ptr{Cint}, Cint, Cint),
             h[], length(vs), length(es), port_off, port_nbr, port_edge, port_rev, edge_dim,
             phys_dim, C_NULL, 1, 0), "bp_set_topology")
-    # site tensors permuted to [site][port 0]...[port k-1], row-major, Float64
-    sites = Float64[]
+    # site tensors permuted to [site][port 0]...[port k-1], row-major, Float64 / ComplexF64
+    sites = T[]
     for v in vs
         t = kettensor(nn, v)
         order = vcat(collect(siteinds(nn.ket, v)),
             reduce(vcat, [collect(linkinds(nn.ket, (v, w))) for w in nbrs[v]]; init = []))
         a = permutedims(unnamed(t, order...), reverse(1:length(order)))  # row-major flatten
-        append!(sites, vec(Array{Float64}(a)))
+        append!(sites, vec(Array{T}(a)))
     end
     GC.@preserve sites check(ccall((:bp_upload_sites, libbp[]), Cint,
-        (Ptr{Cvoid}, Ptr{Float64}), h[], sites), "bp_upload_sites")
+        (Ptr{Cvoid}, Ptr{Cvoid}), h[], sites), "bp_upload_sites")
     slots = Dict{Any, UnitRange{Int}}()
     dims = Dict{Any, Int}()
     off = 0
@@ -110,7 +116,7 @@ function DeviceContext(nn::NormNetwork; dtype::Symbol = :f32, device::Integer = 
         dims[(v, w)] = D
         off += D * D
     end
-    ctx = DeviceContext(h[], nn, vs, slots, dims, off)
+    ctx = DeviceContext(h[], nn, vs, slots, dims, off, T)
     finalizer(c -> ccall((:bp_destroy, libbp[]), Cint, (Ptr{Cvoid},), c.handle), ctx)
     return ctx
 end
@@ -126,29 +132,29 @@ function message_names(nn::NormNetwork, v, w)
     return (k, braname(nn, k))
 end
 
-# the message of edge v => w as a D x D Float64 matrix indexed [ket, bra]
-function ketbra_matrix(nn::NormNetwork, cache, v, w)
+# the message of edge v => w as a D x D matrix of element type T indexed [ket, bra]
+function ketbra_matrix(nn::NormNetwork, cache, v, w, T::DataType = Float64)
     m = cache[v => w]
-    return Array{Float64}(unnamed(m, message_names(nn, v, w)...))
+    return Array{T}(unnamed(m, message_names(nn, v, w)...))
 end
 
 """Messages [ket][bra] as row-major D x D blocks of the flat buffer (the message of
 edge v => w is produced at v and consumed at w)."""
 function upload!(ctx::DeviceContext, cache)
-    flat = zeros(Float64, ctx.nmsg)
+    flat = zeros(ctx.T, ctx.nmsg)
     for (e, r) in ctx.slots
-        m = ketbra_matrix(ctx.nn, cache, e[1], e[2])
+        m = ketbra_matrix(ctx.nn, cache, e[1], e[2], ctx.T)
         flat[r] = vec(permutedims(m, (2, 1)))  # row-major [ket][bra]
     end
     GC.@preserve flat check(ccall((:bp_init_messages, libbp[]), Cint,
-        (Ptr{Cvoid}, Cint, Ptr{Float64}), ctx.handle, BP_INIT_UPLOAD, flat), "bp_init_messages")
+        (Ptr{Cvoid}, Cint, Ptr{Cvoid}), ctx.handle, BP_INIT_UPLOAD, flat), "bp_init_messages")
     return ctx
 end
 
 function download!(cache, ctx::DeviceContext)
-    flat = zeros(Float64, ctx.nmsg)
+    flat = zeros(ctx.T, ctx.nmsg)
     GC.@preserve flat check(ccall((:bp_download_messages, libbp[]), Cint,
-        (Ptr{Cvoid}, Ptr{Float64}), ctx.handle, flat), "bp_download_messages")
+        (Ptr{Cvoid}, Ptr{Cvoid}), ctx.handle, flat), "bp_download_messages")
     for (e, r) in ctx.slots
         D = ctx.dims[e]
         msg = cache[e[1] => e[2]]
@@ -195,14 +201,14 @@ struct DeviceMessageCache
 end
 
 """A host snapshot of a device cache's messages (flat, the engine's slot layout)."""
-struct DeviceSnapshot
-    flat::Vector{Float64}
+struct DeviceSnapshot{T}
+    flat::Vector{T}
 end
 
 function Base.copy(c::DeviceMessageCache)
-    flat = zeros(Float64, c.ctx.nmsg)
+    flat = zeros(c.ctx.T, c.ctx.nmsg)
     GC.@preserve flat check(ccall((:bp_download_messages, libbp[]), Cint,
-        (Ptr{Cvoid}, Ptr{Float64}), c.ctx.handle, flat), "bp_download_messages")
+        (Ptr{Cvoid}, Ptr{Cvoid}), c.ctx.handle, flat), "bp_download_messages")
     return DeviceSnapshot(flat)
 end
 
@@ -210,7 +216,7 @@ end
 function AIE.iterate_diff(c::DeviceMessageCache, prev::DeviceSnapshot)
     out = Ref{Float64}(0.0)
     GC.@preserve prev check(ccall((:bp_iterate_diff, libbp[]), Cint,
-        (Ptr{Cvoid}, Ptr{Float64}, Ref{Float64}), c.ctx.handle, prev.flat, out),
+        (Ptr{Cvoid}, Ptr{Cvoid}, Ref{Float64}), c.ctx.handle, prev.flat, out),
         "bp_iterate_diff")
     return out[]
 end
@@ -244,7 +250,7 @@ function device_beliefpropagation(nn::NormNetwork, messages; stopping_criterion,
     check(ccall((:bp_run, libbp[]), Cint,
             (Ptr{Cvoid}, Cint, Cdouble, Cint, Cint, Ref{Cint}, Ref{Float64}, Ref{Cint}),
             ctx.handle, maxiter, tol, BP_CRIT_FIDELITY, 1, sweeps, delta, at), "bp_run")
-    if alg.polish > 0
+    if alg.polish > 0 && !(ctx.T <: Complex)  # complex contexts compute in FP64 already
         check(ccall((:bp_polish_fp64, libbp[]), Cint, (Ptr{Cvoid}, Cint, Ptr{Float64}),
             ctx.handle, alg.polish, C_NULL), "bp_polish_fp64")
     end
@@ -258,7 +264,8 @@ function device_lognorm(ctx::DeviceContext)
     im = Ref{Float64}(0.0)
     check(ccall((:bp_lognorm, libbp[]), Cint, (Ptr{Cvoid}, Ref{Float64}, Ref{Float64}),
         ctx.handle, re, im), "bp_lognorm")
-    return im[] == 0 ? re[] : complex(re[], im[])
+    # complex element types give a complex log-norm, as sum(log.(terms)) does
+    return (ctx.T <: Complex || im[] != 0) ? complex(re[], im[]) : re[]
 end
 
 """Gradient of the BP log-norm with respect to every site tensor (flat, site layout)."""
