@@ -27,6 +27,7 @@
 #include "vertex.h"
 #include "vertex16.h"
 #include "vertex64.h"
+#include "vertex4.h"
 #include "adjoint.h"
 #include "vertex_adjoint.h"
 #include "contract.h"
@@ -317,6 +318,8 @@ struct bp_context {
                                 // 1: one buffer per wave, two waves per SIMD; -1: the faster
                                 // one at 32x32 D=8: 1 in FP32 with d >= 2, else 0)
     int v16_waves = 16;         // waves per slice workgroup of the bond-dimension-16 kernel
+    int vx_small = 1;           // option "small_bond": FP32 networks of bond dimension <= 4 on
+                                // the small-bond kernels (vertex4.h), not padded to 8
 
     // FP64 D <= 8 vertex kernel (vertex64.h): the sweeps of FP64 contexts and the FP64
     // polish of every context.  Built on first use for the current topology and mask.
@@ -823,6 +826,7 @@ static const float *generic_sites32(const bp_context *c) {
 // alone updates every message of every network.
 static bool fold_possible(const bp_context *c) {
     return c->vplan_ok && c->vplan.all && c->vplan.nv_k3 == 0 && c->vplan.nv16 == 0 &&
+           c->vplan.ns_total() == 0 &&
            c->path != BP_PATH_GENERIC && c->d_fold_acc && c->fold_n == c->nnet &&
            c->vplan.nv > 0 && c->vplan.d_lead && c->fold_on;
 }
@@ -865,6 +869,9 @@ static int sweep_once(bp_context *c, const NetCtl &done, bool *folded = nullptr,
                                c->d_res_fid, c->d_res_abs));
         RETCHK(vertex16_sweep(&c->vplan, c->stream, c->topo(), src, dst, done, c->d_res_fid,
                               c->d_res_abs));
+        if (vertex_s_sweep(&c->vplan, c->stream, c->topo(), src, dst, done, c->d_res_fid,
+                           c->d_res_abs) != 0)
+            return set_err(BP_ERR_HIP, "small-bond sweep launch failed");
         HIPCHK(hipGetLastError());
     }
     if (path != BP_PATH_VERTEX) {
@@ -891,7 +898,9 @@ static int upload_vertex_sites(bp_context *c) {
         vertex_k3_upload_sites(&c->vplan, c->stream, c->d_sites64, c->d_site_off,
                                c->d_port_off, c->d_port_edge, c->d_edge_dim) != BP_OK ||
         vertex16_upload_sites(&c->vplan, c->stream, c->d_sites64, c->d_site_off, c->d_port_off,
-                              c->d_port_edge, c->d_edge_dim) != BP_OK)
+                              c->d_port_edge, c->d_edge_dim) != BP_OK ||
+        vertex_s_upload_sites(&c->vplan, c->stream, c->d_sites64, c->d_site_off,
+                              c->d_port_off, c->d_port_edge, c->d_edge_dim) != 0)
         return set_err(BP_ERR_HIP, "vertex-path site layout failed");
     HIPCHK(hipStreamSynchronize(c->stream));
     return BP_OK;
@@ -912,7 +921,7 @@ static int rebuild_vertex_plan(bp_context *c) {
     c->vplan_ok = vertex_plan_build(&c->vplan, c->dtype, c->plain, c->nv, c->port_off.data(),
                                     c->port_edge.data(), c->edge_dim.data(), c->phys_dim.data(),
                                     c->site_off.data(), c->msg_off.data(), c->port_rev.data(),
-                                    c->vnet.data(), c->vx_mask) == BP_OK;
+                                    c->vnet.data(), c->vx_mask, c->vx_small) == BP_OK;
     if (c->vplan_ok && (vertex_set_lds_limits(&c->vplan) != BP_OK ||
                         (c->vplan.nv16 > 0 && vertex16_set_lds_limit() != BP_OK))) {
         c->vplan_ok = false;
@@ -1364,6 +1373,12 @@ int bp_set_option(bp_context *c, const char *name, int value) {
         if (c->have_topo) RETCHK(rebuild_vertex_plan(c));
         return BP_OK;
     }
+    if (strcmp(name, "small_bond") == 0) {
+        if (value != 0 && value != 1) return set_err(BP_ERR_ARG, "small_bond is 0 or 1");
+        c->vx_small = value;
+        if (c->have_topo) RETCHK(rebuild_vertex_plan(c));
+        return BP_OK;
+    }
     return set_err(BP_ERR_ARG, "unknown option %s", name);
 }
 
@@ -1374,6 +1389,7 @@ int bp_get_option(bp_context *c, const char *name, int *value) {
     if (strcmp(name, "vertex_bond_mask") == 0) { *value = (int)c->vx_mask; return BP_OK; }
     if (strcmp(name, "fold_bookkeeping") == 0) { *value = c->fold_on; return BP_OK; }
     if (strcmp(name, "normalize") == 0) { *value = c->normalize; return BP_OK; }
+    if (strcmp(name, "small_bond") == 0) { *value = c->vx_small; return BP_OK; }
     return set_err(BP_ERR_ARG, "unknown option %s", name);
 }
 
@@ -1899,7 +1915,8 @@ static const void *current_msgs(bp_context *c, bool *is64) {
 // generic kernels otherwise, and always for FP64 messages or a forced generic path).
 static bool vertex_adjoint_ok(const bp_context *c, bool is64) {
     return !is64 && c->dtype == BP_F32 && c->vplan_ok && c->vplan.all && !c->vplan.bf16 &&
-           c->vplan.nv16 == 0 && c->path != BP_PATH_GENERIC && c->d_vsites != nullptr;
+           c->vplan.nv16 == 0 && c->vplan.ns_total() == 0 && c->path != BP_PATH_GENERIC &&
+           c->d_vsites != nullptr;
 }
 
 // per-slot distances of the current messages against `other` (device buffer) into

@@ -26,19 +26,20 @@ __device__ void block_mode_product(const T *in, T *out, const int *dims, int r, 
     for (int i = ax + 1; i < r; ++i) post *= dims[i];
     const int na = dims[ax];
     const long long ncol = pre * post;
+    constexpr int NB = sizeof(T) > 8 ? 8 : 16;  // outputs per thread (complex: 8, no spills)
     for (long long c = threadIdx.x; c < ncol; c += blockDim.x) {
         const long long p = c / post, t = c - p * post;
         const T *src = in + p * na * post + t;
         T *dst = out + p * (long long)nb * post + t;
-        for (int b0 = 0; b0 < nb; b0 += 16) {
-            T acc[16];
+        for (int b0 = 0; b0 < nb; b0 += NB) {
+            T acc[NB];
 #pragma unroll
-            for (int b = 0; b < 16; ++b) acc[b] = T(0);
+            for (int b = 0; b < NB; ++b) acc[b] = T(0);
             if (trans) {
                 for (int a = 0; a < na; ++a) {
                     const T x = src[(long long)a * post];
 #pragma unroll
-                    for (int b = 0; b < 16; ++b)
+                    for (int b = 0; b < NB; ++b)
                         if (b0 + b < nb) acc[b] += x * m[(long long)(b0 + b) * na + a];
                 }
             } else {
@@ -46,12 +47,12 @@ __device__ void block_mode_product(const T *in, T *out, const int *dims, int r, 
                     const T x = src[(long long)a * post];
                     const T *mr = m + (long long)a * nb + b0;
 #pragma unroll
-                    for (int b = 0; b < 16; ++b)
+                    for (int b = 0; b < NB; ++b)
                         if (b0 + b < nb) acc[b] += x * mr[b];
                 }
             }
 #pragma unroll
-            for (int b = 0; b < 16; ++b)
+            for (int b = 0; b < NB; ++b)
                 if (b0 + b < nb) dst[(long long)(b0 + b) * post] = acc[b];
         }
     }

@@ -85,6 +85,13 @@ struct VertexPlan {
     int v16_waves = 16;                    // waves per slice workgroup: 4, 8 or 16
     float *d_vsites_y = nullptr;           // FP32: the padded slices in layout Y order (the
                                            // bra rows of k_vertex_d8_mo<NS, true>)
+    // FP32 networks whose bond dimensions are all <= 4: k_vertex_s<DP> (vertex4.h), one
+    // bucket per padded dimension DP = 2, 3, 4 (index DP - 2)
+    int ns[3] = {0, 0, 0};
+    int4 *d_vs_in[3] = {}, *d_vs_out[3] = {}, *d_vs_dim[3] = {};
+    int *d_vs_net[3] = {}, *d_vs_list[3] = {};
+    float *d_vs_sites[3] = {};
+    int ns_total() const { return ns[0] + ns[1] + ns[2]; }
 };
 
 static constexpr int VX_D = 8;
@@ -112,6 +119,16 @@ static void vertex_plan_release(VertexPlan *p) {
     p->nv16 = 0;
     if (p->d_vsites_y) (void)hipFree(p->d_vsites_y);
     p->d_vsites_y = nullptr;
+    for (int k = 0; k < 3; ++k) {
+        void *sm[6] = {p->d_vs_in[k], p->d_vs_out[k], p->d_vs_dim[k], p->d_vs_net[k],
+                       p->d_vs_list[k], p->d_vs_sites[k]};
+        for (void *x : sm)
+            if (x) (void)hipFree(x);
+        p->d_vs_in[k] = p->d_vs_out[k] = p->d_vs_dim[k] = nullptr;
+        p->d_vs_net[k] = p->d_vs_list[k] = nullptr;
+        p->d_vs_sites[k] = nullptr;
+        p->ns[k] = 0;
+    }
     if (p->d_vnet) (void)hipFree(p->d_vnet);
     if (p->d_lead) (void)hipFree(p->d_lead);
     p->d_lead = nullptr;
@@ -131,10 +148,13 @@ static int vx_upload(T **dst, const std::vector<T> &v) {
     return hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice) == hipSuccess ? 0 : 2;
 }
 
+// small != 0: FP32 networks whose bond dimensions are all <= 4 go to the small-bond
+// buckets (vertex4.h) instead of the padded D <= 8 kernels.
 static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const int *port_off,
                              const int *port_edge, const int *edge_dim, const int *phys_dim,
                              const long long *site_off, const long long *msg_off,
-                             const int *port_rev, const int *vnet, unsigned mask) {
+                             const int *port_rev, const int *vnet, unsigned mask,
+                             int small = 0) {
     (void)site_off;
     (void)msg_off;
     p->why.clear();
@@ -145,7 +165,7 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     // bucket
     const int dlim = dtype == 2 ? 16 : 12;
     int d = -1;
-    std::vector<int> vlist, vlist16, gen_slots, vlist_k3;
+    std::vector<int> vlist, vlist16, gen_slots, vlist_k3, vlist_s[3];
     p->mask = mask;
     std::string last_reason = "no network has every bond dimension <= 8";
     for (int u0 = 0; u0 < nv;) {
@@ -176,8 +196,11 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
         if (ok && d < 0) d = phys_dim[u0];
         // FP32 networks of degree <= 3 throughout: their own kernel, at the bucket's end
         const bool k3 = ok && dtype == 1 && dmax <= VX_D && kmax <= 3;
+        // FP32 networks of bond dimension <= 4: the small-bond buckets, by padded dimension
+        const int sk = (ok && small && dtype == 1 && dmax <= 4) ? std::max(dmax, 2) - 2 : -1;
         for (int u = u0; u < u1; ++u) {
-            if (k3) vlist_k3.push_back(u);
+            if (sk >= 0) vlist_s[sk].push_back(u);
+            else if (k3) vlist_k3.push_back(u);
             else if (ok && dmax <= VX_D) vlist.push_back(u);
             else if (ok) vlist16.push_back(u);
             else for (int q = port_off[u]; q < port_off[u + 1]; ++q) gen_slots.push_back(q);
@@ -186,7 +209,11 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     }
     const int nk3 = (int)vlist_k3.size();
     vlist.insert(vlist.end(), vlist_k3.begin(), vlist_k3.end());
-    if (vlist.empty() && vlist16.empty()) { p->why = last_reason; return 4; }
+    if (vlist.empty() && vlist16.empty() && vlist_s[0].empty() && vlist_s[1].empty() &&
+        vlist_s[2].empty()) {
+        p->why = last_reason;
+        return 4;
+    }
     const int nx = (int)vlist.size();
     std::vector<int4> vin(nx), vout(nx), vdim(nx);
     std::vector<int> vn(nx);
@@ -253,6 +280,30 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     }
     if (!rc) rc = vx_upload(&p->d_vlist, vlist);
     if (!rc) rc = vx_upload(&p->d_gen_slots, gen_slots);
+    for (int k = 0; k < 3 && !rc; ++k) {  // the small-bond buckets
+        const int n = (int)vlist_s[k].size();
+        std::vector<int4> si(n), so(n), sd(n);
+        std::vector<int> sn(n);
+        for (int b = 0; b < n; ++b) {
+            const int u = vlist_s[k][b];
+            int in[4] = {-1, -1, -1, -1}, out[4] = {-1, -1, -1, -1}, dm[4] = {0, 0, 0, 0};
+            for (int j = 0; j < port_off[u + 1] - port_off[u]; ++j) {
+                out[j] = port_off[u] + j;
+                in[j] = port_rev[port_off[u] + j];
+                dm[j] = edge_dim[port_edge[port_off[u] + j]];
+            }
+            si[b] = make_int4(in[0], in[1], in[2], in[3]);
+            so[b] = make_int4(out[0], out[1], out[2], out[3]);
+            sd[b] = make_int4(dm[0], dm[1], dm[2], dm[3]);
+            sn[b] = vnet[u];
+        }
+        rc = vx_upload(&p->d_vs_in[k], si);
+        if (!rc) rc = vx_upload(&p->d_vs_out[k], so);
+        if (!rc) rc = vx_upload(&p->d_vs_dim[k], sd);
+        if (!rc) rc = vx_upload(&p->d_vs_net[k], sn);
+        if (!rc) rc = vx_upload(&p->d_vs_list[k], vlist_s[k]);
+        p->ns[k] = n;
+    }
     if (rc) {
         p->why = "allocation failed";
         vertex_plan_release(p);
