@@ -48,6 +48,7 @@ PEAK = {  # MI355X published dense peaks
     "hbm_GBps": 8000.0,
     "fp32_mfma_TFLOPs": 157.3,
     "fp64_TFLOPs": 78.6,
+    "fp32_valu_TFLOPs": 78.6,  # non-packed FP32 FMA on the vector units
     "bf16_mfma_TFLOPs": 2500.0,
 }
 
@@ -400,6 +401,42 @@ def large_bond_timing(pkg, torch, seed, L=24, D=16):
     return out
 
 
+def small_bond_timing(pkg, seed, L=256, Ds=(2, 3, 4), d=2):
+    """Bond dimensions below 8: the small-bond kernel (k_vertex_s, VALU) on an L x L FP32
+    lattice per D, against the same sweep zero-padded to 8 on the MFMA kernel and on the
+    generic kernel.  Algorithmic bytes: FP32 sites read once, messages read and written once;
+    algorithmic FLOPs: sum over directed messages of 2 k d D^(k+1)."""
+    out = []
+    for D in Ds:
+        t = pkg.build_topology(pkg.named_grid((L, L)), D, d)
+        s = pkg.peps.random_sites(t, seed, scale=pkg.peps.peps_scale(d, D))
+        flops = algorithmic_flops_per_sweep(t)
+        nbytes = algorithmic_bytes_per_sweep(t, 4)
+        row = {"workload": f"{L}x{L} PEPS norm network, d={d}, D={D}, FP32 Jacobi sweep",
+               "messages_per_sweep": int(t.nports), "algorithmic_bytes": nbytes,
+               "algorithmic_flops": flops}
+        for name, opts, path, reps in (("small_bond", {"small_bond": 1}, "auto", 50),
+                                       ("padded_to_8", {"small_bond": 0}, "auto", 5),
+                                       ("generic", {}, "generic", 5)):
+            with pkg.Engine(t, s, dtype="f32", path=path) as e:
+                for k, v in opts.items():
+                    e.set_option(k, v)
+                e.init_messages("identity")
+                ms = e.time_sweeps(reps)
+            row[f"{name}_kernel_ms"] = ms
+            if name == "small_bond":
+                gbs = nbytes / (ms * 1e-3) / 1e9
+                row.update({"path": path, "achieved_GBs": gbs,
+                            "frac_hbm_peak": gbs / PEAK["hbm_GBps"],
+                            "algorithmic_TFLOPs": flops / (ms * 1e-3) / 1e12,
+                            "frac_fp32_valu_peak": flops / (ms * 1e-3) / 1e12 / PEAK["fp32_valu_TFLOPs"],
+                            "msg_per_s": t.nports / (ms * 1e-3)})
+        row["speedup_vs_padded"] = row["padded_to_8_kernel_ms"] / row["small_bond_kernel_ms"]
+        row["speedup_vs_generic"] = row["generic_kernel_ms"] / row["small_bond_kernel_ms"]
+        out.append(row)
+    return out
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -482,14 +519,15 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--only", choices=["adjoint", "large_bond", "uneven", "local_energy",
-                                       "contract", "gates"],
+                                       "contract", "gates", "small_bond"],
                     default=None,
                     help="adjoint: run only the forward + adjoint workload (L x L, D; three "
                          "energy + gradient calls); large_bond: only the 24x24 D=16 BF16 + "
                          "FP64-polish workload; uneven: only the 64-network batched "
                          "workload; local_energy: only the expectation-value / local-energy "
                          "timings; contract: only the exact-contraction timings; gates: only "
-                         "the gate-layer timing; print the timings, for profiling")
+                         "the gate-layer timing; small_bond: only the bond-dimension <= 4 "
+                         "timings; print the timings, for profiling")
     ap.add_argument("--vertex-variant", type=int, choices=[0, 1, 2, 3], default=None,
                     help="FP32 vertex sweep kernel to run instead of the tuned one (0: two LDS "
                          "buffers per wave, 1: one buffer and two waves per SIMD, 2: MFMA open "
@@ -532,6 +570,10 @@ def main(argv=None):
         torch.cuda.init()
         print(json.dumps({"uneven_only": uneven_batch_timing(pkg, torch, args.seed)}),
               flush=True)
+        return 0
+    if args.only == "small_bond":
+        torch.cuda.init()
+        print(json.dumps({"small_bond_only": small_bond_timing(pkg, args.seed)}), flush=True)
         return 0
     if args.only == "large_bond":
         torch.cuda.init()
@@ -668,6 +710,7 @@ def main(argv=None):
     if not args.no_extras and args.dtype == "f32":
         extras["uneven_batch"] = uneven_batch_timing(pkg, torch, args.seed)
         extras["large_bond_bf16"] = large_bond_timing(pkg, torch, args.seed)
+        extras["small_bond"] = small_bond_timing(pkg, args.seed)
         extras["local_energy"] = local_energy_timing(pkg, torch, args.seed)
         extras["contract_network"] = contract_timing(pkg, args.seed)
         extras["gate_layer"] = gate_timing(pkg, args.seed)
