@@ -88,7 +88,9 @@ struct VertexPlan {
     // FP32 networks whose bond dimensions are all <= 4: k_vertex_s<DP> (vertex4.h), one
     // bucket per padded dimension DP = 2, 3, 4 (index DP - 2)
     int ns[3] = {0, 0, 0};
-    int4 *d_vs_in[3] = {}, *d_vs_out[3] = {}, *d_vs_dim[3] = {};
+    long long *d_vs_ino[3] = {};   // per item and port: offset of the incoming message (-1: padded port)
+    long long *d_vs_outo[3] = {};  // per item and port: offset of the outgoing message
+    int *d_vs_out[3] = {}, *d_vs_dim[3] = {};  // per item and port: outgoing slot (-1), bond dim
     int *d_vs_net[3] = {}, *d_vs_list[3] = {};
     float *d_vs_sites[3] = {};
     int ns_total() const { return ns[0] + ns[1] + ns[2]; }
@@ -120,11 +122,12 @@ static void vertex_plan_release(VertexPlan *p) {
     if (p->d_vsites_y) (void)hipFree(p->d_vsites_y);
     p->d_vsites_y = nullptr;
     for (int k = 0; k < 3; ++k) {
-        void *sm[6] = {p->d_vs_in[k], p->d_vs_out[k], p->d_vs_dim[k], p->d_vs_net[k],
-                       p->d_vs_list[k], p->d_vs_sites[k]};
+        void *sm[7] = {p->d_vs_ino[k], p->d_vs_outo[k], p->d_vs_out[k], p->d_vs_dim[k],
+                       p->d_vs_net[k], p->d_vs_list[k], p->d_vs_sites[k]};
         for (void *x : sm)
             if (x) (void)hipFree(x);
-        p->d_vs_in[k] = p->d_vs_out[k] = p->d_vs_dim[k] = nullptr;
+        p->d_vs_ino[k] = p->d_vs_outo[k] = nullptr;
+        p->d_vs_out[k] = p->d_vs_dim[k] = nullptr;
         p->d_vs_net[k] = p->d_vs_list[k] = nullptr;
         p->d_vs_sites[k] = nullptr;
         p->ns[k] = 0;
@@ -156,7 +159,6 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
                              const int *port_rev, const int *vnet, unsigned mask,
                              int small = 0) {
     (void)site_off;
-    (void)msg_off;
     p->why.clear();
     if (plain) { p->why = "plain (vector-message) networks use the generic path"; return 4; }
     if (dtype != 1 && dtype != 2) { p->why = "the vertex path computes in FP32 or BF16"; return 4; }
@@ -282,22 +284,21 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     if (!rc) rc = vx_upload(&p->d_gen_slots, gen_slots);
     for (int k = 0; k < 3 && !rc; ++k) {  // the small-bond buckets
         const int n = (int)vlist_s[k].size();
-        std::vector<int4> si(n), so(n), sd(n);
-        std::vector<int> sn(n);
+        std::vector<long long> sio(4 * (size_t)n, -1), soo(4 * (size_t)n, 0);
+        std::vector<int> so(4 * (size_t)n, -1), sd(4 * (size_t)n, 0), sn(n);
         for (int b = 0; b < n; ++b) {
             const int u = vlist_s[k][b];
-            int in[4] = {-1, -1, -1, -1}, out[4] = {-1, -1, -1, -1}, dm[4] = {0, 0, 0, 0};
             for (int j = 0; j < port_off[u + 1] - port_off[u]; ++j) {
-                out[j] = port_off[u] + j;
-                in[j] = port_rev[port_off[u] + j];
-                dm[j] = edge_dim[port_edge[port_off[u] + j]];
+                const int q = port_off[u] + j;
+                so[4 * b + j] = q;
+                soo[4 * b + j] = msg_off[q];
+                sio[4 * b + j] = msg_off[port_rev[q]];
+                sd[4 * b + j] = edge_dim[port_edge[q]];
             }
-            si[b] = make_int4(in[0], in[1], in[2], in[3]);
-            so[b] = make_int4(out[0], out[1], out[2], out[3]);
-            sd[b] = make_int4(dm[0], dm[1], dm[2], dm[3]);
             sn[b] = vnet[u];
         }
-        rc = vx_upload(&p->d_vs_in[k], si);
+        rc = vx_upload(&p->d_vs_ino[k], sio);
+        if (!rc) rc = vx_upload(&p->d_vs_outo[k], soo);
         if (!rc) rc = vx_upload(&p->d_vs_out[k], so);
         if (!rc) rc = vx_upload(&p->d_vs_dim[k], sd);
         if (!rc) rc = vx_upload(&p->d_vs_net[k], sn);

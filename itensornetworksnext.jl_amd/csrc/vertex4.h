@@ -31,16 +31,6 @@ struct VsShape {
     static constexpr int M2 = DP * DP;            // entries of a padded message
 };
 
-__device__ __forceinline__ float quad_bcast(float v, int p) {
-    // quad_perm(p, p, p, p): every lane of the quad reads lane p of the quad
-    switch (p) {
-        case 0: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x00, 0xF, 0xF, false));
-        case 1: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x55, 0xF, 0xF, false));
-        case 2: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xAA, 0xF, 0xF, false));
-        default: return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xFF, 0xF, 0xF, false));
-    }
-}
-
 // sum over the 4 lanes of the quad (quad_perm xor 1, then xor 2): every lane gets it
 __device__ __forceinline__ float quad_sum(float v) {
     v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
@@ -48,41 +38,44 @@ __device__ __forceinline__ float quad_sum(float v) {
     return v;
 }
 
-// In-lane mode product on port P (1, 2 or 3), in place: X[.., b, ..] = sum_p X[.., p, ..] m[p][b]
+// v_mfma_f32_4x4x1f32 works on 16 blocks of 4 lanes, one block per quad here: register i of
+// lane j of a quad accumulates a(lane i) * b(lane j).
+
+// In-lane mode product on port P (1, 2 or 3), in place, on the matrix cores:
+// X[.., b, ..] = sum_p X[.., p, ..] m[p][b].  With a = mc[p] = m[p][lane j] and b = the lane's
+// own X[.., p, ..], register i of lane j collects sum_p m[p][i] X_j[.., p, ..].
 template <int DP, int P>
-__device__ __forceinline__ void vs_mode_in(float (&X)[VsShape<DP>::RP], const float *m) {
-    float mm[DP * DP];
-#pragma unroll
-    for (int e = 0; e < DP * DP; ++e) mm[e] = m[e];
+__device__ __forceinline__ void vs_mode_in(float (&X)[VsShape<DP>::RP], const float (&mc)[DP]) {
     constexpr int st = P == 1 ? DP * DP : P == 2 ? DP : 1;  // register stride of port P
 #pragma unroll
     for (int o = 0; o < DP * DP; ++o) {
-        // the other two in-lane digits: o = (hi, lo) in the order of the remaining ports
-        const int hi = o / DP, lo = o % DP;
+        const int hi = o / DP, lo = o % DP;  // the other two in-lane digits
         const int base = P == 1 ? hi * DP + lo : P == 2 ? hi * DP * DP + lo : (hi * DP + lo) * DP;
-        float v[DP];
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int p = 0; p < DP; ++p) v[p] = X[base + p * st];
+        for (int p = 0; p < DP; ++p)
+            acc = __builtin_amdgcn_mfma_f32_4x4x1f32(mc[p], X[base + p * st], acc, 0, 0, 0);
 #pragma unroll
-        for (int b = 0; b < DP; ++b) {
-            float acc = 0.f;
-#pragma unroll
-            for (int p = 0; p < DP; ++p) acc = fmaf(v[p], mm[p * DP + b], acc);
-            X[base + b * st] = acc;
-        }
+        for (int i = 0; i < DP; ++i) X[base + i * st] = acc[i];
     }
 }
 
-// Cross-lane mode product on port 0, in place: lane b of the quad gets
-// X_b[r] = sum_p X_p[r] m0[p][b] (c[p] = m0[p][this lane's b], zero for lanes >= DP).
+// The value x of every lane of the quad (a = x, b = 1): g[i] = x of lane i.
+__device__ __forceinline__ v4f quad_gather(float x) {
+    const v4f z = {0.f, 0.f, 0.f, 0.f};
+    return __builtin_amdgcn_mfma_f32_4x4x1f32(x, 1.f, z, 0, 0, 0);
+}
+
+// Port-0 mode product, in place: lane b of the quad gets X_b[r] = sum_p X_p[r] m0[p][b]
+// (c[p] = m0[p][this lane's b], zero for lanes >= DP): a gather, then DP FMAs.
 template <int DP>
 __device__ __forceinline__ void vs_mode_0(float (&X)[VsShape<DP>::RP], const float (&c)[DP]) {
 #pragma unroll
     for (int r = 0; r < VsShape<DP>::R; ++r) {
-        const float x = X[r];
+        const v4f gx = quad_gather(X[r]);
         float acc = 0.f;
 #pragma unroll
-        for (int p = 0; p < DP; ++p) acc = fmaf(quad_bcast(x, p), c[p], acc);
+        for (int p = 0; p < DP; ++p) acc = fmaf(gx[p], c[p], acc);
         X[r] = acc;
     }
 }
@@ -90,9 +83,9 @@ __device__ __forceinline__ void vs_mode_0(float (&X)[VsShape<DP>::RP], const flo
 template <int DP>
 __global__ __launch_bounds__(64) void k_vertex_s(
     const float *__restrict__ vs_sites, const float *__restrict__ mold, float *__restrict__ mnew,
-    const int4 *__restrict__ vin, const int4 *__restrict__ vout, const int4 *__restrict__ vdim,
-    const int *__restrict__ vnet, const long long *__restrict__ msg_off, NetCtl ctl,
-    double *__restrict__ res_fid, double *__restrict__ res_abs, int nitems, int d) {
+    const long long *__restrict__ vs_ino, const long long *__restrict__ vs_outo,
+    const int *__restrict__ vs_out, const int *__restrict__ vs_dim, const int *__restrict__ vnet,
+    NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs, int nitems, int d) {
     using Shp = VsShape<DP>;
     constexpr int R = Shp::R, RP = Shp::RP, M2 = Shp::M2;
     __shared__ float msm[16 * 4 * M2];   // [vi][port][p * DP + b], padded incoming messages
@@ -101,51 +94,22 @@ __global__ __launch_bounds__(64) void k_vertex_s(
     const int quad = lane >> 2, j = lane & 3;
     const int W = 16 / d;
 
-    // incoming messages: lane (vi2, port) loads one, zero-padded to DP x DP
-    {
-        const int vi2 = lane >> 2, port = lane & 3, b2 = g * W + vi2;
-        float *dst = msm + (vi2 * 4 + port) * M2;
-        int q = -2, D = 0;
-        if (vi2 < W && b2 < nitems) {
-            const int4 i4 = vin[b2], d4 = vdim[b2];
-            q = port == 0 ? i4.x : port == 1 ? i4.y : port == 2 ? i4.z : i4.w;
-            D = port == 0 ? d4.x : port == 1 ? d4.y : port == 2 ? d4.z : d4.w;
-        }
-        const long long o = q >= 0 ? msg_off[q] : 0;
-#pragma unroll
-        for (int e = 0; e < M2; ++e) {
-            const int p = e / DP, b = e % DP;
-            float v = 0.f;
-            if (q == -1) v = (p == 0 && b == 0) ? 1.f : 0.f;  // padded port
-            else if (q >= 0 && p < D && b < D) v = mold[o + p * D + b];
-            dst[e] = v;
-        }
+    // every independent load first: this lane's message (vertex vi2, port), its slice
+    const int vi2 = lane >> 2, port = lane & 3, b2 = g * W + vi2;
+    const bool mine = vi2 < W && b2 < nitems;
+    long long ino = -2, outo = 0;
+    int qout = -1, Dm = 0, net2 = 0;
+    if (mine) {
+        ino = vs_ino[4 * b2 + port];
+        outo = vs_outo[4 * b2 + port];
+        qout = vs_out[4 * b2 + port];
+        Dm = vs_dim[4 * b2 + port];
+        net2 = vnet[b2];
     }
-    __syncthreads();
-
     const int vi = quad / d, b = g * W + vi;
-    const bool live = vi < W && b < nitems && !(ctl.enabled && ctl.done[vnet[b]]);
-    float *pq = part + quad * 4 * M2;
-    // one open's in-lane partial sums: summed over the quad (port 0), stored to part
-    auto store = [&](float (&o)[M2], int port) {
-#pragma unroll
-        for (int e = 0; e < M2; ++e) o[e] = quad_sum(o[e]);
-#pragma unroll
-        for (int e = 0; e < M2; ++e)
-            if ((e & 3) == j) pq[port * M2 + e] = o[e];
-    };
-    float o[M2];
-    if (!live) {  // uniform over the quad; an idle or stopped slice contributes zeros
-#pragma unroll
-        for (int e = 0; e < M2; ++e) o[e] = 0.f;
-        store(o, 0);
-        store(o, 1);
-        store(o, 2);
-        store(o, 3);
-    } else {  // uniform over the quad (DPP reads stay inside it)
-        const float *m0 = msm + (vi * 4 + 0) * M2, *m1 = msm + (vi * 4 + 1) * M2;
-        const float *m2 = msm + (vi * 4 + 2) * M2, *m3 = msm + (vi * 4 + 3) * M2;
-        float A[RP], E[RP];
+    const bool inr = vi < W && b < nitems;  // uniform over the quad
+    float A[RP];
+    if (inr) {
         const float4 *src = reinterpret_cast<const float4 *>(vs_sites) + (size_t)g * (RP / 4) * 64;
 #pragma unroll
         for (int k = 0; k < RP / 4; ++k) {
@@ -155,30 +119,69 @@ __global__ __launch_bounds__(64) void k_vertex_s(
             A[4 * k + 2] = v.z;
             A[4 * k + 3] = v.w;
         }
-        float c0[DP];  // m0[p][this lane] for the port-0 products
+    } else {
 #pragma unroll
-        for (int p = 0; p < DP; ++p) c0[p] = j < DP ? m0[p * DP + j] : 0.f;
+        for (int r = 0; r < RP; ++r) A[r] = 0.f;
+    }
+    const bool live = inr && !(ctl.enabled && ctl.done[vnet[b]]);
+    const bool done2 = mine && ctl.enabled && ctl.done[net2];
+    // the incoming message of (vi2, port), zero-padded to DP x DP (padded port: e0 e0^T)
+    {
+        float *dst = msm + (vi2 * 4 + port) * M2;
+#pragma unroll
+        for (int e = 0; e < M2; ++e) {
+            const int p = e / DP, q = e % DP;
+            float v = 0.f;
+            if (ino == -1) v = (p == 0 && q == 0) ? 1.f : 0.f;
+            else if (ino >= 0 && p < Dm && q < Dm) v = mold[ino + p * Dm + q];
+            dst[e] = v;
+        }
+    }
+    __syncthreads();
+
+    float *pq = part + quad * 4 * M2;
+    // an in-lane open's partial sums, summed over the quad (port 0) and stored by every lane
+    // of the quad (the same values)
+    auto store = [&](float (&o)[M2], int prt) {
+#pragma unroll
+        for (int e = 0; e < M2; ++e) pq[prt * M2 + e] = quad_sum(o[e]);
+    };
+    if (!live) {  // uniform over the quad; an idle or stopped slice contributes zeros
+#pragma unroll
+        for (int e = 0; e < 4 * M2; ++e) pq[e] = 0.f;
+    } else {
+        const float *m0 = msm + (vi * 4 + 0) * M2, *m1 = msm + (vi * 4 + 1) * M2;
+        const float *m2 = msm + (vi * 4 + 2) * M2, *m3 = msm + (vi * 4 + 3) * M2;
+        float c0[DP], c1[DP], c2[DP], c3[DP];  // m_k[p][this lane]
+#pragma unroll
+        for (int p = 0; p < DP; ++p) {
+            c0[p] = j < DP ? m0[p * DP + j] : 0.f;
+            c1[p] = j < DP ? m1[p * DP + j] : 0.f;
+            c2[p] = j < DP ? m2[p * DP + j] : 0.f;
+            c3[p] = j < DP ? m3[p * DP + j] : 0.f;
+        }
+        float E[RP], o[M2];
 
         // ---- phase A: E = A x0 m0 x1 m1
 #pragma unroll
         for (int r = 0; r < RP; ++r) E[r] = A[r];
         vs_mode_0<DP>(E, c0);
-        vs_mode_in<DP, 1>(E, m1);
-        // out2[a][a'] = sum_{p1,b3} (sum_p3 E[p1,a,p3] m3[p3][b3]) A[p1,a',b3]
+        vs_mode_in<DP, 1>(E, c1);
+        // out2[a][a'] = sum_{p1,b3} G[p1,a,b3] A[p1,a',b3], G = E x3 m3 block by block (p1)
 #pragma unroll
         for (int e = 0; e < M2; ++e) o[e] = 0.f;
 #pragma unroll
         for (int p1 = 0; p1 < DP; ++p1) {
             float G[M2];  // [a = p2][b3]
 #pragma unroll
-            for (int x = 0; x < DP; ++x)
+            for (int x = 0; x < DP; ++x) {
+                v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int y = 0; y < DP; ++y) {
-                    float t = 0.f;
+                for (int p = 0; p < DP; ++p)
+                    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(c3[p], E[(p1 * DP + x) * DP + p], acc, 0, 0, 0);
 #pragma unroll
-                    for (int p = 0; p < DP; ++p) t = fmaf(E[(p1 * DP + x) * DP + p], m3[p * DP + y], t);
-                    G[x * DP + y] = t;
-                }
+                for (int y = 0; y < DP; ++y) G[x * DP + y] = acc[y];
+            }
 #pragma unroll
             for (int a = 0; a < DP; ++a)
 #pragma unroll
@@ -188,21 +191,21 @@ __global__ __launch_bounds__(64) void k_vertex_s(
                         o[a * DP + ap] = fmaf(G[a * DP + t], A[(p1 * DP + ap) * DP + t], o[a * DP + ap]);
         }
         store(o, 2);
-        // out3[a][a'] = sum_{p1,b2} (sum_p2 E[p1,p2,a] m2[p2][b2]) A[p1,b2,a']
+        // out3[a][a'] = sum_{p1,b2} G[p1,b2,a] A[p1,b2,a'], G = E x2 m2 block by block (p1)
 #pragma unroll
         for (int e = 0; e < M2; ++e) o[e] = 0.f;
 #pragma unroll
         for (int p1 = 0; p1 < DP; ++p1) {
             float G[M2];  // [b2][a = p3]
 #pragma unroll
-            for (int x = 0; x < DP; ++x)
+            for (int y = 0; y < DP; ++y) {
+                v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int y = 0; y < DP; ++y) {
-                    float t = 0.f;
+                for (int p = 0; p < DP; ++p)
+                    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(c2[p], E[(p1 * DP + p) * DP + y], acc, 0, 0, 0);
 #pragma unroll
-                    for (int p = 0; p < DP; ++p) t = fmaf(E[(p1 * DP + p) * DP + y], m2[p * DP + x], t);
-                    G[x * DP + y] = t;
-                }
+                for (int x = 0; x < DP; ++x) G[x * DP + y] = acc[x];
+            }
 #pragma unroll
             for (int a = 0; a < DP; ++a)
 #pragma unroll
@@ -216,8 +219,8 @@ __global__ __launch_bounds__(64) void k_vertex_s(
         // ---- phase B: E = A x2 m2 x3 m3
 #pragma unroll
         for (int r = 0; r < RP; ++r) E[r] = A[r];
-        vs_mode_in<DP, 2>(E, m2);
-        vs_mode_in<DP, 3>(E, m3);
+        vs_mode_in<DP, 2>(E, c2);
+        vs_mode_in<DP, 3>(E, c3);
         // out1 = open_1(E x0 m0, A), the port-0 product fused entry by entry
 #pragma unroll
         for (int e = 0; e < M2; ++e) o[e] = 0.f;
@@ -225,44 +228,33 @@ __global__ __launch_bounds__(64) void k_vertex_s(
         for (int a = 0; a < DP; ++a)
 #pragma unroll
             for (int t = 0; t < DP * DP; ++t) {
-                const float x = E[a * DP * DP + t];
+                const v4f gx = quad_gather(E[a * DP * DP + t]);
                 float gv = 0.f;
 #pragma unroll
-                for (int p = 0; p < DP; ++p) gv = fmaf(quad_bcast(x, p), c0[p], gv);
+                for (int p = 0; p < DP; ++p) gv = fmaf(gx[p], c0[p], gv);
 #pragma unroll
                 for (int ap = 0; ap < DP; ++ap) o[a * DP + ap] = fmaf(gv, A[ap * DP * DP + t], o[a * DP + ap]);
             }
         store(o, 1);
-        // out0 = open_0(E x1 m1, A): this lane holds row a = j of out0
-        vs_mode_in<DP, 1>(E, m1);
-        float o0[DP];
+        // out0 = open_0(E x1 m1, A) on the matrix cores: with a = G (row a = lane i) and
+        // b = A (column a' = lane j), register i of lane j collects out0[i][j]
+        vs_mode_in<DP, 1>(E, c1);
+        v4f o0 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ap = 0; ap < DP; ++ap) o0[ap] = 0.f;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const float x = E[r], y = A[r];
-#pragma unroll
-            for (int ap = 0; ap < DP; ++ap) o0[ap] = fmaf(x, quad_bcast(y, ap), o0[ap]);
-        }
+        for (int r = 0; r < R; ++r) o0 = __builtin_amdgcn_mfma_f32_4x4x1f32(E[r], A[r], o0, 0, 0, 0);
         if (j < DP) {
 #pragma unroll
-            for (int ap = 0; ap < DP; ++ap) pq[j * DP + ap] = o0[ap];
+            for (int i = 0; i < DP; ++i) pq[i * DP + j] = o0[i];
         }
     }
     __syncthreads();
 
     // ---- one lane per message: sum over slices, normalise, residuals, write
-    const int vi2 = lane >> 2, port = lane & 3, b2 = g * W + vi2;
-    if (vi2 >= W || b2 >= nitems) return;
-    const int4 o4 = vout[b2], d4 = vdim[b2];
-    const int q = port == 0 ? o4.x : port == 1 ? o4.y : port == 2 ? o4.z : o4.w;
-    if (q < 0) return;
-    const int D = port == 0 ? d4.x : port == 1 ? d4.y : port == 2 ? d4.z : d4.w;
-    const long long mo = msg_off[q];
-    if (ctl.enabled && ctl.done[vnet[b2]]) {
-        for (int e = 0; e < D * D; ++e) mnew[mo + e] = mold[mo + e];
-        res_fid[q] = 0.0;
-        res_abs[q] = 0.0;
+    if (qout < 0) return;
+    if (done2) {
+        for (int e = 0; e < Dm * Dm; ++e) mnew[outo + e] = mold[outo + e];
+        res_fid[qout] = 0.0;
+        res_abs[qout] = 0.0;
         return;
     }
     float x[M2];
@@ -272,15 +264,15 @@ __global__ __launch_bounds__(64) void k_vertex_s(
         float v = 0.f;
         for (int s = 0; s < d; ++s) v += part[((vi2 * d + s) * 4 + port) * M2 + e];
         x[e] = v;
-        if (e / DP < D && e % DP < D) S += (double)v;
+        S += (double)v;  // entries beyond the bond dimension are exactly zero
     }
     double dn = 0.0, nn = 0.0, oo = 0.0, mx = 0.0;
 #pragma unroll
     for (int e = 0; e < M2; ++e) {
         const int a = e / DP, ap = e % DP;
-        if (a >= D || ap >= D) continue;
+        if (a >= Dm || ap >= Dm) continue;
         const float xf = (float)(S != 0.0 ? (double)x[e] / S : (double)x[e]);
-        const long long o = mo + a * D + ap;
+        const long long o = outo + a * Dm + ap;
         const double xo = (double)mold[o];
         mnew[o] = xf;
         dn += (double)xf * xo;
@@ -290,8 +282,8 @@ __global__ __launch_bounds__(64) void k_vertex_s(
         mx = (df > mx || df != df) ? df : mx;
     }
     const double c = dn / sqrt(nn * oo);
-    res_fid[q] = 1.0 - c * c;
-    res_abs[q] = mx;
+    res_fid[qout] = 1.0 - c * c;
+    res_abs[qout] = mx;
 }
 
 // Pad the sites of one DP bucket into its layout (grid-stride over every float).
@@ -366,8 +358,8 @@ static int vertex_s_sweep(const VertexPlan *p, hipStream_t st, DevTopo tp, const
         const int ng = vertex_s_groups(p, k);
 #define VS_GO(DPV)                                                                               \
     hipLaunchKernelGGL((k_vertex_s<DPV>), dim3(ng), dim3(64), 0, st, p->d_vs_sites[k],             \
-                       (const float *)src, (float *)dst, p->d_vs_in[k], p->d_vs_out[k],          \
-                       p->d_vs_dim[k], p->d_vs_net[k], tp.msg_off, ctl, res_fid, res_abs,        \
+                       (const float *)src, (float *)dst, p->d_vs_ino[k], p->d_vs_outo[k],        \
+                       p->d_vs_out[k], p->d_vs_dim[k], p->d_vs_net[k], ctl, res_fid, res_abs,    \
                        p->ns[k], p->d)
         if (k == 0) VS_GO(2);
         else if (k == 1) VS_GO(3);
