@@ -401,13 +401,13 @@ def large_bond_timing(pkg, torch, seed, L=24, D=16):
     return out
 
 
-def small_bond_timing(pkg, seed, L=256, Ds=(2, 3, 4), d=2):
-    """Bond dimensions below 8: the small-bond kernel (k_vertex_s, VALU) on an L x L FP32
-    lattice per D, against the same sweep zero-padded to 8 on the MFMA kernel and on the
+def small_bond_timing(pkg, seed, cases=((512, 2), (256, 3), (256, 4)), d=2):
+    """Bond dimensions below 8: the small-bond kernel (k_vertex_s) on an L x L FP32 lattice per
+    (L, D), against the same sweep zero-padded to 8 on the D <= 8 MFMA kernel and on the
     generic kernel.  Algorithmic bytes: FP32 sites read once, messages read and written once;
     algorithmic FLOPs: sum over directed messages of 2 k d D^(k+1)."""
     out = []
-    for D in Ds:
+    for L, D in cases:
         t = pkg.build_topology(pkg.named_grid((L, L)), D, d)
         s = pkg.peps.random_sites(t, seed, scale=pkg.peps.peps_scale(d, D))
         flops = algorithmic_flops_per_sweep(t)

@@ -286,6 +286,215 @@ __global__ __launch_bounds__(64) void k_vertex_s(
     res_abs[qout] = mx;
 }
 
+// ---------------------------------------------------------------- bond dimension <= 2
+//
+// At DP = 2 a slice is 16 values: one lane holds a whole slice A_u[s] (register
+// r = p0*8 + p1*4 + p2*2 + p3) and every product is in-lane, so a wave holds W2 = 64 / d
+// vertices (lane = vi * d + s).  The d slices of a vertex meet in LDS; lane s of the vertex
+// then finishes the messages of ports j with j % d == s.  Site layout per group:
+// float[group][4][64 lanes][4] (four float4 loads per lane, each a contiguous 1 KiB line).
+
+__device__ __forceinline__ int t2_idx(int p0, int p1, int p2, int p3) {
+    return p0 * 8 + p1 * 4 + p2 * 2 + p3;
+}
+
+// X[.., b at port P, ..] = sum_p X[.., p, ..] m[p][b], in place (m: 2 x 2, [p][b])
+template <int P>
+__device__ __forceinline__ void t2_mode(float (&X)[16], const float (&m)[4]) {
+    constexpr int st = 8 >> P;  // register stride of port P
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        if (r & st) continue;  // r has digit 0 at port P
+        const float x0 = X[r], x1 = X[r + st];
+        X[r] = fmaf(x0, m[0], x1 * m[2]);
+        X[r + st] = fmaf(x0, m[1], x1 * m[3]);
+    }
+}
+
+// out[a][a'] = sum over the other ports of G[.., a at P, ..] A[.., a' at P, ..]
+template <int P>
+__device__ __forceinline__ void t2_open(const float (&G)[16], const float (&A)[16], float (&out)[4]) {
+    constexpr int st = 8 >> P;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) out[e] = 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        if (r & st) continue;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int ap = 0; ap < 2; ++ap)
+                out[a * 2 + ap] = fmaf(G[r + a * st], A[r + ap * st], out[a * 2 + ap]);
+    }
+}
+
+// One message of the bond-dimension <= 2 bucket: 2 x 2 zero-padded (a padded port gives
+// e0 e0^T); a full 2 x 2 message at a 16-byte aligned offset is one float4 access.
+__device__ __forceinline__ float4 t2_load_msg(const float *m, long long off, int D) {
+    if (off == -1) return make_float4(1.f, 0.f, 0.f, 0.f);
+    if (D == 2 && (off & 3) == 0) return *reinterpret_cast<const float4 *>(m + off);
+    if (D == 2) return make_float4(m[off], m[off + 1], m[off + 2], m[off + 3]);
+    return make_float4(D == 1 ? m[off] : 0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(64) void k_vertex_t2(
+    const float *__restrict__ vs_sites, const float *__restrict__ mold, float *__restrict__ mnew,
+    const long long *__restrict__ vs_ino, const long long *__restrict__ vs_outo,
+    const int *__restrict__ vs_out, const int *__restrict__ vs_dim, const int *__restrict__ vnet,
+    NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs, int nitems, int d) {
+    __shared__ float4 msm[256];      // incoming messages, [vi * 4 + port]
+    __shared__ float4 part[64 * 4];  // per-slice results, [lane * 4 + port]
+    const int g = blockIdx.x, lane = threadIdx.x;
+    const int W2 = 64 / d, nm = W2 * 4;  // vertices and messages of this group
+    const int vi = lane / d, s = lane % d, b = g * W2 + vi;
+    const bool inr = vi < W2 && b < nitems;
+    float A[16];
+    if (inr) {
+        const float4 *src = reinterpret_cast<const float4 *>(vs_sites) + (size_t)g * 4 * 64;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4 v = src[k * 64 + lane];
+            A[4 * k] = v.x;
+            A[4 * k + 1] = v.y;
+            A[4 * k + 2] = v.z;
+            A[4 * k + 3] = v.w;
+        }
+    }
+    // incoming messages: one lane per message, 16-byte loads
+    for (int mi = lane; mi < nm; mi += 64) {
+        const int bb = g * W2 + mi / 4;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (bb < nitems) v = t2_load_msg(mold, vs_ino[4 * bb + (mi & 3)], vs_dim[4 * bb + (mi & 3)]);
+        msm[mi] = v;
+    }
+    wave_lds_sync();  // one wave per workgroup
+    const bool live = inr && !(ctl.enabled && ctl.done[vnet[b]]);
+    float o[4][4];  // this slice's results, [port][a * 2 + a']
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) o[j][e] = 0.f;
+    if (live) {
+        float m[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float4 v = msm[vi * 4 + j];
+            m[j][0] = v.x;
+            m[j][1] = v.y;
+            m[j][2] = v.z;
+            m[j][3] = v.w;
+        }
+        float E[16], G[16];
+        // phase A: E = A x0 m0 x1 m1; out2 = open_2(E x3 m3, A), out3 = open_3(E x2 m2, A)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) E[r] = A[r];
+        t2_mode<0>(E, m[0]);
+        t2_mode<1>(E, m[1]);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) G[r] = E[r];
+        t2_mode<3>(G, m[3]);
+        t2_open<2>(G, A, o[2]);
+        t2_mode<2>(E, m[2]);
+        t2_open<3>(E, A, o[3]);
+        // phase B: E = A x2 m2 x3 m3; out0 = open_0(E x1 m1, A), out1 = open_1(E x0 m0, A)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) E[r] = A[r];
+        t2_mode<2>(E, m[2]);
+        t2_mode<3>(E, m[3]);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) G[r] = E[r];
+        t2_mode<1>(G, m[1]);
+        t2_open<0>(G, A, o[0]);
+        t2_mode<0>(E, m[0]);
+        t2_open<1>(E, A, o[1]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) part[lane * 4 + j] = make_float4(o[j][0], o[j][1], o[j][2], o[j][3]);
+    wave_lds_sync();
+    // one lane per outgoing message: sum over the vertex's slices, normalise, residuals
+    for (int mi = lane; mi < nm; mi += 64) {
+        const int v2 = mi / 4, j = mi & 3, bb = g * W2 + v2;
+        if (bb >= nitems) continue;
+        const int qout = vs_out[4 * bb + j];
+        if (qout < 0) continue;
+        const int D = vs_dim[4 * bb + j];
+        const long long mo = vs_outo[4 * bb + j];
+        const float4 xo4 = t2_load_msg(mold, mo, D);
+        const float xo[4] = {xo4.x, xo4.y, xo4.z, xo4.w};
+        if (ctl.enabled && ctl.done[vnet[bb]]) {
+            for (int e = 0; e < D * D; ++e) mnew[mo + e] = xo[e];
+            res_fid[qout] = 0.0;
+            res_abs[qout] = 0.0;
+            continue;
+        }
+        float x[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < d; ++t) {
+            const float4 pv = part[(v2 * d + t) * 4 + j];
+            x[0] += pv.x;
+            x[1] += pv.y;
+            x[2] += pv.z;
+            x[3] += pv.w;
+        }
+        const double S = (double)x[0] + (double)x[1] + (double)x[2] + (double)x[3];
+        float xf[4];
+        double dn = 0.0, nn = 0.0, oo = 0.0, mx = 0.0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            // entries beyond the bond dimension are exactly zero; the message is stored D x D
+            const int a = e >> 1, ap = e & 1;
+            xf[e] = (float)(S != 0.0 ? (double)x[e] / S : (double)x[e]);
+            if (a >= D || ap >= D) continue;
+            const double xv = (double)xf[e], xw = (double)xo[a * D + ap];
+            dn += xv * xw;
+            nn += xv * xv;
+            oo += xw * xw;
+            const double df = fabs(xv - xw);
+            mx = (df > mx || df != df) ? df : mx;
+        }
+        if (D == 2 && (mo & 3) == 0) {
+            *reinterpret_cast<float4 *>(mnew + mo) = make_float4(xf[0], xf[1], xf[2], xf[3]);
+        } else if (D == 2) {
+            for (int e = 0; e < 4; ++e) mnew[mo + e] = xf[e];
+        } else {
+            mnew[mo] = xf[0];
+        }
+        const double c = dn / sqrt(nn * oo);
+        res_fid[qout] = 1.0 - c * c;
+        res_abs[qout] = mx;
+    }
+}
+
+// Sites of the bond-dimension <= 2 bucket in the lane-per-slice layout.
+__global__ void k_t2_pad_sites(const double *__restrict__ sites, const long long *__restrict__ site_off,
+                               const int *__restrict__ port_off, const int *__restrict__ port_edge,
+                               const int *__restrict__ edge_dim, const int *__restrict__ vlist,
+                               int nitems, int d, int ngroup, float *__restrict__ out) {
+    const int W2 = 64 / d;
+    const long long total = (long long)ngroup * 16 * 64;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(((i / 256) % 4) * 4 + (i % 4));
+        const int lane = (int)((i / 4) % 64);
+        const int g = (int)(i / 1024);
+        const int vi = lane / d, s = lane % d, b = g * W2 + vi;
+        float v = 0.f;
+        if (vi < W2 && b < nitems) {
+            const int u = vlist[b];
+            const int x[4] = {(r >> 3) & 1, (r >> 2) & 1, (r >> 1) & 1, r & 1};
+            const int k = port_off[u + 1] - port_off[u];
+            long long idx = s;
+            bool ok = true;
+            for (int t = 0; t < 4; ++t) {
+                const int D = t < k ? edge_dim[port_edge[port_off[u] + t]] : 1;
+                if (x[t] >= D) { ok = false; break; }
+                if (t < k) idx = idx * D + x[t];
+            }
+            if (ok) v = (float)sites[site_off[u] + idx];
+        }
+        out[i] = v;
+    }
+}
+
 // Pad the sites of one DP bucket into its layout (grid-stride over every float).
 template <int DP>
 __global__ void k_vs_pad_sites(const double *__restrict__ sites, const long long *__restrict__ site_off,
@@ -321,7 +530,7 @@ __global__ void k_vs_pad_sites(const double *__restrict__ sites, const long long
 }
 
 static int vertex_s_groups(const VertexPlan *p, int k) {
-    const int W = 16 / p->d;
+    const int W = k == 0 ? 64 / p->d : 16 / p->d;  // DP = 2: a slice per lane
     return (p->ns[k] + W - 1) / W;
 }
 
@@ -332,18 +541,24 @@ static int vertex_s_upload_sites(VertexPlan *p, hipStream_t st, const double *d_
         if (p->d_vs_sites[k]) (void)hipFree(p->d_vs_sites[k]);
         p->d_vs_sites[k] = nullptr;
         if (p->ns[k] == 0) continue;
-        const int DP = k + 2, RP = (DP * DP * DP + 3) & ~3;
+        const int DP = k + 2, RP = DP == 2 ? 16 : (DP * DP * DP + 3) & ~3;
         const int ng = vertex_s_groups(p, k);
         if (hipMalloc((void **)&p->d_vs_sites[k], sizeof(float) * (size_t)ng * RP * 64) != hipSuccess)
             return 5;
         const long long total = (long long)ng * RP * 64;
         const int blocks = (int)std::min<long long>((total + 255) / 256, 4096);
+        if (DP == 2) {
+            hipLaunchKernelGGL(k_t2_pad_sites, dim3(std::max(blocks, 1)), dim3(256), 0, st,
+                               d_sites64, d_site_off, d_port_off, d_port_edge, d_edge_dim,
+                               p->d_vs_list[k], p->ns[k], p->d, ng, p->d_vs_sites[k]);
+            if (hipGetLastError() != hipSuccess) return 2;
+            continue;
+        }
 #define VS_PAD(DPV)                                                                              \
     hipLaunchKernelGGL((k_vs_pad_sites<DPV>), dim3(std::max(blocks, 1)), dim3(256), 0, st,         \
                        d_sites64, d_site_off, d_port_off, d_port_edge, d_edge_dim,               \
                        p->d_vs_list[k], p->ns[k], p->d, ng, p->d_vs_sites[k])
-        if (DP == 2) VS_PAD(2);
-        else if (DP == 3) VS_PAD(3);
+        if (DP == 3) VS_PAD(3);
         else VS_PAD(4);
 #undef VS_PAD
         if (hipGetLastError() != hipSuccess) return 2;
@@ -361,7 +576,11 @@ static int vertex_s_sweep(const VertexPlan *p, hipStream_t st, DevTopo tp, const
                        (const float *)src, (float *)dst, p->d_vs_ino[k], p->d_vs_outo[k],        \
                        p->d_vs_out[k], p->d_vs_dim[k], p->d_vs_net[k], ctl, res_fid, res_abs,    \
                        p->ns[k], p->d)
-        if (k == 0) VS_GO(2);
+        if (k == 0)
+            hipLaunchKernelGGL(k_vertex_t2, dim3(ng), dim3(64), 0, st, p->d_vs_sites[k],
+                               (const float *)src, (float *)dst, p->d_vs_ino[k], p->d_vs_outo[k],
+                               p->d_vs_out[k], p->d_vs_dim[k], p->d_vs_net[k], ctl, res_fid,
+                               res_abs, p->ns[k], p->d);
         else if (k == 1) VS_GO(3);
         else VS_GO(4);
 #undef VS_GO
