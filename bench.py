@@ -132,10 +132,17 @@ def uneven_batch_timing(pkg, torch, seed, sweeps=20):
     r = e.run(maxiter=sweeps, use_graph=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t1
+    flops = algorithmic_flops_per_sweep(b)
+    nbytes = algorithmic_bytes_per_sweep(b, 4)
+    per = wall / sweeps
     out = {"workload": f"64 uneven PEPS norm networks ({b.nv} vertices, {b.nports} messages), "
                        f"{sweeps} FP32 Jacobi sweeps, autotuned",
            "path": e.path(), "msg_per_s": b.nports * sweeps / wall,
-           "ms_per_sweep": 1e3 * wall / sweeps, "tuned_decisions": ntuned, "tune_s": t_tune}
+           "ms_per_sweep": 1e3 * per, "tuned_decisions": ntuned, "tune_s": t_tune,
+           "algorithmic_flops_per_sweep": flops, "algorithmic_bytes_per_sweep": nbytes,
+           "algorithmic_TFLOPs": flops / per / 1e12,
+           "frac_fp32_mfma_peak": flops / per / 1e12 / PEAK["fp32_mfma_TFLOPs"],
+           "achieved_GBs": nbytes / per / 1e9}
     e.close()
     return out
 

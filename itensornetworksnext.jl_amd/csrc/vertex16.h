@@ -82,7 +82,8 @@ __device__ __forceinline__ void v16_mode(unsigned char *S, v4s Am) {
         loff[t] = v16_off(p[0], p[1], p[2], p[3]);
     }
 #pragma unroll 4
-    for (int g = w; g < 256; g += NW) {
+    for (int i = 0; i < 256 / NW; ++i) {  // this wave's tiles g = w, w + NW, ...
+        const int g = w + NW * i;
         int q[4] = {0, 0, 0, 0};  // iteration fields: ports P, Q (never the last port)
         q[PT::P] = g >> 4;
         q[PT::Q] = g & 15;
