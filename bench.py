@@ -390,21 +390,31 @@ def large_bond_timing(pkg, torch, seed, L=24, D=16):
     e.polish_fp64(3)
     m_pol2, ln_pol2 = e.messages(), e.lognorm()
     flops = algorithmic_flops_per_sweep(t)
+    bf_path = e.path()
+    e.close()
+    # the FP64 sweep alone (the polish's kernel), against the generic FP64 kernel
+    fp64 = {}
+    for path in ("auto", "generic"):
+        with pkg.Engine(t, s, dtype="f64", path=path) as e64:
+            e64.init_messages("identity")
+            fp64[path] = (e64.path(), e64.time_sweeps(3 if path == "generic" else 10))
 
     def rel(a, b):
         return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
     out = {"workload": f"{L}x{L} PEPS norm network, d=2, D={D}, BF16 MFMA operands / FP32 sums, "
                        f"then 3 FP64 polishing sweeps",
-           "path": e.path(), "kernel_ms_per_sweep": kms,
+           "path": bf_path, "kernel_ms_per_sweep": kms,
            "msg_per_s_kernel": t.nports / (kms * 1e-3),
            "algorithmic_TFLOPs": flops / (kms * 1e-3) / 1e12,
            "bf16_sweeps": int(r.sweeps[0]), "bf16_s": t_bf, "polish3_s": t_pol,
            "msg_change_bf16_to_polished": rel(m_bf, m_pol),
            "msg_change_polished_3_to_6": rel(m_pol, m_pol2),
            "lognorm_change_bf16_to_polished": abs(ln_bf - ln_pol) / abs(ln_pol),
-           "lognorm_change_polished_3_to_6": abs(ln_pol - ln_pol2) / abs(ln_pol2)}
-    e.close()
+           "lognorm_change_polished_3_to_6": abs(ln_pol - ln_pol2) / abs(ln_pol2),
+           "fp64_sweep_path": fp64["auto"][0], "fp64_sweep_ms": fp64["auto"][1],
+           "fp64_sweep_algorithmic_TFLOPs": flops / (fp64["auto"][1] * 1e-3) / 1e12,
+           "fp64_generic_sweep_ms": fp64["generic"][1]}
     return out
 
 

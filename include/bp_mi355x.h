@@ -176,7 +176,9 @@ int bp_lognorm(bp_context *ctx, double *out_re, double *out_im);
 
 /* FP64 polishing: run `nsweeps` FP64 Jacobi sweeps from the current messages on the FP64
  * copy of the sites, and make the FP64 result the context's messages for scalars and
- * download.  Per-network last delta (max-abs) in out_delta (may be NULL). */
+ * download.  The sweeps run on an FP64 vertex kernel when one covers every network
+ * (largest bond dimension 6..8, or 9..16 with the large-bond kernel), else on the generic
+ * kernel.  Per-network last delta (max-abs) in out_delta (may be NULL). */
 int bp_polish_fp64(bp_context *ctx, int nsweeps, double *out_delta);
 
 /* Gradient of the BP log-norm with respect to every site tensor at the current fixed
@@ -245,7 +247,11 @@ int bp_local_energy_device(bp_context *ctx, const double *ops_v, const double *o
  *   "vertex16_waves"   waves per workgroup of the BF16 kernel for bond dimensions 9..16
  *                      (4, 8 or 16; default 16);
  *   "vertex_bond_mask" bit D set: networks whose largest bond dimension is D may run on the
- *                      vertex kernels (D 1..12 in FP32, 1..16 in BF16; default 0x1FFFE);
+ *                      vertex kernels (D 1..12 in FP32, 1..16 in BF16, default 0x1FFFE;
+ *                      FP64: 6..8 on the D <= 8 kernel, 9..16 on the large-bond kernel,
+ *                      default 0x1FFC0);
+ *   "small_bond"       1 (default): FP32 networks whose bond dimensions are all <= 4 run on
+ *                      the small-bond kernels; 0: padded to 8 on the D <= 8 kernels;
  *   "fold_bookkeeping" 1 (default): when one D <= 8 vertex launch covers every network, the
  *                      sweep kernels also do the per-network convergence bookkeeping;
  *                      0: a separate bookkeeping kernel after every sweep (same results);
@@ -262,7 +268,7 @@ int bp_set_option(bp_context *ctx, const char *name, int value);
 int bp_get_option(bp_context *ctx, const char *name, int *value);
 
 /* Autotuner.  Times, on the context's own sites and messages, (a) for every largest bond
- * dimension D present that the vertex kernels take (<= 12 FP32, <= 16 BF16; norm networks,
+ * dimension D present that the vertex kernels take (<= 12 FP32, <= 16 BF16 and FP64; norm networks,
  * automatic path) whether its networks sweep faster on the vertex kernels (bonds padded to
  * 8, 12 or 16) or on the generic kernel, and
  * (b) the D <= 8 vertex sweep kernel variant ("vertex_variant" of bp_set_option) and the

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(CSRC, "libbpmi355x.so")
 SOURCES = ["bp_engine.hip"]
 HEADERS = ["common.h", "generic.h", "complex.h", "vertex.h", "vertex16.h", "vertex12.h",
-           "vertex64.h", "vertex4.h", "adjoint.h", "vertex_adjoint.h", "contract.h", "gate.h"]
+           "vertex64.h", "vertex64l.h", "vertex4.h", "adjoint.h", "vertex_adjoint.h", "contract.h", "gate.h"]
 
 
 def hipcc() -> str:

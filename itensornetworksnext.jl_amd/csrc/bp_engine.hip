@@ -27,6 +27,7 @@
 #include "vertex.h"
 #include "vertex16.h"
 #include "vertex64.h"
+#include "vertex64l.h"
 #include "vertex4.h"
 #include "adjoint.h"
 #include "vertex_adjoint.h"
@@ -324,6 +325,8 @@ struct bp_context {
     // FP64 D <= 8 vertex kernel (vertex64.h): the sweeps of FP64 contexts and the FP64
     // polish of every context.  Built on first use for the current topology and mask.
     Vertex64Plan v64;
+    Vertex64LPlan v64l;         // bond dimensions 9..16 (vertex64l.h), when the D <= 8 plan
+    int v64_kind = 0;           // declines: 1 the D <= 8 kernel, 2 the large-bond one
     bool v64_tried = false;     // plan attempted (v64_plan_ok tells the outcome)
     bool v64_plan_ok = false;
     bool v64_sites = false;     // padded FP64 slices match the current sites
@@ -422,6 +425,8 @@ static void free_all(bp_context *c) {
     c->adj.release();
     vertex_plan_release(&c->vplan);
     vertex64_release(&c->v64);
+    vertex64l_release(&c->v64l);
+    c->v64_kind = 0;
     c->v64_tried = c->v64_plan_ok = c->v64_sites = c->v64_ok = false;
     for (auto &b : c->gb)
         if (b.d_slots) (void)hipFree(b.d_slots);
@@ -688,26 +693,43 @@ static int launch_edge_scalars(bp_context *c, const T *msgs) {
 
 
 // The FP64 vertex plan for this context's networks (bond dimensions enabled: the context's
-// vertex mask in FP64 contexts, 6..8 for the polish of FP32/BF16 contexts), with its padded
-// FP64 slices when sites are present.  Allocates: never called inside a graph capture
+// vertex mask in FP64 contexts; 6..8 and 9..16 for the polish of FP32/BF16 contexts): the
+// D <= 8 kernel (vertex64.h) when every network's largest bond dimension is 6..8, else the
+// large-bond kernel (vertex64l.h) when it is 9..16 for every network, with the padded FP64
+// slices when sites are present.  Allocates: never called inside a graph capture
 // (prepare_sweep runs first).
 static int ensure_v64(bp_context *c) {
     if (!c->have_topo) return BP_OK;
     if (!c->v64_tried) {
         const unsigned mask = c->dtype == BP_F64 ? c->vx_mask : 0x1C0u;
-        c->v64_plan_ok = vertex64_plan_build(&c->v64, c->plain, c->nv, c->port_off.data(),
-                                             c->port_edge.data(), c->edge_dim.data(),
-                                             c->phys_dim.data(), c->port_rev.data(),
-                                             c->vnet.data(), mask) == 0 &&
-                         vertex64_set_lds_limit(&c->v64) == 0;
+        c->v64_kind = vertex64_plan_build(&c->v64, c->plain, c->nv, c->port_off.data(),
+                                          c->port_edge.data(), c->edge_dim.data(),
+                                          c->phys_dim.data(), c->port_rev.data(),
+                                          c->vnet.data(), mask) == 0 &&
+                              vertex64_set_lds_limit(&c->v64) == 0
+                          ? 1 : 0;
+        if (!c->v64_kind) {
+            const unsigned lmask = c->dtype == BP_F64 ? c->vx_mask : 0x1FE00u;
+            if (vertex64l_plan_build(&c->v64l, c->plain, c->nv, c->port_off.data(),
+                                     c->port_edge.data(), c->edge_dim.data(),
+                                     c->phys_dim.data(), c->port_rev.data(), c->vnet.data(),
+                                     lmask) == 0 &&
+                vertex64l_set_lds_limit() == 0)
+                c->v64_kind = 2;
+        }
+        c->v64_plan_ok = c->v64_kind != 0;
         c->v64_tried = true;
         c->v64_sites = false;
         bump_epoch();
     }
     if (c->v64_plan_ok && c->have_sites && !c->v64_sites) {
-        if (vertex64_upload_sites(&c->v64, c->stream, c->d_sites64, c->d_site_off,
-                                  c->d_port_off, c->d_port_edge, c->d_edge_dim) != 0)
-            return set_err(BP_ERR_HIP, "FP64 vertex site layout failed");
+        const int rc = c->v64_kind == 1
+                           ? vertex64_upload_sites(&c->v64, c->stream, c->d_sites64, c->d_site_off,
+                                                   c->d_port_off, c->d_port_edge, c->d_edge_dim)
+                           : vertex64l_upload_sites(&c->v64l, c->stream, c->d_sites64,
+                                                    c->d_site_off, c->d_port_off,
+                                                    c->d_port_edge, c->d_edge_dim);
+        if (rc != 0) return set_err(BP_ERR_HIP, "FP64 vertex site layout failed");
         HIPCHK(hipStreamSynchronize(c->stream));
         c->v64_sites = true;
         bump_epoch();
@@ -718,7 +740,19 @@ static int ensure_v64(bp_context *c) {
 
 static void v64_invalidate(bp_context *c) {
     vertex64_release(&c->v64);
+    vertex64l_release(&c->v64l);
+    c->v64_kind = 0;
     c->v64_tried = c->v64_plan_ok = c->v64_sites = c->v64_ok = false;
+}
+
+// One FP64 sweep on the context's FP64 vertex kernel (ensure_v64 has made it ready).
+static int v64_sweep(bp_context *c, const double *src, double *dst, const NetCtl &ctl) {
+    const int rc = c->v64_kind == 1
+                       ? vertex64_sweep(&c->v64, c->stream, c->d_msg_off, src, dst, ctl,
+                                        c->d_res_fid, c->d_res_abs)
+                       : vertex64l_sweep(&c->v64l, c->stream, c->d_msg_off, src, dst, ctl,
+                                         c->d_res_fid, c->d_res_abs);
+    return rc != 0 ? set_err(BP_ERR_HIP, "FP64 vertex sweep launch failed") : BP_OK;
 }
 
 // Flags of the generic kernel for this context (BF16 operands, no normalisation).
@@ -859,9 +893,7 @@ static int sweep_once(bp_context *c, const NetCtl &done, bool *folded = nullptr,
         *folded = true;
     }
     if (path == BP_PATH_VERTEX && c->dtype == BP_F64) {
-        if (vertex64_sweep(&c->v64, c->stream, c->d_msg_off, (const double *)src, (double *)dst,
-                           done, c->d_res_fid, c->d_res_abs) != 0)
-            return set_err(BP_ERR_HIP, "FP64 vertex sweep launch failed");
+        RETCHK(v64_sweep(c, (const double *)src, (double *)dst, done));
     } else if (path == BP_PATH_VERTEX || path == BP_PATH_MIXED) {
         RETCHK(vertex_sweep(&c->vplan, c->stream, c->topo(), c->d_vsites, src, dst, done,
                             c->d_res_fid, c->d_res_abs, fold));
@@ -972,9 +1004,9 @@ int bp_create(bp_context **out, int device, int dtype) {
     bp_context *c = new bp_context();
     c->device = device;
     c->dtype = dtype;
-    // FP64 contexts: the FP64 vertex kernel for largest bond dimensions 6..8 by default
+    // FP64 contexts: the FP64 vertex kernels for largest bond dimensions 6..16 by default
     // (below 6 the zero padding to 8 costs more than the generic kernel saves)
-    if (dtype == BP_F64) c->vx_mask = 0x1C0u;
+    if (dtype == BP_F64) c->vx_mask = 0x1FFC0u;
     if (dtype == BP_C128) c->vx_mask = 0u;
     if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
@@ -1326,8 +1358,8 @@ int bp_set_path(bp_context *c, int path) {
     if (path == BP_PATH_VERTEX && c->dtype == BP_F64) {
         RETCHK(ensure_v64(c));
         if (!c->v64_plan_ok)
-            return set_err(BP_ERR_UNSUPPORTED, "FP64 vertex path not available: %s",
-                           c->v64.why.c_str());
+            return set_err(BP_ERR_UNSUPPORTED, "FP64 vertex path not available: %s; %s",
+                           c->v64.why.c_str(), c->v64l.why.c_str());
     } else if (path == BP_PATH_VERTEX && !(c->vplan_ok && c->vplan.all)) {
         return set_err(BP_ERR_UNSUPPORTED, "vertex path not available for every network: %s",
                        c->vplan.why.c_str());
@@ -1691,7 +1723,7 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
                 for (int q = c->port_off[u]; q < c->port_off[u + 1]; ++q)
                     dm = std::max(dm, c->edge_dim[c->port_edge[q]]);
             }
-            const int dlim = c->dtype == BP_BF16 ? 16 : c->dtype == BP_F32 ? 12 : VX_D;
+            const int dlim = c->dtype == BP_BF16 ? 16 : c->dtype == BP_F32 ? 12 : 16;
             if (dm >= 1 && dm <= dlim && deg <= 4 && d <= 4) dmax_d[dm] = d;
         }
         for (const auto &kv : dmax_d) {
@@ -2176,16 +2208,14 @@ int bp_polish_fp64(bp_context *c, int nsweeps, double *out_delta) {
         c->cur64 = 0;
         c->fp64_active = true;
     }
-    // FP64 sweeps on the FP64 vertex kernel when it covers the networks (bond dimensions
-    // 6..8 in FP32/BF16 contexts), else on the generic kernel
+    // FP64 sweeps on an FP64 vertex kernel when one covers the networks (largest bond
+    // dimensions 6..8 or 9..16 in FP32/BF16 contexts), else on the generic kernel
     if (c->normalize) RETCHK(ensure_v64(c));
     const bool v64 = c->normalize && c->v64_ok && c->path != BP_PATH_GENERIC;
     for (int i = 0; i < nsweeps; ++i) {
         if (v64) {
-            if (vertex64_sweep(&c->v64, c->stream, c->d_msg_off, c->d_msg64[c->cur64],
-                               c->d_msg64[c->cur64 ^ 1], make_ctl(c, false), c->d_res_fid,
-                               c->d_res_abs) != 0)
-                return set_err(BP_ERR_HIP, "FP64 vertex sweep launch failed");
+            RETCHK(v64_sweep(c, c->d_msg64[c->cur64], c->d_msg64[c->cur64 ^ 1],
+                             make_ctl(c, false)));
         } else {
             RETCHK(launch_update_generic<double>(c, c->d_sites64, c->d_msg64[c->cur64],
                                                  c->d_msg64[c->cur64 ^ 1], make_ctl(c, false), 0,
