@@ -31,7 +31,7 @@
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 static constexpr int V64L_ELEMS = 65536;  // 16^4 per slice
-static constexpr int V64L_CHUNKS = 32;      // partial-message chunks per slice and port
+static constexpr int V64L_CHUNKS = 16;      // partial-message chunks per slice and port
 
 struct Vertex64LPlan {
     std::string why;
@@ -41,9 +41,9 @@ struct Vertex64LPlan {
     double *d_sites = nullptr;  // [item][s][16^4] padded slices, natural port order
     double *d_x = nullptr;      // E01 of every slice
     double *d_y = nullptr;      // E23 of every slice
-    double *d_part = nullptr;   // [item][s][chunk 32][port][256] partial messages (ports 0, 1:
-                                // one chunk per (p2, half of p3); ports 2, 3: chunks 0..3, one
-                                // per quarter of p0)
+    double *d_part = nullptr;   // [item][s][chunk 16][port][256] partial messages (ports 0, 1:
+                                // one chunk per p2; ports 2, 3: chunks 0..3, one per quarter
+                                // of p0)
 };
 
 static void vertex64l_release(Vertex64LPlan *p) {
@@ -353,25 +353,14 @@ __global__ __launch_bounds__(256) void k_v64l_pass2(V64LArgs a) {
     v64l_store_partials(red, acc2, acc3, P + 2 * 256, P + 3 * 256);
 }
 
-// Half slabs of pass 3: [p0][p1][8 of p3] with rows padded to V64L_HP, so that two
-// workgroups per (slice, p2) split the columns and four fit a compute unit's LDS.
-static constexpr int V64L_HP = 9;
-static constexpr int V64L_HSLAB = 256 * V64L_HP;
-
-__device__ __forceinline__ void v64l_hslab_load(double *slab, const double *g, int p2, int h) {
-    for (int i = threadIdx.x; i < 2048; i += blockDim.x) {
-        const int row = i >> 3, c = i & 7;  // row = p0 * 16 + p1
-        slab[row * V64L_HP + c] = g[row * 256 + p2 * 16 + 8 * h + c];
-    }
-}
-
-// pass 3: out0, out1 per column (p2, p3) of E23, one workgroup of 8 waves per (slice, p2,
-// half h) with the half slabs of E23 and A in LDS, wave w takes p3 = 8h + w.
+// pass 3: out0, out1 per column (p2, p3) of E23, one workgroup of 8 waves per (slice, p2)
+// with the slabs of E23 and A in LDS, wave w takes p3 = 2w, 2w + 1.  (Half slabs, two
+// workgroups per (slice, p2) at four per compute unit, measured 670 against 701 us at 24x24
+// D=16 but fetched 2.4 GB instead of 1.2 GB: each half row drew a whole 128-byte line.)
 __global__ __launch_bounds__(512) void k_v64l_pass3(V64LArgs a) {
-    extern __shared__ double v64l_sm3[];
-    double *slab_e = v64l_sm3, *slab_a = v64l_sm3 + V64L_HSLAB;
-    const int us = blockIdx.x >> 5, p2 = (blockIdx.x >> 1) & 15, h = blockIdx.x & 1;
-    const int item = us / a.d;
+    extern __shared__ double v64l_sm3[];  // two slabs (dynamic: more than 64 KiB)
+    double *slab_e = v64l_sm3, *slab_a = v64l_sm3 + V64L_SLAB;
+    const int us = blockIdx.x >> 4, p2 = blockIdx.x & 15, item = us / a.d;
     int4 in4, dim4;
     if (!v64l_begin(a, item, in4, dim4)) return;
     double m0[4], m1[4];
@@ -380,21 +369,23 @@ __global__ __launch_bounds__(512) void k_v64l_pass3(V64LArgs a) {
     const int w = threadIdx.x >> 6;
     const double *A = a.sites + (size_t)us * V64L_ELEMS;
     const double *Y = a.y + (size_t)us * V64L_ELEMS;
-    v64l_hslab_load(slab_e, Y, p2, h);
-    v64l_hslab_load(slab_a, A, p2, h);
+    v64l_slab_load(slab_e, Y, p2);
+    v64l_slab_load(slab_a, A, p2);
     __syncthreads();
     v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-    {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int p3 = 2 * w + k;
         double Er[4], Ec[4], Ar[4], Ac[4];
-        v64l_row(slab_e + w, 16 * V64L_HP, V64L_HP, Er);
-        v64l_col(slab_e + w, 16 * V64L_HP, V64L_HP, Ec);
-        v64l_row(slab_a + w, 16 * V64L_HP, V64L_HP, Ar);
-        v64l_col(slab_a + w, 16 * V64L_HP, V64L_HP, Ac);
+        v64l_row(slab_e + p3, 16 * V64L_SP, V64L_SP, Er);
+        v64l_col(slab_e + p3, 16 * V64L_SP, V64L_SP, Ec);
+        v64l_row(slab_a + p3, 16 * V64L_SP, V64L_SP, Ar);
+        v64l_col(slab_a + p3, 16 * V64L_SP, V64L_SP, Ac);
         v64l_open_last(Er, Ar, m1, acc0);   // out0: open over p0 of E23 x1 m1
         v64l_open_first(Ec, Ac, m0, acc1);  // out1: open over p1 of E23 x0 m0
     }
     __syncthreads();  // the slabs are no longer read: reuse them for the reduction
-    double *P = a.part + ((size_t)us * V64L_CHUNKS + p2 * 2 + h) * 4 * 256;
+    double *P = a.part + ((size_t)us * V64L_CHUNKS + p2) * 4 * 256;
     v64l_store_partials(v64l_sm3, acc0, acc1, P + 0 * 256, P + 1 * 256);
 }
 
@@ -429,7 +420,7 @@ __global__ __launch_bounds__(256) void k_v64l_finish(const double *__restrict__ 
         return;
     }
     double x = 0.0;
-    const int nch = j < 2 ? V64L_CHUNKS : 4;  // pass 3: per (p2, half); pass 2: per quarter
+    const int nch = j < 2 ? V64L_CHUNKS : 4;  // pass 3: one per p2; pass 2: one per quarter
     for (int s = 0; s < d; ++s)
         for (int c = 0; c < nch; ++c)
             x += part[(((size_t)(item * d + s) * V64L_CHUNKS + c) * 4 + j) * 256 + e];
@@ -456,10 +447,10 @@ static int vertex64l_sweep(const Vertex64LPlan *p, hipStream_t st, const long lo
     V64LArgs a{p->d_sites, p->d_x, p->d_y, p->d_part, src, p->d_vin, p->d_vdim, p->d_vnet,
                msg_off, ctl, p->d};
     const int ns = p->nv * p->d;
-    const size_t lds = sizeof(double) * 2 * V64L_SLAB, lds3 = sizeof(double) * 2 * V64L_HSLAB;
+    const size_t lds = sizeof(double) * 2 * V64L_SLAB;
     hipLaunchKernelGGL(k_v64l_pass1, dim3(ns * 16), dim3(512), lds, st, a);
     hipLaunchKernelGGL(k_v64l_pass2, dim3(ns * 4), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(k_v64l_pass3, dim3(ns * 32), dim3(512), lds3, st, a);
+    hipLaunchKernelGGL(k_v64l_pass3, dim3(ns * 16), dim3(512), lds, st, a);
     hipLaunchKernelGGL(k_v64l_finish, dim3(p->nv * 4), dim3(256), 0, st, p->d_part, src, dst,
                        p->d_vout, p->d_vdim, p->d_vnet, msg_off, ctl, p->d, res_fid, res_abs);
     return hipGetLastError() == hipSuccess ? 0 : 2;
