@@ -713,8 +713,7 @@ static int ensure_v64(bp_context *c) {
             if (vertex64l_plan_build(&c->v64l, c->plain, c->nv, c->port_off.data(),
                                      c->port_edge.data(), c->edge_dim.data(),
                                      c->phys_dim.data(), c->port_rev.data(), c->vnet.data(),
-                                     lmask) == 0 &&
-                vertex64l_set_lds_limit() == 0)
+                                     lmask) == 0)
                 c->v64_kind = 2;
         }
         c->v64_plan_ok = c->v64_kind != 0;

@@ -11,9 +11,11 @@
 //   pass 3, per column (p2, p3):  out0 += open of (E23 col m1) with A col,
 //                                 out1 += open of (m0^T E23 col) with A col
 // which is 8 mode products and 4 open contractions per slice (25 MFLOP at D = 16), A read
-// three times and E01, E23 written and read once.  Each pass is one launch: one workgroup of
-// four waves per (slice, quarter of the outer index), a wave per 16 tiles; the per-wave
-// partial sums of the opens meet in LDS and go to a per-(slice, quarter) buffer that the
+// three times and E01, E23 written and read once.  Each pass is one launch.  Passes 1 and 3
+// run one workgroup of 8 waves per (slice, p2) and read their column tiles from a slab
+// [p0][p1][p3] staged in LDS, so that HBM sees whole 128-byte rows; pass 2 runs one
+// workgroup of 4 waves per (slice, quarter of p0), a wave per 16 blocks.  The per-wave
+// partial sums of the opens meet in LDS and go to a per-(slice, chunk) buffer that the
 // finishing kernel sums in a fixed order (no atomics: the result does not depend on
 // scheduling), normalises (beliefpropagation.jl:248-253) and compares with the previous
 // message (fidelity and max-abs residuals).
@@ -119,9 +121,6 @@ static int vertex64l_plan_build(Vertex64LPlan *p, int plain, int nv, const int *
     return 0;
 }
 
-// The slab kernels take two padded slabs of dynamic LDS (69.6 KiB); raise their limit once.
-static int vertex64l_set_lds_limit();
-
 // Padded FP64 slices [item][s][16^4], natural port order (missing ports: index 0 only).
 __global__ void k_vertex64l_pad_sites(const double *sites64, const long long *site_off,
                                       const int *port_off, const int *port_edge,
@@ -207,6 +206,19 @@ __device__ __forceinline__ v4d v64l_sandwich(const double (&Xr)[4], const double
     return e;
 }
 
+// The two halves of an open contraction: lead = sum_t m[t] (x) X[t] (the product of a message
+// with a tile, as the B operand of the next product), close: acc += T lead.
+__device__ __forceinline__ v4d v64l_lead(const double (&m)[4], const double (&X)[4]) {
+    v4d g = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) g = v64l_mf(m[t], X[t], g);
+    return g;
+}
+__device__ __forceinline__ void v64l_close(const double (&T)[4], const v4d &g, v4d &acc) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc = v64l_mf(T[t], g[t], acc);
+}
+
 // acc[r] += out[a = li][a' = lk + 4r] of the open over the tile's first index of (X m_l)
 // against the A tile: out[a][a'] = sum_b (X m_l)[a][b] A[a'][b].
 __device__ __forceinline__ void v64l_open_last(const double (&Xr)[4], const double (&Ar)[4],
@@ -290,10 +302,12 @@ __device__ __forceinline__ void v64l_slab_load(double *slab, const double *g, in
 }
 
 // pass 1: E01 columns, one workgroup of 8 waves per (slice, p2), wave w takes p3 = 2w, 2w + 1.
-// Column (p2, p3) of the slab: element (p0, p1) at (p0 * 16 + p1) * V64L_SP + p3.
+// Column (p2, p3) of the slab: element (p0, p1) at (p0 * 16 + p1) * V64L_SP + p3.  One slab
+// of LDS (34 KiB, four workgroups per compute unit): the column tiles are read into
+// registers, then the slab takes the E01 columns for a coalesced store (342 us per launch
+// at 24x24 D=16, against 396 with a second slab for the results).
 __global__ __launch_bounds__(512) void k_v64l_pass1(V64LArgs a) {
-    extern __shared__ double v64l_sm1[];  // two slabs (dynamic: more than 64 KiB)
-    double *slab = v64l_sm1, *outs = v64l_sm1 + V64L_SLAB;
+    __shared__ double slab[V64L_SLAB];
     const int us = blockIdx.x >> 4, p2 = blockIdx.x & 15, item = us / a.d;
     int4 in4, dim4;
     if (!v64l_begin(a, item, in4, dim4)) return;
@@ -305,23 +319,27 @@ __global__ __launch_bounds__(512) void k_v64l_pass1(V64LArgs a) {
     double *X = a.x + (size_t)us * V64L_ELEMS;
     v64l_slab_load(slab, A, p2);
     __syncthreads();
+    double R[2][4];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) v64l_row(slab + 2 * w + k, 16 * V64L_SP, V64L_SP, R[k]);
+    __syncthreads();  // every tile is in registers: the slab takes the results
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int p3 = 2 * w + k;
-        double R[4];
-        v64l_row(slab + p3, 16 * V64L_SP, V64L_SP, R);
-        const v4d e = v64l_sandwich(R, m0, m1);
+        const v4d e = v64l_sandwich(R[k], m0, m1);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) outs[((lk + 4 * r) * 16 + li) * V64L_SP + p3] = e[r];
+        for (int r = 0; r < 4; ++r) slab[((lk + 4 * r) * 16 + li) * V64L_SP + p3] = e[r];
     }
     __syncthreads();
     for (int i = threadIdx.x; i < 4096; i += blockDim.x) {
         const int row = i >> 4, p3 = i & 15;
-        X[row * 256 + p2 * 16 + p3] = outs[row * V64L_SP + p3];
+        X[row * 256 + p2 * 16 + p3] = slab[row * V64L_SP + p3];
     }
 }
 
 // pass 2: out2, out3 and E23, per block (p0, p1) = 256 contiguous entries [p2][p3].
+// (Loading block p1 + 1 into registers while block p1 is contracted, at four waves per SIMD
+// instead of three, measured 406 against 401 us at 24x24 D=16.)
 __global__ __launch_bounds__(256) void k_v64l_pass2(V64LArgs a) {
     __shared__ double red[8 * 256];
     const int us = blockIdx.x >> 2, c4 = blockIdx.x & 3, item = us / a.d;
@@ -353,13 +371,15 @@ __global__ __launch_bounds__(256) void k_v64l_pass2(V64LArgs a) {
     v64l_store_partials(red, acc2, acc3, P + 2 * 256, P + 3 * 256);
 }
 
-// pass 3: out0, out1 per column (p2, p3) of E23, one workgroup of 8 waves per (slice, p2)
-// with the slabs of E23 and A in LDS, wave w takes p3 = 2w, 2w + 1.  (Half slabs, two
-// workgroups per (slice, p2) at four per compute unit, measured 670 against 701 us at 24x24
-// D=16 but fetched 2.4 GB instead of 1.2 GB: each half row drew a whole 128-byte line.)
-__global__ __launch_bounds__(512) void k_v64l_pass3(V64LArgs a) {
-    extern __shared__ double v64l_sm3[];  // two slabs (dynamic: more than 64 KiB)
-    double *slab_e = v64l_sm3, *slab_a = v64l_sm3 + V64L_SLAB;
+// pass 3: out0, out1 per column (p2, p3) of E23, one workgroup of 8 waves per (slice, p2),
+// wave w takes p3 = 2w, 2w + 1.  One slab of LDS (34 KiB): the E23 slab is staged and its
+// tiles' first products made, then the slab takes A for the second products, then the
+// reduction.  At 8 waves per SIMD (64 registers, one spilled) four workgroups share a
+// compute unit: 569 us per launch at 24x24 D=16, against 634 at 6 waves per SIMD and 701
+// with both slabs resident (two workgroups per compute unit).  Half slabs, two workgroups
+// per (slice, p2), fetched twice the bytes: each half row drew a whole 128-byte line.
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_v64l_pass3(V64LArgs a) {
+    __shared__ double slab[V64L_SLAB];
     const int us = blockIdx.x >> 4, p2 = blockIdx.x & 15, item = us / a.d;
     int4 in4, dim4;
     if (!v64l_begin(a, item, in4, dim4)) return;
@@ -369,24 +389,32 @@ __global__ __launch_bounds__(512) void k_v64l_pass3(V64LArgs a) {
     const int w = threadIdx.x >> 6;
     const double *A = a.sites + (size_t)us * V64L_ELEMS;
     const double *Y = a.y + (size_t)us * V64L_ELEMS;
-    v64l_slab_load(slab_e, Y, p2);
-    v64l_slab_load(slab_a, A, p2);
+    v64l_slab_load(slab, Y, p2);
+    __syncthreads();
+    v4d g0[2], g1[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        double Er[4], Ec[4];
+        v64l_row(slab + 2 * w + k, 16 * V64L_SP, V64L_SP, Er);
+        v64l_col(slab + 2 * w + k, 16 * V64L_SP, V64L_SP, Ec);
+        g0[k] = v64l_lead(m1, Er);  // (E23 x1 m1)^T
+        g1[k] = v64l_lead(m0, Ec);  // m0^T E23
+    }
+    __syncthreads();
+    v64l_slab_load(slab, A, p2);
     __syncthreads();
     v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-        const int p3 = 2 * w + k;
-        double Er[4], Ec[4], Ar[4], Ac[4];
-        v64l_row(slab_e + p3, 16 * V64L_SP, V64L_SP, Er);
-        v64l_col(slab_e + p3, 16 * V64L_SP, V64L_SP, Ec);
-        v64l_row(slab_a + p3, 16 * V64L_SP, V64L_SP, Ar);
-        v64l_col(slab_a + p3, 16 * V64L_SP, V64L_SP, Ac);
-        v64l_open_last(Er, Ar, m1, acc0);   // out0: open over p0 of E23 x1 m1
-        v64l_open_first(Ec, Ac, m0, acc1);  // out1: open over p1 of E23 x0 m0
+        double Ar[4], Ac[4];
+        v64l_row(slab + 2 * w + k, 16 * V64L_SP, V64L_SP, Ar);
+        v64l_col(slab + 2 * w + k, 16 * V64L_SP, V64L_SP, Ac);
+        v64l_close(Ar, g0[k], acc0);  // out0: open over p0
+        v64l_close(Ac, g1[k], acc1);  // out1: open over p1
     }
-    __syncthreads();  // the slabs are no longer read: reuse them for the reduction
+    __syncthreads();  // the slab is no longer read: reuse it for the reduction
     double *P = a.part + ((size_t)us * V64L_CHUNKS + p2) * 4 * 256;
-    v64l_store_partials(v64l_sm3, acc0, acc1, P + 0 * 256, P + 1 * 256);
+    v64l_store_partials(slab, acc0, acc1, P + 0 * 256, P + 1 * 256);
 }
 
 // Finish, one workgroup of 256 threads per (item, port): entry e = (a, a') summed over the
@@ -447,24 +475,11 @@ static int vertex64l_sweep(const Vertex64LPlan *p, hipStream_t st, const long lo
     V64LArgs a{p->d_sites, p->d_x, p->d_y, p->d_part, src, p->d_vin, p->d_vdim, p->d_vnet,
                msg_off, ctl, p->d};
     const int ns = p->nv * p->d;
-    const size_t lds = sizeof(double) * 2 * V64L_SLAB;
-    hipLaunchKernelGGL(k_v64l_pass1, dim3(ns * 16), dim3(512), lds, st, a);
+    hipLaunchKernelGGL(k_v64l_pass1, dim3(ns * 16), dim3(512), 0, st, a);
     hipLaunchKernelGGL(k_v64l_pass2, dim3(ns * 4), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(k_v64l_pass3, dim3(ns * 16), dim3(512), lds, st, a);
+    hipLaunchKernelGGL(k_v64l_pass3, dim3(ns * 16), dim3(512), 0, st, a);
     hipLaunchKernelGGL(k_v64l_finish, dim3(p->nv * 4), dim3(256), 0, st, p->d_part, src, dst,
                        p->d_vout, p->d_vdim, p->d_vnet, msg_off, ctl, p->d, res_fid, res_abs);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
-static int vertex64l_set_lds_limit() {
-    static int done = 0;
-    if (done) return 0;
-    const int lds = (int)(sizeof(double) * 2 * V64L_SLAB);
-    if (hipFuncSetAttribute((const void *)k_v64l_pass1, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            lds) != hipSuccess ||
-        hipFuncSetAttribute((const void *)k_v64l_pass3, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            lds) != hipSuccess)
-        return 2;
-    done = 1;
-    return 0;
-}
