@@ -420,37 +420,31 @@ __global__ __launch_bounds__(256) void k_v16_finish(
 
 // Padded BF16 site entries of the bucket: vsites16[item][s][p0][p1][p2][p3] (16 each), and
 // vsites16t with the last two ports swapped; missing ports keep index 0 only, bond indices
-// beyond D are zero.
-__global__ void k_v16_pad_sites(const double *sites64, const long long *site_off,
-                                const int *port_off, const int *port_edge, const int *edge_dim,
-                                const int *vlist, int d, int nv, unsigned short *out,
-                                unsigned short *out_t) {
-    const long long total = (long long)nv * d * V16_ELEMS;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
-         i += (long long)gridDim.x * blockDim.x) {
-        const int e = (int)(i & (V16_ELEMS - 1));
-        const long long us = i >> 16;
-        const int s = (int)(us % d);
-        const int u = vlist[us / d];
-        const int k = port_off[u + 1] - port_off[u];
-        const int idx[4] = {(e >> 12) & 15, (e >> 8) & 15, (e >> 4) & 15, e & 15};
-        long long off = 0, cnt = 1;
-        bool real = true;
-        for (int j = 0; j < 4; ++j) {
-            if (j < k) {
-                const int D = edge_dim[port_edge[port_off[u] + j]];
-                if (idx[j] >= D) real = false;
-                off = off * D + idx[j];
-                cnt *= D;
-            } else if (idx[j] != 0) {
-                real = false;
-            }
-        }
+// beyond D are zero.  One workgroup per (slice, p0): the vertex's bond dimensions are read
+// once (missing ports count as dimension 1), then the 4096 entries [p1][p2][p3] follow.
+__global__ __launch_bounds__(256) void k_v16_pad_sites(const double *sites64,
+                                                       const long long *site_off,
+                                                       const int *port_off, const int *port_edge,
+                                                       const int *edge_dim, const int *vlist,
+                                                       int d, unsigned short *out,
+                                                       unsigned short *out_t) {
+    const int us = blockIdx.x >> 4, p0 = blockIdx.x & 15;
+    const int s = us % d, u = vlist[us / d];
+    const int k = port_off[u + 1] - port_off[u];
+    int D[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) D[j] = j < k ? edge_dim[port_edge[port_off[u] + j]] : 1;
+    const long long n = (long long)D[0] * D[1] * D[2] * D[3];
+    const double *src = sites64 + site_off[u] + s * n;
+    const size_t base = (size_t)us * V16_ELEMS + p0 * 4096;
+    for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
+        const int p1 = e >> 8, p2 = (e >> 4) & 15, p3 = e & 15;
+        const bool real = p0 < D[0] && p1 < D[1] && p2 < D[2] && p3 < D[3];
         const unsigned short v =
-            real ? bf16_rne((float)sites64[site_off[u] + s * cnt + off]) : (unsigned short)0;
-        out[i] = v;
-        // the copy with the last two ports swapped: [p0][p1][p3][p2]
-        out_t[(i & ~(long long)0xff) | ((e & 15) << 4) | ((e >> 4) & 15)] = v;
+            real ? bf16_rne((float)src[((p0 * D[1] + p1) * D[2] + p2) * D[3] + p3])
+                 : (unsigned short)0;
+        out[base + e] = v;
+        out_t[base + (p1 << 8) + (p3 << 4) + p2] = v;  // [p0][p1][p3][p2]
     }
 }
 
@@ -486,8 +480,8 @@ static int vertex16_upload_sites(VertexPlan *p, hipStream_t st, const double *d_
     if (p->d_v16sites) (void)hipFree(p->d_v16sites);
     p->d_v16sites = nullptr;
     if (hipMalloc((void **)&p->d_v16sites, 2 * bytes) != hipSuccess) return 5;
-    hipLaunchKernelGGL(k_v16_pad_sites, dim3(4096), dim3(256), 0, st, d_sites64, d_site_off,
-                       d_port_off, d_port_edge, d_edge_dim, p->d_v16list, p->d, p->nv16,
+    hipLaunchKernelGGL(k_v16_pad_sites, dim3(p->nv16 * p->d * 16), dim3(256), 0, st, d_sites64,
+                       d_site_off, d_port_off, d_port_edge, d_edge_dim, p->d_v16list, p->d,
                        p->d_v16sites, p->d_v16sites + bytes / 2);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }

@@ -122,32 +122,28 @@ static int vertex64l_plan_build(Vertex64LPlan *p, int plain, int nv, const int *
 }
 
 // Padded FP64 slices [item][s][16^4], natural port order (missing ports: index 0 only).
-__global__ void k_vertex64l_pad_sites(const double *sites64, const long long *site_off,
-                                      const int *port_off, const int *port_edge,
-                                      const int *edge_dim, const int *vlist, int d, int nv,
-                                      double *out) {
-    const long long total = (long long)nv * d * V64L_ELEMS;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
-         i += (long long)gridDim.x * blockDim.x) {
-        const int e = (int)(i & (V64L_ELEMS - 1));
-        const long long us = i >> 16;
-        const int s = (int)(us % d);
-        const int u = vlist[us / d];
-        const int k = port_off[u + 1] - port_off[u];
-        const int idx[4] = {(e >> 12) & 15, (e >> 8) & 15, (e >> 4) & 15, e & 15};
-        long long off = 0, n = 1;
-        bool real = true;
-        for (int j = 0; j < 4 && real; ++j) {
-            if (j < k) {
-                const int D = edge_dim[port_edge[port_off[u] + j]];
-                if (idx[j] >= D) real = false;
-                off = off * D + idx[j];
-                n *= D;
-            } else if (idx[j] != 0) {
-                real = false;
-            }
-        }
-        out[i] = real ? sites64[site_off[u] + s * n + off] : 0.0;
+// One workgroup per (slice, p0): the vertex's bond dimensions are read once, then the 4096
+// entries [p1][p2][p3] are written contiguously (missing ports count as dimension 1).
+__global__ __launch_bounds__(256) void k_vertex64l_pad_sites(const double *sites64,
+                                                             const long long *site_off,
+                                                             const int *port_off,
+                                                             const int *port_edge,
+                                                             const int *edge_dim,
+                                                             const int *vlist, int d,
+                                                             double *out) {
+    const int us = blockIdx.x >> 4, p0 = blockIdx.x & 15;
+    const int s = us % d, u = vlist[us / d];
+    const int k = port_off[u + 1] - port_off[u];
+    int D[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) D[j] = j < k ? edge_dim[port_edge[port_off[u] + j]] : 1;
+    const long long n = (long long)D[0] * D[1] * D[2] * D[3];
+    const double *src = sites64 + site_off[u] + s * n;
+    double *dst = out + (size_t)us * V64L_ELEMS + p0 * 4096;
+    for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
+        const int p1 = e >> 8, p2 = (e >> 4) & 15, p3 = e & 15;
+        const bool real = p0 < D[0] && p1 < D[1] && p2 < D[2] && p3 < D[3];
+        dst[e] = real ? src[((p0 * D[1] + p1) * D[2] + p2) * D[3] + p3] : 0.0;
     }
 }
 
@@ -159,8 +155,9 @@ static int vertex64l_upload_sites(Vertex64LPlan *p, hipStream_t st, const double
     p->d_sites = nullptr;
     if (n == 0) return 0;
     if (hipMalloc((void **)&p->d_sites, sizeof(double) * n) != hipSuccess) return 5;
-    hipLaunchKernelGGL(k_vertex64l_pad_sites, dim3(4096), dim3(256), 0, st, d_sites64, d_site_off,
-                       d_port_off, d_port_edge, d_edge_dim, p->d_vlist, p->d, p->nv, p->d_sites);
+    hipLaunchKernelGGL(k_vertex64l_pad_sites, dim3(p->nv * p->d * 16), dim3(256), 0, st,
+                       d_sites64, d_site_off, d_port_off, d_port_edge, d_edge_dim, p->d_vlist,
+                       p->d, p->d_sites);
     return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
