@@ -629,6 +629,8 @@ def main(argv=None):
     if args.warmup > 0:
         eng.run(maxiter=args.warmup, use_graph=True)
     eng.init_messages("identity")
+    # capture the sweep graphs the timed run replays (no sweep runs), outside the timed region
+    eng.prepare_run(args.steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     res = eng.run(maxiter=args.steps, use_graph=True)

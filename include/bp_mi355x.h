@@ -136,6 +136,14 @@ int bp_sweep(bp_context *ctx);
 int bp_run(bp_context *ctx, int maxiter, double tol, int criterion, int use_graph,
            int *out_sweeps, double *out_delta, int *out_at_iteration);
 
+/* Prepares a later bp_run(ctx, maxiter, ..., use_graph != 0): allocates what the sweeps
+ * need and captures the sweep graphs that run replays (16-sweep chunks and the last,
+ * maxiter % 16 sweeps, each from either message buffer), running no sweep.  No counterpart
+ * in the reference (its solve has no setup step to hoist): for callers that time runs or
+ * start them in latency-sensitive loops.  A later change that invalidates them (topology,
+ * options, kernel choices) makes bp_run capture again. */
+int bp_prepare_run(bp_context *ctx, int maxiter);
+
 /* Sequential (Gauss-Seidel) update of one message slot in place, normalised. */
 int bp_update_slot(bp_context *ctx, int slot);
 

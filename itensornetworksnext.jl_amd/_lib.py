@@ -41,6 +41,7 @@ SIGNATURES = {
     "bp_get_path": (C.c_int, [_vp, _ip]),
     "bp_sweep": (C.c_int, [_vp]),
     "bp_run": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int, _ip, _dp, _ip]),
+    "bp_prepare_run": (C.c_int, [_vp, C.c_int]),
     "bp_update_slot": (C.c_int, [_vp, C.c_int]),
     "bp_run_sequential": (C.c_int, [_vp, _ip, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _dp,
                                     _ip]),

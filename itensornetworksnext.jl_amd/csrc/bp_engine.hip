@@ -336,9 +336,11 @@ struct bp_context {
     // while graph_epoch equals the process-wide plan epoch (which moves whenever a captured
     // pointer or kernel choice may have changed)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    // a run's last, shorter chunk (maxiter % 16 sweeps), captured by bp_prepare_run only
+    hipGraphExec_t gtail[2] = {nullptr, nullptr};
+    int gtail_n[2] = {0, 0};
     unsigned graph_epoch = 0;
     RunParams *d_runp = nullptr;
-    RunParams runp_host{};
 
     // generic-kernel buckets of the resolved path (rebuilt when the path or plan changes)
     std::vector<GenBucket> gb;
@@ -408,6 +410,11 @@ static void graphs_release(bp_context *c) {
     for (auto &g : c->gexec) {
         if (g) (void)hipGraphExecDestroy(g);
         g = nullptr;
+    }
+    for (int i = 0; i < 2; ++i) {
+        if (c->gtail[i]) (void)hipGraphExecDestroy(c->gtail[i]);
+        c->gtail[i] = nullptr;
+        c->gtail_n[i] = 0;
     }
     c->graph_epoch = 0;
 }
@@ -1458,6 +1465,73 @@ int bp_sweep(bp_context *c) {
     return BP_OK;
 }
 
+// Sweeps per captured chunk of bp_run (between host checks of the done flags with a
+// tolerance); a multiple of FOLD_BUFS, so every chunk starts on accumulator residue 0.
+static constexpr int RUN_CHUNK = 16;
+static_assert(RUN_CHUNK % FOLD_BUFS == 0, "a captured sweep chunk must span whole accumulator cycles");
+
+// Capture and instantiate n sweeps (each with its bookkeeping) starting at the current
+// parity into *out.  A chunk starts on accumulator residue 0 (runs queue whole chunks of
+// 16, a multiple of FOLD_BUFS), so sweep i of the chunk is captured as sweep i.  Capture
+// executes nothing: the parity is restored, and each replay flips it n times.
+static int capture_sweeps(bp_context *c, const NetCtl &ctl, int n, hipGraphExec_t *out) {
+    hipGraph_t graph = nullptr;
+    const int cur0 = c->cur;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = BP_OK;
+    bool folded = false;
+    for (int i = 0; i < n && rc == BP_OK; ++i) {
+        rc = sweep_once(c, ctl, &folded, i, i > 0);
+        if (rc == BP_OK && !folded) rc = launch_reduce(c);
+    }
+    if (rc == BP_OK && folded) rc = launch_reduce_folded(c, n - 1);
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    c->cur = cur0;
+    if (rc != BP_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return set_err(BP_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    const hipError_t ie = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        *out = nullptr;
+        return set_err(BP_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
+    }
+    c->graph_epoch = g_epoch.load();
+    return BP_OK;
+}
+
+int bp_prepare_run(bp_context *c, int maxiter) {
+    RETCHK(ready(c));
+    if (maxiter < 0) return set_err(BP_ERR_ARG, "maxiter must be >= 0");
+    HIPCHK(hipSetDevice(c->device));
+    RETCHK(prepare_sweep(c));
+    if (c->graph_epoch != g_epoch.load()) graphs_release(c);
+    const NetCtl ctl = make_ctl(c, true);
+    const int tail = maxiter % RUN_CHUNK;
+    const int cur0 = c->cur;
+    for (int par = 0; par < 2; ++par) {  // a run may start on either message buffer
+        c->cur = par;
+        int rc = BP_OK;
+        if (maxiter >= RUN_CHUNK && !c->gexec[par])
+            rc = capture_sweeps(c, ctl, RUN_CHUNK, &c->gexec[par]);
+        if (rc == BP_OK && tail > 0 && c->gtail_n[par] != tail) {
+            if (c->gtail[par]) (void)hipGraphExecDestroy(c->gtail[par]);
+            c->gtail[par] = nullptr;
+            c->gtail_n[par] = 0;
+            rc = capture_sweeps(c, ctl, tail, &c->gtail[par]);
+            if (rc == BP_OK) c->gtail_n[par] = tail;
+        }
+        if (rc != BP_OK) {
+            c->cur = cur0;
+            return rc;
+        }
+    }
+    c->cur = cur0;
+    return BP_OK;
+}
+
 int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph, int *out_sweeps,
            double *out_delta, int *out_at) {
     RETCHK(ready(c));
@@ -1469,70 +1543,46 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
     c->fp64_active = false;
     RETCHK(prepare_sweep(c));
     const int cap = maxiter > 0 ? maxiter : 100000;  // tol-only runs are capped
+    // The run state (delta, done, iter, at: one block) comes back through a pinned buffer,
+    // which also stages the run's parameters (an asynchronous copy from pinned memory).
+    const int nn = c->nnet;
+    const size_t sbytes = run_state_bytes(nn);
+    const size_t poff = (sbytes + 63) & ~(size_t)63;
+    if (c->h_state_bytes < poff + sizeof(RunParams)) {
+        if (c->h_state) (void)hipHostFree(c->h_state);
+        c->h_state = nullptr;
+        c->h_state_bytes = 0;
+        HIPCHK(hipHostMalloc((void **)&c->h_state, poff + sizeof(RunParams), hipHostMallocDefault));
+        c->h_state_bytes = poff + sizeof(RunParams);
+    }
     {
-        RunParams hp;
-        hp.maxiter = cap;
-        hp.crit = criterion;
-        hp.tol = tol;
-        c->runp_host = hp;  // stays alive until the copy below has run (context lifetime)
-        HIPCHK(hipMemcpyAsync(c->d_runp, &c->runp_host, sizeof(RunParams),
-                              hipMemcpyHostToDevice, c->stream));
+        RunParams *hp = reinterpret_cast<RunParams *>(c->h_state + poff);
+        hp->maxiter = cap;  // the previous run has synchronised: the buffer is free
+        hp->crit = criterion;
+        hp->tol = tol;
+        HIPCHK(hipMemcpyAsync(c->d_runp, hp, sizeof(RunParams), hipMemcpyHostToDevice, c->stream));
     }
     hipLaunchKernelGGL(k_reset_nets, dim3((c->nnet + 255) / 256), dim3(256), 0, c->stream,
                        c->nnet, c->d_done, c->d_iter, c->d_delta, c->d_at, c->d_fold_acc,
                        c->fold_n);
     const NetCtl ctl = make_ctl(c, true);
     HIPCHK(hipGetLastError());
-    // sweeps between host checks of the done flags; a captured chunk bakes in the folded
-    // accumulator indices k % FOLD_BUFS, so every replay must start on residue 0
-    constexpr int chunk = 16;
-    static_assert(chunk % FOLD_BUFS == 0, "a captured sweep chunk must span whole accumulator cycles");
-    // The run state (delta, done, iter, at: one block) comes back through a pinned buffer.
-    // With a tolerance the host reads it after every chunk and stops once every network has
-    // stopped; without one no network stops before maxiter, so every chunk is queued at once
-    // and the state is read once at the end (no idle gaps between chunks).
-    const int nn = c->nnet;
-    const size_t sbytes = run_state_bytes(nn);
-    if (c->h_state_bytes < sbytes) {
-        if (c->h_state) (void)hipHostFree(c->h_state);
-        c->h_state = nullptr;
-        c->h_state_bytes = 0;
-        HIPCHK(hipHostMalloc((void **)&c->h_state, sbytes, hipHostMallocDefault));
-        c->h_state_bytes = sbytes;
-    }
+    // a captured chunk bakes in the folded accumulator indices k % FOLD_BUFS, so every
+    // replay must start on residue 0
+    constexpr int chunk = RUN_CHUNK;
+    // With a tolerance the host reads the run state after every chunk and stops once every
+    // network has stopped; without one no network stops before maxiter, so every chunk is
+    // queued at once and the state is read once at the end (no idle gaps between chunks).
     const bool check_chunks = tol >= 0.0;
     if (c->graph_epoch != g_epoch.load()) graphs_release(c);
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     int launched = 0;
     while (launched < cap) {
         const int n = std::min(chunk, cap - launched);
-        if (use_graph && n == chunk) {
-            hipGraphExec_t &gexec = c->gexec[c->cur];
-            if (!gexec) {
-                hipGraph_t graph = nullptr;
-                const int cur0 = c->cur;
-                HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                int rc = BP_OK;
-                bool folded = false;
-                for (int i = 0; i < n && rc == BP_OK; ++i) {
-                    rc = sweep_once(c, ctl, &folded, launched + i, i > 0);
-                    if (rc == BP_OK && !folded) rc = launch_reduce(c);
-                }
-                if (rc == BP_OK && folded) rc = launch_reduce_folded(c, launched + n - 1);
-                hipError_t e = hipStreamEndCapture(c->stream, &graph);
-                if (rc != BP_OK) return rc;
-                if (e != hipSuccess) return set_err(BP_ERR_HIP, "graph capture failed: %s",
-                                                    hipGetErrorString(e));
-                const hipError_t ie = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(graph);
-                if (ie != hipSuccess) {
-                    gexec = nullptr;
-                    return set_err(BP_ERR_HIP, "graph instantiate failed: %s",
-                                   hipGetErrorString(ie));
-                }
-                c->graph_epoch = g_epoch.load();
-                c->cur = cur0;  // capture does not execute; replay flips parity n times
-            }
+        const bool tail = use_graph && n < chunk && c->gtail[c->cur] && c->gtail_n[c->cur] == n;
+        if (use_graph && (n == chunk || tail)) {
+            hipGraphExec_t &gexec = tail ? c->gtail[c->cur] : c->gexec[c->cur];
+            if (!gexec) RETCHK(capture_sweeps(c, ctl, n, &gexec));
             HIPCHK(hipGraphLaunch(gexec, c->stream));
             if (n % 2) c->cur ^= 1;
         } else {

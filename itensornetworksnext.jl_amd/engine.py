@@ -180,6 +180,11 @@ class Engine:
         L.check(self.lib.bp_last_run_ms(self.h, C.byref(ms)), "bp_last_run_ms")
         return RunResult(sw, dl, at, ms.value)
 
+    def prepare_run(self, maxiter: int) -> None:
+        """Capture the sweep graphs that `run(maxiter, use_graph=True)` replays (chunks of 16
+        and the last, shorter chunk, from either message buffer) without running a sweep."""
+        L.check(self.lib.bp_prepare_run(self.h, int(maxiter)), "bp_prepare_run")
+
     def run_sequential(self, order, maxiter: Optional[int] = None, tol: Optional[float] = None,
                        criterion: str = "fidelity") -> RunResult:
         """Gauss-Seidel sweeps over `order` (message slots), reference semantics."""

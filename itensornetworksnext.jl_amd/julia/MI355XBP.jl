@@ -401,6 +401,18 @@ function autotune!(ctx::DeviceContext; cache::AbstractString = "", mode::Integer
     return Int(n[])
 end
 
+"""
+    prepare_run!(ctx, maxiter)
+
+Capture the sweep graphs a later `maxiter`-sweep run on `ctx` replays, without running a
+sweep (bp_prepare_run), so that loops timing or repeating runs start them at once.
+"""
+function prepare_run!(ctx::DeviceContext, maxiter::Integer)
+    check(ccall((:bp_prepare_run, libbp[]), Cint, (Ptr{Cvoid}, Cint), ctx.handle, maxiter),
+        "bp_prepare_run")
+    return ctx
+end
+
 # ChainRulesCore rrules: the pullback of a converged-BP quantity with respect to the flat
 # site buffer is its adjoint gradient (implicit differentiation of the fixed point).  Each
 # rrule converges BP once and takes the value and the gradient from one library call.

@@ -54,3 +54,29 @@ def test_graph_reused_across_runs(pkg, dtype):
         assert np.array_equal(r.sweeps, rw.sweeps)
         assert np.array_equal(got, want)
     e.close()
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_prepared_run_tail_graphs(pkg, dtype):
+    """bp_prepare_run captures the 16-sweep graphs and the tail chunk (maxiter % 16) from
+    both message buffers without running a sweep (messages unchanged); runs afterwards
+    (from either buffer, with and without a tolerance) equal eager runs bit for bit."""
+    _say(f"case prepared run {dtype}")
+    t = pkg.build_topology(pkg.named_grid((6, 5)), 6, 2)
+    s = pkg.peps.random_sites(t, 52, scale=pkg.peps.peps_scale(2, 6))
+    e = pkg.Engine(t, s, dtype=dtype)
+    e.init_messages("identity")
+    m0 = e.messages()
+    e.prepare_run(20)
+    assert np.array_equal(e.messages(), m0)
+    runs = [dict(maxiter=20), dict(maxiter=5), dict(maxiter=20),
+            dict(maxiter=36, tol=1e-30, criterion="maxabs"), dict(maxiter=20, tol=1e-6)]
+    for k, kw in enumerate(runs):
+        start = e.messages()
+        r = e.run(use_graph=True, **kw)
+        got = e.messages()
+        rw, want = _fresh_run(pkg, t, s, dtype, "auto", start, kw)
+        _say(f"run {k} {kw} sweeps {r.sweeps.tolist()} fresh {rw.sweeps.tolist()}")
+        assert np.array_equal(r.sweeps, rw.sweeps) and np.array_equal(r.at_iteration, rw.at_iteration)
+        assert np.array_equal(got, want)
+    e.close()
