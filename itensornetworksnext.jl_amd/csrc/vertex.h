@@ -73,8 +73,11 @@ struct VertexPlan {
     int *d_v16net = nullptr, *d_v16list = nullptr;
     unsigned short *d_v16sites = nullptr;  // BF16: padded slices, 16^4 entries each
     float *d_v12sites = nullptr;           // FP32: padded slices, 12^4 entries each (2 layouts)
+    float *d_v10sites = nullptr;           // FP32: the same at side 10 (items [0, nv16_k10))
     int nv16_k4 = 0;                       // FP32: items [0, nv16_k4) have degree 4, the
-                                           // rest degree <= 3 (k_vertex_d12_k3)
+                                           // rest degree <= 3 (k_vertex_k3<12>)
+    int nv16_k10 = 0;                      // FP32: degree-4 items [0, nv16_k10) have bond
+                                           // dimensions <= 10 (k_vertex_d12<10>)
     float *d_v3sites = nullptr;            // FP32, degree <= 3: 12^3 slices (3 layouts)
     // FP32 networks of bond dimension <= 8 with every vertex of degree <= 3 (heavy-hex
     // lattices, trees): the last nv_k3 items of the D <= 8 bucket, swept by k_vertex_k3<8>
@@ -104,17 +107,17 @@ static void vertex_plan_release(VertexPlan *p) {
     if (p->d_vin) (void)hipFree(p->d_vin);
     if (p->d_vout) (void)hipFree(p->d_vout);
     if (p->d_vdim) (void)hipFree(p->d_vdim);
-    void *v16[11] = {p->d_v16in, p->d_v16out, p->d_v16dim, p->d_v16net, p->d_v16list,
-                     p->d_v16sites, p->d_v16part, p->d_v12sites, p->d_v3sites,
+    void *v16[12] = {p->d_v16in, p->d_v16out, p->d_v16dim, p->d_v16net, p->d_v16list,
+                     p->d_v16sites, p->d_v16part, p->d_v12sites, p->d_v10sites, p->d_v3sites,
                      p->d_vk3sites, p->d_vk3part};
     for (void *x : v16)
         if (x) (void)hipFree(x);
     p->d_v16in = p->d_v16out = p->d_v16dim = nullptr;
     p->d_v16net = p->d_v16list = nullptr;
     p->d_v16sites = nullptr;
-    p->d_v12sites = nullptr;
+    p->d_v12sites = p->d_v10sites = nullptr;
     p->d_v3sites = nullptr;
-    p->nv16_k4 = 0;
+    p->nv16_k4 = p->nv16_k10 = 0;
     p->d_vk3sites = p->d_vk3part = nullptr;
     p->nv_k3 = 0;
     p->d_v16part = nullptr;
@@ -232,14 +235,25 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
         vdim[b] = make_int4(dm[0], dm[1], dm[2], dm[3]);
         vn[b] = vnet[u];
     }
-    // the large-bond bucket, laid out the same way; in FP32 the degree-4 items first, then
-    // the degree <= 3 items (their own kernel)
-    int n16_k4 = (int)vlist16.size();
+    // the large-bond bucket, laid out the same way; in FP32 the degree-4 items first (those
+    // whose bond dimensions are all <= 10 leading, padded to 10), then the degree <= 3 items
+    // (their own kernel)
+    int n16_k4 = (int)vlist16.size(), n16_k10 = 0;
     if (dtype == 1) {
+        auto deg4 = [&](int u) { return port_off[u + 1] - port_off[u] == 4; };
+        auto le10 = [&](int u) {
+            for (int q = port_off[u]; q < port_off[u + 1]; ++q)
+                if (edge_dim[port_edge[q]] > 10) return false;
+            return true;
+        };
         std::stable_partition(vlist16.begin(), vlist16.end(),
-                              [&](int u) { return port_off[u + 1] - port_off[u] == 4; });
-        n16_k4 = 0;
-        for (int u : vlist16) n16_k4 += port_off[u + 1] - port_off[u] == 4;
+                              [&](int u) { return deg4(u) && le10(u); });
+        std::stable_partition(vlist16.begin(), vlist16.end(), deg4);
+        n16_k4 = n16_k10 = 0;
+        for (int u : vlist16) {
+            n16_k4 += deg4(u);
+            n16_k10 += deg4(u) && le10(u);
+        }
     }
     const int n16 = (int)vlist16.size();
     std::vector<int4> vin16(n16), vout16(n16), vdim16(n16);
@@ -314,6 +328,7 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     p->nv = nx;
     p->nv16 = n16;
     p->nv16_k4 = n16_k4;
+    p->nv16_k10 = n16_k10;
     p->nv_k3 = nk3;
     p->gen_slots = gen_slots;
     p->ngen = (int)gen_slots.size();

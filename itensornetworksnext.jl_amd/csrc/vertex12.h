@@ -1,9 +1,14 @@
-// vertex12.h -- FP32 MFMA sweep for bond dimensions 9..12 (padded to 12) on gfx950.
+// vertex12.h -- FP32 MFMA sweep for bond dimensions 9..12 (padded to 12, or to 10 for
+// vertices whose bond dimensions are all <= 10) on gfx950.
 //
 // The FP32 counterpart of vertex16.h: one workgroup of 12 waves per (vertex, physical slice
 // s), the slice A[s] (four ports of dimension 12, 12^4 entries) in LDS as FP32 (82 KiB),
 // with rows of 144 (p2, p3) entries padded to 145 and blocks of 12 rows padded by one entry,
-// so that the stride along every port is odd or 12.
+// so that the stride along every port is odd or 12.  Side 10 is the same kernel with 10
+// waves, 10^4 entries (40 KiB, two workgroups per compute unit instead of one), 7 fibre
+// tiles (the last one partly idle) and the contracted index stopping at 10.  (Dealing the
+// tiles over 16 waves at side 12, or 8 at side 10, with the tile indices computed at run
+// time instead of unrolled per wave, measured 20-25% slower.)
 //
 // Mode product on port J ("T x_J m_J", in place) runs on v_mfma_f32_16x16x4_f32: A operand
 // = m_J transposed (lane l holds m_J[4 kc + l/16][l%16], kc = 0..2 for the 12 values of the
@@ -32,16 +37,20 @@
 
 #include "vertex.h"
 
-static constexpr int V12_ELEMS = 20736;  // 12^4 per slice (HBM copies, unpadded)
-static constexpr int V12_NW = 12;        // waves per slice workgroup
-// LDS strides of (p0, p1, p2, p3)
-static constexpr int V12_S0 = 1741, V12_S1 = 145, V12_S2 = 12, V12_S3 = 1;
-static constexpr int V12_LDS_ELEMS = 12 * V12_S0;
-static constexpr size_t v12_lds() { return sizeof(float) * (V12_LDS_ELEMS + V12_NW * 256); }
-
-__host__ __device__ constexpr int v12_str(int j) {
-    return j == 0 ? V12_S0 : (j == 1 ? V12_S1 : (j == 2 ? V12_S2 : V12_S3));
-}
+// Layout of a padded degree-4 slice of side DP (12, or 10 for items whose bond dimensions
+// are all <= 10) in LDS: rows of DP^2 (p2, p3) entries padded by one, blocks of DP rows
+// padded by one, so that the stride along every port is odd or DP.  One wave per value of
+// a tile's first other port: wave w owns the FT fibre tiles of that value.
+template <int DP> struct VQ {
+    static constexpr int ELEMS = DP * DP * DP * DP;  // per slice (HBM copies, unpadded)
+    static constexpr int NW = DP;                    // waves per slice workgroup
+    static constexpr int S3 = 1, S2 = DP, S1 = DP * DP + 1, S0 = DP * S1 + 1;
+    static constexpr int LDS_ELEMS = DP * S0;
+    static constexpr int FT = (DP * DP + 15) / 16;   // fibre tiles of 16 over two ports
+    static constexpr int PARK = (LDS_ELEMS + 64 * NW - 1) / (64 * NW);
+    static constexpr size_t lds() { return sizeof(float) * (LDS_ELEMS + NW * 256); }
+    static constexpr int str(int j) { return j == 0 ? S0 : (j == 1 ? S1 : (j == 2 ? S2 : S3)); }
+};
 
 // The other three ports of J in ascending order: tiles are indexed by P (one per wave),
 // fibres by (Q, R) flattened.
@@ -51,90 +60,104 @@ template <int J> struct V12Ports {
     static constexpr int R = J == 3 ? 2 : 3;
 };
 
-// Mode product on port J of the slice in LDS, in place.  Am: the message as A operand.
-template <int J>
+// Mode product on port J of the slice in LDS, in place.  Am: the message as A operand (three
+// chunks of 4 of the contracted index; entries beyond the bond dimension are zero).  At
+// DP = 10 the last fibre tile is partly idle (100 = 6 x 16 + 4) and the contracted index
+// stops at 10: idle lanes and indices read zeros and store nothing.
+template <int DP, int J>
 __device__ __forceinline__ void v12_mode(float *S, const float (&Am)[3]) {
+    using C = VQ<DP>;
     using PT = V12Ports<J>;
-    constexpr int SJ = v12_str(J), SP = v12_str(PT::P), SQ = v12_str(PT::Q),
-                  SR = v12_str(PT::R);
+    constexpr int SJ = C::str(J), SP = C::str(PT::P), SQ = C::str(PT::Q), SR = C::str(PT::R);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = lane & 15, kk = lane >> 4;
     const v4f zero = {0.f, 0.f, 0.f, 0.f};
     const int pbase = w * SP;
 #pragma unroll
-    for (int jf = 0; jf < 9; ++jf) {
-        const int f = 16 * jf + n;  // fibre over (Q, R): 0..143
-        const int q = f / 12, r = f - 12 * q;
+    for (int jf = 0; jf < C::FT; ++jf) {
+        const int f = 16 * jf + n;  // fibre over (Q, R): 0..DP^2 - 1
+        const bool live = f < DP * DP;
+        const int q = f / DP, r = f - DP * q;
         const int base = pbase + q * SQ + r * SR;
         float b[3];
 #pragma unroll
-        for (int kc = 0; kc < 3; ++kc) b[kc] = S[base + (4 * kc + kk) * SJ];
+        for (int kc = 0; kc < 3; ++kc) {
+            const int k = 4 * kc + kk;
+            b[kc] = (live && k < DP) ? S[base + k * SJ] : 0.f;
+        }
         v4f c = zero;
 #pragma unroll
         for (int kc = 0; kc < 3; ++kc)
             c = __builtin_amdgcn_mfma_f32_16x16x4f32(Am[kc], b[kc], c, 0, 0, 0);
-        if (kk < 3) {
+        if (live) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) S[base + (4 * kk + t) * SJ] = c[t];
+            for (int t = 0; t < 4; ++t)
+                if (4 * kk + t < DP) S[base + (4 * kk + t) * SJ] = c[t];
         }
     }
 }
 
-// Bra operands of this wave's 36 MFMA steps of the open contraction on port O: step
-// (jm, t) of lane (n, kk) takes K = (hi = w, lo = 16 jm + 4 kk + t), a' = n.
-template <int O>
+// Bra operands of this wave's MFMA steps of the open contraction on port O: step (jm, t) of
+// lane (n, kk) takes K = (hi = w, lo = 16 jm + 4 kk + t), a' = n (zero for lo >= DP^2).
+template <int DP, int O>
 __device__ __forceinline__ void v12_bra_prefetch(const float *natural, const float *swapped,
-                                                 float4 (&pre)[9]) {
+                                                 float4 (&pre)[VQ<DP>::FT]) {
+    constexpr int D2 = DP * DP, D3 = DP * DP * DP;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = lane & 15, kk = lane >> 4;
     const float *src = O < 2 ? natural : swapped;
-    const int row = (O == 0 || O == 2) ? n * 1728 + 144 * w : w * 1728 + n * 144;
+    const int row = (O == 0 || O == 2) ? n * D3 + D2 * w : w * D3 + n * D2;
 #pragma unroll
-    for (int jm = 0; jm < 9; ++jm) {
-        if (n < 12) pre[jm] = *reinterpret_cast<const float4 *>(src + row + 16 * jm + 4 * kk);
+    for (int jm = 0; jm < VQ<DP>::FT; ++jm) {
+        const int lo = 16 * jm + 4 * kk;
+        if (n < DP && lo < D2) pre[jm] = *reinterpret_cast<const float4 *>(src + row + lo);
         else pre[jm] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
 }
 
 // This wave's share of out_O[a][a'] = sum_K G[a, K] A[a', K]; lane l returns entries
 // (4(l/16)+i, l%16).  K = (hi, lo) maps to ports as: O = 0: (p1, p2, p3), O = 1: (p0, p2,
-// p3), O = 2: (p3, p0, p1), O = 3: (p2, p0, p1), with lo = 12 x + y over the last two.
-template <int O>
-__device__ __forceinline__ v4f v12_open(const float *S, const float4 (&pre)[9]) {
+// p3), O = 2: (p3, p0, p1), O = 3: (p2, p0, p1), with lo = DP x + y over the last two.
+template <int DP, int O>
+__device__ __forceinline__ v4f v12_open(const float *S, const float4 (&pre)[VQ<DP>::FT]) {
+    using C = VQ<DP>;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = lane & 15, kk = lane >> 4;
-    const bool live = n < 12;
+    const bool live = n < DP;
     const int a = live ? n : 0;
     int base;
-    if constexpr (O == 0) base = a * V12_S0 + w * V12_S1;
-    else if constexpr (O == 1) base = w * V12_S0 + a * V12_S1;
-    else if constexpr (O == 2) base = a * V12_S2 + w * V12_S3;
-    else base = w * V12_S2 + a * V12_S3;
+    if constexpr (O == 0) base = a * C::S0 + w * C::S1;
+    else if constexpr (O == 1) base = w * C::S0 + a * C::S1;
+    else if constexpr (O == 2) base = a * C::S2 + w * C::S3;
+    else base = w * C::S2 + a * C::S3;
     v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int jm = 0; jm < 9; ++jm) {
+    for (int jm = 0; jm < C::FT; ++jm) {
         const float pv[4] = {pre[jm].x, pre[jm].y, pre[jm].z, pre[jm].w};
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int lo = 16 * jm + 4 * kk + t;
-            const int x = lo / 12, y = lo - 12 * x;
-            const int off = O < 2 ? x * V12_S2 + y * V12_S3 : x * V12_S0 + y * V12_S1;
-            const float g = live ? S[base + off] : 0.f;
+            const int x = lo / DP, y = lo - DP * x;
+            const int off = O < 2 ? x * C::S2 + y * C::S3 : x * C::S0 + y * C::S1;
+            const float g = (live && lo < DP * DP) ? S[base + off] : 0.f;
             acc = __builtin_amdgcn_mfma_f32_16x16x4f32(g, pv[t], acc, 0, 0, 0);
         }
     }
     return acc;
 }
 
-// Copy the slice's natural-layout entries into the padded LDS layout (16-byte loads).
+// Copy the slice's natural-layout entries into the padded LDS layout (16-byte loads; DP^2
+// is a multiple of 4, so no load straddles two rows).
+template <int DP>
 __device__ __forceinline__ void v12_load_slice(float *S, const float *src) {
+    using C = VQ<DP>;
     const float4 *g = reinterpret_cast<const float4 *>(src);
-    for (int c = threadIdx.x; c < V12_ELEMS / 4; c += 64 * V12_NW) {
+    for (int c = threadIdx.x; c < C::ELEMS / 4; c += 64 * C::NW) {
         const float4 v = g[c];
         const int e = 4 * c;
-        const int row = e / 144, col = e - 144 * row;  // row = 12 p0 + p1, col = 12 p2 + p3
-        const int p0 = row / 12, p1 = row - 12 * p0;
-        float *d = S + p0 * V12_S0 + p1 * V12_S1 + col;
+        const int row = e / (DP * DP), col = e - DP * DP * row;  // row = DP p0 + p1
+        const int p0 = row / DP, p1 = row - DP * p0;
+        float *d = S + p0 * C::S0 + p1 * C::S1 + col;
         d[0] = v.x;
         d[1] = v.y;
         d[2] = v.z;
@@ -142,101 +165,108 @@ __device__ __forceinline__ void v12_load_slice(float *S, const float *src) {
     }
 }
 
-template <int J>
+template <int DP, int J>
 __device__ __forceinline__ void v12_mode_if(float *S, const int (&ins)[4],
                                             const float (&Am)[4][3]) {
     if (ins[J] < 0) return;  // identity message on a padded port (uniform)
-    v12_mode<J>(S, Am[J]);
+    v12_mode<DP, J>(S, Am[J]);
     __syncthreads();
 }
 
 // Open on port O with the bra operands in `pre`, reduce the waves' shares through LDS,
 // store the unnormalised 16 x 16 block.
-template <int O>
+template <int DP, int O>
 __device__ __forceinline__ void v12_open_store(const float *S, float *red, float *dst,
-                                               const float4 (&pre)[9]) {
+                                               const float4 (&pre)[VQ<DP>::FT]) {
+    constexpr int NW = VQ<DP>::NW;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n = lane & 15, kk = lane >> 4;
-    const v4f acc = v12_open<O>(S, pre);
+    const v4f acc = v12_open<DP, O>(S, pre);
 #pragma unroll
     for (int i = 0; i < 4; ++i) red[w * 256 + (4 * kk + i) * 16 + n] = acc[i];
     __syncthreads();
-    for (int e = threadIdx.x; e < 256; e += 64 * V12_NW) {
+    for (int e = threadIdx.x; e < 256; e += 64 * NW) {
         float x = 0.f;
 #pragma unroll
-        for (int v = 0; v < V12_NW; ++v) x += red[v * 256 + e];
+        for (int v = 0; v < NW; ++v) x += red[v * 256 + e];
         dst[O * 256 + e] = x;
     }
 }
 
-// Raw LDS words of the slice buffer held by this thread while the buffer is reused
-// (20892 words over 768 threads: 28 each, the last partly).
-static constexpr int V12_PARK = (V12_LDS_ELEMS + 64 * V12_NW - 1) / (64 * V12_NW);
-
-__device__ __forceinline__ void v12_park(const float *S, float (&pk)[V12_PARK]) {
+// Raw LDS words of the slice buffer held by this thread while the buffer is reused (DP = 12:
+// 20892 words over 768 threads, 28 each; DP = 10: 10110 over 640, 16 each).
+template <int DP>
+__device__ __forceinline__ void v12_park(const float *S, float (&pk)[VQ<DP>::PARK]) {
+    using C = VQ<DP>;
 #pragma unroll
-    for (int i = 0; i < V12_PARK; ++i) {
-        const int a = threadIdx.x + 64 * V12_NW * i;
-        pk[i] = a < V12_LDS_ELEMS ? S[a] : 0.f;
+    for (int i = 0; i < C::PARK; ++i) {
+        const int a = threadIdx.x + 64 * C::NW * i;
+        pk[i] = a < C::LDS_ELEMS ? S[a] : 0.f;
     }
 }
 
-__device__ __forceinline__ void v12_unpark(float *S, const float (&pk)[V12_PARK]) {
+template <int DP>
+__device__ __forceinline__ void v12_unpark(float *S, const float (&pk)[VQ<DP>::PARK]) {
+    using C = VQ<DP>;
 #pragma unroll
-    for (int i = 0; i < V12_PARK; ++i) {
-        const int a = threadIdx.x + 64 * V12_NW * i;
-        if (a < V12_LDS_ELEMS) S[a] = pk[i];
+    for (int i = 0; i < C::PARK; ++i) {
+        const int a = threadIdx.x + 64 * C::NW * i;
+        if (a < C::LDS_ELEMS) S[a] = pk[i];
     }
 }
 
 // Outgoing ports O1 and O2 of the slice from one shared partial product: reload the slice,
 // absorb ports X0, X1 (E23 for O = 0, 1; E01 for O = 2, 3), keep the partial in registers,
 // then absorb Y1 and open O1, restore the partial, absorb Y2 and open O2.
-template <int O1, int O2, int X0, int X1, int Y1, int Y2>
+template <int DP, int O1, int O2, int X0, int X1, int Y1, int Y2>
 __device__ __forceinline__ void v12_pair(float *S, float *red, const float *src,
                                          const float *src_w, float *dst, const int (&ins)[4],
                                          const int (&outs)[4], const float (&Am)[4][3]) {
     const bool first = outs[O1] >= 0, second = outs[O2] >= 0;  // uniform
     if (!first && !second) return;
-    float4 pre[9];
-    if (first) v12_bra_prefetch<O1>(src, src_w, pre);
-    else v12_bra_prefetch<O2>(src, src_w, pre);
+    float4 pre[VQ<DP>::FT];
+    if (first) v12_bra_prefetch<DP, O1>(src, src_w, pre);
+    else v12_bra_prefetch<DP, O2>(src, src_w, pre);
     __syncthreads();  // earlier opens have finished with the slice and the partials
-    v12_load_slice(S, src);
+    v12_load_slice<DP>(S, src);
     __syncthreads();
-    v12_mode_if<X0>(S, ins, Am);
-    v12_mode_if<X1>(S, ins, Am);
-    float pk[V12_PARK];
+    v12_mode_if<DP, X0>(S, ins, Am);
+    v12_mode_if<DP, X1>(S, ins, Am);
+    float pk[VQ<DP>::PARK];
     if (first && second) {
-        v12_park(S, pk);
+        v12_park<DP>(S, pk);
         __syncthreads();
     }
     if (first) {
-        v12_mode_if<Y1>(S, ins, Am);
-        v12_open_store<O1>(S, red, dst, pre);
+        v12_mode_if<DP, Y1>(S, ins, Am);
+        v12_open_store<DP, O1>(S, red, dst, pre);
     }
     if (second) {
         if (first) {
-            v12_bra_prefetch<O2>(src, src_w, pre);
-            v12_unpark(S, pk);  // the open above has passed its barrier
+            v12_bra_prefetch<DP, O2>(src, src_w, pre);
+            v12_unpark<DP>(S, pk);  // the open above has passed its barrier
             __syncthreads();
         }
-        v12_mode_if<Y2>(S, ins, Am);
-        v12_open_store<O2>(S, red, dst, pre);
+        v12_mode_if<DP, Y2>(S, ins, Am);
+        v12_open_store<DP, O2>(S, red, dst, pre);
     }
 }
 
-// grid: items x d (one workgroup per slice), 768 threads.  part[((item * d + s) * 4 + j) *
-// 256 + a * 16 + a'] receives slice s's unnormalised out_j (the layout k_v16_finish reads).
-__global__ __launch_bounds__(64 * V12_NW) void k_vertex_d12(
+// grid: items x d (one workgroup per slice of items i0 .. i0 + grid / d - 1), 64 DP threads.
+// vsites_n / vsites_w hold those items' padded slices from item i0 on.  part[((item * d + s)
+// * 4 + j) * 256 + a * 16 + a'] receives slice s's unnormalised out_j (the layout
+// k_v16_finish reads).
+template <int DP>
+__global__ __launch_bounds__(64 * DP) void k_vertex_d12(
     const float *__restrict__ vsites_n, const float *__restrict__ vsites_w,
     const float *__restrict__ mold, const int4 *__restrict__ vin, const int4 *__restrict__ vout,
     const int4 *__restrict__ vdim, const int *__restrict__ vnet,
-    const long long *__restrict__ msg_off, NetCtl ctl, int d, float *__restrict__ part) {
+    const long long *__restrict__ msg_off, NetCtl ctl, int d, int i0, float *__restrict__ part) {
+    using C = VQ<DP>;
     extern __shared__ __align__(16) float v12sm[];
     float *S = v12sm;
-    float *red = v12sm + V12_LDS_ELEMS;
-    const int item = blockIdx.x / d, s = blockIdx.x - (blockIdx.x / d) * d;
+    float *red = v12sm + C::LDS_ELEMS;
+    const int il = blockIdx.x / d, s = blockIdx.x - il * d, item = i0 + il;
     const int lane = threadIdx.x & 63;
     const int n = lane & 15, kk = lane >> 4;
     if (ctl.enabled && ctl.done[vnet[item]]) return;  // k_v16_finish carries the messages
@@ -256,44 +286,41 @@ __global__ __launch_bounds__(64 * V12_NW) void k_vertex_d12(
                                                          : 0.f;
         }
     }
-    const long long so = ((long long)item * d + s) * V12_ELEMS;
+    const long long so = ((long long)il * d + s) * C::ELEMS;
     const float *src = vsites_n + so, *src_w = vsites_w + so;
     float *dst = part + ((long long)item * d + s) * 4 * 256;
     // E23 serves outgoing ports 0 (absorb 1) and 1 (absorb 0); E01 serves 2 and 3
-    v12_pair<0, 1, 2, 3, 1, 0>(S, red, src, src_w, dst, ins, outs, Am);
-    v12_pair<2, 3, 0, 1, 3, 2>(S, red, src, src_w, dst, ins, outs, Am);
+    v12_pair<DP, 0, 1, 2, 3, 1, 0>(S, red, src, src_w, dst, ins, outs, Am);
+    v12_pair<DP, 2, 3, 0, 1, 3, 2>(S, red, src, src_w, dst, ins, outs, Am);
 }
 
-// Padded FP32 site entries of the bucket: vsites_n[item][s][p0][p1][p2][p3] (12 each) and
-// vsites_w[item][s][p2][p3][p0][p1]; missing ports keep index 0 only, bond indices beyond D
-// are zero.
-__global__ void k_v12_pad_sites(const double *sites64, const long long *site_off,
-                                const int *port_off, const int *port_edge, const int *edge_dim,
-                                const int *vlist, int d, int nv, float *out_n, float *out_w) {
-    const long long total = (long long)nv * d * V12_ELEMS;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
-         i += (long long)gridDim.x * blockDim.x) {
-        const int e = (int)(i % V12_ELEMS);
-        const long long us = i / V12_ELEMS;
-        const int s = (int)(us % d);
-        const int u = vlist[us / d];
-        const int k = port_off[u + 1] - port_off[u];
-        const int idx[4] = {e / 1728, (e / 144) % 12, (e / 12) % 12, e % 12};
-        long long off = 0, cnt = 1;
-        bool real = true;
-        for (int j = 0; j < 4; ++j) {
-            if (j < k) {
-                const int D = edge_dim[port_edge[port_off[u] + j]];
-                if (idx[j] >= D) real = false;
-                off = off * D + idx[j];
-                cnt *= D;
-            } else if (idx[j] != 0) {
-                real = false;
-            }
-        }
-        const float v = real ? (float)sites64[site_off[u] + s * cnt + off] : 0.f;
-        out_n[i] = v;
-        out_w[us * V12_ELEMS + ((idx[2] * 12 + idx[3]) * 12 + idx[0]) * 12 + idx[1]] = v;
+// Padded FP32 site entries of DP-side items: vsites_n[item][s][p0][p1][p2][p3] (DP each)
+// and vsites_w[item][s][p2][p3][p0][p1]; missing ports keep index 0 only, bond indices
+// beyond D are zero.  One workgroup per (slice, p0): the vertex's bond dimensions are read
+// once (missing ports count as dimension 1).
+template <int DP>
+__global__ __launch_bounds__(256) void k_v12_pad_sites(const double *sites64,
+                                                       const long long *site_off,
+                                                       const int *port_off, const int *port_edge,
+                                                       const int *edge_dim, const int *vlist,
+                                                       int d, float *out_n, float *out_w) {
+    constexpr int D2 = DP * DP, D3 = DP * DP * DP;
+    const int us = blockIdx.x / DP, p0 = blockIdx.x - DP * us;
+    const int s = us % d, u = vlist[us / d];
+    const int k = port_off[u + 1] - port_off[u];
+    int D[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) D[j] = j < k ? edge_dim[port_edge[port_off[u] + j]] : 1;
+    const long long cnt = (long long)D[0] * D[1] * D[2] * D[3];
+    const double *src = sites64 + site_off[u] + s * cnt;
+    float *dn = out_n + (size_t)us * VQ<DP>::ELEMS + p0 * D3;
+    float *dw = out_w + (size_t)us * VQ<DP>::ELEMS;
+    for (int e = threadIdx.x; e < D3; e += blockDim.x) {
+        const int p1 = e / D2, r = e - D2 * p1, p2 = r / DP, p3 = r - DP * p2;
+        const bool real = p0 < D[0] && p1 < D[1] && p2 < D[2] && p3 < D[3];
+        const float v = real ? (float)src[((p0 * D[1] + p1) * D[2] + p2) * D[3] + p3] : 0.f;
+        dn[e] = v;
+        dw[((p2 * DP + p3) * DP + p0) * DP + p1] = v;
     }
 }
 

@@ -454,18 +454,28 @@ static int vertex16_upload_sites(VertexPlan *p, hipStream_t st, const double *d_
                                  const long long *d_site_off, const int *d_port_off,
                                  const int *d_port_edge, const int *d_edge_dim) {
     if (p->nv16 == 0) return 0;
-    if (!p->bf16) {  // FP32: the bond-dimension-12 kernels' layouts (degree 4, degree <= 3)
+    if (!p->bf16) {  // FP32: the bond-dimension-12 kernels' layouts (degree 4 at side 10 and
+                     // 12, degree <= 3)
+        const int n10 = p->nv16_k10, n12 = p->nv16_k4 - p->nv16_k10;
         const int n4 = p->nv16_k4, n3 = p->nv16 - p->nv16_k4;
         if (p->d_v12sites) (void)hipFree(p->d_v12sites);
+        if (p->d_v10sites) (void)hipFree(p->d_v10sites);
         if (p->d_v3sites) (void)hipFree(p->d_v3sites);
-        p->d_v12sites = nullptr;
+        p->d_v12sites = p->d_v10sites = nullptr;
         p->d_v3sites = nullptr;
-        if (n4 > 0) {
-            const size_t n = (size_t)n4 * p->d * V12_ELEMS;
+        if (n10 > 0) {
+            const size_t n = (size_t)n10 * p->d * VQ<10>::ELEMS;
+            if (hipMalloc((void **)&p->d_v10sites, 2 * n * sizeof(float)) != hipSuccess) return 5;
+            hipLaunchKernelGGL(k_v12_pad_sites<10>, dim3(n10 * p->d * 10), dim3(256), 0, st,
+                               d_sites64, d_site_off, d_port_off, d_port_edge, d_edge_dim,
+                               p->d_v16list, p->d, p->d_v10sites, p->d_v10sites + n);
+        }
+        if (n12 > 0) {
+            const size_t n = (size_t)n12 * p->d * VQ<12>::ELEMS;
             if (hipMalloc((void **)&p->d_v12sites, 2 * n * sizeof(float)) != hipSuccess) return 5;
-            hipLaunchKernelGGL(k_v12_pad_sites, dim3(4096), dim3(256), 0, st, d_sites64,
-                               d_site_off, d_port_off, d_port_edge, d_edge_dim, p->d_v16list,
-                               p->d, n4, p->d_v12sites, p->d_v12sites + n);
+            hipLaunchKernelGGL(k_v12_pad_sites<12>, dim3(n12 * p->d * 12), dim3(256), 0, st,
+                               d_sites64, d_site_off, d_port_off, d_port_edge, d_edge_dim,
+                               p->d_v16list + n10, p->d, p->d_v12sites, p->d_v12sites + n);
         }
         if (n3 > 0) {
             const size_t n = (size_t)n3 * p->d * 3 * V3<12>::ELEMS;
@@ -496,8 +506,12 @@ static int vertex16_set_lds_limit() {
         if (hipFuncSetAttribute(f[i], hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)v16_lds(nw[i])) != hipSuccess)
             return 2;
-    if (hipFuncSetAttribute((const void *)k_vertex_d12, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)v12_lds()) != hipSuccess)
+    if (hipFuncSetAttribute((const void *)k_vertex_d12<12>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)VQ<12>::lds()) != hipSuccess ||
+        hipFuncSetAttribute((const void *)k_vertex_d12<10>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)VQ<10>::lds()) != hipSuccess)
         return 2;
     done = 1;
     return 0;
@@ -508,14 +522,25 @@ static int vertex16_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void 
                           const NetCtl &ctl, double *res_fid, double *res_abs) {
     if (p->nv16 == 0) return 0;
     if (!p->bf16) {
+        const int n10 = p->nv16_k10, n12 = p->nv16_k4 - p->nv16_k10;
         const int n4 = p->nv16_k4, n3 = p->nv16 - p->nv16_k4;
-        if (n4 > 0) {
-            const size_t n = (size_t)n4 * p->d * V12_ELEMS;
-            hipLaunchKernelGGL(k_vertex_d12, dim3(n4 * p->d), dim3(64 * V12_NW), v12_lds(), st,
+        if (n10 > 0) {
+            const size_t n = (size_t)n10 * p->d * VQ<10>::ELEMS;
+            hipLaunchKernelGGL((k_vertex_d12<10>), dim3(n10 * p->d),
+                               dim3(64 * VQ<10>::NW), (VQ<10>::lds()), st,
+                               (const float *)p->d_v10sites,
+                               (const float *)(p->d_v10sites + n), (const float *)src,
+                               p->d_v16in, p->d_v16out, p->d_v16dim, p->d_v16net, tp.msg_off,
+                               ctl, p->d, 0, p->d_v16part);
+        }
+        if (n12 > 0) {
+            const size_t n = (size_t)n12 * p->d * VQ<12>::ELEMS;
+            hipLaunchKernelGGL((k_vertex_d12<12>), dim3(n12 * p->d),
+                               dim3(64 * VQ<12>::NW), (VQ<12>::lds()), st,
                                (const float *)p->d_v12sites,
                                (const float *)(p->d_v12sites + n), (const float *)src,
                                p->d_v16in, p->d_v16out, p->d_v16dim, p->d_v16net, tp.msg_off,
-                               ctl, p->d, p->d_v16part);
+                               ctl, p->d, n10, p->d_v16part);
         }
         if (n3 > 0) {
             const int nsl = n3 * p->d;

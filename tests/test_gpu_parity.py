@@ -484,9 +484,10 @@ def test_kernel_options_round_trip(pkg):
 # ---------------------------------------------------------------- bond dimensions 9..12 (FP32)
 
 @pytest.mark.parametrize("shape,D,d", [((4, 3), 12, 2), ((3, 3), 10, 2), ((3, 4), 9, 1),
-                                       ((3, 3), 12, 3), ((2, 5), 11, 4)])
+                                       ((3, 3), 12, 3), ((2, 5), 11, 4), ((3, 4), 10, 4)])
 def test_vertex12_single_sweep_fp32(pkg, orc, shape, D, d):
-    """The FP32 MFMA kernel for bond dimensions 9..12 (padded to 12): one sweep from random
+    """The FP32 MFMA kernel for bond dimensions 9..12 (padded to 12, or to 10 when every bond
+    dimension of the vertex is <= 10): one sweep from random
     non-symmetric messages within the FP32 error bound, and from identity messages within
     1e-5 relative of the FP64 oracle."""
     _say(f"case vertex12 single sweep {shape} D={D} d={d}")
@@ -588,6 +589,35 @@ def test_vertex12_mixed_dims_and_generic_agree(pkg, orc):
             res[path] = (e.messages(), e.lognorm())
     assert _rel(res["auto"][0], res["generic"][0]) < 1e-5
     assert res["auto"][1] == pytest.approx(res["generic"][1], rel=1e-6)
+
+
+def test_vertex12_side10_and_side12_in_one_network(pkg, orc):
+    """Degree-4 vertices whose bond dimensions are all <= 10 run at side 10, the rest at side
+    12, in the same sweep (a 5x5 lattice with D = 10 on every edge with an end in the two
+    left columns, D = 12 elsewhere, d = 3): one sweep within the FP32 bound, then converged
+    within 1e-4 (messages) and 1e-5 (log-norm) of the FP64 oracle."""
+    _say("case vertex12 side 10 and 12 in one network")
+    g = pkg.named_grid((5, 5))
+    dims = {e: 10 if min(e[0][1], e[1][1]) <= 2 else 12 for e in g.edges()}  # columns 1..5
+    t = pkg.build_topology(g, dims, 3)
+    s = pkg.peps.random_sites(t, 66, scale=pkg.peps.peps_scale(3, 11))
+    m = pkg.peps.random_messages(t, 66)
+    with pkg.Engine(t, s, dtype="f32") as e:
+        assert e.path() == "vertex"
+        e.set_messages(m)
+        e.sweep()
+        assert _fp32_sweep_check(pkg, orc, t, s, m, e.messages()) < 1.0
+    m0 = pkg.peps.identity_messages(t)
+    want, _, _, _ = orc.bp(t, s, m0, maxiter=400, tol=1e-12, criterion="maxabs", nthreads=8)
+    with pkg.Engine(t, s, dtype="f32") as e:
+        e.set_messages(m0)
+        r = e.run(maxiter=400, tol=1e-10, criterion="fidelity")
+        got, ln = e.messages(), e.lognorm()
+    assert r.converged[0]
+    lw = orc.bethe_free_energy(t, s, want)
+    _say(f"msg rel err {_rel(got, want):.2e}, log-norm rel err {abs(ln - lw) / abs(lw):.2e}")
+    assert _rel(got, want) < 1e-4
+    assert abs(ln - lw) / abs(lw) < 1e-5
 
 
 def test_vertex12_converged_fp32_vs_oracle(pkg, orc):
