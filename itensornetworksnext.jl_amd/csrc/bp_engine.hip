@@ -2138,7 +2138,19 @@ static int scalars(bp_context *c) {
         RETCHK(launch_vertex_scalars<cplx>(c, (const cplx *)c->d_sites64, (const cplx *)m));
         RETCHK(launch_edge_scalars<cplx>(c, (const cplx *)m));
     } else if (is64) {
-        RETCHK(launch_vertex_scalars<double>(c, c->d_sites64, (const double *)m));
+        // FP64 messages at largest bond dimensions 9..16: the large-bond kernel's first two
+        // passes (the plan the FP64 sweeps or the polish already use), else the generic kernel
+        bool done = false;
+        if (!c->plain && c->path != BP_PATH_GENERIC) {
+            RETCHK(ensure_v64(c));
+            if (c->v64_ok && c->v64_kind == 2) {
+                if (vertex64l_scalars(&c->v64l, c->stream, c->d_msg_off, (const double *)m,
+                                      c->d_zv) != 0)
+                    return set_err(BP_ERR_HIP, "FP64 vertex scalar launch failed");
+                done = true;
+            }
+        }
+        if (!done) RETCHK(launch_vertex_scalars<double>(c, c->d_sites64, (const double *)m));
         RETCHK(launch_edge_scalars<double>(c, (const double *)m));
     } else {
         RETCHK(launch_vertex_scalars<float>(c, (const float *)c->d_sites, (const float *)m));

@@ -464,6 +464,45 @@ __global__ __launch_bounds__(256) void k_v64l_finish(const double *__restrict__ 
     }
 }
 
+// Vertex scalars (messagecache.jl:139-143): Z_u = <out_2, m_in,2>, the unnormalised port-2
+// result of passes 1 and 2 (summed over the slices and the four chunks in a fixed order)
+// against the incoming port-2 message; a padded port 2 holds e0 e0^T, so the formula covers
+// every degree.  One workgroup of 256 threads per vertex, entry e = (a, a').
+__global__ __launch_bounds__(256) void k_v64l_zv(const double *__restrict__ part,
+                                                 const double *__restrict__ msgs,
+                                                 const int4 *__restrict__ vin,
+                                                 const int4 *__restrict__ vdim,
+                                                 const long long *__restrict__ msg_off, int d,
+                                                 double *__restrict__ zv) {
+    __shared__ double red[BP_BLOCK / 64 + 1];
+    const int item = blockIdx.x, e = threadIdx.x, a = e >> 4, ap = e & 15;
+    const int q = vin[item].z, D = vdim[item].z;
+    double m;
+    if (q < 0) m = e == 0 ? 1.0 : 0.0;
+    else m = (a < D && ap < D) ? msgs[msg_off[q] + a * D + ap] : 0.0;
+    double x = 0.0;
+    for (int s = 0; s < d; ++s)
+        for (int c = 0; c < 4; ++c)
+            x += part[(((size_t)(item * d + s) * V64L_CHUNKS + c) * 4 + 2) * 256 + e];
+    const double z = block_sum(x * m, red);
+    if (threadIdx.x == 0) zv[item] = z;
+}
+
+// Every vertex's scalar from the FP64 messages `msgs` into zv[vertex] (passes 1 and 2, then
+// k_v64l_zv; E23 is made and not used).
+static int vertex64l_scalars(const Vertex64LPlan *p, hipStream_t st, const long long *msg_off,
+                             const double *msgs, double *zv) {
+    if (p->nv == 0) return 0;
+    V64LArgs a{p->d_sites, p->d_x, p->d_y, p->d_part, msgs, p->d_vin, p->d_vdim, p->d_vnet,
+               msg_off, NetCtl{nullptr, 0}, p->d};
+    const int ns = p->nv * p->d;
+    hipLaunchKernelGGL(k_v64l_pass1, dim3(ns * 16), dim3(512), 0, st, a);
+    hipLaunchKernelGGL(k_v64l_pass2, dim3(ns * 4), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_v64l_zv, dim3(p->nv), dim3(256), 0, st, p->d_part, msgs, p->d_vin,
+                       p->d_vdim, msg_off, p->d, zv);
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
 // One FP64 sweep of every network: src -> dst (message buffers of the context's FP64 layout).
 static int vertex64l_sweep(const Vertex64LPlan *p, hipStream_t st, const long long *msg_off,
                            const double *src, double *dst, const NetCtl &ctl, double *res_fid,

@@ -122,3 +122,50 @@ def test_polish_matches_generic_fp64_sweeps(pkg):
         ref, lr = e.messages(), e.lognorm()
     assert _rel(pol, ref) < 1e-12
     assert ln == pytest.approx(lr, rel=1e-13)
+
+
+@pytest.mark.parametrize("kind,D,d", [("grid", 16, 2), ("comb", 11, 3), ("path", 9, 1)])
+def test_vertex_scalars_and_lognorm(pkg, orc, kind, D, d):
+    """FP64 vertex scalars from the large-bond kernel's passes 1 and 2 (Z_u = <out_2, m_2>,
+    e0 e0^T on a padded port 2: degrees 1..4) against the FP64 oracle and the generic FP64
+    kernel (1e-12 relative), and the log-norm against the oracle's bethe_free_energy."""
+    _say(f"case v64l vertex scalars {kind} D={D} d={d}")
+    g = {"grid": lambda: pkg.named_grid((3, 4)), "comb": lambda: pkg.named_comb_tree((3, 3)),
+         "path": lambda: pkg.named_path_graph(4)}[kind]()
+    t = pkg.build_topology(g, D, d)
+    s = pkg.peps.random_sites(t, 96, scale=pkg.peps.peps_scale(d, D))
+    with pkg.Engine(t, s, dtype="f64", path="generic") as e:  # PSD messages: real logs
+        e.init_messages("identity")
+        e.run(maxiter=3)
+        m = e.messages()
+    res = {}
+    for path in ("auto", "generic"):
+        with pkg.Engine(t, s, dtype="f64", path=path) as e:
+            e.set_messages(m)
+            res[path] = (e.path(), e.vertex_scalars(), e.lognorm())
+    assert res["auto"][0] == "vertex" and res["generic"][0] == "generic"
+    want = orc.vertex_scalars(t, s, m)
+    assert _rel(res["auto"][1], want) < 1e-12
+    assert _rel(res["auto"][1], res["generic"][1]) < 1e-12
+    lw = orc.bethe_free_energy(t, s, m)
+    assert abs(res["auto"][2] - lw) < 1e-11 * max(1.0, abs(lw))
+
+
+def test_lognorm_after_polish_uses_fp64_messages(pkg, orc):
+    """After bp_polish_fp64 at D = 16 the log-norm comes from the FP64 messages on the
+    large-bond kernel: equal (1e-12) to the log-norm of an FP64 context holding the same
+    messages, and to the oracle's bethe_free_energy."""
+    _say("case v64l lognorm after polish")
+    t = pkg.build_topology(pkg.named_grid((4, 4)), 16, 2)
+    s = pkg.peps.random_sites(t, 97, scale=pkg.peps.peps_scale(2, 16))
+    with pkg.Engine(t, s, dtype="bf16") as e:
+        e.init_messages("identity")
+        e.run(maxiter=50, tol=1e-5)
+        e.polish_fp64(3)
+        pol, ln = e.messages(), e.lognorm()
+    with pkg.Engine(t, s, dtype="f64", path="generic") as e:
+        e.set_messages(pol)
+        lg = e.lognorm()
+    lw = orc.bethe_free_energy(t, s, pol)
+    assert ln == pytest.approx(lg, rel=1e-12)
+    assert ln == pytest.approx(lw, rel=1e-12)
