@@ -2152,6 +2152,14 @@ static int scalars(bp_context *c) {
         }
         if (!done) RETCHK(launch_vertex_scalars<double>(c, c->d_sites64, (const double *)m));
         RETCHK(launch_edge_scalars<double>(c, (const double *)m));
+    } else if (vertex_adjoint_ok(c, false) && vd_set_limits(c->vplan.d) == 0) {
+        // FP32 networks on the D <= 8 vertex kernels: the message-form dual kernel without
+        // overrides gives Z_u = <G_{u, port 0}, m_in,0> from the unnormalised port-0 result
+        const VdCtx vc{&c->vplan, c->stream, c->topo(), (const float *)c->d_vsites};
+        if (vd_msg(vc, (const float *)m, nullptr, OP_I, 1.0, nullptr, 0, nullptr, nullptr,
+                   nullptr, nullptr, c->d_zv) != 0)
+            return set_err(BP_ERR_HIP, "vertex scalar launch failed");
+        RETCHK(launch_edge_scalars<float>(c, (const float *)m));
     } else {
         RETCHK(launch_vertex_scalars<float>(c, (const float *)c->d_sites, (const float *)m));
         RETCHK(launch_edge_scalars<float>(c, (const float *)m));

@@ -706,3 +706,28 @@ def test_bf16_batch_all_kernels(pkg, orc):
         _say(f"network {i} D={specs[i][1]}: sweeps {r.sweeps[i]} rel err {err:.2e}")
         assert err < 5e-2
         off += n
+
+
+@pytest.mark.parametrize("shape,D,d", [((4, 5), 8, 2), ((3, 3), 6, 1), ((3, 4), 7, 3)])
+def test_vertex_scalars_fp32_vertex_path(pkg, orc, shape, D, d):
+    """FP32 vertex scalars on the D <= 8 vertex path (the message-form kernel's
+    Z_u = <G_{u, port 0}, m_in,0>, degrees 2..4) against the FP64 oracle and the generic FP32
+    kernel (1e-5 relative to the largest), and the log-norm within 1e-6 relative."""
+    _say(f"case vertex scalars fp32 {shape} D={D} d={d}")
+    t = pkg.build_topology(pkg.named_grid(shape), D, d)
+    s = pkg.peps.random_sites(t, 67, scale=pkg.peps.peps_scale(d, D))
+    with pkg.Engine(t, s, dtype="f64", path="generic") as e:  # PSD messages: real logs
+        e.init_messages("identity")
+        e.run(maxiter=4)
+        m = e.messages()
+    res = {}
+    for path in ("auto", "generic"):
+        with pkg.Engine(t, s, dtype="f32", path=path) as e:
+            e.set_messages(m)
+            res[path] = (e.path(), e.vertex_scalars(), e.lognorm())
+    assert res["auto"][0] == "vertex" and res["generic"][0] == "generic"
+    want = orc.vertex_scalars(t, s, m.astype(np.float32).astype(np.float64))
+    assert _rel(res["auto"][1], want) < 1e-5
+    assert _rel(res["auto"][1], res["generic"][1]) < 1e-5
+    lw = orc.bethe_free_energy(t, s, m.astype(np.float32).astype(np.float64))
+    assert abs(res["auto"][2] - lw) < 1e-6 * abs(lw)
