@@ -2131,39 +2131,56 @@ int bp_run_sequential(bp_context *c, const int *order, int norder, int maxiter, 
     return BP_OK;
 }
 
+// Vertex scalars on the FP64 large-bond kernel (passes 1 and 2) when every network's largest
+// bond dimension is 9..16 and the plan builds: FP64 messages directly, FP32 messages through
+// an FP64 copy in a polish buffer (free while the FP32 messages are current).  *done tells
+// whether it ran.
+static int v64l_scalars(bp_context *c, const void *m, bool is64, bool *done) {
+    *done = false;
+    if (c->plain || c->path == BP_PATH_GENERIC || c->maxD < 9 || c->maxD > 16) return BP_OK;
+    RETCHK(ensure_v64(c));
+    if (!c->v64_ok || c->v64_kind != 2) return BP_OK;
+    const double *m64 = (const double *)m;
+    if (!is64) {
+        if (!c->d_msg64[0]) RETCHK(dev_alloc((void **)&c->d_msg64[0], sizeof(double) * c->nmsg));
+        const int blocks = (int)std::max<long long>(1, std::min<long long>((c->nmsg + 255) / 256, 65535));
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(blocks), dim3(256), 0, c->stream,
+                           (const float *)m, c->d_msg64[0], c->nmsg);
+        m64 = c->d_msg64[0];
+    }
+    if (vertex64l_scalars(&c->v64l, c->stream, c->d_msg_off, m64, c->d_zv) != 0)
+        return set_err(BP_ERR_HIP, "FP64 vertex scalar launch failed");
+    *done = true;
+    return BP_OK;
+}
+
 static int scalars(bp_context *c) {
     bool is64 = false;
     const void *m = current_msgs(c, &is64);
     if (c->is_cplx()) {
         RETCHK(launch_vertex_scalars<cplx>(c, (const cplx *)c->d_sites64, (const cplx *)m));
         RETCHK(launch_edge_scalars<cplx>(c, (const cplx *)m));
-    } else if (is64) {
-        // FP64 messages at largest bond dimensions 9..16: the large-bond kernel's first two
-        // passes (the plan the FP64 sweeps or the polish already use), else the generic kernel
-        bool done = false;
-        if (!c->plain && c->path != BP_PATH_GENERIC) {
-            RETCHK(ensure_v64(c));
-            if (c->v64_ok && c->v64_kind == 2) {
-                if (vertex64l_scalars(&c->v64l, c->stream, c->d_msg_off, (const double *)m,
-                                      c->d_zv) != 0)
-                    return set_err(BP_ERR_HIP, "FP64 vertex scalar launch failed");
-                done = true;
-            }
-        }
+        return BP_OK;
+    }
+    // largest bond dimensions 9..16: FP64 scalars on the large-bond kernel, in every dtype
+    bool done = false;
+    RETCHK(v64l_scalars(c, m, is64, &done));
+    if (is64) {
         if (!done) RETCHK(launch_vertex_scalars<double>(c, c->d_sites64, (const double *)m));
         RETCHK(launch_edge_scalars<double>(c, (const double *)m));
-    } else if (vertex_adjoint_ok(c, false) && vd_set_limits(c->vplan.d) == 0) {
+        return BP_OK;
+    }
+    if (!done && vertex_adjoint_ok(c, false) && vd_set_limits(c->vplan.d) == 0) {
         // FP32 networks on the D <= 8 vertex kernels: the message-form dual kernel without
         // overrides gives Z_u = <G_{u, port 0}, m_in,0> from the unnormalised port-0 result
         const VdCtx vc{&c->vplan, c->stream, c->topo(), (const float *)c->d_vsites};
         if (vd_msg(vc, (const float *)m, nullptr, OP_I, 1.0, nullptr, 0, nullptr, nullptr,
                    nullptr, nullptr, c->d_zv) != 0)
             return set_err(BP_ERR_HIP, "vertex scalar launch failed");
-        RETCHK(launch_edge_scalars<float>(c, (const float *)m));
-    } else {
-        RETCHK(launch_vertex_scalars<float>(c, (const float *)c->d_sites, (const float *)m));
-        RETCHK(launch_edge_scalars<float>(c, (const float *)m));
+        done = true;
     }
+    if (!done) RETCHK(launch_vertex_scalars<float>(c, (const float *)c->d_sites, (const float *)m));
+    RETCHK(launch_edge_scalars<float>(c, (const float *)m));
     return BP_OK;
 }
 
