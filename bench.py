@@ -625,12 +625,11 @@ def main(argv=None):
     if fp32_vertex and args.vertex_variant is not None:
         eng.set_option("vertex_variant", args.vertex_variant)
     variant = eng.get_option("vertex_variant") if fp32_vertex else 0
-    # warmup
+    # capture the sweep graphs the timed run replays (no sweep runs), then warm up
+    eng.prepare_run(args.steps)
     if args.warmup > 0:
         eng.run(maxiter=args.warmup, use_graph=True)
     eng.init_messages("identity")
-    # capture the sweep graphs the timed run replays (no sweep runs), outside the timed region
-    eng.prepare_run(args.steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     res = eng.run(maxiter=args.steps, use_graph=True)

@@ -111,9 +111,13 @@ __global__ void k_round_bf16(const double *in, float *out, long long n) {
         out[i] = bf16r((float)in[i]);
 }
 
+// Start of a run: clear every network's state and the folded accumulators, and set the
+// run's stopping parameters (passed by value, so no host copy precedes the sweeps).
 __global__ void k_reset_nets(int nnet, int *done, int *iter, double *delta, int *at,
-                             unsigned long long *fold_acc, int fold_n) {
+                             unsigned long long *fold_acc, int fold_n, RunParams *rp,
+                             RunParams params) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n == 0 && rp) *rp = params;
     if (n >= nnet) return;
     done[n] = 0;
     iter[n] = 0;
@@ -1498,6 +1502,8 @@ static int capture_sweeps(bp_context *c, const NetCtl &ctl, int n, hipGraphExec_
         *out = nullptr;
         return set_err(BP_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(ie));
     }
+    // upload now, so that the first replay does not pay for it
+    HIPCHK(hipGraphUpload(*out, c->stream));
     c->graph_epoch = g_epoch.load();
     return BP_OK;
 }
@@ -1543,28 +1549,23 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
     c->fp64_active = false;
     RETCHK(prepare_sweep(c));
     const int cap = maxiter > 0 ? maxiter : 100000;  // tol-only runs are capped
-    // The run state (delta, done, iter, at: one block) comes back through a pinned buffer,
-    // which also stages the run's parameters (an asynchronous copy from pinned memory).
+    // The run state (delta, done, iter, at: one block) comes back through a pinned buffer.
     const int nn = c->nnet;
     const size_t sbytes = run_state_bytes(nn);
-    const size_t poff = (sbytes + 63) & ~(size_t)63;
-    if (c->h_state_bytes < poff + sizeof(RunParams)) {
+    if (c->h_state_bytes < sbytes) {
         if (c->h_state) (void)hipHostFree(c->h_state);
         c->h_state = nullptr;
         c->h_state_bytes = 0;
-        HIPCHK(hipHostMalloc((void **)&c->h_state, poff + sizeof(RunParams), hipHostMallocDefault));
-        c->h_state_bytes = poff + sizeof(RunParams);
+        HIPCHK(hipHostMalloc((void **)&c->h_state, sbytes, hipHostMallocDefault));
+        c->h_state_bytes = sbytes;
     }
-    {
-        RunParams *hp = reinterpret_cast<RunParams *>(c->h_state + poff);
-        hp->maxiter = cap;  // the previous run has synchronised: the buffer is free
-        hp->crit = criterion;
-        hp->tol = tol;
-        HIPCHK(hipMemcpyAsync(c->d_runp, hp, sizeof(RunParams), hipMemcpyHostToDevice, c->stream));
-    }
+    RunParams rp;
+    rp.maxiter = cap;
+    rp.crit = criterion;
+    rp.tol = tol;
     hipLaunchKernelGGL(k_reset_nets, dim3((c->nnet + 255) / 256), dim3(256), 0, c->stream,
                        c->nnet, c->d_done, c->d_iter, c->d_delta, c->d_at, c->d_fold_acc,
-                       c->fold_n);
+                       c->fold_n, c->d_runp, rp);
     const NetCtl ctl = make_ctl(c, true);
     HIPCHK(hipGetLastError());
     // a captured chunk bakes in the folded accumulator indices k % FOLD_BUFS, so every
