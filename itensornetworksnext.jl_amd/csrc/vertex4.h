@@ -343,9 +343,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     const int *__restrict__ vs_out, const int *__restrict__ vs_dim, const int *__restrict__ vnet,
     NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs, int nitems, int d) {
     // 4 KiB of LDS: the incoming messages [vi * 4 + port], then, once every lane holds its
-    // four, the per-slice results [lane * 4 + port].  With 64 registers (six spilled) eight
-    // one-wave workgroups share a SIMD: 25.0 us at 512x512 D=2, against 26.2 at seven and
-    // 27.7 with 8 KiB of LDS (five).
+    // four, the per-slice results [lane * 4 + port], so that eight one-wave workgroups share
+    // a SIMD (64 registers).
     __shared__ float4 msm[256];
     float4 *part = msm;
     const int g = blockIdx.x, lane = threadIdx.x;
@@ -373,22 +372,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     }
     wave_lds_sync();  // one wave per workgroup
     const bool live = inr && !(ctl.enabled && ctl.done[vnet[b]]);
-    float o[4][4];  // this slice's results, [port][a * 2 + a']
+    float m[4][4];  // vi * 4 + j < 256 for every lane
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) o[j][e] = 0.f;
+    for (int j = 0; j < 4; ++j) {
+        const float4 v = msm[vi * 4 + j];
+        m[j][0] = v.x;
+        m[j][1] = v.y;
+        m[j][2] = v.z;
+        m[j][3] = v.w;
+    }
+    wave_lds_sync();  // every lane holds its messages: the buffer takes the results
+    // each result goes to LDS as soon as it is made (four registers live, not sixteen)
+    auto put = [&](int j, const float (&o)[4]) {
+        part[lane * 4 + j] = make_float4(o[0], o[1], o[2], o[3]);
+    };
     if (live) {
-        float m[4][4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const float4 v = msm[vi * 4 + j];
-            m[j][0] = v.x;
-            m[j][1] = v.y;
-            m[j][2] = v.z;
-            m[j][3] = v.w;
-        }
-        float E[16], G[16];
+        float E[16], G[16], o[4];
         // phase A: E = A x0 m0 x1 m1; out2 = open_2(E x3 m3, A), out3 = open_3(E x2 m2, A)
 #pragma unroll
         for (int r = 0; r < 16; ++r) E[r] = A[r];
@@ -397,9 +396,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 #pragma unroll
         for (int r = 0; r < 16; ++r) G[r] = E[r];
         t2_mode<3>(G, m[3]);
-        t2_open<2>(G, A, o[2]);
+        t2_open<2>(G, A, o);
+        put(2, o);
         t2_mode<2>(E, m[2]);
-        t2_open<3>(E, A, o[3]);
+        t2_open<3>(E, A, o);
+        put(3, o);
         // phase B: E = A x2 m2 x3 m3; out0 = open_0(E x1 m1, A), out1 = open_1(E x0 m0, A)
 #pragma unroll
         for (int r = 0; r < 16; ++r) E[r] = A[r];
@@ -408,13 +409,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 #pragma unroll
         for (int r = 0; r < 16; ++r) G[r] = E[r];
         t2_mode<1>(G, m[1]);
-        t2_open<0>(G, A, o[0]);
+        t2_open<0>(G, A, o);
+        put(0, o);
         t2_mode<0>(E, m[0]);
-        t2_open<1>(E, A, o[1]);
-    }
-    wave_lds_sync();  // every lane has read its messages: the buffer takes the results
+        t2_open<1>(E, A, o);
+        put(1, o);
+    } else {
+        const float z[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int j = 0; j < 4; ++j) part[lane * 4 + j] = make_float4(o[j][0], o[j][1], o[j][2], o[j][3]);
+        for (int j = 0; j < 4; ++j) put(j, z);
+    }
     wave_lds_sync();
     // one lane per outgoing message: sum over the vertex's slices, normalise, residuals
     for (int mi = lane; mi < nm; mi += 64) {
