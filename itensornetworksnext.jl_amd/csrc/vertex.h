@@ -91,9 +91,11 @@ struct VertexPlan {
     // FP32 networks whose bond dimensions are all <= 4: k_vertex_s<DP> (vertex4.h), one
     // bucket per padded dimension DP = 2, 3, 4 (index DP - 2)
     int ns[3] = {0, 0, 0};
-    long long *d_vs_ino[3] = {};   // per item and port: offset of the incoming message (-1: padded port)
-    long long *d_vs_outo[3] = {};  // per item and port: offset of the outgoing message
-    int *d_vs_out[3] = {}, *d_vs_dim[3] = {};  // per item and port: outgoing slot (-1), bond dim
+    int *d_vs_ino[3] = {};   // per item and port: offset of the incoming message (-1: padded
+                             // port; offsets fit 32 bits, or the bucket is not used)
+    int *d_vs_outo[3] = {};  // per item and port: offset of the outgoing message
+    int *d_vs_out[3] = {};   // per item and port: outgoing slot (-1: padded port)
+    int *d_vs_dim[3] = {};   // per item: the four ports' bond dimensions, one byte each
     int *d_vs_net[3] = {}, *d_vs_list[3] = {};
     float *d_vs_sites[3] = {};
     int ns_total() const { return ns[0] + ns[1] + ns[2]; }
@@ -169,6 +171,8 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     // go to the D <= 8 kernels, larger ones up to 16 (BF16) or 12 (FP32) to the large-bond
     // bucket
     const int dlim = dtype == 2 ? 16 : 12;
+    // the small-bond buckets keep message offsets in 32 bits
+    const bool off32 = msg_off[port_off[nv]] < (1LL << 31);
     int d = -1;
     std::vector<int> vlist, vlist16, gen_slots, vlist_k3, vlist_s[3];
     p->mask = mask;
@@ -202,7 +206,7 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
         // FP32 networks of degree <= 3 throughout: their own kernel, at the bucket's end
         const bool k3 = ok && dtype == 1 && dmax <= VX_D && kmax <= 3;
         // FP32 networks of bond dimension <= 4: the small-bond buckets, by padded dimension
-        const int sk = (ok && small && dtype == 1 && dmax <= 4) ? std::max(dmax, 2) - 2 : -1;
+        const int sk = (ok && small && dtype == 1 && dmax <= 4 && off32) ? std::max(dmax, 2) - 2 : -1;
         for (int u = u0; u < u1; ++u) {
             if (sk >= 0) vlist_s[sk].push_back(u);
             else if (k3) vlist_k3.push_back(u);
@@ -298,16 +302,16 @@ static int vertex_plan_build(VertexPlan *p, int dtype, int plain, int nv, const 
     if (!rc) rc = vx_upload(&p->d_gen_slots, gen_slots);
     for (int k = 0; k < 3 && !rc; ++k) {  // the small-bond buckets
         const int n = (int)vlist_s[k].size();
-        std::vector<long long> sio(4 * (size_t)n, -1), soo(4 * (size_t)n, 0);
-        std::vector<int> so(4 * (size_t)n, -1), sd(4 * (size_t)n, 0), sn(n);
+        std::vector<int> sio(4 * (size_t)n, -1), soo(4 * (size_t)n, 0);
+        std::vector<int> so(4 * (size_t)n, -1), sd(n, 0), sn(n);
         for (int b = 0; b < n; ++b) {
             const int u = vlist_s[k][b];
             for (int j = 0; j < port_off[u + 1] - port_off[u]; ++j) {
                 const int q = port_off[u] + j;
                 so[4 * b + j] = q;
-                soo[4 * b + j] = msg_off[q];
-                sio[4 * b + j] = msg_off[port_rev[q]];
-                sd[4 * b + j] = edge_dim[port_edge[q]];
+                soo[4 * b + j] = (int)msg_off[q];
+                sio[4 * b + j] = (int)msg_off[port_rev[q]];
+                sd[b] |= (edge_dim[port_edge[q]] & 255) << (8 * j);
             }
             sn[b] = vnet[u];
         }

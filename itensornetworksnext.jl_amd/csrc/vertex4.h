@@ -83,7 +83,7 @@ __device__ __forceinline__ void vs_mode_0(float (&X)[VsShape<DP>::RP], const flo
 template <int DP>
 __global__ __launch_bounds__(64) void k_vertex_s(
     const float *__restrict__ vs_sites, const float *__restrict__ mold, float *__restrict__ mnew,
-    const long long *__restrict__ vs_ino, const long long *__restrict__ vs_outo,
+    const int *__restrict__ vs_ino, const int *__restrict__ vs_outo,
     const int *__restrict__ vs_out, const int *__restrict__ vs_dim, const int *__restrict__ vnet,
     NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs, int nitems, int d) {
     using Shp = VsShape<DP>;
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(64) void k_vertex_s(
         ino = vs_ino[4 * b2 + port];
         outo = vs_outo[4 * b2 + port];
         qout = vs_out[4 * b2 + port];
-        Dm = vs_dim[4 * b2 + port];
+        Dm = (vs_dim[b2] >> (8 * port)) & 255;
         net2 = vnet[b2];
     }
     const int vi = quad / d, b = g * W + vi;
@@ -339,7 +339,7 @@ __device__ __forceinline__ float4 t2_load_msg(const float *m, long long off, int
 
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_vertex_t2(
     const float *__restrict__ vs_sites, const float *__restrict__ mold, float *__restrict__ mnew,
-    const long long *__restrict__ vs_ino, const long long *__restrict__ vs_outo,
+    const int *__restrict__ vs_ino, const int *__restrict__ vs_outo,
     const int *__restrict__ vs_out, const int *__restrict__ vs_dim, const int *__restrict__ vnet,
     NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs, int nitems, int d) {
     // 4 KiB of LDS: the incoming messages [vi * 4 + port], then, once every lane holds its
@@ -367,7 +367,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     for (int mi = lane; mi < nm; mi += 64) {
         const int bb = g * W2 + mi / 4;
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (bb < nitems) v = t2_load_msg(mold, vs_ino[4 * bb + (mi & 3)], vs_dim[4 * bb + (mi & 3)]);
+        if (bb < nitems)
+            v = t2_load_msg(mold, vs_ino[4 * bb + (mi & 3)], (vs_dim[bb] >> (8 * (mi & 3))) & 255);
         msm[mi] = v;
     }
     wave_lds_sync();  // one wave per workgroup
@@ -426,7 +427,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         if (bb >= nitems) continue;
         const int qout = vs_out[4 * bb + j];
         if (qout < 0) continue;
-        const int D = vs_dim[4 * bb + j];
+        const int D = (vs_dim[bb] >> (8 * j)) & 255;
         const long long mo = vs_outo[4 * bb + j];
         const float4 xo4 = t2_load_msg(mold, mo, D);
         const float xo[4] = {xo4.x, xo4.y, xo4.z, xo4.w};
