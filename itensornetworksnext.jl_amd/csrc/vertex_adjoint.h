@@ -235,7 +235,7 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
               double *__restrict__ z_out, const float *__restrict__ Gn,
               const double *__restrict__ Sn, unsigned long long *__restrict__ resacc,
               int rslot, VdLoopState *lst, double tol, int chk,
-              const float *__restrict__ W2) {
+              const float *__restrict__ W2, float *__restrict__ out_z) {
     extern __shared__ __align__(16) float vsm[];
     const int u = blockIdx.x;
     const int w = threadIdx.x >> 6;
@@ -353,9 +353,13 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         if (j % NS != w) continue;
-        float x = 0.f;
+        float x = 0.f, xz = 0.f;  // xz: the Z-inserted result (slice 1 negated, d = 2)
 #pragma unroll
-        for (int s = 0; s < NS; ++s) x += part[(s * 4 + j) * 64 + lane];
+        for (int s = 0; s < NS; ++s) {
+            const float v = part[(s * 4 + j) * 64 + lane];
+            x += v;
+            xz += s == 1 ? -v : v;
+        }
         if (j == 0 && z_out) {
             const double z = wave_sum_v((double)x * (double)msm[lane]);
             if (lane == 0) z_out[vid] = z;
@@ -367,6 +371,7 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
         if (!to_in) {
             if (sl.outs[j] < 0) continue;
             if (out && real) out[sl.outo[j] + ent] = x;
+            if (out_z && real) out_z[sl.outo[j] + ent] = xz;
             if (S_out) {
                 const double S = wave_sum_v((double)x);
                 if (lane == 0) S_out[sl.outs[j]] = S;
@@ -604,18 +609,20 @@ static int vd_msg(const VdCtx &c, const float *msgs, const float *W, int op, dou
                   const float *ref, double *S_out, double *z_out, const float *Gn = nullptr,
                   const double *Sn = nullptr, unsigned long long *resacc = nullptr,
                   int rslot = 0, VdLoopState *lst = nullptr, double tol = 0.0,
-                  int chk = -1, const float *W2 = nullptr) {
+                  int chk = -1, const float *W2 = nullptr, float *out_z = nullptr) {
     const VertexPlan &p = *c.vp;
     const dim3 grid(p.nv), block(64 * p.d);
     const size_t lds = vd_msg_lds(p.d);
     const bool hw = W != nullptr;
     if (op != OP_I && p.d != 2) return 4;  // X and Z pair the two slices of a qubit
     if (W2 && (p.d != 2 || op != OP_I || !W || Gn)) return 4;
+    if (out_z && (p.d != 2 || op != OP_I || W || to_in)) return 4;
 #define VD_MSG(NS, HW)                                                                         \
     hipLaunchKernelGGL((k_vd_msg<NS, HW>), grid, block, lds, c.st, c.vsites, msgs, W, op,      \
                        cv_const, cv_vertex, p.d_vin, p.d_vout, p.d_vdim, p.d_vlist,            \
                        c.tp.msg_off, to_in,                                                    \
-                       add, out, ref, S_out, z_out, Gn, Sn, resacc, rslot, lst, tol, chk, W2)
+                       add, out, ref, S_out, z_out, Gn, Sn, resacc, rslot, lst, tol, chk, W2,    \
+                       out_z)
     switch (p.d * 2 + (hw ? 1 : 0)) {
         case 2: VD_MSG(1, false); break;
         case 3: VD_MSG(1, true); break;
@@ -721,12 +728,11 @@ static int vadj_tfim(AdjointScratch *s, const VdCtx &c, int ne, const float *msg
     if (!z || !zx || !G || !GX || !GZ || !S || !nz || !dd || !cv1 || !cv0 || !cq1 || !cq0 ||
         !ev || !eq || !esum)
         VD_RET(5);
-    // forward opens with I, X, Z on the ket: G (+ sums, Z_u), GX (+ Z_u^X), GZ
+    // forward opens with I, X, Z on the ket: G (+ sums, Z_u) and GZ from one launch (Z only
+    // negates slice 1), GX (+ Z_u^X)
     VD_RET(vd_msg(c, msgs, nullptr, OP_I, 1.0, nullptr, 0, nullptr, G, nullptr, S, z, nullptr,
-                  nullptr));
+                  nullptr, nullptr, 0, nullptr, 0.0, -1, nullptr, GZ));
     VD_RET(vd_msg(c, msgs, nullptr, OP_X, 1.0, nullptr, 0, nullptr, GX, nullptr, nullptr, zx,
-                  nullptr, nullptr));
-    VD_RET(vd_msg(c, msgs, nullptr, OP_Z, 1.0, nullptr, 0, nullptr, GZ, nullptr, nullptr, nullptr,
                   nullptr, nullptr));
     if (np > 0)
         hipLaunchKernelGGL(k_adj_edge_dots<float>, dim3((np + 255) / 256), dim3(256), 0, c.st, tp,
