@@ -794,6 +794,7 @@ static int vadj_tfim(AdjointScratch *s, const VdCtx &c, int ne, const float *msg
     // sweep queued after convergence costs one empty launch, a host read one round trip)
     int chunk = 16;
     int queued = 0;
+    AdjFloor floor_;  // FP32 sweeps: stop at the rounding floor
     while (np > 0 && queued < adj_maxiter) {
         const int n = std::min(chunk, adj_maxiter - queued);
         chunk = std::min(2 * chunk, 64);
@@ -810,7 +811,7 @@ static int vadj_tfim(AdjointScratch *s, const VdCtx &c, int ne, const float *msg
                 hipSuccess ||
             hipStreamSynchronize(c.st) != hipSuccess)
             VD_RET(2);
-        if (hst.stop) break;
+        if (hst.stop || floor_.stalled(hst.r, hst.it)) break;
     }
     const int it = hst.it;
     const double r = np > 0 ? hst.r : INFINITY;

@@ -123,3 +123,22 @@ def test_device_gradient_matches_host(pkg, dtype, path):
     assert _rel(ld.numpy().astype(np.float64), lh) < tol
     gd.free()
     ld.free()
+
+
+@pytest.mark.parametrize("path", ["auto", "generic"])
+def test_fp32_adjoint_stops_at_its_rounding_floor(pkg, path):
+    """An FP32 adjoint asked for a tolerance below its rounding floor (relative change about
+    1e-7) stops once the change no longer falls (within 128 sweeps here, not at adj_maxiter),
+    reports the residual it reached, and gives the gradient of a 1e-6 solve to 1e-5."""
+    _say(f"case fp32 adjoint floor {path}")
+    t = pkg.build_topology(pkg.named_grid((8, 8)), 6, 2)
+    s = pkg.peps.random_sites(t, 46, scale=pkg.peps.peps_scale(2, 6))
+    with pkg.Engine(t, s, dtype="f32", path=path) as e:
+        e.init_messages("identity")
+        assert e.run(maxiter=500, tol=1e-12).converged[0]
+        _, g, it, res = e.tfim_energy(1.0, 3.0, grad=True, adj_tol=1e-10, adj_maxiter=1000)
+        _, g6, it6, res6 = e.tfim_energy(1.0, 3.0, grad=True, adj_tol=1e-6, adj_maxiter=1000)
+    _say(f"floor stop after {it} sweeps at {res:.2e}; 1e-6 solve {it6} sweeps at {res6:.2e}")
+    assert it <= 128 and 1e-10 <= res < 1e-6
+    assert res6 < 1e-6 and it6 < it
+    assert _rel(g, g6) < 1e-5
