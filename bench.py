@@ -758,7 +758,7 @@ def main(argv=None):
                          f"{threads} threads; the 1-thread leg ran {n1} sweeps",
                "value_1_thread": nmsg * n1 / w1, "cpu_model": cpu_model()}
         if not args.no_extras and args.dtype == "f32" and (L, D) == (32, 8):
-            # the oracle as the checker: errors of the GPU results against FP64
+            # the oracle as the reference values: errors of the GPU results against FP64
             extras["precision_vs_fp64_oracle"] = precision_vs_oracle(pkg, oracle, threads,
                                                                      args.seed)
 

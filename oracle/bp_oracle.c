@@ -1,7 +1,7 @@
 /*
  * bp_oracle.c -- FP64 CPU restatement of the reference belief-propagation path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  Loaded by the test suite, the smoke check
+ * TEST INFRASTRUCTURE ONLY (reference values).  Loaded by the test suite, the smoke check
  * and the benchmark's CPU-baseline leg; never by the product library.
  *
  * Reference sources restated here (ITensorNetworksNext.jl, paths relative to its root):

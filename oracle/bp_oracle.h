@@ -1,7 +1,7 @@
 /*
  * bp_oracle.h -- FP64 CPU restatement of the reference belief-propagation path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine: only
+ * TEST INFRASTRUCTURE ONLY.  This library gives the reference values for the HIP engine: only
  * the test suite, the smoke check and the CPU-baseline leg of the benchmark load
  * it.  The product library never links or calls it.
  *

@@ -1,7 +1,7 @@
-"""Complex (ComplexF64) restatement of the BP path in numpy, the checker of the engine's
-complex contexts.
+"""Complex (ComplexF64) restatement of the BP path in numpy, the reference values for the
+engine's complex contexts.
 
-TEST INFRASTRUCTURE ONLY: imported by the test suite as the checker; the product package
+TEST INFRASTRUCTURE ONLY: imported by the test suite for reference values; the product package
 never imports it.  The real FP64 oracle (bp_oracle.c) covers real element types; this file
 restates the same functions for complex ones, which the reference runs for ComplexF64
 (test/test_beliefpropagation.jl:154, test/test_normnetwork.jl:148):

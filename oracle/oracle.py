@@ -1,7 +1,7 @@
 """ctypes front end of the FP64 CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by the test suite, the smoke check and the
-benchmark's CPU-baseline leg as the checker; the product package never imports it.
+benchmark's CPU-baseline leg for reference values; the product package never imports it.
 Each wrapper names the reference function it restates (see bp_oracle.c for lines).
 
 `topo` arguments are duck-typed: any object with the flat CSR arrays of the engine's

@@ -1,7 +1,7 @@
 """BP-gauged gate application on the device (bp_apply_gates; apply_operator /
 apply_operators with BPApplyGate), checked as the reference's own tests check it
 (test/test_apply_operator.jl), with the oracle's exact contraction (pinned in
-test_oracle.py) as the checker and numpy's SVD for the global optimum:
+test_oracle.py) as the reference and numpy's SVD for the global optimum:
 
   * untruncated one- and two-site gates on a 4-cycle are exact (:52-74);
   * a truncated two-site gate on a 4-path with converged BP messages equals the global

@@ -1,4 +1,4 @@
-"""Pins of the FP64 CPU oracle (the checker) against the reference's own known answers
+"""Pins of the FP64 CPU oracle against the reference's own known answers
 and against what the mathematics fixes.  CPU only.
 
 Reference tests mirrored (ITensorNetworksNext.jl test/):
