@@ -143,7 +143,11 @@ def _port_array(t: ITensor, sites, links) -> np.ndarray:
                             "links to its neighbours")
     d = int(np.prod([i.dim for i in sites])) if sites else 1
     dims = [int(np.prod([i.dim for i in ls])) for ls in links]
-    return np.ascontiguousarray(t.permute(order), dtype=np.float64).reshape([d] + dims)
+    data = t.permute(order)
+    if np.issubdtype(data.dtype, np.complexfloating):
+        raise ArgumentError(f"element type {data.dtype}: the gate kernel applies real gates to "
+                            "real networks (Float64 or Float32)")
+    return np.ascontiguousarray(data, dtype=np.float64).reshape([d] + dims)
 
 
 def _gate_vertices(op: Operator, tn: ITensorNetwork) -> List[Hashable]:
@@ -197,13 +201,26 @@ class _Gate:
             desc[5] = desc[6] = -1
         desc[7] = maxdim
         desc[8] = int(normalize)
-        parts = [A.reshape(-1) for A in self.arrays]
+        # element type: the network keeps its own; the kernel computes in FP64, Float32
+        # tensors and gates get Float32 results, complex inputs are refused
+        msgs = []
         for s, (v, (sites, nbrs, links)) in enumerate(zip(vs, self.layouts)):
             for j, w in enumerate(nbrs):
                 if j == self.ports[s] or (len(vs) == 1 and not normalize):
                     continue
-                parts.append(np.ascontiguousarray(env[(w, v)], dtype=np.float64).reshape(-1))
-        parts.append(op.matrix(doms).reshape(-1))
+                msgs.append(np.asarray(env[(w, v)]))
+        # the tensors' and the gate's type decide (BP messages of a Float32 network may come
+        # back in FP64); complex messages are refused as well
+        dt = np.result_type(*([tn[v].data for v in vs] + [op.data]))
+        if any(np.iscomplexobj(m) for m in msgs):
+            dt = np.result_type(dt, np.complex128)
+        if np.issubdtype(dt, np.complexfloating):
+            raise ArgumentError(f"element type {dt}: the gate kernel applies real gates to "
+                                "real networks (Float64 or Float32)")
+        self.out_dtype = np.float32 if dt == np.float32 else np.float64
+        parts = [A.reshape(-1) for A in self.arrays]
+        parts += [np.ascontiguousarray(m, dtype=np.float64).reshape(-1) for m in msgs]
+        parts.append(np.asarray(op.matrix(doms), dtype=np.float64).reshape(-1))
         self.desc = desc
         self.data = np.concatenate(parts)
 
@@ -213,7 +230,7 @@ class _Gate:
         if len(self.vs) == 1:
             v = self.vs[0]
             sites, nbrs, links = self.layouts[0]
-            A = out[:self.arrays[0].size]
+            A = out[:self.arrays[0].size].astype(self.out_dtype)
             inds = list(sites) + [i for ls in links for i in ls]
             new[v] = ITensor(A.reshape([i.dim for i in inds]), inds)
             return new, None, None
@@ -226,9 +243,10 @@ class _Gate:
             for j, ls in enumerate(links):
                 inds += [bond] if j == self.ports[s] else list(ls)
             n = int(np.prod([i.dim for i in inds]))
-            new[v] = ITensor(out[off:off + n].reshape([i.dim for i in inds]), inds)
+            new[v] = ITensor(out[off:off + n].astype(self.out_dtype).reshape(
+                [i.dim for i in inds]), inds)
             off += caps[s]
-        S = out[off:off + chi].copy()
+        S = out[off:off + chi].astype(self.out_dtype)
         return new, bond, S
 
 

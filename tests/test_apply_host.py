@@ -73,3 +73,20 @@ def test_gate_argument_errors(pkg):
         pkg.apply_operator(op_on([site[1], site[3]]), tn, env)
     with pytest.raises(pkg.ArgumentError, match="not allowed"):
         pkg.apply_operator(op_on([site[1]]), tn, env, alg=pkg.BPApplyGate(), normalize=True)
+
+
+def test_complex_element_types_are_refused(pkg):
+    """The gate kernel is real: a complex gate or a complex site tensor raises before any
+    device call (never a silent cast that drops the imaginary part)."""
+    g = pkg.named_path_graph(3)
+    tn, site = _net(pkg, g)
+    env = pkg.message_environment(pkg.one, pkg.normnetwork(tn))
+    d = site[2].dim
+    cgate = pkg.operator(np.eye(d, dtype=np.complex128) * 1j, [pkg.uniquename(site[2])], [site[2]])
+    with pytest.raises(pkg.ArgumentError, match="complex"):
+        pkg.apply_operator(cgate, tn, env)
+    tc = tn.copy()
+    tc[2] = pkg.ITensor(tn[2].data.astype(np.complex128), tn[2].inds)
+    rgate = pkg.operator(np.eye(d), [pkg.uniquename(site[2])], [site[2]])
+    with pytest.raises(pkg.ArgumentError, match="complex"):
+        pkg.apply_operator(rgate, tc, env)

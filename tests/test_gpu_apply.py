@@ -9,9 +9,10 @@ test_oracle.py) as the reference and numpy's SVD for the global optimum:
     to the state's Schmidt values (:76-98);
   * apply_operators applies a sequence (:100-120).
 
-Real FP64 only (the Float32 / ComplexF64 / U(1)-graded variants of the reference test are
-outside this engine's real FP64 kernels).  Tolerance: 1e-9 relative (the reference allows
-eps^(1/3) = 6e-6 for Float64).
+Float64, and Float32 (computed in FP64, Float32 results); the ComplexF64 and U(1)-graded
+variants of the reference test are outside the engine's real kernels (complex inputs raise
+ArgumentError).  Tolerance: 1e-9 relative for Float64 (the reference allows eps^(1/3) =
+6e-6), 1e-5 for Float32 (the reference allows eps(Float32)^(1/3) = 4.9e-3).
 """
 import numpy as np
 import pytest
@@ -197,3 +198,29 @@ def test_normalize(pkg, orc):
     # incoming messages, i.e. the vertex scalar, must be 1
     z = pkg.vertex_scalar(pkg.normnetwork(gated), env, 3)
     assert z == pytest.approx(1.0, rel=1e-10)
+
+
+def test_float32_network_keeps_its_element_type(pkg, orc):
+    """The reference's Float32 variant (test_apply_operator.jl:55-58): a Float32 network and
+    Float32 gates give Float32 tensors (computed in FP64; the message cache keeps its FP64
+    messages), and untruncated gates stay exact within the reference's eps(Float32)^(1/3);
+    measured at Float32 rounding."""
+    _say("case float32 gates exact on a 4-cycle")
+    rng = np.random.default_rng(123)
+    g = _cycle(pkg, 4)
+    site = {v: pkg.Index(3) for v in g.vertices()}
+    tn = random_state(pkg, rng, g, site, 2, pkg.truncrank(4))
+    tn32 = tn.copy()
+    for v in g.vertices():
+        tn32[v] = pkg.ITensor(tn[v].data.astype(np.float32), tn[v].inds)
+    env = _bp_env(pkg, tn32)
+    order = [site[v] for v in g.vertices()]
+    psi = _state(orc, tn32, order)
+    for doms in ([site[2]], [site[2], site[3]]):
+        gate = randn_operator(pkg, rng, doms)
+        gate = pkg.operator(gate.data.astype(np.float32), gate.codomain, gate.domain)
+        gated, env2 = pkg.apply_operator(gate, tn32, env)
+        assert all(gated[v].data.dtype == np.float32 for v in g.vertices())
+        err = _rel(_state(orc, gated, order), _apply_full(gate, psi, order))
+        _say(f"{len(doms)}-site Float32 gate: rel err {err:.2e}")
+        assert err < np.finfo(np.float32).eps ** (1 / 3) and err < 1e-5
