@@ -111,7 +111,9 @@ template <int S>
 __device__ __forceinline__ float row_partner(float v) {
     static_assert(S == 1 || S == 2 || S == 4 || S == 8, "row partner distance");
     constexpr int ctrl = S == 1 ? 0xB1 : S == 2 ? 0x4E : S == 4 ? 0x141 : 0x140;
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, false));
+    // every partner is inside the row, so the bound control never applies; with it set the
+    // compiler can fold the move into the consuming add (v_add_f32 with a DPP source)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xF, 0xF, true));
 }
 
 template <int S>

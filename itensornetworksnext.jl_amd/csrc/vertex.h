@@ -596,13 +596,20 @@ __device__ __forceinline__ float vx_final_acc(const float (&G)[64], const float 
     return vx_reduce64(acc);
 }
 
-// The open contraction in two halves of 32 accumulators (a < 4, then a >= 4), each reduced
-// over the wave: entry (a, a') of half h ends on lanes 32h + (a - 4h) * 8 + a'.  Half the
-// registers of vx_final_acc for twice the reads of the bra tensor.
+// The open contraction in two halves of 32 accumulators (a < 4, then a >= 4), for half the
+// registers of vx_final_acc at twice the reads of the bra tensor.  Each half takes the first two
+// halving steps on its own (lane bit 5 against local entry bit 4 through v_permlane32_swap,
+// lane bit 4 against bit 3 through v_permlane16_swap), which leaves 8 values per half; the
+// two halves' 16 values then share the four DPP steps, lane bit 3 selecting the half.  Lane l
+// ends with entry vx_split_entry(l) = 32 b3 + 16 b5 + 8 b4 + l % 8 (b_k = bit k of l).
+__device__ __forceinline__ int vx_split_entry(int lane) {
+    return ((lane & 8) << 2) | ((lane & 32) >> 1) | ((lane & 16) >> 1) | (lane & 7);
+}
+
 template <bool HI, bool XL, int PITCH>
 __device__ __forceinline__ float vx_final_half(const float (&G)[64], const float *bufA) {
     const int lane = threadIdx.x & 63;
-    float res = 0.f;
+    float hv[16];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         float acc[32];
@@ -619,27 +626,28 @@ __device__ __forceinline__ float vx_final_half(const float (&G)[64], const float
                 else acc[a * 8 + lo] += G[hi * 8 + ag] * av;
             }
         }
-        // sum over lane bit 5 (both halves keep the total), then halve over bits 4..0
-#pragma unroll
-        for (int i = 0; i < 32; ++i) {
-            const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc[i]),
-                                                            __float_as_uint(acc[i]), false, false);
-            acc[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
-        }
+        // lanes with bit 5 clear keep local entries i, the others i + 16
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
-            const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(acc[i]),
+            const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc[i]),
                                                             __float_as_uint(acc[i + 16]), false,
                                                             false);
             acc[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
         }
-        vx_halve<8>(acc, lane);
-        vx_halve<4>(acc, lane);
-        vx_halve<2>(acc, lane);
-        vx_halve<1>(acc, lane);
-        if ((lane >> 5) == h) res = acc[0];
+        // lanes with bit 4 clear keep i, the others i + 8
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(acc[i]),
+                                                            __float_as_uint(acc[i + 8]), false,
+                                                            false);
+            hv[h * 8 + i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+        }
     }
-    return res;
+    vx_halve<8>(hv, lane);
+    vx_halve<4>(hv, lane);
+    vx_halve<2>(hv, lane);
+    vx_halve<1>(hv, lane);
+    return hv[0];
 }
 
 template <bool HI, bool XL, bool BF, int PITCH = VX_PITCH>
@@ -1178,8 +1186,9 @@ void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__
     }
     __syncthreads();
     // results go to LDS as they are made (fewer live registers; with one wave per
-    // workgroup the compiler allocates best when they stay in registers to the end)
-    float *mypart = part + w * 4 * 64 + lane;
+    // workgroup the compiler allocates best when they stay in registers to the end); lane l
+    // holds entry vx_split_entry(l) of each result
+    float *mypart = part + w * 4 * 64 + vx_split_entry(lane);
     float out[4];
     auto put = [&](int j, float v) {
         if constexpr (NS == 1) out[j] = v; else mypart[j * 64] = v;

@@ -344,10 +344,11 @@ void k_vd_msg(const float *__restrict__ vsites, const float *__restrict__ msgs,
         o[1] = vx_final_half<false, true, VX_PITCH>(R1, bufA);
     }
 
-    // ---- sum over slices, write
+    // ---- sum over slices, write (lane l holds entry vx_split_entry(l) of each result)
     const float scale = HASW ? sg : sg * cv;
+    const int ent_l = vx_split_entry(lane);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) part[(w * 4 + j) * 64 + lane] = o[j] * scale;
+    for (int j = 0; j < 4; ++j) part[(w * 4 + j) * 64 + ent_l] = o[j] * scale;
     __syncthreads();
     double rd = 0.0, rv = 0.0;  // this wave's residual maxima (resacc)
 #pragma unroll
