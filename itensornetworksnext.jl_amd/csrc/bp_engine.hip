@@ -1699,21 +1699,24 @@ static int tune_save(const bp_context *c, const char *path) {
     return f ? BP_OK : set_err(BP_ERR_ARG, "cannot write the autotuner cache %s", path);
 }
 
-// min over `reps` of one full plain sweep (every network), in ms; -1 on error
-static double tune_time_sweep(bp_context *c, int reps) {
+// One full plain sweep (every network), in ms: the minimum over `reps` of the mean over `per`
+// back-to-back sweeps (one event pair around several launches keeps the event overhead and
+// a single launch's jitter out of the comparison); -1 on error
+static double tune_time_sweep(bp_context *c, int reps, int per = 4) {
     if (prepare_sweep(c) != BP_OK) return -1.0;
     const NetCtl ctl = make_ctl(c, false);
     if (sweep_once(c, ctl) != BP_OK) return -1.0;
     double best = 1e30;
     for (int r = 0; r < reps; ++r) {
         if (hipEventRecord(c->ev0, c->stream) != hipSuccess) return -1.0;
-        if (sweep_once(c, ctl) != BP_OK) return -1.0;
+        for (int k = 0; k < per; ++k)
+            if (sweep_once(c, ctl) != BP_OK) return -1.0;
         if (hipEventRecord(c->ev1, c->stream) != hipSuccess ||
             hipEventSynchronize(c->ev1) != hipSuccess)
             return -1.0;
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-        best = std::min(best, (double)ms);
+        best = std::min(best, (double)ms / per);
     }
     return best;
 }
@@ -1819,12 +1822,17 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
             const int nvar = c->dtype == BP_F32 ? 4 : 2;
             double t[4] = {-1.0, -1.0, -1.0, -1.0};
             int best = 0;
-            for (int v = 0; v < nvar && rc == BP_OK; ++v) {
-                c->vplan.variant = v;
-                t[v] = tune_time_sweep(c, 3);
-                if (t[v] < 0.0) rc = set_err(BP_ERR_HIP, "autotuner sweep failed");
-                else if (t[v] < t[best]) best = v;
-            }
+            // the candidates in turn, three rounds, each keeping its best (drift in clocks or
+            // in the rest of the device then weighs on every candidate alike)
+            for (int round = 0; round < 3 && rc == BP_OK; ++round)
+                for (int v = 0; v < nvar && rc == BP_OK; ++v) {
+                    c->vplan.variant = v;
+                    const double tv = tune_time_sweep(c, 1);
+                    if (tv < 0.0) rc = set_err(BP_ERR_HIP, "autotuner sweep failed");
+                    else if (t[v] < 0.0 || tv < t[v]) t[v] = tv;
+                }
+            for (int v = 1; v < nvar && rc == BP_OK; ++v)
+                if (t[v] < t[best]) best = v;
             if (rc == BP_OK) {
                 double alt = 1e30;
                 for (int v = 0; v < nvar; ++v)
