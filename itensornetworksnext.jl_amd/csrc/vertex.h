@@ -65,7 +65,7 @@ struct VertexPlan {
     int ngen = 0;
     size_t lds_bytes = 0;
     int variant = 0;            // D <= 8 sweep kernel: 0 k_vertex_d8, 1 k_vertex_d8_lo,
-                                // 2 k_vertex_d8_mo (FP32)
+                                // 2, 3 k_vertex_d8_mo, 4 k_vertex_d8_hy (FP32)
     // networks whose largest bond dimension is 9..16 in BF16 (k_vertex_d16, vertex16.h) or
     // 9..12 in FP32 (k_vertex_d12, vertex12.h): the large-bond bucket
     int nv16 = 0;
@@ -1286,6 +1286,230 @@ void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__
     }
 }
 
+// ---------------------------------------------------------------- lane-port MFMA opens (FP32)
+//
+// Variant 4 of the D <= 8 FP32 sweep (k_vertex_d8_hy): the low-LDS kernel with two of the four
+// open contractions per slice on the MFMA pipe.  An open on a port that sits in the low three
+// lane bits needs no staging: v_mfma_f32_4x4x1_16b_f32 pairs lane 4b + i (first operand, row
+// i) with lane 4b + j (second operand, column j), so with the ket in registers (K = register
+// index and lane bits 3..5) and the bra read from LDS, block b accumulates
+// out[4c + i][4c' + j] over the registers, c = lane bit 2 and c' = c (the bra at the lane
+// itself) or 1 - c (the bra at lane ^ 4).  The other partner message is absorbed into the
+// bra, not the ket:
+//   out3 = open_p3(E01, A x2 m2^T)   (E01 in layout X: port 3 = lane bits 0..2)
+//   out1 = open_p1(E23, A x0 m0^T)   (E23 in layout Y: port 1 = lane bits 0..2)
+// which keeps eight mode products per slice.  out2 and out0 stay on the VALU as in the
+// low-LDS kernel.
+
+// Mode product with the transposed message: O[b] = sum_a T[a] m[b][a] on a register port.
+template <bool HI>
+__device__ __forceinline__ void vx_mode_t(const float (&T)[64], float (&O)[64], const float *msg) {
+    float M[16];
+    const int c = threadIdx.x & 3;
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) M[a * 2 + h] = msg[(4 * h + c) * 8 + a];
+    vx_mp<HI>(T, O, M);
+}
+
+// Entry (a * 8 + a') of the open contraction that lane l holds after vx_open_lane:
+// a = 4 b2 + 2 b4 + b3, a' = 4 (b2 ^ b5) + l % 4 (b_k = bit k of l).
+__device__ __forceinline__ int vx_lane_entry(int lane) {
+    const int b2 = (lane >> 2) & 1, b3 = (lane >> 3) & 1, b4 = (lane >> 4) & 1,
+              b5 = (lane >> 5) & 1;
+    return (4 * b2 + 2 * b4 + b3) * 8 + 4 * (b2 ^ b5) + (lane & 3);
+}
+
+// out[a][a'] = sum over the registers r and lane bits 3..5 of K[r](a) bra[r](a'), the open
+// port in lane bits 0..2; bra element (r, lane) at bra[r * VX_PITCH + lane].
+__device__ __forceinline__ float vx_open_lane(const float (&K)[64], const float *bra) {
+    const int lane = threadIdx.x & 63;
+    const float *bd = bra + lane, *bo = bra + (lane ^ 4);
+    v4f d0 = {0.f, 0.f, 0.f, 0.f}, d1 = d0, o0 = d0, o1 = d0;
+#pragma unroll
+    for (int r = 0; r < 64; r += 2) {
+        d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[r], bd[r * VX_PITCH], d0, 0, 0, 0);
+        o0 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[r], bo[r * VX_PITCH], o0, 0, 0, 0);
+        d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[r + 1], bd[(r + 1) * VX_PITCH], d1, 0, 0, 0);
+        o1 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[r + 1], bo[(r + 1) * VX_PITCH], o1, 0, 0, 0);
+    }
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        v[i] = d0[i] + d1[i];
+        v[4 + i] = o0[i] + o1[i];
+    }
+    // sum over lane bits 5, 4, 3 (port values of the contracted lane port): lanes with bit 5
+    // set keep the other-half tile, bit 4 and bit 3 select the row i
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[k]),
+                                                        __float_as_uint(v[k + 4]), false, false);
+        v[k] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[k]),
+                                                        __float_as_uint(v[k + 2]), false, false);
+        v[k] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+    const bool up = (lane & 8) != 0;
+    const float send = up ? v[0] : v[1], keep = up ? v[1] : v[0];
+    return keep + vmo_ror<8>(send);  // row_ror:8 pairs lane l with l ^ 8 exactly
+}
+
+template <int NS>
+__global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__ mold,
+                    float *__restrict__ mnew, const int4 *__restrict__ vin,
+                    const int4 *__restrict__ vout, const int4 *__restrict__ vdim,
+                    const int *__restrict__ vnet, const long long *__restrict__ msg_off,
+                    NetCtl ctl, double *__restrict__ res_fid, double *__restrict__ res_abs,
+                    SweepFold fold) {
+    extern __shared__ __align__(16) float vsm[];
+    __shared__ double fold_m[4];
+    const int u = blockIdx.x;
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    float *buf = vsm + w * VX_BUF;
+    float *msm = vsm + NS * VX_BUF;      // 4 x 64 incoming messages
+    float *part = msm + 4 * 64;          // [s][port][64] per-slice results
+    const int4 vi = vin[u], vo = vout[u], vdm = vdim[u];
+    const int ins[4] = {vi.x, vi.y, vi.z, vi.w};
+    const int outs[4] = {vo.x, vo.y, vo.z, vo.w};
+    const int dims[4] = {vdm.x, vdm.y, vdm.z, vdm.w};
+    long long ino[4], outo[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        ino[j] = ins[j] >= 0 ? msg_off[ins[j]] : 0;
+        outo[j] = outs[j] >= 0 ? msg_off[outs[j]] : 0;
+    }
+    const int la = lane >> 3, lb = lane & 7;
+    const int net = vnet[u];
+    if (vx_fold_begin(ctl, fold, net, u)) {  // uniform over the workgroup
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j % NS != w || outs[j] < 0) continue;
+            if (lane < dims[j] * dims[j]) mnew[outo[j] + lane] = mold[outo[j] + lane];
+            if (lane == 0) { res_fid[outs[j]] = 0.0; res_abs[outs[j]] = 0.0; }
+        }
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j % NS != w) continue;
+        const int D = dims[j];
+        float x;
+        if (ins[j] < 0) x = la == lb ? 1.f : 0.f;
+        else x = (la < D && lb < D) ? mold[ino[j] + la * D + lb] : 0.f;
+        msm[j * 64 + lane] = x;
+    }
+    float T[64], E[64], G[64];
+    {
+        const float *src = (const float *)vsites_v + ((long long)u * NS + w) * 4096;
+#pragma unroll
+        for (int r = 0; r < 64; ++r) T[r] = src[r * 64 + lane];
+    }
+    __syncthreads();
+    float *mypart = part + w * 4 * 64;
+    const int e_split = vx_split_entry(lane), e_lane = vx_lane_entry(lane);
+
+    // ---- phase A (layout X: registers p0 p1, lanes p2 (bits 3..5) p3 (bits 0..2))
+    vx_mode<true, false>(T, E, msm + 0 * 64);
+    vx_mode<false, false>(E, G, msm + 1 * 64);  // G = E01 (X), kept for out3
+#pragma unroll
+    for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = G[r];
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) E[r] = buf[lane * VX_PITCH + r];  // E01 in Y
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = T[r];  // A, the bra of out2
+    wave_lds_sync();
+    vx_mode<false, false>(E, T, msm + 3 * 64);  // E01 x3 m3 (Y)
+    mypart[2 * 64 + e_split] = vx_final_half<true, false, VX_PITCH>(T, buf);
+#pragma unroll
+    for (int r = 0; r < 64; ++r) E[r] = buf[lane * VX_PITCH + r];  // A in Y (kept to phase B)
+    vx_mode_t<true>(E, T, msm + 2 * 64);        // B2 = A x2 m2^T (Y)
+    wave_lds_sync();  // every lane has finished reading A from the buffer
+#pragma unroll
+    for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = T[r];  // B2, readable as X
+    wave_lds_sync();
+    mypart[3 * 64 + e_lane] = vx_open_lane(G, buf);
+
+    // ---- phase B (layout Y: registers p2 p3, lanes p0 (bits 3..5) p1 (bits 0..2))
+    vx_mode<true, false>(E, T, msm + 2 * 64);
+    vx_mode<false, false>(T, G, msm + 3 * 64);  // G = E23 (Y), kept for out1
+    wave_lds_sync();  // every lane has finished reading B2
+#pragma unroll
+    for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = E[r];  // A, readable as X
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) T[r] = buf[r * VX_PITCH + lane];  // A in X
+    vx_mode_t<true>(T, E, msm + 0 * 64);        // B0 = A x0 m0^T (X)
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = E[r];  // B0, readable as Y
+    wave_lds_sync();
+    mypart[1 * 64 + e_lane] = vx_open_lane(G, buf);
+    wave_lds_sync();  // every lane has finished reading B0
+#pragma unroll
+    for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = G[r];
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) E[r] = buf[lane * VX_PITCH + r];  // E23 in X
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = T[r];  // A, the bra of out0
+    wave_lds_sync();
+    vx_mode<false, false>(E, G, msm + 1 * 64);  // E23 x1 m1 (X)
+    mypart[0 * 64 + e_split] = vx_final_half<true, true, VX_PITCH>(G, buf);
+
+    // ---- sum over s, normalise, residual, write (as k_vertex_d8_lo)
+    double fm = 0.0;
+    const bool fcrit = fold.acc && fold.rp->crit != 0;
+    __syncthreads();
+    const int4 vo2 = vout[u], vd2 = vdim[u];
+    const int outs2[4] = {vo2.x, vo2.y, vo2.z, vo2.w};
+    const int dims2[4] = {vd2.x, vd2.y, vd2.z, vd2.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int q = outs2[j];
+        if (j % NS != w || q < 0) continue;
+        float x = 0.f;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) x += part[(s * 4 + j) * 64 + lane];
+        const double S = wave_sum_v((double)x);
+        const double xn = S != 0.0 ? (double)x / S : (double)x;
+        const float xf = (float)xn;
+        const int D = dims2[j];
+        const bool real = la < D && lb < D;
+        const long long o = msg_off[q] + la * D + lb;
+        const double xo = real ? (double)mold[o] : 0.0;
+        if (real) mnew[o] = xf;
+        const double dn = wave_sum_v((double)xf * xo),
+                     nn = wave_sum_v((double)xf * (double)xf), oo = wave_sum_v(xo * xo);
+        const double mx = wave_max_v(fabs((double)xf - xo));
+        if (lane == 0) {
+            const double c = dn / sqrt(nn * oo);
+            res_fid[q] = 1.0 - c * c;
+            res_abs[q] = mx;
+            fm = fold_max(fm, fcrit ? mx : 1.0 - c * c);
+        }
+    }
+    if (fold.acc) {
+        if (lane == 0) fold_m[w] = fm;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = 0.0;
+            for (int s = 0; s < NS; ++s) m = fold_max(m, fold_m[s]);
+            atomicMax(fold.acc + (long long)net * FOLD_SLOTS + (u % FOLD_SLOTS),
+                      (unsigned long long)__double_as_longlong(m));
+        }
+    }
+}
+
 static size_t vertex_lo_lds(int ns) {
     return sizeof(float) * ((size_t)ns * VX_BUF + 4 * 64 + (size_t)ns * 4 * 64);
 }
@@ -1316,6 +1540,25 @@ static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *v
             VX_MO(3, 1)
             VX_MO(4, 1)
 #undef VX_MO
+            default:
+                return 4;
+        }
+        return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
+    if (p->variant == 4 && !p->bf16) {  // lane-port MFMA opens (FP32)
+        const size_t lds = vertex_lo_lds(p->d);
+        switch (p->d) {
+#define VX_HY(NS)                                                                              \
+    case NS:                                                                                   \
+        hipLaunchKernelGGL((k_vertex_d8_hy<NS>), dim3(n4), dim3(64 * NS), lds, st, vsites,     \
+                           (const float *)src, (float *)dst, p->d_vin, p->d_vout, p->d_vdim,   \
+                           p->d_vnet, tp.msg_off, ctl, res_fid, res_abs, fold);                \
+        break;
+            VX_HY(1)
+            VX_HY(2)
+            VX_HY(3)
+            VX_HY(4)
+#undef VX_HY
             default:
                 return 4;
         }
@@ -1376,10 +1619,14 @@ static int vertex_set_lds_limits(const VertexPlan *p) {
                          (const void *)k_vertex_d8_mo<3, false>, (const void *)k_vertex_d8_mo<4, false>};
     const void *mg[4] = {(const void *)k_vertex_d8_mo<1, true>, (const void *)k_vertex_d8_mo<2, true>,
                          (const void *)k_vertex_d8_mo<3, true>, (const void *)k_vertex_d8_mo<4, true>};
+    const void *hy[4] = {(const void *)k_vertex_d8_hy<1>, (const void *)k_vertex_d8_hy<2>,
+                         (const void *)k_vertex_d8_hy<3>, (const void *)k_vertex_d8_hy<4>};
     if (p->d < 1 || p->d > 4) return 4;
     if (!p->bf16 && (hipFuncSetAttribute(mo[p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)p->lds_bytes) != hipSuccess ||
                      hipFuncSetAttribute(mg[p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)vertex_lo_lds(p->d)) != hipSuccess ||
+                     hipFuncSetAttribute(hy[p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)vertex_lo_lds(p->d)) != hipSuccess))
         return 2;
     return hipFuncSetAttribute(fns[(p->bf16 ? 4 : 0) + p->d - 1],

@@ -440,15 +440,15 @@ def test_vertex_padded_matches_generic_d6(pkg):
 @pytest.mark.parametrize("shape,D,d", [((6, 5), 8, 2), ((4, 4), 8, 1), ((3, 5), 6, 3),
                                        ((5, 4), 8, 4)])
 def test_vertex_low_lds_variant(pkg, orc, shape, D, d):
-    """The one-buffer-per-wave FP32 vertex kernel (two waves per SIMD) and the kernel with
-    MFMA open contractions compute what the default kernel computes: one sweep within the
-    FP32 bound and 20 sweeps agreeing."""
+    """The one-buffer-per-wave FP32 vertex kernel (two waves per SIMD), the kernels with MFMA
+    open contractions, and the one with half of them on MFMA in place compute what the
+    default kernel computes: one sweep within the FP32 bound and 20 sweeps agreeing."""
     _say(f"case vertex kernel variants {shape} D={D} d={d}")
     t = pkg.build_topology(pkg.named_grid(shape), D, d)
     s = pkg.peps.random_sites(t, 43, scale=pkg.peps.peps_scale(d, D))
     m = pkg.peps.random_messages(t, 43)
     res = {}
-    for v in (0, 1, 2, 3):
+    for v in (0, 1, 2, 3, 4):
         with pkg.Engine(t, s, dtype="f32") as e:
             e.set_option("vertex_variant", v)
             e.set_messages(m)
@@ -458,7 +458,7 @@ def test_vertex_low_lds_variant(pkg, orc, shape, D, d):
             e.init_messages("identity")
             e.run(maxiter=20)
             res[v] = (one, e.messages())
-    for v in (1, 2, 3):
+    for v in (1, 2, 3, 4):
         # one sweep from random non-symmetric messages: each variant is within the FP32
         # bound above (ill-conditioned messages make a flat relative check loose); after
         # 20 sweeps from the identity the messages are well conditioned
@@ -473,7 +473,7 @@ def test_kernel_options_round_trip(pkg):
     s = pkg.peps.random_sites(t, 44, scale=pkg.peps.peps_scale(2, 8))
     with pkg.Engine(t, s, dtype="f32") as e:
         for name, value in (("vertex_variant", 1), ("vertex_variant", 2), ("vertex_variant", 3),
-                            ("vertex_variant", 0),
+                            ("vertex_variant", 4), ("vertex_variant", 0),
                             ("vertex16_waves", 8), ("vertex_bond_mask", 0x1FE)):
             e.set_option(name, value)
             assert e.get_option(name) == value
