@@ -42,7 +42,8 @@ SWEEP_KERNEL = {("vertex", "f32", 0): "k_vertex_d8<NS, false>",
                 ("vertex", "bf16", 1): "k_vertex_d8_lo<NS, true>",
                 ("vertex", "f32", 2): "k_vertex_d8_mo<NS, false>",
                 ("vertex", "f32", 3): "k_vertex_d8_mo<NS, true>",
-                ("vertex", "f32", 4): "k_vertex_d8_hy<NS>",
+                ("vertex", "f32", 4): "k_vertex_d8_hy<NS, false>",
+                ("vertex", "f32", 5): "k_vertex_d8_hy<NS, true>",
                 ("vertex", "f64", 0): "k_vertex_d8_f64<NS>"}
 
 PEAK = {  # MI355X published dense peaks
@@ -546,11 +547,11 @@ def main(argv=None):
                          "timings; contract: only the exact-contraction timings; gates: only "
                          "the gate-layer timing; small_bond: only the bond-dimension <= 4 "
                          "timings; print the timings, for profiling")
-    ap.add_argument("--vertex-variant", type=int, choices=[0, 1, 2, 3, 4], default=None,
+    ap.add_argument("--vertex-variant", type=int, choices=[0, 1, 2, 3, 4, 5], default=None,
                     help="FP32 vertex sweep kernel to run instead of the tuned one (0: two LDS "
                          "buffers per wave, 1: one buffer and two waves per SIMD, 2: MFMA open "
                          "contractions, bra in LDS, 3: MFMA open contractions, bra from HBM, "
-                         "4: half of the open contractions on MFMA in place)")
+                         "4: half of the open contractions on MFMA in place, 5: all of them)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of "
                          "known traffic, used to calibrate rocprofv3 FETCH_SIZE/WRITE_SIZE)")

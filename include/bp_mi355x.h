@@ -253,7 +253,7 @@ int bp_local_energy_device(bp_context *ctx, const double *ops_v, const double *o
  *                      bra in LDS (one wave per SIMD), 3: open contractions on MFMA, bra
  *                      rows from padded X/Y site copies in HBM (two waves per SIMD),
  *                      4: as 1 with two of the four open contractions per slice on MFMA
- *                      (the partner message absorbed into the bra);
+ *                      (the partner message absorbed into the bra), 5: all four;
  *   "vertex16_waves"   waves per workgroup of the BF16 kernel for bond dimensions 9..16
  *                      (4, 8 or 16; default 16);
  *   "vertex_bond_mask" bit D set: networks whose largest bond dimension is D may run on the

@@ -1383,7 +1383,7 @@ int bp_set_option(bp_context *c, const char *name, int value) {
     if (!c || !name) return set_err(BP_ERR_ARG, "NULL argument");
     bump_epoch();
     if (strcmp(name, "vertex_variant") == 0) {
-        if (value < 0 || value > 4) return set_err(BP_ERR_ARG, "vertex_variant is 0, 1, 2, 3 or 4");
+        if (value < 0 || value > 5) return set_err(BP_ERR_ARG, "vertex_variant is 0..5");
         if (value >= 2 && c->dtype != BP_F32)
             return set_err(BP_ERR_UNSUPPORTED, "vertex_variant %d (MFMA open contractions) is FP32 only",
                            value);
@@ -1818,10 +1818,10 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
         const std::string key = o.str();
         auto it = c->tune.find(key);
         if (it == c->tune.end() && mode >= 1) {
-            // FP32: five kernels (two with VALU open contractions, two with MFMA ones staged
-            // through LDS, one with half of them on MFMA in place); BF16: two
-            const int nvar = c->dtype == BP_F32 ? 5 : 2;
-            double t[5] = {-1.0, -1.0, -1.0, -1.0, -1.0};
+            // FP32: six kernels (two with VALU open contractions, two with MFMA ones staged
+            // through LDS, two with half or all of them on MFMA in place); BF16: two
+            const int nvar = c->dtype == BP_F32 ? 6 : 2;
+            double t[6] = {-1.0, -1.0, -1.0, -1.0, -1.0, -1.0};
             int best = 0;
             // the candidates in turn, three rounds, each keeping its best (drift in clocks or
             // in the rest of the device then weighs on every candidate alike)

@@ -65,7 +65,7 @@ struct VertexPlan {
     int ngen = 0;
     size_t lds_bytes = 0;
     int variant = 0;            // D <= 8 sweep kernel: 0 k_vertex_d8, 1 k_vertex_d8_lo,
-                                // 2, 3 k_vertex_d8_mo, 4 k_vertex_d8_hy (FP32)
+                                // 2, 3 k_vertex_d8_mo, 4, 5 k_vertex_d8_hy (FP32)
     // networks whose largest bond dimension is 9..16 in BF16 (k_vertex_d16, vertex16.h) or
     // 9..12 in FP32 (k_vertex_d12, vertex12.h): the large-bond bucket
     int nv16 = 0;
@@ -1298,8 +1298,11 @@ void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__
 // bra, not the ket:
 //   out3 = open_p3(E01, A x2 m2^T)   (E01 in layout X: port 3 = lane bits 0..2)
 //   out1 = open_p1(E23, A x0 m0^T)   (E23 in layout Y: port 1 = lane bits 0..2)
-// which keeps eight mode products per slice.  out2 and out0 stay on the VALU as in the
-// low-LDS kernel.
+// which keeps eight mode products per slice.  ALL = false: out2 and out0 stay on the VALU as
+// in the low-LDS kernel (variant 4).  ALL = true (variant 5): they run the same way from the
+// lane-swapped layouts X' (registers p0 p1, lanes p3 (bits 3..5) p2 (bits 0..2)) and Y'
+// (registers p2 p3, lanes p1 p0), made through the buffer like the X <-> Y transposes:
+//   out2 = open_p2(E01 in X', A x3 m3^T in X')   out0 = open_p0(E23 in Y', A x1 m1^T in Y')
 
 // Mode product with the transposed message: O[b] = sum_a T[a] m[b][a] on a register port.
 template <bool HI>
@@ -1359,7 +1362,10 @@ __device__ __forceinline__ float vx_open_lane(const float (&K)[64], const float 
     return keep + vmo_ror<8>(send);  // row_ror:8 pairs lane l with l ^ 8 exactly
 }
 
-template <int NS>
+// Lane index with its two 3-bit halves swapped ((p, q) -> (q, p)).
+__device__ __forceinline__ int vx_lane_swap(int l) { return ((l & 7) << 3) | (l >> 3); }
+
+template <int NS, bool ALL>
 __global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__ mold,
                     float *__restrict__ mnew, const int4 *__restrict__ vin,
@@ -1418,30 +1424,53 @@ void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__
     // ---- phase A (layout X: registers p0 p1, lanes p2 (bits 3..5) p3 (bits 0..2))
     vx_mode<true, false>(T, E, msm + 0 * 64);
     vx_mode<false, false>(E, G, msm + 1 * 64);  // G = E01 (X), kept for out3
+    if constexpr (ALL) {
 #pragma unroll
-    for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = G[r];
-    wave_lds_sync();
+        for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = T[r];  // A (X)
+        wave_lds_sync();
 #pragma unroll
-    for (int r = 0; r < 64; ++r) E[r] = buf[lane * VX_PITCH + r];  // E01 in Y
-    wave_lds_sync();
+        for (int r = 0; r < 64; ++r) E[r] = buf[lane * VX_PITCH + r];  // A in Y (kept to phase B)
+    } else {
 #pragma unroll
-    for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = T[r];  // A, the bra of out2
-    wave_lds_sync();
-    vx_mode<false, false>(E, T, msm + 3 * 64);  // E01 x3 m3 (Y)
-    mypart[2 * 64 + e_split] = vx_final_half<true, false, VX_PITCH>(T, buf);
+        for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = G[r];
+        wave_lds_sync();
 #pragma unroll
-    for (int r = 0; r < 64; ++r) E[r] = buf[lane * VX_PITCH + r];  // A in Y (kept to phase B)
+        for (int r = 0; r < 64; ++r) E[r] = buf[lane * VX_PITCH + r];  // E01 in Y
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = T[r];  // A, the bra of out2
+        wave_lds_sync();
+        vx_mode<false, false>(E, T, msm + 3 * 64);  // E01 x3 m3 (Y)
+        mypart[2 * 64 + e_split] = vx_final_half<true, false, VX_PITCH>(T, buf);
+#pragma unroll
+        for (int r = 0; r < 64; ++r) E[r] = buf[lane * VX_PITCH + r];  // A in Y (kept to phase B)
+    }
     vx_mode_t<true>(E, T, msm + 2 * 64);        // B2 = A x2 m2^T (Y)
     wave_lds_sync();  // every lane has finished reading A from the buffer
 #pragma unroll
     for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = T[r];  // B2, readable as X
     wave_lds_sync();
     mypart[3 * 64 + e_lane] = vx_open_lane(G, buf);
+    if constexpr (ALL) {
+        vx_mode_t<false>(E, T, msm + 3 * 64);   // B3 = A x3 m3^T (Y)
+        wave_lds_sync();  // every lane has finished reading B2
+#pragma unroll
+        for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = G[r];
+        wave_lds_sync();
+        const int ls = vx_lane_swap(lane);
+#pragma unroll
+        for (int r = 0; r < 64; ++r) G[r] = buf[r * VX_PITCH + ls];  // E01 in X'
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + vx_lane_swap(r)] = T[r];  // B3 as X'
+        wave_lds_sync();
+        mypart[2 * 64 + e_lane] = vx_open_lane(G, buf);
+    }
 
     // ---- phase B (layout Y: registers p2 p3, lanes p0 (bits 3..5) p1 (bits 0..2))
     vx_mode<true, false>(E, T, msm + 2 * 64);
     vx_mode<false, false>(T, G, msm + 3 * 64);  // G = E23 (Y), kept for out1
-    wave_lds_sync();  // every lane has finished reading B2
+    wave_lds_sync();  // every lane has finished reading the last bra
 #pragma unroll
     for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = E[r];  // A, readable as X
     wave_lds_sync();
@@ -1453,18 +1482,34 @@ void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__
     for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = E[r];  // B0, readable as Y
     wave_lds_sync();
     mypart[1 * 64 + e_lane] = vx_open_lane(G, buf);
-    wave_lds_sync();  // every lane has finished reading B0
+    if constexpr (ALL) {
+        vx_mode_t<false>(T, E, msm + 1 * 64);   // B1 = A x1 m1^T (X)
+        wave_lds_sync();  // every lane has finished reading B0
 #pragma unroll
-    for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = G[r];
-    wave_lds_sync();
+        for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = G[r];
+        wave_lds_sync();
+        const int ls = vx_lane_swap(lane);
 #pragma unroll
-    for (int r = 0; r < 64; ++r) E[r] = buf[lane * VX_PITCH + r];  // E23 in X
-    wave_lds_sync();
+        for (int r = 0; r < 64; ++r) G[r] = buf[r * VX_PITCH + ls];  // E23 in Y'
+        wave_lds_sync();
 #pragma unroll
-    for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = T[r];  // A, the bra of out0
-    wave_lds_sync();
-    vx_mode<false, false>(E, G, msm + 1 * 64);  // E23 x1 m1 (X)
-    mypart[0 * 64 + e_split] = vx_final_half<true, true, VX_PITCH>(G, buf);
+        for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + vx_lane_swap(r)] = E[r];  // B1 as Y'
+        wave_lds_sync();
+        mypart[0 * 64 + e_lane] = vx_open_lane(G, buf);
+    } else {
+        wave_lds_sync();  // every lane has finished reading B0
+#pragma unroll
+        for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = G[r];
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < 64; ++r) E[r] = buf[lane * VX_PITCH + r];  // E23 in X
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = T[r];  // A, the bra of out0
+        wave_lds_sync();
+        vx_mode<false, false>(E, G, msm + 1 * 64);  // E23 x1 m1 (X)
+        mypart[0 * 64 + e_split] = vx_final_half<true, true, VX_PITCH>(G, buf);
+    }
 
     // ---- sum over s, normalise, residual, write (as k_vertex_d8_lo)
     double fm = 0.0;
@@ -1545,19 +1590,23 @@ static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *v
         }
         return hipGetLastError() == hipSuccess ? 0 : 2;
     }
-    if (p->variant == 4 && !p->bf16) {  // lane-port MFMA opens (FP32)
+    if ((p->variant == 4 || p->variant == 5) && !p->bf16) {  // lane-port MFMA opens (FP32)
         const size_t lds = vertex_lo_lds(p->d);
-        switch (p->d) {
-#define VX_HY(NS)                                                                              \
-    case NS:                                                                                   \
-        hipLaunchKernelGGL((k_vertex_d8_hy<NS>), dim3(n4), dim3(64 * NS), lds, st, vsites,     \
-                           (const float *)src, (float *)dst, p->d_vin, p->d_vout, p->d_vdim,   \
-                           p->d_vnet, tp.msg_off, ctl, res_fid, res_abs, fold);                \
+        switch (p->d * 2 + (p->variant == 5 ? 1 : 0)) {
+#define VX_HY(NS, ALL)                                                                         \
+    case NS * 2 + ALL:                                                                         \
+        hipLaunchKernelGGL((k_vertex_d8_hy<NS, ALL>), dim3(n4), dim3(64 * NS), lds, st,        \
+                           vsites, (const float *)src, (float *)dst, p->d_vin, p->d_vout,      \
+                           p->d_vdim, p->d_vnet, tp.msg_off, ctl, res_fid, res_abs, fold);     \
         break;
-            VX_HY(1)
-            VX_HY(2)
-            VX_HY(3)
-            VX_HY(4)
+            VX_HY(1, 0)
+            VX_HY(2, 0)
+            VX_HY(3, 0)
+            VX_HY(4, 0)
+            VX_HY(1, 1)
+            VX_HY(2, 1)
+            VX_HY(3, 1)
+            VX_HY(4, 1)
 #undef VX_HY
             default:
                 return 4;
@@ -1619,14 +1668,18 @@ static int vertex_set_lds_limits(const VertexPlan *p) {
                          (const void *)k_vertex_d8_mo<3, false>, (const void *)k_vertex_d8_mo<4, false>};
     const void *mg[4] = {(const void *)k_vertex_d8_mo<1, true>, (const void *)k_vertex_d8_mo<2, true>,
                          (const void *)k_vertex_d8_mo<3, true>, (const void *)k_vertex_d8_mo<4, true>};
-    const void *hy[4] = {(const void *)k_vertex_d8_hy<1>, (const void *)k_vertex_d8_hy<2>,
-                         (const void *)k_vertex_d8_hy<3>, (const void *)k_vertex_d8_hy<4>};
+    const void *hy[8] = {(const void *)k_vertex_d8_hy<1, false>, (const void *)k_vertex_d8_hy<2, false>,
+                         (const void *)k_vertex_d8_hy<3, false>, (const void *)k_vertex_d8_hy<4, false>,
+                         (const void *)k_vertex_d8_hy<1, true>, (const void *)k_vertex_d8_hy<2, true>,
+                         (const void *)k_vertex_d8_hy<3, true>, (const void *)k_vertex_d8_hy<4, true>};
     if (p->d < 1 || p->d > 4) return 4;
     if (!p->bf16 && (hipFuncSetAttribute(mo[p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)p->lds_bytes) != hipSuccess ||
                      hipFuncSetAttribute(mg[p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)vertex_lo_lds(p->d)) != hipSuccess ||
                      hipFuncSetAttribute(hy[p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)vertex_lo_lds(p->d)) != hipSuccess ||
+                     hipFuncSetAttribute(hy[4 + p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)vertex_lo_lds(p->d)) != hipSuccess))
         return 2;
     return hipFuncSetAttribute(fns[(p->bf16 ? 4 : 0) + p->d - 1],

@@ -314,7 +314,7 @@ class Engine:
         return chi[:n], S[:n]
 
     def set_option(self, name: str, value: int) -> None:
-        """Kernel options (bp_set_option): "vertex_variant" (D <= 8 sweep kernel, 0..4; 2..4
+        """Kernel options (bp_set_option): "vertex_variant" (D <= 8 sweep kernel, 0..5; 2..5
         FP32 only) and
         "vertex_bond_mask" (bit D lets networks of largest bond dimension D use the vertex
         kernels)."""

@@ -448,7 +448,7 @@ def test_vertex_low_lds_variant(pkg, orc, shape, D, d):
     s = pkg.peps.random_sites(t, 43, scale=pkg.peps.peps_scale(d, D))
     m = pkg.peps.random_messages(t, 43)
     res = {}
-    for v in (0, 1, 2, 3, 4):
+    for v in (0, 1, 2, 3, 4, 5):
         with pkg.Engine(t, s, dtype="f32") as e:
             e.set_option("vertex_variant", v)
             e.set_messages(m)
@@ -458,7 +458,7 @@ def test_vertex_low_lds_variant(pkg, orc, shape, D, d):
             e.init_messages("identity")
             e.run(maxiter=20)
             res[v] = (one, e.messages())
-    for v in (1, 2, 3, 4):
+    for v in (1, 2, 3, 4, 5):
         # one sweep from random non-symmetric messages: each variant is within the FP32
         # bound above (ill-conditioned messages make a flat relative check loose); after
         # 20 sweeps from the identity the messages are well conditioned
@@ -473,7 +473,7 @@ def test_kernel_options_round_trip(pkg):
     s = pkg.peps.random_sites(t, 44, scale=pkg.peps.peps_scale(2, 8))
     with pkg.Engine(t, s, dtype="f32") as e:
         for name, value in (("vertex_variant", 1), ("vertex_variant", 2), ("vertex_variant", 3),
-                            ("vertex_variant", 4), ("vertex_variant", 0),
+                            ("vertex_variant", 4), ("vertex_variant", 5), ("vertex_variant", 0),
                             ("vertex16_waves", 8), ("vertex_bond_mask", 0x1FE)):
             e.set_option(name, value)
             assert e.get_option(name) == value
