@@ -2,9 +2,10 @@
 generic-kernel adjoint on the same converged messages and against the FP64 oracle.
 
 Energy convention: H = -J sum_<uv> Z_u Z_v - h sum_v X_v, J = 1, h = 3.
-Stated tolerances: vertex vs generic FP32 (same messages, different summation order):
-1e-6 relative on the energy, 2e-4 of the largest gradient entry; vertex FP32 vs the FP64
-oracle: 1e-6 relative on the energy, 1e-3 of the largest gradient entry.
+Stated tolerances: vertex FP32 vs the FP64 oracle: 1e-6 relative on the energy, 1e-3 of the
+largest gradient entry; vertex vs generic FP32 (same messages, different summation order):
+2e-6 relative on the energy (each path within 1e-6 of the FP64 value, on opposite sides at
+worst: measured 5.4e-7 and 5.6e-7 on the 6x6 lattice), 2e-4 of the largest gradient entry.
 """
 import numpy as np
 import pytest
@@ -51,7 +52,7 @@ def test_vertex_tfim_matches_generic(pkg, kind, L):
     eg.close()
     _say(f"E vertex={Ev:.9f} generic={Eg:.9f} sweeps {itv}/{itg} res {resv:.1e}/{resg:.1e} "
          f"grad rel diff={_rel(gv, gg):.2e}")
-    assert Ev == pytest.approx(Eg, rel=1e-6)
+    assert Ev == pytest.approx(Eg, rel=2e-6)
     assert resv < 1e-6
     assert _rel(gv, gg) < 2e-4
 
@@ -64,7 +65,7 @@ def test_vertex_tfim_energy_only(pkg):
     Eg = eg.tfim_energy(1.0, 3.0)
     ev.close()
     eg.close()
-    assert Ev == pytest.approx(Eg, rel=1e-6)
+    assert Ev == pytest.approx(Eg, rel=2e-6)
 
 
 def test_vertex_grad_lognorm_matches_generic(pkg):
