@@ -247,13 +247,14 @@ int bp_local_energy_device(bp_context *ctx, const double *ops_v, const double *o
 
 /* Kernel options (what the autotuner decides, settable by hand):
  *   "vertex_variant"   0: D <= 8 vertex sweep kernel (FP32 or BF16) with two LDS buffers
- *                      per wave (the default in BF16, and in FP32 at physical dimension 1),
- *                      1: one buffer per wave and two waves per SIMD (the FP32 default at
- *                      physical dimension >= 2); FP32 only: 2: open contractions on MFMA,
- *                      bra in LDS (one wave per SIMD), 3: open contractions on MFMA, bra
- *                      rows from padded X/Y site copies in HBM (two waves per SIMD),
+ *                      per wave, 1: one buffer per wave and two waves per SIMD (the default
+ *                      in BF16, and in FP32 at physical dimension 1); FP32 only: 2: open
+ *                      contractions on MFMA, bra in LDS (one wave per SIMD), 3: open
+ *                      contractions on MFMA, bra rows from padded X/Y site copies in HBM
+ *                      (two waves per SIMD),
  *                      4: as 1 with two of the four open contractions per slice on MFMA
- *                      (the partner message absorbed into the bra), 5: all four;
+ *                      (the partner message absorbed into the bra), 5: all four (the FP32
+ *                      default at physical dimension >= 2);
  *   "vertex16_waves"   waves per workgroup of the BF16 kernel for bond dimensions 9..16
  *                      (4, 8 or 16; default 16);
  *   "vertex_bond_mask" bit D set: networks whose largest bond dimension is D may run on the

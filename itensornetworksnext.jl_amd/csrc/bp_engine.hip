@@ -400,12 +400,14 @@ static int dev_alloc(void **p, size_t bytes) {
     return BP_OK;
 }
 
-// Measured at 32x32 D=8 (sweep kernel, us): FP32 d=1 23.3 / 24.4, d=2 40.6 / 36.4,
-// d=3 72.6 / 59.6, d=4 75.5 / 65.3 (two-buffer / low-LDS); BF16 d=2 31.3 / 32.4.
+// Measured at 32x32 D=8 (100 back-to-back sweeps, us per sweep), FP32 variants 0 / 1 / 4 / 5:
+// d=1 23.5 / 22.4 / 23.0 / 22.7, d=2 40.8 / 34.4 / 32.8 / 31.8, d=3 73.9 / 54.1 / 54.1 / 51.2,
+// d=4 75.7 / 60.5 / 57.8 / 57.0; BF16 d=2 variants 0 / 1: 33.2 / 30.5.
 static int vx_variant_of(const bp_context *c) {
     if (c->vx_variant >= 0) return c->vx_variant;
     if (!c->vplan_ok) return -1;  // no vertex plan: no D <= 8 sweep kernel is chosen yet
-    return c->dtype == BP_F32 && c->vplan.d >= 2 ? 1 : 0;
+    if (c->dtype == BP_F32) return c->vplan.d >= 2 ? 5 : 1;
+    return 1;
 }
 
 static size_t run_state_bytes(int nnet) { return (sizeof(double) + 3 * sizeof(int)) * nnet; }
