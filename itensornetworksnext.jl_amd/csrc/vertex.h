@@ -1324,18 +1324,31 @@ __device__ __forceinline__ int vx_lane_entry(int lane) {
     return (4 * b2 + 2 * b4 + b3) * 8 + 4 * (b2 ^ b5) + (lane & 3);
 }
 
+// The bras of these opens are stored lane-major, bra element (r, lane) at
+// bra[lane * VH_PITCH + r]: each lane reads its own row (and that of lane ^ 4) as 16-byte
+// words, and a pitch of 68 floats puts the 16 lanes of one read cycle on distinct banks.  The
+// buffer is VH_BUF floats; the X <-> Y transposes inside it keep the pitch VX_PITCH.
+constexpr int VH_PITCH = 68;
+constexpr int VH_BUF = 64 * VH_PITCH;
+
 // out[a][a'] = sum over the registers r and lane bits 3..5 of K[r](a) bra[r](a'), the open
-// port in lane bits 0..2; bra element (r, lane) at bra[r * VX_PITCH + lane].
+// port in lane bits 0..2.
 __device__ __forceinline__ float vx_open_lane(const float (&K)[64], const float *bra) {
     const int lane = threadIdx.x & 63;
-    const float *bd = bra + lane, *bo = bra + (lane ^ 4);
+    const float4 *bd = reinterpret_cast<const float4 *>(bra + lane * VH_PITCH);
+    const float4 *bo = reinterpret_cast<const float4 *>(bra + (lane ^ 4) * VH_PITCH);
     v4f d0 = {0.f, 0.f, 0.f, 0.f}, d1 = d0, o0 = d0, o1 = d0;
 #pragma unroll
-    for (int r = 0; r < 64; r += 2) {
-        d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[r], bd[r * VX_PITCH], d0, 0, 0, 0);
-        o0 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[r], bo[r * VX_PITCH], o0, 0, 0, 0);
-        d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[r + 1], bd[(r + 1) * VX_PITCH], d1, 0, 0, 0);
-        o1 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[r + 1], bo[(r + 1) * VX_PITCH], o1, 0, 0, 0);
+    for (int q = 0; q < 16; ++q) {
+        const float4 x = bd[q], y = bo[q];
+        d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q], x.x, d0, 0, 0, 0);
+        o0 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q], y.x, o0, 0, 0, 0);
+        d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 1], x.y, d1, 0, 0, 0);
+        o1 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 1], y.y, o1, 0, 0, 0);
+        d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 2], x.z, d0, 0, 0, 0);
+        o0 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 2], y.z, o0, 0, 0, 0);
+        d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 3], x.w, d1, 0, 0, 0);
+        o1 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 3], y.w, o1, 0, 0, 0);
     }
     float v[8];
 #pragma unroll
@@ -1378,8 +1391,8 @@ void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__
     const int u = blockIdx.x;
     const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    float *buf = vsm + w * VX_BUF;
-    float *msm = vsm + NS * VX_BUF;      // 4 x 64 incoming messages
+    float *buf = vsm + w * VH_BUF;
+    float *msm = vsm + NS * VH_BUF;      // 4 x 64 incoming messages
     float *part = msm + 4 * 64;          // [s][port][64] per-slice results
     const int4 vi = vin[u], vo = vout[u], vdm = vdim[u];
     const int ins[4] = {vi.x, vi.y, vi.z, vi.w};
@@ -1448,7 +1461,7 @@ void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__
     vx_mode_t<true>(E, T, msm + 2 * 64);        // B2 = A x2 m2^T (Y)
     wave_lds_sync();  // every lane has finished reading A from the buffer
 #pragma unroll
-    for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = T[r];  // B2, readable as X
+    for (int r = 0; r < 64; ++r) buf[r * VH_PITCH + lane] = T[r];  // B2, lane-major in X
     wave_lds_sync();
     mypart[3 * 64 + e_lane] = vx_open_lane(G, buf);
     if constexpr (ALL) {
@@ -1462,7 +1475,7 @@ void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__
         for (int r = 0; r < 64; ++r) G[r] = buf[r * VX_PITCH + ls];  // E01 in X'
         wave_lds_sync();
 #pragma unroll
-        for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + vx_lane_swap(r)] = T[r];  // B3 as X'
+        for (int r = 0; r < 64; ++r) buf[vx_lane_swap(r) * VH_PITCH + lane] = T[r];  // B3 in X'
         wave_lds_sync();
         mypart[2 * 64 + e_lane] = vx_open_lane(G, buf);
     }
@@ -1479,7 +1492,7 @@ void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__
     vx_mode_t<true>(T, E, msm + 0 * 64);        // B0 = A x0 m0^T (X)
     wave_lds_sync();
 #pragma unroll
-    for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = E[r];  // B0, readable as Y
+    for (int r = 0; r < 64; ++r) buf[r * VH_PITCH + lane] = E[r];  // B0, lane-major in Y
     wave_lds_sync();
     mypart[1 * 64 + e_lane] = vx_open_lane(G, buf);
     if constexpr (ALL) {
@@ -1493,7 +1506,7 @@ void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__
         for (int r = 0; r < 64; ++r) G[r] = buf[r * VX_PITCH + ls];  // E23 in Y'
         wave_lds_sync();
 #pragma unroll
-        for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + vx_lane_swap(r)] = E[r];  // B1 as Y'
+        for (int r = 0; r < 64; ++r) buf[vx_lane_swap(r) * VH_PITCH + lane] = E[r];  // B1 in Y'
         wave_lds_sync();
         mypart[0 * 64 + e_lane] = vx_open_lane(G, buf);
     } else {
@@ -1559,6 +1572,10 @@ static size_t vertex_lo_lds(int ns) {
     return sizeof(float) * ((size_t)ns * VX_BUF + 4 * 64 + (size_t)ns * 4 * 64);
 }
 
+static size_t vertex_hy_lds(int ns) {
+    return sizeof(float) * ((size_t)ns * VH_BUF + 4 * 64 + (size_t)ns * 4 * 64);
+}
+
 static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *vsites,
                         const void *src, void *dst, const NetCtl &ctl, double *res_fid,
                         double *res_abs, const SweepFold &fold) {
@@ -1591,7 +1608,7 @@ static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *v
         return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     if ((p->variant == 4 || p->variant == 5) && !p->bf16) {  // lane-port MFMA opens (FP32)
-        const size_t lds = vertex_lo_lds(p->d);
+        const size_t lds = vertex_hy_lds(p->d);
         switch (p->d * 2 + (p->variant == 5 ? 1 : 0)) {
 #define VX_HY(NS, ALL)                                                                         \
     case NS * 2 + ALL:                                                                         \
@@ -1678,9 +1695,9 @@ static int vertex_set_lds_limits(const VertexPlan *p) {
                      hipFuncSetAttribute(mg[p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)vertex_lo_lds(p->d)) != hipSuccess ||
                      hipFuncSetAttribute(hy[p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)vertex_lo_lds(p->d)) != hipSuccess ||
+                                         (int)vertex_hy_lds(p->d)) != hipSuccess ||
                      hipFuncSetAttribute(hy[4 + p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)vertex_lo_lds(p->d)) != hipSuccess))
+                                         (int)vertex_hy_lds(p->d)) != hipSuccess))
         return 2;
     return hipFuncSetAttribute(fns[(p->bf16 ? 4 : 0) + p->d - 1],
                                hipFuncAttributeMaxDynamicSharedMemorySize,
