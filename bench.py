@@ -43,8 +43,9 @@ SWEEP_KERNEL = {("vertex", "f32", 0): "k_vertex_d8<NS, false>",
                 ("vertex", "bf16", 1): "k_vertex_d8_lo<NS, true>",
                 ("vertex", "f32", 2): "k_vertex_d8_mo<NS, false>",
                 ("vertex", "f32", 3): "k_vertex_d8_mo<NS, true>",
-                ("vertex", "f32", 4): "k_vertex_d8_hy<NS, false>",
-                ("vertex", "f32", 5): "k_vertex_d8_hy<NS, true>",
+                ("vertex", "f32", 4): "k_vertex_d8_hy<NS, false, false>",
+                ("vertex", "f32", 5): "k_vertex_d8_hy<NS, true, false>",
+                ("vertex", "bf16", 5): "k_vertex_d8_hy<NS, true, true>",
                 ("vertex", "f64", 0): "k_vertex_d8_f64<NS>"}
 
 PEAK = {  # MI355X published dense peaks
@@ -668,8 +669,8 @@ def main(argv=None):
               "device_ms_per_step": round(res.ms / args.steps, 5),
               "autotune": {"decisions": ntuned, "seconds": round(t_tune, 3)}}
     if path == "vertex" and args.dtype == "f32":
-        # both FP32 sweep kernels, timed the same way (the run above used the tuned one)
-        for v in (0, 1):
+        # three FP32 sweep kernels, timed the same way (the run above used the tuned one)
+        for v in (0, 1, 5):
             e2 = pkg.Engine(topo, sites, dtype="f32")
             e2.set_option("vertex_variant", v)
             e2.init_messages("identity")
@@ -715,7 +716,7 @@ def main(argv=None):
         extras["bf16"] = {"msg_per_s": nmsg * args.steps / wb, "kernel_ms": kb,
                           "algorithmic_TFLOPs": flops / (kb * 1e-3) / 1e12,
                           "sweep_kernel": SWEEP_KERNEL.get(("vertex", "bf16", vb))}
-        for v in (0, 1):
+        for v in (0, 1, 5):
             eb.set_option("vertex_variant", v)
             extras["bf16"][f"vertex_variant_{v}_kernel_ms"] = eb.time_sweeps(100)
         eb.close()

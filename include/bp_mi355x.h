@@ -254,7 +254,8 @@ int bp_local_energy_device(bp_context *ctx, const double *ops_v, const double *o
  *                      (two waves per SIMD),
  *                      4: as 1 with two of the four open contractions per slice on MFMA
  *                      (the partner message absorbed into the bra), 5: all four (the FP32
- *                      default at physical dimension >= 2);
+ *                      default at physical dimension >= 2; also BF16, with BF16 mode
+ *                      products);
  *   "vertex16_waves"   waves per workgroup of the BF16 kernel for bond dimensions 9..16
  *                      (4, 8 or 16; default 16);
  *   "vertex_bond_mask" bit D set: networks whose largest bond dimension is D may run on the

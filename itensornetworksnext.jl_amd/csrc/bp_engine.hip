@@ -1386,9 +1386,11 @@ int bp_set_option(bp_context *c, const char *name, int value) {
     bump_epoch();
     if (strcmp(name, "vertex_variant") == 0) {
         if (value < 0 || value > 5) return set_err(BP_ERR_ARG, "vertex_variant is 0..5");
-        if (value >= 2 && c->dtype != BP_F32)
+        if (value >= 2 && value <= 4 && c->dtype != BP_F32)
             return set_err(BP_ERR_UNSUPPORTED, "vertex_variant %d (MFMA open contractions) is FP32 only",
                            value);
+        if (value == 5 && c->dtype != BP_F32 && c->dtype != BP_BF16)
+            return set_err(BP_ERR_UNSUPPORTED, "vertex_variant 5 is FP32 or BF16");
         c->vx_variant = value;
         c->vplan.variant = value;
         return BP_OK;
@@ -1821,25 +1823,29 @@ int bp_autotune(bp_context *c, const char *cache_path, int mode, int *out_tuned)
         auto it = c->tune.find(key);
         if (it == c->tune.end() && mode >= 1) {
             // FP32: six kernels (two with VALU open contractions, two with MFMA ones staged
-            // through LDS, two with half or all of them on MFMA in place); BF16: two
-            const int nvar = c->dtype == BP_F32 ? 6 : 2;
+            // through LDS, two with half or all of them on MFMA in place); BF16: variants 0,
+            // 1 and 5
+            const int nvar = 6;
+            const bool cand[6] = {true, true, c->dtype == BP_F32, c->dtype == BP_F32,
+                                  c->dtype == BP_F32, true};
             double t[6] = {-1.0, -1.0, -1.0, -1.0, -1.0, -1.0};
             int best = 0;
             // the candidates in turn, three rounds, each keeping its best (drift in clocks or
             // in the rest of the device then weighs on every candidate alike)
             for (int round = 0; round < 3 && rc == BP_OK; ++round)
                 for (int v = 0; v < nvar && rc == BP_OK; ++v) {
+                    if (!cand[v]) continue;
                     c->vplan.variant = v;
                     const double tv = tune_time_sweep(c, 1);
                     if (tv < 0.0) rc = set_err(BP_ERR_HIP, "autotuner sweep failed");
                     else if (t[v] < 0.0 || tv < t[v]) t[v] = tv;
                 }
             for (int v = 1; v < nvar && rc == BP_OK; ++v)
-                if (t[v] < t[best]) best = v;
+                if (cand[v] && t[v] < t[best]) best = v;
             if (rc == BP_OK) {
                 double alt = 1e30;
                 for (int v = 0; v < nvar; ++v)
-                    if (v != best) alt = std::min(alt, t[v]);
+                    if (cand[v] && v != best) alt = std::min(alt, t[v]);
                 TuneEntry e;
                 e.vertex = best;  // the chosen variant
                 e.us = 1e3 * t[best];

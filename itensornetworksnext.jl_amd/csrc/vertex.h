@@ -1304,16 +1304,29 @@ void k_vertex_d8_lo(const void *__restrict__ vsites_v, const float *__restrict__
 // (registers p2 p3, lanes p1 p0), made through the buffer like the X <-> Y transposes:
 //   out2 = open_p2(E01 in X', A x3 m3^T in X')   out0 = open_p0(E23 in Y', A x1 m1^T in Y')
 
-// Mode product with the transposed message: O[b] = sum_a T[a] m[b][a] on a register port.
-template <bool HI>
+// Mode product with the transposed message: O[b] = sum_a T[a] m[b][a] on a register port, in
+// the precision of the kernel (BF16: Mb[g * 2 + h] packs m[4h + lane % 4][4g + t], t = 0..3).
+template <bool HI, bool BF = false>
 __device__ __forceinline__ void vx_mode_t(const float (&T)[64], float (&O)[64], const float *msg) {
-    float M[16];
     const int c = threadIdx.x & 3;
+    if constexpr (BF) {
+        v4s Mb[4];
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
+        for (int g = 0; g < 2; ++g)
 #pragma unroll
-        for (int h = 0; h < 2; ++h) M[a * 2 + h] = msg[(4 * h + c) * 8 + a];
-    vx_mp<HI>(T, O, M);
+            for (int h = 0; h < 2; ++h) {
+                const float *p = msg + (4 * h + c) * 8 + 4 * g;
+                Mb[g * 2 + h] = bf16x4(p[0], p[1], p[2], p[3]);
+            }
+        vx_mp_bf16<HI>(T, O, Mb);
+    } else {
+        float M[16];
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) M[a * 2 + h] = msg[(4 * h + c) * 8 + a];
+        vx_mp<HI>(T, O, M);
+    }
 }
 
 // Entry (a * 8 + a') of the open contraction that lane l holds after vx_open_lane:
@@ -1378,7 +1391,10 @@ __device__ __forceinline__ float vx_open_lane(const float (&K)[64], const float 
 // Lane index with its two 3-bit halves swapped ((p, q) -> (q, p)).
 __device__ __forceinline__ int vx_lane_swap(int l) { return ((l & 7) << 3) | (l >> 3); }
 
-template <int NS, bool ALL>
+// BF = true (BF16 contexts, with ALL): the slices arrive as BF16, the eight mode products run
+// on v_mfma_f32_4x4x4_16b_bf16 (operands rounded to BF16, incoming messages scaled by
+// 1 / max|m| first, as in the other BF16 kernels) and the open contractions stay FP32.
+template <int NS, bool ALL, bool BF = false>
 __global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__ mold,
                     float *__restrict__ mnew, const int4 *__restrict__ vin,
@@ -1422,10 +1438,22 @@ void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__
         float x;
         if (ins[j] < 0) x = la == lb ? 1.f : 0.f;
         else x = (la < D && lb < D) ? mold[ino[j] + la * D + lb] : 0.f;
+        if constexpr (BF) {
+            const float mx = wave_max(fabsf(x));
+            x = mx > 0.f ? x / mx : x;
+        }
         msm[j * 64 + lane] = x;
     }
     float T[64], E[64], G[64];
-    {
+    if constexpr (BF) {
+        const unsigned *src = (const unsigned *)vsites_v + ((long long)u * NS + w) * 2048;
+#pragma unroll
+        for (int r2 = 0; r2 < 32; ++r2) {
+            const unsigned pr = src[r2 * 64 + lane];
+            T[2 * r2] = bf16_to_f((unsigned short)(pr & 0xffffu));
+            T[2 * r2 + 1] = bf16_to_f((unsigned short)(pr >> 16));
+        }
+    } else {
         const float *src = (const float *)vsites_v + ((long long)u * NS + w) * 4096;
 #pragma unroll
         for (int r = 0; r < 64; ++r) T[r] = src[r * 64 + lane];
@@ -1435,8 +1463,8 @@ void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__
     const int e_split = vx_split_entry(lane), e_lane = vx_lane_entry(lane);
 
     // ---- phase A (layout X: registers p0 p1, lanes p2 (bits 3..5) p3 (bits 0..2))
-    vx_mode<true, false>(T, E, msm + 0 * 64);
-    vx_mode<false, false>(E, G, msm + 1 * 64);  // G = E01 (X), kept for out3
+    vx_mode<true, BF>(T, E, msm + 0 * 64);
+    vx_mode<false, BF>(E, G, msm + 1 * 64);  // G = E01 (X), kept for out3
     if constexpr (ALL) {
 #pragma unroll
         for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = T[r];  // A (X)
@@ -1453,19 +1481,19 @@ void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__
 #pragma unroll
         for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = T[r];  // A, the bra of out2
         wave_lds_sync();
-        vx_mode<false, false>(E, T, msm + 3 * 64);  // E01 x3 m3 (Y)
+        vx_mode<false, BF>(E, T, msm + 3 * 64);  // E01 x3 m3 (Y)
         mypart[2 * 64 + e_split] = vx_final_half<true, false, VX_PITCH>(T, buf);
 #pragma unroll
         for (int r = 0; r < 64; ++r) E[r] = buf[lane * VX_PITCH + r];  // A in Y (kept to phase B)
     }
-    vx_mode_t<true>(E, T, msm + 2 * 64);        // B2 = A x2 m2^T (Y)
+    vx_mode_t<true, BF>(E, T, msm + 2 * 64);        // B2 = A x2 m2^T (Y)
     wave_lds_sync();  // every lane has finished reading A from the buffer
 #pragma unroll
     for (int r = 0; r < 64; ++r) buf[r * VH_PITCH + lane] = T[r];  // B2, lane-major in X
     wave_lds_sync();
     mypart[3 * 64 + e_lane] = vx_open_lane(G, buf);
     if constexpr (ALL) {
-        vx_mode_t<false>(E, T, msm + 3 * 64);   // B3 = A x3 m3^T (Y)
+        vx_mode_t<false, BF>(E, T, msm + 3 * 64);   // B3 = A x3 m3^T (Y)
         wave_lds_sync();  // every lane has finished reading B2
 #pragma unroll
         for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = G[r];
@@ -1481,22 +1509,22 @@ void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__
     }
 
     // ---- phase B (layout Y: registers p2 p3, lanes p0 (bits 3..5) p1 (bits 0..2))
-    vx_mode<true, false>(E, T, msm + 2 * 64);
-    vx_mode<false, false>(T, G, msm + 3 * 64);  // G = E23 (Y), kept for out1
+    vx_mode<true, BF>(E, T, msm + 2 * 64);
+    vx_mode<false, BF>(T, G, msm + 3 * 64);  // G = E23 (Y), kept for out1
     wave_lds_sync();  // every lane has finished reading the last bra
 #pragma unroll
     for (int r = 0; r < 64; ++r) buf[lane * VX_PITCH + r] = E[r];  // A, readable as X
     wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < 64; ++r) T[r] = buf[r * VX_PITCH + lane];  // A in X
-    vx_mode_t<true>(T, E, msm + 0 * 64);        // B0 = A x0 m0^T (X)
+    vx_mode_t<true, BF>(T, E, msm + 0 * 64);        // B0 = A x0 m0^T (X)
     wave_lds_sync();
 #pragma unroll
     for (int r = 0; r < 64; ++r) buf[r * VH_PITCH + lane] = E[r];  // B0, lane-major in Y
     wave_lds_sync();
     mypart[1 * 64 + e_lane] = vx_open_lane(G, buf);
     if constexpr (ALL) {
-        vx_mode_t<false>(T, E, msm + 1 * 64);   // B1 = A x1 m1^T (X)
+        vx_mode_t<false, BF>(T, E, msm + 1 * 64);   // B1 = A x1 m1^T (X)
         wave_lds_sync();  // every lane has finished reading B0
 #pragma unroll
         for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = G[r];
@@ -1520,7 +1548,7 @@ void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__
 #pragma unroll
         for (int r = 0; r < 64; ++r) buf[r * VX_PITCH + lane] = T[r];  // A, the bra of out0
         wave_lds_sync();
-        vx_mode<false, false>(E, G, msm + 1 * 64);  // E23 x1 m1 (X)
+        vx_mode<false, BF>(E, G, msm + 1 * 64);  // E23 x1 m1 (X)
         mypart[0 * 64 + e_split] = vx_final_half<true, true, VX_PITCH>(G, buf);
     }
 
@@ -1607,23 +1635,28 @@ static int vertex_sweep(VertexPlan *p, hipStream_t st, DevTopo tp, const void *v
         }
         return hipGetLastError() == hipSuccess ? 0 : 2;
     }
-    if ((p->variant == 4 || p->variant == 5) && !p->bf16) {  // lane-port MFMA opens (FP32)
+    // lane-port MFMA opens: variant 4 (FP32), variant 5 (FP32 or BF16)
+    if ((p->variant == 4 && !p->bf16) || p->variant == 5) {
         const size_t lds = vertex_hy_lds(p->d);
-        switch (p->d * 2 + (p->variant == 5 ? 1 : 0)) {
-#define VX_HY(NS, ALL)                                                                         \
-    case NS * 2 + ALL:                                                                         \
-        hipLaunchKernelGGL((k_vertex_d8_hy<NS, ALL>), dim3(n4), dim3(64 * NS), lds, st,        \
+        switch (p->d * 4 + (p->variant == 5 ? 2 : 0) + (p->bf16 ? 1 : 0)) {
+#define VX_HY(NS, ALL, BF)                                                                     \
+    case NS * 4 + ALL * 2 + BF:                                                                \
+        hipLaunchKernelGGL((k_vertex_d8_hy<NS, ALL, BF>), dim3(n4), dim3(64 * NS), lds, st,    \
                            vsites, (const float *)src, (float *)dst, p->d_vin, p->d_vout,      \
                            p->d_vdim, p->d_vnet, tp.msg_off, ctl, res_fid, res_abs, fold);     \
         break;
-            VX_HY(1, 0)
-            VX_HY(2, 0)
-            VX_HY(3, 0)
-            VX_HY(4, 0)
-            VX_HY(1, 1)
-            VX_HY(2, 1)
-            VX_HY(3, 1)
-            VX_HY(4, 1)
+            VX_HY(1, 0, 0)
+            VX_HY(2, 0, 0)
+            VX_HY(3, 0, 0)
+            VX_HY(4, 0, 0)
+            VX_HY(1, 1, 0)
+            VX_HY(2, 1, 0)
+            VX_HY(3, 1, 0)
+            VX_HY(4, 1, 0)
+            VX_HY(1, 1, 1)
+            VX_HY(2, 1, 1)
+            VX_HY(3, 1, 1)
+            VX_HY(4, 1, 1)
 #undef VX_HY
             default:
                 return 4;
@@ -1685,10 +1718,12 @@ static int vertex_set_lds_limits(const VertexPlan *p) {
                          (const void *)k_vertex_d8_mo<3, false>, (const void *)k_vertex_d8_mo<4, false>};
     const void *mg[4] = {(const void *)k_vertex_d8_mo<1, true>, (const void *)k_vertex_d8_mo<2, true>,
                          (const void *)k_vertex_d8_mo<3, true>, (const void *)k_vertex_d8_mo<4, true>};
-    const void *hy[8] = {(const void *)k_vertex_d8_hy<1, false>, (const void *)k_vertex_d8_hy<2, false>,
-                         (const void *)k_vertex_d8_hy<3, false>, (const void *)k_vertex_d8_hy<4, false>,
-                         (const void *)k_vertex_d8_hy<1, true>, (const void *)k_vertex_d8_hy<2, true>,
-                         (const void *)k_vertex_d8_hy<3, true>, (const void *)k_vertex_d8_hy<4, true>};
+    const void *hy[12] = {(const void *)k_vertex_d8_hy<1, false>, (const void *)k_vertex_d8_hy<2, false>,
+                          (const void *)k_vertex_d8_hy<3, false>, (const void *)k_vertex_d8_hy<4, false>,
+                          (const void *)k_vertex_d8_hy<1, true>, (const void *)k_vertex_d8_hy<2, true>,
+                          (const void *)k_vertex_d8_hy<3, true>, (const void *)k_vertex_d8_hy<4, true>,
+                          (const void *)k_vertex_d8_hy<1, true, true>, (const void *)k_vertex_d8_hy<2, true, true>,
+                          (const void *)k_vertex_d8_hy<3, true, true>, (const void *)k_vertex_d8_hy<4, true, true>};
     if (p->d < 1 || p->d > 4) return 4;
     if (!p->bf16 && (hipFuncSetAttribute(mo[p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)p->lds_bytes) != hipSuccess ||
@@ -1698,6 +1733,9 @@ static int vertex_set_lds_limits(const VertexPlan *p) {
                                          (int)vertex_hy_lds(p->d)) != hipSuccess ||
                      hipFuncSetAttribute(hy[4 + p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)vertex_hy_lds(p->d)) != hipSuccess))
+        return 2;
+    if (p->bf16 && hipFuncSetAttribute(hy[8 + p->d - 1], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)vertex_hy_lds(p->d)) != hipSuccess)
         return 2;
     return hipFuncSetAttribute(fns[(p->bf16 ? 4 : 0) + p->d - 1],
                                hipFuncAttributeMaxDynamicSharedMemorySize,

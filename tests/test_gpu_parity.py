@@ -264,12 +264,13 @@ def test_vertex_batch_with_masks(pkg, orc):
 
 # ---------------------------------------------------------------- BF16 + FP64 polish
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 5])
 def test_bf16_single_sweep_within_bound(pkg, orc, variant):
     """BF16 operands / FP32 accumulation: one sweep within the error bound with the BF16
     unit roundoff (gamma = 24 u, u = 2^-8, covering the ~10 roundings along a product
-    chain), from well-conditioned (identity) messages; both D <= 8 sweep kernels (two LDS
-    buffers per wave, or one buffer and two waves per SIMD)."""
+    chain), from well-conditioned (identity) messages; the D <= 8 sweep kernels with two LDS
+    buffers per wave, one buffer and two waves per SIMD, and the open contractions on MFMA
+    (BF16 mode products, FP32 opens)."""
     _say(f"case bf16 single sweep, kernel variant {variant}")
     t = pkg.build_topology(pkg.named_grid((6, 5)), 8, 2)
     s = pkg.peps.random_sites(t, 19, scale=pkg.peps.peps_scale(2, 8))
@@ -283,12 +284,13 @@ def test_bf16_single_sweep_within_bound(pkg, orc, variant):
     assert _fp32_sweep_check(pkg, orc, t, s, m, got, gamma=24 * 2.0 ** -8) < 1.0
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 5])
 def test_bf16_converged_then_fp64_polish(pkg, orc, variant):
     """Mixed-precision workload at D = 8: converge in BF16, report the error against the
     FP64 oracle, then 3 FP64 polishing sweeps shrink it (stated bounds: BF16 messages
     within 5e-2 relative, log-norm within 1e-2 relative; after polishing, messages at
-    least 4x closer and log-norm within 1e-4 relative); both D <= 8 sweep kernels."""
+    least 4x closer and log-norm within 1e-4 relative); the three BF16 D <= 8 sweep
+    kernels."""
     _say(f"case bf16 converged + polish, kernel variant {variant}")
     t = _grid(pkg, 12, 8)
     s = pkg.peps.random_sites(t, 23, scale=pkg.peps.peps_scale(2, 8))
