@@ -34,7 +34,7 @@ sys.path.insert(0, ROOT)
 # profiles/k_vertex_d8*_32x32_D8_fp32_hbm_traffic.md.  Keyed by (path, dtype, L, D, FP32
 # vertex kernel variant).
 MEASURED_TRAFFIC = {("vertex", "f32", 32, 8, 0): 37.54e6, ("vertex", "f32", 32, 8, 1): 41.14e6,
-                    ("vertex", "f32", 32, 8, 4): 37.97e6, ("vertex", "f32", 32, 8, 5): 44.39e6,
+                    ("vertex", "f32", 32, 8, 4): 37.97e6, ("vertex", "f32", 32, 8, 5): 42.63e6,
                     ("vertex", "f64", 32, 8, 0): 210.3e6}
 # the sweep kernel that dominates each configuration (NS: the physical dimension)
 SWEEP_KERNEL = {("vertex", "f32", 0): "k_vertex_d8<NS, false>",
