@@ -1634,10 +1634,11 @@ int bp_run(bp_context *c, int maxiter, double tol, int criterion, int use_graph,
 //       networks whose largest bond dimension is D run on the vertex kernels (padded to 8,
 //       12 or 16; FP64: the FP64 vertex kernel, D <= 8) or on the generic kernel,
 //       whichever sweeps faster;
-//   "vxk|<f32|bf16>|d<d>": {"variant": 0..3, "us": t_chosen, "us_alt": t_next}
+//   "vxk|<f32|bf16>|d<d>": {"variant": 0..5, "us": t_chosen, "us_alt": t_next}
 //       the D <= 8 vertex sweep kernel: 0 two LDS buffers per wave, 1 one buffer per wave
 //       with two waves per SIMD, 2 and 3 (FP32 only) open contractions on MFMA with the
-//       bra from LDS or from the padded site copies in HBM;
+//       bra from LDS or from the padded site copies in HBM, 4 (FP32 only) and 5 half or
+//       all of them on MFMA in place (the partner message absorbed into the bra);
 //   "v16w|bf16|d<d>": {"waves": 4|8|16, "us": t_best, "us_alt": t_worst}
 //       waves per slice workgroup of the bond-dimension-16 kernel;
 //   "gen|<dtype>|d<d>|D<D>": {"block": 128|256|512, "lds": 0|1|2, "us": t, "us_alt": t0}
