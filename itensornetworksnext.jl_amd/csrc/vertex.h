@@ -1344,33 +1344,36 @@ __device__ __forceinline__ int vx_lane_entry(int lane) {
 constexpr int VH_PITCH = 68;
 constexpr int VH_BUF = 64 * VH_PITCH;
 
-// out[a][a'] = sum over the registers r and lane bits 3..5 of K[r](a) bra[r](a'), the open
-// port in lane bits 0..2.
-__device__ __forceinline__ float vx_open_lane(const float (&K)[64], const float *bra) {
+// Accumulates, per MFMA block, the products of vx_open_lane for one ket and bra: acc[0], acc[1]
+// the bra at the lane itself, acc[2], acc[3] the bra at lane ^ 4.
+__device__ __forceinline__ void vx_open_acc(const float (&K)[64], const float *bra, v4f (&acc)[4]) {
     const int lane = threadIdx.x & 63;
     const float4 *bd = reinterpret_cast<const float4 *>(bra + lane * VH_PITCH);
     const float4 *bo = reinterpret_cast<const float4 *>(bra + (lane ^ 4) * VH_PITCH);
-    v4f d0 = {0.f, 0.f, 0.f, 0.f}, d1 = d0, o0 = d0, o1 = d0;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const float4 x = bd[q], y = bo[q];
-        d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q], x.x, d0, 0, 0, 0);
-        o0 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q], y.x, o0, 0, 0, 0);
-        d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 1], x.y, d1, 0, 0, 0);
-        o1 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 1], y.y, o1, 0, 0, 0);
-        d0 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 2], x.z, d0, 0, 0, 0);
-        o0 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 2], y.z, o0, 0, 0, 0);
-        d1 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 3], x.w, d1, 0, 0, 0);
-        o1 = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 3], y.w, o1, 0, 0, 0);
+        acc[0] = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q], x.x, acc[0], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q], y.x, acc[2], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 1], x.y, acc[1], 0, 0, 0);
+        acc[3] = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 1], y.y, acc[3], 0, 0, 0);
+        acc[0] = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 2], x.z, acc[0], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 2], y.z, acc[2], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 3], x.w, acc[1], 0, 0, 0);
+        acc[3] = __builtin_amdgcn_mfma_f32_4x4x1f32(K[4 * q + 3], y.w, acc[3], 0, 0, 0);
     }
+}
+
+// The sum over the MFMA blocks of the same open port (lane bits 5, 4, 3): lanes with bit 5
+// set keep the other-half tile, bit 4 and bit 3 select the row i.
+__device__ __forceinline__ float vx_open_fin(const v4f (&acc)[4]) {
+    const int lane = threadIdx.x & 63;
     float v[8];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        v[i] = d0[i] + d1[i];
-        v[4 + i] = o0[i] + o1[i];
+        v[i] = acc[0][i] + acc[1][i];
+        v[4 + i] = acc[2][i] + acc[3][i];
     }
-    // sum over lane bits 5, 4, 3 (port values of the contracted lane port): lanes with bit 5
-    // set keep the other-half tile, bit 4 and bit 3 select the row i
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[k]),
@@ -1388,12 +1391,18 @@ __device__ __forceinline__ float vx_open_lane(const float (&K)[64], const float 
     return keep + vmo_ror<8>(send);  // row_ror:8 pairs lane l with l ^ 8 exactly
 }
 
+// out[a][a'] = sum over the registers r and lane bits 3..5 of K[r](a) bra[r](a'), the open
+// port in lane bits 0..2.
+__device__ __forceinline__ float vx_open_lane(const float (&K)[64], const float *bra) {
+    v4f acc[4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f},
+                  {0.f, 0.f, 0.f, 0.f}};
+    vx_open_acc(K, bra, acc);
+    return vx_open_fin(acc);
+}
+
 // Lane index with its two 3-bit halves swapped ((p, q) -> (q, p)).
 __device__ __forceinline__ int vx_lane_swap(int l) { return ((l & 7) << 3) | (l >> 3); }
 
-// BF = true (BF16 contexts, with ALL): the slices arrive as BF16, the eight mode products run
-// on v_mfma_f32_4x4x4_16b_bf16 (operands rounded to BF16, incoming messages scaled by
-// 1 / max|m| first, as in the other BF16 kernels) and the open contractions stay FP32.
 template <int NS, bool ALL, bool BF = false>
 __global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(2, 2)))
 void k_vertex_d8_hy(const void *__restrict__ vsites_v, const float *__restrict__ mold,
