@@ -56,6 +56,7 @@ class Engine:
                  path: str = "auto"):
         self.lib = L.load()
         self.topo = topo
+        self._run_bufs = None
         self.dtype = dtype
         self.cplx = L.DTYPES[dtype] == L.BP_C128
         self.device = int(device)
@@ -168,17 +169,23 @@ class Engine:
 
     def run(self, maxiter: Optional[int] = None, tol: Optional[float] = None,
             criterion: str = "fidelity", use_graph: bool = True) -> RunResult:
-        nn = int(self.topo.nnet)
-        sw = np.zeros(nn, dtype=np.int32)
-        dl = np.zeros(nn)
-        at = np.zeros(nn, dtype=np.int32)
+        # the output buffers and their ctypes pointers are made once per engine (building
+        # the pointers costs about 15 us a call, 2% of a 20-sweep run at 32x32 D=8); the
+        # result holds copies
+        if self._run_bufs is None:
+            nn = int(self.topo.nnet)
+            sw = np.zeros(nn, dtype=np.int32)
+            dl = np.zeros(nn)
+            at = np.zeros(nn, dtype=np.int32)
+            ms = C.c_double(0)
+            self._run_bufs = (sw, dl, at, _ip(sw), _dp(dl), _ip(at), ms, C.byref(ms))
+        sw, dl, at, psw, pdl, pat, ms, pms = self._run_bufs
         crit = L.BP_CRIT_FIDELITY if criterion == "fidelity" else L.BP_CRIT_MAXABS
         L.check(self.lib.bp_run(self.h, int(maxiter) if maxiter is not None else 0,
                                 float(tol) if tol is not None else -1.0, crit,
-                                int(bool(use_graph)), _ip(sw), _dp(dl), _ip(at)), "bp_run")
-        ms = C.c_double(0)
-        L.check(self.lib.bp_last_run_ms(self.h, C.byref(ms)), "bp_last_run_ms")
-        return RunResult(sw, dl, at, ms.value)
+                                int(bool(use_graph)), psw, pdl, pat), "bp_run")
+        L.check(self.lib.bp_last_run_ms(self.h, pms), "bp_last_run_ms")
+        return RunResult(sw.copy(), dl.copy(), at.copy(), ms.value)
 
     def prepare_run(self, maxiter: int) -> None:
         """Capture the sweep graphs that `run(maxiter, use_graph=True)` replays (chunks of 16
